@@ -1,0 +1,232 @@
+/*
+ * mgx.h - C ABI of the MI355X (gfx950) Feautrier formal solver.
+ *
+ * The library replaces the plain-method hot path of Magritte:
+ *   Model::compute_radiation_field_feautrier_order_2   src/model/model.cpp:353-390
+ *     -> Solver::setup<CoMoving,adaptive>              src/solver/solver.tpp:1-13, 32-126
+ *     -> Solver::solve_feautrier_order_2<approx,adapt> src/solver/solver.tpp:459-540
+ *        (trace_ray_for_line :774-800, set_data_for_line :1057-1097,
+ *         solve_feautrier_order_2(o,f) :2194-2356, update_Lambda :1761-1850)
+ *   Model::compute_image_new                           src/model/model.cpp:1008-1035
+ *     -> Solver::image_feautrier_order_2_new_imager     src/solver/solver.tpp:577-627
+ *
+ * Conventions
+ *   - Plain C: fixed-width integers, plain pointers, no exceptions cross it.
+ *   - Every entry point returns 0 on success and a negative MGX_E* code on
+ *     failure; mgx_last_error() then returns a message for the calling thread.
+ *   - Arrays are row-major and mirror the reference's buffers
+ *     (src/bindings/pybindings.cpp:680-822):  position (P,3), u (hnrays,P,F),
+ *     J (P,F), image I (Nx*Ny, F) with pixel index y + Ny*x.
+ *   - The Lambda diagonal is returned densely as (P, nlines): the entry
+ *     (p, l) with l = nrad_cum[species] + k is the sum the reference collects
+ *     in LineProducingSpecies::lambda Ls[p][k] for nr == p
+ *     (src/model/lines/lineProducingSpecies/lambda/lambda.tpp:79-89) when
+ *     n_off_diag == 0.
+ *   - Entry points named mgx_*_device take DEVICE pointers for every array in
+ *     the model and the outputs, and an optional hipStream_t (NULL = default
+ *     stream).  They do not synchronise the stream.
+ */
+#ifndef MGX_H
+#define MGX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    MGX_OK = 0,
+    MGX_E_ARG = -1,        /* invalid argument (sizes, null pointers, options) */
+    MGX_E_HIP = -2,        /* HIP runtime error */
+    MGX_E_RAY_LENGTH = -3, /* a traced ray exceeds the kernel's staging capacity */
+    MGX_E_NO_NEXT = -4,    /* an interior point has no valid next point on a ray
+                              (reference prints "ERROR: no valid neighbor",
+                              src/solver/solver.tpp:96-98) */
+    MGX_E_UNSUPPORTED = -5 /* option combination not implemented */
+};
+
+enum { MGX_APPROX_NONE = 0, MGX_APPROX_ONE_LINE = 1, MGX_APPROX_CLOSE_LINES = 2 };
+enum { MGX_BC_ZERO = 0, MGX_BC_THERMAL = 1, MGX_BC_CMB = 2 };
+enum { MGX_PREC_FP64 = 0, MGX_PREC_MIXED = 1 };
+
+/*
+ * Model view.  Mirrors the data the reference's Solver reads from Model&
+ * (src/model/geometry/, src/model/lines/, src/model/thermodynamics/,
+ * src/model/radiation/frequencies/).  All arrays are borrowed.
+ */
+typedef struct mgx_model {
+    uint32_t npoints, nrays, hnrays, nboundary;
+    uint32_t nlines, nlspecs, nquads, nfreqs;
+    int32_t dimension;           /* 1 or 3 (Parameters::dimension) */
+    int32_t spherical_symmetry;  /* Parameters::spherical_symmetry */
+    int32_t adaptive_directions; /* Rays::use_adaptive_directions */
+    int32_t reserved0;
+
+    /* geometry: Points / Rays / Boundary */
+    const double *position;            /* (P,3) [m] */
+    const double *velocity;            /* (P,3) [fraction of c] */
+    const uint32_t *n_neighbors;       /* (P) */
+    const uint32_t *cum_n_neighbors;   /* (P) exclusive scan of n_neighbors */
+    const uint32_t *neighbors;         /* (sum n_neighbors) */
+    const uint32_t *point2boundary;    /* (P) boundary index, or P for interior */
+    const uint32_t *boundary2point;    /* (nboundary) */
+    const int32_t *boundary_condition; /* (nboundary) MGX_BC_* */
+    const double *boundary_temperature;/* (nboundary) [K] */
+    const double *ray_direction;       /* (nrays,3) or (P,nrays,3) if adaptive */
+    const double *ray_weight;          /* (nrays) or (P,nrays) if adaptive */
+    const uint32_t *antipod;           /* (nrays) */
+    double center[3];                  /* Points::center, bounding-box centre */
+
+    /* thermodynamics */
+    const double *temperature; /* (P) gas temperature [K] */
+    const double *vturb2;      /* (P) (turbulent velocity / c)^2 */
+
+    /* lines (global line index l = nrad_cum[species] + k) */
+    const double *line;              /* (L) line centre frequency [Hz] */
+    const double *sorted_line;       /* (L) ascending (Lines::sorted_line) */
+    const uint32_t *sorted_line_map; /* (L) sorted -> line index */
+    const double *emissivity;        /* (P,L) */
+    const double *opacity;           /* (P,L) */
+    const double *inverse_width;     /* (P,L) [1/Hz] */
+    const double *interp_crit;       /* (P,L) opacity / population_tot
+                                        (InterpHelper::interpolation_criterion) */
+    const double *dshift_max;        /* (P) Model::dshift_max */
+    double max_inverse_mass;         /* Lines::max_inverse_mass [1/amu] */
+    const uint32_t *line_species;    /* (L) species of line l */
+    const uint32_t *line_k;          /* (L) transition index k of line l in its species */
+    const double *line_A;            /* (L) Einstein A of line l */
+    const double *species_inverse_mass; /* (nlspecs) [1/amu] */
+    const double *quad_weights;      /* (nlspecs, nquads) */
+
+    /* frequencies (Frequencies) */
+    const double *nu;                /* (P,F) [Hz] */
+    const uint32_t *f_line;          /* (F) corresponding_line */
+    const uint32_t *f_species;       /* (F) corresponding_l_for_spec */
+    const uint32_t *f_k;             /* (F) corresponding_k_for_tran */
+    const uint32_t *f_z;             /* (F) corresponding_z_for_line */
+    const uint8_t *f_in_line;        /* (F) appears_in_line_integral */
+} mgx_model;
+
+/* Numerical knobs of Parameters used on this path (src/model/parameters/parameters.hpp:66-124). */
+typedef struct mgx_options {
+    int32_t approx;      /* MGX_APPROX_*  (model.cpp:366-389 dispatch) */
+    uint32_t n_off_diag; /* must be 0 */
+    int32_t precision;   /* MGX_PREC_FP64 (bit-faithful) or MGX_PREC_MIXED */
+    int32_t reserved0;
+    double min_opacity;                       /* default 1e-26 */
+    double max_distance_opacity_contribution; /* default 10 */
+    double max_interpolation_diff;            /* default 1.4 */
+} mgx_options;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *mgx_last_error(void);
+
+/* Library version string. */
+const char *mgx_version(void);
+
+/*
+ * Plain-method formal solve on the device: fills u (hnrays,P,F) [may be NULL],
+ * J (P,F) and lambda_diag (P,L) [may be NULL].
+ * Replaces Solver::solve_feautrier_order_2<approx,adaptive>(Model&)
+ * (src/solver/solver.tpp:459-540) including Solver::setup (:1-13).
+ */
+int mgx_feautrier_forward_device(const mgx_model *model_dev, const mgx_options *opts,
+                                 double *u, double *J, double *lambda_diag, void *stream);
+
+/*
+ * Traced point counts n_tot(rr, o, l) = last + 1 - first for every ray pair,
+ * origin and line, written as (hnrays, P, L) uint32.
+ * Same trace as Solver::trace_ray_for_line (src/solver/solver.tpp:774-800,
+ * 487-495).
+ */
+int mgx_trace_counts_device(const mgx_model *model_dev, const mgx_options *opts,
+                            uint32_t *n_tot, void *stream);
+
+/*
+ * Expanded traced lists for nsel selected (rr, o, l) triples.  Triple i
+ * writes its entries first..last (first at index 0) into row i of the
+ * (nsel, capacity) arrays; centre_idx[i] = centre - first, count[i] = n_tot.
+ * Entries are nr, nr_interp, shift, interp_factor and dZ exactly as the
+ * reference's per-thread scratch (src/solver/solver.tpp:1057-1097); dZ[j] is
+ * the increment between entries j and j+1 (its last element is unused).
+ */
+int mgx_trace_rays_device(const mgx_model *model_dev, const mgx_options *opts, uint32_t nsel,
+                          const uint32_t *sel_rr, const uint32_t *sel_o, const uint32_t *sel_l,
+                          uint32_t capacity, uint32_t *count, uint32_t *centre_idx,
+                          uint32_t *nr, uint32_t *nr_interp, double *shift,
+                          double *interp_factor, double *dZ, void *stream);
+
+
+/* ------------------------------------------------------------------------ */
+/* Host-side producers of the path's inputs (plain host pointers).           */
+/* ------------------------------------------------------------------------ */
+
+/* Line data of one line-producing species
+ * (src/model/lines/lineProducingSpecies/linedata/linedata.hpp). */
+typedef struct mgx_linedata {
+    uint32_t num;            /* species index in the abundance table */
+    uint32_t nlev, nrad;
+    uint32_t reserved0;
+    double inverse_mass;     /* [1/amu] */
+    const double *energy;    /* (nlev) [J] */
+    const double *weight;    /* (nlev) statistical weights */
+    const uint32_t *irad;    /* (nrad) upper level */
+    const uint32_t *jrad;    /* (nrad) lower level */
+    const double *A;         /* (nrad) [1/s] */
+    const double *Bs;        /* (nrad) */
+    const double *Ba;        /* (nrad) */
+    const double *frequency; /* (nrad) [Hz] */
+} mgx_linedata;
+
+/* Lines::set_inverse_width (src/model/lines/lines.tpp:45-60): (P, L). */
+int mgx_compute_inverse_widths(uint32_t npoints, uint32_t nlspecs, const mgx_linedata *ld,
+                               const double *temperature, const double *vturb2,
+                               double *inverse_width);
+
+/* LineProducingSpecies::update_using_LTE (lineProducingSpecies.tpp:52-75) and
+ * Lines::set_emissivity_and_opacity (lines.tpp:29-41), populations in long
+ * double as the reference (VectorXld).  population: per species s a block of
+ * (P, nlev_s) doubles, blocks concatenated in species order; population_tot
+ * (nlspecs, P); emissivity, opacity (P, L).  abundance is (P, nspec). */
+int mgx_compute_lte(uint32_t npoints, uint32_t nspec, const double *abundance,
+                    const double *temperature, uint32_t nlspecs, const mgx_linedata *ld,
+                    double min_line_emissivity, double min_line_opacity,
+                    double *population, double *population_tot, double *emissivity,
+                    double *opacity);
+
+/* Model::compute_spectral_discretisation() (src/model/model.cpp:70-161):
+ * nu (P, F) with F = nlines * nquads; roots (nlspecs, nquads); per-frequency
+ * maps (F) as in Frequencies. */
+int mgx_compute_spectral_discretisation(uint32_t npoints, uint32_t nlspecs,
+                                        const mgx_linedata *ld, uint32_t nquads,
+                                        const double *roots, const double *temperature,
+                                        const double *vturb2, double *nu, uint32_t *f_line,
+                                        uint32_t *f_species, uint32_t *f_k, uint32_t *f_z,
+                                        uint8_t *f_in_line);
+
+/* Model::compute_spectral_discretisation(nu_min, nu_max, n) (model.cpp:257-298):
+ * the same n image frequencies at every point; maps set to "not in a line". */
+int mgx_compute_image_discretisation(uint32_t npoints, double nu_min, double nu_max, uint32_t n,
+                                     double *nu, uint32_t *f_line, uint32_t *f_species,
+                                     uint32_t *f_k, uint32_t *f_z, uint8_t *f_in_line);
+
+/* Model::set_dshift_max (model.cpp:1158-1178): (P). */
+int mgx_compute_dshift_max(uint32_t npoints, uint32_t nlspecs, const mgx_linedata *ld,
+                           const double *temperature, const double *vturb2,
+                           double max_width_fraction, double *dshift_max);
+
+/* InterpHelper constructor (solver_interp_helper.hpp:15-30): opacity / population_tot, (P, L). */
+int mgx_compute_interp_crit(uint32_t npoints, uint32_t nlspecs, const mgx_linedata *ld,
+                            const double *opacity, const double *population_tot,
+                            double *interp_crit);
+
+/* Lines::read line sorting (lines.cpp:44-64, heapsort tools/heapsort.hpp:64-88). */
+int mgx_sort_lines(uint32_t nlines, const double *line, double *sorted_line,
+                   uint32_t *sorted_line_map);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MGX_H */

@@ -1,0 +1,146 @@
+"""ctypes mirror of include/mgx.h and the loader of the in-tree libmgx.so.
+
+The structs below follow the C declarations field by field; the library is
+loaded from this package directory (built by ``__graft_entry__.build()``).
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmgx.so")
+
+MGX_OK = 0
+MGX_E_ARG = -1
+MGX_E_HIP = -2
+MGX_E_RAY_LENGTH = -3
+MGX_E_NO_NEXT = -4
+MGX_E_UNSUPPORTED = -5
+
+APPROX_NONE, APPROX_ONE_LINE, APPROX_CLOSE_LINES = 0, 1, 2
+BC_ZERO, BC_THERMAL, BC_CMB = 0, 1, 2
+PREC_FP64, PREC_MIXED = 0, 1
+
+_pd = C.POINTER(C.c_double)
+_pu = C.POINTER(C.c_uint32)
+_pi = C.POINTER(C.c_int32)
+_pb = C.POINTER(C.c_uint8)
+
+
+class MgxModel(C.Structure):
+    _fields_ = [
+        ("npoints", C.c_uint32), ("nrays", C.c_uint32), ("hnrays", C.c_uint32),
+        ("nboundary", C.c_uint32),
+        ("nlines", C.c_uint32), ("nlspecs", C.c_uint32), ("nquads", C.c_uint32),
+        ("nfreqs", C.c_uint32),
+        ("dimension", C.c_int32), ("spherical_symmetry", C.c_int32),
+        ("adaptive_directions", C.c_int32), ("reserved0", C.c_int32),
+        ("position", _pd), ("velocity", _pd),
+        ("n_neighbors", _pu), ("cum_n_neighbors", _pu), ("neighbors", _pu),
+        ("point2boundary", _pu), ("boundary2point", _pu),
+        ("boundary_condition", _pi), ("boundary_temperature", _pd),
+        ("ray_direction", _pd), ("ray_weight", _pd), ("antipod", _pu),
+        ("center", C.c_double * 3),
+        ("temperature", _pd), ("vturb2", _pd),
+        ("line", _pd), ("sorted_line", _pd), ("sorted_line_map", _pu),
+        ("emissivity", _pd), ("opacity", _pd), ("inverse_width", _pd),
+        ("interp_crit", _pd), ("dshift_max", _pd),
+        ("max_inverse_mass", C.c_double),
+        ("line_species", _pu), ("line_k", _pu), ("line_A", _pd),
+        ("species_inverse_mass", _pd), ("quad_weights", _pd),
+        ("nu", _pd), ("f_line", _pu), ("f_species", _pu), ("f_k", _pu), ("f_z", _pu),
+        ("f_in_line", _pb),
+    ]
+
+
+class MgxOptions(C.Structure):
+    _fields_ = [
+        ("approx", C.c_int32), ("n_off_diag", C.c_uint32), ("precision", C.c_int32),
+        ("reserved0", C.c_int32),
+        ("min_opacity", C.c_double), ("max_distance_opacity_contribution", C.c_double),
+        ("max_interpolation_diff", C.c_double),
+    ]
+
+
+class MgxLinedata(C.Structure):
+    _fields_ = [
+        ("num", C.c_uint32), ("nlev", C.c_uint32), ("nrad", C.c_uint32),
+        ("reserved0", C.c_uint32),
+        ("inverse_mass", C.c_double),
+        ("energy", _pd), ("weight", _pd), ("irad", _pu), ("jrad", _pu),
+        ("A", _pd), ("Bs", _pd), ("Ba", _pd), ("frequency", _pd),
+    ]
+
+
+# (name, restype, argtypes) of every function include/mgx.h declares
+_VP = C.c_void_p
+SIGNATURES = [
+    ("mgx_last_error", C.c_char_p, []),
+    ("mgx_version", C.c_char_p, []),
+    ("mgx_feautrier_forward_device", C.c_int,
+     [C.POINTER(MgxModel), C.POINTER(MgxOptions), _VP, _VP, _VP, _VP]),
+    ("mgx_trace_counts_device", C.c_int,
+     [C.POINTER(MgxModel), C.POINTER(MgxOptions), _VP, _VP]),
+    ("mgx_trace_rays_device", C.c_int,
+     [C.POINTER(MgxModel), C.POINTER(MgxOptions), C.c_uint32, _VP, _VP, _VP, C.c_uint32,
+      _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
+    ("mgx_compute_inverse_widths", C.c_int,
+     [C.c_uint32, C.c_uint32, C.POINTER(MgxLinedata), _VP, _VP, _VP]),
+    ("mgx_compute_lte", C.c_int,
+     [C.c_uint32, C.c_uint32, _VP, _VP, C.c_uint32, C.POINTER(MgxLinedata), C.c_double,
+      C.c_double, _VP, _VP, _VP, _VP]),
+    ("mgx_compute_spectral_discretisation", C.c_int,
+     [C.c_uint32, C.c_uint32, C.POINTER(MgxLinedata), C.c_uint32, _VP, _VP, _VP, _VP, _VP,
+      _VP, _VP, _VP, _VP]),
+    ("mgx_compute_image_discretisation", C.c_int,
+     [C.c_uint32, C.c_double, C.c_double, C.c_uint32, _VP, _VP, _VP, _VP, _VP, _VP]),
+    ("mgx_compute_dshift_max", C.c_int,
+     [C.c_uint32, C.c_uint32, C.POINTER(MgxLinedata), _VP, _VP, C.c_double, _VP]),
+    ("mgx_compute_interp_crit", C.c_int,
+     [C.c_uint32, C.c_uint32, C.POINTER(MgxLinedata), _VP, _VP, _VP]),
+    ("mgx_sort_lines", C.c_int, [C.c_uint32, _VP, _VP, _VP]),
+]
+
+_lib = None
+
+
+def lib():
+    """Load libmgx.so once (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        # torch carries its own HIP runtime; loading it first makes libmgx bind
+        # to that same runtime (a second HIP runtime in one process sees no GPU)
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
+        h = C.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            f = getattr(h, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = h
+    return _lib
+
+
+class MgxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[mgx error {code}] {msg}")
+        self.code = code
+
+
+def check(code):
+    if code != MGX_OK:
+        msg = lib().mgx_last_error().decode(errors="replace")
+        raise MgxError(code, msg)
+    return code
+
+
+def ptr(a):
+    """Address of a numpy array or torch tensor (None -> NULL)."""
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):
+        return a.data_ptr()
+    return a.ctypes.data

@@ -1,0 +1,1112 @@
+/*
+ * gfx950 kernels of the plain-method Feautrier formal solver, and the C-ABI
+ * entry points that launch them.
+ *
+ * Work decomposition (one 64-lane wavefront per origin point o):
+ *   for each ray pair rr (sequential, so J and Lambda accumulate in the
+ *   reference's order without atomics):
+ *     1. walk the neighbour lists from o along ray rr and along its antipod.
+ *        The 64 lanes evaluate the neighbours of the current point in
+ *        parallel and a lexicographic (d2, list index) arg-min selects the
+ *        next point; that is the first-wins strict "<" of
+ *        Geometry::get_next_general_geometry (geometry.tpp:14-44).  The walk
+ *        (point, dZ, Doppler shift per step) is staged in LDS.
+ *     2. lanes take the quadrature frequencies of up to two lines (32 lanes
+ *        per line).  Each lane expands its line's interpolation points on the
+ *        fly (set_data_for_line, solver.tpp:1057-1097) and solves the
+ *        second-order Feautrier system of its frequency with a two-sided
+ *        elimination: backward from the antipod-side boundary to the origin,
+ *        forward from the rr-side boundary to the origin, then joined at the
+ *        origin.  This gives u(o) and the Lambda diagonal with O(1) state per
+ *        lane; it is algebraically the reference's forward elimination plus
+ *        back substitution (solver.tpp:2194-2318), with the same source,
+ *        optical-depth and boundary terms.
+ *     3. u(rr,o,f) is stored coalesced, J(o,f) and Lambda(o,l) updated by the
+ *        owning wave.
+ *
+ * Numerics: FP64 throughout, no FMA contraction (-ffp-contract=off), and the
+ * float transcendentals of mgx_fmath.h, so the traced lists and the source /
+ * optical-depth terms are bit-identical to the CPU restatement.
+ */
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "mgx.h"
+#include "mgx_constants.h"
+#include "mgx_error.h"
+#include "mgx_fmath.h"
+
+#define MGX_WAVE 64
+
+namespace {
+
+enum { ERR_CAP = 1, ERR_NO_NEXT = 2, ERR_OUT = 4 };
+
+/* Device copy of the model view plus the per-call derived arrays. */
+struct KParams {
+    mgx_model m;
+    mgx_options op;
+    const double4 *lpack; /* (P, L): inverse_width, emissivity, opacity, emissivity/opacity */
+    const double *ub;     /* (P): sqrt(2kT/(amu c^2) * max_inverse_mass + vturb2) */
+    double log_max_interp; /* (double)logf((float)max_interpolation_diff) */
+    uint32_t cap;         /* LDS walk capacity (steps per half ray) */
+    uint32_t nslots;      /* number of 32-lane line slots = L * ceil(Q/32) */
+    uint32_t slots_per_line;
+    uint32_t *status;     /* [0] error bits, [1] largest walk length seen */
+};
+
+__device__ inline double dot3(const double *a, const double *b)
+{
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+__device__ inline const double *ray_dir(const mgx_model &m, uint32_t o, uint32_t r)
+{
+    return m.adaptive_directions ? &m.ray_direction[3 * ((size_t)o * m.nrays + r)]
+                                 : &m.ray_direction[3 * (size_t)r];
+}
+
+__device__ inline double ray_weight(const mgx_model &m, uint32_t o, uint32_t r)
+{
+    return m.adaptive_directions ? m.ray_weight[(size_t)o * m.nrays + r] : m.ray_weight[r];
+}
+
+/* ------------------------------------------------------------------------- */
+/* Doppler shift (Geometry::get_shift<CoMoving>, geometry.tpp:204-207, 379-396) */
+
+__device__ inline double shift_comoving(const mgx_model &m, const double *origin,
+                                        const double *ovel, const double *dir, uint32_t c,
+                                        double Z)
+{
+    if (m.spherical_symmetry) {
+        if (dot3(origin, origin) == 0.0) return 1.0;
+        const double *pc = &m.position[3 * (size_t)c];
+        if (pc[0] == 0.0) return 1.0 - dot3(ovel, dir);
+        const double Rcos_plus_Z = dot3(origin, dir) + Z;
+        return 1.0 + (m.velocity[3 * (size_t)c] * Rcos_plus_Z / pc[0] - dot3(ovel, dir));
+    }
+    double dv[3];
+    dv[0] = m.velocity[3 * (size_t)c + 0] - ovel[0];
+    dv[1] = m.velocity[3 * (size_t)c + 1] - ovel[1];
+    dv[2] = m.velocity[3 * (size_t)c + 2] - ovel[2];
+    return 1.0 + dot3(dv, dir);
+}
+
+/* ------------------------------------------------------------------------- */
+/* next point on the ray, wave-cooperative                                    */
+
+/* lexicographic arg-min over the wave: smaller d2 wins, ties go to the smaller list index */
+__device__ inline void wave_argmin(double &d2, uint32_t &idx, double &zn)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const double od2 = __shfl_xor(d2, off, MGX_WAVE);
+        const uint32_t oidx = __shfl_xor(idx, off, MGX_WAVE);
+        const double ozn = __shfl_xor(zn, off, MGX_WAVE);
+        const bool take = (od2 < d2) || (od2 == d2 && oidx < idx);
+        if (take) {
+            d2 = od2;
+            idx = oidx;
+            zn = ozn;
+        }
+    }
+}
+
+/* get_next_general_geometry<Defaulttracer> (geometry.tpp:14-44), all lanes return the same */
+__device__ inline uint32_t next_general(const mgx_model &m, const double *origin,
+                                        const double *dir, uint32_t c, double &Z, double &dZ)
+{
+    const uint32_t lane = threadIdx.x & (MGX_WAVE - 1);
+    const uint32_t n_nbs = m.n_neighbors[c];
+    const uint32_t cum = m.cum_n_neighbors[c];
+    double best_d2 = __builtin_inf();
+    uint32_t best_i = 0xffffffffu;
+    double best_zn = 0.0;
+    for (uint32_t i = lane; i < n_nbs; i += MGX_WAVE) {
+        const uint32_t n = m.neighbors[cum + i];
+        double R[3];
+        R[0] = m.position[3 * (size_t)n + 0] - origin[0];
+        R[1] = m.position[3 * (size_t)n + 1] - origin[1];
+        R[2] = m.position[3 * (size_t)n + 2] - origin[2];
+        const double Z_new = dot3(R, dir);
+        if (Z_new > Z) {
+            const double d2 = dot3(R, R) - Z_new * Z_new;
+            if (d2 < DBL_MAX && d2 < best_d2) {
+                best_d2 = d2;
+                best_i = i;
+                best_zn = Z_new;
+            }
+        }
+    }
+    wave_argmin(best_d2, best_i, best_zn);
+    if (best_i == 0xffffffffu) return m.npoints;
+    dZ = best_zn - Z;
+    Z += dZ;
+    return m.neighbors[cum + best_i];
+}
+
+/* get_next_spherical_symmetry (geometry.tpp:118-159), evaluated redundantly by every lane */
+__device__ inline uint32_t next_spherical(const mgx_model &m, const double *origin,
+                                          const double *dir, uint32_t c, double &Z, double &dZ)
+{
+    uint32_t next;
+    const double Rcos = dot3(origin, dir);
+    const double R2 = dot3(origin, origin);
+    const double Rsin2 = R2 - Rcos * Rcos;
+    const double Rcos_plus_Z = Rcos + Z;
+    if (Z < -Rcos) {
+        if (c <= 0) return m.npoints;
+        const double *pm = &m.position[3 * (size_t)(c - 1)];
+        if (dot3(pm, pm) >= Rsin2) {
+            next = c - 1;
+            const double *pn = &m.position[3 * (size_t)next];
+            dZ = -sqrt(dot3(pn, pn) - Rsin2) - Rcos_plus_Z;
+        } else {
+            next = c;
+            dZ = -2.0 * Rcos_plus_Z;
+        }
+    } else {
+        if (c >= m.npoints - 1) return m.npoints;
+        next = c + 1;
+        const double *pn = &m.position[3 * (size_t)next];
+        dZ = +sqrt(dot3(pn, pn) - Rsin2) - Rcos_plus_Z;
+    }
+    Z += dZ;
+    return next;
+}
+
+/*
+ * Walk one half ray from o along ray r (trace_ray_for_line's geometry,
+ * solver.tpp:774-800) into LDS: step s goes from crt_s (o for s = 0) to
+ * s_nxt[s] over distance s_dZ[s]; s_shift[s] is the shift of s_nxt[s].
+ * Returns 0 or an ERR_* code; nsteps and shift0 (shift of o) are set.
+ */
+__device__ int walk_half(const KParams &kp, uint32_t o, uint32_t r, uint32_t *s_nxt, double *s_dZ,
+                         double *s_shift, uint32_t &nsteps, double &shift0)
+{
+    const mgx_model &m = kp.m;
+    const uint32_t lane = threadIdx.x & (MGX_WAVE - 1);
+    const double *origin = &m.position[3 * (size_t)o];
+    const double *ovel = &m.velocity[3 * (size_t)o];
+    const double *dir = ray_dir(m, o, r);
+    double Z = 0.0, dZ = 0.0;
+    uint32_t crt = o;
+    nsteps = 0;
+    shift0 = shift_comoving(m, origin, ovel, dir, o, 0.0);
+    for (;;) {
+        const uint32_t nxt = m.spherical_symmetry ? next_spherical(m, origin, dir, crt, Z, dZ)
+                                                  : next_general(m, origin, dir, crt, Z, dZ);
+        if (nxt >= m.npoints) {
+            if (nsteps == 0) break; /* nothing in this direction */
+            return ERR_NO_NEXT;
+        }
+        if (nsteps >= kp.cap) return ERR_CAP;
+        const double sh = shift_comoving(m, origin, ovel, dir, nxt, Z);
+        if (lane == 0) {
+            s_nxt[nsteps] = nxt;
+            s_dZ[nsteps] = dZ;
+            s_shift[nsteps] = sh;
+        }
+        nsteps++;
+        if (m.point2boundary[nxt] != m.npoints) break;
+        crt = nxt;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* per-line interpolation count (InterpHelper::get_n_interp_for_line, :55-75)  */
+
+__device__ inline uint32_t n_interp_for_line(const KParams &kp, uint32_t l, uint32_t c, uint32_t n)
+{
+    const uint32_t L = kp.m.nlines;
+    const double curr_source = kp.m.interp_crit[(size_t)c * L + l];
+    const double next_source = kp.m.interp_crit[(size_t)n * L + l];
+    const double source_diff = (double)fabsf(mgx_logf((float)(next_source / curr_source)));
+    const uint32_t k = (uint32_t)ceil(source_diff / kp.log_max_interp);
+    return (k > 1) ? k : 1;
+}
+
+__device__ inline double interp_linear(double a, double b, double t)
+{
+    if (t == 0.0) return a;
+    if (t == 1.0) return b;
+    return (b - a) * t + a;
+}
+
+__device__ inline double interp_log(double a, double b, double t)
+{
+    if (t == 0.0) return a;
+    if (t == 1.0) return b;
+    return a * (double)mgx_powf((float)(b / a), (float)t);
+}
+
+/* ------------------------------------------------------------------------- */
+/* emissivity, opacity and optical depth (solver.tpp:1106-2455)               */
+
+__device__ inline double gaussian(double inverse_width, double diff)
+{
+    const double sqrt_exp = inverse_width * diff;
+    return inverse_width * MGX_INVERSE_SQRT_PI * (double)mgx_expf((float)(-sqrt_exp * sqrt_exp));
+}
+
+__device__ inline double planck(double temp, double freq)
+{
+    return MGX_TWO_HH_OVER_CC_SQUARED * (freq * freq * freq) /
+           (double)mgx_expm1f((float)(MGX_HH_OVER_KB * freq / temp));
+}
+
+__device__ inline double boundary_intensity(const mgx_model &m, uint32_t p, double freq)
+{
+    const uint32_t b = m.point2boundary[p];
+    const int32_t bc = (b < m.nboundary) ? m.boundary_condition[b] : MGX_BC_CMB;
+    if (bc == MGX_BC_ZERO) return 0.0;
+    if (bc == MGX_BC_THERMAL) return planck(m.boundary_temperature[b], freq);
+    return planck(MGX_T_CMB, freq);
+}
+
+__device__ inline uint32_t lower_bound_d(const double *a, uint32_t n, double v)
+{
+    uint32_t lo = 0, len = n;
+    while (len > 0) {
+        const uint32_t h = len / 2;
+        if (a[lo + h] < v) {
+            lo = lo + h + 1;
+            len = len - h - 1;
+        } else {
+            len = h;
+        }
+    }
+    return lo;
+}
+
+__device__ inline uint32_t upper_bound_d(const double *a, uint32_t n, double v)
+{
+    uint32_t lo = 0, len = n;
+    while (len > 0) {
+        const uint32_t h = len / 2;
+        if (!(v < a[lo + h])) {
+            lo = lo + h + 1;
+            len = len - h - 1;
+        } else {
+            len = h;
+        }
+    }
+    return lo;
+}
+
+__device__ inline double4 lp(const KParams &kp, uint32_t p, uint32_t l)
+{
+    return kp.lpack[(size_t)p * kp.m.nlines + l];
+}
+
+/* get_eta_and_chi_interpolated<approx> (solver.tpp:1177-1202, 1260-1302, 1345-1351) */
+template <int APPROX>
+__device__ void eta_chi_interp(const KParams &kp, uint32_t p1, uint32_t p2, double t, uint32_t ll,
+                               double freq, double &eta, double &chi)
+{
+    const mgx_model &m = kp.m;
+    if (APPROX == MGX_APPROX_ONE_LINE) {
+        const double4 a = lp(kp, p1, ll), b = lp(kp, p2, ll);
+        const double diff = freq - m.line[ll];
+        const double prof = gaussian(interp_linear(a.x, b.x, t), diff);
+        eta = prof * interp_log(a.y, b.y, t);
+        chi = prof * interp_log(a.z, b.z, t) + kp.op.min_opacity;
+        return;
+    }
+    eta = 0.0;
+    chi = kp.op.min_opacity;
+    uint32_t lo = 0, hi = m.nlines;
+    if (APPROX == MGX_APPROX_CLOSE_LINES) {
+        const double W = kp.op.max_distance_opacity_contribution * (freq * kp.ub[p1]);
+        lo = lower_bound_d(m.sorted_line, m.nlines, freq - W);
+        hi = upper_bound_d(m.sorted_line, m.nlines, freq + W);
+    }
+    for (uint32_t s = lo; s < hi; s++) {
+        uint32_t l;
+        double diff;
+        if (APPROX == MGX_APPROX_CLOSE_LINES) {
+            l = m.sorted_line_map[s];
+            diff = freq - m.sorted_line[s];
+        } else {
+            l = s;
+            diff = freq - m.line[l];
+        }
+        const double4 a = lp(kp, p1, l), b = lp(kp, p2, l);
+        const double prof = gaussian(interp_linear(a.x, b.x, t), diff);
+        eta += prof * interp_log(a.y, b.y, t);
+        chi += prof * interp_log(a.z, b.z, t);
+    }
+}
+
+/* compute_dtau_single_line (solver.tpp:2374-2455) */
+__device__ double dtau_single_line(const KParams &kp, uint32_t ci, uint32_t ni, uint32_t cii,
+                                   uint32_t nii, uint32_t l, double curr_freq, double next_freq,
+                                   double curr_interp, double next_interp, double dz)
+{
+    const double4 c = lp(kp, ci, l), cI = lp(kp, cii, l), n = lp(kp, ni, l), nI = lp(kp, nii, l);
+    const double linefreq = kp.m.line[l];
+    const double iw_c = interp_linear(c.x, cI.x, curr_interp);
+    const double iw_n = interp_linear(n.x, nI.x, next_interp);
+    const double avg_iw = (iw_c + iw_n) / 2.0;
+    const double op_c = interp_log(c.z, cI.z, curr_interp);
+    const double op_n = interp_log(n.z, nI.z, next_interp);
+    if (curr_freq == next_freq) {
+        const double diff = curr_freq - linefreq;
+        const double prof = gaussian(avg_iw, diff);
+        const double avg_op = (op_c + op_n) / 2.0;
+        return dz * (prof * avg_op + kp.op.min_opacity);
+    }
+    const double next_pos = (linefreq - next_freq) * avg_iw;
+    const double curr_pos = (linefreq - curr_freq) * avg_iw;
+    const double diff_pos = next_pos - curr_pos;
+    const double avg_op = (op_n + op_c) / 2.0;
+    const float erf_diff = mgx_erff((float)next_pos) - mgx_erff((float)curr_pos);
+    const double erfterm = avg_op / diff_pos / 2.0 * (double)erf_diff;
+    return dz * (avg_iw * erfterm + kp.op.min_opacity);
+}
+
+/* compute_S_dtau_line_integrated<approx> (solver.tpp:1879-2111) */
+template <int APPROX>
+__device__ void S_dtau_line_integrated(const KParams &kp, uint32_t cp, uint32_t np, uint32_t cpi,
+                                       uint32_t npi, uint32_t lineidx, double currfreq,
+                                       double nextfreq, double curr_interp, double next_interp,
+                                       double dZ, double &dtau, double &Scurr, double &Snext)
+{
+    const mgx_model &m = kp.m;
+    if (APPROX == MGX_APPROX_ONE_LINE) {
+        dtau = dtau_single_line(kp, cp, np, cpi, npi, lineidx, currfreq, nextfreq, curr_interp,
+                                next_interp, dZ);
+        Scurr = interp_log(lp(kp, cp, lineidx).w, lp(kp, cpi, lineidx).w, curr_interp);
+        Snext = interp_log(lp(kp, np, lineidx).w, lp(kp, npi, lineidx).w, next_interp);
+        return;
+    }
+    double sum_dtau = 0.0, sum_Sc = 0.0, sum_Sn = 0.0;
+    uint32_t lo = 0, hi = m.nlines;
+    if (APPROX == MGX_APPROX_CLOSE_LINES) {
+        double left_freq, right_freq;
+        if (currfreq < nextfreq) {
+            left_freq = currfreq;
+            right_freq = nextfreq;
+        } else {
+            right_freq = currfreq;
+            left_freq = nextfreq;
+        }
+        const double cb = kp.op.max_distance_opacity_contribution * (right_freq * kp.ub[cp]);
+        const double nb = kp.op.max_distance_opacity_contribution * (right_freq * kp.ub[np]);
+        const double W = (cb < nb) ? nb : cb;
+        lo = lower_bound_d(m.sorted_line, m.nlines, left_freq - W);
+        hi = upper_bound_d(m.sorted_line, m.nlines, right_freq + W);
+    }
+    for (uint32_t s = lo; s < hi; s++) {
+        const uint32_t l = (APPROX == MGX_APPROX_CLOSE_LINES) ? m.sorted_line_map[s] : s;
+        const double line_dtau = dtau_single_line(kp, cp, np, cpi, npi, l, currfreq, nextfreq,
+                                                  curr_interp, next_interp, dZ);
+        const double Sc = interp_log(lp(kp, cp, l).w, lp(kp, cpi, l).w, curr_interp);
+        const double Sn = interp_log(lp(kp, np, l).w, lp(kp, npi, l).w, next_interp);
+        sum_dtau += line_dtau;
+        sum_Sc += line_dtau * Sc;
+        sum_Sn += line_dtau * Sn;
+    }
+    double d = sum_dtau;
+    if (APPROX == MGX_APPROX_CLOSE_LINES) {
+        const double bound_min_dtau = kp.op.min_opacity * dZ;
+        if (-bound_min_dtau < d) d = (bound_min_dtau < d) ? d : bound_min_dtau;
+    }
+    dtau = d;
+    Scurr = sum_Sc / d;
+    Snext = sum_Sn / d;
+}
+
+/* ------------------------------------------------------------------------- */
+/* traced entries                                                             */
+
+/* One entry of the expanded list: point, interpolation partner, factor, shift. */
+struct Entry {
+    uint32_t nr, nri;
+    double ifac, shift;
+};
+
+/* Cached emissivity/opacity at an entry (valid = computed). */
+struct PointEC {
+    double eta, chi;
+    bool valid;
+};
+
+/*
+ * One segment (compute_source_dtau, solver.tpp:2140-2186) between list
+ * entries cu (closer to the rr-side boundary) and nx.  Returns dtau and the
+ * source values of the two rows; uses/updates the caches of the two entries.
+ */
+template <int APPROX>
+__device__ inline void segment(const KParams &kp, const Entry &cu, const Entry &nx, double dZ,
+                               double freq, uint32_t line, PointEC &ec_cu, PointEC &ec_nx,
+                               double &dtau, double &S_cu, double &S_nx)
+{
+    const double dshift = nx.shift - cu.shift;
+    const double dshift_abs = fabs(dshift);
+    const double a = kp.m.dshift_max[cu.nr], b = kp.m.dshift_max[nx.nr];
+    const double dshift_max = (b < a) ? b : a;
+    const double cf = freq * cu.shift, nf = freq * nx.shift;
+    if (dshift_abs > dshift_max) {
+        S_dtau_line_integrated<APPROX>(kp, cu.nr, nx.nr, cu.nri, nx.nri, line, cf, nf, cu.ifac,
+                                       nx.ifac, dZ, dtau, S_cu, S_nx);
+    } else {
+        if (!ec_cu.valid) {
+            eta_chi_interp<APPROX>(kp, cu.nr, cu.nri, cu.ifac, line, cf, ec_cu.eta, ec_cu.chi);
+            ec_cu.valid = true;
+        }
+        if (!ec_nx.valid) {
+            eta_chi_interp<APPROX>(kp, nx.nr, nx.nri, nx.ifac, line, nf, ec_nx.eta, ec_nx.chi);
+            ec_nx.valid = true;
+        }
+        S_cu = ec_cu.eta / ec_cu.chi;
+        S_nx = ec_nx.eta / ec_nx.chi;
+        dtau = 0.5 * (ec_cu.chi + ec_nx.chi) * dZ;
+    }
+}
+
+/*
+ * Iterator over one half ray's expanded entries from the outer end inward.
+ * Step s of the walk (crt_s -> nxt_s) expands into m_s entries i = m_s..1;
+ * entry (s, i) is the i-th interpolation point (i = m_s at nxt_s) and the
+ * segment from it to the next entry inward has length dZ_s / m_s.
+ */
+struct HalfIter {
+    int s;     /* current step, -1 = at the origin */
+    uint32_t i, m_s;
+    uint32_t crt, nxt;
+    double shift_crt, shift_nxt, dZ, dshift_interpl;
+};
+
+__device__ inline void hi_load_step(const KParams &kp, HalfIter &it, uint32_t l, uint32_t o,
+                                    double shift0, const uint32_t *s_nxt, const double *s_dZ,
+                                    const double *s_shift)
+{
+    const int s = it.s;
+    it.nxt = s_nxt[s];
+    it.crt = (s == 0) ? o : s_nxt[s - 1];
+    it.shift_nxt = s_shift[s];
+    it.shift_crt = (s == 0) ? shift0 : s_shift[s - 1];
+    it.m_s = n_interp_for_line(kp, l, it.crt, it.nxt);
+    it.i = it.m_s;
+    const double dshift = it.shift_nxt - it.shift_crt;
+    it.dZ = (it.m_s > 1) ? s_dZ[s] / it.m_s : s_dZ[s];
+    it.dshift_interpl = dshift / it.m_s;
+}
+
+__device__ inline Entry hi_entry(const HalfIter &it, uint32_t o)
+{
+    Entry e;
+    if (it.s < 0) {
+        e.nr = o;
+        e.nri = o;
+        e.ifac = 0.0;
+        e.shift = 1.0;
+    } else if (it.m_s > 1) {
+        e.nr = it.nxt;
+        e.nri = it.crt;
+        e.ifac = 1.0 - ((double)it.i) / it.m_s;
+        e.shift = it.shift_crt + it.i * it.dshift_interpl;
+    } else {
+        e.nr = it.nxt;
+        e.nri = it.crt;
+        e.ifac = 0.0;
+        e.shift = it.shift_nxt;
+    }
+    return e;
+}
+
+/* advance inward; returns the length of the segment just crossed */
+__device__ inline double hi_advance(const KParams &kp, HalfIter &it, uint32_t l, uint32_t o,
+                                    double shift0, const uint32_t *s_nxt, const double *s_dZ,
+                                    const double *s_shift)
+{
+    const double d = it.dZ;
+    if (it.i > 1) {
+        it.i--;
+    } else {
+        it.s--;
+        if (it.s >= 0) hi_load_step(kp, it, l, o, shift0, s_nxt, s_dZ, s_shift);
+    }
+    return d;
+}
+
+/* ------------------------------------------------------------------------- */
+/* the forward-solve kernel                                                   */
+
+template <int APPROX>
+__global__ __launch_bounds__(MGX_WAVE) void feautrier_kernel(KParams kp, double *u, double *J,
+                                                             double *lam)
+{
+    extern __shared__ unsigned char smem[];
+    const mgx_model &m = kp.m;
+    const uint32_t o = blockIdx.x;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t cap = kp.cap;
+    uint32_t *s_nxt0 = reinterpret_cast<uint32_t *>(smem);
+    uint32_t *s_nxt1 = s_nxt0 + cap;
+    double *s_dZ0 = reinterpret_cast<double *>(smem + 8 * ((2 * cap * 4 + 7) / 8));
+    double *s_dZ1 = s_dZ0 + cap;
+    double *s_sh0 = s_dZ1 + cap;
+    double *s_sh1 = s_sh0 + cap;
+
+    const uint32_t P = m.npoints, F = m.nfreqs, L = m.nlines, Q = m.nquads;
+    const size_t PF = (size_t)P * F;
+
+    for (uint32_t rr = 0; rr < m.hnrays; rr++) {
+        const uint32_t ar = m.antipod[rr];
+        uint32_t S0, S1;
+        double sh00, sh01;
+        int e = walk_half(kp, o, rr, s_nxt0, s_dZ0, s_sh0, S0, sh00);
+        if (!e) e = walk_half(kp, o, ar, s_nxt1, s_dZ1, s_sh1, S1, sh01);
+        if (e) {
+            if (lane == 0) {
+                atomicOr(&kp.status[0], (uint32_t)e);
+                if (e == ERR_CAP) atomicMax(&kp.status[1], kp.cap + 1);
+            }
+            return;
+        }
+        __syncthreads();
+        const double w = ray_weight(m, o, rr);
+
+        for (uint32_t slot0 = 0; slot0 < kp.nslots; slot0 += 2) {
+            const uint32_t slot = slot0 + (lane >> 5);
+            const uint32_t l = slot / kp.slots_per_line;
+            const uint32_t z = (slot % kp.slots_per_line) * 32 + (lane & 31);
+            const bool active = (slot < kp.nslots) && (z < Q);
+            double Lval = 0.0;
+            if (active) {
+                const uint32_t f = l * Q + z;
+                const size_t of = (size_t)o * F + f;
+                const double freq = m.nu[of];
+                double uc;
+                if (S0 == 0 && S1 == 0) {
+                    uc = boundary_intensity(m, o, freq);
+                    J[of] = ((rr == 0) ? 0.0 : J[of]) + 2.0 * w * uc;
+                } else {
+                    /* ---- backward sweep over the antipod half (last -> centre+1) ---- */
+                    double GP = 0.0, wv = 0.0, dtau_c = 0.0, S_c = 0.0;
+                    PointEC ec_cent = {0.0, 0.0, false};
+                    if (S1 > 0) {
+                        HalfIter it;
+                        it.s = (int)S1 - 1;
+                        hi_load_step(kp, it, l, o, sh01, s_nxt1, s_dZ1, s_sh1);
+                        Entry outer = hi_entry(it, o);
+                        PointEC ec_out = {0.0, 0.0, false};
+                        bool is_last = true;
+                        double prev_dtau = 0.0, prev_Scurr = 0.0;
+                        for (;;) {
+                            const double dz = hi_advance(kp, it, l, o, sh01, s_nxt1, s_dZ1, s_sh1);
+                            const Entry inner = hi_entry(it, o);
+                            PointEC ec_in = {0.0, 0.0, false};
+                            double dtau, S_in, S_out;
+                            segment<APPROX>(kp, inner, outer, dz, freq, l, ec_in, ec_out, dtau,
+                                            S_in, S_out);
+                            if (is_last) {
+                                const double I_l = boundary_intensity(m, outer.nr, freq * outer.shift);
+                                const double idt = 1.0 / dtau;
+                                const double A_l = 2.0 * idt * idt;
+                                const double Bl_min_Al = 1.0 + 2.0 * idt;
+                                const double Bl = Bl_min_Al + A_l;
+                                const double GG = 0.5 * Bl_min_Al * dtau * dtau;
+                                GP = GG / (1.0 + GG);
+                                wv = (S_out + 2.0 * I_l * idt) / Bl;
+                                is_last = false;
+                            } else {
+                                /* row "outer": A side = this segment, C side = previous one */
+                                const double dtau_avg = 0.5 * (dtau + prev_dtau);
+                                const double iA = dtau_avg * dtau;
+                                const double iC = dtau_avg * prev_dtau;
+                                const double C = 1.0 / iC;
+                                const double GG = (C * GP + 1.0) * iA;
+                                const double GI = 1.0 / (1.0 + GG);
+                                GP = GG / (1.0 + GG);
+                                wv = (C * wv + prev_Scurr) * GI * iA;
+                            }
+                            prev_dtau = dtau;
+                            prev_Scurr = S_in;
+                            outer = inner;
+                            ec_out = ec_in;
+                            if (it.s < 0) break;
+                        }
+                        dtau_c = prev_dtau;
+                        S_c = prev_Scurr;
+                        ec_cent = ec_out;
+                    }
+                    /* ---- forward sweep over the rr half (first -> centre) ---- */
+                    double FF = 0.0, FI = 0.0, Su = 0.0, iC_c = 0.0;
+                    double prev_dtau = 0.0, prev_Snext = 0.0;
+                    bool is_first = true;
+                    if (S0 > 0) {
+                        HalfIter it;
+                        it.s = (int)S0 - 1;
+                        hi_load_step(kp, it, l, o, sh00, s_nxt0, s_dZ0, s_sh0);
+                        Entry outer = hi_entry(it, o);
+                        PointEC ec_out = {0.0, 0.0, false};
+                        for (;;) {
+                            const double dz = hi_advance(kp, it, l, o, sh00, s_nxt0, s_dZ0, s_sh0);
+                            const Entry inner = hi_entry(it, o);
+                            PointEC ec_in = (it.s < 0) ? ec_cent : PointEC{0.0, 0.0, false};
+                            double dtau, S_out, S_in;
+                            segment<APPROX>(kp, outer, inner, dz, freq, l, ec_out, ec_in, dtau,
+                                            S_out, S_in);
+                            if (is_first) {
+                                const double I_f = boundary_intensity(m, outer.nr, freq * outer.shift);
+                                const double idt = 1.0 / dtau;
+                                const double Cf = 2.0 * idt * idt;
+                                const double Bf_min_Cf = 1.0 + 2.0 * idt;
+                                const double Bf = Bf_min_Cf + Cf;
+                                Su = S_out + 2.0 * I_f * idt;
+                                Su /= Bf;
+                                FF = 0.5 * Bf_min_Cf * dtau * dtau;
+                                FI = 1.0 / (1.0 + FF);
+                                is_first = false;
+                            } else {
+                                /* row "outer": A side = previous segment, C side = this one */
+                                const double dtau_avg = 0.5 * (prev_dtau + dtau);
+                                const double iA = dtau_avg * prev_dtau;
+                                const double iC = dtau_avg * dtau;
+                                const double A = 1.0 / iA;
+                                FF = (A * FF * FI + 1.0) * iC;
+                                FI = 1.0 / (1.0 + FF);
+                                Su = (A * Su + S_out) * FI * iC;
+                            }
+                            prev_dtau = dtau;
+                            prev_Snext = S_in;
+                            outer = inner;
+                            ec_out = ec_in;
+                            if (it.s < 0) break;
+                        }
+                    }
+                    /* ---- join at the origin ---- */
+                    double Lc;
+                    if (S1 > 0) {
+                        if (is_first) { /* the origin is the first row */
+                            const double I_f = boundary_intensity(m, o, freq);
+                            const double idt = 1.0 / dtau_c;
+                            const double Cf = 2.0 * idt * idt;
+                            iC_c = 1.0 / Cf;
+                            const double Bf_min_Cf = 1.0 + 2.0 * idt;
+                            const double Bf = Bf_min_Cf + Cf;
+                            Su = S_c + 2.0 * I_f * idt;
+                            Su /= Bf;
+                            FF = 0.5 * Bf_min_Cf * dtau_c * dtau_c;
+                        } else {
+                            const double dtau_avg = 0.5 * (prev_dtau + dtau_c);
+                            const double iA = dtau_avg * prev_dtau;
+                            iC_c = dtau_avg * dtau_c;
+                            const double A = 1.0 / iA;
+                            FF = (A * FF * FI + 1.0) * iC_c;
+                            FI = 1.0 / (1.0 + FF);
+                            Su = (A * Su + S_c) * FI * iC_c;
+                        }
+                        const double den = FF + GP;
+                        uc = (Su * (1.0 + FF) + wv) / den;
+                        Lc = iC_c / den;
+                    } else { /* the origin is the last row */
+                        const double I_l = boundary_intensity(m, o, freq);
+                        const double idt = 1.0 / prev_dtau;
+                        const double A_l = 2.0 * idt * idt;
+                        const double Bl_min_Al = 1.0 + 2.0 * idt;
+                        const double Bl = Bl_min_Al + A_l;
+                        const double denominator = 1.0 / (Bl * FF + Bl_min_Al);
+                        const double Sl = prev_Snext + 2.0 * I_l * idt;
+                        uc = (A_l * Su + Sl) * (1.0 + FF) * denominator;
+                        Lc = (1.0 + FF) / (Bl_min_Al + Bl * FF);
+                    }
+                    J[of] = ((rr == 0) ? 0.0 : J[of]) + uc * 2.0 * w;
+                    /* ---- Lambda diagonal (update_Lambda, solver.tpp:1761-1800) ---- */
+                    if (lam && m.f_in_line[f]) {
+                        const uint32_t spec = m.f_species[f];
+                        const uint32_t k = m.f_k[f];
+                        const double freq_line = m.line[l];
+                        const double w_ang = 2.0 * w;
+                        const double constante =
+                            m.line_A[l] * m.quad_weights[(size_t)spec * Q + m.f_z[f]] * w_ang;
+                        double eta, chi;
+                        eta_chi_interp<APPROX>(kp, o, o, 0.0, k, freq_line, eta, chi);
+                        const double inverse_chi = 1.0 / chi;
+                        const double frq = freq * 1.0;
+                        const double inv_mass = m.species_inverse_mass[spec];
+                        const double width =
+                            freq_line * sqrt(MGX_TWO_KB_OVER_AMU_CC_SQUARED * inv_mass *
+                                                 m.temperature[o] +
+                                             m.vturb2[o]);
+                        const double inverse_width = 1.0 / width;
+                        const double se = inverse_width * (frq - freq_line);
+                        const double ex = se * se;
+                        const double phi = inverse_width * MGX_INVERSE_SQRT_PI *
+                                           (double)mgx_expf((float)(-ex));
+                        Lval = constante * frq * phi * Lc * inverse_chi;
+                    }
+                }
+                if (u) u[(size_t)rr * PF + of] = uc;
+            }
+            /* per-line sum of the Lambda contributions over the 32 lanes of a slot;
+               lane 0 then adds both slots of the pass in order (they may share a line) */
+            if (lam) {
+                double v = Lval;
+#pragma unroll
+                for (int off = 16; off >= 1; off >>= 1) v += __shfl_xor(v, off, MGX_WAVE);
+                const double v_hi = __shfl(v, 32, MGX_WAVE);
+                if (lane == 0) {
+                    for (uint32_t h = 0; h < 2; h++) {
+                        const uint32_t slot_h = slot0 + h;
+                        if (slot_h >= kp.nslots) break;
+                        const uint32_t ll = slot_h / kp.slots_per_line;
+                        const size_t idx = (size_t)o * L + ll;
+                        const bool fresh = (rr == 0) && (slot_h % kp.slots_per_line == 0);
+                        lam[idx] = (fresh ? 0.0 : lam[idx]) + (h ? v_hi : v);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* trace kernels: counts and dumps of expanded lists                          */
+
+/* one wave per (rr, o): n_tot for every line */
+__global__ __launch_bounds__(MGX_WAVE) void trace_counts_kernel(KParams kp, uint32_t *n_tot)
+{
+    extern __shared__ unsigned char smem[];
+    const mgx_model &m = kp.m;
+    const uint32_t o = blockIdx.x, rr = blockIdx.y, lane = threadIdx.x;
+    const uint32_t cap = kp.cap;
+    uint32_t *s_nxt0 = reinterpret_cast<uint32_t *>(smem);
+    uint32_t *s_nxt1 = s_nxt0 + cap;
+    double *s_dZ0 = reinterpret_cast<double *>(smem + 8 * ((2 * cap * 4 + 7) / 8));
+    double *s_dZ1 = s_dZ0 + cap;
+    double *s_sh0 = s_dZ1 + cap;
+    double *s_sh1 = s_sh0 + cap;
+    uint32_t S0, S1;
+    double sh00, sh01;
+    int e = walk_half(kp, o, rr, s_nxt0, s_dZ0, s_sh0, S0, sh00);
+    if (!e) e = walk_half(kp, o, m.antipod[rr], s_nxt1, s_dZ1, s_sh1, S1, sh01);
+    if (e) {
+        if (lane == 0) {
+            atomicOr(&kp.status[0], (uint32_t)e);
+            if (e == ERR_CAP) atomicMax(&kp.status[1], kp.cap + 1);
+        }
+        return;
+    }
+    __syncthreads();
+    for (uint32_t l = lane; l < m.nlines; l += MGX_WAVE) {
+        uint32_t n = 1;
+        for (uint32_t s = 0; s < S0; s++)
+            n += n_interp_for_line(kp, l, s == 0 ? o : s_nxt0[s - 1], s_nxt0[s]);
+        for (uint32_t s = 0; s < S1; s++)
+            n += n_interp_for_line(kp, l, s == 0 ? o : s_nxt1[s - 1], s_nxt1[s]);
+        n_tot[((size_t)rr * m.npoints + o) * m.nlines + l] = n;
+    }
+}
+
+/* one wave per selected (rr, o, l); lane 0 writes the expanded list */
+__global__ __launch_bounds__(MGX_WAVE) void trace_rays_kernel(
+    KParams kp, const uint32_t *sel_rr, const uint32_t *sel_o, const uint32_t *sel_l,
+    uint32_t capacity, uint32_t *count, uint32_t *centre_idx, uint32_t *nr, uint32_t *nr_interp,
+    double *shift, double *interp_factor, double *dZ)
+{
+    extern __shared__ unsigned char smem[];
+    const mgx_model &m = kp.m;
+    const uint32_t i = blockIdx.x, lane = threadIdx.x;
+    const uint32_t rr = sel_rr[i], o = sel_o[i], l = sel_l[i];
+    const uint32_t cap = kp.cap;
+    uint32_t *s_nxt0 = reinterpret_cast<uint32_t *>(smem);
+    uint32_t *s_nxt1 = s_nxt0 + cap;
+    double *s_dZ0 = reinterpret_cast<double *>(smem + 8 * ((2 * cap * 4 + 7) / 8));
+    double *s_dZ1 = s_dZ0 + cap;
+    double *s_sh0 = s_dZ1 + cap;
+    double *s_sh1 = s_sh0 + cap;
+    uint32_t S0, S1;
+    double sh00, sh01;
+    int e = walk_half(kp, o, rr, s_nxt0, s_dZ0, s_sh0, S0, sh00);
+    if (!e) e = walk_half(kp, o, m.antipod[rr], s_nxt1, s_dZ1, s_sh1, S1, sh01);
+    if (e) {
+        if (lane == 0) {
+            atomicOr(&kp.status[0], (uint32_t)e);
+            if (e == ERR_CAP) atomicMax(&kp.status[1], kp.cap + 1);
+        }
+        return;
+    }
+    __syncthreads();
+    if (lane != 0) return;
+    /* count entries of each half */
+    uint32_t N0 = 0, N1 = 0;
+    for (uint32_t s = 0; s < S0; s++) N0 += n_interp_for_line(kp, l, s == 0 ? o : s_nxt0[s - 1], s_nxt0[s]);
+    for (uint32_t s = 0; s < S1; s++) N1 += n_interp_for_line(kp, l, s == 0 ? o : s_nxt1[s - 1], s_nxt1[s]);
+    const uint32_t n = N0 + N1 + 1;
+    count[i] = n;
+    centre_idx[i] = N0;
+    if (n > capacity) {
+        atomicOr(&kp.status[0], (uint32_t)ERR_OUT);
+        return;
+    }
+    const size_t row = (size_t)i * capacity;
+    /* rr half, outermost first: list index j = 0..N0-1 */
+    {
+        HalfIter it;
+        uint32_t j = 0;
+        if (S0 > 0) {
+            it.s = (int)S0 - 1;
+            hi_load_step(kp, it, l, o, sh00, s_nxt0, s_dZ0, s_sh0);
+            while (it.s >= 0) {
+                const Entry en = hi_entry(it, o);
+                nr[row + j] = en.nr;
+                nr_interp[row + j] = en.nri;
+                interp_factor[row + j] = en.ifac;
+                shift[row + j] = en.shift;
+                dZ[row + j] = hi_advance(kp, it, l, o, sh00, s_nxt0, s_dZ0, s_sh0);
+                j++;
+            }
+        }
+        /* centre */
+        nr[row + j] = o;
+        nr_interp[row + j] = 0; /* never written by the reference's trace (zero scratch) */
+        interp_factor[row + j] = 0.0;
+        shift[row + j] = 1.0;
+    }
+    /* antipod half, innermost first: list index N0+1 .. N0+N1; dZ[j-1] for entry j */
+    if (S1 > 0) {
+        /* iterate outward by collecting per-step counts, then entries in order */
+        uint32_t j = N0 + 1;
+        double shift_crt = sh01;
+        uint32_t crt = o;
+        for (uint32_t s = 0; s < S1; s++) {
+            const uint32_t nxt = s_nxt1[s];
+            const uint32_t ms = n_interp_for_line(kp, l, crt, nxt);
+            const double shift_nxt = s_sh1[s];
+            const double dshift = shift_nxt - shift_crt;
+            if (ms > 1) {
+                const double dZ_interpl = s_dZ1[s] / ms;
+                const double dshift_interpl = dshift / ms;
+                for (uint32_t q = 1; q <= ms; q++, j++) {
+                    nr[row + j] = nxt;
+                    shift[row + j] = shift_crt + q * dshift_interpl;
+                    nr_interp[row + j] = crt;
+                    interp_factor[row + j] = 1.0 - ((double)q) / ms;
+                    dZ[row + j - 1] = dZ_interpl;
+                }
+            } else {
+                nr[row + j] = nxt;
+                shift[row + j] = shift_nxt;
+                nr_interp[row + j] = crt;
+                interp_factor[row + j] = 0.0;
+                dZ[row + j - 1] = s_dZ1[s];
+                j++;
+            }
+            crt = nxt;
+            shift_crt = shift_nxt;
+        }
+    }
+    dZ[row + n - 1] = 0.0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* preparation kernels                                                       */
+
+__global__ void pack_lines_kernel(const mgx_model m, double4 *lpack, double *ub)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t PL = (size_t)m.npoints * m.nlines;
+    if (i < PL) {
+        const double e = m.emissivity[i], k = m.opacity[i];
+        lpack[i] = make_double4(m.inverse_width[i], e, k, e / k);
+    }
+    if (i < m.npoints) {
+        ub[i] = sqrt(MGX_TWO_KB_OVER_AMU_CC_SQUARED * m.max_inverse_mass * m.temperature[i] +
+                     m.vturb2[i]);
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* host side                                                                 */
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return mgx::fail(MGX_E_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));        \
+    } while (0)
+
+struct DevBuffers {
+    double4 *lpack = nullptr;
+    double *ub = nullptr;
+    uint32_t *status = nullptr;
+    ~DevBuffers()
+    {
+        if (lpack) (void)hipFree(lpack);
+        if (ub) (void)hipFree(ub);
+        if (status) (void)hipFree(status);
+    }
+};
+
+int check_view(const mgx_model *m, const mgx_options *op)
+{
+    if (!m || !op) return mgx::fail(MGX_E_ARG, "null model or options");
+    if (op->n_off_diag != 0)
+        return mgx::fail(MGX_E_UNSUPPORTED, "n_off_diag > 0 is not supported (only the Lambda diagonal)");
+    if (op->approx < 0 || op->approx > 2) return mgx::fail(MGX_E_ARG, "unknown approximation");
+    if (op->precision != MGX_PREC_FP64 && op->precision != MGX_PREC_MIXED)
+        return mgx::fail(MGX_E_ARG, "unknown precision mode");
+    if (m->npoints == 0 || m->nlines == 0 || m->nquads == 0 || m->nfreqs == 0)
+        return mgx::fail(MGX_E_ARG, "empty model");
+    if (m->nfreqs != m->nlines * m->nquads)
+        return mgx::fail(MGX_E_UNSUPPORTED,
+                         "the formal solve needs the line spectral discretisation (nfreqs = nlines*nquads)");
+    if (m->hnrays == 0 || m->hnrays > m->nrays) return mgx::fail(MGX_E_ARG, "bad hnrays");
+    if (!m->position || !m->velocity || !m->n_neighbors || !m->cum_n_neighbors ||
+        !m->neighbors || !m->point2boundary || !m->ray_direction || !m->ray_weight ||
+        !m->antipod || !m->temperature || !m->vturb2 || !m->line || !m->sorted_line ||
+        !m->sorted_line_map || !m->emissivity || !m->opacity || !m->inverse_width ||
+        !m->interp_crit || !m->dshift_max || !m->nu || !m->f_line || !m->f_species ||
+        !m->f_k || !m->f_z || !m->f_in_line || !m->line_A || !m->species_inverse_mass ||
+        !m->quad_weights)
+        return mgx::fail(MGX_E_ARG, "null array in model view");
+    return MGX_OK;
+}
+
+size_t walk_lds_bytes(uint32_t cap) { return 8 * ((2 * (size_t)cap * 4 + 7) / 8) + 4 * (size_t)cap * 8; }
+
+const uint32_t kMaxCap = 1536; /* 1536 steps per half: 61 KB of LDS */
+
+int prepare(const mgx_model *m, const mgx_options *op, hipStream_t st, DevBuffers &b, KParams &kp)
+{
+    const size_t PL = (size_t)m->npoints * m->nlines;
+    HIP_TRY(hipMalloc(&b.lpack, PL * sizeof(double4)));
+    HIP_TRY(hipMalloc(&b.ub, (size_t)m->npoints * sizeof(double)));
+    HIP_TRY(hipMalloc(&b.status, 2 * sizeof(uint32_t)));
+    HIP_TRY(hipMemsetAsync(b.status, 0, 2 * sizeof(uint32_t), st));
+    const size_t n = PL > m->npoints ? PL : m->npoints;
+    hipLaunchKernelGGL(pack_lines_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, *m,
+                       b.lpack, b.ub);
+    HIP_TRY(hipGetLastError());
+    kp.m = *m;
+    kp.op = *op;
+    kp.lpack = b.lpack;
+    kp.ub = b.ub;
+    kp.log_max_interp = (double)mgx_logf((float)op->max_interpolation_diff);
+    kp.slots_per_line = (m->nquads + 31) / 32;
+    kp.nslots = m->nlines * kp.slots_per_line;
+    kp.status = b.status;
+    return MGX_OK;
+}
+
+/* reads the status words; returns MGX_OK, or an error (needed capacity in *need) */
+int read_status(hipStream_t st, const DevBuffers &b, uint32_t *need)
+{
+    uint32_t h[2] = {0, 0};
+    HIP_TRY(hipMemcpyAsync(h, b.status, sizeof h, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    *need = h[1];
+    if (h[0] & ERR_NO_NEXT)
+        return mgx::fail(MGX_E_NO_NEXT, "a ray from an interior point found no valid next point");
+    if (h[0] & ERR_OUT) return mgx::fail(MGX_E_RAY_LENGTH, "a traced list exceeds the output capacity");
+    if (h[0] & ERR_CAP) return MGX_E_RAY_LENGTH;
+    return MGX_OK;
+}
+
+template <typename Launch>
+int launch_with_capacity(hipStream_t st, DevBuffers &b, KParams &kp, Launch launch)
+{
+    uint32_t cap = 128;
+    for (;;) {
+        kp.cap = cap;
+        HIP_TRY(hipMemsetAsync(b.status, 0, 2 * sizeof(uint32_t), st));
+        launch(walk_lds_bytes(cap));
+        HIP_TRY(hipGetLastError());
+        uint32_t need = 0;
+        const int e = read_status(st, b, &need);
+        if (e != MGX_E_RAY_LENGTH) return e;
+        if (cap >= kMaxCap)
+            return mgx::fail(MGX_E_RAY_LENGTH, "a ray needs more than %u walk steps", kMaxCap);
+        cap = cap * 2 > kMaxCap ? kMaxCap : cap * 2;
+    }
+}
+
+} // namespace
+
+extern "C" {
+
+const char *mgx_last_error(void) { return mgx::error_buffer(); }
+
+const char *mgx_version(void) { return "magritte_amd 0.1.0 (gfx950)"; }
+
+int mgx_feautrier_forward_device(const mgx_model *model, const mgx_options *opts, double *u,
+                                 double *J, double *lambda_diag, void *stream)
+{
+    mgx::clear_error();
+    if (int e = check_view(model, opts)) return e;
+    if (!J) return mgx::fail(MGX_E_ARG, "J is required");
+    if (opts->precision != MGX_PREC_FP64)
+        return mgx::fail(MGX_E_UNSUPPORTED, "mixed precision is not built yet");
+    hipStream_t st = (hipStream_t)stream;
+    DevBuffers b;
+    KParams kp;
+    if (int e = prepare(model, opts, st, b, kp)) return e;
+    return launch_with_capacity(st, b, kp, [&](size_t lds) {
+        const dim3 grid(model->npoints), block(MGX_WAVE);
+        switch (opts->approx) {
+        case MGX_APPROX_NONE:
+            hipLaunchKernelGGL(feautrier_kernel<MGX_APPROX_NONE>, grid, block, lds, st, kp, u, J,
+                               lambda_diag);
+            break;
+        case MGX_APPROX_ONE_LINE:
+            hipLaunchKernelGGL(feautrier_kernel<MGX_APPROX_ONE_LINE>, grid, block, lds, st, kp, u,
+                               J, lambda_diag);
+            break;
+        default:
+            hipLaunchKernelGGL(feautrier_kernel<MGX_APPROX_CLOSE_LINES>, grid, block, lds, st, kp,
+                               u, J, lambda_diag);
+        }
+    });
+}
+
+int mgx_trace_counts_device(const mgx_model *model, const mgx_options *opts, uint32_t *n_tot,
+                            void *stream)
+{
+    mgx::clear_error();
+    if (int e = check_view(model, opts)) return e;
+    if (!n_tot) return mgx::fail(MGX_E_ARG, "n_tot is required");
+    hipStream_t st = (hipStream_t)stream;
+    DevBuffers b;
+    KParams kp;
+    if (int e = prepare(model, opts, st, b, kp)) return e;
+    return launch_with_capacity(st, b, kp, [&](size_t lds) {
+        hipLaunchKernelGGL(trace_counts_kernel, dim3(model->npoints, model->hnrays),
+                           dim3(MGX_WAVE), lds, st, kp, n_tot);
+    });
+}
+
+int mgx_trace_rays_device(const mgx_model *model, const mgx_options *opts, uint32_t nsel,
+                          const uint32_t *sel_rr, const uint32_t *sel_o, const uint32_t *sel_l,
+                          uint32_t capacity, uint32_t *count, uint32_t *centre_idx, uint32_t *nr,
+                          uint32_t *nr_interp, double *shift, double *interp_factor, double *dZ,
+                          void *stream)
+{
+    mgx::clear_error();
+    if (int e = check_view(model, opts)) return e;
+    if (nsel == 0) return MGX_OK;
+    if (!sel_rr || !sel_o || !sel_l || !count || !centre_idx || !nr || !nr_interp || !shift ||
+        !interp_factor || !dZ || capacity == 0)
+        return mgx::fail(MGX_E_ARG, "null output or zero capacity");
+    hipStream_t st = (hipStream_t)stream;
+    DevBuffers b;
+    KParams kp;
+    if (int e = prepare(model, opts, st, b, kp)) return e;
+    const int e = launch_with_capacity(st, b, kp, [&](size_t lds) {
+        hipLaunchKernelGGL(trace_rays_kernel, dim3(nsel), dim3(MGX_WAVE), lds, st, kp, sel_rr,
+                           sel_o, sel_l, capacity, count, centre_idx, nr, nr_interp, shift,
+                           interp_factor, dZ);
+    });
+    return e;
+}
+
+} // extern "C"

@@ -1,0 +1,440 @@
+"""Model data of the formal solver and seeded synthetic model generators.
+
+``ModelData`` holds the arrays the reference's ``Model`` carries on the
+formal-solver path (geometry, thermodynamics, chemistry, line data,
+quadrature) and runs the host-side producers of libmgx
+(``mgx_compute_*``) to derive the solver inputs: spectral grid, inverse
+widths, LTE populations, emissivities/opacities, dshift_max and the
+interpolation criterion.
+
+The generators restate the conventions of the reference's model-building
+helpers (magritte/setup.py): 1D neighbour lists (setup.py:45-98), Delaunay
+neighbour lists (setup.py:84-92), HEALPix directions rotated so the first ray
+points along +x (setup.py:101-122, 174-223), Gauss-Hermite quadrature with
+weights normalised to one (setup.py:480-503) and LAMDA-style derived line data
+(setup.py:759-767).  HEALPix is computed here (healpy is not available).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+
+# src/tools/constants.hpp
+CC = 2.99792458e8
+HH = 6.62607004e-34
+KB = 1.38064852e-23
+AMU = 1.66053904e-27
+T_CMB = 2.72548
+
+# CGS constants for Einstein A of a rotor
+_H_CGS = 6.62607004e-27
+_C_CGS = 2.99792458e10
+_DEBYE = 1.0e-18
+
+SEED_BASE = 20261202
+
+
+def seed_for(config, model_index=0):
+    return SEED_BASE + 1000 * config + model_index
+
+
+# ---------------------------------------------------------------------------
+# line data
+
+
+def linedata_two_level(freq, A, g=(1.0, 3.0), mass=28.0, num=1):
+    """A two-level species with line frequency ``freq`` (setup.py:759-767 derived terms)."""
+    energy = np.array([0.0, HH * freq])
+    weight = np.array(g, dtype=np.float64)
+    return _linedata(num, mass, energy, weight, [1], [0], [A])
+
+
+def linedata_linear_rotor(B, dipole_debye, nlev, mass=28.0, num=1):
+    """Linear rotor: E_J = h B J(J+1), g_J = 2J+1, J -> J-1 lines, A from the dipole."""
+    J = np.arange(nlev, dtype=np.float64)
+    energy = HH * B * J * (J + 1.0)
+    weight = 2.0 * J + 1.0
+    irad = list(range(1, nlev))
+    jrad = list(range(0, nlev - 1))
+    mu = dipole_debye * _DEBYE
+    A = []
+    for Ju in irad:
+        nu = 2.0 * B * Ju
+        A.append(64.0 * np.pi**4 * nu**3 * mu**2 / (3.0 * _H_CGS * _C_CGS**3) * Ju / (2.0 * Ju + 1.0))
+    return _linedata(num, mass, energy, weight, irad, jrad, A)
+
+
+def _linedata(num, mass, energy, weight, irad, jrad, A):
+    irad = np.asarray(irad, dtype=np.uint32)
+    jrad = np.asarray(jrad, dtype=np.uint32)
+    A = np.asarray(A, dtype=np.float64)
+    nrad = len(irad)
+    freq = np.empty(nrad)
+    Bs = np.empty(nrad)
+    Ba = np.empty(nrad)
+    for k in range(nrad):
+        i, j = int(irad[k]), int(jrad[k])
+        freq[k] = (energy[i] - energy[j]) / HH
+        Bs[k] = A[k] * CC**2 / (2.0 * HH * freq[k] ** 3)
+        Ba[k] = weight[i] / weight[j] * Bs[k]
+    return dict(num=num, nlev=len(energy), nrad=nrad, inverse_mass=float(1.0 / mass),
+                energy=np.ascontiguousarray(energy, dtype=np.float64),
+                weight=np.ascontiguousarray(weight, dtype=np.float64),
+                irad=irad, jrad=jrad, A=A, Bs=Bs, Ba=Ba, frequency=freq)
+
+
+def quadrature(nquads):
+    roots, weights = np.polynomial.hermite.hermgauss(nquads)
+    return roots, weights / np.sum(weights)
+
+
+# ---------------------------------------------------------------------------
+# HEALPix (RING scheme) pixel centres
+
+
+def healpix_pix2vec(nside):
+    npix = 12 * nside * nside
+    ncap = 2 * nside * (nside - 1)
+    out = np.empty((npix, 3))
+    for ip in range(npix):
+        if ip < ncap:
+            iring = int((1 + int(np.sqrt(1 + 2 * ip))) // 2)
+            while 2 * iring * (iring - 1) > ip:
+                iring -= 1
+            while 2 * (iring + 1) * iring <= ip:
+                iring += 1
+            iphi = ip - 2 * iring * (iring - 1) + 1
+            z = 1.0 - iring * iring / (3.0 * nside * nside)
+            phi = (iphi - 0.5) * np.pi / (2.0 * iring)
+        elif ip < npix - ncap:
+            q = ip - ncap
+            iring = q // (4 * nside) + nside
+            iphi = q % (4 * nside) + 1
+            fodd = 0.5 * (1 + ((iring + nside) & 1))
+            z = (2 * nside - iring) * 2.0 / (3.0 * nside)
+            phi = (iphi - fodd) * np.pi / (2.0 * nside)
+        else:
+            q = npix - ip
+            iring = int((1 + int(np.sqrt(2 * q - 1))) // 2)
+            while 2 * iring * (iring - 1) >= q:
+                iring -= 1
+            while 2 * (iring + 1) * iring < q:
+                iring += 1
+            iphi = 4 * iring + 1 - (q - 2 * iring * (iring - 1))
+            z = -1.0 + iring * iring / (3.0 * nside * nside)
+            phi = (iphi - 0.5) * np.pi / (2.0 * iring)
+        sth = np.sqrt(max(0.0, (1.0 - z) * (1.0 + z)))
+        out[ip] = (sth * np.cos(phi), sth * np.sin(phi), z)
+    return out
+
+
+def _rotation_matrix(n):
+    """setup.py:101-122"""
+    nx = np.array([1.0, 0.0, 0.0])
+    ny = np.array([0.0, 1.0, 0.0])
+    n1 = np.array(n, dtype=np.float64)
+    n1 /= np.linalg.norm(n1)
+    if np.linalg.norm(n1 - nx) < 1.0e-6:
+        n2 = np.cross(ny, n1)
+    else:
+        n2 = np.cross(nx, n1)
+    n2 /= np.linalg.norm(n2)
+    n3 = np.cross(n2, n1)
+    n3 /= np.linalg.norm(n3)
+    return np.array([n1, n2, n3]).T
+
+
+def uniform_rays(nrays, first_ray=(1.0, 0.0, 0.0)):
+    """HEALPix directions rotated so ray 0 points along first_ray (setup.py:174-223)."""
+    nside = int(round(np.sqrt(nrays / 12)))
+    assert 12 * nside * nside == nrays, "nrays must be 12*nside^2"
+    d = healpix_pix2vec(nside)
+    R1 = _rotation_matrix(d[0])
+    R2 = _rotation_matrix(np.asarray(first_ray, dtype=np.float64))
+    R = R2 @ np.linalg.inv(R1)
+    return np.ascontiguousarray(d @ R.T)
+
+
+# ---------------------------------------------------------------------------
+
+
+class ModelData:
+    """Arrays of one model on the formal-solver path (host numpy)."""
+
+    def __init__(self):
+        self.dimension = 3
+        self.spherical_symmetry = 0
+        self.adaptive_directions = 0
+        self.linedata = []
+        self.params = dict(max_width_fraction=0.35, min_opacity=1.0e-26,
+                           min_line_opacity=1.0e-13, min_line_emissivity=1.0e-50,
+                           max_interpolation_diff=1.4, max_distance_opacity_contribution=10.0)
+
+    # -- geometry ---------------------------------------------------------
+    def set_geometry(self, position, velocity, neighbors_list, boundary, directions, weights,
+                     boundary_condition=None, boundary_temperature=None):
+        P = len(position)
+        self.npoints = P
+        self.position = np.ascontiguousarray(position, dtype=np.float64)
+        self.velocity = np.ascontiguousarray(velocity, dtype=np.float64)
+        if isinstance(neighbors_list, tuple):  # CSR (n_neighbors, neighbors)
+            n_nbs, nbs = neighbors_list
+        else:
+            n_nbs = [len(x) for x in neighbors_list]
+            nbs = [v for x in neighbors_list for v in x]
+        self.n_neighbors = np.ascontiguousarray(n_nbs, dtype=np.uint32)
+        self.neighbors = np.ascontiguousarray(nbs, dtype=np.uint32)
+        cum = np.zeros(P, dtype=np.uint32)
+        cum[1:] = np.cumsum(self.n_neighbors[:-1], dtype=np.uint64).astype(np.uint32)
+        self.cum_n_neighbors = cum
+        self.boundary2point = np.ascontiguousarray(boundary, dtype=np.uint32)
+        self.nboundary = len(self.boundary2point)
+        p2b = np.full(P, P, dtype=np.uint32)
+        p2b[self.boundary2point] = np.arange(self.nboundary, dtype=np.uint32)
+        self.point2boundary = p2b
+        nb = self.nboundary
+        self.boundary_condition = np.ascontiguousarray(
+            np.full(nb, _abi.BC_CMB) if boundary_condition is None else boundary_condition,
+            dtype=np.int32)
+        self.boundary_temperature = np.ascontiguousarray(
+            np.full(nb, T_CMB) if boundary_temperature is None else boundary_temperature,
+            dtype=np.float64)
+        self.ray_direction = np.ascontiguousarray(directions, dtype=np.float64)
+        self.ray_weight = np.ascontiguousarray(weights, dtype=np.float64)
+        self.nrays = len(self.ray_weight) if not self.adaptive_directions else self.ray_weight.size // P
+        self.hnrays = self.nrays // 2
+        # antipods (rays.cpp:52-61): last r2 with |d1 + d2|^2 < 1e-9
+        d = self.ray_direction.reshape(-1, 3)[: self.nrays]
+        anti = np.zeros(self.nrays, dtype=np.uint32)
+        for r1 in range(self.nrays):
+            for r2 in range(self.nrays):
+                s = d[r1] + d[r2]
+                if s[0] * s[0] + s[1] * s[1] + s[2] * s[2] < 1.0e-9:
+                    anti[r1] = r2
+        self.antipod = anti
+        # Points::center (points.cpp:29-48), including its max initialisation
+        lo = np.full(3, np.finfo(np.float64).max)
+        hi = np.full(3, np.finfo(np.float64).tiny)
+        lo = np.minimum(lo, self.position.min(axis=0))
+        hi = np.maximum(hi, self.position.max(axis=0))
+        self.center = (lo + hi) / 2
+
+    # -- physics ----------------------------------------------------------
+    def set_thermo(self, temperature, vturb, abundance):
+        self.temperature = np.ascontiguousarray(temperature, dtype=np.float64)
+        self.vturb2 = np.ascontiguousarray((np.asarray(vturb, dtype=np.float64) / CC) ** 2)
+        self.abundance = np.ascontiguousarray(abundance, dtype=np.float64)
+        self.nspec = self.abundance.shape[1]
+
+    def set_lines(self, linedata, nquads):
+        self.linedata = list(linedata)
+        self.nlspecs = len(self.linedata)
+        self.nquads = nquads
+        roots, weights = quadrature(nquads)
+        self.quad_roots = np.ascontiguousarray(np.tile(roots, (self.nlspecs, 1)))
+        self.quad_weights = np.ascontiguousarray(np.tile(weights, (self.nlspecs, 1)))
+
+    def _ld_array(self):
+        arr = (_abi.MgxLinedata * self.nlspecs)()
+        keep = []
+        for s, d in enumerate(self.linedata):
+            a = arr[s]
+            a.num, a.nlev, a.nrad = d["num"], d["nlev"], d["nrad"]
+            a.inverse_mass = d["inverse_mass"]
+            for f in ("energy", "weight", "A", "Bs", "Ba", "frequency"):
+                x = np.ascontiguousarray(d[f], dtype=np.float64)
+                keep.append(x)
+                setattr(a, f, x.ctypes.data_as(C.POINTER(C.c_double)))
+            for f in ("irad", "jrad"):
+                x = np.ascontiguousarray(d[f], dtype=np.uint32)
+                keep.append(x)
+                setattr(a, f, x.ctypes.data_as(C.POINTER(C.c_uint32)))
+        return arr, keep
+
+    def finalize(self):
+        """Run the host producers (Model::compute_* on the reference's path)."""
+        L = _abi.lib()
+        P, Q = self.npoints, self.nquads
+        ld, keep = self._ld_array()
+        line, lsp, lk, lA = [], [], [], []
+        for s, d in enumerate(self.linedata):
+            for k in range(d["nrad"]):
+                line.append(d["frequency"][k])
+                lsp.append(s)
+                lk.append(k)
+                lA.append(d["A"][k])
+        self.nlines = len(line)
+        self.line = np.ascontiguousarray(line, dtype=np.float64)
+        self.line_species = np.ascontiguousarray(lsp, dtype=np.uint32)
+        self.line_k = np.ascontiguousarray(lk, dtype=np.uint32)
+        self.line_A = np.ascontiguousarray(lA, dtype=np.float64)
+        self.species_inverse_mass = np.ascontiguousarray(
+            [d["inverse_mass"] for d in self.linedata], dtype=np.float64)
+        self.max_inverse_mass = float(max(0.0, *[d["inverse_mass"] for d in self.linedata]))
+        self.sorted_line = np.empty(self.nlines)
+        self.sorted_line_map = np.empty(self.nlines, dtype=np.uint32)
+        _abi.check(L.mgx_sort_lines(self.nlines, _abi.ptr(self.line), _abi.ptr(self.sorted_line),
+                                    _abi.ptr(self.sorted_line_map)))
+        self.compute_spectral_discretisation()
+        self.inverse_width = np.empty((P, self.nlines))
+        _abi.check(L.mgx_compute_inverse_widths(P, self.nlspecs, ld, _abi.ptr(self.temperature),
+                                                _abi.ptr(self.vturb2), _abi.ptr(self.inverse_width)))
+        self.compute_lte()
+        self.dshift_max = np.empty(P)
+        _abi.check(L.mgx_compute_dshift_max(P, self.nlspecs, ld, _abi.ptr(self.temperature),
+                                            _abi.ptr(self.vturb2),
+                                            self.params["max_width_fraction"],
+                                            _abi.ptr(self.dshift_max)))
+        del keep
+        return self
+
+    def compute_spectral_discretisation(self):
+        L = _abi.lib()
+        ld, keep = self._ld_array()
+        P, Q = self.npoints, self.nquads
+        F = self.nlines * Q
+        self.nfreqs = F
+        self.nu = np.empty((P, F))
+        self.f_line = np.empty(F, dtype=np.uint32)
+        self.f_species = np.empty(F, dtype=np.uint32)
+        self.f_k = np.empty(F, dtype=np.uint32)
+        self.f_z = np.empty(F, dtype=np.uint32)
+        self.f_in_line = np.empty(F, dtype=np.uint8)
+        _abi.check(L.mgx_compute_spectral_discretisation(
+            P, self.nlspecs, ld, Q, _abi.ptr(self.quad_roots), _abi.ptr(self.temperature),
+            _abi.ptr(self.vturb2), _abi.ptr(self.nu), _abi.ptr(self.f_line),
+            _abi.ptr(self.f_species), _abi.ptr(self.f_k), _abi.ptr(self.f_z),
+            _abi.ptr(self.f_in_line)))
+        del keep
+
+    def compute_lte(self):
+        L = _abi.lib()
+        ld, keep = self._ld_array()
+        P = self.npoints
+        npop = sum(d["nlev"] for d in self.linedata) * P
+        self.population = np.empty(npop)
+        self.population_tot = np.empty((self.nlspecs, P))
+        self.emissivity = np.empty((P, self.nlines))
+        self.opacity = np.empty((P, self.nlines))
+        _abi.check(L.mgx_compute_lte(P, self.nspec, _abi.ptr(self.abundance),
+                                     _abi.ptr(self.temperature), self.nlspecs, ld,
+                                     self.params["min_line_emissivity"],
+                                     self.params["min_line_opacity"], _abi.ptr(self.population),
+                                     _abi.ptr(self.population_tot), _abi.ptr(self.emissivity),
+                                     _abi.ptr(self.opacity)))
+        self.interp_crit = np.empty((P, self.nlines))
+        _abi.check(L.mgx_compute_interp_crit(P, self.nlspecs, ld, _abi.ptr(self.opacity),
+                                             _abi.ptr(self.population_tot),
+                                             _abi.ptr(self.interp_crit)))
+        del keep
+
+    # -- views ------------------------------------------------------------
+    ARRAYS = ("position", "velocity", "n_neighbors", "cum_n_neighbors", "neighbors",
+              "point2boundary", "boundary2point", "boundary_condition", "boundary_temperature",
+              "ray_direction", "ray_weight", "antipod", "temperature", "vturb2", "line",
+              "sorted_line", "sorted_line_map", "emissivity", "opacity", "inverse_width",
+              "interp_crit", "dshift_max", "line_species", "line_k", "line_A",
+              "species_inverse_mass", "quad_weights", "nu", "f_line", "f_species", "f_k",
+              "f_z", "f_in_line")
+
+    def options(self, approx=_abi.APPROX_CLOSE_LINES, precision=_abi.PREC_FP64):
+        o = _abi.MgxOptions()
+        o.approx = approx
+        o.n_off_diag = 0
+        o.precision = precision
+        o.min_opacity = self.params["min_opacity"]
+        o.max_distance_opacity_contribution = self.params["max_distance_opacity_contribution"]
+        o.max_interpolation_diff = self.params["max_interpolation_diff"]
+        return o
+
+    def view(self, arrays=None):
+        """mgx_model whose pointers refer to ``arrays`` (default: the numpy arrays)."""
+        arrays = arrays if arrays is not None else {k: getattr(self, k) for k in self.ARRAYS}
+        m = _abi.MgxModel()
+        for f in ("npoints", "nrays", "hnrays", "nboundary", "nlines", "nlspecs", "nquads",
+                  "nfreqs", "dimension", "spherical_symmetry", "adaptive_directions"):
+            setattr(m, f, int(getattr(self, f)))
+        for k in self.ARRAYS:
+            ftype = dict(m._fields_)[k]
+            setattr(m, k, C.cast(C.c_void_p(_abi.ptr(arrays[k])), ftype))
+        m.center[:] = [float(x) for x in self.center]
+        m.max_inverse_mass = self.max_inverse_mass
+        return m
+
+    def to_device(self, device="cuda"):
+        """Copy every solver input to the GPU (torch tensors); returns (view, tensors)."""
+        import torch
+        t = {k: torch.from_numpy(np.ascontiguousarray(getattr(self, k))).to(device)
+             for k in self.ARRAYS}
+        return self.view(t), t
+
+
+# ---------------------------------------------------------------------------
+# synthetic workloads
+
+
+def model_line_1d(npoints=100, length=1.0e13, nquads=31, vmax=1.0e4, density=1.0e12,
+                  temperature=20.0, vturb=150.0, freq=115.27e9, A=7.2e-8):
+    """1D line of points with neighbours i-1, i+1 and rays -x, +x; one two-level species."""
+    P = npoints
+    x = np.arange(P, dtype=np.float64) * (length / (P - 1))
+    pos = np.zeros((P, 3))
+    pos[:, 0] = x
+    vel = np.zeros((P, 3))
+    vel[:, 0] = (vmax * np.arange(P) / (P - 1)) / CC
+    nbs = [[1]] + [[p - 1, p + 1] for p in range(1, P - 1)] + [[P - 2]]
+    m = ModelData()
+    m.dimension = 1
+    m.set_geometry(pos, vel, nbs, [0, P - 1], [[-1.0, 0.0, 0.0], [1.0, 0.0, 0.0]], [0.5, 0.5])
+    abund = np.zeros((P, 2))
+    abund[:, 0] = density
+    abund[:, 1] = 1.0e-4 * density
+    m.set_thermo(np.full(P, temperature), np.full(P, vturb), abund)
+    m.set_lines([linedata_two_level(freq, A)], nquads)
+    return m.finalize()
+
+
+def jittered_grid(n_edge, half_size, rng, jitter=0.3):
+    g = np.linspace(-half_size, half_size, n_edge)
+    h = g[1] - g[0]
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    pos = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    pos = pos + rng.uniform(-jitter, jitter, size=pos.shape) * h
+    I, Jg, K = np.meshgrid(np.arange(n_edge), np.arange(n_edge), np.arange(n_edge), indexing="ij")
+    idx = np.stack([I.ravel(), Jg.ravel(), K.ravel()], axis=1)
+    outer = np.any((idx == 0) | (idx == n_edge - 1), axis=1)
+    return np.ascontiguousarray(pos), np.nonzero(outer)[0].astype(np.uint32)
+
+
+def delaunay_neighbors(pos):
+    from scipy.spatial import Delaunay
+
+    indptr, nbs = Delaunay(pos).vertex_neighbor_vertices
+    n_nbs = np.diff(indptr)
+    return (n_nbs.astype(np.uint32), nbs.astype(np.uint32))
+
+
+def model_wind(n_edge=32, half_size=1.0e14, nrays=12, nquads=31, seed=None, nlev=10,
+               B=57.6e9, dipole=0.11, v_out=15.0e3, v_rot=0.0, vturb=150.0, clumps=None):
+    """Jittered-grid spherical wind: n = 1e14 (r/1e12 m)^-2, T = 2000 (r/1e12 m)^-0.7,
+    radial outflow, linear rotor; boundary = the outer grid layer (CMB)."""
+    rng = np.random.Generator(np.random.PCG64(seed_for(1) if seed is None else seed))
+    pos, bdy = jittered_grid(n_edge, half_size, rng)
+    r = np.sqrt((pos**2).sum(axis=1))
+    rhat = pos / r[:, None]
+    vel = v_out * rhat
+    if v_rot:
+        vel = vel + v_rot * np.stack([-rhat[:, 1], rhat[:, 0], np.zeros_like(r)], axis=1)
+    vel = vel / CC
+    dens = 1.0e14 * (r / 1.0e12) ** -2
+    if clumps is not None:
+        dens = dens * clumps
+    temp = 2000.0 * (r / 1.0e12) ** -0.7
+    m = ModelData()
+    m.set_geometry(pos, vel, delaunay_neighbors(pos), bdy, uniform_rays(nrays),
+                   np.full(nrays, 1.0 / nrays))
+    abund = np.stack([dens, 1.0e-4 * dens], axis=1)
+    m.set_thermo(temp, np.full(len(r), vturb), abund)
+    m.set_lines([linedata_linear_rotor(B, dipole, nlev)], nquads)
+    return m.finalize()

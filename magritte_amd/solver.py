@@ -1,0 +1,92 @@
+"""Device entry points of the formal solver (ctypes over include/mgx.h).
+
+Inputs are copied to HBM once (``ModelData.to_device``); every call then runs
+the HIP kernels on those resident arrays.  torch is used for device memory
+and streams only.  There is no CPU fallback: without the GPU library these
+functions raise.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+
+
+def _require_gpu():
+    import torch
+
+    if not torch.cuda.is_available():
+        raise RuntimeError("magritte_amd: no GPU visible; the formal solver runs only on the device")
+    return torch
+
+
+class DeviceModel:
+    """Model arrays resident on the GPU plus the mgx_model view of them."""
+
+    def __init__(self, model, device="cuda"):
+        torch = _require_gpu()
+        self.model = model
+        self.device = torch.device(device)
+        self.view, self.tensors = model.to_device(self.device)
+
+    def stream_ptr(self):
+        import torch
+
+        return torch.cuda.current_stream(self.device).cuda_stream
+
+
+def feautrier_forward(dm, opts, with_u=True, with_lambda=True, out=None):
+    """Plain-method formal solve: returns dict(u=(hnrays,P,F), J=(P,F), lambda_diag=(P,L))."""
+    torch = _require_gpu()
+    m = dm.model
+    dev = dm.device
+    if out is None:
+        out = {}
+        out["u"] = torch.empty((m.hnrays, m.npoints, m.nfreqs), dtype=torch.float64,
+                               device=dev) if with_u else None
+        out["J"] = torch.empty((m.npoints, m.nfreqs), dtype=torch.float64, device=dev)
+        out["lambda_diag"] = torch.empty((m.npoints, m.nlines), dtype=torch.float64,
+                                         device=dev) if with_lambda else None
+    _abi.check(_abi.lib().mgx_feautrier_forward_device(
+        C.byref(dm.view), C.byref(opts), _abi.ptr(out["u"]), _abi.ptr(out["J"]),
+        _abi.ptr(out["lambda_diag"]), dm.stream_ptr()))
+    return out
+
+
+def trace_counts(dm, opts):
+    torch = _require_gpu()
+    m = dm.model
+    n = torch.empty((m.hnrays, m.npoints, m.nlines), dtype=torch.int32, device=dm.device)
+    _abi.check(_abi.lib().mgx_trace_counts_device(C.byref(dm.view), C.byref(opts), _abi.ptr(n),
+                                                  dm.stream_ptr()))
+    return n.cpu().numpy().view(np.uint32)
+
+
+def trace_rays(dm, opts, sel, capacity=1024):
+    """Expanded traced lists of the (rr, o, l) triples in ``sel`` (array (n,3))."""
+    torch = _require_gpu()
+    sel = np.ascontiguousarray(sel, dtype=np.uint32)
+    n = len(sel)
+    dev = dm.device
+    s = [torch.from_numpy(np.ascontiguousarray(sel[:, i]).view(np.int32)).to(dev) for i in range(3)]
+    cnt = torch.empty(n, dtype=torch.int32, device=dev)
+    cen = torch.empty(n, dtype=torch.int32, device=dev)
+    nr = torch.zeros((n, capacity), dtype=torch.int32, device=dev)
+    nri = torch.zeros((n, capacity), dtype=torch.int32, device=dev)
+    sh = torch.zeros((n, capacity), dtype=torch.float64, device=dev)
+    fa = torch.zeros((n, capacity), dtype=torch.float64, device=dev)
+    dz = torch.zeros((n, capacity), dtype=torch.float64, device=dev)
+    _abi.check(_abi.lib().mgx_trace_rays_device(
+        C.byref(dm.view), C.byref(opts), n, _abi.ptr(s[0]), _abi.ptr(s[1]), _abi.ptr(s[2]),
+        capacity, _abi.ptr(cnt), _abi.ptr(cen), _abi.ptr(nr), _abi.ptr(nri), _abi.ptr(sh),
+        _abi.ptr(fa), _abi.ptr(dz), dm.stream_ptr()))
+    cnt = cnt.cpu().numpy().view(np.uint32)
+    cen = cen.cpu().numpy().view(np.uint32)
+    nr, nri = nr.cpu().numpy().view(np.uint32), nri.cpu().numpy().view(np.uint32)
+    sh, fa, dz = sh.cpu().numpy(), fa.cpu().numpy(), dz.cpu().numpy()
+    res = []
+    for i in range(n):
+        k = int(cnt[i])
+        res.append(dict(count=k, centre=int(cen[i]), nr=nr[i, :k], nr_interp=nri[i, :k],
+                        shift=sh[i, :k], interp_factor=fa[i, :k], dZ=dz[i, :k]))
+    return res

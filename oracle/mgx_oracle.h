@@ -1,0 +1,40 @@
+/*
+ * CPU ORACLE -- test infrastructure only (tests/, smoke(), bench.py's
+ * cpu_baseline).  A C restatement of Magritte's plain-method formal solver;
+ * see mgx_oracle.c for the reference functions it follows.
+ */
+#ifndef MGX_ORACLE_H
+#define MGX_ORACLE_H
+
+#include "mgx.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Model::compute_radiation_field_feautrier_order_2 (setup + solve). u may be NULL. */
+int oracle_solve_feautrier_order_2(const mgx_model *m, const mgx_options *op, double *u,
+                                   double *J, double *lambda_diag, int nthreads);
+/* n_tot(rr, o, l) for every ray pair, origin and line, (hnrays, P, L). */
+int oracle_trace_counts(const mgx_model *m, const mgx_options *op, uint32_t *n_tot, int nthreads);
+/* expanded list of one (rr, o, l); same layout as mgx_trace_rays_device. */
+int oracle_trace_ray(const mgx_model *m, const mgx_options *op, uint32_t rr, uint32_t o,
+                     uint32_t l, uint32_t capacity, uint32_t *count, uint32_t *centre_idx,
+                     uint32_t *nr, uint32_t *nr_interp, double *shift, double *interp_factor,
+                     double *dZ);
+/* Model::compute_image_new(raydir, nx, ny): I (nx*ny, F), ImX, ImY (nx*ny). */
+int oracle_image_new(const mgx_model *m, const mgx_options *op, const double *raydir, uint32_t nx,
+                     uint32_t ny, double *I, double *ImX, double *ImY, int nthreads);
+uint32_t oracle_closest_bdy_point(const mgx_model *m, const double *raydir);
+
+float oracle_expf(float x);
+float oracle_expm1f(float x);
+float oracle_logf(float x);
+float oracle_powf(float x, float y);
+float oracle_erff(float x);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,96 @@
+"""CPU ORACLE bindings -- test infrastructure only.
+
+ctypes bindings of oracle/liboracle.so (a C restatement of the reference's
+plain-method formal solver, see mgx_oracle.c).  Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg use it, as the checker.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_libs = {}
+
+
+def lib(kind="cr"):
+    """kind='cr': correctly rounded float functions (shared with the kernels);
+    kind='glibc': the C library's expf/expm1f/logf/powf/erff."""
+    if kind not in _libs:
+        name = "liboracle.so" if kind == "cr" else "liboracle_glibc.so"
+        path = os.path.join(_HERE, name)
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} missing: run `make -C oracle` (or __graft_entry__.build())")
+        h = C.CDLL(path)
+        vp = C.c_void_p
+        h.oracle_solve_feautrier_order_2.argtypes = [vp, vp, vp, vp, vp, C.c_int]
+        h.oracle_trace_counts.argtypes = [vp, vp, vp, C.c_int]
+        h.oracle_trace_ray.argtypes = [vp, vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                       vp, vp, vp, vp, vp, vp, vp]
+        h.oracle_image_new.argtypes = [vp, vp, vp, C.c_uint32, C.c_uint32, vp, vp, vp, C.c_int]
+        h.oracle_closest_bdy_point.argtypes = [vp, vp]
+        h.oracle_closest_bdy_point.restype = C.c_uint32
+        for f in ("expf", "expm1f", "logf", "erff"):
+            fn = getattr(h, "oracle_" + f)
+            fn.argtypes = [C.c_float]
+            fn.restype = C.c_float
+        h.oracle_powf.argtypes = [C.c_float, C.c_float]
+        h.oracle_powf.restype = C.c_float
+        _libs[kind] = h
+    return _libs[kind]
+
+
+def _check(code, what):
+    if code != 0:
+        raise RuntimeError(f"oracle {what} failed with code {code}")
+
+
+def solve(model, opts, with_u=True, nthreads=0, kind="cr"):
+    """Returns (u, J, lambda_diag) of the reference plain method."""
+    m = model.view()
+    P, F, L, R = model.npoints, model.nfreqs, model.nlines, model.hnrays
+    u = np.zeros((R, P, F)) if with_u else None
+    J = np.zeros((P, F))
+    lam = np.zeros((P, L))
+    _check(lib(kind).oracle_solve_feautrier_order_2(
+        C.addressof(m), C.addressof(opts), None if u is None else u.ctypes.data,
+        J.ctypes.data, lam.ctypes.data, nthreads), "solve")
+    return u, J, lam
+
+
+def trace_counts(model, opts, nthreads=0, kind="cr"):
+    m = model.view()
+    n = np.zeros((model.hnrays, model.npoints, model.nlines), dtype=np.uint32)
+    _check(lib(kind).oracle_trace_counts(C.addressof(m), C.addressof(opts), n.ctypes.data,
+                                         nthreads), "trace_counts")
+    return n
+
+
+def trace_ray(model, opts, rr, o, l, capacity=4096, kind="cr"):
+    m = model.view()
+    cnt = np.zeros(1, dtype=np.uint32)
+    cen = np.zeros(1, dtype=np.uint32)
+    nr = np.zeros(capacity, dtype=np.uint32)
+    nri = np.zeros(capacity, dtype=np.uint32)
+    sh = np.zeros(capacity)
+    fa = np.zeros(capacity)
+    dz = np.zeros(capacity)
+    _check(lib(kind).oracle_trace_ray(C.addressof(m), C.addressof(opts), rr, o, l, capacity,
+                                      cnt.ctypes.data, cen.ctypes.data, nr.ctypes.data,
+                                      nri.ctypes.data, sh.ctypes.data, fa.ctypes.data,
+                                      dz.ctypes.data), "trace_ray")
+    n = int(cnt[0])
+    return dict(count=n, centre=int(cen[0]), nr=nr[:n], nr_interp=nri[:n], shift=sh[:n],
+                interp_factor=fa[:n], dZ=dz[:n])
+
+
+def image_new(model, opts, raydir, nx, ny, nthreads=0, kind="cr"):
+    m = model.view()
+    rd = np.ascontiguousarray(raydir, dtype=np.float64)
+    I = np.zeros((nx * ny, model.nfreqs))
+    ImX = np.zeros(nx * ny)
+    ImY = np.zeros(nx * ny)
+    _check(lib(kind).oracle_image_new(C.addressof(m), C.addressof(opts), rd.ctypes.data, nx, ny,
+                                      I.ctypes.data, ImX.ctypes.data, ImY.ctypes.data, nthreads),
+           "image_new")
+    return I, ImX, ImY
