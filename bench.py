@@ -1,0 +1,165 @@
+"""Benchmark of the Feautrier formal solve on one MI355X.
+
+Metric: Feautrier ray.point.freq/s -- u(rr, o, f) values produced per second,
+U = hnrays * npoints * nfreqs per solve (the shape of the reference's u
+tensor).  One step = one complete plain-method formal solve
+(Model::compute_radiation_field_feautrier_order_2: trace, Feautrier
+elimination, u, J and the Lambda diagonal) of the 32^3 jittered spherical wind
+(32768 points, Delaunay neighbours, 12 HEALPix rays, 10-level linear rotor,
+9 lines x 31 quadrature bins), with every input already resident in HBM.
+
+The JSON line also carries the roofline of the dominant kernel (algorithmic
+bytes per launch / kernel time from HIP events on the launch stream) and the
+CPU baseline: the C restatement of the reference solver (oracle/) timed on
+this host's cores on the same model.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--n-edge", type=int, default=32)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    return ap.parse_args()
+
+
+def algorithmic_bytes(stats, model, precision):
+    """Bytes one solve must move at minimum (per-unit figures of the survey):
+    36 B per walk step (neighbour count/offset, boundary flag, velocity),
+    28 B per neighbour examined (index + position), line data per traced entry
+    (inverse width, emissivity, opacity: 24 B FP64 / 12 B FP32) and again per
+    interpolated entry, plus the outputs u, J and the Lambda diagonal."""
+    S, nbs, T, Ti = stats
+    U = model.hnrays * model.npoints * model.nfreqs
+    line_b = 24 if precision == "fp64" else 12
+    return (36 * S + 28 * nbs + line_b * T + line_b * Ti + 8 * U
+            + 8 * model.npoints * model.nfreqs + 8 * model.npoints * model.nlines)
+
+
+def read_traffic(workload_key):
+    """HBM bytes per launch measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE,
+    scaled by the copy-kernel calibration), from profiles/ if recorded."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as fh:
+            d = json.load(fh)
+        return d.get(workload_key)
+    except Exception:
+        return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                          "(the multi-GPU sharding by ray pair is described in DESIGN.md)"}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from magritte_amd import _abi, models, solver
+
+    rank = int(os.environ.get("RANK", "0"))
+    prec = _abi.PREC_FP64 if args.precision == "fp64" else _abi.PREC_MIXED
+    t0 = time.time()
+    model = models.model_wind(n_edge=args.n_edge)
+    op = model.options(precision=prec)
+    dm = solver.DeviceModel(model, "cuda:0")
+    print(f"# model built in {time.time() - t0:.1f}s: P={model.npoints} hnrays={model.hnrays} "
+          f"F={model.nfreqs}", file=sys.stderr, flush=True)
+
+    P, F, L, R = model.npoints, model.nfreqs, model.nlines, model.hnrays
+    out = {
+        "u": torch.empty((R, P, F), dtype=torch.float64, device="cuda:0"),
+        "J": torch.empty((P, F), dtype=torch.float64, device="cuda:0"),
+        "lambda_diag": torch.empty((P, L), dtype=torch.float64, device="cuda:0"),
+    }
+    lib = _abi.lib()
+    lib.mgx_set_kernel_timing(1)
+    for _ in range(args.warmup):
+        solver.feautrier_forward(dm, op, out=out)
+    torch.cuda.synchronize()
+    kernel_ms = []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        solver.feautrier_forward(dm, op, out=out)
+        kernel_ms.append(lib.mgx_last_kernel_ms())
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    lib.mgx_set_kernel_timing(0)
+    ms_per_step = 1e3 * elapsed / args.steps
+    U = R * P * F
+    value = U / (elapsed / args.steps)
+    print(f"# {args.steps} steps: {ms_per_step:.3f} ms/step, kernel {np.mean(kernel_ms):.3f} ms",
+          file=sys.stderr, flush=True)
+
+    stats = solver.trace_stats(dm, op)
+    B = algorithmic_bytes(stats, model, args.precision)
+    k_ms = float(np.mean(kernel_ms))
+    achieved = B / (k_ms * 1e-3) / 1e9
+    workload = f"wind{args.n_edge}^3_{args.precision}"
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": read_traffic(workload), "kernel": "feautrier_kernel",
+                "kernel_ms": round(k_ms, 4), "alg_bytes": int(B),
+                "walk_steps": stats[0], "neighbours": stats[1], "traced_entries": stats[2],
+                "interp_entries": stats[3]}
+
+    J_gpu = out["J"].cpu().numpy()
+    cpu = None
+    j_err = None
+    if not args.no_cpu_baseline and rank == 0:
+        from oracle import oracle
+
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        t1 = time.perf_counter()
+        _, J_ref, _ = oracle.solve(model, model.options(), with_u=False, nthreads=threads)
+        tc = time.perf_counter() - t1
+        cpu = {"value": round(U / tc, 1), "unit": "ray*point*freq/s", "cores": threads,
+               "kind": "port",
+               "sample": f"one full solve of the same wind{args.n_edge}^3 model "
+                         f"(setup pass + trace + solve + J + Lambda), {tc:.2f}s, "
+                         f"{threads} OpenMP threads"}
+        j_err = float(np.max(np.abs(J_gpu - J_ref) / np.maximum(np.abs(J_ref), 1e-300)))
+
+    line = {
+        "metric": "Feautrier ray.point.freq/s (forward)",
+        "value": round(value, 1),
+        "unit": "ray*point*freq/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.n_edge}^3 jittered spherical wind, Delaunay neighbours, "
+                               f"12 HEALPix rays, 10-level linear rotor, 9 lines x 31 bins",
+                   "npoints": P, "hnrays": R, "nfreqs": F, "precision": args.precision,
+                   "approx": "CloseLines", "J_rel_err_vs_cpu_fp64": j_err},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if rank == 0:
+        print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

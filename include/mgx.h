@@ -225,6 +225,26 @@ int mgx_compute_interp_crit(uint32_t npoints, uint32_t nlspecs, const mgx_lineda
 int mgx_sort_lines(uint32_t nlines, const double *line, double *sorted_line,
                    uint32_t *sorted_line_map);
 
+/*
+ * Workload statistics of the trace, for the algorithmic-byte model of the
+ * solve: stats[0] = walk steps S (both halves, all ray pairs and origins),
+ * stats[1] = neighbours examined (sum of n_neighbors over the points stepped
+ * from), stats[2] = traced entries T (sum of n_tot over ray pairs, origins and
+ * lines), stats[3] = interpolated entries (interp_factor != 0).  No
+ * counterpart in the reference (measurement aid).
+ */
+int mgx_trace_stats_device(const mgx_model *model_dev, const mgx_options *opts,
+                           uint64_t *stats, void *stream);
+
+/*
+ * Kernel timing: when enabled, the device entry points record HIP events
+ * around their main kernel on the caller's stream; mgx_last_kernel_ms()
+ * returns the duration of the last one (ms).  No counterpart in the reference
+ * (measurement aid).
+ */
+void mgx_set_kernel_timing(int enable);
+double mgx_last_kernel_ms(void);
+
 #ifdef __cplusplus
 }
 #endif

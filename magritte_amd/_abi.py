@@ -98,6 +98,9 @@ SIGNATURES = [
     ("mgx_compute_interp_crit", C.c_int,
      [C.c_uint32, C.c_uint32, C.POINTER(MgxLinedata), _VP, _VP, _VP]),
     ("mgx_sort_lines", C.c_int, [C.c_uint32, _VP, _VP, _VP]),
+    ("mgx_trace_stats_device", C.c_int, [C.POINTER(MgxModel), C.POINTER(MgxOptions), _VP, _VP]),
+    ("mgx_set_kernel_timing", None, [C.c_int]),
+    ("mgx_last_kernel_ms", C.c_double, []),
 ]
 
 _lib = None

@@ -808,6 +808,64 @@ __global__ __launch_bounds__(MGX_WAVE) void trace_counts_kernel(KParams kp, uint
     }
 }
 
+/* one wave per (rr, o): walk and trace statistics (S, neighbours, T, T_interp) */
+__global__ __launch_bounds__(MGX_WAVE) void trace_stats_kernel(KParams kp, unsigned long long *stats)
+{
+    extern __shared__ unsigned char smem[];
+    const mgx_model &m = kp.m;
+    const uint32_t o = blockIdx.x, rr = blockIdx.y, lane = threadIdx.x;
+    const uint32_t cap = kp.cap;
+    uint32_t *s_nxt0 = reinterpret_cast<uint32_t *>(smem);
+    uint32_t *s_nxt1 = s_nxt0 + cap;
+    double *s_dZ0 = reinterpret_cast<double *>(smem + 8 * ((2 * cap * 4 + 7) / 8));
+    double *s_dZ1 = s_dZ0 + cap;
+    double *s_sh0 = s_dZ1 + cap;
+    double *s_sh1 = s_sh0 + cap;
+    uint32_t S0, S1;
+    double sh00, sh01;
+    int e = walk_half(kp, o, rr, s_nxt0, s_dZ0, s_sh0, S0, sh00);
+    if (!e) e = walk_half(kp, o, m.antipod[rr], s_nxt1, s_dZ1, s_sh1, S1, sh01);
+    if (e) {
+        if (lane == 0) {
+            atomicOr(&kp.status[0], (uint32_t)e);
+            if (e == ERR_CAP) atomicMax(&kp.status[1], kp.cap + 1);
+        }
+        return;
+    }
+    __syncthreads();
+    unsigned long long nb = 0, T = 0, Ti = 0;
+    for (uint32_t l = lane; l < m.nlines; l += MGX_WAVE) {
+        uint32_t n = 1, ni = 0;
+        for (uint32_t s = 0; s < S0; s++) {
+            const uint32_t k = n_interp_for_line(kp, l, s == 0 ? o : s_nxt0[s - 1], s_nxt0[s]);
+            n += k;
+            ni += k - 1;
+        }
+        for (uint32_t s = 0; s < S1; s++) {
+            const uint32_t k = n_interp_for_line(kp, l, s == 0 ? o : s_nxt1[s - 1], s_nxt1[s]);
+            n += k;
+            ni += k - 1;
+        }
+        T += n;
+        Ti += ni;
+    }
+    if (lane == 0) {
+        for (uint32_t s = 0; s < S0; s++) nb += m.n_neighbors[s == 0 ? o : s_nxt0[s - 1]];
+        for (uint32_t s = 0; s < S1; s++) nb += m.n_neighbors[s == 0 ? o : s_nxt1[s - 1]];
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        T += __shfl_xor(T, off, MGX_WAVE);
+        Ti += __shfl_xor(Ti, off, MGX_WAVE);
+    }
+    if (lane == 0) {
+        atomicAdd(&stats[0], (unsigned long long)(S0 + S1));
+        atomicAdd(&stats[1], nb);
+        atomicAdd(&stats[2], T);
+        atomicAdd(&stats[3], Ti);
+    }
+}
+
 /* one wave per selected (rr, o, l); lane 0 writes the expanded list */
 __global__ __launch_bounds__(MGX_WAVE) void trace_rays_kernel(
     KParams kp, const uint32_t *sel_rr, const uint32_t *sel_o, const uint32_t *sel_l,
@@ -977,6 +1035,31 @@ size_t walk_lds_bytes(uint32_t cap) { return 8 * ((2 * (size_t)cap * 4 + 7) / 8)
 
 const uint32_t kMaxCap = 1536; /* 1536 steps per half: 61 KB of LDS */
 
+bool g_timing = false;
+double g_last_kernel_ms = 0.0;
+
+/* launches fn() between two events on st when timing is enabled */
+template <typename Fn>
+void timed(hipStream_t st, Fn fn)
+{
+    if (!g_timing) {
+        fn();
+        return;
+    }
+    hipEvent_t a, b;
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    (void)hipEventRecord(a, st);
+    fn();
+    (void)hipEventRecord(b, st);
+    (void)hipEventSynchronize(b);
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, a, b);
+    g_last_kernel_ms = ms;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+}
+
 int prepare(const mgx_model *m, const mgx_options *op, hipStream_t st, DevBuffers &b, KParams &kp)
 {
     const size_t PL = (size_t)m->npoints * m->nlines;
@@ -1053,6 +1136,7 @@ int mgx_feautrier_forward_device(const mgx_model *model, const mgx_options *opts
     if (int e = prepare(model, opts, st, b, kp)) return e;
     return launch_with_capacity(st, b, kp, [&](size_t lds) {
         const dim3 grid(model->npoints), block(MGX_WAVE);
+        timed(st, [&] {
         switch (opts->approx) {
         case MGX_APPROX_NONE:
             hipLaunchKernelGGL(feautrier_kernel<MGX_APPROX_NONE>, grid, block, lds, st, kp, u, J,
@@ -1066,7 +1150,37 @@ int mgx_feautrier_forward_device(const mgx_model *model, const mgx_options *opts
             hipLaunchKernelGGL(feautrier_kernel<MGX_APPROX_CLOSE_LINES>, grid, block, lds, st, kp,
                                u, J, lambda_diag);
         }
+        });
     });
+}
+
+void mgx_set_kernel_timing(int enable) { g_timing = enable != 0; }
+
+double mgx_last_kernel_ms(void) { return g_last_kernel_ms; }
+
+int mgx_trace_stats_device(const mgx_model *model, const mgx_options *opts, uint64_t *stats,
+                           void *stream)
+{
+    mgx::clear_error();
+    if (int e = check_view(model, opts)) return e;
+    if (!stats) return mgx::fail(MGX_E_ARG, "stats is required");
+    hipStream_t st = (hipStream_t)stream;
+    DevBuffers b;
+    KParams kp;
+    if (int e = prepare(model, opts, st, b, kp)) return e;
+    unsigned long long *d = nullptr;
+    HIP_TRY(hipMalloc(&d, 4 * sizeof(unsigned long long)));
+    int e = launch_with_capacity(st, b, kp, [&](size_t lds) {
+        (void)hipMemsetAsync(d, 0, 4 * sizeof(unsigned long long), st);
+        hipLaunchKernelGGL(trace_stats_kernel, dim3(model->npoints, model->hnrays),
+                           dim3(MGX_WAVE), lds, st, kp, d);
+    });
+    if (!e) {
+        hipError_t he = hipMemcpy(stats, d, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost);
+        if (he != hipSuccess) e = mgx::fail(MGX_E_HIP, "stats copy: %s", hipGetErrorString(he));
+    }
+    (void)hipFree(d);
+    return e;
 }
 
 int mgx_trace_counts_device(const mgx_model *model, const mgx_options *opts, uint32_t *n_tot,

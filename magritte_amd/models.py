@@ -26,6 +26,7 @@ HH = 6.62607004e-34
 KB = 1.38064852e-23
 AMU = 1.66053904e-27
 T_CMB = 2.72548
+CC_SQUARED = 8.98755179e16  # the rounded literal of constants.hpp
 
 # CGS constants for Einstein A of a rotor
 _H_CGS = 6.62607004e-27
@@ -437,4 +438,49 @@ def model_wind(n_edge=32, half_size=1.0e14, nrays=12, nquads=31, seed=None, nlev
     abund = np.stack([dens, 1.0e-4 * dens], axis=1)
     m.set_thermo(temp, np.full(len(r), vturb), abund)
     m.set_lines([linedata_linear_rotor(B, dipole, nlev)], nquads)
+    return m.finalize()
+
+
+def read_lamda(path, num=1):
+    """Radiative part of a LAMDA-format line data file, as set_linedata_from_LAMDA_file
+    stores it (setup.py:632-779): energies in J, levels 0-based, derived Bs, Ba, frequency."""
+    with open(path) as fh:
+        lines = [l.strip() for l in fh if l.strip()]
+    data = [l for l in lines if not l.startswith("!")]
+    mass = float(data[1])
+    nlev = int(data[2])
+    energy, weight = [], []
+    for i in range(nlev):
+        parts = data[3 + i].split()
+        energy.append(float(parts[1]) * 1.0e2 * HH * CC)
+        weight.append(float(parts[2]))
+    nrad = int(data[3 + nlev])
+    irad, jrad, A = [], [], []
+    for k in range(nrad):
+        parts = data[4 + nlev + k].split()
+        irad.append(int(parts[1]) - 1)
+        jrad.append(int(parts[2]) - 1)
+        A.append(float(parts[3]))
+    return _linedata(num, mass, np.array(energy), np.array(weight), irad, jrad, A)
+
+
+def model_constant_slab(npoints=50, dx=1.0e12, n_species=1.0e3, n_h2=1.0e12, temperature=4.5,
+                        turb=0.0, nquads=1, linedata=None):
+    """The reference's all-constant single-ray benchmark geometry
+    (tests/benchmarks/analytic/all_constant_single_ray.py:18-70)."""
+    P = npoints
+    pos = np.zeros((P, 3))
+    pos[:, 0] = np.arange(P) * dx
+    vel = np.zeros((P, 3))
+    nbs = [[1]] + [[p - 1, p + 1] for p in range(1, P - 1)] + [[P - 2]]
+    m = ModelData()
+    m.dimension = 1
+    m.set_geometry(pos, vel, nbs, [0, P - 1], [[-1.0, 0.0, 0.0], [1.0, 0.0, 0.0]], [0.5, 0.5])
+    abund = np.zeros((P, 3))
+    abund[:, 0] = n_species
+    abund[:, 1] = n_h2
+    m.set_thermo(np.full(P, temperature), np.full(P, turb), abund)
+    ld = linedata if linedata is not None else linedata_two_level(179.87547e9, 1.0e-4, mass=1.0, num=0)
+    ld = dict(ld, num=0)
+    m.set_lines([ld], nquads)
     return m.finalize()

@@ -90,3 +90,12 @@ def trace_rays(dm, opts, sel, capacity=1024):
         res.append(dict(count=k, centre=int(cen[i]), nr=nr[i, :k], nr_interp=nri[i, :k],
                         shift=sh[i, :k], interp_factor=fa[i, :k], dZ=dz[i, :k]))
     return res
+
+
+def trace_stats(dm, opts):
+    """(walk steps S, neighbours examined, traced entries T, interpolated entries)."""
+    _require_gpu()
+    st = np.zeros(4, dtype=np.uint64)
+    _abi.check(_abi.lib().mgx_trace_stats_device(C.byref(dm.view), C.byref(opts),
+                                                 st.ctypes.data, dm.stream_ptr()))
+    return tuple(int(x) for x in st)

@@ -1296,6 +1296,61 @@ int oracle_image_new(const mgx_model *m, const mgx_options *op, const double *ra
     return 0;
 }
 
+
+/*
+ * The tridiagonal system behind one (rr, o, f) solve, for the brute-force
+ * checks: dtau[j] of the segment between list entries j and j+1 and the row
+ * sources S[j] (j = 0 .. n_tot-1, entry 0 = first), the boundary intensities,
+ * plus u(centre) and the Lambda diagonal from the reference elimination.
+ */
+int oracle_feautrier_system(const mgx_model *m, const mgx_options *op, uint32_t rr, uint32_t o,
+                            uint32_t f, uint32_t capacity, uint32_t *n_out, uint32_t *centre_out,
+                            double *dtau, double *S, double *I_bdy, double *u_c, double *L_c)
+{
+    int err = check_model(m, op);
+    if (err) return err;
+    const uint32_t length = 2 * get_ray_lengths_max(m, op, &err) + 1;
+    if (err) return err;
+    Scratch s;
+    if (scratch_init(&s, length)) return MGX_E_ARG;
+    const uint32_t l = m->f_line[f];
+    const uint32_t ar = m->antipod[rr];
+    s.nr[s.centre] = o;
+    s.shift[s.centre] = 1.0;
+    s.first = trace_ray_for_line(m, op, &s, l, o, rr, -1, s.centre - 1, s.centre - 1, &err) + 1;
+    s.last = trace_ray_for_line(m, op, &s, l, o, ar, +1, s.centre + 1, s.centre, &err) - 1;
+    s.n_tot = (s.last + 1) - s.first;
+    *n_out = s.n_tot;
+    *centre_out = s.centre - s.first;
+    if (!err && (s.n_tot < 2 || s.n_tot > capacity)) err = MGX_E_RAY_LENGTH;
+    if (!err) {
+        const double freq = m->nu[(size_t)o * m->nfreqs + f];
+        double dtau_n, chi_c = 0.0, chi_n = 0.0, term_c, term_n;
+        int cco = 1;
+        const uint32_t *nr = s.nr, *nri = s.nr_interp;
+        for (uint32_t n = s.first; n < s.last; n++) {
+            if (n > s.first) {
+                term_c = term_n;
+                chi_c = chi_n;
+            }
+            compute_source_dtau(m, op, nr[n], nr[n + 1], nri[n], nri[n + 1], l, freq * s.shift[n],
+                                freq * s.shift[n + 1], s.shift[n], s.shift[n + 1], s.dZ[n],
+                                s.interp_factor[n], s.interp_factor[n + 1], &cco, &dtau_n, &chi_c,
+                                &chi_n, &term_c, &term_n);
+            dtau[n - s.first] = dtau_n;
+            S[n - s.first] = term_c;
+        }
+        S[s.n_tot - 1] = term_n;
+        I_bdy[0] = boundary_intensity(m, nr[s.first], freq * s.shift[s.first]);
+        I_bdy[1] = boundary_intensity(m, nr[s.last], freq * s.shift[s.last]);
+        solve_feautrier_order_2_of(m, op, &s, o, f);
+        *u_c = s.Su[s.centre];
+        *L_c = s.L_diag[s.centre];
+    }
+    scratch_free(&s);
+    return err;
+}
+
 /* exported transcendentals, for the tests that pin them */
 float oracle_expf(float x) { return O_EXPF(x); }
 float oracle_expm1f(float x) { return O_EXPM1F(x); }

@@ -26,6 +26,10 @@ int oracle_trace_ray(const mgx_model *m, const mgx_options *op, uint32_t rr, uin
 int oracle_image_new(const mgx_model *m, const mgx_options *op, const double *raydir, uint32_t nx,
                      uint32_t ny, double *I, double *ImX, double *ImY, int nthreads);
 uint32_t oracle_closest_bdy_point(const mgx_model *m, const double *raydir);
+/* tridiagonal system of one (rr, o, f) solve: dtau (n-1), S (n), I_bdy (2), u_c, L_c */
+int oracle_feautrier_system(const mgx_model *m, const mgx_options *op, uint32_t rr, uint32_t o,
+                            uint32_t f, uint32_t capacity, uint32_t *n_out, uint32_t *centre_out,
+                            double *dtau, double *S, double *I_bdy, double *u_c, double *L_c);
 
 float oracle_expf(float x);
 float oracle_expm1f(float x);

@@ -30,6 +30,8 @@ def lib(kind="cr"):
         h.oracle_image_new.argtypes = [vp, vp, vp, C.c_uint32, C.c_uint32, vp, vp, vp, C.c_int]
         h.oracle_closest_bdy_point.argtypes = [vp, vp]
         h.oracle_closest_bdy_point.restype = C.c_uint32
+        h.oracle_feautrier_system.argtypes = [vp, vp, C.c_uint32, C.c_uint32, C.c_uint32,
+                                              C.c_uint32, vp, vp, vp, vp, vp, vp, vp]
         for f in ("expf", "expm1f", "logf", "erff"):
             fn = getattr(h, "oracle_" + f)
             fn.argtypes = [C.c_float]
@@ -94,3 +96,22 @@ def image_new(model, opts, raydir, nx, ny, nthreads=0, kind="cr"):
                                       I.ctypes.data, ImX.ctypes.data, ImY.ctypes.data, nthreads),
            "image_new")
     return I, ImX, ImY
+
+
+def feautrier_system(model, opts, rr, o, f, capacity=4096, kind="cr"):
+    """dtau (n-1), S (n), I_bdy (2), centre, u_c and L_c of one (rr, o, f) solve."""
+    m = model.view()
+    n = np.zeros(1, dtype=np.uint32)
+    c = np.zeros(1, dtype=np.uint32)
+    dtau = np.zeros(capacity)
+    S = np.zeros(capacity)
+    I = np.zeros(2)
+    uc = np.zeros(1)
+    lc = np.zeros(1)
+    _check(lib(kind).oracle_feautrier_system(C.addressof(m), C.addressof(opts), rr, o, f, capacity,
+                                             n.ctypes.data, c.ctypes.data, dtau.ctypes.data,
+                                             S.ctypes.data, I.ctypes.data, uc.ctypes.data,
+                                             lc.ctypes.data), "feautrier_system")
+    k = int(n[0])
+    return dict(n=k, centre=int(c[0]), dtau=dtau[:k - 1], S=S[:k], I_bdy=I, u_c=float(uc[0]),
+                L_c=float(lc[0]))

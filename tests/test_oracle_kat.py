@@ -1,0 +1,175 @@
+"""Known-answer and brute-force checks that pin the CPU oracle.
+
+* constant slab: the reference's all-constant single-ray benchmark closed form
+  u = (I0 + I1)/2, I = S + (I_bdy - S) exp(-chi x)
+  (tests/benchmarks/analytic/all_constant_single_ray.py:112-131, bound :180);
+* the elimination: u(centre) and the Lambda diagonal of the oracle against an
+  exact (50-digit) solve of the tridiagonal Feautrier system built from the
+  same optical depths and sources (the reference's T u = rhs and
+  Lambda = diag(T^-1) idea, tests/test_solver_lambda.cpp:15-88);
+* the walk: on a 1D chain the traced list is exactly o-1, ..., 0 and
+  o+1, ..., P-1; on any model the ends are boundary points, dZ > 0 and the
+  interpolation counts equal ceil(|log(ratio)| / log(1.4)) (exact logs).
+"""
+import math
+import os
+
+import mpmath as mp
+import numpy as np
+import pytest
+
+from magritte_amd import models
+from oracle import oracle
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def rel(a, b):
+    return 2.0 * (a - b) / (a + b)  # magritte/tools.py relative_error
+
+
+def test_constant_slab_closed_form():
+    ld = models.read_lamda(os.path.join(HERE, "golden", "lamda_test.txt"), num=0)
+    m = models.model_constant_slab(linedata=ld)
+    op = m.options()
+    u, J, lam = oracle.solve(m, op)
+    # closed form (magritte/tools.py LTEpop, profile, lineEmissivity, lineOpacity, I_CMB)
+    T = 4.5
+    w = ld["weight"] * np.exp(-ld["energy"] / (models.KB * T))
+    pop = w / w.sum() * 1.0e3
+    frq = ld["frequency"][0]
+    width = frq * math.sqrt(2.0 * models.KB * T / (models.AMU * models.CC_SQUARED) * ld["inverse_mass"])
+    phi = 1.0 / (width * math.sqrt(math.pi))
+    hv4pi = models.HH * frq / (4.0 * math.pi)
+    eta = hv4pi * ld["A"][0] * pop[1] * phi
+    chi = hv4pi * (pop[0] * ld["Ba"][0] - pop[1] * ld["Bs"][0]) * phi
+    src = eta / chi
+    bdy = 2.0 * models.HH / models.CC_SQUARED * frq**3 / math.expm1(models.HH * frq / (models.KB * models.T_CMB))
+    x = m.position[:, 0]
+    I0 = src + (bdy - src) * np.exp(-chi * x)
+    I1 = src + (bdy - src) * np.exp(-chi * (x[-1] - x))
+    ua = 0.5 * (I0 + I1)
+    err = np.max(np.abs(rel(ua, u[0, :, 0])))
+    assert err < 1.7e-4, err
+    # and it is a real test: the slab is neither transparent nor opaque everywhere
+    assert 0.1 < chi * x[-1] < 1e3
+
+
+def _exact_tridiag(sys_):
+    """u and diag(T^-1) at the centre of the reference's second-order Feautrier system."""
+    mp.mp.dps = 50
+    n = sys_["n"]
+    dt = [mp.mpf(float(v)) for v in sys_["dtau"]]
+    S = [mp.mpf(float(v)) for v in sys_["S"]]
+    If, Il = (mp.mpf(float(v)) for v in sys_["I_bdy"])
+    a = [mp.mpf(0)] * n  # sub-diagonal (coefficient of u_{j-1})
+    b = [mp.mpf(0)] * n
+    c = [mp.mpf(0)] * n  # super-diagonal
+    r = [mp.mpf(0)] * n
+    Cf = 2 / dt[0] ** 2
+    b[0] = 1 + 2 / dt[0] + Cf
+    c[0] = -Cf
+    r[0] = S[0] + 2 * If / dt[0]
+    for j in range(1, n - 1):
+        avg = (dt[j - 1] + dt[j]) / 2
+        A = 1 / (avg * dt[j - 1])
+        C = 1 / (avg * dt[j])
+        a[j], b[j], c[j], r[j] = -A, 1 + A + C, -C, S[j]
+    Al = 2 / dt[n - 2] ** 2
+    a[n - 1] = -Al
+    b[n - 1] = 1 + 2 / dt[n - 2] + Al
+    r[n - 1] = S[n - 1] + 2 * Il / dt[n - 2]
+
+    def solve(rhs):
+        cp = [mp.mpf(0)] * n
+        dp = [mp.mpf(0)] * n
+        cp[0] = c[0] / b[0]
+        dp[0] = rhs[0] / b[0]
+        for j in range(1, n):
+            den = b[j] - a[j] * cp[j - 1]
+            cp[j] = c[j] / den if j < n - 1 else 0
+            dp[j] = (rhs[j] - a[j] * dp[j - 1]) / den
+        x = [mp.mpf(0)] * n
+        x[n - 1] = dp[n - 1]
+        for j in range(n - 2, -1, -1):
+            x[j] = dp[j] - cp[j] * x[j + 1]
+        return x
+
+    cidx = sys_["centre"]
+    u = solve(r)[cidx]
+    e = [mp.mpf(0)] * n
+    e[cidx] = mp.mpf(1)
+    lam = solve(e)[cidx]
+    return float(u), float(lam)
+
+
+@pytest.mark.parametrize("which", ["line_model", "small_wind"])
+def test_elimination_matches_exact_solve(which, request):
+    m = request.getfixturevalue(which)
+    op = m.options()
+    rng = np.random.default_rng(11)
+    checked = 0
+    worst_u = worst_l = 0.0
+    for _ in range(40):
+        rr = int(rng.integers(0, m.hnrays))
+        o = int(rng.integers(0, m.npoints))
+        f = int(rng.integers(0, m.nfreqs))
+        try:
+            s = oracle.feautrier_system(m, op, rr, o, f)
+        except RuntimeError:
+            continue  # n_tot < 2: boundary branch, no system
+        u_ex, l_ex = _exact_tridiag(s)
+        worst_u = max(worst_u, abs(s["u_c"] - u_ex) / abs(u_ex))
+        worst_l = max(worst_l, abs(s["L_c"] - l_ex) / abs(l_ex))
+        checked += 1
+    assert checked >= 20
+    assert worst_u < 1e-11 and worst_l < 1e-11, (worst_u, worst_l)
+
+
+def test_walk_1d_chain_brute_force(line_model):
+    m = line_model
+    op = m.options()
+    P = m.npoints
+    for o in (0, 1, 37, P - 2, P - 1):
+        t = oracle.trace_ray(m, op, 0, o, 0)
+        c = t["centre"]
+        # ray 0 points along -x: entries first..centre-1 are 0..o-1, then o, then o+1..P-1
+        np.testing.assert_array_equal(t["nr"], np.arange(P))
+        assert c == o
+        assert np.all(t["interp_factor"] == 0.0)
+        dx = m.position[1, 0] - m.position[0, 0]
+        assert np.allclose(t["dZ"][:-1], dx, rtol=1e-12)
+
+
+def test_walk_invariants_and_interpolation_counts(small_wind):
+    m = small_wind
+    op = m.options()
+    rng = np.random.default_rng(3)
+    log14 = float(np.float32(mp.log(mp.mpf(float(np.float32(1.4))))))
+    for _ in range(30):
+        rr, o, l = int(rng.integers(0, m.hnrays)), int(rng.integers(0, m.npoints)), int(rng.integers(0, m.nlines))
+        t = oracle.trace_ray(m, op, rr, o, l)
+        n, c = t["count"], t["centre"]
+        nr = t["nr"]
+        if n > 1:
+            # both ends are boundary points (unless the origin itself is the end)
+            assert m.point2boundary[nr[0]] != m.npoints or c == 0
+            assert m.point2boundary[nr[-1]] != m.npoints or c == n - 1
+            assert np.all(t["dZ"][:-1] > 0.0)
+        # interpolation counts: group entries by their (nr, nr_interp) segment
+        j = 0
+        while j < n:
+            if j == c:
+                j += 1
+                continue
+            seg = (nr[j], t["nr_interp"][j])
+            k = j
+            while k < n and k != c and (nr[k], t["nr_interp"][k]) == seg:
+                k += 1
+            cnt = k - j
+            a, b = m.interp_crit[seg[1], l], m.interp_crit[seg[0], l]
+            ratio = float(np.float32(b / a))
+            lg = float(np.float32(abs(mp.log(mp.mpf(ratio)))))
+            expect = max(1, math.ceil(lg / log14))
+            assert cnt == expect, (rr, o, l, j, cnt, expect)
+            j = k
