@@ -31,7 +31,6 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--n-edge", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -63,26 +62,14 @@ def read_traffic(workload_key):
         return None
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                          "(the multi-GPU sharding by ray pair is described in DESIGN.md)"}))
-        sys.exit(2)
+def run_variant(dm, model, precision, steps, warmup):
+    """Times `steps` solves; returns (ms_per_step, kernel_ms, J)."""
     import numpy as np
     import torch
 
-    from magritte_amd import _abi, models, solver
+    from magritte_amd import _abi, solver
 
-    rank = int(os.environ.get("RANK", "0"))
-    prec = _abi.PREC_FP64 if args.precision == "fp64" else _abi.PREC_MIXED
-    t0 = time.time()
-    model = models.model_wind(n_edge=args.n_edge)
-    op = model.options(precision=prec)
-    dm = solver.DeviceModel(model, "cuda:0")
-    print(f"# model built in {time.time() - t0:.1f}s: P={model.npoints} hnrays={model.hnrays} "
-          f"F={model.nfreqs}", file=sys.stderr, flush=True)
-
+    op = model.options(precision=_abi.PREC_FP64 if precision == "fp64" else _abi.PREC_MIXED)
     P, F, L, R = model.npoints, model.nfreqs, model.nlines, model.hnrays
     out = {
         "u": torch.empty((R, P, F), dtype=torch.float64, device="cuda:0"),
@@ -91,38 +78,48 @@ def main():
     }
     lib = _abi.lib()
     lib.mgx_set_kernel_timing(1)
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         solver.feautrier_forward(dm, op, out=out)
     torch.cuda.synchronize()
     kernel_ms = []
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
+    t0 = time.perf_counter()
+    for _ in range(steps):
         solver.feautrier_forward(dm, op, out=out)
         kernel_ms.append(lib.mgx_last_kernel_ms())
     torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t_start
+    elapsed = time.perf_counter() - t0
     lib.mgx_set_kernel_timing(0)
-    ms_per_step = 1e3 * elapsed / args.steps
+    return 1e3 * elapsed / steps, float(np.mean(kernel_ms)), out["J"].cpu().numpy()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                          "(the multi-GPU sharding by ray pair is described in DESIGN.md)"}))
+        sys.exit(2)
+    import numpy as np
+
+    from magritte_amd import models, solver
+
+    rank = int(os.environ.get("RANK", "0"))
+    t0 = time.time()
+    model = models.model_wind(n_edge=args.n_edge)
+    dm = solver.DeviceModel(model, "cuda:0")
+    P, F, L, R = model.npoints, model.nfreqs, model.nlines, model.hnrays
     U = R * P * F
-    value = U / (elapsed / args.steps)
-    print(f"# {args.steps} steps: {ms_per_step:.3f} ms/step, kernel {np.mean(kernel_ms):.3f} ms",
-          file=sys.stderr, flush=True)
+    print(f"# model built in {time.time() - t0:.1f}s: P={P} hnrays={R} F={F}", file=sys.stderr,
+          flush=True)
 
-    stats = solver.trace_stats(dm, op)
-    B = algorithmic_bytes(stats, model, args.precision)
-    k_ms = float(np.mean(kernel_ms))
-    achieved = B / (k_ms * 1e-3) / 1e9
-    workload = f"wind{args.n_edge}^3_{args.precision}"
-    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": read_traffic(workload), "kernel": "feautrier_kernel",
-                "kernel_ms": round(k_ms, 4), "alg_bytes": int(B),
-                "walk_steps": stats[0], "neighbours": stats[1], "traced_entries": stats[2],
-                "interp_entries": stats[3]}
+    variants = {}
+    for prec in ("fp64", "mixed"):
+        ms, kms, J = run_variant(dm, model, prec, args.steps, args.warmup)
+        variants[prec] = {"ms_per_step": round(ms, 4), "kernel_ms": round(kms, 4),
+                          "value": round(U / (ms * 1e-3), 1), "J": J}
+        print(f"# {prec}: {ms:.3f} ms/step, kernel {kms:.3f} ms", file=sys.stderr, flush=True)
 
-    J_gpu = out["J"].cpu().numpy()
+    stats = solver.trace_stats(dm, model.options())
     cpu = None
-    j_err = None
     if not args.no_cpu_baseline and rank == 0:
         from oracle import oracle
 
@@ -135,25 +132,43 @@ def main():
                "sample": f"one full solve of the same wind{args.n_edge}^3 model "
                          f"(setup pass + trace + solve + J + Lambda), {tc:.2f}s, "
                          f"{threads} OpenMP threads"}
-        j_err = float(np.max(np.abs(J_gpu - J_ref) / np.maximum(np.abs(J_ref), 1e-300)))
+        for v in variants.values():
+            v["J_rel_err_vs_cpu_fp64"] = float(
+                np.max(np.abs(v["J"] - J_ref) / np.maximum(np.abs(J_ref), 1e-300)))
+    for v in variants.values():
+        del v["J"]
 
+    # headline: the mixed-precision solve when it meets the J bar, else FP64
+    head = "mixed"
+    if variants["mixed"].get("J_rel_err_vs_cpu_fp64", 0.0) > 1e-5:
+        head = "fp64"
+    hv = variants[head]
+    B = algorithmic_bytes(stats, model, head)
+    achieved = B / (hv["kernel_ms"] * 1e-3) / 1e9
+    workload = f"wind{args.n_edge}^3_{head}"
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": read_traffic(workload), "kernel": "feautrier_kernel",
+                "kernel_ms": hv["kernel_ms"], "alg_bytes": int(B),
+                "walk_steps": stats[0], "neighbours": stats[1], "traced_entries": stats[2],
+                "interp_entries": stats[3]}
     line = {
         "metric": "Feautrier ray.point.freq/s (forward)",
-        "value": round(value, 1),
+        "value": hv["value"],
         "unit": "ray*point*freq/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": hv["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "dtype": "f64" if head == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {"workload": f"{args.n_edge}^3 jittered spherical wind, Delaunay neighbours, "
                                f"12 HEALPix rays, 10-level linear rotor, 9 lines x 31 bins",
-                   "npoints": P, "hnrays": R, "nfreqs": F, "precision": args.precision,
-                   "approx": "CloseLines", "J_rel_err_vs_cpu_fp64": j_err},
+                   "npoints": P, "hnrays": R, "nfreqs": F, "precision": head,
+                   "approx": "CloseLines", "variants": variants},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -134,7 +134,8 @@ MGX_FM_HD float mgx_powf(float x, float y)
     return (float)mgx_exp_d((double)y * mgx_log_d(xd));
 }
 
-MGX_FM_HD float mgx_erff(float x)
+/* erf with the interval coefficient table passed in (the kernels keep a copy in LDS) */
+MGX_FM_HD float mgx_erff_tab(float x, const double *tab)
 {
     double xd = (double)x;
     if (xd != xd) return x;
@@ -149,7 +150,7 @@ MGX_FM_HD float mgx_erff(float x)
     } else if (a < 4.0) {
         const int j = (int)(a * 4.0); /* 1..15 */
         const double t = a - ((double)j * 0.25 + 0.125);
-        const double *c = &MGX_ERF_TAB[(j - 1) * (MGX_ERF_DEG + 1)];
+        const double *c = &tab[(j - 1) * (MGX_ERF_DEG + 1)];
         double p = c[MGX_ERF_DEG];
         for (int n = MGX_ERF_DEG - 1; n >= 0; n--) p = p * t + c[n];
         r = p;
@@ -158,3 +159,5 @@ MGX_FM_HD float mgx_erff(float x)
     }
     return (float)(sgn * r);
 }
+
+MGX_FM_HD float mgx_erff(float x) { return mgx_erff_tab(x, MGX_ERF_TAB); }

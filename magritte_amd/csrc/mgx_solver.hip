@@ -231,6 +231,29 @@ __device__ inline uint32_t n_interp_for_line(const KParams &kp, uint32_t l, uint
     return (k > 1) ? k : 1;
 }
 
+/* float transcendentals: correctly rounded (mgx_fmath.h) in FP64 mode, the
+   hardware/ROCm-libm single-precision functions in mixed mode */
+template <int PREC>
+__device__ inline float f_exp(float x)
+{
+    if (PREC == MGX_PREC_FP64) return mgx_expf(x);
+    return __expf(x);
+}
+
+template <int PREC>
+__device__ inline float f_pow(float x, float y)
+{
+    if (PREC == MGX_PREC_FP64) return mgx_powf(x, y);
+    return __powf(x, y);
+}
+
+template <int PREC>
+__device__ inline float f_erf(float x, const double *tab)
+{
+    if (PREC == MGX_PREC_FP64) return mgx_erff_tab(x, tab);
+    return erff(x);
+}
+
 __device__ inline double interp_linear(double a, double b, double t)
 {
     if (t == 0.0) return a;
@@ -238,20 +261,22 @@ __device__ inline double interp_linear(double a, double b, double t)
     return (b - a) * t + a;
 }
 
+template <int PREC>
 __device__ inline double interp_log(double a, double b, double t)
 {
     if (t == 0.0) return a;
     if (t == 1.0) return b;
-    return a * (double)mgx_powf((float)(b / a), (float)t);
+    return a * (double)f_pow<PREC>((float)(b / a), (float)t);
 }
 
 /* ------------------------------------------------------------------------- */
 /* emissivity, opacity and optical depth (solver.tpp:1106-2455)               */
 
+template <int PREC>
 __device__ inline double gaussian(double inverse_width, double diff)
 {
     const double sqrt_exp = inverse_width * diff;
-    return inverse_width * MGX_INVERSE_SQRT_PI * (double)mgx_expf((float)(-sqrt_exp * sqrt_exp));
+    return inverse_width * MGX_INVERSE_SQRT_PI * (double)f_exp<PREC>((float)(-sqrt_exp * sqrt_exp));
 }
 
 __device__ inline double planck(double temp, double freq)
@@ -305,7 +330,7 @@ __device__ inline double4 lp(const KParams &kp, uint32_t p, uint32_t l)
 }
 
 /* get_eta_and_chi_interpolated<approx> (solver.tpp:1177-1202, 1260-1302, 1345-1351) */
-template <int APPROX>
+template <int APPROX, int PREC>
 __device__ void eta_chi_interp(const KParams &kp, uint32_t p1, uint32_t p2, double t, uint32_t ll,
                                double freq, double &eta, double &chi)
 {
@@ -313,9 +338,9 @@ __device__ void eta_chi_interp(const KParams &kp, uint32_t p1, uint32_t p2, doub
     if (APPROX == MGX_APPROX_ONE_LINE) {
         const double4 a = lp(kp, p1, ll), b = lp(kp, p2, ll);
         const double diff = freq - m.line[ll];
-        const double prof = gaussian(interp_linear(a.x, b.x, t), diff);
-        eta = prof * interp_log(a.y, b.y, t);
-        chi = prof * interp_log(a.z, b.z, t) + kp.op.min_opacity;
+        const double prof = gaussian<PREC>(interp_linear(a.x, b.x, t), diff);
+        eta = prof * interp_log<PREC>(a.y, b.y, t);
+        chi = prof * interp_log<PREC>(a.z, b.z, t) + kp.op.min_opacity;
         return;
     }
     eta = 0.0;
@@ -337,27 +362,29 @@ __device__ void eta_chi_interp(const KParams &kp, uint32_t p1, uint32_t p2, doub
             diff = freq - m.line[l];
         }
         const double4 a = lp(kp, p1, l), b = lp(kp, p2, l);
-        const double prof = gaussian(interp_linear(a.x, b.x, t), diff);
-        eta += prof * interp_log(a.y, b.y, t);
-        chi += prof * interp_log(a.z, b.z, t);
+        const double prof = gaussian<PREC>(interp_linear(a.x, b.x, t), diff);
+        eta += prof * interp_log<PREC>(a.y, b.y, t);
+        chi += prof * interp_log<PREC>(a.z, b.z, t);
     }
 }
 
 /* compute_dtau_single_line (solver.tpp:2374-2455) */
-__device__ double dtau_single_line(const KParams &kp, uint32_t ci, uint32_t ni, uint32_t cii,
-                                   uint32_t nii, uint32_t l, double curr_freq, double next_freq,
-                                   double curr_interp, double next_interp, double dz)
+template <int PREC>
+__device__ double dtau_single_line(const KParams &kp, const double *erf_tab, uint32_t ci,
+                                   uint32_t ni, uint32_t cii, uint32_t nii, uint32_t l,
+                                   double curr_freq, double next_freq, double curr_interp,
+                                   double next_interp, double dz)
 {
     const double4 c = lp(kp, ci, l), cI = lp(kp, cii, l), n = lp(kp, ni, l), nI = lp(kp, nii, l);
     const double linefreq = kp.m.line[l];
     const double iw_c = interp_linear(c.x, cI.x, curr_interp);
     const double iw_n = interp_linear(n.x, nI.x, next_interp);
     const double avg_iw = (iw_c + iw_n) / 2.0;
-    const double op_c = interp_log(c.z, cI.z, curr_interp);
-    const double op_n = interp_log(n.z, nI.z, next_interp);
+    const double op_c = interp_log<PREC>(c.z, cI.z, curr_interp);
+    const double op_n = interp_log<PREC>(n.z, nI.z, next_interp);
     if (curr_freq == next_freq) {
         const double diff = curr_freq - linefreq;
-        const double prof = gaussian(avg_iw, diff);
+        const double prof = gaussian<PREC>(avg_iw, diff);
         const double avg_op = (op_c + op_n) / 2.0;
         return dz * (prof * avg_op + kp.op.min_opacity);
     }
@@ -365,24 +392,24 @@ __device__ double dtau_single_line(const KParams &kp, uint32_t ci, uint32_t ni, 
     const double curr_pos = (linefreq - curr_freq) * avg_iw;
     const double diff_pos = next_pos - curr_pos;
     const double avg_op = (op_n + op_c) / 2.0;
-    const float erf_diff = mgx_erff((float)next_pos) - mgx_erff((float)curr_pos);
+    const float erf_diff = f_erf<PREC>((float)next_pos, erf_tab) - f_erf<PREC>((float)curr_pos, erf_tab);
     const double erfterm = avg_op / diff_pos / 2.0 * (double)erf_diff;
     return dz * (avg_iw * erfterm + kp.op.min_opacity);
 }
 
 /* compute_S_dtau_line_integrated<approx> (solver.tpp:1879-2111) */
-template <int APPROX>
-__device__ void S_dtau_line_integrated(const KParams &kp, uint32_t cp, uint32_t np, uint32_t cpi,
+template <int APPROX, int PREC>
+__device__ void S_dtau_line_integrated(const KParams &kp, const double *erf_tab, uint32_t cp, uint32_t np, uint32_t cpi,
                                        uint32_t npi, uint32_t lineidx, double currfreq,
                                        double nextfreq, double curr_interp, double next_interp,
                                        double dZ, double &dtau, double &Scurr, double &Snext)
 {
     const mgx_model &m = kp.m;
     if (APPROX == MGX_APPROX_ONE_LINE) {
-        dtau = dtau_single_line(kp, cp, np, cpi, npi, lineidx, currfreq, nextfreq, curr_interp,
-                                next_interp, dZ);
-        Scurr = interp_log(lp(kp, cp, lineidx).w, lp(kp, cpi, lineidx).w, curr_interp);
-        Snext = interp_log(lp(kp, np, lineidx).w, lp(kp, npi, lineidx).w, next_interp);
+        dtau = dtau_single_line<PREC>(kp, erf_tab, cp, np, cpi, npi, lineidx, currfreq, nextfreq,
+                                      curr_interp, next_interp, dZ);
+        Scurr = interp_log<PREC>(lp(kp, cp, lineidx).w, lp(kp, cpi, lineidx).w, curr_interp);
+        Snext = interp_log<PREC>(lp(kp, np, lineidx).w, lp(kp, npi, lineidx).w, next_interp);
         return;
     }
     double sum_dtau = 0.0, sum_Sc = 0.0, sum_Sn = 0.0;
@@ -404,10 +431,11 @@ __device__ void S_dtau_line_integrated(const KParams &kp, uint32_t cp, uint32_t 
     }
     for (uint32_t s = lo; s < hi; s++) {
         const uint32_t l = (APPROX == MGX_APPROX_CLOSE_LINES) ? m.sorted_line_map[s] : s;
-        const double line_dtau = dtau_single_line(kp, cp, np, cpi, npi, l, currfreq, nextfreq,
-                                                  curr_interp, next_interp, dZ);
-        const double Sc = interp_log(lp(kp, cp, l).w, lp(kp, cpi, l).w, curr_interp);
-        const double Sn = interp_log(lp(kp, np, l).w, lp(kp, npi, l).w, next_interp);
+        const double line_dtau = dtau_single_line<PREC>(kp, erf_tab, cp, np, cpi, npi, l,
+                                                        currfreq, nextfreq, curr_interp,
+                                                        next_interp, dZ);
+        const double Sc = interp_log<PREC>(lp(kp, cp, l).w, lp(kp, cpi, l).w, curr_interp);
+        const double Sn = interp_log<PREC>(lp(kp, np, l).w, lp(kp, npi, l).w, next_interp);
         sum_dtau += line_dtau;
         sum_Sc += line_dtau * Sc;
         sum_Sn += line_dtau * Sn;
@@ -442,8 +470,8 @@ struct PointEC {
  * entries cu (closer to the rr-side boundary) and nx.  Returns dtau and the
  * source values of the two rows; uses/updates the caches of the two entries.
  */
-template <int APPROX>
-__device__ inline void segment(const KParams &kp, const Entry &cu, const Entry &nx, double dZ,
+template <int APPROX, int PREC>
+__device__ inline void segment(const KParams &kp, const double *erf_tab, const Entry &cu, const Entry &nx, double dZ,
                                double freq, uint32_t line, PointEC &ec_cu, PointEC &ec_nx,
                                double &dtau, double &S_cu, double &S_nx)
 {
@@ -453,15 +481,15 @@ __device__ inline void segment(const KParams &kp, const Entry &cu, const Entry &
     const double dshift_max = (b < a) ? b : a;
     const double cf = freq * cu.shift, nf = freq * nx.shift;
     if (dshift_abs > dshift_max) {
-        S_dtau_line_integrated<APPROX>(kp, cu.nr, nx.nr, cu.nri, nx.nri, line, cf, nf, cu.ifac,
-                                       nx.ifac, dZ, dtau, S_cu, S_nx);
+        S_dtau_line_integrated<APPROX, PREC>(kp, erf_tab, cu.nr, nx.nr, cu.nri, nx.nri, line, cf,
+                                             nf, cu.ifac, nx.ifac, dZ, dtau, S_cu, S_nx);
     } else {
         if (!ec_cu.valid) {
-            eta_chi_interp<APPROX>(kp, cu.nr, cu.nri, cu.ifac, line, cf, ec_cu.eta, ec_cu.chi);
+            eta_chi_interp<APPROX, PREC>(kp, cu.nr, cu.nri, cu.ifac, line, cf, ec_cu.eta, ec_cu.chi);
             ec_cu.valid = true;
         }
         if (!ec_nx.valid) {
-            eta_chi_interp<APPROX>(kp, nx.nr, nx.nri, nx.ifac, line, nf, ec_nx.eta, ec_nx.chi);
+            eta_chi_interp<APPROX, PREC>(kp, nx.nr, nx.nri, nx.ifac, line, nf, ec_nx.eta, ec_nx.chi);
             ec_nx.valid = true;
         }
         S_cu = ec_cu.eta / ec_cu.chi;
@@ -539,11 +567,16 @@ __device__ inline double hi_advance(const KParams &kp, HalfIter &it, uint32_t l,
 /* ------------------------------------------------------------------------- */
 /* the forward-solve kernel                                                   */
 
-template <int APPROX>
+template <int APPROX, int PREC>
 __global__ __launch_bounds__(MGX_WAVE) void feautrier_kernel(KParams kp, double *u, double *J,
                                                              double *lam)
 {
     extern __shared__ unsigned char smem[];
+    __shared__ double s_erf[MGX_ERF_NINT * (MGX_ERF_DEG + 1)];
+    if (PREC == MGX_PREC_FP64) {
+        for (uint32_t i = threadIdx.x; i < MGX_ERF_NINT * (MGX_ERF_DEG + 1); i += MGX_WAVE)
+            s_erf[i] = MGX_ERF_TAB[i];
+    }
     const mgx_model &m = kp.m;
     const uint32_t o = blockIdx.x;
     const uint32_t lane = threadIdx.x;
@@ -605,8 +638,8 @@ __global__ __launch_bounds__(MGX_WAVE) void feautrier_kernel(KParams kp, double 
                             const Entry inner = hi_entry(it, o);
                             PointEC ec_in = {0.0, 0.0, false};
                             double dtau, S_in, S_out;
-                            segment<APPROX>(kp, inner, outer, dz, freq, l, ec_in, ec_out, dtau,
-                                            S_in, S_out);
+                            segment<APPROX, PREC>(kp, s_erf, inner, outer, dz, freq, l, ec_in,
+                                                  ec_out, dtau, S_in, S_out);
                             if (is_last) {
                                 const double I_l = boundary_intensity(m, outer.nr, freq * outer.shift);
                                 const double idt = 1.0 / dtau;
@@ -653,8 +686,8 @@ __global__ __launch_bounds__(MGX_WAVE) void feautrier_kernel(KParams kp, double 
                             const Entry inner = hi_entry(it, o);
                             PointEC ec_in = (it.s < 0) ? ec_cent : PointEC{0.0, 0.0, false};
                             double dtau, S_out, S_in;
-                            segment<APPROX>(kp, outer, inner, dz, freq, l, ec_out, ec_in, dtau,
-                                            S_out, S_in);
+                            segment<APPROX, PREC>(kp, s_erf, outer, inner, dz, freq, l, ec_out,
+                                                  ec_in, dtau, S_out, S_in);
                             if (is_first) {
                                 const double I_f = boundary_intensity(m, outer.nr, freq * outer.shift);
                                 const double idt = 1.0 / dtau;
@@ -729,7 +762,7 @@ __global__ __launch_bounds__(MGX_WAVE) void feautrier_kernel(KParams kp, double 
                         const double constante =
                             m.line_A[l] * m.quad_weights[(size_t)spec * Q + m.f_z[f]] * w_ang;
                         double eta, chi;
-                        eta_chi_interp<APPROX>(kp, o, o, 0.0, k, freq_line, eta, chi);
+                        eta_chi_interp<APPROX, PREC>(kp, o, o, 0.0, k, freq_line, eta, chi);
                         const double inverse_chi = 1.0 / chi;
                         const double frq = freq * 1.0;
                         const double inv_mass = m.species_inverse_mass[spec];
@@ -741,7 +774,7 @@ __global__ __launch_bounds__(MGX_WAVE) void feautrier_kernel(KParams kp, double 
                         const double se = inverse_width * (frq - freq_line);
                         const double ex = se * se;
                         const double phi = inverse_width * MGX_INVERSE_SQRT_PI *
-                                           (double)mgx_expf((float)(-ex));
+                                           (double)f_exp<PREC>((float)(-ex));
                         Lval = constante * frq * phi * Lc * inverse_chi;
                     }
                 }
@@ -1128,8 +1161,6 @@ int mgx_feautrier_forward_device(const mgx_model *model, const mgx_options *opts
     mgx::clear_error();
     if (int e = check_view(model, opts)) return e;
     if (!J) return mgx::fail(MGX_E_ARG, "J is required");
-    if (opts->precision != MGX_PREC_FP64)
-        return mgx::fail(MGX_E_UNSUPPORTED, "mixed precision is not built yet");
     hipStream_t st = (hipStream_t)stream;
     DevBuffers b;
     KParams kp;
@@ -1137,19 +1168,32 @@ int mgx_feautrier_forward_device(const mgx_model *model, const mgx_options *opts
     return launch_with_capacity(st, b, kp, [&](size_t lds) {
         const dim3 grid(model->npoints), block(MGX_WAVE);
         timed(st, [&] {
-        switch (opts->approx) {
-        case MGX_APPROX_NONE:
-            hipLaunchKernelGGL(feautrier_kernel<MGX_APPROX_NONE>, grid, block, lds, st, kp, u, J,
-                               lambda_diag);
-            break;
-        case MGX_APPROX_ONE_LINE:
-            hipLaunchKernelGGL(feautrier_kernel<MGX_APPROX_ONE_LINE>, grid, block, lds, st, kp, u,
-                               J, lambda_diag);
-            break;
-        default:
-            hipLaunchKernelGGL(feautrier_kernel<MGX_APPROX_CLOSE_LINES>, grid, block, lds, st, kp,
-                               u, J, lambda_diag);
-        }
+            const bool fp64 = opts->precision == MGX_PREC_FP64;
+            switch (opts->approx) {
+            case MGX_APPROX_NONE:
+                if (fp64)
+                    hipLaunchKernelGGL((feautrier_kernel<MGX_APPROX_NONE, MGX_PREC_FP64>), grid,
+                                       block, lds, st, kp, u, J, lambda_diag);
+                else
+                    hipLaunchKernelGGL((feautrier_kernel<MGX_APPROX_NONE, MGX_PREC_MIXED>), grid,
+                                       block, lds, st, kp, u, J, lambda_diag);
+                break;
+            case MGX_APPROX_ONE_LINE:
+                if (fp64)
+                    hipLaunchKernelGGL((feautrier_kernel<MGX_APPROX_ONE_LINE, MGX_PREC_FP64>), grid,
+                                       block, lds, st, kp, u, J, lambda_diag);
+                else
+                    hipLaunchKernelGGL((feautrier_kernel<MGX_APPROX_ONE_LINE, MGX_PREC_MIXED>),
+                                       grid, block, lds, st, kp, u, J, lambda_diag);
+                break;
+            default:
+                if (fp64)
+                    hipLaunchKernelGGL((feautrier_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_FP64>),
+                                       grid, block, lds, st, kp, u, J, lambda_diag);
+                else
+                    hipLaunchKernelGGL((feautrier_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_MIXED>),
+                                       grid, block, lds, st, kp, u, J, lambda_diag);
+            }
         });
     });
 }

@@ -94,3 +94,39 @@ def test_forward_fp64_parity(which, approx, request):
     eu, eJ, eL = _rel_err(u, ur), _rel_err(J, Jr), _rel_err(lam, lr)
     print(f"  max rel err u={eu:.3e} J={eJ:.3e} lambda={eL:.3e}", flush=True)
     assert eu < RTOL and eJ < RTOL and eL < RTOL
+
+
+# Mixed precision (single-precision profile, erf and log-interpolation
+# transcendentals; FP64 walk, interpolation counts and elimination) against the
+# FP64 oracle.  Bars: J and Lambda within 1e-5 relative (the metric's J bar),
+# u within 1e-4 relative.
+MIXED_J_RTOL = 1e-5
+MIXED_U_RTOL = 1e-4
+
+
+@pytest.mark.parametrize("which", ["line_model", "small_wind", "wind32"])
+def test_forward_mixed_accuracy(which, request):
+    from magritte_amd import solver
+    from oracle import oracle
+
+    m = request.getfixturevalue(which)
+    op = m.options(precision=_abi.PREC_MIXED)
+    print(f"[forward mixed] model={which}", flush=True)
+    out = solver.feautrier_forward(_dev(m), op)
+    u = out["u"].cpu().numpy()
+    J = out["J"].cpu().numpy()
+    lam = out["lambda_diag"].cpu().numpy()
+    ur, Jr, lr = oracle.solve(m, m.options())
+    eu, eJ, eL = _rel_err(u, ur), _rel_err(J, Jr), _rel_err(lam, lr)
+    print(f"  mixed max rel err u={eu:.3e} J={eJ:.3e} lambda={eL:.3e}", flush=True)
+    assert eJ < MIXED_J_RTOL and eL < MIXED_J_RTOL and eu < MIXED_U_RTOL
+
+
+def test_mixed_mode_keeps_traces(small_wind):
+    from magritte_amd import solver
+
+    m = small_wind
+    dm = _dev(m)
+    a = solver.trace_counts(dm, m.options(precision=_abi.PREC_FP64))
+    b = solver.trace_counts(dm, m.options(precision=_abi.PREC_MIXED))
+    np.testing.assert_array_equal(a, b)
