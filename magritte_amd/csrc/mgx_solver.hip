@@ -247,11 +247,14 @@ __device__ inline float f_pow(float x, float y)
     return __powf(x, y);
 }
 
+/* erf stays correctly rounded in both modes: the reference takes the
+   difference of two float erf values (compute_dtau_single_line), which cancels
+   in the line wings, so a one-ulp change of either value moves u by up to
+   ~1e-3 relative there */
 template <int PREC>
 __device__ inline float f_erf(float x, const double *tab)
 {
-    if (PREC == MGX_PREC_FP64) return mgx_erff_tab(x, tab);
-    return erff(x);
+    return mgx_erff_tab(x, tab);
 }
 
 __device__ inline double interp_linear(double a, double b, double t)
@@ -573,10 +576,8 @@ __global__ __launch_bounds__(MGX_WAVE) void feautrier_kernel(KParams kp, double 
 {
     extern __shared__ unsigned char smem[];
     __shared__ double s_erf[MGX_ERF_NINT * (MGX_ERF_DEG + 1)];
-    if (PREC == MGX_PREC_FP64) {
-        for (uint32_t i = threadIdx.x; i < MGX_ERF_NINT * (MGX_ERF_DEG + 1); i += MGX_WAVE)
-            s_erf[i] = MGX_ERF_TAB[i];
-    }
+    for (uint32_t i = threadIdx.x; i < MGX_ERF_NINT * (MGX_ERF_DEG + 1); i += MGX_WAVE)
+        s_erf[i] = MGX_ERF_TAB[i];
     const mgx_model &m = kp.m;
     const uint32_t o = blockIdx.x;
     const uint32_t lane = threadIdx.x;

@@ -484,3 +484,65 @@ def model_constant_slab(npoints=50, dx=1.0e12, n_species=1.0e3, n_h2=1.0e12, tem
     ld = dict(ld, num=0)
     m.set_lines([ld], nquads)
     return m.finalize()
+
+
+def rays_spherical_symmetry(nrays):
+    """Uniform ray set of a 1D spherically symmetric model (setup.py:226-339, uniform=True):
+    hnrays directions from +x to +y in the xy-plane, weights from the covered solid angle."""
+    nextra = nrays // 2 - 1
+    Rx, Ry, Rz = [1.0], [0.0], [0.0]
+    angle_max = np.arctan(Ry[-1] / Rx[-1])
+    angle_extra = (0.5 * np.pi - angle_max) / nextra
+    for k in range(1, nextra):
+        Rx.append(np.cos(angle_max + k * angle_extra))
+        Ry.append(np.sin(angle_max + k * angle_extra))
+        Rz.append(0.0)
+    Rx.append(0.0)
+    Ry.append(1.0)
+    Rz.append(0.0)
+    n = 2 * len(Rx)
+    Wt = []
+    for i in range(n // 2):
+        if i == 0:
+            ux, uy = 0.5 * (Rx[i] + Rx[i + 1]), 0.5 * (Ry[i] + Ry[i + 1])
+            lx, ly = Rx[0], Ry[0]
+        elif i == n // 2 - 1:
+            ux, uy = Rx[-1], Ry[-1]
+            lx, ly = 0.5 * (Rx[i] + Rx[i - 1]), 0.5 * (Ry[i] + Ry[i - 1])
+        else:
+            ux, uy = 0.5 * (Rx[i] + Rx[i + 1]), 0.5 * (Ry[i] + Ry[i + 1])
+            lx, ly = 0.5 * (Rx[i] + Rx[i - 1]), 0.5 * (Ry[i] + Ry[i - 1])
+        Wt.append(lx / np.sqrt(lx**2 + ly**2) - ux / np.sqrt(ux**2 + uy**2))
+    for i in range(n // 2):
+        Rx.append(-Rx[i])
+        Ry.append(-Ry[i])
+        Rz.append(-Rz[i])
+        Wt.append(Wt[i])
+    d = np.array((Rx, Ry, Rz)).T
+    w = np.array(Wt)
+    return np.ascontiguousarray(d), w / np.sum(w)
+
+
+def model_velocity_gradient_1d(linedata, npoints=100, nrays=100, nquads=100, dx=1.0e4,
+                               dv=2.5e2, n_species=1.0e8, n_h2=1.0e12, temperature=45.0,
+                               turb=0.0, max_width_fraction=0.35):
+    """The reference's constant-velocity-gradient 1D spherical benchmark
+    (tests/benchmarks/analytic/constant_velocity_gradient_1D.py:18-77)."""
+    P = npoints
+    pos = np.zeros((P, 3))
+    pos[:, 0] = (np.arange(P) + 1) * dx
+    vel = np.zeros((P, 3))
+    vel[:, 0] = (np.arange(P) + 1) * (dv / CC)
+    nbs = [[1]] + [[p - 1, p + 1] for p in range(1, P - 1)] + [[P - 2]]
+    m = ModelData()
+    m.dimension = 1
+    m.spherical_symmetry = 1
+    m.params["max_width_fraction"] = max_width_fraction
+    d, w = rays_spherical_symmetry(nrays)
+    m.set_geometry(pos, vel, nbs, [0, P - 1], d, w)
+    abund = np.zeros((P, 3))
+    abund[:, 0] = n_species
+    abund[:, 1] = n_h2
+    m.set_thermo(np.full(P, temperature), np.full(P, turb), abund)
+    m.set_lines([dict(linedata, num=0)], nquads)
+    return m.finalize()

@@ -40,3 +40,21 @@ def wind32():
     from magritte_amd import models
 
     return models.model_wind(n_edge=32)
+
+
+@pytest.fixture(scope="session")
+def vgrad_model():
+    """1D spherical velocity-gradient benchmark model; max_width_fraction 0.10
+    sends every segment through the erf (line-integrated) branch."""
+    from magritte_amd import models
+
+    ld = models.read_lamda(os.path.join(ROOT, "tests", "golden", "lamda_test.txt"), num=0)
+    return models.model_velocity_gradient_1d(ld, max_width_fraction=0.10)
+
+
+@pytest.fixture(scope="session")
+def slab_model():
+    from magritte_amd import models
+
+    ld = models.read_lamda(os.path.join(ROOT, "tests", "golden", "lamda_test.txt"), num=0)
+    return models.model_constant_slab(linedata=ld)

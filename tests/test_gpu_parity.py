@@ -27,7 +27,7 @@ def _rel_err(a, b):
     return float(np.max(np.abs(a - b) / den))
 
 
-@pytest.mark.parametrize("which", ["line_model", "small_wind", "wind32"])
+@pytest.mark.parametrize("which", ["line_model", "vgrad_model", "small_wind", "wind32"])
 def test_trace_counts_bit_exact(which, request):
     from magritte_amd import solver
     from oracle import oracle
@@ -42,7 +42,7 @@ def test_trace_counts_bit_exact(which, request):
     assert len(bad) == 0, f"{len(bad)} mismatching counts, first {bad[:5]} got {got[tuple(bad[0])]} ref {ref[tuple(bad[0])]}"
 
 
-@pytest.mark.parametrize("which", ["line_model", "small_wind", "wind32"])
+@pytest.mark.parametrize("which", ["line_model", "vgrad_model", "small_wind", "wind32"])
 def test_trace_rays_bit_exact(which, request):
     from magritte_amd import solver
     from oracle import oracle
@@ -73,6 +73,9 @@ def test_trace_rays_bit_exact(which, request):
     ("line_model", _abi.APPROX_CLOSE_LINES),
     ("line_model", _abi.APPROX_NONE),
     ("line_model", _abi.APPROX_ONE_LINE),
+    ("slab_model", _abi.APPROX_CLOSE_LINES),
+    ("vgrad_model", _abi.APPROX_CLOSE_LINES),
+    ("vgrad_model", _abi.APPROX_NONE),
     ("small_wind", _abi.APPROX_CLOSE_LINES),
     ("small_wind", _abi.APPROX_NONE),
     ("small_wind", _abi.APPROX_ONE_LINE),
@@ -104,7 +107,7 @@ MIXED_J_RTOL = 1e-5
 MIXED_U_RTOL = 1e-4
 
 
-@pytest.mark.parametrize("which", ["line_model", "small_wind", "wind32"])
+@pytest.mark.parametrize("which", ["line_model", "vgrad_model", "small_wind", "wind32"])
 def test_forward_mixed_accuracy(which, request):
     from magritte_amd import solver
     from oracle import oracle

@@ -173,3 +173,86 @@ def test_walk_invariants_and_interpolation_counts(small_wind):
             expect = max(1, math.ceil(lg / log14))
             assert cnt == expect, (rr, o, l, j, cnt, expect)
             j = k
+
+
+def _velocity_gradient_closed_form(m, ld, temp, n_species, dx, dv, P):
+    """u(ray, r, nu) of the reference's constant-velocity-gradient benchmark
+    (constant_velocity_gradient_1D.py:130-177)."""
+    from scipy.special import erf
+
+    k = 0
+    frq = ld["frequency"][k]
+    w = ld["weight"] * np.exp(-ld["energy"] / (models.KB * temp))
+    pop = w / w.sum() * n_species
+    hv = models.HH * frq / (4.0 * math.pi)
+    chi = hv * (pop[0] * ld["Ba"][0] - pop[1] * ld["Bs"][0])
+    eta = hv * ld["A"][0] * pop[1]
+    src = eta / chi
+    dnu = frq * math.sqrt(2.0 * models.KB * temp / (models.AMU * models.CC**2) * ld["inverse_mass"])
+    rs = m.position[:, 0]
+    r_in, r_out = rs[0], rs[-1]
+    L = dx * P
+    vmax = dv / models.CC * P
+
+    def v_(r):
+        return dv / models.CC * (1 + (r - r_in) / dx)
+
+    def I_cmb(nu):
+        return 2.0 * models.HH * nu**3 / models.CC**2 / math.expm1(models.HH * nu / (models.KB * models.T_CMB))
+
+    def bdy(nu, r, th):
+        r_shift = math.cos(th) * v_(r)
+        if th < math.asin(r_in / r):
+            e = math.sqrt(1 - ((r / r_in) * abs(math.sin(th))) ** 2) * v_(r_in)
+        else:
+            e = math.sqrt(1 - ((r / r_out) * abs(math.sin(th))) ** 2) * v_(r_out)
+        return I_cmb(nu * (1.0 + (e - r_shift)))
+
+    def z_max(r, th):
+        if th < math.asin(r_in / r):
+            return r * math.cos(th) - math.sqrt(r_in**2 - (r * math.sin(th)) ** 2)
+        return r * math.cos(th) + math.sqrt(L**2 - (r * math.sin(th)) ** 2)
+
+    def tau(nu, r, th):
+        l = z_max(r, th)
+        arg = (nu - frq) / dnu
+        fct = vmax * nu / dnu
+        return chi * L / (fct * dnu) * 0.5 * (erf(-arg) + erf(fct * l / L + arg))
+
+    def I_(nu, r, th):
+        return src + (bdy(nu, r, th) - src) * math.exp(-tau(nu, r, th))
+
+    angles = np.arctan2(m.ray_direction[: m.hnrays, 1], m.ray_direction[: m.hnrays, 0])
+    nu = m.nu[0]
+    return np.array([[[0.5 * (I_(f, r, a) + I_(f, r, np.pi + a)) for f in nu] for r in rs]
+                     for a in angles])
+
+
+@pytest.mark.parametrize("mwf,bound", [(0.5, 1e-6), (0.35, 5e-7), (0.10, 5.31e-8)])
+def test_velocity_gradient_erf_branch_closed_form(mwf, bound):
+    """Pins the line-integrated (erf) optical depth and the spherical tracer:
+    mean error within the reference's bounds for max_width_fraction 0.5 / 0.35 /
+    0.10 (constant_velocity_gradient_1D.py:91-95, 249-256).  At 0.10 every
+    segment takes the erf branch."""
+    ld = models.read_lamda(os.path.join(HERE, "golden", "lamda_test.txt"), num=0)
+    m = models.model_velocity_gradient_1d(ld, max_width_fraction=mwf)
+    op = m.options()
+    u, _, _ = oracle.solve(m, op)
+    us = _velocity_gradient_closed_form(m, ld, 45.0, 1.0e8, 1.0e4, 2.5e2, m.npoints)
+    err = np.abs(rel(us, u))
+    assert np.mean(err) < bound, np.mean(err)
+
+
+def test_reference_is_sensitive_to_the_erff_rounding(small_wind):
+    """Documents why the kernels keep the correctly rounded erf in every
+    precision mode: the reference forms erf(b) - erf(a) in float, so the same
+    solve with glibc's erff (not correctly rounded) instead of the correctly
+    rounded one moves J by ~1e-4 relative, while the traced counts do not move."""
+    m = small_wind
+    op = m.options()
+    _, J1, _ = oracle.solve(m, op, with_u=False, kind="cr")
+    _, J2, _ = oracle.solve(m, op, with_u=False, kind="glibc")
+    d = np.max(np.abs(J2 - J1) / np.abs(J1))
+    assert 1e-6 < d < 1e-2, d
+    np.testing.assert_array_equal(oracle.trace_counts(m, op, kind="cr"),
+                                  oracle.trace_counts(m, op, kind="glibc"))
