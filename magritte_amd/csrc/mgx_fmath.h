@@ -8,9 +8,11 @@
  * traced point counts (through logf) and every opacity, source function and
  * optical depth.  To let the CPU restatement and the gfx950 kernels produce the
  * same bits, both evaluate the functions below: each widens its float
- * argument to double, evaluates a fixed polynomial with plain IEEE double
- * operations in a fixed order (the file must be compiled with
- * -ffp-contract=off, so no FMA is formed), and rounds once to float.
+ * argument to double, evaluates a fixed polynomial in a fixed order with
+ * correctly rounded IEEE operations -- explicit fused multiply-adds (fma() of
+ * the C library on the host, v_fma_f64 on the device) and plain operations
+ * elsewhere (the file must be compiled with -ffp-contract=off, so the compiler
+ * forms no other FMA) -- and rounds once to float.
  *
  * Accuracy: the double results are within a few units in the last place of
  * double, so the float result is the correctly rounded one except when the
@@ -21,7 +23,10 @@
  */
 #pragma once
 
+#include <math.h>
 #include <stdint.h>
+
+#define MGX_FMA(a, b, c) fma((a), (b), (c))
 
 #if defined(__HIPCC__)
 #define MGX_FM_HD __host__ __device__ static inline
@@ -71,9 +76,9 @@ MGX_FM_HD double mgx_exp_d(double x)
     if (x > 709.0) return 1.0 / 0.0;
     if (x < -708.0) return 0.0;
     const double kd = mgx_fm_rint(x * MGX_LOG2E);
-    const double r = (x - kd * MGX_LN2_HI) - kd * MGX_LN2_LO;
+    const double r = MGX_FMA(-kd, MGX_LN2_LO, MGX_FMA(-kd, MGX_LN2_HI, x));
     double p = MGX_EXP_C[13];
-    for (int n = 12; n >= 0; n--) p = p * r + MGX_EXP_C[n];
+    for (int n = 12; n >= 0; n--) p = MGX_FMA(p, r, MGX_EXP_C[n]);
     /* p in [0.70, 1.42]; the product stays normal for the clamped range */
     const int k = (int)kd;
     if (k < -1000) return (p * mgx_fm_pow2(k + 200)) * mgx_fm_pow2(-200);
@@ -103,9 +108,9 @@ MGX_FM_HD double mgx_log_d(double x)
     const double s = (m - 1.0) / (m + 1.0);
     const double s2 = s * s;
     double p = MGX_LOG_C[11];
-    for (int n = 10; n >= 0; n--) p = p * s2 + MGX_LOG_C[n];
+    for (int n = 10; n >= 0; n--) p = MGX_FMA(p, s2, MGX_LOG_C[n]);
     const double ed = (double)e;
-    return ed * MGX_LN2_HI + (ed * MGX_LN2_LO + s * p);
+    return MGX_FMA(ed, MGX_LN2_HI, MGX_FMA(ed, MGX_LN2_LO, s * p));
 }
 
 MGX_FM_HD float mgx_expf(float x) { return (float)mgx_exp_d((double)x); }
@@ -118,7 +123,7 @@ MGX_FM_HD float mgx_expm1f(float x)
     if (xd != xd) return x;
     if (xd > -0.5 && xd < 0.5) {
         double p = MGX_EXPM1_C[17];
-        for (int n = 16; n >= 0; n--) p = p * xd + MGX_EXPM1_C[n];
+        for (int n = 16; n >= 0; n--) p = MGX_FMA(p, xd, MGX_EXPM1_C[n]);
         return (float)(p * xd);
     }
     return (float)(mgx_exp_d(xd) - 1.0);
@@ -145,14 +150,14 @@ MGX_FM_HD float mgx_erff_tab(float x, const double *tab)
     if (a < 0.25) {
         const double a2 = a * a;
         double p = MGX_ERF0_C[11];
-        for (int n = 10; n >= 0; n--) p = p * a2 + MGX_ERF0_C[n];
+        for (int n = 10; n >= 0; n--) p = MGX_FMA(p, a2, MGX_ERF0_C[n]);
         r = a * p;
     } else if (a < 4.0) {
-        const int j = (int)(a * 4.0); /* 1..15 */
-        const double t = a - ((double)j * 0.25 + 0.125);
-        const double *c = &tab[(j - 1) * (MGX_ERF_DEG + 1)];
+        const int j = (int)(a * 8.0); /* 2..31 */
+        const double t = a - ((double)j * 0.125 + 0.0625);
+        const double *c = &tab[(j - 2) * (MGX_ERF_DEG + 1)];
         double p = c[MGX_ERF_DEG];
-        for (int n = MGX_ERF_DEG - 1; n >= 0; n--) p = p * t + c[n];
+        for (int n = MGX_ERF_DEG - 1; n >= 0; n--) p = MGX_FMA(p, t, c[n]);
         r = p;
     } else {
         r = 1.0;

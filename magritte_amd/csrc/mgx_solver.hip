@@ -41,6 +41,10 @@
 #include "mgx_fmath.h"
 
 #define MGX_WAVE 64
+/* minimum waves per SIMD requested for the solve kernel (register budget) */
+#ifndef MGX_SOLVE_WAVES_PER_SIMD
+#define MGX_SOLVE_WAVES_PER_SIMD 2
+#endif
 
 namespace {
 
@@ -275,6 +279,18 @@ __device__ inline double interp_log(double a, double b, double t)
 /* ------------------------------------------------------------------------- */
 /* emissivity, opacity and optical depth (solver.tpp:1106-2455)               */
 
+/* 1/x to within one unit in the last place for finite normal x (v_rcp_f64 and
+   two Newton steps); used only where the result does not feed a float
+   conversion, so it cannot change which float argument a transcendental sees */
+__device__ inline double rcp(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
 template <int PREC>
 __device__ inline double gaussian(double inverse_width, double diff)
 {
@@ -327,6 +343,38 @@ __device__ inline uint32_t upper_bound_d(const double *a, uint32_t n, double v)
     return lo;
 }
 
+/*
+ * The lane's own line in the sorted line list: its position g and the
+ * neighbouring frequencies.  A CloseLines window [left, right] almost always
+ * starts and ends at g; checking that against a[g-1], a[g], a[g+1] gives the
+ * same indices as std::lower_bound / std::upper_bound without a search.
+ */
+struct OwnLine {
+    uint32_t g;
+    double a_lo, a_g, a_hi;
+};
+
+__device__ inline OwnLine own_line(const mgx_model &m, uint32_t l)
+{
+    OwnLine r;
+    uint32_t g = 0;
+    for (uint32_t s = 0; s < m.nlines; s++)
+        if (m.sorted_line_map[s] == l) g = s;
+    r.g = g;
+    r.a_g = m.sorted_line[g];
+    r.a_lo = (g > 0) ? m.sorted_line[g - 1] : -__builtin_inf();
+    r.a_hi = (g + 1 < m.nlines) ? m.sorted_line[g + 1] : __builtin_inf();
+    return r;
+}
+
+__device__ inline void line_window(const mgx_model &m, const OwnLine &ol, double left, double right,
+                                   uint32_t &lo, uint32_t &hi)
+{
+    lo = (!(ol.a_g < left) && ol.a_lo < left) ? ol.g : lower_bound_d(m.sorted_line, m.nlines, left);
+    hi = (!(right < ol.a_g) && right < ol.a_hi) ? ol.g + 1
+                                                 : upper_bound_d(m.sorted_line, m.nlines, right);
+}
+
 __device__ inline double4 lp(const KParams &kp, uint32_t p, uint32_t l)
 {
     return kp.lpack[(size_t)p * kp.m.nlines + l];
@@ -334,8 +382,8 @@ __device__ inline double4 lp(const KParams &kp, uint32_t p, uint32_t l)
 
 /* get_eta_and_chi_interpolated<approx> (solver.tpp:1177-1202, 1260-1302, 1345-1351) */
 template <int APPROX, int PREC>
-__device__ void eta_chi_interp(const KParams &kp, uint32_t p1, uint32_t p2, double t, uint32_t ll,
-                               double freq, double &eta, double &chi)
+__device__ void eta_chi_interp(const KParams &kp, const OwnLine &ol, uint32_t p1, uint32_t p2,
+                               double t, uint32_t ll, double freq, double &eta, double &chi)
 {
     const mgx_model &m = kp.m;
     if (APPROX == MGX_APPROX_ONE_LINE) {
@@ -351,8 +399,7 @@ __device__ void eta_chi_interp(const KParams &kp, uint32_t p1, uint32_t p2, doub
     uint32_t lo = 0, hi = m.nlines;
     if (APPROX == MGX_APPROX_CLOSE_LINES) {
         const double W = kp.op.max_distance_opacity_contribution * (freq * kp.ub[p1]);
-        lo = lower_bound_d(m.sorted_line, m.nlines, freq - W);
-        hi = upper_bound_d(m.sorted_line, m.nlines, freq + W);
+        line_window(m, ol, freq - W, freq + W, lo, hi);
     }
     for (uint32_t s = lo; s < hi; s++) {
         uint32_t l;
@@ -371,15 +418,14 @@ __device__ void eta_chi_interp(const KParams &kp, uint32_t p1, uint32_t p2, doub
     }
 }
 
-/* compute_dtau_single_line (solver.tpp:2374-2455) */
+/* compute_dtau_single_line (solver.tpp:2374-2455).  Everything up to the
+   float conversions follows the reference's operation order. */
 template <int PREC>
-__device__ double dtau_single_line(const KParams &kp, const double *erf_tab, uint32_t ci,
-                                   uint32_t ni, uint32_t cii, uint32_t nii, uint32_t l,
-                                   double curr_freq, double next_freq, double curr_interp,
-                                   double next_interp, double dz)
+__device__ double dtau_single_line(const KParams &kp, const double *erf_tab, double4 c, double4 cI,
+                                   double4 n, double4 nI, double linefreq, double curr_freq,
+                                   double next_freq, double curr_interp, double next_interp,
+                                   double dz)
 {
-    const double4 c = lp(kp, ci, l), cI = lp(kp, cii, l), n = lp(kp, ni, l), nI = lp(kp, nii, l);
-    const double linefreq = kp.m.line[l];
     const double iw_c = interp_linear(c.x, cI.x, curr_interp);
     const double iw_n = interp_linear(n.x, nI.x, next_interp);
     const double avg_iw = (iw_c + iw_n) / 2.0;
@@ -396,23 +442,26 @@ __device__ double dtau_single_line(const KParams &kp, const double *erf_tab, uin
     const double diff_pos = next_pos - curr_pos;
     const double avg_op = (op_n + op_c) / 2.0;
     const float erf_diff = f_erf<PREC>((float)next_pos, erf_tab) - f_erf<PREC>((float)curr_pos, erf_tab);
-    const double erfterm = avg_op / diff_pos / 2.0 * (double)erf_diff;
-    return dz * (avg_iw * erfterm + kp.op.min_opacity);
+    const double erfterm = avg_op * (double)erf_diff * (0.5 * rcp(diff_pos));
+    return dz * fma(avg_iw, erfterm, kp.op.min_opacity);
 }
 
 /* compute_S_dtau_line_integrated<approx> (solver.tpp:1879-2111) */
 template <int APPROX, int PREC>
-__device__ void S_dtau_line_integrated(const KParams &kp, const double *erf_tab, uint32_t cp, uint32_t np, uint32_t cpi,
-                                       uint32_t npi, uint32_t lineidx, double currfreq,
-                                       double nextfreq, double curr_interp, double next_interp,
-                                       double dZ, double &dtau, double &Scurr, double &Snext)
+__device__ void S_dtau_line_integrated(const KParams &kp, const double *erf_tab, const OwnLine &ol,
+                                       uint32_t cp, uint32_t np, uint32_t cpi, uint32_t npi,
+                                       uint32_t lineidx, double currfreq, double nextfreq,
+                                       double curr_interp, double next_interp, double dZ,
+                                       double &dtau, double &Scurr, double &Snext)
 {
     const mgx_model &m = kp.m;
     if (APPROX == MGX_APPROX_ONE_LINE) {
-        dtau = dtau_single_line<PREC>(kp, erf_tab, cp, np, cpi, npi, lineidx, currfreq, nextfreq,
-                                      curr_interp, next_interp, dZ);
-        Scurr = interp_log<PREC>(lp(kp, cp, lineidx).w, lp(kp, cpi, lineidx).w, curr_interp);
-        Snext = interp_log<PREC>(lp(kp, np, lineidx).w, lp(kp, npi, lineidx).w, next_interp);
+        const double4 c = lp(kp, cp, lineidx), cI = lp(kp, cpi, lineidx);
+        const double4 n = lp(kp, np, lineidx), nI = lp(kp, npi, lineidx);
+        dtau = dtau_single_line<PREC>(kp, erf_tab, c, cI, n, nI, m.line[lineidx], currfreq,
+                                      nextfreq, curr_interp, next_interp, dZ);
+        Scurr = interp_log<PREC>(c.w, cI.w, curr_interp);
+        Snext = interp_log<PREC>(n.w, nI.w, next_interp);
         return;
     }
     double sum_dtau = 0.0, sum_Sc = 0.0, sum_Sn = 0.0;
@@ -429,19 +478,20 @@ __device__ void S_dtau_line_integrated(const KParams &kp, const double *erf_tab,
         const double cb = kp.op.max_distance_opacity_contribution * (right_freq * kp.ub[cp]);
         const double nb = kp.op.max_distance_opacity_contribution * (right_freq * kp.ub[np]);
         const double W = (cb < nb) ? nb : cb;
-        lo = lower_bound_d(m.sorted_line, m.nlines, left_freq - W);
-        hi = upper_bound_d(m.sorted_line, m.nlines, right_freq + W);
+        line_window(m, ol, left_freq - W, right_freq + W, lo, hi);
     }
     for (uint32_t s = lo; s < hi; s++) {
         const uint32_t l = (APPROX == MGX_APPROX_CLOSE_LINES) ? m.sorted_line_map[s] : s;
-        const double line_dtau = dtau_single_line<PREC>(kp, erf_tab, cp, np, cpi, npi, l,
+        const double4 c = lp(kp, cp, l), cI = lp(kp, cpi, l);
+        const double4 n = lp(kp, np, l), nI = lp(kp, npi, l);
+        const double line_dtau = dtau_single_line<PREC>(kp, erf_tab, c, cI, n, nI, m.line[l],
                                                         currfreq, nextfreq, curr_interp,
                                                         next_interp, dZ);
-        const double Sc = interp_log<PREC>(lp(kp, cp, l).w, lp(kp, cpi, l).w, curr_interp);
-        const double Sn = interp_log<PREC>(lp(kp, np, l).w, lp(kp, npi, l).w, next_interp);
+        const double Sc = interp_log<PREC>(c.w, cI.w, curr_interp);
+        const double Sn = interp_log<PREC>(n.w, nI.w, next_interp);
         sum_dtau += line_dtau;
-        sum_Sc += line_dtau * Sc;
-        sum_Sn += line_dtau * Sn;
+        sum_Sc = fma(line_dtau, Sc, sum_Sc);
+        sum_Sn = fma(line_dtau, Sn, sum_Sn);
     }
     double d = sum_dtau;
     if (APPROX == MGX_APPROX_CLOSE_LINES) {
@@ -449,8 +499,9 @@ __device__ void S_dtau_line_integrated(const KParams &kp, const double *erf_tab,
         if (-bound_min_dtau < d) d = (bound_min_dtau < d) ? d : bound_min_dtau;
     }
     dtau = d;
-    Scurr = sum_Sc / d;
-    Snext = sum_Sn / d;
+    const double rd = rcp(d);
+    Scurr = sum_Sc * rd;
+    Snext = sum_Sn * rd;
 }
 
 /* ------------------------------------------------------------------------- */
@@ -474,9 +525,10 @@ struct PointEC {
  * source values of the two rows; uses/updates the caches of the two entries.
  */
 template <int APPROX, int PREC>
-__device__ inline void segment(const KParams &kp, const double *erf_tab, const Entry &cu, const Entry &nx, double dZ,
-                               double freq, uint32_t line, PointEC &ec_cu, PointEC &ec_nx,
-                               double &dtau, double &S_cu, double &S_nx)
+__device__ inline void segment(const KParams &kp, const double *erf_tab, const OwnLine &ol,
+                               const Entry &cu, const Entry &nx, double dZ, double freq,
+                               uint32_t line, PointEC &ec_cu, PointEC &ec_nx, double &dtau,
+                               double &S_cu, double &S_nx)
 {
     const double dshift = nx.shift - cu.shift;
     const double dshift_abs = fabs(dshift);
@@ -484,19 +536,21 @@ __device__ inline void segment(const KParams &kp, const double *erf_tab, const E
     const double dshift_max = (b < a) ? b : a;
     const double cf = freq * cu.shift, nf = freq * nx.shift;
     if (dshift_abs > dshift_max) {
-        S_dtau_line_integrated<APPROX, PREC>(kp, erf_tab, cu.nr, nx.nr, cu.nri, nx.nri, line, cf,
-                                             nf, cu.ifac, nx.ifac, dZ, dtau, S_cu, S_nx);
+        S_dtau_line_integrated<APPROX, PREC>(kp, erf_tab, ol, cu.nr, nx.nr, cu.nri, nx.nri, line,
+                                             cf, nf, cu.ifac, nx.ifac, dZ, dtau, S_cu, S_nx);
     } else {
         if (!ec_cu.valid) {
-            eta_chi_interp<APPROX, PREC>(kp, cu.nr, cu.nri, cu.ifac, line, cf, ec_cu.eta, ec_cu.chi);
+            eta_chi_interp<APPROX, PREC>(kp, ol, cu.nr, cu.nri, cu.ifac, line, cf, ec_cu.eta,
+                                         ec_cu.chi);
             ec_cu.valid = true;
         }
         if (!ec_nx.valid) {
-            eta_chi_interp<APPROX, PREC>(kp, nx.nr, nx.nri, nx.ifac, line, nf, ec_nx.eta, ec_nx.chi);
+            eta_chi_interp<APPROX, PREC>(kp, ol, nx.nr, nx.nri, nx.ifac, line, nf, ec_nx.eta,
+                                         ec_nx.chi);
             ec_nx.valid = true;
         }
-        S_cu = ec_cu.eta / ec_cu.chi;
-        S_nx = ec_nx.eta / ec_nx.chi;
+        S_cu = ec_cu.eta * rcp(ec_cu.chi);
+        S_nx = ec_nx.eta * rcp(ec_nx.chi);
         dtau = 0.5 * (ec_cu.chi + ec_nx.chi) * dZ;
     }
 }
@@ -505,7 +559,8 @@ __device__ inline void segment(const KParams &kp, const double *erf_tab, const E
  * Iterator over one half ray's expanded entries from the outer end inward.
  * Step s of the walk (crt_s -> nxt_s) expands into m_s entries i = m_s..1;
  * entry (s, i) is the i-th interpolation point (i = m_s at nxt_s) and the
- * segment from it to the next entry inward has length dZ_s / m_s.
+ * segment from it to the next entry inward has length dZ_s / m_s.  The counts
+ * m_s come from mtab when given, else from n_interp_for_line.
  */
 struct HalfIter {
     int s;     /* current step, -1 = at the origin */
@@ -516,14 +571,14 @@ struct HalfIter {
 
 __device__ inline void hi_load_step(const KParams &kp, HalfIter &it, uint32_t l, uint32_t o,
                                     double shift0, const uint32_t *s_nxt, const double *s_dZ,
-                                    const double *s_shift)
+                                    const double *s_shift, const uint16_t *mtab)
 {
     const int s = it.s;
     it.nxt = s_nxt[s];
     it.crt = (s == 0) ? o : s_nxt[s - 1];
     it.shift_nxt = s_shift[s];
     it.shift_crt = (s == 0) ? shift0 : s_shift[s - 1];
-    it.m_s = n_interp_for_line(kp, l, it.crt, it.nxt);
+    it.m_s = mtab ? (uint32_t)mtab[s] : n_interp_for_line(kp, l, it.crt, it.nxt);
     it.i = it.m_s;
     const double dshift = it.shift_nxt - it.shift_crt;
     it.dZ = (it.m_s > 1) ? s_dZ[s] / it.m_s : s_dZ[s];
@@ -555,14 +610,14 @@ __device__ inline Entry hi_entry(const HalfIter &it, uint32_t o)
 /* advance inward; returns the length of the segment just crossed */
 __device__ inline double hi_advance(const KParams &kp, HalfIter &it, uint32_t l, uint32_t o,
                                     double shift0, const uint32_t *s_nxt, const double *s_dZ,
-                                    const double *s_shift)
+                                    const double *s_shift, const uint16_t *mtab)
 {
     const double d = it.dZ;
     if (it.i > 1) {
         it.i--;
     } else {
         it.s--;
-        if (it.s >= 0) hi_load_step(kp, it, l, o, shift0, s_nxt, s_dZ, s_shift);
+        if (it.s >= 0) hi_load_step(kp, it, l, o, shift0, s_nxt, s_dZ, s_shift, mtab);
     }
     return d;
 }
@@ -570,8 +625,15 @@ __device__ inline double hi_advance(const KParams &kp, HalfIter &it, uint32_t l,
 /* ------------------------------------------------------------------------- */
 /* the forward-solve kernel                                                   */
 
+/* LDS layout of the walk buffers of both half rays plus the per-pass
+   interpolation-count tables (2 lines x 2 halves x cap u16) */
+__host__ __device__ inline size_t walk_lds_bytes(uint32_t cap)
+{
+    return 8 * ((2 * (size_t)cap * 4 + 7) / 8) + 4 * (size_t)cap * 8 + 4 * (size_t)cap * 2;
+}
+
 template <int APPROX, int PREC>
-__global__ __launch_bounds__(MGX_WAVE) void feautrier_kernel(KParams kp, double *u, double *J,
+__global__ __launch_bounds__(MGX_WAVE, MGX_SOLVE_WAVES_PER_SIMD) void feautrier_kernel(KParams kp, double *u, double *J,
                                                              double *lam)
 {
     extern __shared__ unsigned char smem[];
@@ -588,6 +650,7 @@ __global__ __launch_bounds__(MGX_WAVE) void feautrier_kernel(KParams kp, double 
     double *s_dZ1 = s_dZ0 + cap;
     double *s_sh0 = s_dZ1 + cap;
     double *s_sh1 = s_sh0 + cap;
+    uint16_t *s_m = reinterpret_cast<uint16_t *>(s_sh1 + cap); /* [line j][half h][step] */
 
     const uint32_t P = m.npoints, F = m.nfreqs, L = m.nlines, Q = m.nquads;
     const size_t PF = (size_t)P * F;
@@ -607,14 +670,34 @@ __global__ __launch_bounds__(MGX_WAVE) void feautrier_kernel(KParams kp, double 
         }
         __syncthreads();
         const double w = ray_weight(m, o, rr);
+        const uint32_t Sall = S0 + S1;
 
         for (uint32_t slot0 = 0; slot0 < kp.nslots; slot0 += 2) {
-            const uint32_t slot = slot0 + (lane >> 5);
-            const uint32_t l = slot / kp.slots_per_line;
+            /* interpolation counts of this pass's two lines, one per (line, step) */
+            const uint32_t lA = slot0 / kp.slots_per_line;
+            const uint32_t lB = (slot0 + 1 < kp.nslots) ? (slot0 + 1) / kp.slots_per_line : lA;
+            for (uint32_t i = lane; i < 2 * Sall; i += MGX_WAVE) {
+                const uint32_t j = i / Sall, r = i % Sall;
+                const uint32_t h = (r < S0) ? 0 : 1;
+                const uint32_t st = h ? r - S0 : r;
+                const uint32_t *nx = h ? s_nxt1 : s_nxt0;
+                const uint32_t crt = (st == 0) ? o : nx[st - 1];
+                const uint32_t mm = n_interp_for_line(kp, j ? lB : lA, crt, nx[st]);
+                if (mm > 0xffffu) atomicOr(&kp.status[0], (uint32_t)ERR_CAP);
+                s_m[(j * 2 + h) * cap + st] = (uint16_t)(mm > 0xffffu ? 0xffffu : mm);
+            }
+            __syncthreads();
+
+            const uint32_t jj = lane >> 5;
+            const uint32_t slot = slot0 + jj;
+            const uint32_t l = jj ? lB : lA;
             const uint32_t z = (slot % kp.slots_per_line) * 32 + (lane & 31);
             const bool active = (slot < kp.nslots) && (z < Q);
+            const uint16_t *mt0 = s_m + (jj * 2 + 0) * cap;
+            const uint16_t *mt1 = s_m + (jj * 2 + 1) * cap;
             double Lval = 0.0;
             if (active) {
+                const OwnLine ol = own_line(m, l);
                 const uint32_t f = l * Q + z;
                 const size_t of = (size_t)o * F + f;
                 const double freq = m.nu[of];
@@ -629,38 +712,38 @@ __global__ __launch_bounds__(MGX_WAVE) void feautrier_kernel(KParams kp, double 
                     if (S1 > 0) {
                         HalfIter it;
                         it.s = (int)S1 - 1;
-                        hi_load_step(kp, it, l, o, sh01, s_nxt1, s_dZ1, s_sh1);
+                        hi_load_step(kp, it, l, o, sh01, s_nxt1, s_dZ1, s_sh1, mt1);
                         Entry outer = hi_entry(it, o);
                         PointEC ec_out = {0.0, 0.0, false};
                         bool is_last = true;
                         double prev_dtau = 0.0, prev_Scurr = 0.0;
                         for (;;) {
-                            const double dz = hi_advance(kp, it, l, o, sh01, s_nxt1, s_dZ1, s_sh1);
+                            const double dz = hi_advance(kp, it, l, o, sh01, s_nxt1, s_dZ1, s_sh1, mt1);
                             const Entry inner = hi_entry(it, o);
                             PointEC ec_in = {0.0, 0.0, false};
                             double dtau, S_in, S_out;
-                            segment<APPROX, PREC>(kp, s_erf, inner, outer, dz, freq, l, ec_in,
+                            segment<APPROX, PREC>(kp, s_erf, ol, inner, outer, dz, freq, l, ec_in,
                                                   ec_out, dtau, S_in, S_out);
                             if (is_last) {
                                 const double I_l = boundary_intensity(m, outer.nr, freq * outer.shift);
-                                const double idt = 1.0 / dtau;
-                                const double A_l = 2.0 * idt * idt;
-                                const double Bl_min_Al = 1.0 + 2.0 * idt;
-                                const double Bl = Bl_min_Al + A_l;
+                                const double idt = rcp(dtau);
+                                const double Bl_min_Al = fma(2.0, idt, 1.0);
+                                const double Bl = fma(2.0 * idt, idt, Bl_min_Al);
                                 const double GG = 0.5 * Bl_min_Al * dtau * dtau;
-                                GP = GG / (1.0 + GG);
-                                wv = (S_out + 2.0 * I_l * idt) / Bl;
+                                const double GI = rcp(1.0 + GG);
+                                GP = GG * GI;
+                                wv = fma(2.0 * I_l, idt, S_out) * rcp(Bl);
                                 is_last = false;
                             } else {
                                 /* row "outer": A side = this segment, C side = previous one */
                                 const double dtau_avg = 0.5 * (dtau + prev_dtau);
                                 const double iA = dtau_avg * dtau;
                                 const double iC = dtau_avg * prev_dtau;
-                                const double C = 1.0 / iC;
-                                const double GG = (C * GP + 1.0) * iA;
-                                const double GI = 1.0 / (1.0 + GG);
-                                GP = GG / (1.0 + GG);
-                                wv = (C * wv + prev_Scurr) * GI * iA;
+                                const double C = rcp(iC);
+                                const double GG = fma(C, GP, 1.0) * iA;
+                                const double GI = rcp(1.0 + GG);
+                                GP = GG * GI;
+                                wv = fma(C, wv, prev_Scurr) * GI * iA;
                             }
                             prev_dtau = dtau;
                             prev_Scurr = S_in;
@@ -679,36 +762,34 @@ __global__ __launch_bounds__(MGX_WAVE) void feautrier_kernel(KParams kp, double 
                     if (S0 > 0) {
                         HalfIter it;
                         it.s = (int)S0 - 1;
-                        hi_load_step(kp, it, l, o, sh00, s_nxt0, s_dZ0, s_sh0);
+                        hi_load_step(kp, it, l, o, sh00, s_nxt0, s_dZ0, s_sh0, mt0);
                         Entry outer = hi_entry(it, o);
                         PointEC ec_out = {0.0, 0.0, false};
                         for (;;) {
-                            const double dz = hi_advance(kp, it, l, o, sh00, s_nxt0, s_dZ0, s_sh0);
+                            const double dz = hi_advance(kp, it, l, o, sh00, s_nxt0, s_dZ0, s_sh0, mt0);
                             const Entry inner = hi_entry(it, o);
                             PointEC ec_in = (it.s < 0) ? ec_cent : PointEC{0.0, 0.0, false};
                             double dtau, S_out, S_in;
-                            segment<APPROX, PREC>(kp, s_erf, outer, inner, dz, freq, l, ec_out,
+                            segment<APPROX, PREC>(kp, s_erf, ol, outer, inner, dz, freq, l, ec_out,
                                                   ec_in, dtau, S_out, S_in);
                             if (is_first) {
                                 const double I_f = boundary_intensity(m, outer.nr, freq * outer.shift);
-                                const double idt = 1.0 / dtau;
-                                const double Cf = 2.0 * idt * idt;
-                                const double Bf_min_Cf = 1.0 + 2.0 * idt;
-                                const double Bf = Bf_min_Cf + Cf;
-                                Su = S_out + 2.0 * I_f * idt;
-                                Su /= Bf;
+                                const double idt = rcp(dtau);
+                                const double Bf_min_Cf = fma(2.0, idt, 1.0);
+                                const double Bf = fma(2.0 * idt, idt, Bf_min_Cf);
+                                Su = fma(2.0 * I_f, idt, S_out) * rcp(Bf);
                                 FF = 0.5 * Bf_min_Cf * dtau * dtau;
-                                FI = 1.0 / (1.0 + FF);
+                                FI = rcp(1.0 + FF);
                                 is_first = false;
                             } else {
                                 /* row "outer": A side = previous segment, C side = this one */
                                 const double dtau_avg = 0.5 * (prev_dtau + dtau);
                                 const double iA = dtau_avg * prev_dtau;
                                 const double iC = dtau_avg * dtau;
-                                const double A = 1.0 / iA;
-                                FF = (A * FF * FI + 1.0) * iC;
-                                FI = 1.0 / (1.0 + FF);
-                                Su = (A * Su + S_out) * FI * iC;
+                                const double A = rcp(iA);
+                                FF = fma(A * FF, FI, 1.0) * iC;
+                                FI = rcp(1.0 + FF);
+                                Su = fma(A, Su, S_out) * FI * iC;
                             }
                             prev_dtau = dtau;
                             prev_Snext = S_in;
@@ -722,36 +803,35 @@ __global__ __launch_bounds__(MGX_WAVE) void feautrier_kernel(KParams kp, double 
                     if (S1 > 0) {
                         if (is_first) { /* the origin is the first row */
                             const double I_f = boundary_intensity(m, o, freq);
-                            const double idt = 1.0 / dtau_c;
+                            const double idt = rcp(dtau_c);
                             const double Cf = 2.0 * idt * idt;
-                            iC_c = 1.0 / Cf;
-                            const double Bf_min_Cf = 1.0 + 2.0 * idt;
+                            iC_c = 0.5 * dtau_c * dtau_c;
+                            const double Bf_min_Cf = fma(2.0, idt, 1.0);
                             const double Bf = Bf_min_Cf + Cf;
-                            Su = S_c + 2.0 * I_f * idt;
-                            Su /= Bf;
+                            Su = fma(2.0 * I_f, idt, S_c) * rcp(Bf);
                             FF = 0.5 * Bf_min_Cf * dtau_c * dtau_c;
                         } else {
                             const double dtau_avg = 0.5 * (prev_dtau + dtau_c);
                             const double iA = dtau_avg * prev_dtau;
                             iC_c = dtau_avg * dtau_c;
-                            const double A = 1.0 / iA;
-                            FF = (A * FF * FI + 1.0) * iC_c;
-                            FI = 1.0 / (1.0 + FF);
-                            Su = (A * Su + S_c) * FI * iC_c;
+                            const double A = rcp(iA);
+                            FF = fma(A * FF, FI, 1.0) * iC_c;
+                            FI = rcp(1.0 + FF);
+                            Su = fma(A, Su, S_c) * FI * iC_c;
                         }
-                        const double den = FF + GP;
-                        uc = (Su * (1.0 + FF) + wv) / den;
-                        Lc = iC_c / den;
+                        const double rden = rcp(FF + GP);
+                        uc = fma(Su, 1.0 + FF, wv) * rden;
+                        Lc = iC_c * rden;
                     } else { /* the origin is the last row */
                         const double I_l = boundary_intensity(m, o, freq);
-                        const double idt = 1.0 / prev_dtau;
+                        const double idt = rcp(prev_dtau);
                         const double A_l = 2.0 * idt * idt;
-                        const double Bl_min_Al = 1.0 + 2.0 * idt;
+                        const double Bl_min_Al = fma(2.0, idt, 1.0);
                         const double Bl = Bl_min_Al + A_l;
-                        const double denominator = 1.0 / (Bl * FF + Bl_min_Al);
-                        const double Sl = prev_Snext + 2.0 * I_l * idt;
-                        uc = (A_l * Su + Sl) * (1.0 + FF) * denominator;
-                        Lc = (1.0 + FF) / (Bl_min_Al + Bl * FF);
+                        const double denominator = rcp(fma(Bl, FF, Bl_min_Al));
+                        const double Sl = fma(2.0 * I_l, idt, prev_Snext);
+                        uc = fma(A_l, Su, Sl) * (1.0 + FF) * denominator;
+                        Lc = (1.0 + FF) * denominator;
                     }
                     J[of] = ((rr == 0) ? 0.0 : J[of]) + uc * 2.0 * w;
                     /* ---- Lambda diagonal (update_Lambda, solver.tpp:1761-1800) ---- */
@@ -763,8 +843,8 @@ __global__ __launch_bounds__(MGX_WAVE) void feautrier_kernel(KParams kp, double 
                         const double constante =
                             m.line_A[l] * m.quad_weights[(size_t)spec * Q + m.f_z[f]] * w_ang;
                         double eta, chi;
-                        eta_chi_interp<APPROX, PREC>(kp, o, o, 0.0, k, freq_line, eta, chi);
-                        const double inverse_chi = 1.0 / chi;
+                        eta_chi_interp<APPROX, PREC>(kp, ol, o, o, 0.0, k, freq_line, eta, chi);
+                        const double inverse_chi = rcp(chi);
                         const double frq = freq * 1.0;
                         const double inv_mass = m.species_inverse_mass[spec];
                         const double width =
@@ -799,8 +879,8 @@ __global__ __launch_bounds__(MGX_WAVE) void feautrier_kernel(KParams kp, double 
                     }
                 }
             }
+            __syncthreads();
         }
-        __syncthreads();
     }
 }
 
@@ -948,14 +1028,14 @@ __global__ __launch_bounds__(MGX_WAVE) void trace_rays_kernel(
         uint32_t j = 0;
         if (S0 > 0) {
             it.s = (int)S0 - 1;
-            hi_load_step(kp, it, l, o, sh00, s_nxt0, s_dZ0, s_sh0);
+            hi_load_step(kp, it, l, o, sh00, s_nxt0, s_dZ0, s_sh0, nullptr);
             while (it.s >= 0) {
                 const Entry en = hi_entry(it, o);
                 nr[row + j] = en.nr;
                 nr_interp[row + j] = en.nri;
                 interp_factor[row + j] = en.ifac;
                 shift[row + j] = en.shift;
-                dZ[row + j] = hi_advance(kp, it, l, o, sh00, s_nxt0, s_dZ0, s_sh0);
+                dZ[row + j] = hi_advance(kp, it, l, o, sh00, s_nxt0, s_dZ0, s_sh0, nullptr);
                 j++;
             }
         }
@@ -1065,9 +1145,8 @@ int check_view(const mgx_model *m, const mgx_options *op)
     return MGX_OK;
 }
 
-size_t walk_lds_bytes(uint32_t cap) { return 8 * ((2 * (size_t)cap * 4 + 7) / 8) + 4 * (size_t)cap * 8; }
 
-const uint32_t kMaxCap = 1536; /* 1536 steps per half: 61 KB of LDS */
+const uint32_t kMaxCap = 1280; /* 1280 steps per half: 60 KB of dynamic LDS */
 
 bool g_timing = false;
 double g_last_kernel_ms = 0.0;
