@@ -158,6 +158,22 @@ int mgx_trace_rays_device(const mgx_model *model_dev, const mgx_options *opts, u
                           double *interp_factor, double *dZ, void *stream);
 
 
+/*
+ * Projection-surface image (the "new imager"): one ray per pixel from a plane
+ * behind the model along raydir, forward Feautrier elimination per image
+ * frequency, I(pix, f) = 2 Su[last] - I_bdy.  Replaces
+ * Model::compute_image_new(raydir, Nx, Ny) (src/model/model.cpp:1008-1035) ->
+ * Solver::image_feautrier_order_2_new_imager (src/solver/solver.tpp:577-627),
+ * Image::set_coordinates_projection_surface (src/model/image/image.cpp:221-366).
+ * raydir (host, 3 doubles) need not be normalised (it is, as in the
+ * reference); the model's frequencies are the image frequencies (nu (P, F)).
+ * Outputs (device): I (nx*ny, F) with pixel index y + ny*x, ImX, ImY (nx*ny).
+ * Errors: zero direction -> MGX_E_ARG; OneLine approximation ->
+ * MGX_E_UNSUPPORTED; 1D model with raydir != (0,1,0) -> MGX_E_ARG.
+ */
+int mgx_image_device(const mgx_model *model_dev, const mgx_options *opts, const double *raydir,
+                     uint32_t nx, uint32_t ny, double *I, double *ImX, double *ImY, void *stream);
+
 /* ------------------------------------------------------------------------ */
 /* Host-side producers of the path's inputs (plain host pointers).           */
 /* ------------------------------------------------------------------------ */

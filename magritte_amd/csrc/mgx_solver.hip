@@ -31,6 +31,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <vector>
@@ -43,7 +44,7 @@
 #define MGX_WAVE 64
 /* minimum waves per SIMD requested for the solve kernel (register budget) */
 #ifndef MGX_SOLVE_WAVES_PER_SIMD
-#define MGX_SOLVE_WAVES_PER_SIMD 2
+#define MGX_SOLVE_WAVES_PER_SIMD 3
 #endif
 
 namespace {
@@ -1082,6 +1083,355 @@ __global__ __launch_bounds__(MGX_WAVE) void trace_rays_kernel(
 }
 
 /* ------------------------------------------------------------------------- */
+/* projection-surface imager (image_feautrier_order_2_new_imager,             */
+/* solver.tpp:577-627; Image::set_coordinates_projection_surface,             */
+/* image.cpp:221-398)                                                         */
+
+struct ImageFrame {
+    double rd[3];          /* normalised ray direction */
+    double ix, iy, jx, jy, jz, denominator;
+    double cpt[3];         /* surface_center_point */
+    double min_x, min_y, deltax, deltay;
+    uint32_t start_bdy;    /* get_closest_bdy_point_in_custom_raydir */
+    uint32_t nx, ny;
+};
+
+/* lexicographic min over the block of (value, key) pairs */
+__device__ inline bool lex_less(double a, uint32_t ka, double b, uint32_t kb)
+{
+    return (a < b) || (a == b && ka < kb);
+}
+
+/* one block of 256 threads: projected bounding box of the boundary points and
+   the closest boundary point along the ray direction */
+__global__ __launch_bounds__(256) void image_frame_kernel(mgx_model m, ImageFrame *out)
+{
+    __shared__ double s_min_x[256], s_max_x[256], s_min_y[256], s_max_y[256], s_pv[256];
+    __shared__ uint32_t s_pk[256], s_pp[256];
+    ImageFrame fr = *out; /* rd, nx, ny set by the host */
+    const double rx = fr.rd[0], ry = fr.rd[1], rz = fr.rd[2];
+    fr.denominator = sqrt(rx * rx + ry * ry);
+    const double inverse_denominator = 1.0 / fr.denominator;
+    fr.ix = ry * inverse_denominator;
+    fr.iy = -rx * inverse_denominator;
+    fr.jx = rx * rz * inverse_denominator;
+    fr.jy = ry * rz * inverse_denominator;
+    fr.jz = -fr.denominator;
+    double mnx = DBL_MAX, mxx = -DBL_MAX, mny = DBL_MAX, mxy = -DBL_MAX;
+    /* closest: initial guess point2boundary[0] used as a point index (key 0),
+       then boundary points b >= 1 (key b), strict "<" */
+    double pv = __builtin_inf();
+    uint32_t pk = 0xffffffffu, pp = 0;
+    const uint32_t t = threadIdx.x;
+    if (t == 0 && !m.spherical_symmetry) {
+        pp = m.point2boundary[0];
+        pv = dot3(fr.rd, &m.position[3 * (size_t)pp]);
+        pk = 0;
+    }
+    for (uint32_t b = t; b < m.nboundary; b += 256) {
+        const double *p = &m.position[3 * (size_t)m.boundary2point[b]];
+        double X, Y;
+        if (m.dimension == 1) {
+            X = p[0];
+            Y = 0.0;
+        } else if (fr.denominator >= 1.0e-9) {
+            X = fr.ix * p[0] + fr.iy * p[1];
+            Y = fr.jx * p[0] + fr.jy * p[1] + fr.jz * p[2];
+        } else {
+            X = p[0];
+            Y = p[1];
+        }
+        mnx = (X < mnx) ? X : mnx;
+        mxx = (mxx < X) ? X : mxx;
+        mny = (Y < mny) ? Y : mny;
+        mxy = (mxy < Y) ? Y : mxy;
+        if (b >= 1 && !m.spherical_symmetry) {
+            const double pd = dot3(fr.rd, p);
+            if (lex_less(pd, b, pv, pk)) {
+                pv = pd;
+                pk = b;
+                pp = m.boundary2point[b];
+            }
+        }
+    }
+    s_min_x[t] = mnx;
+    s_max_x[t] = mxx;
+    s_min_y[t] = mny;
+    s_max_y[t] = mxy;
+    s_pv[t] = pv;
+    s_pk[t] = pk;
+    s_pp[t] = pp;
+    __syncthreads();
+    for (uint32_t h = 128; h > 0; h >>= 1) {
+        if (t < h) {
+            s_min_x[t] = (s_min_x[t + h] < s_min_x[t]) ? s_min_x[t + h] : s_min_x[t];
+            s_max_x[t] = (s_max_x[t] < s_max_x[t + h]) ? s_max_x[t + h] : s_max_x[t];
+            s_min_y[t] = (s_min_y[t + h] < s_min_y[t]) ? s_min_y[t + h] : s_min_y[t];
+            s_max_y[t] = (s_max_y[t] < s_max_y[t + h]) ? s_max_y[t + h] : s_max_y[t];
+            if (lex_less(s_pv[t + h], s_pk[t + h], s_pv[t], s_pk[t])) {
+                s_pv[t] = s_pv[t + h];
+                s_pk[t] = s_pk[t + h];
+                s_pp[t] = s_pp[t + h];
+            }
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        double min_x = s_min_x[0], max_x = s_max_x[0], min_y = s_min_y[0], max_y = s_max_y[0];
+        if (m.dimension == 1) {
+            const double ax = fabs(min_x);
+            max_x = (ax < max_x) ? max_x : ax;
+            min_x = -max_x;
+            min_y = min_x;
+            max_y = max_x;
+        }
+        fr.start_bdy = m.spherical_symmetry ? m.npoints - 1 : s_pp[0];
+        const double *cp = &m.position[3 * (size_t)fr.start_bdy];
+        double dist = dot3(cp, fr.rd);
+        if (m.spherical_symmetry) dist = -cp[0];
+        for (int i = 0; i < 3; i++) fr.cpt[i] = fr.rd[i] * dist;
+        fr.min_x = min_x;
+        fr.min_y = min_y;
+        fr.deltax = (max_x - min_x) / (fr.nx);
+        fr.deltay = (max_y - min_y) / (fr.ny);
+        *out = fr;
+    }
+}
+
+/* get_boundary_point_closer_to_custom_ray (geometry.tpp:310-341), wave-cooperative */
+__device__ inline uint32_t bdy_closer(const mgx_model &m, const double *origin, const double *rd,
+                                      uint32_t crt)
+{
+    const uint32_t lane = threadIdx.x & (MGX_WAVE - 1);
+    const uint32_t n_nbs = m.n_neighbors[crt];
+    const uint32_t cum = m.cum_n_neighbors[crt];
+    double Rc[3];
+    for (int i = 0; i < 3; i++) Rc[i] = m.position[3 * (size_t)crt + i] - origin[i];
+    const double Zc = dot3(Rc, rd);
+    /* the current point is the candidate to beat (key 0); neighbour i has key i + 1 */
+    double best = (lane == 0) ? dot3(Rc, Rc) - Zc * Zc : __builtin_inf();
+    uint32_t key = (lane == 0) ? 0u : 0xffffffffu;
+    double dummy = 0.0;
+    for (uint32_t i = lane; i < n_nbs; i += MGX_WAVE) {
+        const uint32_t n = m.neighbors[cum + i];
+        if (m.point2boundary[n] != m.npoints) {
+            double R[3];
+            for (int j = 0; j < 3; j++) R[j] = m.position[3 * (size_t)n + j] - origin[j];
+            const double Zn = dot3(R, rd);
+            const double d2 = dot3(R, R) - Zn * Zn;
+            if (lex_less(d2, i + 1, best, key)) {
+                best = d2;
+                key = i + 1;
+            }
+        }
+    }
+    wave_argmin(best, key, dummy);
+    return (key == 0 || key == 0xffffffffu) ? crt : m.neighbors[cum + key - 1];
+}
+
+__device__ inline double shift_rest(const mgx_model &m, const double *origin, const double *rd,
+                                    uint32_t c, double Z)
+{
+    if (m.spherical_symmetry) {
+        const double *pc = &m.position[3 * (size_t)c];
+        if (pc[0] == 0.0) return 1.0;
+        const double Rcos_plus_Z = dot3(origin, rd) + Z;
+        return 1.0 + m.velocity[3 * (size_t)c] * Rcos_plus_Z / pc[0];
+    }
+    return 1.0 + dot3(&m.velocity[3 * (size_t)c], rd);
+}
+
+/* all-lines interpolation count (InterpHelper::get_n_interp, :10-46) */
+__device__ inline uint32_t n_interp_all(const KParams &kp, uint32_t c, uint32_t n)
+{
+    const uint32_t L = kp.m.nlines;
+    double mx = 0.0;
+    for (uint32_t l = 0; l < L; l++) {
+        const double d = (double)fabsf(mgx_logf((float)(kp.m.interp_crit[(size_t)n * L + l] /
+                                                        kp.m.interp_crit[(size_t)c * L + l])));
+        if (l == 0 || mx < d) mx = d;
+    }
+    const uint32_t k = (uint32_t)ceil(mx / kp.log_max_interp);
+    return (k > 1) ? k : 1;
+}
+
+template <int APPROX, int PREC>
+__global__ __launch_bounds__(MGX_WAVE) void image_kernel(KParams kp, const ImageFrame *frp,
+                                                         double *I, double *ImX, double *ImY)
+{
+    extern __shared__ unsigned char smem[];
+    __shared__ double s_erf[MGX_ERF_NINT * (MGX_ERF_DEG + 1)];
+    for (uint32_t i = threadIdx.x; i < MGX_ERF_NINT * (MGX_ERF_DEG + 1); i += MGX_WAVE)
+        s_erf[i] = MGX_ERF_TAB[i];
+    const mgx_model &m = kp.m;
+    const ImageFrame fr = *frp;
+    const uint32_t pix = blockIdx.x;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t cap = kp.cap;
+    uint32_t *s_nxt = reinterpret_cast<uint32_t *>(smem);
+    double *s_dZ = reinterpret_cast<double *>(smem + 8 * ((2 * cap * 4 + 7) / 8));
+    double *s_sh = s_dZ + 2 * cap;
+    uint16_t *s_m = reinterpret_cast<uint16_t *>(s_dZ + 4 * cap);
+    const uint32_t F = m.nfreqs;
+
+    /* pixel position on the projection surface and its 3D origin */
+    const uint32_t x_idx = pix / fr.ny, y_idx = pix % fr.ny;
+    const double xloc = fr.min_x + fr.deltax / 2.0 + x_idx * fr.deltax;
+    const double yloc = fr.min_y + fr.deltay / 2.0 + y_idx * fr.deltay;
+    if (lane == 0) {
+        ImX[pix] = xloc;
+        ImY[pix] = yloc;
+    }
+    double origin[3];
+    if (fr.denominator >= 1.0e-9) {
+        origin[0] = (xloc * fr.ix + yloc * fr.jx) + fr.cpt[0];
+        origin[1] = (xloc * fr.iy + yloc * fr.jy) + fr.cpt[1];
+        origin[2] = (yloc * fr.jz) + fr.cpt[2];
+    } else {
+        origin[0] = xloc + fr.cpt[0];
+        origin[1] = yloc + fr.cpt[1];
+        origin[2] = 0.0 + fr.cpt[2];
+    }
+    const double *rd = fr.rd;
+
+    /* trace_ray_imaging_get_start (solver.tpp:808-830) */
+    uint32_t ip = fr.start_bdy;
+    for (uint32_t it = 0; it <= m.nboundary; it++) {
+        const uint32_t na = bdy_closer(m, origin, rd, ip);
+        if (na == ip) break;
+        ip = na;
+    }
+    double Z;
+    {
+        double R[3];
+        for (int i = 0; i < 3; i++) R[i] = m.position[3 * (size_t)ip + i] - origin[i];
+        Z = dot3(rd, R);
+        if (m.spherical_symmetry) {
+            const double Rcos = dot3(origin, rd);
+            const double R2 = dot3(origin, origin);
+            const double Rsin2 = R2 - Rcos * Rcos;
+            const double *pb = &m.position[3 * (size_t)ip];
+            const double h2 = dot3(pb, pb) - Rsin2;
+            Z = (h2 > 0) ? (-sqrt(h2) - Rcos) : -Rcos;
+        }
+    }
+    const double shift_centre = shift_rest(m, origin, rd, ip, Z);
+
+    /* trace_ray_imaging<Rest> (solver.tpp:842-897): walk into LDS */
+    uint32_t S = 0;
+    double shift0 = 0.0;
+    {
+        double dZ = 0.0;
+        uint32_t crt = ip;
+        uint32_t nxt = m.spherical_symmetry ? next_spherical(m, origin, rd, crt, Z, dZ)
+                                            : next_general(m, origin, rd, crt, Z, dZ);
+        bool accessed_non_boundary = false;
+        if (nxt < m.npoints) {
+            shift0 = shift_rest(m, origin, rd, crt, Z - dZ);
+            double shift_nxt = shift_rest(m, origin, rd, nxt, Z);
+            for (;;) {
+                if (S >= cap) {
+                    if (lane == 0) {
+                        atomicOr(&kp.status[0], (uint32_t)ERR_CAP);
+                        atomicMax(&kp.status[1], cap + 1);
+                    }
+                    return;
+                }
+                if (lane == 0) {
+                    s_nxt[S] = nxt;
+                    s_dZ[S] = dZ;
+                    s_sh[S] = shift_nxt;
+                }
+                S++;
+                double rel[3];
+                for (int i = 0; i < 3; i++) rel[i] = m.position[3 * (size_t)nxt + i] - m.center[i];
+                const bool interior = m.point2boundary[nxt] == m.npoints;
+                if (!(interior || dot3(rel, rd) < 0)) break;
+                crt = nxt;
+                if (interior) accessed_non_boundary = true;
+                nxt = m.spherical_symmetry ? next_spherical(m, origin, rd, crt, Z, dZ)
+                                           : next_general(m, origin, rd, crt, Z, dZ);
+                if (nxt >= m.npoints) break;
+                shift_nxt = shift_rest(m, origin, rd, nxt, Z);
+            }
+        }
+        if (!accessed_non_boundary && !m.spherical_symmetry) S = 0;
+    }
+    __syncthreads();
+    for (uint32_t i = lane; i < S; i += MGX_WAVE) {
+        const uint32_t crt = (i == 0) ? ip : s_nxt[i - 1];
+        const uint32_t mm = n_interp_all(kp, crt, s_nxt[i]);
+        if (mm > 0xffffu) atomicOr(&kp.status[0], (uint32_t)ERR_CAP);
+        s_m[i] = (uint16_t)(mm > 0xffffu ? 0xffffu : mm);
+    }
+    __syncthreads();
+
+    for (uint32_t f = lane; f < F; f += MGX_WAVE) {
+        const double nu_ip = m.nu[(size_t)ip * F + f];
+        double Ival;
+        if (S == 0) {
+            Ival = boundary_intensity(m, ip, nu_ip);
+        } else {
+            const double freq = nu_ip;
+            const uint32_t line = m.f_line[f];
+            const OwnLine ol = own_line(m, m.sorted_line_map[upper_bound_d(m.sorted_line, m.nlines, freq) > 0
+                                                               ? upper_bound_d(m.sorted_line, m.nlines, freq) - 1
+                                                               : 0]);
+            HalfIter it;
+            it.s = (int)S - 1;
+            hi_load_step(kp, it, line, ip, shift0, s_nxt, s_dZ, s_sh, s_m);
+            Entry outer = hi_entry(it, ip);
+            PointEC ec_out = {0.0, 0.0, false};
+            double FF = 0.0, FI = 0.0, Su = 0.0, prev_dtau = 0.0, prev_Snext = 0.0;
+            bool is_first = true;
+            for (;;) {
+                const double dz = hi_advance(kp, it, line, ip, shift0, s_nxt, s_dZ, s_sh, s_m);
+                Entry inner = hi_entry(it, ip);
+                if (it.s < 0) inner.shift = shift_centre;
+                PointEC ec_in = {0.0, 0.0, false};
+                double dtau, S_out, S_in;
+                segment<APPROX, PREC>(kp, s_erf, ol, outer, inner, dz, freq, line, ec_out, ec_in,
+                                      dtau, S_out, S_in);
+                if (is_first) {
+                    const double I_f = boundary_intensity(m, outer.nr, freq * outer.shift);
+                    const double idt = rcp(dtau);
+                    const double Bf_min_Cf = fma(2.0, idt, 1.0);
+                    const double Bf = fma(2.0 * idt, idt, Bf_min_Cf);
+                    Su = fma(2.0 * I_f, idt, S_out) * rcp(Bf);
+                    FF = 0.5 * Bf_min_Cf * dtau * dtau;
+                    FI = rcp(1.0 + FF);
+                    is_first = false;
+                } else {
+                    const double dtau_avg = 0.5 * (prev_dtau + dtau);
+                    const double iA = dtau_avg * prev_dtau;
+                    const double iC = dtau_avg * dtau;
+                    const double A = rcp(iA);
+                    FF = fma(A * FF, FI, 1.0) * iC;
+                    FI = rcp(1.0 + FF);
+                    Su = fma(A, Su, S_out) * FI * iC;
+                }
+                prev_dtau = dtau;
+                prev_Snext = S_in;
+                outer = inner;
+                ec_out = ec_in;
+                if (it.s < 0) break;
+            }
+            /* last row at the start point (image_feautrier_order_2, solver.tpp:2546-2559) */
+            const double I_l = boundary_intensity(m, ip, freq * shift_centre);
+            const double idt = rcp(prev_dtau);
+            const double A_l = 2.0 * idt * idt;
+            const double Bl_min_Al = fma(2.0, idt, 1.0);
+            const double Bl = Bl_min_Al + A_l;
+            const double denominator = rcp(fma(Bl, FF, Bl_min_Al));
+            const double Sl = fma(2.0 * I_l, idt, prev_Snext);
+            const double Su_last = fma(A_l, Su, Sl) * (1.0 + FF) * denominator;
+            Ival = 2.0 * Su_last - boundary_intensity(m, ip, nu_ip);
+        }
+        I[(size_t)pix * F + f] = Ival;
+    }
+}
+
+/* ------------------------------------------------------------------------- */
 /* preparation kernels                                                       */
 
 __global__ void pack_lines_kernel(const mgx_model m, double4 *lpack, double *ub)
@@ -1273,6 +1623,70 @@ int mgx_feautrier_forward_device(const mgx_model *model, const mgx_options *opts
                 else
                     hipLaunchKernelGGL((feautrier_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_MIXED>),
                                        grid, block, lds, st, kp, u, J, lambda_diag);
+            }
+        });
+    });
+}
+
+
+int mgx_image_device(const mgx_model *model, const mgx_options *opts, const double *raydir,
+                     uint32_t nx, uint32_t ny, double *I, double *ImX, double *ImY, void *stream)
+{
+    mgx::clear_error();
+    if (!model || !opts || !raydir || !I || !ImX || !ImY)
+        return mgx::fail(MGX_E_ARG, "null argument");
+    if (nx == 0 || ny == 0) return mgx::fail(MGX_E_ARG, "the image needs at least one pixel");
+    const double sq = raydir[0] * raydir[0] + raydir[1] * raydir[1] + raydir[2] * raydir[2];
+    if (sq == 0.0)
+        return mgx::fail(MGX_E_ARG, "The given ray direction vector does not point in a "
+                                    "direction. Please use a non-zero (normed) direction vector "
+                                    "to generate an image.");
+    if (opts->approx == MGX_APPROX_ONE_LINE)
+        return mgx::fail(MGX_E_UNSUPPORTED, "One line approximation is not supported for imaging.");
+    if (!model->position || !model->nu || !model->boundary2point || !model->point2boundary ||
+        !model->interp_crit || !model->sorted_line || !model->emissivity)
+        return mgx::fail(MGX_E_ARG, "null array in model view");
+    if (model->npoints == 0 || model->nfreqs == 0 || model->nlines == 0 || model->nboundary == 0)
+        return mgx::fail(MGX_E_ARG, "empty model");
+    const double inv = 1 / std::sqrt(sq);
+    ImageFrame fr;
+    std::memset(&fr, 0, sizeof fr);
+    for (int i = 0; i < 3; i++) fr.rd[i] = raydir[i] * inv;
+    if (model->dimension == 1 && (fr.rd[0] != 0.0 || fr.rd[1] != 1.0 || fr.rd[2] != 0.0))
+        return mgx::fail(MGX_E_ARG, "In 1D, the image ray has to be (0,1,0)");
+    fr.nx = nx;
+    fr.ny = ny;
+    hipStream_t st = (hipStream_t)stream;
+    DevBuffers b;
+    KParams kp;
+    if (int e = prepare(model, opts, st, b, kp)) return e;
+    ImageFrame *dfr = nullptr;
+    HIP_TRY(hipMalloc(&dfr, sizeof(ImageFrame)));
+    struct Free {
+        ImageFrame *p;
+        ~Free() { (void)hipFree(p); }
+    } free_fr{dfr};
+    HIP_TRY(hipMemcpyAsync(dfr, &fr, sizeof fr, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(image_frame_kernel, dim3(1), dim3(256), 0, st, *model, dfr);
+    HIP_TRY(hipGetLastError());
+    return launch_with_capacity(st, b, kp, [&](size_t lds) {
+        const dim3 grid(nx * ny), block(MGX_WAVE);
+        timed(st, [&] {
+            const bool fp64 = opts->precision == MGX_PREC_FP64;
+            if (opts->approx == MGX_APPROX_NONE) {
+                if (fp64)
+                    hipLaunchKernelGGL((image_kernel<MGX_APPROX_NONE, MGX_PREC_FP64>), grid, block,
+                                       lds, st, kp, dfr, I, ImX, ImY);
+                else
+                    hipLaunchKernelGGL((image_kernel<MGX_APPROX_NONE, MGX_PREC_MIXED>), grid, block,
+                                       lds, st, kp, dfr, I, ImX, ImY);
+            } else {
+                if (fp64)
+                    hipLaunchKernelGGL((image_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_FP64>), grid,
+                                       block, lds, st, kp, dfr, I, ImX, ImY);
+                else
+                    hipLaunchKernelGGL((image_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_MIXED>), grid,
+                                       block, lds, st, kp, dfr, I, ImX, ImY);
             }
         });
     });

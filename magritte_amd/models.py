@@ -349,13 +349,32 @@ class ModelData:
         o.max_interpolation_diff = self.params["max_interpolation_diff"]
         return o
 
-    def view(self, arrays=None):
-        """mgx_model whose pointers refer to ``arrays`` (default: the numpy arrays)."""
-        arrays = arrays if arrays is not None else {k: getattr(self, k) for k in self.ARRAYS}
+    FREQ_ARRAYS = ("nu", "f_line", "f_species", "f_k", "f_z", "f_in_line")
+
+    def image_frequencies(self, nu_min, nu_max, n):
+        """Model::compute_spectral_discretisation(nu_min, nu_max, n) (model.cpp:257-298):
+        the arrays of an image frequency grid (the line grid of the model is kept)."""
+        P = self.npoints
+        d = dict(nu=np.empty((P, n)), f_line=np.empty(n, dtype=np.uint32),
+                 f_species=np.empty(n, dtype=np.uint32), f_k=np.empty(n, dtype=np.uint32),
+                 f_z=np.empty(n, dtype=np.uint32), f_in_line=np.empty(n, dtype=np.uint8))
+        _abi.check(_abi.lib().mgx_compute_image_discretisation(
+            P, float(nu_min), float(nu_max), n, *[_abi.ptr(d[k]) for k in self.FREQ_ARRAYS]))
+        return d
+
+    def view(self, arrays=None, nfreqs=None):
+        """mgx_model whose pointers refer to ``arrays`` (default: the numpy arrays);
+        entries missing from ``arrays`` are taken from the model."""
+        base = {k: getattr(self, k) for k in self.ARRAYS}
+        if arrays is not None:
+            base.update(arrays)
+        arrays = base
         m = _abi.MgxModel()
         for f in ("npoints", "nrays", "hnrays", "nboundary", "nlines", "nlspecs", "nquads",
                   "nfreqs", "dimension", "spherical_symmetry", "adaptive_directions"):
             setattr(m, f, int(getattr(self, f)))
+        if nfreqs is not None:
+            m.nfreqs = int(nfreqs)
         for k in self.ARRAYS:
             ftype = dict(m._fields_)[k]
             setattr(m, k, C.cast(C.c_void_p(_abi.ptr(arrays[k])), ftype))

@@ -99,3 +99,25 @@ def trace_stats(dm, opts):
     _abi.check(_abi.lib().mgx_trace_stats_device(C.byref(dm.view), C.byref(opts),
                                                  st.ctypes.data, dm.stream_ptr()))
     return tuple(int(x) for x in st)
+
+
+def image_new(dm, opts, raydir, nx, ny, freqs):
+    """Projection-surface image along raydir (Model::compute_image_new): returns
+    (I (nx*ny, F), ImX, ImY) as torch tensors; ``freqs`` is the image frequency
+    grid from ModelData.image_frequencies."""
+    torch = _require_gpu()
+    m = dm.model
+    dev = dm.device
+    t = dict(dm.tensors)
+    for k in m.FREQ_ARRAYS:
+        t[k] = torch.from_numpy(np.ascontiguousarray(freqs[k])).to(dev)
+    F = freqs["nu"].shape[1]
+    view = m.view(t, nfreqs=F)
+    I = torch.empty((nx * ny, F), dtype=torch.float64, device=dev)
+    ImX = torch.empty(nx * ny, dtype=torch.float64, device=dev)
+    ImY = torch.empty(nx * ny, dtype=torch.float64, device=dev)
+    rd = np.ascontiguousarray(raydir, dtype=np.float64)
+    _abi.check(_abi.lib().mgx_image_device(C.byref(view), C.byref(opts), rd.ctypes.data, nx, ny,
+                                           _abi.ptr(I), _abi.ptr(ImX), _abi.ptr(ImY),
+                                           dm.stream_ptr()))
+    return I, ImX, ImY

@@ -86,10 +86,11 @@ def trace_ray(model, opts, rr, o, l, capacity=4096, kind="cr"):
                 interp_factor=fa[:n], dZ=dz[:n])
 
 
-def image_new(model, opts, raydir, nx, ny, nthreads=0, kind="cr"):
-    m = model.view()
+def image_new(model, opts, raydir, nx, ny, freqs=None, nthreads=0, kind="cr"):
+    F = model.nfreqs if freqs is None else freqs["nu"].shape[1]
+    m = model.view(freqs, nfreqs=F)
     rd = np.ascontiguousarray(raydir, dtype=np.float64)
-    I = np.zeros((nx * ny, model.nfreqs))
+    I = np.zeros((nx * ny, F))
     ImX = np.zeros(nx * ny)
     ImY = np.zeros(nx * ny)
     _check(lib(kind).oracle_image_new(C.addressof(m), C.addressof(opts), rd.ctypes.data, nx, ny,

@@ -58,3 +58,13 @@ def slab_model():
 
     ld = models.read_lamda(os.path.join(ROOT, "tests", "golden", "lamda_test.txt"), num=0)
     return models.model_constant_slab(linedata=ld)
+
+
+@pytest.fixture(scope="session")
+def imager_model():
+    """The reference's new-imager benchmark model (50 points, 4 rays, 10 quadrature bins)."""
+    from magritte_amd import models
+
+    ld = models.read_lamda(os.path.join(ROOT, "tests", "golden", "lamda_test.txt"), num=0)
+    return models.model_velocity_gradient_1d(ld, npoints=50, nrays=4, nquads=10,
+                                             max_width_fraction=0.5)

@@ -133,3 +133,36 @@ def test_mixed_mode_keeps_traces(small_wind):
     a = solver.trace_counts(dm, m.options(precision=_abi.PREC_FP64))
     b = solver.trace_counts(dm, m.options(precision=_abi.PREC_MIXED))
     np.testing.assert_array_equal(a, b)
+
+
+def _image_case(m, nfreq=31):
+    lo = float(m.line[0]) * (1 - 2e-4)
+    hi = float(m.line[0]) * (1 + 2e-4)
+    return m.image_frequencies(lo, hi, nfreq)
+
+
+@pytest.mark.parametrize("which,raydir,nx,ny", [
+    ("vgrad_model", (0.0, 1.0, 0.0), 12, 10),
+    ("imager_model", (0.0, 1.0, 0.0), 50, 50),
+    ("small_wind", (0.3, -0.5, 0.8), 16, 12),
+    ("small_wind", (0.0, 0.0, 1.0), 9, 9),
+    ("wind32", (0.2, 0.7, -0.4), 24, 24),
+])
+def test_image_fp64_parity(which, raydir, nx, ny, request):
+    from magritte_amd import solver
+    from oracle import oracle
+
+    m = request.getfixturevalue(which)
+    op = m.options()
+    fq = _image_case(m)
+    print(f"[image] model={which} dir={raydir} {nx}x{ny}", flush=True)
+    I, X, Y = solver.image_new(_dev(m), op, raydir, nx, ny, fq)
+    I, X, Y = I.cpu().numpy(), X.cpu().numpy(), Y.cpu().numpy()
+    Ir, Xr, Yr = oracle.image_new(m, op, raydir, nx, ny, fq)
+    np.testing.assert_array_equal(X.view(np.uint64), Xr.view(np.uint64))
+    np.testing.assert_array_equal(Y.view(np.uint64), Yr.view(np.uint64))
+    assert np.isfinite(I).all()
+    scale = np.max(np.abs(Ir))
+    err = float(np.max(np.abs(I - Ir) / np.maximum(np.abs(Ir), 1e-6 * scale)))
+    print(f"  image max rel err {err:.3e}", flush=True)
+    assert err < RTOL

@@ -256,3 +256,58 @@ def test_reference_is_sensitive_to_the_erff_rounding(small_wind):
     assert 1e-6 < d < 1e-2, d
     np.testing.assert_array_equal(oracle.trace_counts(m, op, kind="cr"),
                                   oracle.trace_counts(m, op, kind="glibc"))
+
+
+@pytest.mark.parametrize("mwf,bound", [(0.5, 1.6e-5), (0.35, 1.2e-5), (0.10, 3.3e-7)])
+def test_new_imager_closed_form(mwf, bound):
+    """Pins the projection-surface imager (frame, start-point search, imaging
+    trace, forward elimination, I = 2 Su - I_bdy) against the reference's
+    new-imager benchmark closed form and its bounds
+    (constant_velocity_gradient_1D_new_imager.py:18-31, 85-160, 221-227)."""
+    from scipy.special import erf
+
+    ld = models.read_lamda(os.path.join(HERE, "golden", "lamda_test.txt"), num=0)
+    P, dx, dv, temp, nTT = 50, 1.0e4, 2.5e2, 45.0, 1.0e8
+    m = models.model_velocity_gradient_1d(ld, npoints=P, nrays=4, nquads=10,
+                                          max_width_fraction=mwf)
+    fcen = ld["frequency"][0]
+    dd = 1.0e4 / models.CC
+    fq = m.image_frequencies(fcen - fcen * dd, fcen + fcen * dd, m.nlines * m.nquads)
+    raydir = m.ray_direction[m.hnrays - 1]
+    assert tuple(raydir) == (0.0, 1.0, 0.0)
+    im, imx, imy = oracle.image_new(m, m.options(), raydir, 50, 50, fq)
+    nu = fq["nu"][0]
+    rs = np.sqrt(imx**2 + imy**2)
+    w = ld["weight"] * np.exp(-ld["energy"] / (models.KB * temp))
+    pop = w / w.sum() * nTT
+    hv = models.HH * fcen / (4.0 * math.pi)
+    chi = hv * (pop[0] * ld["Ba"][0] - pop[1] * ld["Bs"][0])
+    src = hv * ld["A"][0] * pop[1] / chi
+    dnu = fcen * math.sqrt(2.0 * models.KB * temp / (models.AMU * models.CC**2) * ld["inverse_mass"])
+    r_in, r_out = m.position[0, 0], m.position[-1, 0]
+    L = dx * P
+    vmax = dv / models.CC * P
+
+    def z_max(r, th):
+        if r >= r_out:
+            return 0.0
+        if th < math.asin(r_in / r):
+            return r * math.cos(th) - math.sqrt(r_in**2 - (r * math.sin(th)) ** 2)
+        return r * math.cos(th) + math.sqrt(L**2 - (r * math.sin(th)) ** 2)
+
+    def tau(f, r, th):
+        l = z_max(r, th)
+        arg = (f - fcen) / dnu
+        fct = vmax * f / dnu
+        up = chi * L / (fct * dnu) * 0.5 * (erf(arg) - erf(arg - fct * l / L))
+        down = chi * L / (-fct * dnu) * 0.5 * (erf(arg) - erf(arg + fct * l / L))
+        return up + down
+
+    def I_im(f, r):
+        f = f + vmax / models.CC * fcen  # as written in the reference test (vmax is already /c)
+        b = 2.0 * models.HH * f**3 / models.CC**2 / math.expm1(models.HH * f / (models.KB * models.T_CMB))
+        return src + (b - src) * math.exp(-tau(f, r, 0.5 * math.pi))
+
+    im_a = np.array([[I_im(f, r) for f in nu] for r in rs])
+    err = np.abs(rel(im, im_a))[:-1]
+    assert np.max(err) < bound, np.max(err)
