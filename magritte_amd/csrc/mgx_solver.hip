@@ -51,12 +51,39 @@ namespace {
 
 enum { ERR_CAP = 1, ERR_NO_NEXT = 2, ERR_OUT = 4 };
 
+/* global-address-space load (the pointers live in a struct, so the compiler
+   would otherwise emit flat loads) */
+#define MGX_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ T gld(const T *p)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return *(const MGX_GLOBAL T *)p;
+#else
+    return *p;
+#endif
+}
+
+/* Per-point record read by the walk and the segments: one 64-byte line. */
+struct PointRec {
+    double x, y, z;    /* position */
+    double vx, vy, vz; /* velocity (fraction of c) */
+    double dshift_max; /* Model::dshift_max */
+    double ub;         /* sqrt(2kT/(amu c^2) * max_inverse_mass + vturb2) */
+};
+
+/* Per-point integers: neighbour count and offset, boundary index (P = interior). */
+struct PointIdx {
+    uint32_t n_nbs, cum, p2b, pad;
+};
+
 /* Device copy of the model view plus the per-call derived arrays. */
 struct KParams {
     mgx_model m;
     mgx_options op;
     const double4 *lpack; /* (P, L): inverse_width, emissivity, opacity, emissivity/opacity */
-    const double *ub;     /* (P): sqrt(2kT/(amu c^2) * max_inverse_mass + vturb2) */
+    const PointRec *pts;  /* (P) */
+    const PointIdx *pidx; /* (P) */
     double log_max_interp; /* (double)logf((float)max_interpolation_diff) */
     uint32_t cap;         /* LDS walk capacity (steps per half ray) */
     uint32_t nslots;      /* number of 32-lane line slots = L * ceil(Q/32) */
@@ -83,21 +110,21 @@ __device__ inline double ray_weight(const mgx_model &m, uint32_t o, uint32_t r)
 /* ------------------------------------------------------------------------- */
 /* Doppler shift (Geometry::get_shift<CoMoving>, geometry.tpp:204-207, 379-396) */
 
-__device__ inline double shift_comoving(const mgx_model &m, const double *origin,
+__device__ inline double shift_comoving(const KParams &kp, const double *origin,
                                         const double *ovel, const double *dir, uint32_t c,
                                         double Z)
 {
-    if (m.spherical_symmetry) {
+    const PointRec pc = gld(&kp.pts[c]);
+    if (kp.m.spherical_symmetry) {
         if (dot3(origin, origin) == 0.0) return 1.0;
-        const double *pc = &m.position[3 * (size_t)c];
-        if (pc[0] == 0.0) return 1.0 - dot3(ovel, dir);
+        if (pc.x == 0.0) return 1.0 - dot3(ovel, dir);
         const double Rcos_plus_Z = dot3(origin, dir) + Z;
-        return 1.0 + (m.velocity[3 * (size_t)c] * Rcos_plus_Z / pc[0] - dot3(ovel, dir));
+        return 1.0 + (pc.vx * Rcos_plus_Z / pc.x - dot3(ovel, dir));
     }
     double dv[3];
-    dv[0] = m.velocity[3 * (size_t)c + 0] - ovel[0];
-    dv[1] = m.velocity[3 * (size_t)c + 1] - ovel[1];
-    dv[2] = m.velocity[3 * (size_t)c + 2] - ovel[2];
+    dv[0] = pc.vx - ovel[0];
+    dv[1] = pc.vy - ovel[1];
+    dv[2] = pc.vz - ovel[2];
     return 1.0 + dot3(dv, dir);
 }
 
@@ -122,21 +149,21 @@ __device__ inline void wave_argmin(double &d2, uint32_t &idx, double &zn)
 }
 
 /* get_next_general_geometry<Defaulttracer> (geometry.tpp:14-44), all lanes return the same */
-__device__ inline uint32_t next_general(const mgx_model &m, const double *origin,
+__device__ inline uint32_t next_general(const KParams &kp, const double *origin,
                                         const double *dir, uint32_t c, double &Z, double &dZ)
 {
     const uint32_t lane = threadIdx.x & (MGX_WAVE - 1);
-    const uint32_t n_nbs = m.n_neighbors[c];
-    const uint32_t cum = m.cum_n_neighbors[c];
+    const PointIdx pi = gld(&kp.pidx[c]);
     double best_d2 = __builtin_inf();
     uint32_t best_i = 0xffffffffu;
     double best_zn = 0.0;
-    for (uint32_t i = lane; i < n_nbs; i += MGX_WAVE) {
-        const uint32_t n = m.neighbors[cum + i];
+    for (uint32_t i = lane; i < pi.n_nbs; i += MGX_WAVE) {
+        const uint32_t n = gld(&kp.m.neighbors[pi.cum + i]);
+        const PointRec pn = gld(&kp.pts[n]);
         double R[3];
-        R[0] = m.position[3 * (size_t)n + 0] - origin[0];
-        R[1] = m.position[3 * (size_t)n + 1] - origin[1];
-        R[2] = m.position[3 * (size_t)n + 2] - origin[2];
+        R[0] = pn.x - origin[0];
+        R[1] = pn.y - origin[1];
+        R[2] = pn.z - origin[2];
         const double Z_new = dot3(R, dir);
         if (Z_new > Z) {
             const double d2 = dot3(R, R) - Z_new * Z_new;
@@ -148,14 +175,20 @@ __device__ inline uint32_t next_general(const mgx_model &m, const double *origin
         }
     }
     wave_argmin(best_d2, best_i, best_zn);
-    if (best_i == 0xffffffffu) return m.npoints;
+    if (best_i == 0xffffffffu) return kp.m.npoints;
     dZ = best_zn - Z;
     Z += dZ;
-    return m.neighbors[cum + best_i];
+    return gld(&kp.m.neighbors[pi.cum + best_i]);
+}
+
+__device__ inline double sqnorm_point(const KParams &kp, uint32_t p)
+{
+    const PointRec r = gld(&kp.pts[p]);
+    return r.x * r.x + r.y * r.y + r.z * r.z;
 }
 
 /* get_next_spherical_symmetry (geometry.tpp:118-159), evaluated redundantly by every lane */
-__device__ inline uint32_t next_spherical(const mgx_model &m, const double *origin,
+__device__ inline uint32_t next_spherical(const KParams &kp, const double *origin,
                                           const double *dir, uint32_t c, double &Z, double &dZ)
 {
     uint32_t next;
@@ -164,24 +197,26 @@ __device__ inline uint32_t next_spherical(const mgx_model &m, const double *orig
     const double Rsin2 = R2 - Rcos * Rcos;
     const double Rcos_plus_Z = Rcos + Z;
     if (Z < -Rcos) {
-        if (c <= 0) return m.npoints;
-        const double *pm = &m.position[3 * (size_t)(c - 1)];
-        if (dot3(pm, pm) >= Rsin2) {
+        if (c <= 0) return kp.m.npoints;
+        if (sqnorm_point(kp, c - 1) >= Rsin2) {
             next = c - 1;
-            const double *pn = &m.position[3 * (size_t)next];
-            dZ = -sqrt(dot3(pn, pn) - Rsin2) - Rcos_plus_Z;
+            dZ = -sqrt(sqnorm_point(kp, next) - Rsin2) - Rcos_plus_Z;
         } else {
             next = c;
             dZ = -2.0 * Rcos_plus_Z;
         }
     } else {
-        if (c >= m.npoints - 1) return m.npoints;
+        if (c >= kp.m.npoints - 1) return kp.m.npoints;
         next = c + 1;
-        const double *pn = &m.position[3 * (size_t)next];
-        dZ = +sqrt(dot3(pn, pn) - Rsin2) - Rcos_plus_Z;
+        dZ = +sqrt(sqnorm_point(kp, next) - Rsin2) - Rcos_plus_Z;
     }
     Z += dZ;
     return next;
+}
+
+__device__ inline bool on_boundary(const KParams &kp, uint32_t p)
+{
+    return gld(&kp.pidx[p]).p2b != kp.m.npoints;
 }
 
 /*
@@ -195,29 +230,31 @@ __device__ int walk_half(const KParams &kp, uint32_t o, uint32_t r, uint32_t *s_
 {
     const mgx_model &m = kp.m;
     const uint32_t lane = threadIdx.x & (MGX_WAVE - 1);
-    const double *origin = &m.position[3 * (size_t)o];
-    const double *ovel = &m.velocity[3 * (size_t)o];
-    const double *dir = ray_dir(m, o, r);
+    const PointRec po = gld(&kp.pts[o]);
+    const double origin[3] = {po.x, po.y, po.z};
+    const double ovel[3] = {po.vx, po.vy, po.vz};
+    const double *dp = ray_dir(m, o, r);
+    const double dir[3] = {gld(&dp[0]), gld(&dp[1]), gld(&dp[2])};
     double Z = 0.0, dZ = 0.0;
     uint32_t crt = o;
     nsteps = 0;
-    shift0 = shift_comoving(m, origin, ovel, dir, o, 0.0);
+    shift0 = shift_comoving(kp, origin, ovel, dir, o, 0.0);
     for (;;) {
-        const uint32_t nxt = m.spherical_symmetry ? next_spherical(m, origin, dir, crt, Z, dZ)
-                                                  : next_general(m, origin, dir, crt, Z, dZ);
+        const uint32_t nxt = m.spherical_symmetry ? next_spherical(kp, origin, dir, crt, Z, dZ)
+                                                  : next_general(kp, origin, dir, crt, Z, dZ);
         if (nxt >= m.npoints) {
             if (nsteps == 0) break; /* nothing in this direction */
             return ERR_NO_NEXT;
         }
         if (nsteps >= kp.cap) return ERR_CAP;
-        const double sh = shift_comoving(m, origin, ovel, dir, nxt, Z);
+        const double sh = shift_comoving(kp, origin, ovel, dir, nxt, Z);
         if (lane == 0) {
             s_nxt[nsteps] = nxt;
             s_dZ[nsteps] = dZ;
             s_shift[nsteps] = sh;
         }
         nsteps++;
-        if (m.point2boundary[nxt] != m.npoints) break;
+        if (on_boundary(kp, nxt)) break;
         crt = nxt;
     }
     return 0;
@@ -229,8 +266,8 @@ __device__ int walk_half(const KParams &kp, uint32_t o, uint32_t r, uint32_t *s_
 __device__ inline uint32_t n_interp_for_line(const KParams &kp, uint32_t l, uint32_t c, uint32_t n)
 {
     const uint32_t L = kp.m.nlines;
-    const double curr_source = kp.m.interp_crit[(size_t)c * L + l];
-    const double next_source = kp.m.interp_crit[(size_t)n * L + l];
+    const double curr_source = gld(&kp.m.interp_crit[(size_t)c * L + l]);
+    const double next_source = gld(&kp.m.interp_crit[(size_t)n * L + l]);
     const double source_diff = (double)fabsf(mgx_logf((float)(next_source / curr_source)));
     const uint32_t k = (uint32_t)ceil(source_diff / kp.log_max_interp);
     return (k > 1) ? k : 1;
@@ -305,9 +342,10 @@ __device__ inline double planck(double temp, double freq)
            (double)mgx_expm1f((float)(MGX_HH_OVER_KB * freq / temp));
 }
 
-__device__ inline double boundary_intensity(const mgx_model &m, uint32_t p, double freq)
+__device__ inline double boundary_intensity(const KParams &kp, uint32_t p, double freq)
 {
-    const uint32_t b = m.point2boundary[p];
+    const mgx_model &m = kp.m;
+    const uint32_t b = gld(&kp.pidx[p]).p2b;
     const int32_t bc = (b < m.nboundary) ? m.boundary_condition[b] : MGX_BC_CMB;
     if (bc == MGX_BC_ZERO) return 0.0;
     if (bc == MGX_BC_THERMAL) return planck(m.boundary_temperature[b], freq);
@@ -378,7 +416,7 @@ __device__ inline void line_window(const mgx_model &m, const OwnLine &ol, double
 
 __device__ inline double4 lp(const KParams &kp, uint32_t p, uint32_t l)
 {
-    return kp.lpack[(size_t)p * kp.m.nlines + l];
+    return gld(&kp.lpack[(size_t)p * kp.m.nlines + l]);
 }
 
 /* get_eta_and_chi_interpolated<approx> (solver.tpp:1177-1202, 1260-1302, 1345-1351) */
@@ -399,7 +437,7 @@ __device__ void eta_chi_interp(const KParams &kp, const OwnLine &ol, uint32_t p1
     chi = kp.op.min_opacity;
     uint32_t lo = 0, hi = m.nlines;
     if (APPROX == MGX_APPROX_CLOSE_LINES) {
-        const double W = kp.op.max_distance_opacity_contribution * (freq * kp.ub[p1]);
+        const double W = kp.op.max_distance_opacity_contribution * (freq * gld(&kp.pts[p1].ub));
         line_window(m, ol, freq - W, freq + W, lo, hi);
     }
     for (uint32_t s = lo; s < hi; s++) {
@@ -476,8 +514,8 @@ __device__ void S_dtau_line_integrated(const KParams &kp, const double *erf_tab,
             right_freq = currfreq;
             left_freq = nextfreq;
         }
-        const double cb = kp.op.max_distance_opacity_contribution * (right_freq * kp.ub[cp]);
-        const double nb = kp.op.max_distance_opacity_contribution * (right_freq * kp.ub[np]);
+        const double cb = kp.op.max_distance_opacity_contribution * (right_freq * gld(&kp.pts[cp].ub));
+        const double nb = kp.op.max_distance_opacity_contribution * (right_freq * gld(&kp.pts[np].ub));
         const double W = (cb < nb) ? nb : cb;
         line_window(m, ol, left_freq - W, right_freq + W, lo, hi);
     }
@@ -533,7 +571,7 @@ __device__ inline void segment(const KParams &kp, const double *erf_tab, const O
 {
     const double dshift = nx.shift - cu.shift;
     const double dshift_abs = fabs(dshift);
-    const double a = kp.m.dshift_max[cu.nr], b = kp.m.dshift_max[nx.nr];
+    const double a = gld(&kp.pts[cu.nr].dshift_max), b = gld(&kp.pts[nx.nr].dshift_max);
     const double dshift_max = (b < a) ? b : a;
     const double cf = freq * cu.shift, nf = freq * nx.shift;
     if (dshift_abs > dshift_max) {
@@ -581,9 +619,14 @@ __device__ inline void hi_load_step(const KParams &kp, HalfIter &it, uint32_t l,
     it.shift_crt = (s == 0) ? shift0 : s_shift[s - 1];
     it.m_s = mtab ? (uint32_t)mtab[s] : n_interp_for_line(kp, l, it.crt, it.nxt);
     it.i = it.m_s;
-    const double dshift = it.shift_nxt - it.shift_crt;
-    it.dZ = (it.m_s > 1) ? s_dZ[s] / it.m_s : s_dZ[s];
-    it.dshift_interpl = dshift / it.m_s;
+    if (it.m_s > 1) {
+        const double dshift = it.shift_nxt - it.shift_crt;
+        it.dZ = s_dZ[s] / it.m_s;
+        it.dshift_interpl = dshift / it.m_s;
+    } else {
+        it.dZ = s_dZ[s];
+        it.dshift_interpl = 0.0;
+    }
 }
 
 __device__ inline Entry hi_entry(const HalfIter &it, uint32_t o)
@@ -633,10 +676,93 @@ __host__ __device__ inline size_t walk_lds_bytes(uint32_t cap)
     return 8 * ((2 * (size_t)cap * 4 + 7) / 8) + 4 * (size_t)cap * 8 + 4 * (size_t)cap * 2;
 }
 
-template <int APPROX, int PREC>
-__global__ __launch_bounds__(MGX_WAVE, MGX_SOLVE_WAVES_PER_SIMD) void feautrier_kernel(KParams kp, double *u, double *J,
-                                                             double *lam)
+/* dynamic LDS of the solve kernel: walk buffers + J (F) and Lambda (L) accumulators */
+__host__ __device__ inline size_t solve_lds_bytes(uint32_t cap, uint32_t F, uint32_t L)
 {
+    return ((walk_lds_bytes(cap) + 7) / 8) * 8 + 8 * ((size_t)F + L);
+}
+
+/*
+ * One half ray of the two-sided elimination.  Walking from the boundary end
+ * toward the origin, both halves run the same recurrence with the sides of
+ * each row swapped: forward (rr half) X = F, XI = 1/(1+F), Y = Su; backward
+ * (antipod half) X = G, XI = 1/(1+G), Y = w.  Boundary row:
+ *   Y = (S + 2 I_bdy/dtau) / (1 + 2/dtau + 2/dtau^2),  X = (1 + 2/dtau) dtau^2 / 2;
+ * interior row with the outer and inner segment depths d_out, d_in:
+ *   q_out = (d_out + d_in)/2 d_out, q_in = (d_out + d_in)/2 d_in,
+ *   X <- ((X XI)/q_out + 1) q_in,  XI <- 1/(1+X),  Y <- (Y/q_out + S) XI q_in
+ * (solver.tpp:2234-2318 forward; the G recurrence of :2296-2305 backward).
+ */
+struct Sweep {
+    double X, XI, Y;
+    double prev_dtau;  /* depth of the last segment crossed (the innermost one at the end) */
+    double S_inner;    /* source of the innermost entry's row from the last segment */
+    double S_prev_row; /* source value carried to the next row (backward half) */
+    bool started;
+};
+
+template <int APPROX, int PREC>
+__device__ inline void sweep_half(const KParams &kp, const double *s_erf, const OwnLine &ol,
+                                  bool backward, uint32_t S, uint32_t o, double shift0,
+                                  const uint32_t *s_nxt, const double *s_dZ, const double *s_sh,
+                                  const uint16_t *mtab, uint32_t l, double freq, Sweep &sw,
+                                  PointEC &ec_centre)
+{
+    HalfIter it;
+    it.s = (int)S - 1;
+    hi_load_step(kp, it, l, o, shift0, s_nxt, s_dZ, s_sh, mtab);
+    Entry outer = hi_entry(it, o);
+    PointEC ec_out = {0.0, 0.0, false};
+    sw.started = false;
+    for (;;) {
+        const double dz = hi_advance(kp, it, l, o, shift0, s_nxt, s_dZ, s_sh, mtab);
+        Entry inner = hi_entry(it, o);
+        PointEC ec_in = (it.s < 0) ? ec_centre : PointEC{0.0, 0.0, false};
+        /* the segment runs from the rr-side entry to the antipod-side entry */
+        const Entry &cu = backward ? inner : outer;
+        const Entry &nx = backward ? outer : inner;
+        PointEC &ec_cu = backward ? ec_in : ec_out;
+        PointEC &ec_nx = backward ? ec_out : ec_in;
+        double dtau, S_cu, S_nx;
+        segment<APPROX, PREC>(kp, s_erf, ol, cu, nx, dz, freq, l, ec_cu, ec_nx, dtau, S_cu, S_nx);
+        const double S_out = backward ? S_nx : S_cu; /* row "outer" from this segment */
+        const double S_in = backward ? S_cu : S_nx;
+        /* row "outer": the boundary row first, then interior rows; the backward
+           half's interior row takes its source from the segment beyond it */
+        const double S_row = backward ? (sw.started ? sw.S_prev_row : S_out) : S_out;
+        if (!sw.started) {
+            const double I_b = boundary_intensity(kp, outer.nr, freq * outer.shift);
+            const double idt = rcp(dtau);
+            const double B_min = fma(2.0, idt, 1.0);
+            const double B = fma(2.0 * idt, idt, B_min);
+            sw.Y = fma(2.0 * I_b, idt, S_row) * rcp(B);
+            sw.X = 0.5 * B_min * dtau * dtau;
+            sw.XI = rcp(1.0 + sw.X);
+            sw.started = true;
+        } else {
+            const double avg = 0.5 * (sw.prev_dtau + dtau);
+            const double q_out = avg * sw.prev_dtau;
+            const double q_in = avg * dtau;
+            const double r = rcp(q_out);
+            sw.X = fma(r * sw.X, sw.XI, 1.0) * q_in;
+            sw.XI = rcp(1.0 + sw.X);
+            sw.Y = fma(r, sw.Y, S_row) * sw.XI * q_in;
+        }
+        sw.prev_dtau = dtau;
+        sw.S_inner = S_in;
+        sw.S_prev_row = S_in;
+        outer = inner;
+        ec_out = ec_in;
+        if (it.s < 0) break;
+    }
+    ec_centre = ec_out;
+}
+
+template <int APPROX, int PREC>
+__global__ __launch_bounds__(MGX_WAVE, MGX_SOLVE_WAVES_PER_SIMD) void feautrier_kernel(
+    const KParams *__restrict__ kpp, double *u, double *J, double *lam)
+{
+    const KParams &kp = *kpp;
     extern __shared__ unsigned char smem[];
     __shared__ double s_erf[MGX_ERF_NINT * (MGX_ERF_DEG + 1)];
     for (uint32_t i = threadIdx.x; i < MGX_ERF_NINT * (MGX_ERF_DEG + 1); i += MGX_WAVE)
@@ -645,6 +771,7 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_SOLVE_WAVES_PER_SIMD) void feautrier_
     const uint32_t o = blockIdx.x;
     const uint32_t lane = threadIdx.x;
     const uint32_t cap = kp.cap;
+    const uint32_t P = m.npoints, F = m.nfreqs, L = m.nlines, Q = m.nquads;
     uint32_t *s_nxt0 = reinterpret_cast<uint32_t *>(smem);
     uint32_t *s_nxt1 = s_nxt0 + cap;
     double *s_dZ0 = reinterpret_cast<double *>(smem + 8 * ((2 * cap * 4 + 7) / 8));
@@ -652,8 +779,10 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_SOLVE_WAVES_PER_SIMD) void feautrier_
     double *s_sh0 = s_dZ1 + cap;
     double *s_sh1 = s_sh0 + cap;
     uint16_t *s_m = reinterpret_cast<uint16_t *>(s_sh1 + cap); /* [line j][half h][step] */
+    double *s_J = reinterpret_cast<double *>(smem + ((walk_lds_bytes(cap) + 7) / 8) * 8);
+    double *s_L = s_J + F;
+    for (uint32_t i = lane; i < F + L; i += MGX_WAVE) s_J[i] = 0.0;
 
-    const uint32_t P = m.npoints, F = m.nfreqs, L = m.nlines, Q = m.nquads;
     const size_t PF = (size_t)P * F;
 
     for (uint32_t rr = 0; rr < m.hnrays; rr++) {
@@ -694,8 +823,6 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_SOLVE_WAVES_PER_SIMD) void feautrier_
             const uint32_t l = jj ? lB : lA;
             const uint32_t z = (slot % kp.slots_per_line) * 32 + (lane & 31);
             const bool active = (slot < kp.nslots) && (z < Q);
-            const uint16_t *mt0 = s_m + (jj * 2 + 0) * cap;
-            const uint16_t *mt1 = s_m + (jj * 2 + 1) * cap;
             double Lval = 0.0;
             if (active) {
                 const OwnLine ol = own_line(m, l);
@@ -704,137 +831,59 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_SOLVE_WAVES_PER_SIMD) void feautrier_
                 const double freq = m.nu[of];
                 double uc;
                 if (S0 == 0 && S1 == 0) {
-                    uc = boundary_intensity(m, o, freq);
-                    J[of] = ((rr == 0) ? 0.0 : J[of]) + 2.0 * w * uc;
+                    uc = boundary_intensity(kp, o, freq);
+                    s_J[f] = s_J[f] + 2.0 * w * uc;
                 } else {
-                    /* ---- backward sweep over the antipod half (last -> centre+1) ---- */
-                    double GP = 0.0, wv = 0.0, dtau_c = 0.0, S_c = 0.0;
                     PointEC ec_cent = {0.0, 0.0, false};
-                    if (S1 > 0) {
-                        HalfIter it;
-                        it.s = (int)S1 - 1;
-                        hi_load_step(kp, it, l, o, sh01, s_nxt1, s_dZ1, s_sh1, mt1);
-                        Entry outer = hi_entry(it, o);
-                        PointEC ec_out = {0.0, 0.0, false};
-                        bool is_last = true;
-                        double prev_dtau = 0.0, prev_Scurr = 0.0;
-                        for (;;) {
-                            const double dz = hi_advance(kp, it, l, o, sh01, s_nxt1, s_dZ1, s_sh1, mt1);
-                            const Entry inner = hi_entry(it, o);
-                            PointEC ec_in = {0.0, 0.0, false};
-                            double dtau, S_in, S_out;
-                            segment<APPROX, PREC>(kp, s_erf, ol, inner, outer, dz, freq, l, ec_in,
-                                                  ec_out, dtau, S_in, S_out);
-                            if (is_last) {
-                                const double I_l = boundary_intensity(m, outer.nr, freq * outer.shift);
-                                const double idt = rcp(dtau);
-                                const double Bl_min_Al = fma(2.0, idt, 1.0);
-                                const double Bl = fma(2.0 * idt, idt, Bl_min_Al);
-                                const double GG = 0.5 * Bl_min_Al * dtau * dtau;
-                                const double GI = rcp(1.0 + GG);
-                                GP = GG * GI;
-                                wv = fma(2.0 * I_l, idt, S_out) * rcp(Bl);
-                                is_last = false;
-                            } else {
-                                /* row "outer": A side = this segment, C side = previous one */
-                                const double dtau_avg = 0.5 * (dtau + prev_dtau);
-                                const double iA = dtau_avg * dtau;
-                                const double iC = dtau_avg * prev_dtau;
-                                const double C = rcp(iC);
-                                const double GG = fma(C, GP, 1.0) * iA;
-                                const double GI = rcp(1.0 + GG);
-                                GP = GG * GI;
-                                wv = fma(C, wv, prev_Scurr) * GI * iA;
-                            }
-                            prev_dtau = dtau;
-                            prev_Scurr = S_in;
-                            outer = inner;
-                            ec_out = ec_in;
-                            if (it.s < 0) break;
-                        }
-                        dtau_c = prev_dtau;
-                        S_c = prev_Scurr;
-                        ec_cent = ec_out;
-                    }
-                    /* ---- forward sweep over the rr half (first -> centre) ---- */
-                    double FF = 0.0, FI = 0.0, Su = 0.0, iC_c = 0.0;
-                    double prev_dtau = 0.0, prev_Snext = 0.0;
-                    bool is_first = true;
-                    if (S0 > 0) {
-                        HalfIter it;
-                        it.s = (int)S0 - 1;
-                        hi_load_step(kp, it, l, o, sh00, s_nxt0, s_dZ0, s_sh0, mt0);
-                        Entry outer = hi_entry(it, o);
-                        PointEC ec_out = {0.0, 0.0, false};
-                        for (;;) {
-                            const double dz = hi_advance(kp, it, l, o, sh00, s_nxt0, s_dZ0, s_sh0, mt0);
-                            const Entry inner = hi_entry(it, o);
-                            PointEC ec_in = (it.s < 0) ? ec_cent : PointEC{0.0, 0.0, false};
-                            double dtau, S_out, S_in;
-                            segment<APPROX, PREC>(kp, s_erf, ol, outer, inner, dz, freq, l, ec_out,
-                                                  ec_in, dtau, S_out, S_in);
-                            if (is_first) {
-                                const double I_f = boundary_intensity(m, outer.nr, freq * outer.shift);
-                                const double idt = rcp(dtau);
-                                const double Bf_min_Cf = fma(2.0, idt, 1.0);
-                                const double Bf = fma(2.0 * idt, idt, Bf_min_Cf);
-                                Su = fma(2.0 * I_f, idt, S_out) * rcp(Bf);
-                                FF = 0.5 * Bf_min_Cf * dtau * dtau;
-                                FI = rcp(1.0 + FF);
-                                is_first = false;
-                            } else {
-                                /* row "outer": A side = previous segment, C side = this one */
-                                const double dtau_avg = 0.5 * (prev_dtau + dtau);
-                                const double iA = dtau_avg * prev_dtau;
-                                const double iC = dtau_avg * dtau;
-                                const double A = rcp(iA);
-                                FF = fma(A * FF, FI, 1.0) * iC;
-                                FI = rcp(1.0 + FF);
-                                Su = fma(A, Su, S_out) * FI * iC;
-                            }
-                            prev_dtau = dtau;
-                            prev_Snext = S_in;
-                            outer = inner;
-                            ec_out = ec_in;
-                            if (it.s < 0) break;
-                        }
-                    }
-                    /* ---- join at the origin ---- */
+                    Sweep bw, fw;
+                    bw.started = fw.started = false;
+                    if (S1 > 0)
+                        sweep_half<APPROX, PREC>(kp, s_erf, ol, true, S1, o, sh01, s_nxt1, s_dZ1,
+                                                 s_sh1, s_m + (jj * 2 + 1) * cap, l, freq, bw,
+                                                 ec_cent);
+                    if (S0 > 0)
+                        sweep_half<APPROX, PREC>(kp, s_erf, ol, false, S0, o, sh00, s_nxt0, s_dZ0,
+                                                 s_sh0, s_m + (jj * 2 + 0) * cap, l, freq, fw,
+                                                 ec_cent);
                     double Lc;
                     if (S1 > 0) {
-                        if (is_first) { /* the origin is the first row */
-                            const double I_f = boundary_intensity(m, o, freq);
+                        /* row of the origin: forward quantities at the origin, joined with
+                           the backward ones of its antipod-side neighbour */
+                        const double dtau_c = bw.prev_dtau;
+                        const double S_c = bw.S_inner;
+                        double FF, Su, iC_c;
+                        if (S0 == 0) { /* the origin is the first row */
+                            const double I_f = boundary_intensity(kp, o, freq);
                             const double idt = rcp(dtau_c);
-                            const double Cf = 2.0 * idt * idt;
+                            const double B_min = fma(2.0, idt, 1.0);
+                            const double B = fma(2.0 * idt, idt, B_min);
                             iC_c = 0.5 * dtau_c * dtau_c;
-                            const double Bf_min_Cf = fma(2.0, idt, 1.0);
-                            const double Bf = Bf_min_Cf + Cf;
-                            Su = fma(2.0 * I_f, idt, S_c) * rcp(Bf);
-                            FF = 0.5 * Bf_min_Cf * dtau_c * dtau_c;
+                            Su = fma(2.0 * I_f, idt, S_c) * rcp(B);
+                            FF = 0.5 * B_min * dtau_c * dtau_c;
                         } else {
-                            const double dtau_avg = 0.5 * (prev_dtau + dtau_c);
-                            const double iA = dtau_avg * prev_dtau;
-                            iC_c = dtau_avg * dtau_c;
-                            const double A = rcp(iA);
-                            FF = fma(A * FF, FI, 1.0) * iC_c;
-                            FI = rcp(1.0 + FF);
-                            Su = fma(A, Su, S_c) * FI * iC_c;
+                            const double avg = 0.5 * (fw.prev_dtau + dtau_c);
+                            const double r = rcp(avg * fw.prev_dtau);
+                            iC_c = avg * dtau_c;
+                            FF = fma(r * fw.X, fw.XI, 1.0) * iC_c;
+                            const double FI = rcp(1.0 + FF);
+                            Su = fma(r, fw.Y, S_c) * FI * iC_c;
                         }
+                        const double GP = bw.X * bw.XI;
                         const double rden = rcp(FF + GP);
-                        uc = fma(Su, 1.0 + FF, wv) * rden;
+                        uc = fma(Su, 1.0 + FF, bw.Y) * rden;
                         Lc = iC_c * rden;
                     } else { /* the origin is the last row */
-                        const double I_l = boundary_intensity(m, o, freq);
-                        const double idt = rcp(prev_dtau);
+                        const double I_l = boundary_intensity(kp, o, freq);
+                        const double idt = rcp(fw.prev_dtau);
                         const double A_l = 2.0 * idt * idt;
-                        const double Bl_min_Al = fma(2.0, idt, 1.0);
-                        const double Bl = Bl_min_Al + A_l;
-                        const double denominator = rcp(fma(Bl, FF, Bl_min_Al));
-                        const double Sl = fma(2.0 * I_l, idt, prev_Snext);
-                        uc = fma(A_l, Su, Sl) * (1.0 + FF) * denominator;
-                        Lc = (1.0 + FF) * denominator;
+                        const double B_min = fma(2.0, idt, 1.0);
+                        const double B = B_min + A_l;
+                        const double den = rcp(fma(B, fw.X, B_min));
+                        const double Sl = fma(2.0 * I_l, idt, fw.S_inner);
+                        uc = fma(A_l, fw.Y, Sl) * (1.0 + fw.X) * den;
+                        Lc = (1.0 + fw.X) * den;
                     }
-                    J[of] = ((rr == 0) ? 0.0 : J[of]) + uc * 2.0 * w;
+                    s_J[f] = s_J[f] + uc * 2.0 * w;
                     /* ---- Lambda diagonal (update_Lambda, solver.tpp:1761-1800) ---- */
                     if (lam && m.f_in_line[f]) {
                         const uint32_t spec = m.f_species[f];
@@ -874,23 +923,25 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_SOLVE_WAVES_PER_SIMD) void feautrier_
                         const uint32_t slot_h = slot0 + h;
                         if (slot_h >= kp.nslots) break;
                         const uint32_t ll = slot_h / kp.slots_per_line;
-                        const size_t idx = (size_t)o * L + ll;
-                        const bool fresh = (rr == 0) && (slot_h % kp.slots_per_line == 0);
-                        lam[idx] = (fresh ? 0.0 : lam[idx]) + (h ? v_hi : v);
+                        s_L[ll] = s_L[ll] + (h ? v_hi : v);
                     }
                 }
             }
             __syncthreads();
         }
     }
+    for (uint32_t i = lane; i < F; i += MGX_WAVE) J[(size_t)o * F + i] = s_J[i];
+    if (lam)
+        for (uint32_t i = lane; i < L; i += MGX_WAVE) lam[(size_t)o * L + i] = s_L[i];
 }
 
 /* ------------------------------------------------------------------------- */
 /* trace kernels: counts and dumps of expanded lists                          */
 
 /* one wave per (rr, o): n_tot for every line */
-__global__ __launch_bounds__(MGX_WAVE) void trace_counts_kernel(KParams kp, uint32_t *n_tot)
+__global__ __launch_bounds__(MGX_WAVE) void trace_counts_kernel(const KParams *__restrict__ kpp, uint32_t *n_tot)
 {
+    const KParams &kp = *kpp;
     extern __shared__ unsigned char smem[];
     const mgx_model &m = kp.m;
     const uint32_t o = blockIdx.x, rr = blockIdx.y, lane = threadIdx.x;
@@ -924,8 +975,9 @@ __global__ __launch_bounds__(MGX_WAVE) void trace_counts_kernel(KParams kp, uint
 }
 
 /* one wave per (rr, o): walk and trace statistics (S, neighbours, T, T_interp) */
-__global__ __launch_bounds__(MGX_WAVE) void trace_stats_kernel(KParams kp, unsigned long long *stats)
+__global__ __launch_bounds__(MGX_WAVE) void trace_stats_kernel(const KParams *__restrict__ kpp, unsigned long long *stats)
 {
+    const KParams &kp = *kpp;
     extern __shared__ unsigned char smem[];
     const mgx_model &m = kp.m;
     const uint32_t o = blockIdx.x, rr = blockIdx.y, lane = threadIdx.x;
@@ -983,10 +1035,11 @@ __global__ __launch_bounds__(MGX_WAVE) void trace_stats_kernel(KParams kp, unsig
 
 /* one wave per selected (rr, o, l); lane 0 writes the expanded list */
 __global__ __launch_bounds__(MGX_WAVE) void trace_rays_kernel(
-    KParams kp, const uint32_t *sel_rr, const uint32_t *sel_o, const uint32_t *sel_l,
+    const KParams *__restrict__ kpp, const uint32_t *sel_rr, const uint32_t *sel_o, const uint32_t *sel_l,
     uint32_t capacity, uint32_t *count, uint32_t *centre_idx, uint32_t *nr, uint32_t *nr_interp,
     double *shift, double *interp_factor, double *dZ)
 {
+    const KParams &kp = *kpp;
     extern __shared__ unsigned char smem[];
     const mgx_model &m = kp.m;
     const uint32_t i = blockIdx.x, lane = threadIdx.x;
@@ -1256,9 +1309,10 @@ __device__ inline uint32_t n_interp_all(const KParams &kp, uint32_t c, uint32_t 
 }
 
 template <int APPROX, int PREC>
-__global__ __launch_bounds__(MGX_WAVE) void image_kernel(KParams kp, const ImageFrame *frp,
+__global__ __launch_bounds__(MGX_WAVE) void image_kernel(const KParams *__restrict__ kpp, const ImageFrame *frp,
                                                          double *I, double *ImX, double *ImY)
 {
+    const KParams &kp = *kpp;
     extern __shared__ unsigned char smem[];
     __shared__ double s_erf[MGX_ERF_NINT * (MGX_ERF_DEG + 1)];
     for (uint32_t i = threadIdx.x; i < MGX_ERF_NINT * (MGX_ERF_DEG + 1); i += MGX_WAVE)
@@ -1323,8 +1377,8 @@ __global__ __launch_bounds__(MGX_WAVE) void image_kernel(KParams kp, const Image
     {
         double dZ = 0.0;
         uint32_t crt = ip;
-        uint32_t nxt = m.spherical_symmetry ? next_spherical(m, origin, rd, crt, Z, dZ)
-                                            : next_general(m, origin, rd, crt, Z, dZ);
+        uint32_t nxt = m.spherical_symmetry ? next_spherical(kp, origin, rd, crt, Z, dZ)
+                                            : next_general(kp, origin, rd, crt, Z, dZ);
         bool accessed_non_boundary = false;
         if (nxt < m.npoints) {
             shift0 = shift_rest(m, origin, rd, crt, Z - dZ);
@@ -1349,8 +1403,8 @@ __global__ __launch_bounds__(MGX_WAVE) void image_kernel(KParams kp, const Image
                 if (!(interior || dot3(rel, rd) < 0)) break;
                 crt = nxt;
                 if (interior) accessed_non_boundary = true;
-                nxt = m.spherical_symmetry ? next_spherical(m, origin, rd, crt, Z, dZ)
-                                           : next_general(m, origin, rd, crt, Z, dZ);
+                nxt = m.spherical_symmetry ? next_spherical(kp, origin, rd, crt, Z, dZ)
+                                           : next_general(kp, origin, rd, crt, Z, dZ);
                 if (nxt >= m.npoints) break;
                 shift_nxt = shift_rest(m, origin, rd, nxt, Z);
             }
@@ -1370,13 +1424,13 @@ __global__ __launch_bounds__(MGX_WAVE) void image_kernel(KParams kp, const Image
         const double nu_ip = m.nu[(size_t)ip * F + f];
         double Ival;
         if (S == 0) {
-            Ival = boundary_intensity(m, ip, nu_ip);
+            Ival = boundary_intensity(kp, ip, nu_ip);
         } else {
             const double freq = nu_ip;
             const uint32_t line = m.f_line[f];
-            const OwnLine ol = own_line(m, m.sorted_line_map[upper_bound_d(m.sorted_line, m.nlines, freq) > 0
-                                                               ? upper_bound_d(m.sorted_line, m.nlines, freq) - 1
-                                                               : 0]);
+            const uint32_t ub_idx = upper_bound_d(m.sorted_line, m.nlines, freq);
+            const uint32_t own_l = m.sorted_line_map[ub_idx > 0 ? ub_idx - 1 : 0];
+            const OwnLine ol = own_line(m, own_l);
             HalfIter it;
             it.s = (int)S - 1;
             hi_load_step(kp, it, line, ip, shift0, s_nxt, s_dZ, s_sh, s_m);
@@ -1393,7 +1447,7 @@ __global__ __launch_bounds__(MGX_WAVE) void image_kernel(KParams kp, const Image
                 segment<APPROX, PREC>(kp, s_erf, ol, outer, inner, dz, freq, line, ec_out, ec_in,
                                       dtau, S_out, S_in);
                 if (is_first) {
-                    const double I_f = boundary_intensity(m, outer.nr, freq * outer.shift);
+                    const double I_f = boundary_intensity(kp, outer.nr, freq * outer.shift);
                     const double idt = rcp(dtau);
                     const double Bf_min_Cf = fma(2.0, idt, 1.0);
                     const double Bf = fma(2.0 * idt, idt, Bf_min_Cf);
@@ -1417,7 +1471,7 @@ __global__ __launch_bounds__(MGX_WAVE) void image_kernel(KParams kp, const Image
                 if (it.s < 0) break;
             }
             /* last row at the start point (image_feautrier_order_2, solver.tpp:2546-2559) */
-            const double I_l = boundary_intensity(m, ip, freq * shift_centre);
+            const double I_l = boundary_intensity(kp, ip, freq * shift_centre);
             const double idt = rcp(prev_dtau);
             const double A_l = 2.0 * idt * idt;
             const double Bl_min_Al = fma(2.0, idt, 1.0);
@@ -1425,7 +1479,7 @@ __global__ __launch_bounds__(MGX_WAVE) void image_kernel(KParams kp, const Image
             const double denominator = rcp(fma(Bl, FF, Bl_min_Al));
             const double Sl = fma(2.0 * I_l, idt, prev_Snext);
             const double Su_last = fma(A_l, Su, Sl) * (1.0 + FF) * denominator;
-            Ival = 2.0 * Su_last - boundary_intensity(m, ip, nu_ip);
+            Ival = 2.0 * Su_last - boundary_intensity(kp, ip, nu_ip);
         }
         I[(size_t)pix * F + f] = Ival;
     }
@@ -1434,7 +1488,7 @@ __global__ __launch_bounds__(MGX_WAVE) void image_kernel(KParams kp, const Image
 /* ------------------------------------------------------------------------- */
 /* preparation kernels                                                       */
 
-__global__ void pack_lines_kernel(const mgx_model m, double4 *lpack, double *ub)
+__global__ void pack_kernel(const mgx_model m, double4 *lpack, PointRec *pts, PointIdx *pidx)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t PL = (size_t)m.npoints * m.nlines;
@@ -1443,8 +1497,23 @@ __global__ void pack_lines_kernel(const mgx_model m, double4 *lpack, double *ub)
         lpack[i] = make_double4(m.inverse_width[i], e, k, e / k);
     }
     if (i < m.npoints) {
-        ub[i] = sqrt(MGX_TWO_KB_OVER_AMU_CC_SQUARED * m.max_inverse_mass * m.temperature[i] +
-                     m.vturb2[i]);
+        PointRec r;
+        r.x = m.position[3 * i + 0];
+        r.y = m.position[3 * i + 1];
+        r.z = m.position[3 * i + 2];
+        r.vx = m.velocity[3 * i + 0];
+        r.vy = m.velocity[3 * i + 1];
+        r.vz = m.velocity[3 * i + 2];
+        r.dshift_max = m.dshift_max[i];
+        r.ub = sqrt(MGX_TWO_KB_OVER_AMU_CC_SQUARED * m.max_inverse_mass * m.temperature[i] +
+                    m.vturb2[i]);
+        pts[i] = r;
+        PointIdx q;
+        q.n_nbs = m.n_neighbors[i];
+        q.cum = m.cum_n_neighbors[i];
+        q.p2b = m.point2boundary[i];
+        q.pad = 0;
+        pidx[i] = q;
     }
 }
 
@@ -1460,12 +1529,16 @@ __global__ void pack_lines_kernel(const mgx_model m, double4 *lpack, double *ub)
 
 struct DevBuffers {
     double4 *lpack = nullptr;
-    double *ub = nullptr;
+    PointRec *pts = nullptr;
+    PointIdx *pidx = nullptr;
     uint32_t *status = nullptr;
+    KParams *kp_dev = nullptr; /* device copy of the launch parameters */
     ~DevBuffers()
     {
+        if (kp_dev) (void)hipFree(kp_dev);
         if (lpack) (void)hipFree(lpack);
-        if (ub) (void)hipFree(ub);
+        if (pts) (void)hipFree(pts);
+        if (pidx) (void)hipFree(pidx);
         if (status) (void)hipFree(status);
     }
 };
@@ -1527,17 +1600,20 @@ int prepare(const mgx_model *m, const mgx_options *op, hipStream_t st, DevBuffer
 {
     const size_t PL = (size_t)m->npoints * m->nlines;
     HIP_TRY(hipMalloc(&b.lpack, PL * sizeof(double4)));
-    HIP_TRY(hipMalloc(&b.ub, (size_t)m->npoints * sizeof(double)));
+    HIP_TRY(hipMalloc(&b.pts, (size_t)m->npoints * sizeof(PointRec)));
+    HIP_TRY(hipMalloc(&b.pidx, (size_t)m->npoints * sizeof(PointIdx)));
     HIP_TRY(hipMalloc(&b.status, 2 * sizeof(uint32_t)));
+    HIP_TRY(hipMalloc(&b.kp_dev, sizeof(KParams)));
     HIP_TRY(hipMemsetAsync(b.status, 0, 2 * sizeof(uint32_t), st));
     const size_t n = PL > m->npoints ? PL : m->npoints;
-    hipLaunchKernelGGL(pack_lines_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, *m,
-                       b.lpack, b.ub);
+    hipLaunchKernelGGL(pack_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, *m,
+                       b.lpack, b.pts, b.pidx);
     HIP_TRY(hipGetLastError());
     kp.m = *m;
     kp.op = *op;
     kp.lpack = b.lpack;
-    kp.ub = b.ub;
+    kp.pts = b.pts;
+    kp.pidx = b.pidx;
     kp.log_max_interp = (double)mgx_logf((float)op->max_interpolation_diff);
     kp.slots_per_line = (m->nquads + 31) / 32;
     kp.nslots = m->nlines * kp.slots_per_line;
@@ -1559,21 +1635,28 @@ int read_status(hipStream_t st, const DevBuffers &b, uint32_t *need)
     return MGX_OK;
 }
 
+/* LDS per workgroup available to dynamic allocations (64 KiB minus the erf table) */
+const size_t kLdsBudget = 65536 - MGX_ERF_NINT * (MGX_ERF_DEG + 1) * sizeof(double);
+
 template <typename Launch>
-int launch_with_capacity(hipStream_t st, DevBuffers &b, KParams &kp, Launch launch)
+int launch_with_capacity(hipStream_t st, DevBuffers &b, KParams &kp, Launch launch,
+                         size_t extra_lds = 0)
 {
     uint32_t cap = 128;
     for (;;) {
         kp.cap = cap;
         HIP_TRY(hipMemsetAsync(b.status, 0, 2 * sizeof(uint32_t), st));
+        HIP_TRY(hipMemcpyAsync(b.kp_dev, &kp, sizeof(KParams), hipMemcpyHostToDevice, st));
         launch(walk_lds_bytes(cap));
         HIP_TRY(hipGetLastError());
         uint32_t need = 0;
         const int e = read_status(st, b, &need);
         if (e != MGX_E_RAY_LENGTH) return e;
-        if (cap >= kMaxCap)
-            return mgx::fail(MGX_E_RAY_LENGTH, "a ray needs more than %u walk steps", kMaxCap);
-        cap = cap * 2 > kMaxCap ? kMaxCap : cap * 2;
+        uint32_t max_cap = kMaxCap;
+        while (max_cap > 16 && walk_lds_bytes(max_cap) + 8 + extra_lds > kLdsBudget) max_cap -= 16;
+        if (cap >= max_cap)
+            return mgx::fail(MGX_E_RAY_LENGTH, "a ray needs more than %u walk steps", max_cap);
+        cap = cap * 2 > max_cap ? max_cap : cap * 2;
     }
 }
 
@@ -1595,37 +1678,38 @@ int mgx_feautrier_forward_device(const mgx_model *model, const mgx_options *opts
     DevBuffers b;
     KParams kp;
     if (int e = prepare(model, opts, st, b, kp)) return e;
-    return launch_with_capacity(st, b, kp, [&](size_t lds) {
+    return launch_with_capacity(st, b, kp, [&](size_t) {
         const dim3 grid(model->npoints), block(MGX_WAVE);
+        const size_t lds = solve_lds_bytes(kp.cap, model->nfreqs, model->nlines);
         timed(st, [&] {
             const bool fp64 = opts->precision == MGX_PREC_FP64;
             switch (opts->approx) {
             case MGX_APPROX_NONE:
                 if (fp64)
                     hipLaunchKernelGGL((feautrier_kernel<MGX_APPROX_NONE, MGX_PREC_FP64>), grid,
-                                       block, lds, st, kp, u, J, lambda_diag);
+                                       block, lds, st, b.kp_dev, u, J, lambda_diag);
                 else
                     hipLaunchKernelGGL((feautrier_kernel<MGX_APPROX_NONE, MGX_PREC_MIXED>), grid,
-                                       block, lds, st, kp, u, J, lambda_diag);
+                                       block, lds, st, b.kp_dev, u, J, lambda_diag);
                 break;
             case MGX_APPROX_ONE_LINE:
                 if (fp64)
                     hipLaunchKernelGGL((feautrier_kernel<MGX_APPROX_ONE_LINE, MGX_PREC_FP64>), grid,
-                                       block, lds, st, kp, u, J, lambda_diag);
+                                       block, lds, st, b.kp_dev, u, J, lambda_diag);
                 else
                     hipLaunchKernelGGL((feautrier_kernel<MGX_APPROX_ONE_LINE, MGX_PREC_MIXED>),
-                                       grid, block, lds, st, kp, u, J, lambda_diag);
+                                       grid, block, lds, st, b.kp_dev, u, J, lambda_diag);
                 break;
             default:
                 if (fp64)
                     hipLaunchKernelGGL((feautrier_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_FP64>),
-                                       grid, block, lds, st, kp, u, J, lambda_diag);
+                                       grid, block, lds, st, b.kp_dev, u, J, lambda_diag);
                 else
                     hipLaunchKernelGGL((feautrier_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_MIXED>),
-                                       grid, block, lds, st, kp, u, J, lambda_diag);
+                                       grid, block, lds, st, b.kp_dev, u, J, lambda_diag);
             }
         });
-    });
+    }, 8 * ((size_t)model->nfreqs + model->nlines));
 }
 
 
@@ -1676,17 +1760,17 @@ int mgx_image_device(const mgx_model *model, const mgx_options *opts, const doub
             if (opts->approx == MGX_APPROX_NONE) {
                 if (fp64)
                     hipLaunchKernelGGL((image_kernel<MGX_APPROX_NONE, MGX_PREC_FP64>), grid, block,
-                                       lds, st, kp, dfr, I, ImX, ImY);
+                                       lds, st, b.kp_dev, dfr, I, ImX, ImY);
                 else
                     hipLaunchKernelGGL((image_kernel<MGX_APPROX_NONE, MGX_PREC_MIXED>), grid, block,
-                                       lds, st, kp, dfr, I, ImX, ImY);
+                                       lds, st, b.kp_dev, dfr, I, ImX, ImY);
             } else {
                 if (fp64)
                     hipLaunchKernelGGL((image_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_FP64>), grid,
-                                       block, lds, st, kp, dfr, I, ImX, ImY);
+                                       block, lds, st, b.kp_dev, dfr, I, ImX, ImY);
                 else
                     hipLaunchKernelGGL((image_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_MIXED>), grid,
-                                       block, lds, st, kp, dfr, I, ImX, ImY);
+                                       block, lds, st, b.kp_dev, dfr, I, ImX, ImY);
             }
         });
     });
@@ -1711,7 +1795,7 @@ int mgx_trace_stats_device(const mgx_model *model, const mgx_options *opts, uint
     int e = launch_with_capacity(st, b, kp, [&](size_t lds) {
         (void)hipMemsetAsync(d, 0, 4 * sizeof(unsigned long long), st);
         hipLaunchKernelGGL(trace_stats_kernel, dim3(model->npoints, model->hnrays),
-                           dim3(MGX_WAVE), lds, st, kp, d);
+                           dim3(MGX_WAVE), lds, st, b.kp_dev, d);
     });
     if (!e) {
         hipError_t he = hipMemcpy(stats, d, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost);
@@ -1733,7 +1817,7 @@ int mgx_trace_counts_device(const mgx_model *model, const mgx_options *opts, uin
     if (int e = prepare(model, opts, st, b, kp)) return e;
     return launch_with_capacity(st, b, kp, [&](size_t lds) {
         hipLaunchKernelGGL(trace_counts_kernel, dim3(model->npoints, model->hnrays),
-                           dim3(MGX_WAVE), lds, st, kp, n_tot);
+                           dim3(MGX_WAVE), lds, st, b.kp_dev, n_tot);
     });
 }
 
@@ -1754,7 +1838,7 @@ int mgx_trace_rays_device(const mgx_model *model, const mgx_options *opts, uint3
     KParams kp;
     if (int e = prepare(model, opts, st, b, kp)) return e;
     const int e = launch_with_capacity(st, b, kp, [&](size_t lds) {
-        hipLaunchKernelGGL(trace_rays_kernel, dim3(nsel), dim3(MGX_WAVE), lds, st, kp, sel_rr,
+        hipLaunchKernelGGL(trace_rays_kernel, dim3(nsel), dim3(MGX_WAVE), lds, st, b.kp_dev, sel_rr,
                            sel_o, sel_l, capacity, count, centre_idx, nr, nr_interp, shift,
                            interp_factor, dZ);
     });

@@ -262,12 +262,27 @@ static inline double interp_linear(double a, double b, double t)
     return (b - a) * t + a;
 }
 
+/* Finite-difference aid of the adjoint checks: with it set, interpolate_log is
+   evaluated as a * pow(b/a, t) in double, so the function is smooth in a and b
+   (the float powf of the reference rounds b/a and the result to float, which
+   puts steps of ~1e-7 relative into a finite difference).  Off by default. */
+static int g_smooth_interp = 0;
+
 /* interpolate_log :89-99 */
 static inline double interp_log(double a, double b, double t)
 {
     if (t == 0.0) return a;
     if (t == 1.0) return b;
+    if (g_smooth_interp) return a * pow(b / a, t);
     return a * (double)O_POWF((float)(b / a), (float)t);
+}
+
+/* partial derivatives of interpolate_log(a, b, t) = a^(1-t) b^t (r its value) */
+static inline void interp_log_partials(double a, double b, double t, double r, double *da,
+                                       double *db)
+{
+    *da = (t == 1.0) ? 0.0 : (1.0 - t) * r / a;
+    *db = (t == 0.0) ? 0.0 : t * r / b;
 }
 
 /* ------------------------------------------------------------------------- */
@@ -1176,31 +1191,32 @@ static void image_surface_to_3d(const ImageFrame *fr, const double *raydir, doub
     }
 }
 
-int oracle_image_new(const mgx_model *m, const mgx_options *op, const double *raydir_in,
-                     uint32_t nx, uint32_t ny, double *I, double *ImX, double *ImY, int nthreads)
+/* Model::compute_image_new :1008-1035 and the frame of Image::set_coordinates_projection_surface
+   :221-366: normalised direction, frame, pixel coordinates, start boundary point */
+static int image_prepare(const mgx_model *m, const mgx_options *op, const double *raydir_in,
+                         uint32_t nx, uint32_t ny, double *rd, ImageFrame *fr, double *ImX,
+                         double *ImY, uint32_t *start_bdy)
 {
-    int err = check_model(m, op);
-    if (err) return err;
     if (nx == 0 || ny == 0) return MGX_E_ARG;
-    /* Model::compute_image_new :1008-1035 */
     const double sq = dot3(raydir_in, raydir_in);
     if (sq == 0.0) return MGX_E_ARG;
     if (op->approx == MGX_APPROX_ONE_LINE) return MGX_E_UNSUPPORTED;
     const double inv = 1 / sqrt(sq);
-    double rd[3] = {raydir_in[0] * inv, raydir_in[1] * inv, raydir_in[2] * inv};
+    rd[0] = raydir_in[0] * inv;
+    rd[1] = raydir_in[1] * inv;
+    rd[2] = raydir_in[2] * inv;
     if (m->dimension == 1 && (rd[0] != 0.0 || rd[1] != 1.0 || rd[2] != 0.0)) return MGX_E_ARG;
 
-    ImageFrame fr;
-    memset(&fr, 0, sizeof fr);
+    memset(fr, 0, sizeof *fr);
     double max_x = -DBL_MAX, max_y = -DBL_MAX, min_x = DBL_MAX, min_y = DBL_MAX;
     const double rx = rd[0], ry = rd[1], rz = rd[2];
-    fr.denominator = sqrt(rx * rx + ry * ry);
-    const double inverse_denominator = 1.0 / fr.denominator;
-    fr.ix = ry * inverse_denominator;
-    fr.iy = -rx * inverse_denominator;
-    fr.jx = rx * rz * inverse_denominator;
-    fr.jy = ry * rz * inverse_denominator;
-    fr.jz = -fr.denominator;
+    fr->denominator = sqrt(rx * rx + ry * ry);
+    const double inverse_denominator = 1.0 / fr->denominator;
+    fr->ix = ry * inverse_denominator;
+    fr->iy = -rx * inverse_denominator;
+    fr->jx = rx * rz * inverse_denominator;
+    fr->jy = ry * rz * inverse_denominator;
+    fr->jz = -fr->denominator;
     if (m->dimension == 1) {
         for (uint32_t b = 0; b < m->nboundary; b++) {
             const double x = m->position[3 * (size_t)m->boundary2point[b]];
@@ -1216,9 +1232,9 @@ int oracle_image_new(const mgx_model *m, const mgx_options *op, const double *ra
         for (uint32_t b = 0; b < m->nboundary; b++) {
             const double *p = &m->position[3 * (size_t)m->boundary2point[b]];
             double X, Y;
-            if (fr.denominator >= 1.0e-9) {
-                X = fr.ix * p[0] + fr.iy * p[1];
-                Y = fr.jx * p[0] + fr.jy * p[1] + fr.jz * p[2];
+            if (fr->denominator >= 1.0e-9) {
+                X = fr->ix * p[0] + fr->iy * p[1];
+                Y = fr->jx * p[0] + fr->jy * p[1] + fr->jz * p[2];
             } else {
                 X = p[0];
                 Y = p[1];
@@ -1233,7 +1249,7 @@ int oracle_image_new(const mgx_model *m, const mgx_options *op, const double *ra
     const double *cp = &m->position[3 * (size_t)closest];
     double dist = dot3(cp, rd);
     if (m->spherical_symmetry) dist = -cp[0];
-    for (int i = 0; i < 3; i++) fr.center_pt[i] = rd[i] * dist;
+    for (int i = 0; i < 3; i++) fr->center_pt[i] = rd[i] * dist;
     const double deltax = (max_x - min_x) / (nx);
     const double deltay = (max_y - min_y) / (ny);
     for (uint32_t xi = 0; xi < nx; xi++) {
@@ -1244,6 +1260,54 @@ int oracle_image_new(const mgx_model *m, const mgx_options *op, const double *ra
             ImY[t] = min_y + deltay / 2.0 + yi * deltay;
         }
     }
+    *start_bdy = closest;
+    return 0;
+}
+
+/* scratch length of setup_new_imager: from the largest per-pixel all-lines length */
+static uint32_t image_scratch_length(const mgx_model *m, const mgx_options *op,
+                                     const ImageFrame *fr, const double *rd, const double *ImX,
+                                     const double *ImY, size_t npix, uint32_t start_bdy)
+{
+    uint32_t mx = 0;
+#pragma omp parallel for schedule(static) reduction(max : mx)
+    for (size_t pix = 0; pix < npix; pix++) {
+        double org[3];
+        image_surface_to_3d(fr, rd, ImX[pix], ImY[pix], org);
+        const uint32_t len = ray_length_new_imager(m, op, org, start_bdy, rd);
+        if (len > mx) mx = len;
+    }
+    return 2 * mx + 1;
+}
+
+/* trace of one pixel ray into s (image_feautrier_order_2_new_imager :595-612); returns the
+   start boundary point */
+static uint32_t image_trace_pixel(const mgx_model *m, const mgx_options *op, Scratch *s,
+                                  const ImageFrame *fr, const double *rd, double x, double y,
+                                  uint32_t start_bdy)
+{
+    double org[3];
+    image_surface_to_3d(fr, rd, x, y, org);
+    double Z = 0.0;
+    const uint32_t cb = imaging_start(m, org, start_bdy, rd, &Z);
+    s->nr[s->centre] = cb;
+    s->shift[s->centre] = shift_rest(m, org, rd, cb, Z);
+    s->first = trace_ray_imaging(m, op, s, org, cb, rd, -1, &Z, s->centre - 1, s->centre - 1) + 1;
+    s->last = s->centre;
+    s->n_tot = (s->last + 1) - s->first;
+    return cb;
+}
+
+int oracle_image_new(const mgx_model *m, const mgx_options *op, const double *raydir_in,
+                     uint32_t nx, uint32_t ny, double *I, double *ImX, double *ImY, int nthreads)
+{
+    int err = check_model(m, op);
+    if (err) return err;
+    double rd[3];
+    ImageFrame fr;
+    uint32_t start_bdy;
+    err = image_prepare(m, op, raydir_in, nx, ny, rd, &fr, ImX, ImY, &start_bdy);
+    if (err) return err;
 
     const int nt = default_threads(nthreads);
 #ifdef _OPENMP
@@ -1251,17 +1315,7 @@ int oracle_image_new(const mgx_model *m, const mgx_options *op, const double *ra
     omp_set_num_threads(nt);
 #endif
     const size_t npix = (size_t)nx * ny;
-    const uint32_t start_bdy = closest;
-    /* setup_new_imager: scratch length from the per-pixel all-lines length */
-    uint32_t mx = 0;
-#pragma omp parallel for schedule(static) reduction(max : mx)
-    for (size_t pix = 0; pix < npix; pix++) {
-        double org[3];
-        image_surface_to_3d(&fr, rd, ImX[pix], ImY[pix], org);
-        const uint32_t len = ray_length_new_imager(m, op, org, start_bdy, rd);
-        if (len > mx) mx = len;
-    }
-    const uint32_t length = 2 * mx + 1;
+    const uint32_t length = image_scratch_length(m, op, &fr, rd, ImX, ImY, npix, start_bdy);
 #pragma omp parallel
     {
         Scratch s;
@@ -1269,15 +1323,7 @@ int oracle_image_new(const mgx_model *m, const mgx_options *op, const double *ra
 #pragma omp for schedule(static)
         for (size_t pix = 0; pix < npix; pix++) {
             if (serr) continue;
-            double org[3];
-            image_surface_to_3d(&fr, rd, ImX[pix], ImY[pix], org);
-            double Z = 0.0;
-            const uint32_t cb = imaging_start(m, org, start_bdy, rd, &Z);
-            s.nr[s.centre] = cb;
-            s.shift[s.centre] = shift_rest(m, org, rd, cb, Z);
-            s.first = trace_ray_imaging(m, op, &s, org, cb, rd, -1, &Z, s.centre - 1, s.centre - 1) + 1;
-            s.last = s.centre;
-            s.n_tot = (s.last + 1) - s.first;
+            const uint32_t cb = image_trace_pixel(m, op, &s, &fr, rd, ImX[pix], ImY[pix], start_bdy);
             for (uint32_t f = 0; f < m->nfreqs; f++) {
                 const double nu_cb = m->nu[(size_t)cb * m->nfreqs + f];
                 if (s.n_tot > 1) {
@@ -1295,7 +1341,6 @@ int oracle_image_new(const mgx_model *m, const mgx_options *op, const double *ra
 #endif
     return 0;
 }
-
 
 /*
  * The tridiagonal system behind one (rr, o, f) solve, for the brute-force
@@ -1348,6 +1393,496 @@ int oracle_feautrier_system(const mgx_model *m, const mgx_options *op, uint32_t 
         *L_c = s.L_diag[s.centre];
     }
     scratch_free(&s);
+    return err;
+}
+
+/* ------------------------------------------------------------------------- */
+/* adjoint: vector-Jacobian products of J and of images with respect to the   */
+/* line emissivity and opacity (P, L).  No reference counterpart; the trace,  */
+/* the erf / trapezoid branch of every segment, the CloseLines windows and    */
+/* the interpolation counts are held fixed (they do not depend on the         */
+/* emissivity, and depend on the opacity only through the discrete counts).   */
+
+void oracle_set_smooth_interp(int on) { g_smooth_interp = on; }
+
+typedef struct {
+    double *eps, *kap; /* (P, L) accumulators */
+    uint32_t L;
+} Grad;
+
+static inline void grad_add(Grad *g, uint32_t p, uint32_t l, double de, double dk)
+{
+    g->eps[(size_t)p * g->L + l] += de;
+    g->kap[(size_t)p * g->L + l] += dk;
+}
+
+/* reverse of get_eta_and_chi_interp: eta_bar d(eta) + chi_bar d(chi) */
+static void eta_chi_interp_bwd(const mgx_model *m, const mgx_options *op, uint32_t p1,
+                               uint32_t p2, double t, uint32_t ll, double freq, double eta_bar,
+                               double chi_bar, Grad *g)
+{
+    uint32_t lo = 0, hi = m->nlines;
+    if (op->approx == MGX_APPROX_ONE_LINE) {
+        lo = ll;
+        hi = ll + 1;
+    } else if (op->approx == MGX_APPROX_CLOSE_LINES) {
+        const double W = op->max_distance_opacity_contribution * pw_upper(m, p1, freq);
+        lo = lower_bound_d(m->sorted_line, m->nlines, freq - W);
+        hi = upper_bound_d(m->sorted_line, m->nlines, freq + W);
+    }
+    for (uint32_t s = lo; s < hi; s++) {
+        const uint32_t l = (op->approx == MGX_APPROX_CLOSE_LINES) ? m->sorted_line_map[s] : s;
+        const double diff = freq - m->line[l];
+        const double iw =
+            interp_linear(m->inverse_width[LIDX(m, p1, l)], m->inverse_width[LIDX(m, p2, l)], t);
+        const double prof = gaussian(iw, diff);
+        const double e1 = m->emissivity[LIDX(m, p1, l)], e2 = m->emissivity[LIDX(m, p2, l)];
+        const double k1 = m->opacity[LIDX(m, p1, l)], k2 = m->opacity[LIDX(m, p2, l)];
+        double de1, de2, dk1, dk2;
+        interp_log_partials(e1, e2, t, interp_log(e1, e2, t), &de1, &de2);
+        interp_log_partials(k1, k2, t, interp_log(k1, k2, t), &dk1, &dk2);
+        grad_add(g, p1, l, prof * eta_bar * de1, prof * chi_bar * dk1);
+        if (t != 0.0) grad_add(g, p2, l, prof * eta_bar * de2, prof * chi_bar * dk2);
+    }
+}
+
+/* d(compute_dtau_single_line)/d(avg_op) */
+static double dtau_single_line_dop(const mgx_model *m, const mgx_options *op, uint32_t ci,
+                                   uint32_t ni, uint32_t cii, uint32_t nii, uint32_t l,
+                                   double curr_freq, double next_freq, double curr_interp,
+                                   double next_interp, double dz)
+{
+    (void)op;
+    const double linefreq = m->line[l];
+    const double iw_c =
+        interp_linear(m->inverse_width[LIDX(m, ci, l)], m->inverse_width[LIDX(m, cii, l)], curr_interp);
+    const double iw_n =
+        interp_linear(m->inverse_width[LIDX(m, ni, l)], m->inverse_width[LIDX(m, nii, l)], next_interp);
+    const double avg_iw = (iw_c + iw_n) / 2.0;
+    if (curr_freq == next_freq) return dz * gaussian(avg_iw, curr_freq - m->line[l]);
+    const double next_pos = (linefreq - next_freq) * avg_iw;
+    const double curr_pos = (linefreq - curr_freq) * avg_iw;
+    const double diff_pos = next_pos - curr_pos;
+    const float erf_diff = O_ERFF((float)next_pos) - O_ERFF((float)curr_pos);
+    return dz * (avg_iw * ((double)erf_diff / diff_pos / 2.0));
+}
+
+/* reverse of interpolate_log(kappa(p), kappa(pi), t) */
+static void opacity_bwd(const mgx_model *m, uint32_t p, uint32_t pi, uint32_t l, double t,
+                        double bar, Grad *g)
+{
+    const double a = m->opacity[LIDX(m, p, l)], b = m->opacity[LIDX(m, pi, l)];
+    double da, db;
+    interp_log_partials(a, b, t, interp_log(a, b, t), &da, &db);
+    grad_add(g, p, l, 0.0, bar * da);
+    if (t != 0.0) grad_add(g, pi, l, 0.0, bar * db);
+}
+
+/* reverse of line_source: S = interpolate_log(eps/kap at p, eps/kap at pi, t) */
+static void line_source_bwd(const mgx_model *m, uint32_t p, uint32_t pi, uint32_t l, double t,
+                            double bar, Grad *g)
+{
+    const double kp = m->opacity[LIDX(m, p, l)], ki = m->opacity[LIDX(m, pi, l)];
+    const double Rp = m->emissivity[LIDX(m, p, l)] / kp;
+    const double Ri = m->emissivity[LIDX(m, pi, l)] / ki;
+    double dRp, dRi;
+    interp_log_partials(Rp, Ri, t, interp_log(Rp, Ri, t), &dRp, &dRi);
+    const double bp = bar * dRp;
+    grad_add(g, p, l, bp / kp, -bp * Rp / kp);
+    if (t != 0.0) {
+        const double bi = bar * dRi;
+        grad_add(g, pi, l, bi / ki, -bi * Ri / ki);
+    }
+}
+
+/* reverse of compute_S_dtau_line_integrated */
+static void S_dtau_line_integrated_bwd(const mgx_model *m, const mgx_options *op, uint32_t cp,
+                                       uint32_t np, uint32_t cpi, uint32_t npi, uint32_t lineidx,
+                                       double currfreq, double nextfreq, double curr_interp,
+                                       double next_interp, double dZ, double dtau_bar,
+                                       double Sc_bar, double Sn_bar, Grad *g)
+{
+    if (op->approx == MGX_APPROX_ONE_LINE) {
+        const double dop = dtau_single_line_dop(m, op, cp, np, cpi, npi, lineidx, currfreq,
+                                                nextfreq, curr_interp, next_interp, dZ);
+        opacity_bwd(m, cp, cpi, lineidx, curr_interp, 0.5 * dop * dtau_bar, g);
+        opacity_bwd(m, np, npi, lineidx, next_interp, 0.5 * dop * dtau_bar, g);
+        line_source_bwd(m, cp, cpi, lineidx, curr_interp, Sc_bar, g);
+        line_source_bwd(m, np, npi, lineidx, next_interp, Sn_bar, g);
+        return;
+    }
+    uint32_t lo = 0, hi = m->nlines;
+    if (op->approx == MGX_APPROX_CLOSE_LINES) {
+        double left_freq, right_freq;
+        if (currfreq < nextfreq) {
+            left_freq = currfreq;
+            right_freq = nextfreq;
+        } else {
+            right_freq = currfreq;
+            left_freq = nextfreq;
+        }
+        const double cb = op->max_distance_opacity_contribution * pw_upper(m, cp, right_freq);
+        const double nb = op->max_distance_opacity_contribution * pw_upper(m, np, right_freq);
+        const double W = (cb < nb) ? nb : cb;
+        lo = lower_bound_d(m->sorted_line, m->nlines, left_freq - W);
+        hi = upper_bound_d(m->sorted_line, m->nlines, right_freq + W);
+    }
+    double sum_dtau = 0.0, sum_Sc = 0.0, sum_Sn = 0.0;
+    for (uint32_t s = lo; s < hi; s++) {
+        const uint32_t l = (op->approx == MGX_APPROX_CLOSE_LINES) ? m->sorted_line_map[s] : s;
+        const double line_dtau = compute_dtau_single_line(m, op, cp, np, cpi, npi, l, currfreq,
+                                                          nextfreq, curr_interp, next_interp, dZ);
+        sum_dtau += line_dtau;
+        sum_Sc += line_dtau * line_source(m, cp, cpi, l, curr_interp);
+        sum_Sn += line_dtau * line_source(m, np, npi, l, next_interp);
+    }
+    double d = sum_dtau;
+    int clamped = 0;
+    if (op->approx == MGX_APPROX_CLOSE_LINES) {
+        const double bound_min_dtau = op->min_opacity * dZ;
+        if (-bound_min_dtau < d && d < bound_min_dtau) {
+            d = bound_min_dtau;
+            clamped = 1;
+        }
+    }
+    const double Scurr = sum_Sc / d, Snext = sum_Sn / d;
+    const double sSc_bar = Sc_bar / d, sSn_bar = Sn_bar / d;
+    const double sdt_bar = clamped ? 0.0 : dtau_bar - sSc_bar * Scurr - sSn_bar * Snext;
+    for (uint32_t s = lo; s < hi; s++) {
+        const uint32_t l = (op->approx == MGX_APPROX_CLOSE_LINES) ? m->sorted_line_map[s] : s;
+        const double line_dtau = compute_dtau_single_line(m, op, cp, np, cpi, npi, l, currfreq,
+                                                          nextfreq, curr_interp, next_interp, dZ);
+        const double Sc = line_source(m, cp, cpi, l, curr_interp);
+        const double Sn = line_source(m, np, npi, l, next_interp);
+        const double ld_bar = sdt_bar + sSc_bar * Sc + sSn_bar * Sn;
+        const double dop = dtau_single_line_dop(m, op, cp, np, cpi, npi, l, currfreq, nextfreq,
+                                                curr_interp, next_interp, dZ);
+        opacity_bwd(m, cp, cpi, l, curr_interp, 0.5 * dop * ld_bar, g);
+        opacity_bwd(m, np, npi, l, next_interp, 0.5 * dop * ld_bar, g);
+        line_source_bwd(m, cp, cpi, l, curr_interp, sSc_bar * line_dtau, g);
+        line_source_bwd(m, np, npi, l, next_interp, sSn_bar * line_dtau, g);
+    }
+}
+
+/* reverse of compute_source_dtau for one segment: dtau_bar, Sc_bar, Sn_bar are the
+   adjoints of its optical depth and of the two source values */
+static void source_dtau_bwd(const mgx_model *m, const mgx_options *op, uint32_t cp, uint32_t np,
+                            uint32_t cpi, uint32_t npi, uint32_t line, double curr_freq,
+                            double next_freq, double curr_shift, double next_shift, double dZ,
+                            double curr_interp, double next_interp, double dtau_bar,
+                            double Sc_bar, double Sn_bar, Grad *g)
+{
+    const double dshift_abs = fabs(next_shift - curr_shift);
+    const double a = m->dshift_max[cp], b = m->dshift_max[np];
+    const double dshift_max = (b < a) ? b : a;
+    if (dshift_abs > dshift_max) {
+        S_dtau_line_integrated_bwd(m, op, cp, np, cpi, npi, line, curr_freq, next_freq,
+                                   curr_interp, next_interp, dZ, dtau_bar, Sc_bar, Sn_bar, g);
+        return;
+    }
+    double eta_c, chi_c, eta_n, chi_n;
+    get_eta_and_chi_interp(m, op, cp, cpi, curr_interp, line, curr_freq, &eta_c, &chi_c);
+    get_eta_and_chi_interp(m, op, np, npi, next_interp, line, next_freq, &eta_n, &chi_n);
+    const double chi_c_bar = 0.5 * dZ * dtau_bar - Sc_bar * eta_c / (chi_c * chi_c);
+    const double chi_n_bar = 0.5 * dZ * dtau_bar - Sn_bar * eta_n / (chi_n * chi_n);
+    eta_chi_interp_bwd(m, op, cp, cpi, curr_interp, line, curr_freq, Sc_bar / chi_c, chi_c_bar, g);
+    eta_chi_interp_bwd(m, op, np, npi, next_interp, line, next_freq, Sn_bar / chi_n, chi_n_bar, g);
+}
+
+/* per-thread work arrays of the ray adjoint */
+typedef struct {
+    double *d, *S, *F, *FI, *Su, *Sx, *u, *x, *lam, *dbar;
+} AdjWork;
+
+static int adjwork_init(AdjWork *w, uint32_t length)
+{
+    double **a[] = {&w->d, &w->S, &w->F, &w->FI, &w->Su, &w->Sx, &w->u, &w->x, &w->lam, &w->dbar};
+    int ok = 1;
+    for (size_t i = 0; i < sizeof a / sizeof a[0]; i++) {
+        *a[i] = calloc(length, sizeof(double));
+        ok = ok && *a[i];
+    }
+    return ok ? 0 : -1;
+}
+
+static void adjwork_free(AdjWork *w)
+{
+    double *a[] = {w->d, w->S, w->F, w->FI, w->Su, w->Sx, w->u, w->x, w->lam, w->dbar};
+    for (size_t i = 0; i < sizeof a / sizeof a[0]; i++) free(a[i]);
+}
+
+/*
+ * Vector-Jacobian product of one (o, f) solve on the traced ray in s: ubar is
+ * dL/du at list entry c (absolute index).  The solve is T u = r with rows
+ *   first: (1 + 2/d0) u0 + (2/d0^2)(u0 - u1) = S0 + 2 I0/d0,
+ *   inner: u_n + A_n (u_n - u_{n-1}) + C_n (u_n - u_{n+1}) = S_n,
+ *          A_n = 2/((d_{n-1}+d_n) d_{n-1}), C_n = 2/((d_{n-1}+d_n) d_n),
+ *   last : the first row mirrored, with d_{N-2} and I_l
+ * (solver.tpp:2234-2290).  The adjoint lam = T^-T e_c ubar is computed as
+ * D T^-1 D^-1 e_c ubar, D = diag(d0/2, (d_{n-1}+d_n)/2, d_{N-2}/2) (D T is
+ * symmetric), with T^-1 e_c from the reference's forward elimination and back
+ * substitution.  Then dL/dS_n = lam_n and dL/dd_j = -sum_i lam_i dR_i/dd_j,
+ * which the segment reverses carry to the emissivity and opacity.
+ */
+static void ray_adjoint(const mgx_model *m, const mgx_options *op, const Scratch *s, AdjWork *w,
+                        uint32_t o, uint32_t f, uint32_t c_abs, double ubar, Grad *g)
+{
+    const uint32_t first = s->first, N = s->n_tot, c = c_abs - first;
+    const uint32_t *nr = s->nr + first, *nri = s->nr_interp + first;
+    const double *dZ = s->dZ + first, *shift = s->shift + first, *ifac = s->interp_factor + first;
+    const double freq = m->nu[(size_t)o * m->nfreqs + f];
+    const uint32_t l = m->f_line[f];
+    double *d = w->d, *S = w->S, *F = w->F, *FI = w->FI, *Su = w->Su, *Sx = w->Sx, *u = w->u,
+           *x = w->x, *lam = w->lam, *dbar = w->dbar;
+    /* segments (compute_source_dtau with its carried state) */
+    {
+        double chi_c = 0.0, chi_n = 0.0, term_c = 0.0, term_n = 0.0;
+        int cco = 1;
+        for (uint32_t j = 0; j + 1 < N; j++) {
+            if (j > 0) {
+                term_c = term_n;
+                chi_c = chi_n;
+            }
+            compute_source_dtau(m, op, nr[j], nr[j + 1], nri[j], nri[j + 1], l,
+                                freq * shift[j], freq * shift[j + 1], shift[j], shift[j + 1],
+                                dZ[j], ifac[j], ifac[j + 1], &cco, &d[j], &chi_c, &chi_n,
+                                &term_c, &term_n);
+            S[j] = term_c;
+        }
+        S[N - 1] = term_n;
+    }
+    const double I0 = boundary_intensity(m, nr[0], freq * shift[0]);
+    const double Il = boundary_intensity(m, nr[N - 1], freq * shift[N - 1]);
+    /* forward elimination for u and for x = T^-1 e_c */
+    {
+        const double idt = 1.0 / d[0];
+        const double Cf = 2.0 * idt * idt;
+        const double BmC = 1.0 + 2.0 * idt;
+        const double B = BmC + Cf;
+        Su[0] = (S[0] + 2.0 * I0 * idt) / B;
+        Sx[0] = (c == 0) ? 1.0 / B : 0.0;
+        F[0] = 0.5 * BmC * d[0] * d[0];
+        FI[0] = 1.0 / (1.0 + F[0]);
+    }
+    for (uint32_t n = 1; n + 1 < N; n++) {
+        const double da = 0.5 * (d[n - 1] + d[n]);
+        const double iA = da * d[n - 1], iC = da * d[n];
+        const double A = 1.0 / iA;
+        F[n] = (A * F[n - 1] * FI[n - 1] + 1.0) * iC;
+        FI[n] = 1.0 / (1.0 + F[n]);
+        Su[n] = (A * Su[n - 1] + S[n]) * FI[n] * iC;
+        Sx[n] = (A * Sx[n - 1] + (n == c ? 1.0 : 0.0)) * FI[n] * iC;
+    }
+    {
+        const double idt = 1.0 / d[N - 2];
+        const double Al = 2.0 * idt * idt;
+        const double BmA = 1.0 + 2.0 * idt;
+        const double Bl = BmA + Al;
+        const double den = 1.0 / (Bl * F[N - 2] + BmA);
+        u[N - 1] = (Al * Su[N - 2] + S[N - 1] + 2.0 * Il * idt) * (1.0 + F[N - 2]) * den;
+        x[N - 1] = (Al * Sx[N - 2] + (c == N - 1 ? 1.0 : 0.0)) * (1.0 + F[N - 2]) * den;
+    }
+    for (long n = (long)N - 2; n >= 0; n--) {
+        u[n] = Su[n] + FI[n] * u[n + 1];
+        x[n] = Sx[n] + FI[n] * x[n + 1];
+    }
+    /* lam = D x ubar / D_c */
+    for (uint32_t n = 0; n < N; n++) {
+        const double Dn = (n == 0) ? 0.5 * d[0] : (n == N - 1) ? 0.5 * d[N - 2]
+                                                              : 0.5 * (d[n - 1] + d[n]);
+        lam[n] = Dn * x[n];
+    }
+    {
+        const double Dc = (c == 0) ? 0.5 * d[0] : (c == N - 1) ? 0.5 * d[N - 2]
+                                                              : 0.5 * (d[c - 1] + d[c]);
+        const double sc = ubar / Dc;
+        for (uint32_t n = 0; n < N; n++) lam[n] *= sc;
+    }
+    /* optical-depth adjoints from the rows.  The differences u_n - u_{n+1} are
+       taken from the elimination, Su_n - F_n FI_n u_{n+1}: at optically thin
+       steps u is constant to ~13 digits and the difference of two u values
+       would keep only the last few. */
+    double *du = Sx; /* reused: du[n] = u_n - u_{n+1} */
+    for (uint32_t n = 0; n + 1 < N; n++) du[n] = Su[n] - F[n] * FI[n] * u[n + 1];
+    for (uint32_t j = 0; j + 1 < N; j++) dbar[j] = 0.0;
+    {
+        const double dd = d[0];
+        const double dR = (2.0 / (dd * dd)) * (I0 - u[0]) - (4.0 / (dd * dd * dd)) * du[0];
+        dbar[0] -= lam[0] * dR;
+    }
+    for (uint32_t n = 1; n + 1 < N; n++) {
+        const double dm = d[n - 1], dp = d[n], sm = dm + dp;
+        const double A = 2.0 / (sm * dm), C = 2.0 / (sm * dp);
+        const double gm = -du[n - 1], gp = du[n];
+        const double dRdm = -A * (1.0 / sm + 1.0 / dm) * gm - (C / sm) * gp;
+        const double dRdp = -(A / sm) * gm - C * (1.0 / sm + 1.0 / dp) * gp;
+        dbar[n - 1] -= lam[n] * dRdm;
+        dbar[n] -= lam[n] * dRdp;
+    }
+    {
+        const double dd = d[N - 2];
+        const double dR = (2.0 / (dd * dd)) * (Il - u[N - 1]) + (4.0 / (dd * dd * dd)) * du[N - 2];
+        dbar[N - 2] -= lam[N - 1] * dR;
+    }
+    /* segments in reverse: row n takes its source from segment n (the last row from
+       segment N-2's far value) */
+    for (uint32_t j = 0; j + 1 < N; j++) {
+        const double Sn_bar = (j + 2 == N) ? lam[N - 1] : 0.0;
+        source_dtau_bwd(m, op, nr[j], nr[j + 1], nri[j], nri[j + 1], l, freq * shift[j],
+                        freq * shift[j + 1], shift[j], shift[j + 1], dZ[j], ifac[j], ifac[j + 1],
+                        dbar[j], lam[j], Sn_bar, g);
+    }
+}
+
+static int grad_threads_init(Grad *g, uint32_t P, uint32_t L)
+{
+    g->L = L;
+    g->eps = calloc((size_t)P * L, sizeof(double));
+    g->kap = calloc((size_t)P * L, sizeof(double));
+    return (g->eps && g->kap) ? 0 : -1;
+}
+
+static void grad_reduce(const Grad *g, double *d_eps, double *d_kap, size_t PL)
+{
+#pragma omp critical
+    {
+        for (size_t i = 0; i < PL; i++) {
+            d_eps[i] += g->eps[i];
+            d_kap[i] += g->kap[i];
+        }
+    }
+}
+
+int oracle_J_adjoint(const mgx_model *m, const mgx_options *op, const double *J_bar,
+                     double *d_eps, double *d_kap, int nthreads)
+{
+    int err = check_model(m, op);
+    if (err) return err;
+    const int nt = default_threads(nthreads);
+#ifdef _OPENMP
+    const int saved = omp_get_max_threads();
+    omp_set_num_threads(nt);
+#endif
+    const uint32_t length = 2 * get_ray_lengths_max(m, op, &err) + 1;
+    const size_t PL = (size_t)m->npoints * m->nlines;
+    if (err) goto done;
+    memset(d_eps, 0, PL * sizeof(double));
+    memset(d_kap, 0, PL * sizeof(double));
+#pragma omp parallel
+    {
+        Scratch s;
+        AdjWork w;
+        Grad g;
+        int serr = scratch_init(&s, length);
+        if (!serr) serr = adjwork_init(&w, length);
+        if (!serr) serr = grad_threads_init(&g, m->npoints, m->nlines);
+        for (uint32_t rr = 0; rr < m->hnrays; rr++) {
+            const uint32_t ar = m->antipod[rr];
+#pragma omp for schedule(static)
+            for (uint32_t o = 0; o < m->npoints; o++) {
+                if (serr) continue;
+                s.nr[s.centre] = o;
+                s.shift[s.centre] = 1.0;
+                for (uint32_t l = 0; l < m->nlines; l++) {
+                    int terr = 0;
+                    s.first = trace_ray_for_line(m, op, &s, l, o, rr, -1, s.centre - 1,
+                                                 s.centre - 1, &terr) + 1;
+                    s.last = trace_ray_for_line(m, op, &s, l, o, ar, +1, s.centre + 1, s.centre,
+                                                &terr) - 1;
+                    s.n_tot = (s.last + 1) - s.first;
+                    s.interp_factor[s.centre] = 0.0;
+                    if (terr) serr = terr;
+                    if (s.n_tot <= 1) continue;
+                    const double w2 = 2.0 * ray_weight(m, o, rr);
+                    for (uint32_t q = 0; q < m->nquads; q++) {
+                        const uint32_t f = l * m->nquads + q;
+                        const double jb = J_bar[(size_t)o * m->nfreqs + f];
+                        if (jb == 0.0) continue;
+                        ray_adjoint(m, op, &s, &w, o, f, s.centre, w2 * jb, &g);
+                    }
+                }
+            }
+        }
+        if (!serr) grad_reduce(&g, d_eps, d_kap, PL);
+#pragma omp critical
+        {
+            if (serr && !err) err = serr;
+        }
+        scratch_free(&s);
+        adjwork_free(&w);
+        free(g.eps);
+        free(g.kap);
+    }
+done:
+#ifdef _OPENMP
+    omp_set_num_threads(saved);
+#endif
+    return err;
+}
+
+int oracle_image_adjoint(const mgx_model *m, const mgx_options *op, const double *raydir_in,
+                         uint32_t nx, uint32_t ny, const double *I_bar, double *d_eps,
+                         double *d_kap, int nthreads)
+{
+    int err = check_model(m, op);
+    if (err) return err;
+    const size_t npix = (size_t)nx * ny;
+    double *ImX = calloc(npix ? npix : 1, sizeof(double));
+    double *ImY = calloc(npix ? npix : 1, sizeof(double));
+    if (!ImX || !ImY) {
+        free(ImX);
+        free(ImY);
+        return MGX_E_ARG;
+    }
+    double rd[3];
+    ImageFrame fr;
+    uint32_t start_bdy = 0;
+    err = image_prepare(m, op, raydir_in, nx, ny, rd, &fr, ImX, ImY, &start_bdy);
+    const size_t PL = (size_t)m->npoints * m->nlines;
+    const int nt = default_threads(nthreads);
+#ifdef _OPENMP
+    const int saved = omp_get_max_threads();
+    omp_set_num_threads(nt);
+#endif
+    if (!err) {
+        memset(d_eps, 0, PL * sizeof(double));
+        memset(d_kap, 0, PL * sizeof(double));
+        const uint32_t length = image_scratch_length(m, op, &fr, rd, ImX, ImY, npix, start_bdy);
+#pragma omp parallel
+        {
+            Scratch s;
+            AdjWork w;
+            Grad g;
+            int serr = scratch_init(&s, length);
+            if (!serr) serr = adjwork_init(&w, length);
+            if (!serr) serr = grad_threads_init(&g, m->npoints, m->nlines);
+#pragma omp for schedule(static)
+            for (size_t pix = 0; pix < npix; pix++) {
+                if (serr) continue;
+                const uint32_t cb =
+                    image_trace_pixel(m, op, &s, &fr, rd, ImX[pix], ImY[pix], start_bdy);
+                if (s.n_tot <= 1) continue;
+                for (uint32_t f = 0; f < m->nfreqs; f++) {
+                    const double ib = I_bar[pix * m->nfreqs + f];
+                    if (ib != 0.0) ray_adjoint(m, op, &s, &w, cb, f, s.last, 2.0 * ib, &g);
+                }
+            }
+            if (!serr) grad_reduce(&g, d_eps, d_kap, PL);
+#pragma omp critical
+            {
+                if (serr && !err) err = serr;
+            }
+            scratch_free(&s);
+            adjwork_free(&w);
+            free(g.eps);
+            free(g.kap);
+        }
+    }
+#ifdef _OPENMP
+    omp_set_num_threads(saved);
+#endif
+    free(ImX);
+    free(ImY);
     return err;
 }
 

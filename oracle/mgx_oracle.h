@@ -31,6 +31,17 @@ int oracle_feautrier_system(const mgx_model *m, const mgx_options *op, uint32_t 
                             uint32_t f, uint32_t capacity, uint32_t *n_out, uint32_t *centre_out,
                             double *dtau, double *S, double *I_bdy, double *u_c, double *L_c);
 
+/* Vector-Jacobian products (adjoints) with respect to emissivity and opacity
+   (P, L): d_eps = (dJ/d eps)^T J_bar and d_kap likewise; the image variant
+   takes I_bar (nx*ny, F).  The outputs are overwritten. */
+int oracle_J_adjoint(const mgx_model *m, const mgx_options *op, const double *J_bar,
+                     double *d_eps, double *d_kap, int nthreads);
+int oracle_image_adjoint(const mgx_model *m, const mgx_options *op, const double *raydir,
+                         uint32_t nx, uint32_t ny, const double *I_bar, double *d_eps,
+                         double *d_kap, int nthreads);
+/* 1: interpolate_log in double (smooth; for finite-difference checks), 0: as the reference */
+void oracle_set_smooth_interp(int on);
+
 float oracle_expf(float x);
 float oracle_expm1f(float x);
 float oracle_logf(float x);

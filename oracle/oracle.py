@@ -36,6 +36,11 @@ def lib(kind="cr"):
             fn = getattr(h, "oracle_" + f)
             fn.argtypes = [C.c_float]
             fn.restype = C.c_float
+        h.oracle_J_adjoint.argtypes = [vp, vp, vp, vp, vp, C.c_int]
+        h.oracle_image_adjoint.argtypes = [vp, vp, vp, C.c_uint32, C.c_uint32, vp, vp, vp,
+                                           C.c_int]
+        h.oracle_set_smooth_interp.argtypes = [C.c_int]
+        h.oracle_set_smooth_interp.restype = None
         h.oracle_powf.argtypes = [C.c_float, C.c_float]
         h.oracle_powf.restype = C.c_float
         _libs[kind] = h
@@ -116,3 +121,39 @@ def feautrier_system(model, opts, rr, o, f, capacity=4096, kind="cr"):
     k = int(n[0])
     return dict(n=k, centre=int(c[0]), dtau=dtau[:k - 1], S=S[:k], I_bdy=I, u_c=float(uc[0]),
                 L_c=float(lc[0]))
+
+
+def set_smooth_interp(on, kind="cr"):
+    """Evaluate interpolate_log in double (a smooth function of its end values),
+    for finite-difference checks of the adjoint; off restores the reference's
+    float powf."""
+    lib(kind).oracle_set_smooth_interp(1 if on else 0)
+
+
+def J_adjoint(model, opts, J_bar, nthreads=0, kind="cr"):
+    """(d_eps, d_kap), each (P, L): the gradient of sum(J_bar * J) with respect to
+    the line emissivity and opacity."""
+    m = model.view()
+    P, L = model.npoints, model.nlines
+    jb = np.ascontiguousarray(J_bar, dtype=np.float64)
+    assert jb.shape == (P, model.nfreqs)
+    de = np.zeros((P, L))
+    dk = np.zeros((P, L))
+    _check(lib(kind).oracle_J_adjoint(C.addressof(m), C.addressof(opts), jb.ctypes.data,
+                                      de.ctypes.data, dk.ctypes.data, nthreads), "J_adjoint")
+    return de, dk
+
+
+def image_adjoint(model, opts, raydir, nx, ny, I_bar, freqs=None, nthreads=0, kind="cr"):
+    """(d_eps, d_kap): the gradient of sum(I_bar * I) for the image_new image."""
+    F = model.nfreqs if freqs is None else freqs["nu"].shape[1]
+    m = model.view(freqs, nfreqs=F)
+    rd = np.ascontiguousarray(raydir, dtype=np.float64)
+    ib = np.ascontiguousarray(I_bar, dtype=np.float64)
+    assert ib.shape == (nx * ny, F)
+    de = np.zeros((model.npoints, model.nlines))
+    dk = np.zeros((model.npoints, model.nlines))
+    _check(lib(kind).oracle_image_adjoint(C.addressof(m), C.addressof(opts), rd.ctypes.data, nx,
+                                          ny, ib.ctypes.data, de.ctypes.data, dk.ctypes.data,
+                                          nthreads), "image_adjoint")
+    return de, dk
