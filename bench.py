@@ -34,6 +34,7 @@ def parse():
     ap.add_argument("--n-edge", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-adjoint", action="store_true")
     return ap.parse_args()
 
 
@@ -60,6 +61,33 @@ def read_traffic(workload_key):
         return d.get(workload_key)
     except Exception:
         return None
+
+
+def run_adjoint(dm, model, precision, steps, warmup):
+    """Times `steps` adjoint solves (vector-Jacobian product of J with a fixed
+    seed); returns (ms_per_step, kernel_ms)."""
+    import numpy as np
+    import torch
+
+    from magritte_amd import _abi, solver
+
+    op = model.options(precision=_abi.PREC_FP64 if precision == "fp64" else _abi.PREC_MIXED)
+    g = torch.Generator(device="cpu").manual_seed(20261202)
+    Jb = torch.randn((model.npoints, model.nfreqs), dtype=torch.float64, generator=g).to("cuda:0")
+    lib = _abi.lib()
+    lib.mgx_set_kernel_timing(1)
+    for _ in range(warmup):
+        solver.feautrier_adjoint(dm, op, Jb)
+    torch.cuda.synchronize()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        solver.feautrier_adjoint(dm, op, Jb)
+        kernel_ms.append(lib.mgx_last_kernel_ms())
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    lib.mgx_set_kernel_timing(0)
+    return 1e3 * elapsed / steps, float(np.mean(kernel_ms))
 
 
 def run_variant(dm, model, precision, steps, warmup):
@@ -117,6 +145,15 @@ def main():
         variants[prec] = {"ms_per_step": round(ms, 4), "kernel_ms": round(kms, 4),
                           "value": round(U / (ms * 1e-3), 1), "J": J}
         print(f"# {prec}: {ms:.3f} ms/step, kernel {kms:.3f} ms", file=sys.stderr, flush=True)
+    adjoint = {}
+    if not args.no_adjoint:
+        for prec in ("fp64", "mixed"):
+            ams, akms = run_adjoint(dm, model, prec, max(1, args.steps // 2), 1)
+            adjoint[prec] = {"ms_per_step": round(ams, 4), "kernel_ms": round(akms, 4),
+                             "value": round(U / (ams * 1e-3), 1),
+                             "cost_vs_forward": round(ams / variants[prec]["ms_per_step"], 3)}
+            print(f"# adjoint {prec}: {ams:.3f} ms/step, kernel {akms:.3f} ms", file=sys.stderr,
+                  flush=True)
 
     stats = solver.trace_stats(dm, model.options())
     cpu = None
@@ -124,14 +161,21 @@ def main():
         from oracle import oracle
 
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        # the reference's own float functions (glibc expf/expm1f/logf/powf/erff)
         t1 = time.perf_counter()
-        _, J_ref, _ = oracle.solve(model, model.options(), with_u=False, nthreads=threads)
+        _, J_glibc, _ = oracle.solve(model, model.options(), with_u=False, nthreads=threads,
+                                     kind="glibc")
         tc = time.perf_counter() - t1
+        # the correctly rounded variant the GPU FP64 path reproduces bit for bit
+        _, J_ref, _ = oracle.solve(model, model.options(), with_u=False, nthreads=threads)
         cpu = {"value": round(U / tc, 1), "unit": "ray*point*freq/s", "cores": threads,
                "kind": "port",
                "sample": f"one full solve of the same wind{args.n_edge}^3 model "
-                         f"(setup pass + trace + solve + J + Lambda), {tc:.2f}s, "
-                         f"{threads} OpenMP threads"}
+                         f"(setup pass + trace + solve + J + Lambda) by the C restatement "
+                         f"with the C library's float functions, {tc:.2f}s, "
+                         f"{threads} OpenMP threads",
+               "J_rel_diff_cr_vs_glibc": float(np.max(np.abs(J_ref - J_glibc) /
+                                                      np.maximum(np.abs(J_glibc), 1e-300)))}
         for v in variants.values():
             v["J_rel_err_vs_cpu_fp64"] = float(
                 np.max(np.abs(v["J"] - J_ref) / np.maximum(np.abs(J_ref), 1e-300)))
@@ -168,7 +212,7 @@ def main():
         "config": {"workload": f"{args.n_edge}^3 jittered spherical wind, Delaunay neighbours, "
                                f"12 HEALPix rays, 10-level linear rotor, 9 lines x 31 bins",
                    "npoints": P, "hnrays": R, "nfreqs": F, "precision": head,
-                   "approx": "CloseLines", "variants": variants},
+                   "approx": "CloseLines", "variants": variants, "adjoint": adjoint},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

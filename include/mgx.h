@@ -175,6 +175,37 @@ int mgx_image_device(const mgx_model *model_dev, const mgx_options *opts, const 
                      uint32_t nx, uint32_t ny, double *I, double *ImX, double *ImY, void *stream);
 
 /* ------------------------------------------------------------------------ */
+/* Adjoint (reverse mode).  No counterpart in the reference: the gradients   */
+/* of the formal solve with respect to the line emissivity and opacity       */
+/* (model_dev->emissivity / ->opacity, (P, L)), for differentiable use of    */
+/* the two entry points above.  The trace, the erf / trapezoid choice of     */
+/* every segment, the CloseLines windows and the interpolation counts are    */
+/* held fixed; the float rounding steps of interpolate_log's powf are        */
+/* differentiated as the smooth a^(1-t) b^t.                                 */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * Vector-Jacobian product of mgx_feautrier_forward_device's J:
+ * d_eps = (dJ/d emissivity)^T J_bar, d_kap = (dJ/d opacity)^T J_bar.
+ * J_bar (P, F), d_eps and d_kap (P, L), all device; the outputs are
+ * overwritten.  Uses a device workspace kept between calls (grow-only, see
+ * mgx_release_workspace); calls are serialised on it.
+ */
+int mgx_feautrier_adjoint_device(const mgx_model *model_dev, const mgx_options *opts,
+                                 const double *J_bar, double *d_eps, double *d_kap, void *stream);
+
+/*
+ * Vector-Jacobian product of mgx_image_device's I: I_bar (nx*ny, F),
+ * d_eps and d_kap (P, L), all device; the outputs are overwritten.
+ */
+int mgx_image_adjoint_device(const mgx_model *model_dev, const mgx_options *opts,
+                             const double *raydir, uint32_t nx, uint32_t ny, const double *I_bar,
+                             double *d_eps, double *d_kap, void *stream);
+
+/* Frees the adjoint's device workspace (it is re-allocated on the next call). */
+void mgx_release_workspace(void);
+
+/* ------------------------------------------------------------------------ */
 /* Host-side producers of the path's inputs (plain host pointers).           */
 /* ------------------------------------------------------------------------ */
 

@@ -86,6 +86,12 @@ SIGNATURES = [
     ("mgx_image_device", C.c_int,
      [C.POINTER(MgxModel), C.POINTER(MgxOptions), _VP, C.c_uint32, C.c_uint32, _VP, _VP, _VP,
       _VP]),
+    ("mgx_feautrier_adjoint_device", C.c_int,
+     [C.POINTER(MgxModel), C.POINTER(MgxOptions), _VP, _VP, _VP, _VP]),
+    ("mgx_image_adjoint_device", C.c_int,
+     [C.POINTER(MgxModel), C.POINTER(MgxOptions), _VP, C.c_uint32, C.c_uint32, _VP, _VP, _VP,
+      _VP]),
+    ("mgx_release_workspace", None, []),
     ("mgx_compute_inverse_widths", C.c_int,
      [C.c_uint32, C.c_uint32, C.POINTER(MgxLinedata), _VP, _VP, _VP]),
     ("mgx_compute_lte", C.c_int,

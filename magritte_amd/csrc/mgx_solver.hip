@@ -34,6 +34,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "mgx.h"
@@ -49,7 +50,7 @@
 
 namespace {
 
-enum { ERR_CAP = 1, ERR_NO_NEXT = 2, ERR_OUT = 4 };
+enum { ERR_CAP = 1, ERR_NO_NEXT = 2, ERR_OUT = 4, ERR_CAPE = 8 };
 
 /* global-address-space load (the pointers live in a struct, so the compiler
    would otherwise emit flat loads) */
@@ -88,7 +89,7 @@ struct KParams {
     uint32_t cap;         /* LDS walk capacity (steps per half ray) */
     uint32_t nslots;      /* number of 32-lane line slots = L * ceil(Q/32) */
     uint32_t slots_per_line;
-    uint32_t *status;     /* [0] error bits, [1] largest walk length seen */
+    uint32_t *status;     /* [0] error bits, [1] walk capacity needed, [2] adjoint segments needed */
 };
 
 __device__ inline double dot3(const double *a, const double *b)
@@ -701,12 +702,22 @@ struct Sweep {
     bool started;
 };
 
-template <int APPROX, int PREC>
+/* Per-segment elimination state kept by the adjoint's forward sweeps: segment k
+   (counted from the outer end) stores Y, X and dtau of its outer row, lane-
+   interleaved: st[(3 k + c) * 64] for c = 0, 1, 2 (st already offset by the lane). */
+struct SweepStore {
+    double *st;
+    uint32_t cap; /* segments that fit */
+    uint32_t n;   /* segments of the half (may exceed cap: nothing is stored past it) */
+};
+
+template <int APPROX, int PREC, bool STORE = false>
 __device__ inline void sweep_half(const KParams &kp, const double *s_erf, const OwnLine &ol,
                                   bool backward, uint32_t S, uint32_t o, double shift0,
                                   const uint32_t *s_nxt, const double *s_dZ, const double *s_sh,
                                   const uint16_t *mtab, uint32_t l, double freq, Sweep &sw,
-                                  PointEC &ec_centre)
+                                  PointEC &ec_centre, SweepStore *store = nullptr,
+                                  double shift_centre = 1.0)
 {
     HalfIter it;
     it.s = (int)S - 1;
@@ -714,9 +725,11 @@ __device__ inline void sweep_half(const KParams &kp, const double *s_erf, const 
     Entry outer = hi_entry(it, o);
     PointEC ec_out = {0.0, 0.0, false};
     sw.started = false;
+    if (STORE) store->n = 0;
     for (;;) {
         const double dz = hi_advance(kp, it, l, o, shift0, s_nxt, s_dZ, s_sh, mtab);
         Entry inner = hi_entry(it, o);
+        if (STORE && it.s < 0) inner.shift = shift_centre;
         PointEC ec_in = (it.s < 0) ? ec_centre : PointEC{0.0, 0.0, false};
         /* the segment runs from the rr-side entry to the antipod-side entry */
         const Entry &cu = backward ? inner : outer;
@@ -751,6 +764,15 @@ __device__ inline void sweep_half(const KParams &kp, const double *s_erf, const 
         sw.prev_dtau = dtau;
         sw.S_inner = S_in;
         sw.S_prev_row = S_in;
+        if (STORE) {
+            if (store->n < store->cap) {
+                double *q = store->st + (size_t)3 * MGX_WAVE * store->n;
+                q[0] = sw.Y;
+                q[MGX_WAVE] = sw.X;
+                q[2 * MGX_WAVE] = dtau;
+            }
+            store->n++;
+        }
         outer = inner;
         ec_out = ec_in;
         if (it.s < 0) break;
@@ -1486,6 +1508,721 @@ __global__ __launch_bounds__(MGX_WAVE) void image_kernel(const KParams *__restri
 }
 
 /* ------------------------------------------------------------------------- */
+/* adjoint: vector-Jacobian products of J and images with respect to the line */
+/* emissivity and opacity (P, L).  No reference counterpart.  The trace, the  */
+/* branch of every segment, the CloseLines windows and the interpolation      */
+/* counts are held fixed.                                                     */
+
+/*
+ * Gradient sink of one lane.  The outward walk of the reverse pass visits the
+ * segments of walk step s (outer entry in step s) together, and all of them
+ * touch only the two points crt_s and nxt_s; the own-line contributions for
+ * those two points are summed in registers, and when the walk moves to step
+ * s+1 the finished point (crt_s) is summed over the lanes that share the line
+ * (width lanes, aligned) and added by one lane.  Contributions to any other
+ * line are added by the lane directly.
+ */
+struct GradSink {
+    double *g_eps, *g_kap;
+    uint32_t L;
+    uint32_t line;   /* own line of the reducing lanes */
+    uint32_t width;  /* lanes summed together (32 or 64), 0: no reduction */
+    uint32_t p_crt, p_nxt;
+    double e_crt, k_crt, e_nxt, k_nxt;
+};
+
+__device__ inline void sink_put(const GradSink &g, uint32_t p, uint32_t l, double de, double dk)
+{
+    const size_t i = (size_t)p * g.L + l;
+    if (de != 0.0) unsafeAtomicAdd(&g.g_eps[i], de);
+    if (dk != 0.0) unsafeAtomicAdd(&g.g_kap[i], dk);
+}
+
+__device__ inline void sink_add(GradSink &g, uint32_t p, uint32_t l, double de, double dk)
+{
+    if (l != g.line) {
+        sink_put(g, p, l, de, dk);
+    } else if (p == g.p_nxt) {
+        g.e_nxt += de;
+        g.k_nxt += dk;
+    } else {
+        g.e_crt += de;
+        g.k_crt += dk;
+    }
+}
+
+/* all lanes of a reduction group call this together */
+__device__ inline void sink_flush(const GradSink &g, uint32_t p, double e, double k)
+{
+    if (g.width == 0) {
+        sink_put(g, p, g.line, e, k);
+        return;
+    }
+    for (uint32_t off = g.width >> 1; off >= 1; off >>= 1) {
+        e += __shfl_xor(e, (int)off, MGX_WAVE);
+        k += __shfl_xor(k, (int)off, MGX_WAVE);
+    }
+    if ((threadIdx.x & (g.width - 1)) == 0) sink_put(g, p, g.line, e, k);
+}
+
+/* the walk moves outward to a step whose points are (crt = old nxt, nxt) */
+__device__ inline void sink_step(GradSink &g, uint32_t nxt)
+{
+    sink_flush(g, g.p_crt, g.e_crt, g.k_crt);
+    g.p_crt = g.p_nxt;
+    g.e_crt = g.e_nxt;
+    g.k_crt = g.k_nxt;
+    g.p_nxt = nxt;
+    g.e_nxt = 0.0;
+    g.k_nxt = 0.0;
+}
+
+__device__ inline void sink_finish(GradSink &g)
+{
+    sink_flush(g, g.p_crt, g.e_crt, g.k_crt);
+    sink_flush(g, g.p_nxt, g.e_nxt, g.k_nxt);
+}
+
+/* partial derivatives of interpolate_log(a, b, t) = a^(1-t) b^t with value r */
+__device__ inline void interp_log_partials(double a, double b, double t, double r, double &da,
+                                           double &db)
+{
+    da = (t == 1.0) ? 0.0 : (1.0 - t) * r / a;
+    db = (t == 0.0) ? 0.0 : t * r / b;
+}
+
+/* reverse of eta_chi_interp: eta_bar d(eta) + chi_bar d(chi) into the sink */
+template <int APPROX, int PREC>
+__device__ void eta_chi_interp_bwd(const KParams &kp, const OwnLine &ol, uint32_t p1, uint32_t p2,
+                                   double t, uint32_t ll, double freq, double eta_bar,
+                                   double chi_bar, GradSink &g)
+{
+    const mgx_model &m = kp.m;
+    uint32_t lo = 0, hi = m.nlines;
+    if (APPROX == MGX_APPROX_ONE_LINE) {
+        lo = ll;
+        hi = ll + 1;
+    } else if (APPROX == MGX_APPROX_CLOSE_LINES) {
+        const double W = kp.op.max_distance_opacity_contribution * (freq * gld(&kp.pts[p1].ub));
+        line_window(m, ol, freq - W, freq + W, lo, hi);
+    }
+    for (uint32_t s = lo; s < hi; s++) {
+        const uint32_t l = (APPROX == MGX_APPROX_CLOSE_LINES) ? m.sorted_line_map[s] : s;
+        const double diff = freq - m.line[l];
+        const double4 a = lp(kp, p1, l), b = lp(kp, p2, l);
+        const double prof = gaussian<PREC>(interp_linear(a.x, b.x, t), diff);
+        double de1, de2, dk1, dk2;
+        interp_log_partials(a.y, b.y, t, interp_log<PREC>(a.y, b.y, t), de1, de2);
+        interp_log_partials(a.z, b.z, t, interp_log<PREC>(a.z, b.z, t), dk1, dk2);
+        const double pe = prof * eta_bar, pk = prof * chi_bar;
+        sink_add(g, p1, l, pe * de1, pk * dk1);
+        if (t != 0.0) sink_add(g, p2, l, pe * de2, pk * dk2);
+    }
+}
+
+/* d(compute_dtau_single_line)/d(avg_op) */
+template <int PREC>
+__device__ double dtau_single_line_dop(const double *erf_tab, double4 c, double4 cI, double4 n,
+                                       double4 nI, double linefreq, double curr_freq,
+                                       double next_freq, double curr_interp, double next_interp,
+                                       double dz)
+{
+    const double iw_c = interp_linear(c.x, cI.x, curr_interp);
+    const double iw_n = interp_linear(n.x, nI.x, next_interp);
+    const double avg_iw = (iw_c + iw_n) / 2.0;
+    if (curr_freq == next_freq) return dz * gaussian<PREC>(avg_iw, curr_freq - linefreq);
+    const double next_pos = (linefreq - next_freq) * avg_iw;
+    const double curr_pos = (linefreq - curr_freq) * avg_iw;
+    const double diff_pos = next_pos - curr_pos;
+    const float erf_diff = f_erf<PREC>((float)next_pos, erf_tab) - f_erf<PREC>((float)curr_pos, erf_tab);
+    return dz * (avg_iw * ((double)erf_diff * (0.5 * rcp(diff_pos))));
+}
+
+/* reverse of the line terms at one end of a segment: opacity (kappa interpolation)
+   and line source (interpolation of eps/kappa) */
+template <int PREC>
+__device__ inline void line_end_bwd(uint32_t p, uint32_t pi, uint32_t l, double t, double4 a,
+                                    double4 b, double op_bar, double S_bar, GradSink &g)
+{
+    double dk1, dk2, dR1, dR2;
+    interp_log_partials(a.z, b.z, t, interp_log<PREC>(a.z, b.z, t), dk1, dk2);
+    interp_log_partials(a.w, b.w, t, interp_log<PREC>(a.w, b.w, t), dR1, dR2);
+    /* R = eps/kappa: dR/deps = 1/kappa, dR/dkappa = -R/kappa */
+    const double r1 = S_bar * dR1 * rcp(a.z);
+    sink_add(g, p, l, r1, op_bar * dk1 - r1 * a.w);
+    if (t != 0.0) {
+        const double r2 = S_bar * dR2 * rcp(b.z);
+        sink_add(g, pi, l, r2, op_bar * dk2 - r2 * b.w);
+    }
+}
+
+/* reverse of S_dtau_line_integrated */
+template <int APPROX, int PREC>
+__device__ void S_dtau_line_integrated_bwd(const KParams &kp, const double *erf_tab,
+                                           const OwnLine &ol, uint32_t cp, uint32_t np,
+                                           uint32_t cpi, uint32_t npi, uint32_t lineidx,
+                                           double currfreq, double nextfreq, double curr_interp,
+                                           double next_interp, double dZ, double dtau_bar,
+                                           double Sc_bar, double Sn_bar, GradSink &g)
+{
+    const mgx_model &m = kp.m;
+    if (APPROX == MGX_APPROX_ONE_LINE) {
+        const double4 c = lp(kp, cp, lineidx), cI = lp(kp, cpi, lineidx);
+        const double4 n = lp(kp, np, lineidx), nI = lp(kp, npi, lineidx);
+        const double dop = dtau_single_line_dop<PREC>(erf_tab, c, cI, n, nI, m.line[lineidx],
+                                                      currfreq, nextfreq, curr_interp,
+                                                      next_interp, dZ);
+        line_end_bwd<PREC>(cp, cpi, lineidx, curr_interp, c, cI, 0.5 * dop * dtau_bar, Sc_bar, g);
+        line_end_bwd<PREC>(np, npi, lineidx, next_interp, n, nI, 0.5 * dop * dtau_bar, Sn_bar, g);
+        return;
+    }
+    uint32_t lo = 0, hi = m.nlines;
+    if (APPROX == MGX_APPROX_CLOSE_LINES) {
+        double left_freq, right_freq;
+        if (currfreq < nextfreq) {
+            left_freq = currfreq;
+            right_freq = nextfreq;
+        } else {
+            right_freq = currfreq;
+            left_freq = nextfreq;
+        }
+        const double cb = kp.op.max_distance_opacity_contribution * (right_freq * gld(&kp.pts[cp].ub));
+        const double nb = kp.op.max_distance_opacity_contribution * (right_freq * gld(&kp.pts[np].ub));
+        const double W = (cb < nb) ? nb : cb;
+        line_window(m, ol, left_freq - W, right_freq + W, lo, hi);
+    }
+    /* forward sums (the same operations as S_dtau_line_integrated) */
+    double sum_dtau = 0.0, sum_Sc = 0.0, sum_Sn = 0.0;
+    for (uint32_t s = lo; s < hi; s++) {
+        const uint32_t l = (APPROX == MGX_APPROX_CLOSE_LINES) ? m.sorted_line_map[s] : s;
+        const double4 c = lp(kp, cp, l), cI = lp(kp, cpi, l);
+        const double4 n = lp(kp, np, l), nI = lp(kp, npi, l);
+        const double line_dtau = dtau_single_line<PREC>(kp, erf_tab, c, cI, n, nI, m.line[l],
+                                                        currfreq, nextfreq, curr_interp,
+                                                        next_interp, dZ);
+        sum_dtau += line_dtau;
+        sum_Sc = fma(line_dtau, interp_log<PREC>(c.w, cI.w, curr_interp), sum_Sc);
+        sum_Sn = fma(line_dtau, interp_log<PREC>(n.w, nI.w, next_interp), sum_Sn);
+    }
+    double d = sum_dtau;
+    bool clamped = false;
+    if (APPROX == MGX_APPROX_CLOSE_LINES) {
+        const double bound_min_dtau = kp.op.min_opacity * dZ;
+        if (-bound_min_dtau < d && d < bound_min_dtau) {
+            d = bound_min_dtau;
+            clamped = true;
+        }
+    }
+    const double rd = rcp(d);
+    const double sSc_bar = Sc_bar * rd, sSn_bar = Sn_bar * rd;
+    const double sdt_bar =
+        clamped ? 0.0 : dtau_bar - sSc_bar * (sum_Sc * rd) - sSn_bar * (sum_Sn * rd);
+    for (uint32_t s = lo; s < hi; s++) {
+        const uint32_t l = (APPROX == MGX_APPROX_CLOSE_LINES) ? m.sorted_line_map[s] : s;
+        const double4 c = lp(kp, cp, l), cI = lp(kp, cpi, l);
+        const double4 n = lp(kp, np, l), nI = lp(kp, npi, l);
+        const double line_dtau = dtau_single_line<PREC>(kp, erf_tab, c, cI, n, nI, m.line[l],
+                                                        currfreq, nextfreq, curr_interp,
+                                                        next_interp, dZ);
+        const double Sc = interp_log<PREC>(c.w, cI.w, curr_interp);
+        const double Sn = interp_log<PREC>(n.w, nI.w, next_interp);
+        const double ld_bar = sdt_bar + sSc_bar * Sc + sSn_bar * Sn;
+        const double dop = dtau_single_line_dop<PREC>(erf_tab, c, cI, n, nI, m.line[l], currfreq,
+                                                      nextfreq, curr_interp, next_interp, dZ);
+        line_end_bwd<PREC>(cp, cpi, l, curr_interp, c, cI, 0.5 * dop * ld_bar, sSc_bar * line_dtau, g);
+        line_end_bwd<PREC>(np, npi, l, next_interp, n, nI, 0.5 * dop * ld_bar, sSn_bar * line_dtau, g);
+    }
+}
+
+/* reverse of segment(): dtau_bar, Sc_bar, Sn_bar are the adjoints of the
+   segment's optical depth and of its two source values */
+template <int APPROX, int PREC>
+__device__ void segment_bwd(const KParams &kp, const double *erf_tab, const OwnLine &ol,
+                            const Entry &cu, const Entry &nx, double dZ, double freq,
+                            uint32_t line, double dtau_bar, double Sc_bar, double Sn_bar,
+                            GradSink &g)
+{
+    const double dshift_abs = fabs(nx.shift - cu.shift);
+    const double a = gld(&kp.pts[cu.nr].dshift_max), b = gld(&kp.pts[nx.nr].dshift_max);
+    const double dshift_max = (b < a) ? b : a;
+    const double cf = freq * cu.shift, nf = freq * nx.shift;
+    if (dshift_abs > dshift_max) {
+        S_dtau_line_integrated_bwd<APPROX, PREC>(kp, erf_tab, ol, cu.nr, nx.nr, cu.nri, nx.nri,
+                                                 line, cf, nf, cu.ifac, nx.ifac, dZ, dtau_bar,
+                                                 Sc_bar, Sn_bar, g);
+        return;
+    }
+    double eta_c, chi_c, eta_n, chi_n;
+    eta_chi_interp<APPROX, PREC>(kp, ol, cu.nr, cu.nri, cu.ifac, line, cf, eta_c, chi_c);
+    eta_chi_interp<APPROX, PREC>(kp, ol, nx.nr, nx.nri, nx.ifac, line, nf, eta_n, chi_n);
+    const double ric = rcp(chi_c), rin = rcp(chi_n);
+    const double chi_c_bar = 0.5 * dZ * dtau_bar - Sc_bar * eta_c * ric * ric;
+    const double chi_n_bar = 0.5 * dZ * dtau_bar - Sn_bar * eta_n * rin * rin;
+    eta_chi_interp_bwd<APPROX, PREC>(kp, ol, cu.nr, cu.nri, cu.ifac, line, cf, Sc_bar * ric,
+                                     chi_c_bar, g);
+    eta_chi_interp_bwd<APPROX, PREC>(kp, ol, nx.nr, nx.nri, nx.ifac, line, nf, Sn_bar * rin,
+                                     chi_n_bar, g);
+}
+
+/* derivative of a row  u + A_a (u - u_a) + A_b (u - u_b) = S  with respect to d_a,
+   A_a = 2/((d_a + d_b) d_a), A_b = 2/((d_a + d_b) d_b); Da = u - u_a, Db = u - u_b */
+__device__ inline double row_dd(double da, double db, double Da, double Db)
+{
+    const double s = da + db;
+    const double rs = rcp(s);
+    const double Aa = 2.0 * rs * rcp(da), Ab = 2.0 * rs * rcp(db);
+    return -Aa * (rs + rcp(da)) * Da - Ab * rs * Db;
+}
+
+/* derivative of a boundary row  (1 + 2/d) u + (2/d^2)(u - u_n) = S + 2 I/d  w.r.t. d */
+__device__ inline double bdy_row_dd(double d, double I, double u, double Dn)
+{
+    const double r = rcp(d);
+    return 2.0 * r * r * (I - u) - 4.0 * r * r * r * Dn;
+}
+
+/*
+ * Reverse pass over one half ray, outward from the centre.  With the system
+ * T u = r of the ray (rows as in the reference's elimination) and the seed
+ * ubar at the centre, lam = T^-T e_c ubar = D T^-1 D^-1 e_c ubar (D T is
+ * symmetric, D = half the sum of a row's two segment depths, or half the one
+ * depth of a boundary row), so lam_k = D_k mu_k with mu_k = XI_k mu_{k+1}, the
+ * same factors that give u_k = Y_k + XI_k u_{k+1}.  dL/dS_k = lam_k and
+ * dL/dd = -sum_rows lam_row dR_row/dd; the differences u_k - u_{k+1} are taken
+ * as Y_k - X_k XI_k u_{k+1} (optically thin steps leave u constant to ~13
+ * digits, so subtracting two u values would lose them).
+ *
+ * Segment sources as the reference uses them (row n takes Scurr of segment n,
+ * the last row Snext of the last segment): rr-side half (backward = false):
+ * segment k supplies its outer row (Sc), and its Sn feeds the centre row when
+ * the centre is the last row; antipod-side half: segment k supplies the row of
+ * its inner entry (Sc) and segment 0 the boundary row (Sn).
+ */
+struct RevCentre {
+    double u, mu, lam; /* u, D^-1-scaled adjoint and adjoint of the centre row */
+    double dbar;       /* the centre row's part of dL/d(depth of the innermost segment) */
+    bool centre_last;  /* rr-side half only: the centre is the system's last row */
+};
+
+template <int APPROX, int PREC>
+__device__ void reverse_half(const KParams &kp, const double *s_erf, const OwnLine &ol,
+                             bool backward, uint32_t S, uint32_t o, double shift0,
+                             double shift_centre, const uint32_t *s_nxt, const double *s_dZ,
+                             const double *s_sh, const uint16_t *mtab, uint32_t l, double freq,
+                             const SweepStore &store, const RevCentre &rc, GradSink &g)
+{
+    const uint32_t K = store.n;
+    /* outward entry iterator: step s, entry i = 1..m_s */
+    HalfIter it;
+    it.s = 0;
+    hi_load_step(kp, it, l, o, shift0, s_nxt, s_dZ, s_sh, mtab);
+    it.i = 1;
+    Entry inner;
+    inner.nr = o;
+    inner.nri = o;
+    inner.ifac = 0.0;
+    inner.shift = shift_centre;
+    g.p_crt = o;
+    g.p_nxt = it.nxt;
+    g.e_crt = g.k_crt = g.e_nxt = g.k_nxt = 0.0;
+
+    double u_in = rc.u, mu_in = rc.mu, lam_in = rc.lam, dbar = rc.dbar;
+    const double *q = store.st + (size_t)3 * MGX_WAVE * (K - 1);
+    double Y = q[0], X = q[MGX_WAVE], d = q[2 * MGX_WAVE];
+    for (uint32_t k = K; k-- > 0;) {
+        const Entry outer = hi_entry(it, o);
+        const double dz = it.dZ;
+        const double XI = rcp(1.0 + X);
+        const double u_k = fma(XI, u_in, Y);
+        const double mu_k = XI * mu_in;
+        const double Din = fma(-X * XI, u_in, Y); /* u_k - u_in */
+        double lam_k, dbar_next = 0.0, Yn = 0.0, Xn = 0.0, dn = 0.0;
+        if (k > 0) {
+            const double *qn = store.st + (size_t)3 * MGX_WAVE * (k - 1);
+            Yn = qn[0];
+            Xn = qn[MGX_WAVE];
+            dn = qn[2 * MGX_WAVE];
+            lam_k = 0.5 * (dn + d) * mu_k;
+            /* u_k - u_{k-1} = X_{k-1} XI_{k-1} u_k - Y_{k-1} */
+            const double XIn = rcp(1.0 + Xn);
+            const double Dout = fma(Xn * XIn, u_k, -Yn);
+            dbar -= lam_k * row_dd(d, dn, Din, Dout);
+            dbar_next = -lam_k * row_dd(dn, d, Dout, Din);
+        } else {
+            lam_k = 0.5 * d * mu_k;
+            const double Ib = boundary_intensity(kp, outer.nr, freq * outer.shift);
+            dbar -= lam_k * bdy_row_dd(d, Ib, u_k, Din);
+        }
+        double Sc_bar, Sn_bar;
+        if (!backward) {
+            Sc_bar = lam_k;
+            Sn_bar = (k == K - 1 && rc.centre_last) ? rc.lam : 0.0;
+            segment_bwd<APPROX, PREC>(kp, s_erf, ol, outer, inner, dz, freq, l, dbar, Sc_bar,
+                                      Sn_bar, g);
+        } else {
+            Sc_bar = lam_in;
+            Sn_bar = (k == 0) ? lam_k : 0.0;
+            segment_bwd<APPROX, PREC>(kp, s_erf, ol, inner, outer, dz, freq, l, dbar, Sc_bar,
+                                      Sn_bar, g);
+        }
+        if (k == 0) break;
+        /* move outward */
+        inner = outer;
+        u_in = u_k;
+        mu_in = mu_k;
+        lam_in = lam_k;
+        dbar = dbar_next;
+        Y = Yn;
+        X = Xn;
+        d = dn;
+        if (it.i < it.m_s) {
+            it.i++;
+        } else {
+            it.s++;
+            hi_load_step(kp, it, l, o, shift0, s_nxt, s_dZ, s_sh, mtab);
+            it.i = 1;
+            sink_step(g, it.nxt);
+        }
+    }
+    sink_finish(g);
+}
+
+/*
+ * Centre row of a ray pair after both sweeps: u at the centre, x_c = (T^-1)_cc
+ * (the Lambda diagonal), D_c and the centre row's derivative with respect to
+ * the depths of the innermost segments of the two halves.
+ */
+struct CentreRow {
+    double u, xc, Dc;
+    double dd_fw, dd_bw; /* dR_c/d(depth), rr side and antipod side */
+};
+
+template <int PREC>
+__device__ inline CentreRow centre_row(const KParams &kp, uint32_t o, double freq, double shift_c,
+                                       bool has_fw, bool has_bw, const Sweep &fw, const Sweep &bw)
+{
+    CentreRow c;
+    c.dd_fw = c.dd_bw = 0.0;
+    if (has_bw) {
+        const double dtau_c = bw.prev_dtau;
+        const double S_c = bw.S_inner;
+        double FF, Su, iC_c;
+        if (!has_fw) {
+            const double I_f = boundary_intensity(kp, o, freq * shift_c);
+            const double idt = rcp(dtau_c);
+            const double B_min = fma(2.0, idt, 1.0);
+            const double B = fma(2.0 * idt, idt, B_min);
+            iC_c = 0.5 * dtau_c * dtau_c;
+            Su = fma(2.0 * I_f, idt, S_c) * rcp(B);
+            FF = 0.5 * B_min * dtau_c * dtau_c;
+        } else {
+            const double avg = 0.5 * (fw.prev_dtau + dtau_c);
+            const double r = rcp(avg * fw.prev_dtau);
+            iC_c = avg * dtau_c;
+            FF = fma(r * fw.X, fw.XI, 1.0) * iC_c;
+            const double FI = rcp(1.0 + FF);
+            Su = fma(r, fw.Y, S_c) * FI * iC_c;
+        }
+        const double GP = bw.X * bw.XI;
+        const double rden = rcp(FF + GP);
+        c.u = fma(Su, 1.0 + FF, bw.Y) * rden;
+        c.xc = iC_c * rden;
+        const double Dbw = fma(GP, c.u, -bw.Y); /* u_c - u_{c+1} */
+        if (has_fw) {
+            const double Dfw = fma(fw.X * fw.XI, c.u, -fw.Y); /* u_c - u_{c-1} */
+            c.Dc = 0.5 * (fw.prev_dtau + dtau_c);
+            c.dd_fw = row_dd(fw.prev_dtau, dtau_c, Dfw, Dbw);
+            c.dd_bw = row_dd(dtau_c, fw.prev_dtau, Dbw, Dfw);
+        } else {
+            const double I_f = boundary_intensity(kp, o, freq * shift_c);
+            c.Dc = 0.5 * dtau_c;
+            c.dd_bw = bdy_row_dd(dtau_c, I_f, c.u, Dbw);
+        }
+    } else {
+        const double I_l = boundary_intensity(kp, o, freq * shift_c);
+        const double idt = rcp(fw.prev_dtau);
+        const double A_l = 2.0 * idt * idt;
+        const double B_min = fma(2.0, idt, 1.0);
+        const double B = B_min + A_l;
+        const double den = rcp(fma(B, fw.X, B_min));
+        const double Sl = fma(2.0 * I_l, idt, fw.S_inner);
+        c.u = fma(A_l, fw.Y, Sl) * (1.0 + fw.X) * den;
+        c.xc = (1.0 + fw.X) * den;
+        const double Dfw = fma(fw.X * fw.XI, c.u, -fw.Y);
+        c.Dc = 0.5 * fw.prev_dtau;
+        c.dd_fw = bdy_row_dd(fw.prev_dtau, I_l, c.u, Dfw);
+    }
+    return c;
+}
+
+/*
+ * Adjoint of the formal solve: given J_bar (P, F), adds (dJ/d eps)^T J_bar and
+ * (dJ/d kappa)^T J_bar into g_eps, g_kap (P, L).  Persistent grid: wave b takes
+ * origins b, b + G, ...; it keeps the elimination state of its current ray
+ * pair in its own slice of the scratch (2 halves x capE segments x 3 x 64).
+ * The lane layout is the forward kernel's (two 32-lane line slots); lanes past
+ * the last quadrature bin repeat it with a zero seed so every slot reduces its
+ * gradients over 32 lanes.
+ */
+template <int APPROX, int PREC>
+__global__ __launch_bounds__(MGX_WAVE, 2) void feautrier_adjoint_kernel(
+    const KParams *__restrict__ kpp, const double *__restrict__ J_bar, double *g_eps,
+    double *g_kap, double *scratch, uint32_t capE)
+{
+    const KParams &kp = *kpp;
+    extern __shared__ unsigned char smem[];
+    __shared__ double s_erf[MGX_ERF_NINT * (MGX_ERF_DEG + 1)];
+    for (uint32_t i = threadIdx.x; i < MGX_ERF_NINT * (MGX_ERF_DEG + 1); i += MGX_WAVE)
+        s_erf[i] = MGX_ERF_TAB[i];
+    const mgx_model &m = kp.m;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t cap = kp.cap;
+    const uint32_t P = m.npoints, F = m.nfreqs, Q = m.nquads;
+    uint32_t *s_nxt0 = reinterpret_cast<uint32_t *>(smem);
+    uint32_t *s_nxt1 = s_nxt0 + cap;
+    double *s_dZ0 = reinterpret_cast<double *>(smem + 8 * ((2 * cap * 4 + 7) / 8));
+    double *s_dZ1 = s_dZ0 + cap;
+    double *s_sh0 = s_dZ1 + cap;
+    double *s_sh1 = s_sh0 + cap;
+    uint16_t *s_m = reinterpret_cast<uint16_t *>(s_sh1 + cap);
+    double *st_wave = scratch + (size_t)blockIdx.x * 2 * capE * 3 * MGX_WAVE;
+
+    for (uint32_t o = blockIdx.x; o < P; o += gridDim.x) {
+        for (uint32_t rr = 0; rr < m.hnrays; rr++) {
+            const uint32_t ar = m.antipod[rr];
+            uint32_t S0, S1;
+            double sh00, sh01;
+            __syncthreads();
+            int e = walk_half(kp, o, rr, s_nxt0, s_dZ0, s_sh0, S0, sh00);
+            if (!e) e = walk_half(kp, o, ar, s_nxt1, s_dZ1, s_sh1, S1, sh01);
+            if (e) {
+                if (lane == 0) {
+                    atomicOr(&kp.status[0], (uint32_t)e);
+                    if (e == ERR_CAP) atomicMax(&kp.status[1], kp.cap + 1);
+                }
+                return;
+            }
+            __syncthreads();
+            if (S0 == 0 && S1 == 0) continue; /* u = I_bdy: no dependence */
+            const double w = ray_weight(m, o, rr);
+            const uint32_t Sall = S0 + S1;
+            for (uint32_t slot0 = 0; slot0 < kp.nslots; slot0 += 2) {
+                const uint32_t lA = slot0 / kp.slots_per_line;
+                const uint32_t lB = (slot0 + 1 < kp.nslots) ? (slot0 + 1) / kp.slots_per_line : lA;
+                __syncthreads();
+                for (uint32_t i = lane; i < 2 * Sall; i += MGX_WAVE) {
+                    const uint32_t j = i / Sall, r = i % Sall;
+                    const uint32_t h = (r < S0) ? 0 : 1;
+                    const uint32_t stp = h ? r - S0 : r;
+                    const uint32_t *nx = h ? s_nxt1 : s_nxt0;
+                    const uint32_t crt = (stp == 0) ? o : nx[stp - 1];
+                    const uint32_t mm = n_interp_for_line(kp, j ? lB : lA, crt, nx[stp]);
+                    if (mm > 0xffffu) atomicOr(&kp.status[0], (uint32_t)ERR_CAP);
+                    s_m[(j * 2 + h) * cap + stp] = (uint16_t)(mm > 0xffffu ? 0xffffu : mm);
+                }
+                __syncthreads();
+                const uint32_t jj = lane >> 5;
+                const uint32_t slot = slot0 + jj;
+                if (slot >= kp.nslots) continue; /* whole 32-lane half idle */
+                const uint32_t l = jj ? lB : lA;
+                const uint32_t z0 = (slot % kp.slots_per_line) * 32 + (lane & 31);
+                const uint32_t z = (z0 < Q) ? z0 : Q - 1;
+                const OwnLine ol = own_line(m, l);
+                const uint32_t f = l * Q + z;
+                const size_t of = (size_t)o * F + f;
+                const double freq = m.nu[of];
+                const double ubar = (z0 < Q) ? 2.0 * w * J_bar[of] : 0.0;
+                PointEC ec_cent = {0.0, 0.0, false};
+                Sweep bw, fw;
+                bw.started = fw.started = false;
+                SweepStore st0{st_wave + lane, capE, 0}, st1{st_wave + (size_t)capE * 3 * MGX_WAVE + lane, capE, 0};
+                if (S1 > 0)
+                    sweep_half<APPROX, PREC, true>(kp, s_erf, ol, true, S1, o, sh01, s_nxt1, s_dZ1,
+                                                   s_sh1, s_m + (jj * 2 + 1) * cap, l, freq, bw,
+                                                   ec_cent, &st1, 1.0);
+                if (S0 > 0)
+                    sweep_half<APPROX, PREC, true>(kp, s_erf, ol, false, S0, o, sh00, s_nxt0, s_dZ0,
+                                                   s_sh0, s_m + (jj * 2 + 0) * cap, l, freq, fw,
+                                                   ec_cent, &st0, 1.0);
+                if (st0.n > capE || st1.n > capE) {
+                    if ((lane & 31) == 0) {
+                        atomicOr(&kp.status[0], (uint32_t)ERR_CAPE);
+                        atomicMax(&kp.status[2], st0.n > st1.n ? st0.n : st1.n);
+                    }
+                    continue;
+                }
+                const CentreRow c = centre_row<PREC>(kp, o, freq, 1.0, S0 > 0, S1 > 0, fw, bw);
+                const double mu_c = ubar * c.xc * rcp(c.Dc);
+                const double lam_c = ubar * c.xc;
+                GradSink g{g_eps, g_kap, m.nlines, l, 32, 0, 0, 0.0, 0.0, 0.0, 0.0};
+                if (S0 > 0) {
+                    const RevCentre rc{c.u, mu_c, lam_c, -lam_c * c.dd_fw, S1 == 0};
+                    reverse_half<APPROX, PREC>(kp, s_erf, ol, false, S0, o, sh00, 1.0, s_nxt0,
+                                               s_dZ0, s_sh0, s_m + (jj * 2 + 0) * cap, l, freq,
+                                               st0, rc, g);
+                }
+                if (S1 > 0) {
+                    const RevCentre rc{c.u, mu_c, lam_c, -lam_c * c.dd_bw, false};
+                    reverse_half<APPROX, PREC>(kp, s_erf, ol, true, S1, o, sh01, 1.0, s_nxt1,
+                                               s_dZ1, s_sh1, s_m + (jj * 2 + 1) * cap, l, freq,
+                                               st1, rc, g);
+                }
+            }
+        }
+    }
+}
+
+/*
+ * Adjoint of the projection-surface image: given I_bar (nx*ny, F), adds
+ * (dI/d eps)^T I_bar and (dI/d kappa)^T I_bar into g_eps, g_kap.  One wave per
+ * pixel (persistent grid), lanes across image frequencies in chunks of 64
+ * (lanes past F repeat the last frequency with a zero seed).  I = 2 u_last -
+ * I_bdy, so the seed of u at the start point is 2 I_bar.
+ */
+template <int APPROX, int PREC>
+__global__ __launch_bounds__(MGX_WAVE, 2) void image_adjoint_kernel(
+    const KParams *__restrict__ kpp, const ImageFrame *frp, const double *__restrict__ I_bar,
+    double *g_eps, double *g_kap, double *scratch, uint32_t capE)
+{
+    const KParams &kp = *kpp;
+    extern __shared__ unsigned char smem[];
+    __shared__ double s_erf[MGX_ERF_NINT * (MGX_ERF_DEG + 1)];
+    for (uint32_t i = threadIdx.x; i < MGX_ERF_NINT * (MGX_ERF_DEG + 1); i += MGX_WAVE)
+        s_erf[i] = MGX_ERF_TAB[i];
+    const mgx_model &m = kp.m;
+    const ImageFrame fr = *frp;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t cap = kp.cap;
+    uint32_t *s_nxt = reinterpret_cast<uint32_t *>(smem);
+    double *s_dZ = reinterpret_cast<double *>(smem + 8 * ((2 * cap * 4 + 7) / 8));
+    double *s_sh = s_dZ + 2 * cap;
+    uint16_t *s_m = reinterpret_cast<uint16_t *>(s_dZ + 4 * cap);
+    const uint32_t F = m.nfreqs;
+    const uint32_t npix = fr.nx * fr.ny;
+    double *st_wave = scratch + (size_t)blockIdx.x * capE * 3 * MGX_WAVE;
+    const double *rd = fr.rd;
+
+    for (uint32_t pix = blockIdx.x; pix < npix; pix += gridDim.x) {
+        const uint32_t x_idx = pix / fr.ny, y_idx = pix % fr.ny;
+        const double xloc = fr.min_x + fr.deltax / 2.0 + x_idx * fr.deltax;
+        const double yloc = fr.min_y + fr.deltay / 2.0 + y_idx * fr.deltay;
+        double origin[3];
+        if (fr.denominator >= 1.0e-9) {
+            origin[0] = (xloc * fr.ix + yloc * fr.jx) + fr.cpt[0];
+            origin[1] = (xloc * fr.iy + yloc * fr.jy) + fr.cpt[1];
+            origin[2] = (yloc * fr.jz) + fr.cpt[2];
+        } else {
+            origin[0] = xloc + fr.cpt[0];
+            origin[1] = yloc + fr.cpt[1];
+            origin[2] = 0.0 + fr.cpt[2];
+        }
+        uint32_t ip = fr.start_bdy;
+        for (uint32_t it = 0; it <= m.nboundary; it++) {
+            const uint32_t na = bdy_closer(m, origin, rd, ip);
+            if (na == ip) break;
+            ip = na;
+        }
+        double Z;
+        {
+            double R[3];
+            for (int i = 0; i < 3; i++) R[i] = m.position[3 * (size_t)ip + i] - origin[i];
+            Z = dot3(rd, R);
+            if (m.spherical_symmetry) {
+                const double Rcos = dot3(origin, rd);
+                const double R2 = dot3(origin, origin);
+                const double Rsin2 = R2 - Rcos * Rcos;
+                const double *pb = &m.position[3 * (size_t)ip];
+                const double h2 = dot3(pb, pb) - Rsin2;
+                Z = (h2 > 0) ? (-sqrt(h2) - Rcos) : -Rcos;
+            }
+        }
+        const double shift_centre = shift_rest(m, origin, rd, ip, Z);
+        __syncthreads();
+        uint32_t S = 0;
+        double shift0 = 0.0;
+        {
+            double dZ = 0.0;
+            uint32_t crt = ip;
+            uint32_t nxt = m.spherical_symmetry ? next_spherical(kp, origin, rd, crt, Z, dZ)
+                                                : next_general(kp, origin, rd, crt, Z, dZ);
+            bool accessed_non_boundary = false;
+            if (nxt < m.npoints) {
+                shift0 = shift_rest(m, origin, rd, crt, Z - dZ);
+                double shift_nxt = shift_rest(m, origin, rd, nxt, Z);
+                for (;;) {
+                    if (S >= cap) {
+                        if (lane == 0) {
+                            atomicOr(&kp.status[0], (uint32_t)ERR_CAP);
+                            atomicMax(&kp.status[1], cap + 1);
+                        }
+                        return;
+                    }
+                    if (lane == 0) {
+                        s_nxt[S] = nxt;
+                        s_dZ[S] = dZ;
+                        s_sh[S] = shift_nxt;
+                    }
+                    S++;
+                    double rel[3];
+                    for (int i = 0; i < 3; i++) rel[i] = m.position[3 * (size_t)nxt + i] - m.center[i];
+                    const bool interior = m.point2boundary[nxt] == m.npoints;
+                    if (!(interior || dot3(rel, rd) < 0)) break;
+                    crt = nxt;
+                    if (interior) accessed_non_boundary = true;
+                    nxt = m.spherical_symmetry ? next_spherical(kp, origin, rd, crt, Z, dZ)
+                                               : next_general(kp, origin, rd, crt, Z, dZ);
+                    if (nxt >= m.npoints) break;
+                    shift_nxt = shift_rest(m, origin, rd, nxt, Z);
+                }
+            }
+            if (!accessed_non_boundary && !m.spherical_symmetry) S = 0;
+        }
+        __syncthreads();
+        for (uint32_t i = lane; i < S; i += MGX_WAVE) {
+            const uint32_t crt = (i == 0) ? ip : s_nxt[i - 1];
+            const uint32_t mm = n_interp_all(kp, crt, s_nxt[i]);
+            if (mm > 0xffffu) atomicOr(&kp.status[0], (uint32_t)ERR_CAP);
+            s_m[i] = (uint16_t)(mm > 0xffffu ? 0xffffu : mm);
+        }
+        __syncthreads();
+        if (S == 0) continue; /* I = I_bdy: no dependence */
+
+        for (uint32_t f0 = 0; f0 < F; f0 += MGX_WAVE) {
+            const uint32_t fl = f0 + lane;
+            const uint32_t f = (fl < F) ? fl : F - 1;
+            const double freq = m.nu[(size_t)ip * F + f];
+            const uint32_t line = m.f_line[f];
+            const uint32_t ub_idx = upper_bound_d(m.sorted_line, m.nlines, freq);
+            const uint32_t own_l = m.sorted_line_map[ub_idx > 0 ? ub_idx - 1 : 0];
+            const OwnLine ol = own_line(m, own_l);
+            const double ubar = (fl < F) ? 2.0 * I_bar[(size_t)pix * F + f] : 0.0;
+            Sweep fw;
+            fw.started = false;
+            PointEC ec = {0.0, 0.0, false};
+            SweepStore st{st_wave + lane, capE, 0};
+            sweep_half<APPROX, PREC, true>(kp, s_erf, ol, false, S, ip, shift0, s_nxt, s_dZ, s_sh,
+                                           s_m, line, freq, fw, ec, &st, shift_centre);
+            if (st.n > capE) {
+                if (lane == 0) {
+                    atomicOr(&kp.status[0], (uint32_t)ERR_CAPE);
+                    atomicMax(&kp.status[2], st.n);
+                }
+                continue;
+            }
+            const CentreRow c = centre_row<PREC>(kp, ip, freq, shift_centre, true, false, fw, fw);
+            const double mu_c = ubar * c.xc * rcp(c.Dc);
+            const double lam_c = ubar * c.xc;
+            const uint32_t l0 = __shfl(own_l, 0, MGX_WAVE);
+            const bool uniform = __all(own_l == l0);
+            GradSink g{g_eps, g_kap, m.nlines, own_l, uniform ? 64u : 0u, 0, 0, 0.0, 0.0, 0.0, 0.0};
+            const RevCentre rc{c.u, mu_c, lam_c, -lam_c * c.dd_fw, true};
+            reverse_half<APPROX, PREC>(kp, s_erf, ol, false, S, ip, shift0, shift_centre, s_nxt,
+                                       s_dZ, s_sh, s_m, line, freq, st, rc, g);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
 /* preparation kernels                                                       */
 
 __global__ void pack_kernel(const mgx_model m, double4 *lpack, PointRec *pts, PointIdx *pidx)
@@ -1602,9 +2339,9 @@ int prepare(const mgx_model *m, const mgx_options *op, hipStream_t st, DevBuffer
     HIP_TRY(hipMalloc(&b.lpack, PL * sizeof(double4)));
     HIP_TRY(hipMalloc(&b.pts, (size_t)m->npoints * sizeof(PointRec)));
     HIP_TRY(hipMalloc(&b.pidx, (size_t)m->npoints * sizeof(PointIdx)));
-    HIP_TRY(hipMalloc(&b.status, 2 * sizeof(uint32_t)));
+    HIP_TRY(hipMalloc(&b.status, 4 * sizeof(uint32_t)));
     HIP_TRY(hipMalloc(&b.kp_dev, sizeof(KParams)));
-    HIP_TRY(hipMemsetAsync(b.status, 0, 2 * sizeof(uint32_t), st));
+    HIP_TRY(hipMemsetAsync(b.status, 0, 4 * sizeof(uint32_t), st));
     const size_t n = PL > m->npoints ? PL : m->npoints;
     hipLaunchKernelGGL(pack_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, *m,
                        b.lpack, b.pts, b.pidx);
@@ -1621,13 +2358,15 @@ int prepare(const mgx_model *m, const mgx_options *op, hipStream_t st, DevBuffer
     return MGX_OK;
 }
 
-/* reads the status words; returns MGX_OK, or an error (needed capacity in *need) */
-int read_status(hipStream_t st, const DevBuffers &b, uint32_t *need)
+/* reads the status words; returns MGX_OK, or an error (needed capacity in *need;
+   segments needed by the adjoint's scratch in *need_e) */
+int read_status(hipStream_t st, const DevBuffers &b, uint32_t *need, uint32_t *need_e = nullptr)
 {
-    uint32_t h[2] = {0, 0};
+    uint32_t h[4] = {0, 0, 0, 0};
     HIP_TRY(hipMemcpyAsync(h, b.status, sizeof h, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     *need = h[1];
+    if (need_e) *need_e = (h[0] & ERR_CAPE) ? (h[2] ? h[2] : 1) : 0;
     if (h[0] & ERR_NO_NEXT)
         return mgx::fail(MGX_E_NO_NEXT, "a ray from an interior point found no valid next point");
     if (h[0] & ERR_OUT) return mgx::fail(MGX_E_RAY_LENGTH, "a traced list exceeds the output capacity");
@@ -1645,7 +2384,7 @@ int launch_with_capacity(hipStream_t st, DevBuffers &b, KParams &kp, Launch laun
     uint32_t cap = 128;
     for (;;) {
         kp.cap = cap;
-        HIP_TRY(hipMemsetAsync(b.status, 0, 2 * sizeof(uint32_t), st));
+        HIP_TRY(hipMemsetAsync(b.status, 0, 4 * sizeof(uint32_t), st));
         HIP_TRY(hipMemcpyAsync(b.kp_dev, &kp, sizeof(KParams), hipMemcpyHostToDevice, st));
         launch(walk_lds_bytes(cap));
         HIP_TRY(hipGetLastError());
@@ -1658,6 +2397,144 @@ int launch_with_capacity(hipStream_t st, DevBuffers &b, KParams &kp, Launch laun
             return mgx::fail(MGX_E_RAY_LENGTH, "a ray needs more than %u walk steps", max_cap);
         cap = cap * 2 > max_cap ? max_cap : cap * 2;
     }
+}
+
+/* argument checks, model preparation and the frame of the projection-surface
+   imager (Model::compute_image_new, model.cpp:1008-1035; the frame reduction
+   runs on the device); dfr is a device ImageFrame the caller frees */
+int image_setup(const mgx_model *model, const mgx_options *opts, const double *raydir, uint32_t nx,
+                uint32_t ny, hipStream_t st, DevBuffers &b, KParams &kp, ImageFrame *&dfr)
+{
+    if (!model || !opts || !raydir) return mgx::fail(MGX_E_ARG, "null argument");
+    if (nx == 0 || ny == 0) return mgx::fail(MGX_E_ARG, "the image needs at least one pixel");
+    const double sq = raydir[0] * raydir[0] + raydir[1] * raydir[1] + raydir[2] * raydir[2];
+    if (sq == 0.0)
+        return mgx::fail(MGX_E_ARG, "The given ray direction vector does not point in a "
+                                    "direction. Please use a non-zero (normed) direction vector "
+                                    "to generate an image.");
+    if (opts->approx == MGX_APPROX_ONE_LINE)
+        return mgx::fail(MGX_E_UNSUPPORTED, "One line approximation is not supported for imaging.");
+    if (!model->position || !model->nu || !model->boundary2point || !model->point2boundary ||
+        !model->interp_crit || !model->sorted_line || !model->emissivity)
+        return mgx::fail(MGX_E_ARG, "null array in model view");
+    if (model->npoints == 0 || model->nfreqs == 0 || model->nlines == 0 || model->nboundary == 0)
+        return mgx::fail(MGX_E_ARG, "empty model");
+    const double inv = 1 / std::sqrt(sq);
+    ImageFrame fr;
+    std::memset(&fr, 0, sizeof fr);
+    for (int i = 0; i < 3; i++) fr.rd[i] = raydir[i] * inv;
+    if (model->dimension == 1 && (fr.rd[0] != 0.0 || fr.rd[1] != 1.0 || fr.rd[2] != 0.0))
+        return mgx::fail(MGX_E_ARG, "In 1D, the image ray has to be (0,1,0)");
+    fr.nx = nx;
+    fr.ny = ny;
+    if (int e = prepare(model, opts, st, b, kp)) return e;
+    HIP_TRY(hipMalloc(&dfr, sizeof(ImageFrame)));
+    HIP_TRY(hipMemcpyAsync(dfr, &fr, sizeof fr, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(image_frame_kernel, dim3(1), dim3(256), 0, st, *model, dfr);
+    HIP_TRY(hipGetLastError());
+    return MGX_OK;
+}
+
+/* Device scratch of the adjoint kernels, kept between calls (grow-only; freed by
+   mgx_release_workspace).  Calls are serialised on it. */
+std::mutex g_ws_mutex;
+double *g_ws = nullptr;
+size_t g_ws_bytes = 0;
+uint32_t g_capE_hint = 128;
+
+int ensure_workspace(size_t bytes)
+{
+    if (bytes <= g_ws_bytes) return MGX_OK;
+    if (g_ws) {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(g_ws);
+        g_ws = nullptr;
+        g_ws_bytes = 0;
+    }
+    HIP_TRY(hipMalloc(&g_ws, bytes));
+    g_ws_bytes = bytes;
+    return MGX_OK;
+}
+
+/* waves of the persistent adjoint grid: resident waves per CU x CUs */
+template <typename K>
+uint32_t persistent_waves(K kernel, size_t lds, uint32_t work)
+{
+    int dev = 0, ncu = 0, nb = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, MGX_WAVE, lds) != hipSuccess ||
+        nb <= 0)
+        nb = 4;
+    const uint64_t g = (uint64_t)(ncu > 0 ? ncu : 256) * (uint64_t)nb;
+    return (uint32_t)(g < work ? g : work);
+}
+
+/*
+ * Launch loop of the adjoint kernels: clears the gradients and the status,
+ * launches, and retries with a larger LDS walk buffer (ERR_CAP) or a larger
+ * scratch (ERR_CAPE) until the launch completes.  launch(lds, grid, capE)
+ * receives the dynamic LDS size; grid_of(lds) the persistent grid size.
+ */
+template <typename Launch, typename Grid>
+int adjoint_launch(hipStream_t st, DevBuffers &b, KParams &kp, double *d_eps, double *d_kap,
+                   size_t PL, size_t per_wave_segments_factor, Grid grid_of, Launch launch)
+{
+    uint32_t cap = 128;
+    uint32_t capE = g_capE_hint;
+    for (;;) {
+        kp.cap = cap;
+        const size_t lds = walk_lds_bytes(cap);
+        const uint32_t grid = grid_of(lds);
+        const size_t bytes = (size_t)grid * per_wave_segments_factor * capE * 3 * MGX_WAVE *
+                             sizeof(double);
+        if (int e = ensure_workspace(bytes)) return e;
+        HIP_TRY(hipMemsetAsync(b.status, 0, 4 * sizeof(uint32_t), st));
+        HIP_TRY(hipMemsetAsync(d_eps, 0, PL * sizeof(double), st));
+        HIP_TRY(hipMemsetAsync(d_kap, 0, PL * sizeof(double), st));
+        HIP_TRY(hipMemcpyAsync(b.kp_dev, &kp, sizeof(KParams), hipMemcpyHostToDevice, st));
+        launch(lds, grid, capE);
+        HIP_TRY(hipGetLastError());
+        uint32_t need = 0, need_e = 0;
+        const int e = read_status(st, b, &need, &need_e);
+        if (e != MGX_OK && e != MGX_E_RAY_LENGTH) return e;
+        bool retry = false;
+        if (e == MGX_E_RAY_LENGTH) {
+            uint32_t max_cap = kMaxCap;
+            while (max_cap > 16 && walk_lds_bytes(max_cap) + 8 > kLdsBudget) max_cap -= 16;
+            if (cap >= max_cap)
+                return mgx::fail(MGX_E_RAY_LENGTH, "a ray needs more than %u walk steps", max_cap);
+            cap = cap * 2 > max_cap ? max_cap : cap * 2;
+            retry = true;
+        }
+        if (need_e) {
+            uint32_t c = capE * 2;
+            while (c < need_e) c *= 2;
+            if (c > (1u << 20))
+                return mgx::fail(MGX_E_RAY_LENGTH, "a ray has more than %u traced entries", 1u << 20);
+            capE = c;
+            g_capE_hint = c;
+            retry = true;
+        }
+        if (!retry) return MGX_OK;
+    }
+}
+
+template <int A, int PR>
+int run_J_adjoint(hipStream_t st, DevBuffers &b, KParams &kp, const double *J_bar, double *d_eps,
+                  double *d_kap)
+{
+    auto kern = feautrier_adjoint_kernel<A, PR>;
+    const uint32_t P = kp.m.npoints;
+    return adjoint_launch(
+        st, b, kp, d_eps, d_kap, (size_t)P * kp.m.nlines, 2,
+        [&](size_t lds) { return persistent_waves(kern, lds, P); },
+        [&](size_t lds, uint32_t grid, uint32_t capE) {
+            timed(st, [&] {
+                hipLaunchKernelGGL(kern, dim3(grid), dim3(MGX_WAVE), lds, st, b.kp_dev, J_bar, d_eps,
+                                   d_kap, g_ws, capE);
+            });
+        });
 }
 
 } // namespace
@@ -1717,42 +2594,16 @@ int mgx_image_device(const mgx_model *model, const mgx_options *opts, const doub
                      uint32_t nx, uint32_t ny, double *I, double *ImX, double *ImY, void *stream)
 {
     mgx::clear_error();
-    if (!model || !opts || !raydir || !I || !ImX || !ImY)
-        return mgx::fail(MGX_E_ARG, "null argument");
-    if (nx == 0 || ny == 0) return mgx::fail(MGX_E_ARG, "the image needs at least one pixel");
-    const double sq = raydir[0] * raydir[0] + raydir[1] * raydir[1] + raydir[2] * raydir[2];
-    if (sq == 0.0)
-        return mgx::fail(MGX_E_ARG, "The given ray direction vector does not point in a "
-                                    "direction. Please use a non-zero (normed) direction vector "
-                                    "to generate an image.");
-    if (opts->approx == MGX_APPROX_ONE_LINE)
-        return mgx::fail(MGX_E_UNSUPPORTED, "One line approximation is not supported for imaging.");
-    if (!model->position || !model->nu || !model->boundary2point || !model->point2boundary ||
-        !model->interp_crit || !model->sorted_line || !model->emissivity)
-        return mgx::fail(MGX_E_ARG, "null array in model view");
-    if (model->npoints == 0 || model->nfreqs == 0 || model->nlines == 0 || model->nboundary == 0)
-        return mgx::fail(MGX_E_ARG, "empty model");
-    const double inv = 1 / std::sqrt(sq);
-    ImageFrame fr;
-    std::memset(&fr, 0, sizeof fr);
-    for (int i = 0; i < 3; i++) fr.rd[i] = raydir[i] * inv;
-    if (model->dimension == 1 && (fr.rd[0] != 0.0 || fr.rd[1] != 1.0 || fr.rd[2] != 0.0))
-        return mgx::fail(MGX_E_ARG, "In 1D, the image ray has to be (0,1,0)");
-    fr.nx = nx;
-    fr.ny = ny;
+    if (!I || !ImX || !ImY) return mgx::fail(MGX_E_ARG, "null argument");
     hipStream_t st = (hipStream_t)stream;
     DevBuffers b;
     KParams kp;
-    if (int e = prepare(model, opts, st, b, kp)) return e;
     ImageFrame *dfr = nullptr;
-    HIP_TRY(hipMalloc(&dfr, sizeof(ImageFrame)));
+    if (int e = image_setup(model, opts, raydir, nx, ny, st, b, kp, dfr)) return e;
     struct Free {
         ImageFrame *p;
         ~Free() { (void)hipFree(p); }
     } free_fr{dfr};
-    HIP_TRY(hipMemcpyAsync(dfr, &fr, sizeof fr, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(image_frame_kernel, dim3(1), dim3(256), 0, st, *model, dfr);
-    HIP_TRY(hipGetLastError());
     return launch_with_capacity(st, b, kp, [&](size_t lds) {
         const dim3 grid(nx * ny), block(MGX_WAVE);
         timed(st, [&] {
@@ -1774,6 +2625,78 @@ int mgx_image_device(const mgx_model *model, const mgx_options *opts, const doub
             }
         });
     });
+}
+
+int mgx_feautrier_adjoint_device(const mgx_model *model, const mgx_options *opts,
+                                 const double *J_bar, double *d_eps, double *d_kap, void *stream)
+{
+    mgx::clear_error();
+    if (int e = check_view(model, opts)) return e;
+    if (!J_bar || !d_eps || !d_kap) return mgx::fail(MGX_E_ARG, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    DevBuffers b;
+    KParams kp;
+    if (int e = prepare(model, opts, st, b, kp)) return e;
+    const bool fp64 = opts->precision == MGX_PREC_FP64;
+    switch (opts->approx) {
+    case MGX_APPROX_NONE:
+        return fp64 ? run_J_adjoint<MGX_APPROX_NONE, MGX_PREC_FP64>(st, b, kp, J_bar, d_eps, d_kap)
+                    : run_J_adjoint<MGX_APPROX_NONE, MGX_PREC_MIXED>(st, b, kp, J_bar, d_eps, d_kap);
+    case MGX_APPROX_ONE_LINE:
+        return fp64 ? run_J_adjoint<MGX_APPROX_ONE_LINE, MGX_PREC_FP64>(st, b, kp, J_bar, d_eps, d_kap)
+                    : run_J_adjoint<MGX_APPROX_ONE_LINE, MGX_PREC_MIXED>(st, b, kp, J_bar, d_eps, d_kap);
+    default:
+        return fp64 ? run_J_adjoint<MGX_APPROX_CLOSE_LINES, MGX_PREC_FP64>(st, b, kp, J_bar, d_eps, d_kap)
+                    : run_J_adjoint<MGX_APPROX_CLOSE_LINES, MGX_PREC_MIXED>(st, b, kp, J_bar, d_eps, d_kap);
+    }
+}
+
+int mgx_image_adjoint_device(const mgx_model *model, const mgx_options *opts, const double *raydir,
+                             uint32_t nx, uint32_t ny, const double *I_bar, double *d_eps,
+                             double *d_kap, void *stream)
+{
+    mgx::clear_error();
+    if (!I_bar || !d_eps || !d_kap) return mgx::fail(MGX_E_ARG, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    DevBuffers b;
+    KParams kp;
+    ImageFrame *dfr = nullptr;
+    if (int e = image_setup(model, opts, raydir, nx, ny, st, b, kp, dfr)) return e;
+    struct Free {
+        ImageFrame *p;
+        ~Free() { (void)hipFree(p); }
+    } free_fr{dfr};
+    const size_t PL = (size_t)model->npoints * model->nlines;
+    const bool fp64 = opts->precision == MGX_PREC_FP64;
+    auto run = [&](auto kern) {
+        return adjoint_launch(
+            st, b, kp, d_eps, d_kap, PL, 1,
+            [&](size_t lds) { return persistent_waves(kern, lds, nx * ny); },
+            [&](size_t lds, uint32_t grid, uint32_t capE) {
+                timed(st, [&] {
+                    hipLaunchKernelGGL(kern, dim3(grid), dim3(MGX_WAVE), lds, st, b.kp_dev, dfr,
+                                       I_bar, d_eps, d_kap, g_ws, capE);
+                });
+            });
+    };
+    if (opts->approx == MGX_APPROX_NONE)
+        return fp64 ? run(image_adjoint_kernel<MGX_APPROX_NONE, MGX_PREC_FP64>)
+                    : run(image_adjoint_kernel<MGX_APPROX_NONE, MGX_PREC_MIXED>);
+    return fp64 ? run(image_adjoint_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_FP64>)
+                : run(image_adjoint_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_MIXED>);
+}
+
+void mgx_release_workspace(void)
+{
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    if (g_ws) {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(g_ws);
+    }
+    g_ws = nullptr;
+    g_ws_bytes = 0;
 }
 
 void mgx_set_kernel_timing(int enable) { g_timing = enable != 0; }

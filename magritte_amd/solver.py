@@ -35,8 +35,9 @@ class DeviceModel:
         return torch.cuda.current_stream(self.device).cuda_stream
 
 
-def feautrier_forward(dm, opts, with_u=True, with_lambda=True, out=None):
-    """Plain-method formal solve: returns dict(u=(hnrays,P,F), J=(P,F), lambda_diag=(P,L))."""
+def feautrier_forward(dm, opts, with_u=True, with_lambda=True, out=None, view=None):
+    """Plain-method formal solve: returns dict(u=(hnrays,P,F), J=(P,F), lambda_diag=(P,L)).
+    ``view`` replaces the resident model's view (e.g. with other emissivities)."""
     torch = _require_gpu()
     m = dm.model
     dev = dm.device
@@ -48,7 +49,7 @@ def feautrier_forward(dm, opts, with_u=True, with_lambda=True, out=None):
         out["lambda_diag"] = torch.empty((m.npoints, m.nlines), dtype=torch.float64,
                                          device=dev) if with_lambda else None
     _abi.check(_abi.lib().mgx_feautrier_forward_device(
-        C.byref(dm.view), C.byref(opts), _abi.ptr(out["u"]), _abi.ptr(out["J"]),
+        C.byref(dm.view if view is None else view), C.byref(opts), _abi.ptr(out["u"]), _abi.ptr(out["J"]),
         _abi.ptr(out["lambda_diag"]), dm.stream_ptr()))
     return out
 
@@ -101,18 +102,63 @@ def trace_stats(dm, opts):
     return tuple(int(x) for x in st)
 
 
-def image_new(dm, opts, raydir, nx, ny, freqs):
+def feautrier_adjoint(dm, opts, J_bar, view=None):
+    """(d_eps, d_kap), each (P, L): the gradient of sum(J_bar * J) with respect to
+    the line emissivity and opacity (mgx_feautrier_adjoint_device)."""
+    torch = _require_gpu()
+    m = dm.model
+    jb = J_bar.to(device=dm.device, dtype=torch.float64).contiguous()
+    if tuple(jb.shape) != (m.npoints, m.nfreqs):
+        raise ValueError(f"J_bar must be ({m.npoints}, {m.nfreqs})")
+    de = torch.empty((m.npoints, m.nlines), dtype=torch.float64, device=dm.device)
+    dk = torch.empty_like(de)
+    v = dm.view if view is None else view
+    _abi.check(_abi.lib().mgx_feautrier_adjoint_device(
+        C.byref(v), C.byref(opts), _abi.ptr(jb), _abi.ptr(de), _abi.ptr(dk), dm.stream_ptr()))
+    return de, dk
+
+
+def image_view(dm, freqs, arrays=None):
+    """mgx_model view of the resident model with the image frequency grid ``freqs``
+    (and any array replaced from ``arrays``)."""
+    torch = _require_gpu()
+    m = dm.model
+    t = dict(dm.tensors)
+    for k in m.FREQ_ARRAYS:
+        t[k] = torch.from_numpy(np.ascontiguousarray(freqs[k])).to(dm.device)
+    if arrays:
+        t.update(arrays)
+    return m.view(t, nfreqs=freqs["nu"].shape[1]), t
+
+
+def image_adjoint(dm, opts, raydir, nx, ny, freqs, I_bar, view=None):
+    """(d_eps, d_kap): the gradient of sum(I_bar * I) for image_new's I."""
+    torch = _require_gpu()
+    m = dm.model
+    F = freqs["nu"].shape[1]
+    ib = I_bar.to(device=dm.device, dtype=torch.float64).contiguous()
+    if tuple(ib.shape) != (nx * ny, F):
+        raise ValueError(f"I_bar must be ({nx * ny}, {F})")
+    if view is None:
+        view, keep = image_view(dm, freqs)
+    de = torch.empty((m.npoints, m.nlines), dtype=torch.float64, device=dm.device)
+    dk = torch.empty_like(de)
+    rd = np.ascontiguousarray(raydir, dtype=np.float64)
+    _abi.check(_abi.lib().mgx_image_adjoint_device(
+        C.byref(view), C.byref(opts), rd.ctypes.data, nx, ny, _abi.ptr(ib), _abi.ptr(de),
+        _abi.ptr(dk), dm.stream_ptr()))
+    return de, dk
+
+
+def image_new(dm, opts, raydir, nx, ny, freqs, view=None):
     """Projection-surface image along raydir (Model::compute_image_new): returns
     (I (nx*ny, F), ImX, ImY) as torch tensors; ``freqs`` is the image frequency
     grid from ModelData.image_frequencies."""
     torch = _require_gpu()
-    m = dm.model
     dev = dm.device
-    t = dict(dm.tensors)
-    for k in m.FREQ_ARRAYS:
-        t[k] = torch.from_numpy(np.ascontiguousarray(freqs[k])).to(dev)
     F = freqs["nu"].shape[1]
-    view = m.view(t, nfreqs=F)
+    if view is None:
+        view, keep = image_view(dm, freqs)
     I = torch.empty((nx * ny, F), dtype=torch.float64, device=dev)
     ImX = torch.empty(nx * ny, dtype=torch.float64, device=dev)
     ImY = torch.empty(nx * ny, dtype=torch.float64, device=dev)

@@ -52,3 +52,18 @@ def test_image_discretisation_rejects_zero_frequencies():
     rc = lib.mgx_compute_image_discretisation(4, 1.0, 2.0, 0, None, None, None, None, None, None)
     assert rc == _abi.MGX_E_ARG
     assert b"At least a single frequency" in lib.mgx_last_error()
+
+
+def test_adjoint_entry_points_reject_null_arguments(line_model):
+    lib = _abi.lib()
+    op = line_model.options()
+    v = line_model.view()
+    rc = lib.mgx_feautrier_adjoint_device(C.byref(v), C.byref(op), None, None, None, None)
+    assert rc == _abi.MGX_E_ARG
+    rd = (C.c_double * 3)(0.0, 1.0, 0.0)
+    rc = lib.mgx_image_adjoint_device(C.byref(v), C.byref(op), rd, 4, 4, None, None, None, None)
+    assert rc == _abi.MGX_E_ARG
+    bad = line_model.options()
+    bad.n_off_diag = 1
+    rc = lib.mgx_feautrier_adjoint_device(C.byref(v), C.byref(bad), None, None, None, None)
+    assert rc == _abi.MGX_E_UNSUPPORTED
