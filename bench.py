@@ -35,6 +35,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy 512 MiB three times (scale of rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
 
 
@@ -120,8 +122,26 @@ def run_variant(dm, model, precision, steps, warmup):
     return 1e3 * elapsed / steps, float(np.mean(kernel_ms)), out["J"].cpu().numpy()
 
 
+def calibrate_copy():
+    """A plain device copy of known size: under rocprofv3 --pmc FETCH_SIZE /
+    WRITE_SIZE it gives the factor between the counters (KiB per dispatch) and
+    the bytes a kernel actually moves."""
+    import torch
+
+    n = 64 * 1024 * 1024  # doubles: 512 MiB
+    src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n * 8, "copies": 3}), flush=True)
+
+
 def main():
     args = parse()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                           "(the multi-GPU sharding by ray pair is described in DESIGN.md)"}))
