@@ -47,6 +47,10 @@
 #ifndef MGX_SOLVE_WAVES_PER_SIMD
 #define MGX_SOLVE_WAVES_PER_SIMD 3
 #endif
+/* the same for the adjoint kernels */
+#ifndef MGX_ADJ_WAVES_PER_SIMD
+#define MGX_ADJ_WAVES_PER_SIMD 2
+#endif
 
 namespace {
 
@@ -702,22 +706,12 @@ struct Sweep {
     bool started;
 };
 
-/* Per-segment elimination state kept by the adjoint's forward sweeps: segment k
-   (counted from the outer end) stores Y, X and dtau of its outer row, lane-
-   interleaved: st[(3 k + c) * 64] for c = 0, 1, 2 (st already offset by the lane). */
-struct SweepStore {
-    double *st;
-    uint32_t cap; /* segments that fit */
-    uint32_t n;   /* segments of the half (may exceed cap: nothing is stored past it) */
-};
-
-template <int APPROX, int PREC, bool STORE = false>
+template <int APPROX, int PREC>
 __device__ inline void sweep_half(const KParams &kp, const double *s_erf, const OwnLine &ol,
                                   bool backward, uint32_t S, uint32_t o, double shift0,
                                   const uint32_t *s_nxt, const double *s_dZ, const double *s_sh,
                                   const uint16_t *mtab, uint32_t l, double freq, Sweep &sw,
-                                  PointEC &ec_centre, SweepStore *store = nullptr,
-                                  double shift_centre = 1.0)
+                                  PointEC &ec_centre)
 {
     HalfIter it;
     it.s = (int)S - 1;
@@ -725,11 +719,9 @@ __device__ inline void sweep_half(const KParams &kp, const double *s_erf, const 
     Entry outer = hi_entry(it, o);
     PointEC ec_out = {0.0, 0.0, false};
     sw.started = false;
-    if (STORE) store->n = 0;
     for (;;) {
         const double dz = hi_advance(kp, it, l, o, shift0, s_nxt, s_dZ, s_sh, mtab);
         Entry inner = hi_entry(it, o);
-        if (STORE && it.s < 0) inner.shift = shift_centre;
         PointEC ec_in = (it.s < 0) ? ec_centre : PointEC{0.0, 0.0, false};
         /* the segment runs from the rr-side entry to the antipod-side entry */
         const Entry &cu = backward ? inner : outer;
@@ -764,15 +756,6 @@ __device__ inline void sweep_half(const KParams &kp, const double *s_erf, const 
         sw.prev_dtau = dtau;
         sw.S_inner = S_in;
         sw.S_prev_row = S_in;
-        if (STORE) {
-            if (store->n < store->cap) {
-                double *q = store->st + (size_t)3 * MGX_WAVE * store->n;
-                q[0] = sw.Y;
-                q[MGX_WAVE] = sw.X;
-                q[2 * MGX_WAVE] = dtau;
-            }
-            store->n++;
-        }
         outer = inner;
         ec_out = ec_in;
         if (it.s < 0) break;
@@ -1513,6 +1496,222 @@ __global__ __launch_bounds__(MGX_WAVE) void image_kernel(const KParams *__restri
 /* branch of every segment, the CloseLines windows and the interpolation      */
 /* counts are held fixed.                                                     */
 
+/* Emissivity and opacity of an entry; phi is the line profile value when the
+   entry has only its own line in the CloseLines window and no interpolation
+   partner (then eta = phi eps and chi = chi_min + phi kappa), NaN otherwise. */
+struct PointECT {
+    double eta, chi, phi;
+    bool valid;
+};
+
+template <int APPROX, int PREC>
+__device__ inline void eta_chi_tape(const KParams &kp, const OwnLine &ol, const Entry &e,
+                                    uint32_t line, double freq, PointECT &ec)
+{
+    ec.valid = true;
+    ec.phi = __builtin_nan("");
+    if (APPROX == MGX_APPROX_CLOSE_LINES && e.ifac == 0.0) {
+        const double W = kp.op.max_distance_opacity_contribution * (freq * gld(&kp.pts[e.nr].ub));
+        const double left = freq - W, right = freq + W;
+        if (!(ol.a_g < left) && ol.a_lo < left && !(right < ol.a_g) && right < ol.a_hi) {
+            /* the arithmetic of eta_chi_interp for a one-line window at t = 0 */
+            const double4 a = lp(kp, e.nr, kp.m.sorted_line_map[ol.g]);
+            const double prof = gaussian<PREC>(a.x, freq - ol.a_g);
+            ec.eta = 0.0 + prof * a.y;
+            ec.chi = kp.op.min_opacity + prof * a.z;
+            ec.phi = prof;
+            return;
+        }
+        if ((ol.a_g < left && right < ol.a_hi) || (right < ol.a_g && ol.a_lo < left)) {
+            /* no line in the window: eta = 0, chi = chi_min (phi = 0 reproduces both) */
+            ec.eta = 0.0;
+            ec.chi = kp.op.min_opacity;
+            ec.phi = 0.0;
+            return;
+        }
+    }
+    eta_chi_interp<APPROX, PREC>(kp, ol, e.nr, e.nri, e.ifac, line, freq, ec.eta, ec.chi);
+}
+
+/* Segment kinds recorded for the reverse pass */
+#define MGX_SEG_ERF_GENERIC (-1.0)
+#define MGX_SEG_TRAPEZOID (-2.0)
+#define MGX_SEG_ERF_EMPTY (-3.0) /* no line in the window: no dependence on the lines */
+
+/*
+ * segment() for the adjoint's forward sweep: the same values, plus the kind of
+ * the segment for the reverse pass in g: g >= 0 for a line-integrated segment
+ * with only the own line in the window, no interpolation partner and no
+ * clamp (g = d dtau / d mean-opacity), MGX_SEG_ERF_GENERIC for the other
+ * line-integrated segments, MGX_SEG_TRAPEZOID for the trapezoid rule.
+ */
+template <int APPROX, int PREC>
+__device__ inline void segment_tape(const KParams &kp, const double *erf_tab, const OwnLine &ol,
+                                    const Entry &cu, const Entry &nx, double dZ, double freq,
+                                    uint32_t line, PointECT &ec_cu, PointECT &ec_nx,
+                                    double &dtau, double &S_cu, double &S_nx, double &g)
+{
+    const mgx_model &m = kp.m;
+    const double dshift = nx.shift - cu.shift;
+    const double dshift_abs = fabs(dshift);
+    const double a = gld(&kp.pts[cu.nr].dshift_max), b = gld(&kp.pts[nx.nr].dshift_max);
+    const double dshift_max = (b < a) ? b : a;
+    const double cf = freq * cu.shift, nf = freq * nx.shift;
+    if (dshift_abs > dshift_max) {
+        if (APPROX == MGX_APPROX_CLOSE_LINES && cu.ifac == 0.0 && nx.ifac == 0.0) {
+            double left_freq, right_freq;
+            if (cf < nf) {
+                left_freq = cf;
+                right_freq = nf;
+            } else {
+                right_freq = cf;
+                left_freq = nf;
+            }
+            const double cb = kp.op.max_distance_opacity_contribution * (right_freq * gld(&kp.pts[cu.nr].ub));
+            const double nb = kp.op.max_distance_opacity_contribution * (right_freq * gld(&kp.pts[nx.nr].ub));
+            const double W = (cb < nb) ? nb : cb;
+            const double left = left_freq - W, right = right_freq + W;
+            if (!(ol.a_g < left) && ol.a_lo < left && !(right < ol.a_g) && right < ol.a_hi) {
+                /* the arithmetic of S_dtau_line_integrated for one line at t = 0 */
+                const uint32_t l = m.sorted_line_map[ol.g];
+                const double4 c = lp(kp, cu.nr, l), n = lp(kp, nx.nr, l);
+                const double linefreq = m.line[l];
+                const double avg_iw = (c.x + n.x) / 2.0;
+                double line_dtau, dop;
+                if (cf == nf) {
+                    const double prof = gaussian<PREC>(avg_iw, cf - linefreq);
+                    line_dtau = dZ * (prof * ((c.z + n.z) / 2.0) + kp.op.min_opacity);
+                    dop = dZ * prof;
+                } else {
+                    const double next_pos = (linefreq - nf) * avg_iw;
+                    const double curr_pos = (linefreq - cf) * avg_iw;
+                    const double diff_pos = next_pos - curr_pos;
+                    const double avg_op = (n.z + c.z) / 2.0;
+                    const float erf_diff = f_erf<PREC>((float)next_pos, erf_tab) -
+                                           f_erf<PREC>((float)curr_pos, erf_tab);
+                    const double h = 0.5 * rcp(diff_pos);
+                    const double erfterm = avg_op * (double)erf_diff * h;
+                    line_dtau = dZ * fma(avg_iw, erfterm, kp.op.min_opacity);
+                    dop = dZ * (avg_iw * ((double)erf_diff * h));
+                }
+                double d = 0.0 + line_dtau;
+                const double sum_Sc = fma(line_dtau, c.w, 0.0), sum_Sn = fma(line_dtau, n.w, 0.0);
+                const double bound_min_dtau = kp.op.min_opacity * dZ;
+                const bool clamped = (-bound_min_dtau < d) && (d < bound_min_dtau);
+                if (-bound_min_dtau < d) d = (bound_min_dtau < d) ? d : bound_min_dtau;
+                dtau = d;
+                const double rd = rcp(d);
+                S_cu = sum_Sc * rd;
+                S_nx = sum_Sn * rd;
+                g = clamped ? MGX_SEG_ERF_GENERIC : dop;
+                return;
+            }
+            if ((ol.a_g < left && right < ol.a_hi) || (right < ol.a_g && ol.a_lo < left)) {
+                /* empty window: the sums are zero and the depth is clamped to its bound */
+                double d = 0.0;
+                const double bound_min_dtau = kp.op.min_opacity * dZ;
+                if (-bound_min_dtau < d) d = (bound_min_dtau < d) ? d : bound_min_dtau;
+                dtau = d;
+                const double rd = rcp(d);
+                S_cu = 0.0 * rd;
+                S_nx = 0.0 * rd;
+                g = MGX_SEG_ERF_EMPTY;
+                return;
+            }
+        }
+        S_dtau_line_integrated<APPROX, PREC>(kp, erf_tab, ol, cu.nr, nx.nr, cu.nri, nx.nri, line,
+                                             cf, nf, cu.ifac, nx.ifac, dZ, dtau, S_cu, S_nx);
+        g = MGX_SEG_ERF_GENERIC;
+        return;
+    }
+    if (!ec_cu.valid) eta_chi_tape<APPROX, PREC>(kp, ol, cu, line, cf, ec_cu);
+    if (!ec_nx.valid) eta_chi_tape<APPROX, PREC>(kp, ol, nx, line, nf, ec_nx);
+    S_cu = ec_cu.eta * rcp(ec_cu.chi);
+    S_nx = ec_nx.eta * rcp(ec_nx.chi);
+    dtau = 0.5 * (ec_cu.chi + ec_nx.chi) * dZ;
+    g = MGX_SEG_TRAPEZOID;
+}
+
+/* Tape of one half ray: segment k (counted from the outer end) stores Y, X,
+   dtau, the segment kind g and the profile value phi of its outer entry,
+   lane-interleaved: st[(5 k + c) * 64] (st already offset by the lane). */
+#define MGX_TAPE_WORDS 5
+struct SweepStore {
+    double *st;
+    uint32_t cap; /* segments that fit */
+    uint32_t n;   /* segments of the half (may exceed cap: nothing is stored past it) */
+    double I_b;   /* boundary intensity of the outer boundary row */
+};
+
+/* sweep_half of the forward kernel, recording the tape */
+template <int APPROX, int PREC>
+__device__ inline void sweep_half_tape(const KParams &kp, const double *s_erf, const OwnLine &ol,
+                                       bool backward, uint32_t S, uint32_t o, double shift0,
+                                       double shift_centre, const uint32_t *s_nxt,
+                                       const double *s_dZ, const double *s_sh,
+                                       const uint16_t *mtab, uint32_t l, double freq, Sweep &sw,
+                                       PointECT &ec_centre, SweepStore &store)
+{
+    HalfIter it;
+    it.s = (int)S - 1;
+    hi_load_step(kp, it, l, o, shift0, s_nxt, s_dZ, s_sh, mtab);
+    Entry outer = hi_entry(it, o);
+    PointECT ec_out = {0.0, 0.0, 0.0, false};
+    sw.started = false;
+    store.n = 0;
+    for (;;) {
+        const double dz = hi_advance(kp, it, l, o, shift0, s_nxt, s_dZ, s_sh, mtab);
+        Entry inner = hi_entry(it, o);
+        if (it.s < 0) inner.shift = shift_centre;
+        PointECT ec_in = (it.s < 0) ? ec_centre : PointECT{0.0, 0.0, 0.0, false};
+        const Entry &cu = backward ? inner : outer;
+        const Entry &nx = backward ? outer : inner;
+        PointECT &ec_cu = backward ? ec_in : ec_out;
+        PointECT &ec_nx = backward ? ec_out : ec_in;
+        double dtau, S_cu, S_nx, g;
+        segment_tape<APPROX, PREC>(kp, s_erf, ol, cu, nx, dz, freq, l, ec_cu, ec_nx, dtau, S_cu,
+                                   S_nx, g);
+        const double S_out = backward ? S_nx : S_cu;
+        const double S_in = backward ? S_cu : S_nx;
+        const double S_row = backward ? (sw.started ? sw.S_prev_row : S_out) : S_out;
+        if (!sw.started) {
+            const double I_b = boundary_intensity(kp, outer.nr, freq * outer.shift);
+            store.I_b = I_b;
+            const double idt = rcp(dtau);
+            const double B_min = fma(2.0, idt, 1.0);
+            const double B = fma(2.0 * idt, idt, B_min);
+            sw.Y = fma(2.0 * I_b, idt, S_row) * rcp(B);
+            sw.X = 0.5 * B_min * dtau * dtau;
+            sw.XI = rcp(1.0 + sw.X);
+            sw.started = true;
+        } else {
+            const double avg = 0.5 * (sw.prev_dtau + dtau);
+            const double q_out = avg * sw.prev_dtau;
+            const double q_in = avg * dtau;
+            const double r = rcp(q_out);
+            sw.X = fma(r * sw.X, sw.XI, 1.0) * q_in;
+            sw.XI = rcp(1.0 + sw.X);
+            sw.Y = fma(r, sw.Y, S_row) * sw.XI * q_in;
+        }
+        sw.prev_dtau = dtau;
+        sw.S_inner = S_in;
+        sw.S_prev_row = S_in;
+        if (store.n < store.cap) {
+            double *q = store.st + (size_t)MGX_TAPE_WORDS * MGX_WAVE * store.n;
+            q[0] = sw.Y;
+            q[MGX_WAVE] = sw.X;
+            q[2 * MGX_WAVE] = dtau;
+            q[3 * MGX_WAVE] = g;
+            q[4 * MGX_WAVE] = ec_out.valid ? ec_out.phi : __builtin_nan("");
+        }
+        store.n++;
+        outer = inner;
+        ec_out = ec_in;
+        if (it.s < 0) break;
+    }
+    ec_centre = ec_out;
+}
+
 /*
  * Gradient sink of one lane.  The outward walk of the reverse pass visits the
  * segments of walk step s (outer entry in step s) together, and all of them
@@ -1591,12 +1790,26 @@ __device__ inline void interp_log_partials(double a, double b, double t, double 
     db = (t == 0.0) ? 0.0 : t * r / b;
 }
 
-/* reverse of eta_chi_interp: eta_bar d(eta) + chi_bar d(chi) into the sink */
-template <int APPROX, int PREC>
-__device__ void eta_chi_interp_bwd(const KParams &kp, const OwnLine &ol, uint32_t p1, uint32_t p2,
-                                   double t, uint32_t ll, double freq, double eta_bar,
-                                   double chi_bar, GradSink &g)
+/* adds (de, dk) to the gradients of (p, l) directly (the rare paths of the reverse
+   pass, which do not go through a lane's sink) */
+__device__ inline void grad_put(double *g_eps, double *g_kap, uint32_t L, uint32_t p, uint32_t l,
+                                double de, double dk)
 {
+    const size_t i = (size_t)p * L + l;
+    if (de != 0.0) unsafeAtomicAdd(&g_eps[i], de);
+    if (dk != 0.0) unsafeAtomicAdd(&g_kap[i], dk);
+}
+
+/* reverse of eta_chi_interp: eta_bar d(eta) + chi_bar d(chi), added directly.
+   A rare path of the reverse pass (interpolated entries, several lines in the
+   window): kept out of line, with every argument by value so that the
+   caller's registers stay out of memory. */
+template <int APPROX, int PREC>
+__device__ __attribute__((noinline)) void eta_chi_interp_bwd(
+    const KParams *kpp, OwnLine ol, uint32_t p1, uint32_t p2, double t, uint32_t ll, double freq,
+    double eta_bar, double chi_bar, double *g_eps, double *g_kap)
+{
+    const KParams &kp = *kpp;
     const mgx_model &m = kp.m;
     uint32_t lo = 0, hi = m.nlines;
     if (APPROX == MGX_APPROX_ONE_LINE) {
@@ -1615,8 +1828,8 @@ __device__ void eta_chi_interp_bwd(const KParams &kp, const OwnLine &ol, uint32_
         interp_log_partials(a.y, b.y, t, interp_log<PREC>(a.y, b.y, t), de1, de2);
         interp_log_partials(a.z, b.z, t, interp_log<PREC>(a.z, b.z, t), dk1, dk2);
         const double pe = prof * eta_bar, pk = prof * chi_bar;
-        sink_add(g, p1, l, pe * de1, pk * dk1);
-        if (t != 0.0) sink_add(g, p2, l, pe * de2, pk * dk2);
+        grad_put(g_eps, g_kap, m.nlines, p1, l, pe * de1, pk * dk1);
+        if (t != 0.0) grad_put(g_eps, g_kap, m.nlines, p2, l, pe * de2, pk * dk2);
     }
 }
 
@@ -1639,41 +1852,44 @@ __device__ double dtau_single_line_dop(const double *erf_tab, double4 c, double4
 }
 
 /* reverse of the line terms at one end of a segment: opacity (kappa interpolation)
-   and line source (interpolation of eps/kappa) */
+   and line source (interpolation of eps/kappa), added directly */
 template <int PREC>
-__device__ inline void line_end_bwd(uint32_t p, uint32_t pi, uint32_t l, double t, double4 a,
-                                    double4 b, double op_bar, double S_bar, GradSink &g)
+__device__ inline void line_end_bwd(double *g_eps, double *g_kap, uint32_t L, uint32_t p,
+                                    uint32_t pi, uint32_t l, double t, double4 a, double4 b,
+                                    double op_bar, double S_bar)
 {
     double dk1, dk2, dR1, dR2;
     interp_log_partials(a.z, b.z, t, interp_log<PREC>(a.z, b.z, t), dk1, dk2);
     interp_log_partials(a.w, b.w, t, interp_log<PREC>(a.w, b.w, t), dR1, dR2);
     /* R = eps/kappa: dR/deps = 1/kappa, dR/dkappa = -R/kappa */
     const double r1 = S_bar * dR1 * rcp(a.z);
-    sink_add(g, p, l, r1, op_bar * dk1 - r1 * a.w);
+    grad_put(g_eps, g_kap, L, p, l, r1, op_bar * dk1 - r1 * a.w);
     if (t != 0.0) {
         const double r2 = S_bar * dR2 * rcp(b.z);
-        sink_add(g, pi, l, r2, op_bar * dk2 - r2 * b.w);
+        grad_put(g_eps, g_kap, L, pi, l, r2, op_bar * dk2 - r2 * b.w);
     }
 }
 
-/* reverse of S_dtau_line_integrated */
+/* reverse of S_dtau_line_integrated (the general case: out of line, arguments by
+   value, gradients added directly) */
 template <int APPROX, int PREC>
-__device__ void S_dtau_line_integrated_bwd(const KParams &kp, const double *erf_tab,
-                                           const OwnLine &ol, uint32_t cp, uint32_t np,
-                                           uint32_t cpi, uint32_t npi, uint32_t lineidx,
-                                           double currfreq, double nextfreq, double curr_interp,
-                                           double next_interp, double dZ, double dtau_bar,
-                                           double Sc_bar, double Sn_bar, GradSink &g)
+__device__ __attribute__((noinline)) void S_dtau_line_integrated_bwd(
+    const KParams *kpp, const double *erf_tab, OwnLine ol, uint32_t cp, uint32_t np, uint32_t cpi,
+    uint32_t npi, uint32_t lineidx, double currfreq, double nextfreq, double curr_interp,
+    double next_interp, double dZ, double dtau_bar, double Sc_bar, double Sn_bar, double *g_eps,
+    double *g_kap)
 {
+    const KParams &kp = *kpp;
     const mgx_model &m = kp.m;
+    const uint32_t L = m.nlines;
     if (APPROX == MGX_APPROX_ONE_LINE) {
         const double4 c = lp(kp, cp, lineidx), cI = lp(kp, cpi, lineidx);
         const double4 n = lp(kp, np, lineidx), nI = lp(kp, npi, lineidx);
         const double dop = dtau_single_line_dop<PREC>(erf_tab, c, cI, n, nI, m.line[lineidx],
                                                       currfreq, nextfreq, curr_interp,
                                                       next_interp, dZ);
-        line_end_bwd<PREC>(cp, cpi, lineidx, curr_interp, c, cI, 0.5 * dop * dtau_bar, Sc_bar, g);
-        line_end_bwd<PREC>(np, npi, lineidx, next_interp, n, nI, 0.5 * dop * dtau_bar, Sn_bar, g);
+        line_end_bwd<PREC>(g_eps, g_kap, L, cp, cpi, lineidx, curr_interp, c, cI, 0.5 * dop * dtau_bar, Sc_bar);
+        line_end_bwd<PREC>(g_eps, g_kap, L, np, npi, lineidx, next_interp, n, nI, 0.5 * dop * dtau_bar, Sn_bar);
         return;
     }
     uint32_t lo = 0, hi = m.nlines;
@@ -1690,6 +1906,55 @@ __device__ void S_dtau_line_integrated_bwd(const KParams &kp, const double *erf_
         const double nb = kp.op.max_distance_opacity_contribution * (right_freq * gld(&kp.pts[np].ub));
         const double W = (cb < nb) ? nb : cb;
         line_window(m, ol, left_freq - W, right_freq + W, lo, hi);
+    }
+    if (hi == lo + 1) {
+        /* one line in the window: a single pass (the erf difference serves the
+           optical depth and its derivative) */
+        const uint32_t l = (APPROX == MGX_APPROX_CLOSE_LINES) ? m.sorted_line_map[lo] : lo;
+        const double4 c = lp(kp, cp, l), cI = lp(kp, cpi, l);
+        const double4 n = lp(kp, np, l), nI = lp(kp, npi, l);
+        const double linefreq = m.line[l];
+        const double iw_c = interp_linear(c.x, cI.x, curr_interp);
+        const double iw_n = interp_linear(n.x, nI.x, next_interp);
+        const double avg_iw = (iw_c + iw_n) / 2.0;
+        const double op_c = interp_log<PREC>(c.z, cI.z, curr_interp);
+        const double op_n = interp_log<PREC>(n.z, nI.z, next_interp);
+        double line_dtau, dop;
+        if (currfreq == nextfreq) {
+            const double prof = gaussian<PREC>(avg_iw, currfreq - linefreq);
+            line_dtau = dZ * (prof * ((op_c + op_n) / 2.0) + kp.op.min_opacity);
+            dop = dZ * prof;
+        } else {
+            const double next_pos = (linefreq - nextfreq) * avg_iw;
+            const double curr_pos = (linefreq - currfreq) * avg_iw;
+            const double diff_pos = next_pos - curr_pos;
+            const double avg_op = (op_n + op_c) / 2.0;
+            const float erf_diff =
+                f_erf<PREC>((float)next_pos, erf_tab) - f_erf<PREC>((float)curr_pos, erf_tab);
+            const double ed = (double)erf_diff * (0.5 * rcp(diff_pos));
+            line_dtau = dZ * fma(avg_iw, avg_op * ed, kp.op.min_opacity);
+            dop = dZ * (avg_iw * ed);
+        }
+        const double Sc = interp_log<PREC>(c.w, cI.w, curr_interp);
+        const double Sn = interp_log<PREC>(n.w, nI.w, next_interp);
+        double d = line_dtau;
+        bool clamped = false;
+        if (APPROX == MGX_APPROX_CLOSE_LINES) {
+            const double bound_min_dtau = kp.op.min_opacity * dZ;
+            if (-bound_min_dtau < d && d < bound_min_dtau) {
+                d = bound_min_dtau;
+                clamped = true;
+            }
+        }
+        const double rd = rcp(d);
+        const double sSc_bar = Sc_bar * rd, sSn_bar = Sn_bar * rd;
+        const double sum_Sc = line_dtau * Sc, sum_Sn = line_dtau * Sn;
+        const double sdt_bar =
+            clamped ? 0.0 : dtau_bar - sSc_bar * (sum_Sc * rd) - sSn_bar * (sum_Sn * rd);
+        const double ld_bar = sdt_bar + sSc_bar * Sc + sSn_bar * Sn;
+        line_end_bwd<PREC>(g_eps, g_kap, L, cp, cpi, l, curr_interp, c, cI, 0.5 * dop * ld_bar, sSc_bar * line_dtau);
+        line_end_bwd<PREC>(g_eps, g_kap, L, np, npi, l, next_interp, n, nI, 0.5 * dop * ld_bar, sSn_bar * line_dtau);
+        return;
     }
     /* forward sums (the same operations as S_dtau_line_integrated) */
     double sum_dtau = 0.0, sum_Sc = 0.0, sum_Sn = 0.0;
@@ -1729,39 +1994,108 @@ __device__ void S_dtau_line_integrated_bwd(const KParams &kp, const double *erf_
         const double ld_bar = sdt_bar + sSc_bar * Sc + sSn_bar * Sn;
         const double dop = dtau_single_line_dop<PREC>(erf_tab, c, cI, n, nI, m.line[l], currfreq,
                                                       nextfreq, curr_interp, next_interp, dZ);
-        line_end_bwd<PREC>(cp, cpi, l, curr_interp, c, cI, 0.5 * dop * ld_bar, sSc_bar * line_dtau, g);
-        line_end_bwd<PREC>(np, npi, l, next_interp, n, nI, 0.5 * dop * ld_bar, sSn_bar * line_dtau, g);
+        line_end_bwd<PREC>(g_eps, g_kap, L, cp, cpi, l, curr_interp, c, cI, 0.5 * dop * ld_bar, sSc_bar * line_dtau);
+        line_end_bwd<PREC>(g_eps, g_kap, L, np, npi, l, next_interp, n, nI, 0.5 * dop * ld_bar, sSn_bar * line_dtau);
     }
 }
 
-/* reverse of segment(): dtau_bar, Sc_bar, Sn_bar are the adjoints of the
-   segment's optical depth and of its two source values */
+/* emissivity/opacity of an entry, its profile value (NaN: not of the simple
+   kind) and the adjoints collected for them from the trapezoid segments on
+   either side (one back-propagation per entry) */
+struct EntryBar {
+    double eta, chi, phi, eta_bar, chi_bar;
+    bool have;
+};
+
+template <int APPROX, int PREC>
+__device__ __attribute__((noinline)) double2 eta_chi_generic(const KParams *kpp, OwnLine ol,
+                                                             uint32_t nr, uint32_t nri,
+                                                             double ifac, double f,
+                                                             uint32_t line)
+{
+    double2 r;
+    eta_chi_interp<APPROX, PREC>(*kpp, ol, nr, nri, ifac, line, f, r.x, r.y);
+    return r;
+}
+
+/* eta and chi of an entry for the reverse pass: from the recorded profile value
+   when there is one (the forward arithmetic of eta_chi_tape), else recomputed */
+template <int APPROX, int PREC>
+__device__ inline void entry_eta_chi(const KParams &kp, const OwnLine &ol, const Entry &e,
+                                     double freq, uint32_t line, uint32_t lown, EntryBar &eb)
+{
+    if (eb.have) return;
+    if (eb.phi == eb.phi) {
+        const double4 a = lp(kp, e.nr, lown);
+        eb.eta = 0.0 + eb.phi * a.y;
+        eb.chi = kp.op.min_opacity + eb.phi * a.z;
+    } else {
+        const double2 r =
+            eta_chi_generic<APPROX, PREC>(&kp, ol, e.nr, e.nri, e.ifac, freq * e.shift, line);
+        eb.eta = r.x;
+        eb.chi = r.y;
+    }
+    eb.have = true;
+}
+
+/* back-propagates an entry's collected (eta, chi) adjoints */
+template <int APPROX, int PREC>
+__device__ inline void entry_bwd(const KParams &kp, const OwnLine &ol, const Entry &e, double freq,
+                                 uint32_t line, uint32_t lown, const EntryBar &eb, GradSink &g)
+{
+    if (eb.eta_bar == 0.0 && eb.chi_bar == 0.0) return;
+    if (eb.phi == eb.phi) {
+        /* eta = phi eps, chi = chi_min + phi kappa at this entry's own point */
+        sink_add(g, e.nr, lown, eb.phi * eb.eta_bar, eb.phi * eb.chi_bar);
+    } else {
+        eta_chi_interp_bwd<APPROX, PREC>(&kp, ol, e.nr, e.nri, e.ifac, line, freq * e.shift,
+                                         eb.eta_bar, eb.chi_bar, g.g_eps, g.g_kap);
+    }
+}
+
+/*
+ * Reverse of one segment.  dtau_bar, Sc_bar, Sn_bar are the adjoints of its
+ * optical depth and of its two source values; kind is the tape's g.
+ * Line-integrated segments scatter their gradients directly (from the recorded
+ * d dtau / d mean-opacity when g >= 0, else by recomputation); trapezoid
+ * segments add the adjoints of (eta, chi) at their two entries to ecu / enx.
+ */
 template <int APPROX, int PREC>
 __device__ void segment_bwd(const KParams &kp, const double *erf_tab, const OwnLine &ol,
                             const Entry &cu, const Entry &nx, double dZ, double freq,
-                            uint32_t line, double dtau_bar, double Sc_bar, double Sn_bar,
-                            GradSink &g)
+                            uint32_t line, uint32_t lown, double d, double kind, double dtau_bar,
+                            double Sc_bar, double Sn_bar, GradSink &g, EntryBar &ecu,
+                            EntryBar &enx)
 {
-    const double dshift_abs = fabs(nx.shift - cu.shift);
-    const double a = gld(&kp.pts[cu.nr].dshift_max), b = gld(&kp.pts[nx.nr].dshift_max);
-    const double dshift_max = (b < a) ? b : a;
-    const double cf = freq * cu.shift, nf = freq * nx.shift;
-    if (dshift_abs > dshift_max) {
-        S_dtau_line_integrated_bwd<APPROX, PREC>(kp, erf_tab, ol, cu.nr, nx.nr, cu.nri, nx.nri,
-                                                 line, cf, nf, cu.ifac, nx.ifac, dZ, dtau_bar,
-                                                 Sc_bar, Sn_bar, g);
+    if (kind >= 0.0) {
+        /* one line, no interpolation partner, not clamped: dtau = line_dtau = d */
+        const double4 c = lp(kp, cu.nr, lown), n = lp(kp, nx.nr, lown);
+        const double rd = rcp(d);
+        const double sSc_bar = Sc_bar * rd, sSn_bar = Sn_bar * rd;
+        const double sdt_bar = dtau_bar - sSc_bar * (fma(d, c.w, 0.0) * rd) -
+                               sSn_bar * (fma(d, n.w, 0.0) * rd);
+        const double ld_bar = sdt_bar + sSc_bar * c.w + sSn_bar * n.w;
+        const double op_bar = 0.5 * kind * ld_bar;
+        const double rc = sSc_bar * d * rcp(c.z), rn = sSn_bar * d * rcp(n.z);
+        sink_add(g, cu.nr, lown, rc, op_bar - rc * c.w);
+        sink_add(g, nx.nr, lown, rn, op_bar - rn * n.w);
         return;
     }
-    double eta_c, chi_c, eta_n, chi_n;
-    eta_chi_interp<APPROX, PREC>(kp, ol, cu.nr, cu.nri, cu.ifac, line, cf, eta_c, chi_c);
-    eta_chi_interp<APPROX, PREC>(kp, ol, nx.nr, nx.nri, nx.ifac, line, nf, eta_n, chi_n);
-    const double ric = rcp(chi_c), rin = rcp(chi_n);
-    const double chi_c_bar = 0.5 * dZ * dtau_bar - Sc_bar * eta_c * ric * ric;
-    const double chi_n_bar = 0.5 * dZ * dtau_bar - Sn_bar * eta_n * rin * rin;
-    eta_chi_interp_bwd<APPROX, PREC>(kp, ol, cu.nr, cu.nri, cu.ifac, line, cf, Sc_bar * ric,
-                                     chi_c_bar, g);
-    eta_chi_interp_bwd<APPROX, PREC>(kp, ol, nx.nr, nx.nri, nx.ifac, line, nf, Sn_bar * rin,
-                                     chi_n_bar, g);
+    if (kind == MGX_SEG_ERF_EMPTY) return;
+    const double cf = freq * cu.shift, nf = freq * nx.shift;
+    if (kind == MGX_SEG_ERF_GENERIC) {
+        S_dtau_line_integrated_bwd<APPROX, PREC>(&kp, erf_tab, ol, cu.nr, nx.nr, cu.nri, nx.nri,
+                                                 line, cf, nf, cu.ifac, nx.ifac, dZ, dtau_bar,
+                                                 Sc_bar, Sn_bar, g.g_eps, g.g_kap);
+        return;
+    }
+    entry_eta_chi<APPROX, PREC>(kp, ol, cu, freq, line, lown, ecu);
+    entry_eta_chi<APPROX, PREC>(kp, ol, nx, freq, line, lown, enx);
+    const double ric = rcp(ecu.chi), rin = rcp(enx.chi);
+    ecu.chi_bar += 0.5 * dZ * dtau_bar - Sc_bar * ecu.eta * ric * ric;
+    enx.chi_bar += 0.5 * dZ * dtau_bar - Sn_bar * enx.eta * rin * rin;
+    ecu.eta_bar += Sc_bar * ric;
+    enx.eta_bar += Sn_bar * rin;
 }
 
 /* derivative of a row  u + A_a (u - u_a) + A_b (u - u_b) = S  with respect to d_a,
@@ -1802,16 +2136,18 @@ struct RevCentre {
     double u, mu, lam; /* u, D^-1-scaled adjoint and adjoint of the centre row */
     double dbar;       /* the centre row's part of dL/d(depth of the innermost segment) */
     bool centre_last;  /* rr-side half only: the centre is the system's last row */
+    PointECT ec;       /* the centre entry's cache from the forward sweeps */
 };
 
 template <int APPROX, int PREC>
-__device__ void reverse_half(const KParams &kp, const double *s_erf, const OwnLine &ol,
+__device__ __forceinline__ void reverse_half(const KParams &kp, const double *s_erf, const OwnLine &ol,
                              bool backward, uint32_t S, uint32_t o, double shift0,
                              double shift_centre, const uint32_t *s_nxt, const double *s_dZ,
                              const double *s_sh, const uint16_t *mtab, uint32_t l, double freq,
-                             const SweepStore &store, const RevCentre &rc, GradSink &g)
+                             const SweepStore store, const RevCentre rc, GradSink g)
 {
     const uint32_t K = store.n;
+    const uint32_t lown = kp.m.sorted_line_map[ol.g];
     /* outward entry iterator: step s, entry i = 1..m_s */
     HalfIter it;
     it.s = 0;
@@ -1827,8 +2163,11 @@ __device__ void reverse_half(const KParams &kp, const double *s_erf, const OwnLi
     g.e_crt = g.k_crt = g.e_nxt = g.k_nxt = 0.0;
 
     double u_in = rc.u, mu_in = rc.mu, lam_in = rc.lam, dbar = rc.dbar;
-    const double *q = store.st + (size_t)3 * MGX_WAVE * (K - 1);
-    double Y = q[0], X = q[MGX_WAVE], d = q[2 * MGX_WAVE];
+    EntryBar ein = {rc.ec.eta, rc.ec.chi, rc.ec.valid ? rc.ec.phi : __builtin_nan(""), 0.0, 0.0,
+                    rc.ec.valid};
+    const double *q = store.st + (size_t)MGX_TAPE_WORDS * MGX_WAVE * (K - 1);
+    double Y = q[0], X = q[MGX_WAVE], d = q[2 * MGX_WAVE], kind = q[3 * MGX_WAVE],
+           phi = q[4 * MGX_WAVE];
     for (uint32_t k = K; k-- > 0;) {
         const Entry outer = hi_entry(it, o);
         const double dz = it.dZ;
@@ -1836,12 +2175,14 @@ __device__ void reverse_half(const KParams &kp, const double *s_erf, const OwnLi
         const double u_k = fma(XI, u_in, Y);
         const double mu_k = XI * mu_in;
         const double Din = fma(-X * XI, u_in, Y); /* u_k - u_in */
-        double lam_k, dbar_next = 0.0, Yn = 0.0, Xn = 0.0, dn = 0.0;
+        double lam_k, dbar_next = 0.0, Yn = 0.0, Xn = 0.0, dn = 0.0, kn = 0.0, pn = 0.0;
         if (k > 0) {
-            const double *qn = store.st + (size_t)3 * MGX_WAVE * (k - 1);
+            const double *qn = store.st + (size_t)MGX_TAPE_WORDS * MGX_WAVE * (k - 1);
             Yn = qn[0];
             Xn = qn[MGX_WAVE];
             dn = qn[2 * MGX_WAVE];
+            kn = qn[3 * MGX_WAVE];
+            pn = qn[4 * MGX_WAVE];
             lam_k = 0.5 * (dn + d) * mu_k;
             /* u_k - u_{k-1} = X_{k-1} XI_{k-1} u_k - Y_{k-1} */
             const double XIn = rcp(1.0 + Xn);
@@ -1850,23 +2191,26 @@ __device__ void reverse_half(const KParams &kp, const double *s_erf, const OwnLi
             dbar_next = -lam_k * row_dd(dn, d, Dout, Din);
         } else {
             lam_k = 0.5 * d * mu_k;
-            const double Ib = boundary_intensity(kp, outer.nr, freq * outer.shift);
-            dbar -= lam_k * bdy_row_dd(d, Ib, u_k, Din);
+            dbar -= lam_k * bdy_row_dd(d, store.I_b, u_k, Din);
         }
-        double Sc_bar, Sn_bar;
+        EntryBar eout = {0.0, 0.0, phi, 0.0, 0.0, false};
         if (!backward) {
-            Sc_bar = lam_k;
-            Sn_bar = (k == K - 1 && rc.centre_last) ? rc.lam : 0.0;
-            segment_bwd<APPROX, PREC>(kp, s_erf, ol, outer, inner, dz, freq, l, dbar, Sc_bar,
-                                      Sn_bar, g);
+            const double Sn_bar = (k == K - 1 && rc.centre_last) ? rc.lam : 0.0;
+            segment_bwd<APPROX, PREC>(kp, s_erf, ol, outer, inner, dz, freq, l, lown, d, kind,
+                                      dbar, lam_k, Sn_bar, g, eout, ein);
         } else {
-            Sc_bar = lam_in;
-            Sn_bar = (k == 0) ? lam_k : 0.0;
-            segment_bwd<APPROX, PREC>(kp, s_erf, ol, inner, outer, dz, freq, l, dbar, Sc_bar,
-                                      Sn_bar, g);
+            const double Sn_bar = (k == 0) ? lam_k : 0.0;
+            segment_bwd<APPROX, PREC>(kp, s_erf, ol, inner, outer, dz, freq, l, lown, d, kind,
+                                      dbar, lam_in, Sn_bar, g, ein, eout);
         }
-        if (k == 0) break;
+        /* the inner entry has all its contributions now */
+        entry_bwd<APPROX, PREC>(kp, ol, inner, freq, l, lown, ein, g);
+        if (k == 0) {
+            entry_bwd<APPROX, PREC>(kp, ol, outer, freq, l, lown, eout, g);
+            break;
+        }
         /* move outward */
+        ein = eout;
         inner = outer;
         u_in = u_k;
         mu_in = mu_k;
@@ -1875,6 +2219,8 @@ __device__ void reverse_half(const KParams &kp, const double *s_erf, const OwnLi
         Y = Yn;
         X = Xn;
         d = dn;
+        kind = kn;
+        phi = pn;
         if (it.i < it.m_s) {
             it.i++;
         } else {
@@ -1965,7 +2311,7 @@ __device__ inline CentreRow centre_row(const KParams &kp, uint32_t o, double fre
  * gradients over 32 lanes.
  */
 template <int APPROX, int PREC>
-__global__ __launch_bounds__(MGX_WAVE, 2) void feautrier_adjoint_kernel(
+__global__ __launch_bounds__(MGX_WAVE, MGX_ADJ_WAVES_PER_SIMD) void feautrier_adjoint_kernel(
     const KParams *__restrict__ kpp, const double *__restrict__ J_bar, double *g_eps,
     double *g_kap, double *scratch, uint32_t capE)
 {
@@ -1985,7 +2331,7 @@ __global__ __launch_bounds__(MGX_WAVE, 2) void feautrier_adjoint_kernel(
     double *s_sh0 = s_dZ1 + cap;
     double *s_sh1 = s_sh0 + cap;
     uint16_t *s_m = reinterpret_cast<uint16_t *>(s_sh1 + cap);
-    double *st_wave = scratch + (size_t)blockIdx.x * 2 * capE * 3 * MGX_WAVE;
+    double *st_wave = scratch + (size_t)blockIdx.x * 2 * capE * MGX_TAPE_WORDS * MGX_WAVE;
 
     for (uint32_t o = blockIdx.x; o < P; o += gridDim.x) {
         for (uint32_t rr = 0; rr < m.hnrays; rr++) {
@@ -2032,18 +2378,19 @@ __global__ __launch_bounds__(MGX_WAVE, 2) void feautrier_adjoint_kernel(
                 const size_t of = (size_t)o * F + f;
                 const double freq = m.nu[of];
                 const double ubar = (z0 < Q) ? 2.0 * w * J_bar[of] : 0.0;
-                PointEC ec_cent = {0.0, 0.0, false};
+                PointECT ec_cent = {0.0, 0.0, 0.0, false};
                 Sweep bw, fw;
                 bw.started = fw.started = false;
-                SweepStore st0{st_wave + lane, capE, 0}, st1{st_wave + (size_t)capE * 3 * MGX_WAVE + lane, capE, 0};
+                SweepStore st0{st_wave + lane, capE, 0, 0.0},
+                    st1{st_wave + (size_t)capE * MGX_TAPE_WORDS * MGX_WAVE + lane, capE, 0, 0.0};
                 if (S1 > 0)
-                    sweep_half<APPROX, PREC, true>(kp, s_erf, ol, true, S1, o, sh01, s_nxt1, s_dZ1,
-                                                   s_sh1, s_m + (jj * 2 + 1) * cap, l, freq, bw,
-                                                   ec_cent, &st1, 1.0);
+                    sweep_half_tape<APPROX, PREC>(kp, s_erf, ol, true, S1, o, sh01, 1.0, s_nxt1,
+                                                  s_dZ1, s_sh1, s_m + (jj * 2 + 1) * cap, l, freq,
+                                                  bw, ec_cent, st1);
                 if (S0 > 0)
-                    sweep_half<APPROX, PREC, true>(kp, s_erf, ol, false, S0, o, sh00, s_nxt0, s_dZ0,
-                                                   s_sh0, s_m + (jj * 2 + 0) * cap, l, freq, fw,
-                                                   ec_cent, &st0, 1.0);
+                    sweep_half_tape<APPROX, PREC>(kp, s_erf, ol, false, S0, o, sh00, 1.0, s_nxt0,
+                                                  s_dZ0, s_sh0, s_m + (jj * 2 + 0) * cap, l, freq,
+                                                  fw, ec_cent, st0);
                 if (st0.n > capE || st1.n > capE) {
                     if ((lane & 31) == 0) {
                         atomicOr(&kp.status[0], (uint32_t)ERR_CAPE);
@@ -2056,13 +2403,13 @@ __global__ __launch_bounds__(MGX_WAVE, 2) void feautrier_adjoint_kernel(
                 const double lam_c = ubar * c.xc;
                 GradSink g{g_eps, g_kap, m.nlines, l, 32, 0, 0, 0.0, 0.0, 0.0, 0.0};
                 if (S0 > 0) {
-                    const RevCentre rc{c.u, mu_c, lam_c, -lam_c * c.dd_fw, S1 == 0};
+                    const RevCentre rc{c.u, mu_c, lam_c, -lam_c * c.dd_fw, S1 == 0, ec_cent};
                     reverse_half<APPROX, PREC>(kp, s_erf, ol, false, S0, o, sh00, 1.0, s_nxt0,
                                                s_dZ0, s_sh0, s_m + (jj * 2 + 0) * cap, l, freq,
                                                st0, rc, g);
                 }
                 if (S1 > 0) {
-                    const RevCentre rc{c.u, mu_c, lam_c, -lam_c * c.dd_bw, false};
+                    const RevCentre rc{c.u, mu_c, lam_c, -lam_c * c.dd_bw, false, ec_cent};
                     reverse_half<APPROX, PREC>(kp, s_erf, ol, true, S1, o, sh01, 1.0, s_nxt1,
                                                s_dZ1, s_sh1, s_m + (jj * 2 + 1) * cap, l, freq,
                                                st1, rc, g);
@@ -2080,7 +2427,7 @@ __global__ __launch_bounds__(MGX_WAVE, 2) void feautrier_adjoint_kernel(
  * I_bdy, so the seed of u at the start point is 2 I_bar.
  */
 template <int APPROX, int PREC>
-__global__ __launch_bounds__(MGX_WAVE, 2) void image_adjoint_kernel(
+__global__ __launch_bounds__(MGX_WAVE, MGX_ADJ_WAVES_PER_SIMD) void image_adjoint_kernel(
     const KParams *__restrict__ kpp, const ImageFrame *frp, const double *__restrict__ I_bar,
     double *g_eps, double *g_kap, double *scratch, uint32_t capE)
 {
@@ -2099,7 +2446,7 @@ __global__ __launch_bounds__(MGX_WAVE, 2) void image_adjoint_kernel(
     uint16_t *s_m = reinterpret_cast<uint16_t *>(s_dZ + 4 * cap);
     const uint32_t F = m.nfreqs;
     const uint32_t npix = fr.nx * fr.ny;
-    double *st_wave = scratch + (size_t)blockIdx.x * capE * 3 * MGX_WAVE;
+    double *st_wave = scratch + (size_t)blockIdx.x * capE * MGX_TAPE_WORDS * MGX_WAVE;
     const double *rd = fr.rd;
 
     for (uint32_t pix = blockIdx.x; pix < npix; pix += gridDim.x) {
@@ -2198,10 +2545,10 @@ __global__ __launch_bounds__(MGX_WAVE, 2) void image_adjoint_kernel(
             const double ubar = (fl < F) ? 2.0 * I_bar[(size_t)pix * F + f] : 0.0;
             Sweep fw;
             fw.started = false;
-            PointEC ec = {0.0, 0.0, false};
-            SweepStore st{st_wave + lane, capE, 0};
-            sweep_half<APPROX, PREC, true>(kp, s_erf, ol, false, S, ip, shift0, s_nxt, s_dZ, s_sh,
-                                           s_m, line, freq, fw, ec, &st, shift_centre);
+            PointECT ec = {0.0, 0.0, 0.0, false};
+            SweepStore st{st_wave + lane, capE, 0, 0.0};
+            sweep_half_tape<APPROX, PREC>(kp, s_erf, ol, false, S, ip, shift0, shift_centre, s_nxt,
+                                          s_dZ, s_sh, s_m, line, freq, fw, ec, st);
             if (st.n > capE) {
                 if (lane == 0) {
                     atomicOr(&kp.status[0], (uint32_t)ERR_CAPE);
@@ -2215,7 +2562,7 @@ __global__ __launch_bounds__(MGX_WAVE, 2) void image_adjoint_kernel(
             const uint32_t l0 = __shfl(own_l, 0, MGX_WAVE);
             const bool uniform = __all(own_l == l0);
             GradSink g{g_eps, g_kap, m.nlines, own_l, uniform ? 64u : 0u, 0, 0, 0.0, 0.0, 0.0, 0.0};
-            const RevCentre rc{c.u, mu_c, lam_c, -lam_c * c.dd_fw, true};
+            const RevCentre rc{c.u, mu_c, lam_c, -lam_c * c.dd_fw, true, ec};
             reverse_half<APPROX, PREC>(kp, s_erf, ol, false, S, ip, shift0, shift_centre, s_nxt,
                                        s_dZ, s_sh, s_m, line, freq, st, rc, g);
         }
@@ -2486,8 +2833,8 @@ int adjoint_launch(hipStream_t st, DevBuffers &b, KParams &kp, double *d_eps, do
         kp.cap = cap;
         const size_t lds = walk_lds_bytes(cap);
         const uint32_t grid = grid_of(lds);
-        const size_t bytes = (size_t)grid * per_wave_segments_factor * capE * 3 * MGX_WAVE *
-                             sizeof(double);
+        const size_t bytes = (size_t)grid * per_wave_segments_factor * capE * MGX_TAPE_WORDS *
+                             MGX_WAVE * sizeof(double);
         if (int e = ensure_workspace(bytes)) return e;
         HIP_TRY(hipMemsetAsync(b.status, 0, 4 * sizeof(uint32_t), st));
         HIP_TRY(hipMemsetAsync(d_eps, 0, PL * sizeof(double), st));
