@@ -35,6 +35,10 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--workload", default="wind32", choices=["wind32", "batch"],
+                    help="wind32: the single-model solve (default); batch: many winds in one "
+                         "persistent launch against per-model launches")
+    ap.add_argument("--n-models", type=int, default=64)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 512 MiB three times (scale of rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
@@ -137,10 +141,74 @@ def calibrate_copy():
     print(json.dumps({"copy_bytes": n * 8, "copies": 3}), flush=True)
 
 
+def run_batch(args):
+    """Batched forward solve of n winds (point counts log-uniform in [2e3, 1.3e5],
+    12/48/192 rays, the same rotor): per-model launches against one persistent
+    launch with the autotuned tiling; value = total u values per second."""
+    import numpy as np
+    import torch
+
+    from magritte_amd import _abi, models, solver
+
+    t0 = time.time()
+    specs = models.batch_wind_specs(n_models=args.n_models)
+    ms = []
+    for k, (e, r, sd) in enumerate(specs):
+        ms.append(models.model_wind(n_edge=e, nrays=r, seed=sd))
+        print(f"# model {k}: n_edge={e} P={e**3} nrays={r}", file=sys.stderr, flush=True)
+    dms = [solver.DeviceModel(m, "cuda:0") for m in ms]
+    U = sum(m.hnrays * m.npoints * m.nfreqs for m in ms)
+    print(f"# {len(ms)} models built in {time.time() - t0:.1f}s, U={U:.3e}", file=sys.stderr,
+          flush=True)
+    res = {}
+    for prec, name in ((_abi.PREC_MIXED, "mixed"), (_abi.PREC_FP64, "fp64")):
+        op = ms[0].options(precision=prec)
+        outs = [None] * len(ms)
+        for k, dm in enumerate(dms):  # warm-up
+            outs[k] = solver.feautrier_forward(dm, op, with_u=False, with_lambda=False)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for k, dm in enumerate(dms):
+            solver.feautrier_forward(dm, op, with_u=False, with_lambda=False, out=outs[k])
+        torch.cuda.synchronize()
+        t_plain = time.perf_counter() - t1
+        print(f"# {name}: per-model launches {t_plain * 1e3:.1f} ms", file=sys.stderr, flush=True)
+        plan = solver.BatchPlan(dms, op)
+        best = plan.tune(sample_stride=16)
+        plan.run()  # warm-up (graph capture)
+        torch.cuda.synchronize()
+        steps = max(1, args.steps // 5)
+        t2 = time.perf_counter()
+        for _ in range(steps):
+            Js = plan.run()
+        torch.cuda.synchronize()
+        t_batch = (time.perf_counter() - t2) / steps
+        err = max(float(np.max(np.abs(J.cpu().numpy() - o["J"].cpu().numpy()) /
+                               np.abs(o["J"].cpu().numpy()))) for J, o in zip(Js, outs))
+        plan.close()
+        res[name] = {"plain_ms": round(t_plain * 1e3, 2), "batch_ms": round(t_batch * 1e3, 2),
+                     "speedup": round(t_plain / t_batch, 3), "tuned": best,
+                     "value": round(U / t_batch, 1), "J_rel_diff_batch_vs_plain": err}
+        print(f"# {name}: batch {t_batch * 1e3:.1f} ms, tuned {best}", file=sys.stderr, flush=True)
+    line = {"metric": "Feautrier ray.point.freq/s (batched forward)", "value": res["mixed"]["value"],
+            "unit": "ray*point*freq/s", "n_gpus": 1, "steps": max(1, args.steps // 5),
+            "higher_is_better": True, "dtype": "f32+f64", "data": "synthetic",
+            "config": {"workload": f"{len(ms)} jittered spherical winds, P log-uniform "
+                                   f"[2e3, 1.3e5], 12/48/192 HEALPix rays, 9 lines x 31 bins",
+                       "U": U, "variants": res}}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.workload == "batch":
+        if args.gpus != 1:
+            print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+            sys.exit(2)
+        run_batch(args)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "

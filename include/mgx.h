@@ -206,6 +206,46 @@ int mgx_image_adjoint_device(const mgx_model *model_dev, const mgx_options *opts
 void mgx_release_workspace(void);
 
 /* ------------------------------------------------------------------------ */
+/* Batched forward solve.  No counterpart in the reference (which solves one */
+/* model per call): many models of uneven size and ray count in one          */
+/* persistent launch, captured in a HIP graph, with an autotuner.            */
+/* ------------------------------------------------------------------------ */
+
+typedef struct mgx_batch mgx_batch;
+
+/*
+ * Plan of a batch: n device model views (models[i] as for
+ * mgx_feautrier_forward_device, all with the same options), the device
+ * outputs J[i] (P_i, F_i) and lambda_diag[i] (P_i, L_i) (lambda_diag may be
+ * NULL).  Packs every model's tables and builds the work list (one item per
+ * origin, or per block of ray pairs).  Results of mgx_batch_run equal those of
+ * n calls of mgx_feautrier_forward_device (within 1e-15 relative when ray
+ * pairs are split over items: their J contributions are added atomically).
+ */
+int mgx_batch_create(const mgx_model *models, uint32_t n, const mgx_options *opts,
+                     double *const *J, double *const *lambda_diag, mgx_batch **out,
+                     void *stream);
+
+/* Sets the tiling by hand: ray pairs per work item (0 = all of an origin),
+ * item order (0 = decreasing cost, 1 = model by model), register budget in
+ * waves per SIMD (2, 3 or 4), and whether runs replay a captured HIP graph. */
+int mgx_batch_configure(mgx_batch *batch, uint32_t rays_per_item, uint32_t order, uint32_t wps,
+                        int use_graph, void *stream);
+
+/* Autotuner: times the candidate tilings on every sample_stride-th work item
+ * and keeps the fastest; best (3 words, may be NULL) receives it.  Leaves the
+ * outputs undefined until the next mgx_batch_run. */
+int mgx_batch_tune(mgx_batch *batch, uint32_t sample_stride, uint32_t *best, void *stream);
+
+/* Runs the batch (line tables, the persistent kernel; synchronises). */
+int mgx_batch_run(mgx_batch *batch, void *stream);
+
+/* Duration (ms) of the last run when kernel timing is enabled. */
+double mgx_batch_last_ms(const mgx_batch *batch);
+
+void mgx_batch_destroy(mgx_batch *batch);
+
+/* ------------------------------------------------------------------------ */
 /* Host-side producers of the path's inputs (plain host pointers).           */
 /* ------------------------------------------------------------------------ */
 

@@ -92,6 +92,14 @@ SIGNATURES = [
      [C.POINTER(MgxModel), C.POINTER(MgxOptions), _VP, C.c_uint32, C.c_uint32, _VP, _VP, _VP,
       _VP]),
     ("mgx_release_workspace", None, []),
+    ("mgx_batch_create", C.c_int,
+     [C.POINTER(MgxModel), C.c_uint32, C.POINTER(MgxOptions), _VP, _VP, C.POINTER(_VP), _VP]),
+    ("mgx_batch_configure", C.c_int,
+     [_VP, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, _VP]),
+    ("mgx_batch_tune", C.c_int, [_VP, C.c_uint32, _VP, _VP]),
+    ("mgx_batch_run", C.c_int, [_VP, _VP]),
+    ("mgx_batch_last_ms", C.c_double, [_VP]),
+    ("mgx_batch_destroy", None, [_VP]),
     ("mgx_compute_inverse_widths", C.c_int,
      [C.c_uint32, C.c_uint32, C.POINTER(MgxLinedata), _VP, _VP, _VP]),
     ("mgx_compute_lte", C.c_int,

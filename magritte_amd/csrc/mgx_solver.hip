@@ -34,6 +34,8 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <algorithm>
+#include <cmath>
 #include <mutex>
 #include <vector>
 
@@ -763,20 +765,21 @@ __device__ inline void sweep_half(const KParams &kp, const double *s_erf, const 
     ec_centre = ec_out;
 }
 
+/*
+ * Ray pairs rr0..rr1-1 of origin o (the body of feautrier_kernel): walk, per-
+ * pass interpolation counts, two-sided elimination per lane, u stored when
+ * given, J and the Lambda diagonal added into the LDS accumulators s_J (F) and
+ * s_L (L).  Returns 0, or an ERR_* code after flagging it in kp.status.
+ */
 template <int APPROX, int PREC>
-__global__ __launch_bounds__(MGX_WAVE, MGX_SOLVE_WAVES_PER_SIMD) void feautrier_kernel(
-    const KParams *__restrict__ kpp, double *u, double *J, double *lam)
+__device__ __forceinline__ int solve_origin(const KParams &kp, uint32_t o, uint32_t rr0,
+                                            uint32_t rr1, unsigned char *smem, const double *s_erf,
+                                            double *u, bool with_lam, double *s_J, double *s_L)
 {
-    const KParams &kp = *kpp;
-    extern __shared__ unsigned char smem[];
-    __shared__ double s_erf[MGX_ERF_NINT * (MGX_ERF_DEG + 1)];
-    for (uint32_t i = threadIdx.x; i < MGX_ERF_NINT * (MGX_ERF_DEG + 1); i += MGX_WAVE)
-        s_erf[i] = MGX_ERF_TAB[i];
     const mgx_model &m = kp.m;
-    const uint32_t o = blockIdx.x;
     const uint32_t lane = threadIdx.x;
     const uint32_t cap = kp.cap;
-    const uint32_t P = m.npoints, F = m.nfreqs, L = m.nlines, Q = m.nquads;
+    const uint32_t P = m.npoints, F = m.nfreqs, Q = m.nquads;
     uint32_t *s_nxt0 = reinterpret_cast<uint32_t *>(smem);
     uint32_t *s_nxt1 = s_nxt0 + cap;
     double *s_dZ0 = reinterpret_cast<double *>(smem + 8 * ((2 * cap * 4 + 7) / 8));
@@ -784,13 +787,9 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_SOLVE_WAVES_PER_SIMD) void feautrier_
     double *s_sh0 = s_dZ1 + cap;
     double *s_sh1 = s_sh0 + cap;
     uint16_t *s_m = reinterpret_cast<uint16_t *>(s_sh1 + cap); /* [line j][half h][step] */
-    double *s_J = reinterpret_cast<double *>(smem + ((walk_lds_bytes(cap) + 7) / 8) * 8);
-    double *s_L = s_J + F;
-    for (uint32_t i = lane; i < F + L; i += MGX_WAVE) s_J[i] = 0.0;
-
     const size_t PF = (size_t)P * F;
 
-    for (uint32_t rr = 0; rr < m.hnrays; rr++) {
+    for (uint32_t rr = rr0; rr < rr1; rr++) {
         const uint32_t ar = m.antipod[rr];
         uint32_t S0, S1;
         double sh00, sh01;
@@ -801,7 +800,7 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_SOLVE_WAVES_PER_SIMD) void feautrier_
                 atomicOr(&kp.status[0], (uint32_t)e);
                 if (e == ERR_CAP) atomicMax(&kp.status[1], kp.cap + 1);
             }
-            return;
+            return e;
         }
         __syncthreads();
         const double w = ray_weight(m, o, rr);
@@ -890,7 +889,7 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_SOLVE_WAVES_PER_SIMD) void feautrier_
                     }
                     s_J[f] = s_J[f] + uc * 2.0 * w;
                     /* ---- Lambda diagonal (update_Lambda, solver.tpp:1761-1800) ---- */
-                    if (lam && m.f_in_line[f]) {
+                    if (with_lam && m.f_in_line[f]) {
                         const uint32_t spec = m.f_species[f];
                         const uint32_t k = m.f_k[f];
                         const double freq_line = m.line[l];
@@ -918,7 +917,7 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_SOLVE_WAVES_PER_SIMD) void feautrier_
             }
             /* per-line sum of the Lambda contributions over the 32 lanes of a slot;
                lane 0 then adds both slots of the pass in order (they may share a line) */
-            if (lam) {
+            if (with_lam) {
                 double v = Lval;
 #pragma unroll
                 for (int off = 16; off >= 1; off >>= 1) v += __shfl_xor(v, off, MGX_WAVE);
@@ -935,9 +934,103 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_SOLVE_WAVES_PER_SIMD) void feautrier_
             __syncthreads();
         }
     }
+    return 0;
+}
+
+template <int APPROX, int PREC>
+__global__ __launch_bounds__(MGX_WAVE, MGX_SOLVE_WAVES_PER_SIMD) void feautrier_kernel(
+    const KParams *__restrict__ kpp, double *u, double *J, double *lam)
+{
+    const KParams &kp = *kpp;
+    extern __shared__ unsigned char smem[];
+    __shared__ double s_erf[MGX_ERF_NINT * (MGX_ERF_DEG + 1)];
+    for (uint32_t i = threadIdx.x; i < MGX_ERF_NINT * (MGX_ERF_DEG + 1); i += MGX_WAVE)
+        s_erf[i] = MGX_ERF_TAB[i];
+    const mgx_model &m = kp.m;
+    const uint32_t o = blockIdx.x;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t F = m.nfreqs, L = m.nlines;
+    double *s_J = reinterpret_cast<double *>(smem + ((walk_lds_bytes(kp.cap) + 7) / 8) * 8);
+    double *s_L = s_J + F;
+    for (uint32_t i = lane; i < F + L; i += MGX_WAVE) s_J[i] = 0.0;
+    if (solve_origin<APPROX, PREC>(kp, o, 0, m.hnrays, smem, s_erf, u, lam != nullptr, s_J, s_L))
+        return;
     for (uint32_t i = lane; i < F; i += MGX_WAVE) J[(size_t)o * F + i] = s_J[i];
     if (lam)
         for (uint32_t i = lane; i < L; i += MGX_WAVE) lam[(size_t)o * L + i] = s_L[i];
+}
+
+/* ------------------------------------------------------------------------- */
+/* batched forward solve: many models in one persistent launch                */
+
+/* one work item: ray pairs rr0..rr1-1 of origin `origin` of model `model` */
+struct BatchItem {
+    uint32_t model, origin, rr0, rr1;
+};
+
+/*
+ * Persistent kernel over a list of work items from several models (each with
+ * its own KParams: geometry, line tables, sizes).  A wave takes the next item
+ * from an atomic counter, runs solve_origin for it and writes J and the Lambda
+ * diagonal of that origin: stored when the item covers all ray pairs of the
+ * origin, added atomically when the origin's ray pairs are split over items
+ * (the outputs are then cleared before the launch).  WPS is the register
+ * budget in waves per SIMD, one of the autotuner's knobs.
+ */
+template <int APPROX, int PREC, int WPS>
+__global__ __launch_bounds__(MGX_WAVE, WPS) void feautrier_batch_kernel(
+    const KParams *__restrict__ kps, const BatchItem *__restrict__ items, uint32_t n_items,
+    uint32_t *counter, double *const *__restrict__ J, double *const *__restrict__ lam)
+{
+    extern __shared__ unsigned char smem[];
+    __shared__ double s_erf[MGX_ERF_NINT * (MGX_ERF_DEG + 1)];
+    __shared__ uint32_t s_item;
+    for (uint32_t i = threadIdx.x; i < MGX_ERF_NINT * (MGX_ERF_DEG + 1); i += MGX_WAVE)
+        s_erf[i] = MGX_ERF_TAB[i];
+    const uint32_t lane = threadIdx.x;
+    for (;;) {
+        __syncthreads();
+        if (lane == 0) s_item = atomicAdd(counter, 1u);
+        __syncthreads();
+        const uint32_t w = s_item;
+        if (w >= n_items) break;
+        const BatchItem it = items[w];
+        const KParams &kp = kps[it.model];
+        const uint32_t F = kp.m.nfreqs, L = kp.m.nlines;
+        double *s_J = reinterpret_cast<double *>(smem + ((walk_lds_bytes(kp.cap) + 7) / 8) * 8);
+        double *s_L = s_J + F;
+        for (uint32_t i = lane; i < F + L; i += MGX_WAVE) s_J[i] = 0.0;
+        double *Jm = J[it.model];
+        double *Lm = lam ? lam[it.model] : nullptr;
+        if (solve_origin<APPROX, PREC>(kp, it.origin, it.rr0, it.rr1, smem, s_erf, nullptr,
+                                       Lm != nullptr, s_J, s_L))
+            return;
+        __syncthreads();
+        const size_t o = it.origin;
+        if (it.rr0 == 0 && it.rr1 == kp.m.hnrays) {
+            for (uint32_t i = lane; i < F; i += MGX_WAVE) Jm[o * F + i] = s_J[i];
+            if (Lm)
+                for (uint32_t i = lane; i < L; i += MGX_WAVE) Lm[o * L + i] = s_L[i];
+        } else {
+            for (uint32_t i = lane; i < F; i += MGX_WAVE) unsafeAtomicAdd(&Jm[o * F + i], s_J[i]);
+            if (Lm)
+                for (uint32_t i = lane; i < L; i += MGX_WAVE)
+                    unsafeAtomicAdd(&Lm[o * L + i], s_L[i]);
+        }
+    }
+}
+
+/* clears n words (graph-safe: a kernel, not a memset node) */
+__global__ void fill_u32_kernel(uint32_t *p, uint32_t n, uint32_t v)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void fill_f64_kernel(double *p, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
 }
 
 /* ------------------------------------------------------------------------- */
@@ -2884,6 +2977,261 @@ int run_J_adjoint(hipStream_t st, DevBuffers &b, KParams &kp, const double *J_ba
         });
 }
 
+
+/* ------------------------------------------------------------------------- */
+/* batched forward solve: plan, autotuner, graph                              */
+
+struct BatchConfig {
+    uint32_t rays_per_item; /* 0: all ray pairs of an origin in one item */
+    uint32_t order;         /* 0: items by decreasing cost; 1: model by model */
+    uint32_t wps;           /* register budget of the kernel: 2, 3 or 4 waves/SIMD */
+};
+
+} // namespace
+
+struct mgx_batch {
+    uint32_t n = 0;
+    mgx_options op{};
+    std::vector<mgx_model> views;
+    std::vector<DevBuffers *> bufs;
+    std::vector<KParams> kps;
+    KParams *kps_dev = nullptr;
+    BatchItem *items_dev = nullptr;
+    uint32_t n_items = 0, items_cap = 0;
+    uint32_t *counter = nullptr, *status = nullptr;
+    double **J_dev = nullptr, **lam_dev = nullptr;
+    std::vector<double *> J, lam;
+    bool with_lam = false;
+    BatchConfig cfg{0, 0, MGX_SOLVE_WAVES_PER_SIMD};
+    uint32_t cap = 128;
+    uint32_t maxF = 0, maxL = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int use_graph = 1;
+    double last_ms = 0.0;
+    ~mgx_batch()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (auto *b : bufs) delete b;
+        if (kps_dev) (void)hipFree(kps_dev);
+        if (items_dev) (void)hipFree(items_dev);
+        if (counter) (void)hipFree(counter);
+        if (status) (void)hipFree(status);
+        if (J_dev) (void)hipFree(J_dev);
+        if (lam_dev) (void)hipFree(lam_dev);
+    }
+};
+
+namespace {
+
+/* work list of a configuration (host) */
+std::vector<BatchItem> batch_items(const mgx_batch &B, const BatchConfig &c)
+{
+    std::vector<BatchItem> it;
+    std::vector<double> cost;
+    for (uint32_t mi = 0; mi < B.n; mi++) {
+        const mgx_model &m = B.views[mi];
+        const uint32_t R = m.hnrays;
+        const uint32_t step = (c.rays_per_item == 0 || c.rays_per_item > R) ? R : c.rays_per_item;
+        /* walk length grows like the model's linear size */
+        const double len = std::cbrt((double)m.npoints);
+        for (uint32_t o = 0; o < m.npoints; o++)
+            for (uint32_t r0 = 0; r0 < R; r0 += step) {
+                const uint32_t r1 = (r0 + step < R) ? r0 + step : R;
+                it.push_back(BatchItem{mi, o, r0, r1});
+                cost.push_back((double)(r1 - r0) * len * m.nlines);
+            }
+    }
+    if (c.order == 0) {
+        std::vector<uint32_t> idx(it.size());
+        for (uint32_t i = 0; i < idx.size(); i++) idx[i] = i;
+        std::stable_sort(idx.begin(), idx.end(),
+                         [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
+        std::vector<BatchItem> s(it.size());
+        for (uint32_t i = 0; i < idx.size(); i++) s[i] = it[idx[i]];
+        it.swap(s);
+    }
+    return it;
+}
+
+template <int A, int PR>
+void launch_batch_kernel(const mgx_batch &B, uint32_t wps, dim3 grid, size_t lds, hipStream_t st,
+                         uint32_t n_items)
+{
+    double *const *lam = B.with_lam ? B.lam_dev : nullptr;
+    if (wps == 2)
+        hipLaunchKernelGGL((feautrier_batch_kernel<A, PR, 2>), grid, dim3(MGX_WAVE), lds, st,
+                           B.kps_dev, B.items_dev, n_items, B.counter, B.J_dev, lam);
+    else if (wps == 4)
+        hipLaunchKernelGGL((feautrier_batch_kernel<A, PR, 4>), grid, dim3(MGX_WAVE), lds, st,
+                           B.kps_dev, B.items_dev, n_items, B.counter, B.J_dev, lam);
+    else
+        hipLaunchKernelGGL((feautrier_batch_kernel<A, PR, 3>), grid, dim3(MGX_WAVE), lds, st,
+                           B.kps_dev, B.items_dev, n_items, B.counter, B.J_dev, lam);
+}
+
+template <int A, int PR>
+uint32_t batch_grid(uint32_t wps, size_t lds)
+{
+    if (wps == 2) return persistent_waves(feautrier_batch_kernel<A, PR, 2>, lds, 0xffffffffu);
+    if (wps == 4) return persistent_waves(feautrier_batch_kernel<A, PR, 4>, lds, 0xffffffffu);
+    return persistent_waves(feautrier_batch_kernel<A, PR, 3>, lds, 0xffffffffu);
+}
+
+size_t batch_lds(const mgx_batch &B)
+{
+    return walk_lds_bytes(B.cap) + 8 + 8 * ((size_t)B.maxF + B.maxL);
+}
+
+/* persistent grid of the current configuration (resident waves, at most n_items) */
+uint32_t batch_grid_size(const mgx_batch &B, uint32_t n_items)
+{
+    const size_t lds = batch_lds(B);
+    const bool fp64 = B.op.precision == MGX_PREC_FP64;
+    uint32_t grid = 0;
+    switch (B.op.approx) {
+    case MGX_APPROX_NONE:
+        grid = fp64 ? batch_grid<MGX_APPROX_NONE, MGX_PREC_FP64>(B.cfg.wps, lds)
+                    : batch_grid<MGX_APPROX_NONE, MGX_PREC_MIXED>(B.cfg.wps, lds);
+        break;
+    case MGX_APPROX_ONE_LINE:
+        grid = fp64 ? batch_grid<MGX_APPROX_ONE_LINE, MGX_PREC_FP64>(B.cfg.wps, lds)
+                    : batch_grid<MGX_APPROX_ONE_LINE, MGX_PREC_MIXED>(B.cfg.wps, lds);
+        break;
+    default:
+        grid = fp64 ? batch_grid<MGX_APPROX_CLOSE_LINES, MGX_PREC_FP64>(B.cfg.wps, lds)
+                    : batch_grid<MGX_APPROX_CLOSE_LINES, MGX_PREC_MIXED>(B.cfg.wps, lds);
+    }
+    if (grid > n_items) grid = n_items ? n_items : 1;
+    return grid;
+}
+
+/* the launches of one execution: line tables, output clearing (split items),
+   counter and status reset, the persistent kernel over n_items items on `grid`
+   waves (no device queries: the sequence is also captured into a graph) */
+void batch_enqueue(const mgx_batch &B, hipStream_t st, uint32_t n_items, bool clear_outputs,
+                   uint32_t grid)
+{
+    for (uint32_t mi = 0; mi < B.n; mi++) {
+        const mgx_model &m = B.views[mi];
+        const size_t PL = (size_t)m.npoints * m.nlines;
+        const size_t n = PL > m.npoints ? PL : m.npoints;
+        hipLaunchKernelGGL(pack_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, m,
+                           B.bufs[mi]->lpack, B.bufs[mi]->pts, B.bufs[mi]->pidx);
+        if (clear_outputs) {
+            const size_t PF = (size_t)m.npoints * m.nfreqs;
+            hipLaunchKernelGGL(fill_f64_kernel, dim3((unsigned)((PF + 255) / 256)), dim3(256), 0,
+                               st, B.J[mi], PF);
+            if (B.with_lam)
+                hipLaunchKernelGGL(fill_f64_kernel, dim3((unsigned)((PL + 255) / 256)), dim3(256),
+                                   0, st, B.lam[mi], PL);
+        }
+    }
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, st, B.counter, 1u, 0u);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, st, B.status, 4u, 0u);
+    const size_t lds = batch_lds(B);
+    const bool fp64 = B.op.precision == MGX_PREC_FP64;
+    const dim3 g(grid);
+    switch (B.op.approx) {
+    case MGX_APPROX_NONE:
+        fp64 ? launch_batch_kernel<MGX_APPROX_NONE, MGX_PREC_FP64>(B, B.cfg.wps, g, lds, st, n_items)
+             : launch_batch_kernel<MGX_APPROX_NONE, MGX_PREC_MIXED>(B, B.cfg.wps, g, lds, st, n_items);
+        break;
+    case MGX_APPROX_ONE_LINE:
+        fp64 ? launch_batch_kernel<MGX_APPROX_ONE_LINE, MGX_PREC_FP64>(B, B.cfg.wps, g, lds, st, n_items)
+             : launch_batch_kernel<MGX_APPROX_ONE_LINE, MGX_PREC_MIXED>(B, B.cfg.wps, g, lds, st, n_items);
+        break;
+    default:
+        fp64 ? launch_batch_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_FP64>(B, B.cfg.wps, g, lds, st, n_items)
+             : launch_batch_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_MIXED>(B, B.cfg.wps, g, lds, st, n_items);
+    }
+}
+
+/* uploads the work list of the current configuration and the per-model
+   parameters with the current walk capacity */
+int batch_upload(mgx_batch &B, hipStream_t st)
+{
+    const std::vector<BatchItem> it = batch_items(B, B.cfg);
+    if (it.size() > B.items_cap) {
+        if (B.items_dev) (void)hipFree(B.items_dev);
+        B.items_dev = nullptr;
+        HIP_TRY(hipMalloc(&B.items_dev, it.size() * sizeof(BatchItem)));
+        B.items_cap = (uint32_t)it.size();
+    }
+    B.n_items = (uint32_t)it.size();
+    HIP_TRY(hipMemcpyAsync(B.items_dev, it.data(), it.size() * sizeof(BatchItem),
+                           hipMemcpyHostToDevice, st));
+    for (auto &k : B.kps) k.cap = B.cap;
+    HIP_TRY(hipMemcpyAsync(B.kps_dev, B.kps.data(), B.kps.size() * sizeof(KParams),
+                           hipMemcpyHostToDevice, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return MGX_OK;
+}
+
+bool batch_split(const mgx_batch &B)
+{
+    if (B.cfg.rays_per_item == 0) return false;
+    for (const auto &v : B.views)
+        if (B.cfg.rays_per_item < v.hnrays) return true;
+    return false;
+}
+
+void batch_drop_graph(mgx_batch &B)
+{
+    if (B.exec) (void)hipGraphExecDestroy(B.exec);
+    if (B.graph) (void)hipGraphDestroy(B.graph);
+    B.exec = nullptr;
+    B.graph = nullptr;
+}
+
+/* one full execution with the walk-capacity retry; the launches go through a
+   captured graph when use_graph is set */
+int batch_execute(mgx_batch &B, hipStream_t st)
+{
+    for (;;) {
+        const bool clear = batch_split(B);
+        const uint32_t grid = batch_grid_size(B, B.n_items);
+        if (B.use_graph) {
+            if (!B.exec) {
+                /* capture on a private stream (the caller's may be the null stream) */
+                hipStream_t cs;
+                HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+                hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+                if (e == hipSuccess) {
+                    batch_enqueue(B, cs, B.n_items, clear, grid);
+                    e = hipStreamEndCapture(cs, &B.graph);
+                }
+                if (e == hipSuccess) e = hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0);
+                (void)hipStreamDestroy(cs);
+                if (e != hipSuccess) {
+                    batch_drop_graph(B);
+                    return mgx::fail(MGX_E_HIP, "graph capture of the batch: %s", hipGetErrorString(e));
+                }
+            }
+            timed(st, [&] { (void)hipGraphLaunch(B.exec, st); });
+        } else {
+            timed(st, [&] { batch_enqueue(B, st, B.n_items, clear, grid); });
+        }
+        HIP_TRY(hipGetLastError());
+        B.last_ms = g_last_kernel_ms;
+        uint32_t h[4] = {0, 0, 0, 0};
+        HIP_TRY(hipMemcpyAsync(h, B.status, sizeof h, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        if (h[0] & ERR_NO_NEXT)
+            return mgx::fail(MGX_E_NO_NEXT, "a ray from an interior point found no valid next point");
+        if (!(h[0] & ERR_CAP)) return MGX_OK;
+        uint32_t max_cap = kMaxCap;
+        const size_t extra = 8 + 8 * ((size_t)B.maxF + B.maxL);
+        while (max_cap > 16 && walk_lds_bytes(max_cap) + extra > kLdsBudget) max_cap -= 16;
+        if (B.cap >= max_cap)
+            return mgx::fail(MGX_E_RAY_LENGTH, "a ray needs more than %u walk steps", max_cap);
+        B.cap = B.cap * 2 > max_cap ? max_cap : B.cap * 2;
+        batch_drop_graph(B);
+        if (int e = batch_upload(B, st)) return e;
+    }
+}
+
 } // namespace
 
 extern "C" {
@@ -3114,5 +3462,136 @@ int mgx_trace_rays_device(const mgx_model *model, const mgx_options *opts, uint3
     });
     return e;
 }
+
+int mgx_batch_create(const mgx_model *models, uint32_t n, const mgx_options *opts,
+                     double *const *J, double *const *lambda_diag, mgx_batch **out, void *stream)
+{
+    mgx::clear_error();
+    if (!models || !opts || !J || !out || n == 0) return mgx::fail(MGX_E_ARG, "null argument");
+    *out = nullptr;
+    for (uint32_t i = 0; i < n; i++) {
+        if (int e = check_view(&models[i], opts)) return e;
+        if (!J[i]) return mgx::fail(MGX_E_ARG, "J of model %u is null", i);
+        if (lambda_diag && !lambda_diag[i]) return mgx::fail(MGX_E_ARG, "lambda of model %u is null", i);
+    }
+    hipStream_t st = (hipStream_t)stream;
+    mgx_batch *B = new mgx_batch();
+    B->n = n;
+    B->op = *opts;
+    B->with_lam = lambda_diag != nullptr;
+    auto fail = [&](int e) {
+        delete B;
+        return e;
+    };
+    if (hipMalloc(&B->status, 4 * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&B->counter, sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&B->kps_dev, n * sizeof(KParams)) != hipSuccess ||
+        hipMalloc(&B->J_dev, n * sizeof(double *)) != hipSuccess ||
+        (B->with_lam && hipMalloc(&B->lam_dev, n * sizeof(double *)) != hipSuccess))
+        return fail(mgx::fail(MGX_E_HIP, "batch allocation failed"));
+    for (uint32_t i = 0; i < n; i++) {
+        B->views.push_back(models[i]);
+        B->bufs.push_back(new DevBuffers());
+        KParams kp;
+        if (int e = prepare(&models[i], opts, st, *B->bufs.back(), kp)) return fail(e);
+        kp.status = B->status;
+        B->kps.push_back(kp);
+        B->J.push_back(J[i]);
+        B->lam.push_back(lambda_diag ? lambda_diag[i] : nullptr);
+        B->maxF = std::max(B->maxF, models[i].nfreqs);
+        B->maxL = std::max(B->maxL, models[i].nlines);
+    }
+    if (hipMemcpyAsync(B->J_dev, B->J.data(), n * sizeof(double *), hipMemcpyHostToDevice, st) !=
+            hipSuccess ||
+        (B->with_lam && hipMemcpyAsync(B->lam_dev, B->lam.data(), n * sizeof(double *),
+                                       hipMemcpyHostToDevice, st) != hipSuccess))
+        return fail(mgx::fail(MGX_E_HIP, "batch upload failed"));
+    if (int e = batch_upload(*B, st)) return fail(e);
+    *out = B;
+    return MGX_OK;
+}
+
+int mgx_batch_configure(mgx_batch *B, uint32_t rays_per_item, uint32_t order, uint32_t wps,
+                        int use_graph, void *stream)
+{
+    mgx::clear_error();
+    if (!B) return mgx::fail(MGX_E_ARG, "null batch");
+    if (order > 1 || (wps != 2 && wps != 3 && wps != 4))
+        return mgx::fail(MGX_E_ARG, "order must be 0 or 1 and wps 2, 3 or 4");
+    B->cfg = BatchConfig{rays_per_item, order, wps};
+    B->use_graph = use_graph != 0;
+    batch_drop_graph(*B);
+    return batch_upload(*B, (hipStream_t)stream);
+}
+
+int mgx_batch_run(mgx_batch *B, void *stream)
+{
+    mgx::clear_error();
+    if (!B) return mgx::fail(MGX_E_ARG, "null batch");
+    return batch_execute(*B, (hipStream_t)stream);
+}
+
+int mgx_batch_tune(mgx_batch *B, uint32_t sample_stride, uint32_t *best, void *stream)
+{
+    mgx::clear_error();
+    if (!B) return mgx::fail(MGX_E_ARG, "null batch");
+    hipStream_t st = (hipStream_t)stream;
+    if (sample_stride == 0) sample_stride = 1;
+    const BatchConfig cands[] = {{0, 0, 3}, {0, 0, 2}, {0, 0, 4}, {0, 1, 3},
+                                 {4, 0, 3}, {2, 0, 3}, {1, 0, 3}, {4, 0, 2}};
+    const bool saved_timing = g_timing;
+    g_timing = true;
+    double best_ms = 1e300;
+    BatchConfig best_cfg = B->cfg;
+    const int saved_graph = B->use_graph;
+    for (const BatchConfig &c : cands) {
+        B->cfg = c;
+        B->use_graph = 0;
+        batch_drop_graph(*B);
+        /* time a strided sample of the work list (every sample_stride-th item) */
+        std::vector<BatchItem> all = batch_items(*B, c);
+        std::vector<BatchItem> sample;
+        for (size_t i = 0; i < all.size(); i += sample_stride) sample.push_back(all[i]);
+        B->n_items = (uint32_t)sample.size();
+        if (B->n_items > B->items_cap) {
+            if (int e = batch_upload(*B, st)) {
+                g_timing = saved_timing;
+                return e;
+            }
+            B->n_items = (uint32_t)sample.size();
+        }
+        if (hipMemcpyAsync(B->items_dev, sample.data(), sample.size() * sizeof(BatchItem),
+                           hipMemcpyHostToDevice, st) != hipSuccess) {
+            g_timing = saved_timing;
+            return mgx::fail(MGX_E_HIP, "tuning upload failed");
+        }
+        double ms = 0.0;
+        for (int rep = 0; rep < 2; rep++) {
+            if (int e = batch_execute(*B, st)) {
+                g_timing = saved_timing;
+                return e;
+            }
+            ms = B->last_ms; /* the second run: caches warm */
+        }
+        if (ms < best_ms) {
+            best_ms = ms;
+            best_cfg = c;
+        }
+    }
+    g_timing = saved_timing;
+    B->cfg = best_cfg;
+    B->use_graph = saved_graph;
+    batch_drop_graph(*B);
+    if (best) {
+        best[0] = best_cfg.rays_per_item;
+        best[1] = best_cfg.order;
+        best[2] = best_cfg.wps;
+    }
+    return batch_upload(*B, st);
+}
+
+double mgx_batch_last_ms(const mgx_batch *B) { return B ? B->last_ms : 0.0; }
+
+void mgx_batch_destroy(mgx_batch *B) { delete B; }
 
 } // extern "C"

@@ -460,6 +460,23 @@ def model_wind(n_edge=32, half_size=1.0e14, nrays=12, nquads=31, seed=None, nlev
     return m.finalize()
 
 
+def batch_wind_specs(n_models=64, p_min=2.0e3, p_max=1.3e5, nrays_choices=(12, 48, 192),
+                     seed=None):
+    """Sizes of a batch of jittered-grid winds: point counts log-uniform in
+    [p_min, p_max] (grid edge round(P^(1/3))) and ray counts drawn from
+    nrays_choices, seeded.  Returns a list of (n_edge, nrays, seed)."""
+    rng = np.random.Generator(np.random.PCG64(seed_for(2) if seed is None else seed))
+    logp = rng.uniform(np.log(p_min), np.log(p_max), n_models)
+    nr = rng.choice(np.asarray(nrays_choices), n_models)
+    return [(int(round(np.exp(lp) ** (1.0 / 3.0))), int(r), seed_for(2, i + 1))
+            for i, (lp, r) in enumerate(zip(logp, nr))]
+
+
+def batch_winds(specs):
+    """The wind models of batch_wind_specs (same rotor as model_wind)."""
+    return [model_wind(n_edge=e, nrays=r, seed=sd) for e, r, sd in specs]
+
+
 def read_lamda(path, num=1):
     """Radiative part of a LAMDA-format line data file, as set_linedata_from_LAMDA_file
     stores it (setup.py:632-779): energies in J, levels 0-based, derived Bs, Ba, frequency."""

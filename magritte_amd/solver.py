@@ -167,3 +167,59 @@ def image_new(dm, opts, raydir, nx, ny, freqs, view=None):
                                            _abi.ptr(I), _abi.ptr(ImX), _abi.ptr(ImY),
                                            dm.stream_ptr()))
     return I, ImX, ImY
+
+
+class BatchPlan:
+    """Many models solved together in one persistent launch (mgx_batch_*): the
+    models' work items go through a single atomic queue, runs replay a captured
+    HIP graph, and tune() picks the tiling (ray pairs per work item, item order,
+    register budget) by timing a sample of the work.  All DeviceModels must be
+    on the same device and use the same options."""
+
+    def __init__(self, dms, opts, with_lambda=False):
+        torch = _require_gpu()
+        self.dms = list(dms)
+        self.opts = opts
+        n = len(self.dms)
+        dev = self.dms[0].device
+        self.views = (_abi.MgxModel * n)(*[dm.view for dm in self.dms])
+        self.J = [torch.empty((dm.model.npoints, dm.model.nfreqs), dtype=torch.float64, device=dev)
+                  for dm in self.dms]
+        self.lam = ([torch.empty((dm.model.npoints, dm.model.nlines), dtype=torch.float64,
+                                 device=dev) for dm in self.dms] if with_lambda else None)
+        self._Jp = (C.c_void_p * n)(*[t.data_ptr() for t in self.J])
+        self._Lp = (C.c_void_p * n)(*[t.data_ptr() for t in self.lam]) if with_lambda else None
+        h = C.c_void_p()
+        _abi.check(_abi.lib().mgx_batch_create(self.views, n, C.byref(opts), self._Jp, self._Lp,
+                                               C.byref(h), self.dms[0].stream_ptr()))
+        self._h = h
+
+    def configure(self, rays_per_item=0, order=0, wps=3, use_graph=True):
+        _abi.check(_abi.lib().mgx_batch_configure(self._h, rays_per_item, order, wps,
+                                                  1 if use_graph else 0,
+                                                  self.dms[0].stream_ptr()))
+
+    def tune(self, sample_stride=8):
+        best = np.zeros(3, dtype=np.uint32)
+        _abi.check(_abi.lib().mgx_batch_tune(self._h, sample_stride, best.ctypes.data,
+                                             self.dms[0].stream_ptr()))
+        return dict(rays_per_item=int(best[0]), order=int(best[1]), wps=int(best[2]))
+
+    def run(self):
+        """Solves every model; returns the list of J tensors (and Lambda, if kept)."""
+        _abi.check(_abi.lib().mgx_batch_run(self._h, self.dms[0].stream_ptr()))
+        return self.J if self.lam is None else (self.J, self.lam)
+
+    def last_ms(self):
+        return _abi.lib().mgx_batch_last_ms(self._h)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _abi.lib().mgx_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
