@@ -1,0 +1,136 @@
+/*
+ * pybind11 module magritte_amd.core: the hot-path subset of the reference's
+ * Python surface (src/bindings/pybindings.cpp) over magritte::Model:
+ *   Model.compute_radiation_field_feautrier_order_2()          (:185-188)
+ *   Model.compute_image_new(ray_nr) / (ray_nr, Nx, Ny) /
+ *        (rx, ry, rz, Nx, Ny)                                   (:227-240)
+ *   Model.parameters knobs of the path                          (:277-327)
+ *   model.radiation.u (hnrays, P, F), .J (P, F) and model.images[i].I,
+ *   .ImX, .ImY as numpy arrays through the buffer protocol      (:109-118, :735-800)
+ * Errors surface as RuntimeError with the library's message.
+ */
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include "magritte_amd.hpp"
+
+namespace py = pybind11;
+using magritte::Image;
+using magritte::Model;
+using magritte::Parameters;
+using magritte::Radiation;
+using magritte::Real;
+using magritte::Size;
+
+namespace {
+
+py::array view_of(std::vector<double> &v, std::vector<py::ssize_t> shape, py::handle base)
+{
+    return py::array(py::dtype::of<double>(), shape, v.data(), base);
+}
+
+} // namespace
+
+PYBIND11_MODULE(core, m)
+{
+    m.doc() = "Formal-solver path of Magritte on MI355X (gfx950): Model, Parameters, Radiation, Image";
+
+    py::class_<Parameters>(m, "Parameters")
+        .def(py::init<>())
+        .def_readwrite("npoints", &Parameters::npoints)
+        .def_readwrite("nrays", &Parameters::nrays)
+        .def_readwrite("hnrays", &Parameters::hnrays)
+        .def_readwrite("nboundary", &Parameters::nboundary)
+        .def_readwrite("nlines", &Parameters::nlines)
+        .def_readwrite("nlspecs", &Parameters::nlspecs)
+        .def_readwrite("nquads", &Parameters::nquads)
+        .def_readwrite("nfreqs", &Parameters::nfreqs)
+        .def_readwrite("dimension", &Parameters::dimension)
+        .def_readwrite("spherical_symmetry", &Parameters::spherical_symmetry)
+        .def_readwrite("adaptive_ray", &Parameters::adaptive_ray)
+        .def_readwrite("one_line_approximation", &Parameters::one_line_approximation)
+        .def_readwrite("sum_opacity_emissivity_over_all_lines",
+                       &Parameters::sum_opacity_emissivity_over_all_lines)
+        .def_readwrite("n_off_diag", &Parameters::n_off_diag)
+        .def_readwrite("min_opacity", &Parameters::min_opacity)
+        .def_readwrite("max_distance_opacity_contribution",
+                       &Parameters::max_distance_opacity_contribution)
+        .def_readwrite("max_interpolation_diff", &Parameters::max_interpolation_diff)
+        .def_readwrite("mixed_precision", &Parameters::mixed_precision);
+
+    py::class_<Radiation>(m, "Radiation")
+        .def_property_readonly("u",
+                               [](py::object self) {
+                                   Radiation &r = self.cast<Radiation &>();
+                                   return view_of(r.u, {r.hnrays, r.npoints, r.nfreqs}, self);
+                               })
+        .def_property_readonly("J", [](py::object self) {
+            Radiation &r = self.cast<Radiation &>();
+            return view_of(r.J, {r.npoints, r.nfreqs}, self);
+        });
+
+    py::class_<Image>(m, "Image")
+        .def_readonly("nfreqs", &Image::nfreqs)
+        .def_readonly("nx", &Image::nx)
+        .def_readonly("ny", &Image::ny)
+        .def_property_readonly("I",
+                               [](py::object self) {
+                                   Image &im = self.cast<Image &>();
+                                   return view_of(im.I, {(py::ssize_t)im.nx * im.ny, im.nfreqs}, self);
+                               })
+        .def_property_readonly("ImX",
+                               [](py::object self) {
+                                   Image &im = self.cast<Image &>();
+                                   return view_of(im.ImX, {(py::ssize_t)im.nx * im.ny}, self);
+                               })
+        .def_property_readonly("ImY",
+                               [](py::object self) {
+                                   Image &im = self.cast<Image &>();
+                                   return view_of(im.ImY, {(py::ssize_t)im.nx * im.ny}, self);
+                               })
+        .def_property_readonly("freqs",
+                               [](py::object self) {
+                                   Image &im = self.cast<Image &>();
+                                   return view_of(im.freqs, {(py::ssize_t)im.nfreqs}, self);
+                               })
+        .def_property_readonly("image_direction", [](const Image &im) {
+            return py::make_tuple(im.direction[0], im.direction[1], im.direction[2]);
+        });
+
+    py::class_<Model>(m, "Model")
+        .def(py::init<>())
+        .def_readonly("parameters", &Model::parameters, py::return_value_policy::reference_internal)
+        .def_readonly("radiation", &Model::radiation, py::return_value_policy::reference_internal)
+        .def_readonly("images", &Model::images, py::return_value_policy::reference_internal)
+        .def_property_readonly("lambda_diag",
+                               [](py::object self) {
+                                   Model &mo = self.cast<Model &>();
+                                   return view_of(mo.lambda_diag,
+                                                  {mo.parameters.npoints, mo.parameters.nlines},
+                                                  self);
+                               })
+        .def("set_array",
+             [](Model &mo, const std::string &name, py::buffer b) {
+                 py::buffer_info info = b.request();
+                 mo.set_array(name, info.ptr, (size_t)info.size * info.itemsize);
+             })
+        .def("set_center",
+             [](Model &mo, Real x, Real y, Real z) { mo.set_center({x, y, z}); })
+        .def("set_max_inverse_mass", &Model::set_max_inverse_mass)
+        .def("compute_radiation_field_feautrier_order_2",
+             &Model::compute_radiation_field_feautrier_order_2,
+             "Compute the radiation field (u, J, Lambda diagonal) with the second-order "
+             "Feautrier solver.")
+        .def("compute_image_new", (int (Model::*)(Size)) & Model::compute_image_new,
+             "New imager along ray ray_nr, 256 x 256 pixels.")
+        .def("compute_image_new", (int (Model::*)(Size, Size, Size)) & Model::compute_image_new,
+             "New imager along ray ray_nr with Nxpix x Nypix pixels.")
+        .def("compute_image_new",
+             (int (Model::*)(Real, Real, Real, Size, Size)) & Model::compute_image_new,
+             "New imager along (rx, ry, rz) with Nxpix x Nypix pixels.")
+        .def("compute_spectral_discretisation",
+             (int (Model::*)(Real, Real, Size)) & Model::compute_spectral_discretisation,
+             "Uniform image frequency grid nu_min..nu_max with n points.")
+        .def("restore_line_grid", &Model::restore_line_grid);
+}

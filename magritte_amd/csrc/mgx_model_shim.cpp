@@ -1,0 +1,297 @@
+/*
+ * magritte::Model / magritte::Solver (include/magritte_amd.hpp): the reference's
+ * C++ entry points of the formal-solver path on top of the C ABI.
+ *   Model::compute_radiation_field_feautrier_order_2   src/model/model.cpp:353-390
+ *   Model::compute_image_new (all overloads)           src/model/model.cpp:987-1035
+ *   Model::compute_spectral_discretisation(min,max,n)  src/model/model.cpp:257-298
+ */
+#include <hip/hip_runtime_api.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "magritte_amd.hpp"
+
+namespace magritte {
+
+namespace {
+
+/* (field name, element size) of every array of mgx_model */
+struct Field {
+    const char *name;
+    size_t elem;
+};
+const Field kFields[] = {
+    {"position", 8},         {"velocity", 8},          {"n_neighbors", 4},
+    {"cum_n_neighbors", 4},  {"neighbors", 4},         {"point2boundary", 4},
+    {"boundary2point", 4},   {"boundary_condition", 4}, {"boundary_temperature", 8},
+    {"ray_direction", 8},    {"ray_weight", 8},        {"antipod", 4},
+    {"temperature", 8},      {"vturb2", 8},            {"line", 8},
+    {"sorted_line", 8},      {"sorted_line_map", 4},   {"emissivity", 8},
+    {"opacity", 8},          {"inverse_width", 8},     {"interp_crit", 8},
+    {"dshift_max", 8},       {"line_species", 4},      {"line_k", 4},
+    {"line_A", 8},           {"species_inverse_mass", 8}, {"quad_weights", 8},
+    {"nu", 8},               {"f_line", 4},            {"f_species", 4},
+    {"f_k", 4},              {"f_z", 4},               {"f_in_line", 1},
+};
+
+const char *kFreqFields[] = {"nu", "f_line", "f_species", "f_k", "f_z", "f_in_line"};
+
+void check(int code)
+{
+    if (code != MGX_OK) throw std::runtime_error(mgx_last_error());
+}
+
+void hip_check(hipError_t e, const char *what)
+{
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename T>
+const T *field_ptr(const std::map<std::string, void *> &dev, const char *name)
+{
+    auto it = dev.find(name);
+    return it == dev.end() ? nullptr : static_cast<const T *>(it->second);
+}
+
+/* a device buffer the solver writes, freed at scope end */
+struct DevOut {
+    double *p = nullptr;
+    explicit DevOut(size_t n)
+    {
+        if (n) hip_check(hipMalloc(&p, n * sizeof(double)), "hipMalloc");
+    }
+    ~DevOut()
+    {
+        if (p) (void)hipFree(p);
+    }
+    void copy_to(std::vector<double> &v, size_t n) const
+    {
+        v.resize(n);
+        if (n) hip_check(hipMemcpy(v.data(), p, n * sizeof(double), hipMemcpyDeviceToHost), "copy");
+    }
+};
+
+} // namespace
+
+Model::~Model()
+{
+    for (auto &kv : dev)
+        if (kv.second) (void)hipFree(kv.second);
+}
+
+void Model::set_array(const std::string &name, const void *data, size_t bytes)
+{
+    bool known = false;
+    for (const Field &f : kFields) known = known || name == f.name;
+    if (!known) throw std::runtime_error("unknown model array '" + name + "'");
+    auto &v = host[name];
+    v.assign(static_cast<const unsigned char *>(data), static_cast<const unsigned char *>(data) + bytes);
+    stale[name] = true;
+}
+
+const std::vector<unsigned char> &Model::get_array(const std::string &name) const
+{
+    auto it = host.find(name);
+    if (it == host.end()) throw std::runtime_error("model array '" + name + "' is not set");
+    return it->second;
+}
+
+const mgx_model &Model::device_view()
+{
+    for (auto &kv : host) {
+        if (!stale[kv.first]) continue;
+        void *&d = dev[kv.first];
+        if (d) (void)hipFree(d);
+        d = nullptr;
+        if (!kv.second.empty()) {
+            hip_check(hipMalloc(&d, kv.second.size()), "hipMalloc");
+            hip_check(hipMemcpy(d, kv.second.data(), kv.second.size(), hipMemcpyHostToDevice),
+                      "hipMemcpy");
+        }
+        stale[kv.first] = false;
+    }
+    const Parameters &p = parameters;
+    mgx_model &v = view;
+    std::memset(&v, 0, sizeof v);
+    v.npoints = p.npoints;
+    v.nrays = p.nrays;
+    v.hnrays = p.hnrays;
+    v.nboundary = p.nboundary;
+    v.nlines = p.nlines;
+    v.nlspecs = p.nlspecs;
+    v.nquads = p.nquads;
+    v.nfreqs = p.nfreqs;
+    v.dimension = (int32_t)p.dimension;
+    v.spherical_symmetry = p.spherical_symmetry;
+    v.adaptive_directions = p.adaptive_ray;
+    v.position = field_ptr<double>(dev, "position");
+    v.velocity = field_ptr<double>(dev, "velocity");
+    v.n_neighbors = field_ptr<uint32_t>(dev, "n_neighbors");
+    v.cum_n_neighbors = field_ptr<uint32_t>(dev, "cum_n_neighbors");
+    v.neighbors = field_ptr<uint32_t>(dev, "neighbors");
+    v.point2boundary = field_ptr<uint32_t>(dev, "point2boundary");
+    v.boundary2point = field_ptr<uint32_t>(dev, "boundary2point");
+    v.boundary_condition = field_ptr<int32_t>(dev, "boundary_condition");
+    v.boundary_temperature = field_ptr<double>(dev, "boundary_temperature");
+    v.ray_direction = field_ptr<double>(dev, "ray_direction");
+    v.ray_weight = field_ptr<double>(dev, "ray_weight");
+    v.antipod = field_ptr<uint32_t>(dev, "antipod");
+    for (int i = 0; i < 3; i++) v.center[i] = center[i];
+    v.temperature = field_ptr<double>(dev, "temperature");
+    v.vturb2 = field_ptr<double>(dev, "vturb2");
+    v.line = field_ptr<double>(dev, "line");
+    v.sorted_line = field_ptr<double>(dev, "sorted_line");
+    v.sorted_line_map = field_ptr<uint32_t>(dev, "sorted_line_map");
+    v.emissivity = field_ptr<double>(dev, "emissivity");
+    v.opacity = field_ptr<double>(dev, "opacity");
+    v.inverse_width = field_ptr<double>(dev, "inverse_width");
+    v.interp_crit = field_ptr<double>(dev, "interp_crit");
+    v.dshift_max = field_ptr<double>(dev, "dshift_max");
+    v.max_inverse_mass = max_inverse_mass;
+    v.line_species = field_ptr<uint32_t>(dev, "line_species");
+    v.line_k = field_ptr<uint32_t>(dev, "line_k");
+    v.line_A = field_ptr<double>(dev, "line_A");
+    v.species_inverse_mass = field_ptr<double>(dev, "species_inverse_mass");
+    v.quad_weights = field_ptr<double>(dev, "quad_weights");
+    v.nu = field_ptr<double>(dev, "nu");
+    v.f_line = field_ptr<uint32_t>(dev, "f_line");
+    v.f_species = field_ptr<uint32_t>(dev, "f_species");
+    v.f_k = field_ptr<uint32_t>(dev, "f_k");
+    v.f_z = field_ptr<uint32_t>(dev, "f_z");
+    v.f_in_line = field_ptr<uint8_t>(dev, "f_in_line");
+    return view;
+}
+
+mgx_options Model::options() const
+{
+    mgx_options o;
+    std::memset(&o, 0, sizeof o);
+    /* the approximation dispatch of model.cpp:366-389 */
+    o.approx = parameters.one_line_approximation ? MGX_APPROX_ONE_LINE
+               : parameters.sum_opacity_emissivity_over_all_lines ? MGX_APPROX_NONE
+                                                                  : MGX_APPROX_CLOSE_LINES;
+    o.n_off_diag = parameters.n_off_diag;
+    o.precision = parameters.mixed_precision ? MGX_PREC_MIXED : MGX_PREC_FP64;
+    o.min_opacity = parameters.min_opacity;
+    o.max_distance_opacity_contribution = parameters.max_distance_opacity_contribution;
+    o.max_interpolation_diff = parameters.max_interpolation_diff;
+    return o;
+}
+
+int Model::compute_radiation_field_feautrier_order_2()
+{
+    Solver solver;
+    solver.setup(*this);
+    solver.solve_feautrier_order_2(*this);
+    return 0;
+}
+
+int Model::compute_image_new(Size ray_nr) { return compute_image_new(ray_nr, 256, 256); }
+
+int Model::compute_image_new(Size ray_nr, Size Nxpix, Size Nypix)
+{
+    /* geometry.rays.get_direction<false>(0, ray_nr) */
+    const auto &d = get_array("ray_direction");
+    if ((size_t)(ray_nr + 1) * 3 * sizeof(double) > d.size())
+        throw std::runtime_error("ray_nr out of range");
+    Vector3D r;
+    std::memcpy(r.data(), d.data() + (size_t)ray_nr * 3 * sizeof(double), sizeof r);
+    return compute_image_new(r, Nxpix, Nypix);
+}
+
+int Model::compute_image_new(Real rx, Real ry, Real rz, Size Nxpix, Size Nypix)
+{
+    return compute_image_new(Vector3D{rx, ry, rz}, Nxpix, Nypix);
+}
+
+int Model::compute_image_new(const Vector3D raydir, Size Nxpix, Size Nypix)
+{
+    /* model.cpp:1009-1024: the checks come before any work */
+    if (raydir[0] * raydir[0] + raydir[1] * raydir[1] + raydir[2] * raydir[2] == 0.0)
+        throw std::runtime_error(
+            "The given ray direction vector does not point in a direction. Please use a "
+            "non-zero (normed) direction vector to generate an image.");
+    if (parameters.one_line_approximation)
+        throw std::runtime_error("One line approximation is not supported for imaging.");
+    Solver solver;
+    solver.image_feautrier_order_2_new_imager(*this, raydir, Nxpix, Nypix);
+    return 0;
+}
+
+int Model::compute_spectral_discretisation(Real nu_min, Real nu_max, Size n)
+{
+    const Size P = parameters.npoints;
+    if (saved_line_grid.empty())
+        for (const char *f : kFreqFields) saved_line_grid[f] = host[f];
+    std::vector<double> nu((size_t)P * n);
+    std::vector<uint32_t> fl(n), fs(n), fk(n), fz(n);
+    std::vector<uint8_t> fin(n);
+    check(mgx_compute_image_discretisation(P, nu_min, nu_max, n, nu.data(), fl.data(), fs.data(),
+                                           fk.data(), fz.data(), fin.data()));
+    set_array("nu", nu.data(), nu.size() * sizeof(double));
+    set_array("f_line", fl.data(), n * 4);
+    set_array("f_species", fs.data(), n * 4);
+    set_array("f_k", fk.data(), n * 4);
+    set_array("f_z", fz.data(), n * 4);
+    set_array("f_in_line", fin.data(), n);
+    parameters.nfreqs = n;
+    return 0;
+}
+
+void Model::restore_line_grid()
+{
+    if (saved_line_grid.empty()) return;
+    for (auto &kv : saved_line_grid) set_array(kv.first, kv.second.data(), kv.second.size());
+    parameters.nfreqs = (Size)(get_array("f_line").size() / 4);
+    saved_line_grid.clear();
+}
+
+void Solver::setup(Model &model)
+{
+    /* the traced-length pass of the reference's setup happens on the device
+       (walk capacities grow on demand); only the device mirror is refreshed */
+    (void)model.device_view();
+}
+
+void Solver::solve_feautrier_order_2(Model &model)
+{
+    const mgx_model &v = model.device_view();
+    const mgx_options o = model.options();
+    const size_t PF = (size_t)v.npoints * v.nfreqs;
+    const size_t U = (size_t)v.hnrays * PF;
+    DevOut u(U), J(PF), lam((size_t)v.npoints * v.nlines);
+    check(mgx_feautrier_forward_device(&v, &o, u.p, J.p, lam.p, nullptr));
+    u.copy_to(model.radiation.u, U);
+    J.copy_to(model.radiation.J, PF);
+    model.radiation.hnrays = v.hnrays;
+    model.radiation.npoints = v.npoints;
+    model.radiation.nfreqs = v.nfreqs;
+    lam.copy_to(model.lambda_diag, (size_t)v.npoints * v.nlines);
+}
+
+void Solver::image_feautrier_order_2_new_imager(Model &model, const Vector3D raydir, Size Nxpix,
+                                                Size Nypix)
+{
+    const mgx_model &v = model.device_view();
+    const mgx_options o = model.options();
+    const size_t npix = (size_t)Nxpix * Nypix;
+    DevOut I(npix * v.nfreqs), X(npix), Y(npix);
+    check(mgx_image_device(&v, &o, raydir.data(), Nxpix, Nypix, I.p, X.p, Y.p, nullptr));
+    Image img;
+    img.nx = Nxpix;
+    img.ny = Nypix;
+    img.nfreqs = v.nfreqs;
+    const Real n = std::sqrt(raydir[0] * raydir[0] + raydir[1] * raydir[1] + raydir[2] * raydir[2]);
+    for (int i = 0; i < 3; i++) img.direction[i] = raydir[i] / n;
+    I.copy_to(img.I, npix * v.nfreqs);
+    X.copy_to(img.ImX, npix);
+    Y.copy_to(img.ImY, npix);
+    /* image frequencies: the grid at the first point (uniform over points) */
+    const auto &nu = model.get_array("nu");
+    img.freqs.resize(v.nfreqs);
+    std::memcpy(img.freqs.data(), nu.data(), v.nfreqs * sizeof(double));
+    model.images.push_back(std::move(img));
+}
+
+} // namespace magritte

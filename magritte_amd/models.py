@@ -382,6 +382,27 @@ class ModelData:
         m.max_inverse_mass = self.max_inverse_mass
         return m
 
+    def to_core(self):
+        """magritte_amd.core.Model (the C++ Model with the reference's method names)
+        holding this model's arrays."""
+        from . import core
+
+        cm = core.Model()
+        p = cm.parameters
+        for f in ("npoints", "nrays", "hnrays", "nboundary", "nlines", "nlspecs", "nquads",
+                  "nfreqs", "dimension"):
+            setattr(p, f, int(getattr(self, f)))
+        p.spherical_symmetry = bool(self.spherical_symmetry)
+        p.adaptive_ray = bool(self.adaptive_directions)
+        p.min_opacity = self.params["min_opacity"]
+        p.max_distance_opacity_contribution = self.params["max_distance_opacity_contribution"]
+        p.max_interpolation_diff = self.params["max_interpolation_diff"]
+        for k in self.ARRAYS:
+            cm.set_array(k, np.ascontiguousarray(getattr(self, k)))
+        cm.set_center(*[float(x) for x in self.center])
+        cm.set_max_inverse_mass(float(self.max_inverse_mass))
+        return cm
+
     def to_device(self, device="cuda"):
         """Copy every solver input to the GPU (torch tensors); returns (view, tensors)."""
         import torch
