@@ -35,9 +35,10 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--workload", default="wind32", choices=["wind32", "batch"],
+    ap.add_argument("--workload", default="wind32", choices=["wind32", "batch", "image-gradient"],
                     help="wind32: the single-model solve (default); batch: many winds in one "
-                         "persistent launch against per-model launches")
+                         "persistent launch against per-model launches; image-gradient: the "
+                         "gradient of an image loss through the adjoint")
     ap.add_argument("--n-models", type=int, default=64)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 512 MiB three times (scale of rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -199,10 +200,113 @@ def run_batch(args):
     print(json.dumps(line), flush=True)
 
 
+def run_image_gradient(args):
+    """Gradient of L = |I - I*|^2 for a 256^2 x 31 image of the 32^3 wind with
+    respect to the line emissivity and opacity (P, L), through the adjoint
+    kernel behind torch.autograd; the target I* is the image of the same wind
+    with density and temperature perturbed by seeded 10% log-normal factors.
+    Reports forward and adjoint time, central differences of the GPU forward
+    (step 1e-6 relative) at 64 random (p, l), and mixed against FP64 gradients."""
+    import numpy as np
+    import torch
+
+    from magritte_amd import _abi, autograd, models, solver
+
+    rng = np.random.Generator(np.random.PCG64(models.seed_for(3)))
+    m = models.model_wind(n_edge=args.n_edge)
+    P = m.npoints
+    pert_n = np.exp(0.1 * rng.standard_normal(P))
+    pert_t = np.exp(0.1 * rng.standard_normal(P))
+    mt = models.model_wind(n_edge=args.n_edge, clumps=pert_n, temp_scale=pert_t)
+    nu0 = m.line[0]
+    fr = m.image_frequencies(nu0 * (1 - 1e-4), nu0 * (1 + 1e-4), 31)
+    rd = np.array([0.3, -0.5, 0.81])
+    nx = ny = 256
+    dm, dmt = solver.DeviceModel(m, "cuda:0"), solver.DeviceModel(mt, "cuda:0")
+    lib = _abi.lib()
+    lib.mgx_set_kernel_timing(1)
+    Istar, _, _ = solver.image_new(dmt, mt.options(), rd, nx, ny, fr)
+    res = {}
+    grads = {}
+    for prec, name in ((_abi.PREC_FP64, "fp64"), (_abi.PREC_MIXED, "mixed")):
+        op = m.options(precision=prec)
+        eps = dm.tensors["emissivity"].clone().requires_grad_(True)
+        kap = dm.tensors["opacity"].clone().requires_grad_(True)
+        for rep in range(2):  # second pass timed (warm)
+            eps.grad = None
+            kap.grad = None
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            I = autograd.image(eps, kap, dm, op, rd, nx, ny, fr)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            fwd_k = lib.mgx_last_kernel_ms()
+            loss = ((I - Istar) ** 2).sum()
+            loss.backward()
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            adj_k = lib.mgx_last_kernel_ms()
+        grads[name] = (eps.grad.cpu().numpy(), kap.grad.cpu().numpy())
+        res[name] = {"forward_ms": round((t1 - t0) * 1e3, 3), "forward_kernel_ms": round(fwd_k, 3),
+                     "backward_ms": round((t2 - t1) * 1e3, 3), "adjoint_kernel_ms": round(adj_k, 3),
+                     "cost_vs_forward": round(adj_k / fwd_k, 3)}
+    # central differences of the FP64 forward at 64 random (p, l) of significant gradient,
+    # with interpolated entries off (max_interpolation_diff huge) so that the forward is
+    # a smooth function of the parameters: the float powf of the log interpolation puts
+    # rounding steps of ~1e-7 relative into difference quotients of step 1e-6
+    op = m.options()
+    op.max_interpolation_diff = 1.0e30
+    eps = dm.tensors["emissivity"].clone().requires_grad_(True)
+    kap = dm.tensors["opacity"].clone().requires_grad_(True)
+    ((autograd.image(eps, kap, dm, op, rd, nx, ny, fr) - Istar) ** 2).sum().backward()
+    ge, gk = eps.grad.cpu().numpy(), kap.grad.cpu().numpy()
+    I0 = Istar  # reference only for the loss difference formula
+    h = 1.0e-6
+    errs = []
+    base_e = dm.tensors["emissivity"]
+    base_k = dm.tensors["opacity"]
+    for which, g in (("eps", ge), ("kap", gk)):
+        cand = np.flatnonzero(np.abs(g) > 1e-3 * np.abs(g).max())
+        for i in rng.choice(cand, 32, replace=False):
+            p, l = divmod(int(i), m.nlines)
+            Is = []
+            for sgn in (1.0, -1.0):
+                e2, k2 = base_e.clone(), base_k.clone()
+                t = e2 if which == "eps" else k2
+                t[p, l] = t[p, l] * (1.0 + sgn * h)
+                Is.append(autograd.image(e2, k2, dm, op, rd, nx, ny, fr))
+            x0 = (base_e if which == "eps" else base_k)[p, l].item()
+            dL = ((Is[0] - Is[1]) * (Is[0] + Is[1] - 2.0 * I0)).sum().item()
+            fd = dL / (2 * h * x0)
+            errs.append(abs(g[p, l] - fd) / abs(fd))
+    errs = np.array(errs)
+    mix_err = {}
+    for k, (a, b) in zip(("eps", "kap"), zip(grads["mixed"], grads["fp64"])):
+        sig = np.abs(b) > 1e-3 * np.abs(b).max()
+        mix_err[k] = float(np.max(np.abs(a[sig] - b[sig]) / np.abs(b[sig])))
+    U_img = nx * ny * fr["nu"].shape[1]
+    line = {"metric": "image gradient (adjoint) ray.point.freq/s", "unit": "pixel*freq/s",
+            "value": round(U_img / (res["mixed"]["adjoint_kernel_ms"] * 1e-3), 1),
+            "n_gpus": 1, "higher_is_better": True, "data": "synthetic",
+            "config": {"workload": f"{args.n_edge}^3 wind, {nx}x{ny} image at 31 frequencies, "
+                                   f"L2 loss against 10% log-normal n/T perturbation",
+                       "variants": res,
+                       "fd_check": {"points": int(len(errs)), "step": h,
+                                    "interpolation": "off",
+                                    "max_rel_err": float(errs.max()),
+                                    "median_rel_err": float(np.median(errs)),
+                                    "frac_below_1e-4": float(np.mean(errs < 1e-4))},
+                       "mixed_vs_fp64_grad_max_rel_err": mix_err}}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.workload == "image-gradient":
+        run_image_gradient(args)
         return
     if args.workload == "batch":
         if args.gpus != 1:

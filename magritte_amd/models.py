@@ -457,7 +457,8 @@ def delaunay_neighbors(pos):
 
 
 def model_wind(n_edge=32, half_size=1.0e14, nrays=12, nquads=31, seed=None, nlev=10,
-               B=57.6e9, dipole=0.11, v_out=15.0e3, v_rot=0.0, vturb=150.0, clumps=None):
+               B=57.6e9, dipole=0.11, v_out=15.0e3, v_rot=0.0, vturb=150.0, clumps=None,
+               temp_scale=None):
     """Jittered-grid spherical wind: n = 1e14 (r/1e12 m)^-2, T = 2000 (r/1e12 m)^-0.7,
     radial outflow, linear rotor; boundary = the outer grid layer (CMB)."""
     rng = np.random.Generator(np.random.PCG64(seed_for(1) if seed is None else seed))
@@ -472,6 +473,8 @@ def model_wind(n_edge=32, half_size=1.0e14, nrays=12, nquads=31, seed=None, nlev
     if clumps is not None:
         dens = dens * clumps
     temp = 2000.0 * (r / 1.0e12) ** -0.7
+    if temp_scale is not None:
+        temp = temp * temp_scale
     m = ModelData()
     m.set_geometry(pos, vel, delaunay_neighbors(pos), bdy, uniform_rays(nrays),
                    np.full(nrays, 1.0 / nrays))
