@@ -312,6 +312,157 @@ int mgx_compute_interp_crit(uint32_t npoints, uint32_t nlspecs, const mgx_lineda
 int mgx_sort_lines(uint32_t nlines, const double *line, double *sorted_line,
                    uint32_t *sorted_line_map);
 
+/* ------------------------------------------------------------------------ */
+/* Level populations (non-LTE), the sparse variant: line-integrated J and    */
+/* the Lambda diagonal of a formal solve -> J_eff -> per-point statistical   */
+/* equilibrium -> negative-population correction -> convergence statistics  */
+/* -> line emissivity / opacity / interpolation criterion, plus general-     */
+/* order Ng acceleration.  Populations live on the device in FP64 (the       */
+/* reference keeps them in long double).                                     */
+/* ------------------------------------------------------------------------ */
+
+/* One collision partner of a line-producing species
+ * (src/model/lines/lineProducingSpecies/linedata/collisionPartner/collisionPartner.hpp:7-33). */
+typedef struct mgx_colldata {
+    uint32_t num_col_partner; /* abundance column of the partner */
+    int32_t orth_or_para_H2;  /* 'n', 'o' or 'p' */
+    uint32_t ntmp, ncol;
+    const uint32_t *icol;     /* (ncol) upper level */
+    const uint32_t *jcol;     /* (ncol) lower level */
+    const double *tmp;        /* (ntmp) ascending [K] */
+    const double *Cd;         /* (ntmp, ncol) de-excitation coefficients [m^3/s] */
+    const double *Ce;         /* (ntmp, ncol) excitation coefficients [m^3/s] */
+} mgx_colldata;
+
+/* Knobs of Parameters read by the level-population loop
+ * (src/model/parameters/parameters.hpp:76-104), reference defaults in brackets. */
+typedef struct mgx_nlte_params {
+    double pop_prec;                      /* [1e-6] convergence threshold */
+    double min_rel_pop_for_convergence;   /* [1e-10] */
+    double population_inversion_fraction; /* [-1] (inversion -> LTE test off) */
+    double min_line_emissivity;           /* [1e-50] */
+    double min_line_opacity;              /* [1e-13] */
+    double convergence_fraction;          /* [0.995] */
+    uint32_t use_adaptive_Ng_acceleration;              /* [1] */
+    uint32_t Ng_acceleration_mem_limit;                 /* [9] */
+    uint32_t adaptive_Ng_acceleration_min_order;        /* [2] */
+    uint32_t adaptive_Ng_acceleration_use_max_criterion;/* [1] */
+    uint32_t Ng_acceleration_remove_N_its;              /* [0] */
+    uint32_t reserved0;
+} mgx_nlte_params;
+
+/* Fills p with the reference's defaults. */
+void mgx_nlte_default_params(mgx_nlte_params *p);
+
+/* Convergence statistics of one species (LineProducingSpecies::fraction_not_converged,
+ * relative_change_mean, relative_change_max; lineProducingSpecies.tpp:127-163). */
+typedef struct mgx_nlte_conv {
+    double fraction_not_converged, relative_change_mean, relative_change_max;
+} mgx_nlte_conv;
+
+typedef struct mgx_nlte mgx_nlte;
+
+/*
+ * Level-population state of every line-producing species of a model:
+ * packs the line and collision data (host pointers ld[nlspecs],
+ * ncolpar[nlspecs], colpar[sum ncolpar] in species order) into device
+ * tables and allocates the populations and the Ng history
+ * (Ng_acceleration_mem_limit + Ng_acceleration_remove_N_its + 2 vectors of
+ * P * sum(nlev) doubles).  model_dev is the device view the formal solve
+ * uses (its temperature, quad_weights and sizes are read; the handle keeps
+ * the pointers); abundance_dev is (P, nspec) on the device.  Populations
+ * start at zero: call mgx_nlte_set_lte or mgx_nlte_set_populations.
+ * Levels per species: at most 64.
+ */
+int mgx_nlte_create(const mgx_model *model_dev, uint32_t nlspecs, const mgx_linedata *ld,
+                    const uint32_t *ncolpar, const mgx_colldata *colpar, uint32_t nspec,
+                    const double *abundance_dev, const mgx_nlte_params *params,
+                    mgx_nlte **out, void *stream);
+void mgx_nlte_destroy(mgx_nlte *h);
+
+/*
+ * LineProducingSpecies::update_using_LTE for every species
+ * (lineProducingSpecies.tpp:52-75) and Lines::set_emissivity_and_opacity
+ * (lines.tpp:29-41) plus the interpolation criterion opacity / population_tot
+ * (solver_interp_helper.hpp:15-30) into emissivity, opacity, interp_crit
+ * (P, L, device; the arrays of the model view).  The Ng history restarts
+ * with the LTE populations.
+ */
+int mgx_nlte_set_lte(mgx_nlte *h, double *emissivity, double *opacity, double *interp_crit,
+                     void *stream);
+
+/* Replaces the populations (device, per species a (P, nlev_s) block, blocks
+ * concatenated) and restarts the Ng history with them; emissivity, opacity
+ * and interp_crit are recomputed as in mgx_nlte_set_lte. */
+int mgx_nlte_set_populations(mgx_nlte *h, const double *population_dev, double *emissivity,
+                             double *opacity, double *interp_crit, void *stream);
+
+/* Copies the current populations (layout of mgx_nlte_set_populations) to a
+ * device array; population_tot (nlspecs, P) may be NULL. */
+int mgx_nlte_get_populations(const mgx_nlte *h, double *population_dev,
+                             double *population_tot_dev, void *stream);
+
+/*
+ * One statistical-equilibrium iteration of the sparse loop from a formal
+ * solve's J (P, F) and lambda_diag (P, L) (device, as written by
+ * mgx_feautrier_forward_device):
+ *   Model::compute_Jeff_sparse                      model.cpp:549-570
+ *     (J_lin(p, k) = sum_z w_z J(p, l*nquads + z), the per-line J of
+ *      Solver::solve_feautrier_order_2_sparse, solver.tpp:298-363)
+ *   Lines::iteration_using_statistical_equilibrium_sparse  lines.cpp:139-150
+ *     update_using_statistical_equilibrium_sparse   lineProducingSpecies.tpp:589-683
+ *     correct_negative_populations                  :711-802
+ *     check_for_convergence                         :127-163
+ *     set_emissivity_and_opacity                    lines.tpp:29-41
+ * Each point's nlev x nlev system is solved in FP64 by Gaussian elimination
+ * with partial pivoting and two steps of iterative refinement with
+ * double-double residuals (the reference: long-double column-pivoted QR).
+ * conv (host, nlspecs) receives the statistics; the call synchronises the
+ * stream.  J_lin, J_eff (P, L, device) may be NULL; when given they receive
+ * the values used.
+ */
+int mgx_nlte_iteration_stateq(mgx_nlte *h, const double *J, const double *lambda_diag,
+                              double *emissivity, double *opacity, double *interp_crit,
+                              mgx_nlte_conv *conv, double *J_lin, double *J_eff, void *stream);
+
+/*
+ * Lines::iteration_using_Ng_acceleration(abundance, T, pop_prec, order)
+ * (lines.cpp:106-117): LineProducingSpecies::update_using_acceleration(order)
+ * (lineProducingSpecies.tpp:266-314; the order x order residual Gram matrix
+ * on the device, its long-double column-pivoted QR solve on the host), then
+ * the negative-population correction, the convergence check and the line
+ * emissivity / opacity.  MGX_E_ARG when fewer than order residuals are held.
+ */
+int mgx_nlte_iteration_ng(mgx_nlte *h, uint32_t order, double *emissivity, double *opacity,
+                          double *interp_crit, mgx_nlte_conv *conv, void *stream);
+
+/* Lines::trial_iteration_using_adaptive_Ng_acceleration (lines.cpp:119-124):
+ * the extrapolation into the trial populations and check_for_convergence_trial
+ * (lineProducingSpecies.tpp:167-201); the populations are unchanged. */
+int mgx_nlte_trial_ng(mgx_nlte *h, uint32_t order, mgx_nlte_conv *conv, void *stream);
+
+/* Number of residuals the Ng history holds (LineProducingSpecies::residuals.size()). */
+int mgx_nlte_history(const mgx_nlte *h, uint32_t *n_residuals);
+
+/*
+ * Model::compute_level_populations_sparse(use_Ng_acceleration, max_niterations)
+ * (model.cpp:756-850) with the Default / Adaptive Ng criteria
+ * (model.cpp:580-645): per iteration a formal solve
+ * (mgx_feautrier_forward_device with u = NULL into the handle's J and Lambda
+ * buffers) and mgx_nlte_iteration_stateq, or an Ng step.  emissivity, opacity
+ * and interp_crit must be the arrays model_dev points to.  iterations
+ * receives the number of statistical-equilibrium iterations; error_mean and
+ * error_max (host, capacity max_niterations * nlspecs, may be NULL) the
+ * per-iteration relative_change_mean / _max in the reference's order
+ * (Model::error_mean / error_max).
+ */
+int mgx_compute_level_populations_sparse(const mgx_model *model_dev, const mgx_options *opts,
+                                         mgx_nlte *h, int use_Ng_acceleration,
+                                         int64_t max_niterations, double *emissivity,
+                                         double *opacity, double *interp_crit,
+                                         int32_t *iterations, double *error_mean,
+                                         double *error_max, void *stream);
+
 /*
  * Workload statistics of the trace, for the algorithmic-byte model of the
  * solve: stats[0] = walk steps S (both halves, all ray pairs and origins),

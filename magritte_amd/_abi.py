@@ -73,6 +73,52 @@ class MgxLinedata(C.Structure):
 
 # (name, restype, argtypes) of every function include/mgx.h declares
 _VP = C.c_void_p
+class MgxColldata(C.Structure):
+    _fields_ = [
+        ("num_col_partner", C.c_uint32), ("orth_or_para_H2", C.c_int32),
+        ("ntmp", C.c_uint32), ("ncol", C.c_uint32),
+        ("icol", _pu), ("jcol", _pu), ("tmp", _pd), ("Cd", _pd), ("Ce", _pd),
+    ]
+
+
+class MgxNlteParams(C.Structure):
+    _fields_ = [
+        ("pop_prec", C.c_double), ("min_rel_pop_for_convergence", C.c_double),
+        ("population_inversion_fraction", C.c_double), ("min_line_emissivity", C.c_double),
+        ("min_line_opacity", C.c_double), ("convergence_fraction", C.c_double),
+        ("use_adaptive_Ng_acceleration", C.c_uint32), ("Ng_acceleration_mem_limit", C.c_uint32),
+        ("adaptive_Ng_acceleration_min_order", C.c_uint32),
+        ("adaptive_Ng_acceleration_use_max_criterion", C.c_uint32),
+        ("Ng_acceleration_remove_N_its", C.c_uint32), ("reserved0", C.c_uint32),
+    ]
+
+
+class MgxNlteConv(C.Structure):
+    _fields_ = [("fraction_not_converged", C.c_double), ("relative_change_mean", C.c_double),
+                ("relative_change_max", C.c_double)]
+
+
+def default_nlte_params(**kw):
+    """mgx_nlte_params with the reference's defaults (parameters.hpp:76-104), overridden by kw."""
+    p = MgxNlteParams()
+    p.pop_prec = 1.0e-6
+    p.min_rel_pop_for_convergence = 1.0e-10
+    p.population_inversion_fraction = -1.0
+    p.min_line_emissivity = 1.0e-50
+    p.min_line_opacity = 1.0e-13
+    p.convergence_fraction = 0.995
+    p.use_adaptive_Ng_acceleration = 1
+    p.Ng_acceleration_mem_limit = 9
+    p.adaptive_Ng_acceleration_min_order = 2
+    p.adaptive_Ng_acceleration_use_max_criterion = 1
+    p.Ng_acceleration_remove_N_its = 0
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise AttributeError(k)
+        setattr(p, k, v)
+    return p
+
+
 SIGNATURES = [
     ("mgx_last_error", C.c_char_p, []),
     ("mgx_version", C.c_char_p, []),
@@ -115,6 +161,23 @@ SIGNATURES = [
     ("mgx_compute_interp_crit", C.c_int,
      [C.c_uint32, C.c_uint32, C.POINTER(MgxLinedata), _VP, _VP, _VP]),
     ("mgx_sort_lines", C.c_int, [C.c_uint32, _VP, _VP, _VP]),
+    ("mgx_nlte_default_params", None, [C.POINTER(MgxNlteParams)]),
+    ("mgx_nlte_create", C.c_int,
+     [C.POINTER(MgxModel), C.c_uint32, C.POINTER(MgxLinedata), _VP, C.POINTER(MgxColldata),
+      C.c_uint32, _VP, C.POINTER(MgxNlteParams), C.POINTER(_VP), _VP]),
+    ("mgx_nlte_destroy", None, [_VP]),
+    ("mgx_nlte_set_lte", C.c_int, [_VP, _VP, _VP, _VP, _VP]),
+    ("mgx_nlte_set_populations", C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP]),
+    ("mgx_nlte_get_populations", C.c_int, [_VP, _VP, _VP, _VP]),
+    ("mgx_nlte_iteration_stateq", C.c_int,
+     [_VP, _VP, _VP, _VP, _VP, _VP, C.POINTER(MgxNlteConv), _VP, _VP, _VP]),
+    ("mgx_nlte_iteration_ng", C.c_int,
+     [_VP, C.c_uint32, _VP, _VP, _VP, C.POINTER(MgxNlteConv), _VP]),
+    ("mgx_nlte_trial_ng", C.c_int, [_VP, C.c_uint32, C.POINTER(MgxNlteConv), _VP]),
+    ("mgx_nlte_history", C.c_int, [_VP, _VP]),
+    ("mgx_compute_level_populations_sparse", C.c_int,
+     [C.POINTER(MgxModel), C.POINTER(MgxOptions), _VP, C.c_int, C.c_int64, _VP, _VP, _VP,
+      _VP, _VP, _VP, _VP]),
     ("mgx_trace_stats_device", C.c_int, [C.POINTER(MgxModel), C.POINTER(MgxOptions), _VP, _VP]),
     ("mgx_set_kernel_timing", None, [C.c_int]),
     ("mgx_last_kernel_ms", C.c_double, []),

@@ -85,6 +85,37 @@ def _linedata(num, mass, energy, weight, irad, jrad, A):
                 irad=irad, jrad=jrad, A=A, Bs=Bs, Ba=Ba, frequency=freq)
 
 
+def colpar_constant(ld, num_col_partner, rate, tmp=None, orth_or_para="n"):
+    """A collision partner with the same de-excitation coefficient ``rate`` [m^3/s]
+    for every level pair (upper i > lower j) at every tabulated temperature, and
+    excitation coefficients from detailed balance at each of them,
+    Ce = Cd g_i/g_j exp(-(E_i - E_j)/(k T)) (the derivation of setup.py:722-728)."""
+    tmp = np.geomspace(5.0, 2.0e4, 13) if tmp is None else np.asarray(tmp, dtype=np.float64)
+    nlev = ld["nlev"]
+    icol, jcol = [], []
+    for i in range(1, nlev):
+        for j in range(i):
+            icol.append(i)
+            jcol.append(j)
+    return _colpar(ld, num_col_partner, orth_or_para, icol, jcol, tmp,
+                   np.full((len(tmp), len(icol)), float(rate)))
+
+
+def _colpar(ld, num_col_partner, orth_or_para, icol, jcol, tmp, Cd):
+    icol = np.ascontiguousarray(icol, dtype=np.uint32)
+    jcol = np.ascontiguousarray(jcol, dtype=np.uint32)
+    tmp = np.ascontiguousarray(tmp, dtype=np.float64)
+    Cd = np.ascontiguousarray(Cd, dtype=np.float64).reshape(len(tmp), len(icol))
+    E, g = ld["energy"], ld["weight"]
+    Ce = np.empty_like(Cd)
+    for t in range(len(tmp)):
+        for k in range(len(icol)):
+            i, j = int(icol[k]), int(jcol[k])
+            Ce[t, k] = Cd[t, k] * g[i] / g[j] * np.exp(-(E[i] - E[j]) / (KB * tmp[t]))
+    return dict(num_col_partner=int(num_col_partner), orth_or_para=orth_or_para, icol=icol,
+                jcol=jcol, tmp=tmp, Cd=Cd, Ce=Ce)
+
+
 def quadrature(nquads):
     roots, weights = np.polynomial.hermite.hermgauss(nquads)
     return roots, weights / np.sum(weights)
@@ -252,6 +283,27 @@ class ModelData:
                 keep.append(x)
                 setattr(a, f, x.ctypes.data_as(C.POINTER(C.c_uint32)))
         return arr, keep
+
+    def _colpar_arrays(self):
+        """(ncolpar (nlspecs,) uint32, MgxColldata array in species order, keep-alive list)."""
+        ncol = np.ascontiguousarray([len(d.get("colpar", [])) for d in self.linedata],
+                                    dtype=np.uint32)
+        allc = [c for d in self.linedata for c in d.get("colpar", [])]
+        arr = (_abi.MgxColldata * max(1, len(allc)))()
+        keep = [ncol]
+        for a, c in zip(arr, allc):
+            a.num_col_partner = c["num_col_partner"]
+            a.orth_or_para_H2 = ord(c["orth_or_para"])
+            a.ntmp, a.ncol = len(c["tmp"]), len(c["icol"])
+            for f in ("icol", "jcol"):
+                x = np.ascontiguousarray(c[f], dtype=np.uint32)
+                keep.append(x)
+                setattr(a, f, x.ctypes.data_as(C.POINTER(C.c_uint32)))
+            for f in ("tmp", "Cd", "Ce"):
+                x = np.ascontiguousarray(c[f], dtype=np.float64)
+                keep.append(x)
+                setattr(a, f, x.ctypes.data_as(C.POINTER(C.c_double)))
+        return ncol, arr, keep
 
     def finalize(self):
         """Run the host producers (Model::compute_* on the reference's path)."""
@@ -432,7 +484,9 @@ def model_line_1d(npoints=100, length=1.0e13, nquads=31, vmax=1.0e4, density=1.0
     abund[:, 0] = density
     abund[:, 1] = 1.0e-4 * density
     m.set_thermo(np.full(P, temperature), np.full(P, vturb), abund)
-    m.set_lines([linedata_two_level(freq, A)], nquads)
+    ld = linedata_two_level(freq, A)
+    ld["colpar"] = [colpar_constant(ld, 0, 1.0e-16)]
+    m.set_lines([ld], nquads)
     return m.finalize()
 
 
@@ -480,7 +534,9 @@ def model_wind(n_edge=32, half_size=1.0e14, nrays=12, nquads=31, seed=None, nlev
                    np.full(nrays, 1.0 / nrays))
     abund = np.stack([dens, 1.0e-4 * dens], axis=1)
     m.set_thermo(temp, np.full(len(r), vturb), abund)
-    m.set_lines([linedata_linear_rotor(B, dipole, nlev)], nquads)
+    ld = linedata_linear_rotor(B, dipole, nlev)
+    ld["colpar"] = [colpar_constant(ld, 0, 1.0e-16)]
+    m.set_lines([ld], nquads)
     return m.finalize()
 
 
@@ -501,9 +557,20 @@ def batch_winds(specs):
     return [model_wind(n_edge=e, nrays=r, seed=sd) for e, r, sd in specs]
 
 
-def read_lamda(path, num=1):
-    """Radiative part of a LAMDA-format line data file, as set_linedata_from_LAMDA_file
-    stores it (setup.py:632-779): energies in J, levels 0-based, derived Bs, Ba, frequency."""
+# LAMDA collision-partner codes -> (name, ortho/para flag)
+LAMDA_PARTNERS = {1: ("H2", "n"), 2: ("H2", "p"), 3: ("H2", "o"), 4: ("e", "n"), 5: ("H", "n"),
+                  6: ("He", "n"), 7: ("H+", "n")}
+
+
+def read_lamda(path, num=1, species_index=None):
+    """A LAMDA-format line data file as set_linedata_from_LAMDA_file stores it
+    (setup.py:632-779): energies in J, levels 0-based, derived Bs, Ba, frequency;
+    collision partners ("colpar") with Cd converted from cm^3/s to m^3/s and Ce
+    from detailed balance at each tabulated temperature (setup.py:710-737).
+    species_index maps partner names (LAMDA_PARTNERS) to abundance columns,
+    default {"H2": 1}; an unknown partner gets column 0, the reference's dummy
+    density (setup.py:568-572)."""
+    species_index = {"H2": 1} if species_index is None else species_index
     with open(path) as fh:
         lines = [l.strip() for l in fh if l.strip()]
     data = [l for l in lines if not l.startswith("!")]
@@ -521,7 +588,32 @@ def read_lamda(path, num=1):
         irad.append(int(parts[1]) - 1)
         jrad.append(int(parts[2]) - 1)
         A.append(float(parts[3]))
-    return _linedata(num, mass, np.array(energy), np.array(weight), irad, jrad, A)
+    ld = _linedata(num, mass, np.array(energy), np.array(weight), irad, jrad, A)
+    at = 4 + nlev + nrad
+    colpar = []
+    if at < len(data):
+        ncolpar = int(data[at].split()[0])
+        at += 1
+        for _ in range(ncolpar):
+            code = int(data[at].split()[0])
+            ncol = int(data[at + 1].split()[0])
+            ntmp = int(data[at + 2].split()[0])
+            at += 3
+            tmp = []
+            while len(tmp) < ntmp:
+                tmp.extend(float(t) for t in data[at].split())
+                at += 1
+            icol, jcol, Cd = [], [], np.empty((ntmp, ncol))
+            for k in range(ncol):
+                parts = data[at + k].split()
+                icol.append(int(parts[1]) - 1)
+                jcol.append(int(parts[2]) - 1)
+                Cd[:, k] = [float(x) * 1.0e-6 for x in parts[3:3 + ntmp]]
+            at += ncol
+            name, op = LAMDA_PARTNERS.get(code, ("?", "n"))
+            colpar.append(_colpar(ld, species_index.get(name, 0), op, icol, jcol, tmp, Cd))
+    ld["colpar"] = colpar
+    return ld
 
 
 def model_constant_slab(npoints=50, dx=1.0e12, n_species=1.0e3, n_h2=1.0e12, temperature=4.5,
@@ -581,6 +673,32 @@ def rays_spherical_symmetry(nrays):
     d = np.array((Rx, Ry, Rz)).T
     w = np.array(Wt)
     return np.ascontiguousarray(d), w / np.sum(w)
+
+
+def model_vanzadelhoff_1d(lamda_path, a_or_b="a", npoints=250, nrays=200, nquads=11):
+    """The reference's van Zadelhoff 1 non-LTE benchmark in 1D spherical symmetry
+    (tests/benchmarks/numeric/vanZadelhoff_1_1D.py:17-84): a two-level test
+    species (LAMDA file) in a static r^-2 H2 envelope from 1e13 to 7.8e16 m,
+    n_H2 = 2e13 (1e13 m / r)^2 m^-3, abundance 1e-8 ('a') or 1e-6 ('b'),
+    T = 20 K, turbulence 150 m/s, CMB boundary, neighbours i-1, i+1."""
+    r_in, r_out, nH2_in = 1.0e13, 7.8e16, 2.0e13
+    X_mol = {"a": 1.0e-8, "b": 1.0e-6}[a_or_b]
+    P = npoints
+    rs = np.logspace(np.log10(r_in), np.log10(r_out), P, endpoint=True)
+    pos = np.zeros((P, 3))
+    pos[:, 0] = rs
+    nbs = [[1]] + [[p - 1, p + 1] for p in range(1, P - 1)] + [[P - 2]]
+    m = ModelData()
+    m.dimension = 1
+    m.spherical_symmetry = 1
+    d, w = rays_spherical_symmetry(nrays)
+    m.set_geometry(pos, np.zeros((P, 3)), nbs, [0, P - 1], d, w)
+    nH2 = nH2_in * np.power(r_in / rs, 2.0)
+    abund = np.stack([X_mol * nH2, nH2, np.zeros(P)], axis=1)
+    m.set_thermo(np.full(P, 20.0), np.full(P, 150.0), abund)
+    ld = read_lamda(lamda_path, num=0, species_index={"H2": 1})
+    m.set_lines([ld], nquads)
+    return m.finalize()
 
 
 def model_velocity_gradient_1d(linedata, npoints=100, nrays=100, nquads=100, dx=1.0e4,

@@ -223,3 +223,112 @@ class BatchPlan:
             self.close()
         except Exception:
             pass
+
+
+class LevelPopulations:
+    """Device level-population state (mgx_nlte) of a DeviceModel, with the
+    reference's LineProducingSpecies operations: set_lte (update_using_LTE),
+    stateq (compute_Jeff_sparse + iteration_using_statistical_equilibrium_sparse),
+    ng / trial_ng (iteration_using_Ng_acceleration, the adaptive trial).  Every
+    update rewrites the DeviceModel's emissivity, opacity and interp_crit."""
+
+    def __init__(self, dm, params=None):
+        torch = _require_gpu()
+        m = dm.model
+        self.dm = dm
+        self.params = _abi.default_nlte_params() if params is None else params
+        ld, self._keep_ld = m._ld_array()
+        ncol, cp, self._keep_cp = m._colpar_arrays()
+        self.abundance = torch.from_numpy(np.ascontiguousarray(m.abundance)).to(dm.device)
+        self.nlev = [d["nlev"] for d in m.linedata]
+        h = C.c_void_p()
+        _abi.check(_abi.lib().mgx_nlte_create(C.byref(dm.view), m.nlspecs, ld, _abi.ptr(ncol), cp,
+                                              m.nspec, _abi.ptr(self.abundance),
+                                              C.byref(self.params), C.byref(h), dm.stream_ptr()))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            _abi.lib().mgx_nlte_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _eoi(self):
+        t = self.dm.tensors
+        return _abi.ptr(t["emissivity"]), _abi.ptr(t["opacity"]), _abi.ptr(t["interp_crit"])
+
+    def _conv(self, c):
+        return [(c[s].fraction_not_converged, c[s].relative_change_mean, c[s].relative_change_max)
+                for s in range(len(self.nlev))]
+
+    def set_lte(self):
+        _abi.check(_abi.lib().mgx_nlte_set_lte(self.h, *self._eoi(), self.dm.stream_ptr()))
+
+    def set_populations(self, pop):
+        torch = _require_gpu()
+        p = torch.as_tensor(np.ascontiguousarray(pop, dtype=np.float64)).to(self.dm.device)
+        _abi.check(_abi.lib().mgx_nlte_set_populations(self.h, _abi.ptr(p), *self._eoi(),
+                                                       self.dm.stream_ptr()))
+
+    def stateq(self, J, lambda_diag, want_J=False):
+        """One statistical-equilibrium iteration from a formal solve's J (P, F) and
+        lambda_diag (P, L); returns the convergence statistics per species (and
+        J_lin, J_eff (P, L) when want_J)."""
+        torch = _require_gpu()
+        m = self.dm.model
+        conv = (_abi.MgxNlteConv * len(self.nlev))()
+        Jl = Je = None
+        if want_J:
+            Jl = torch.empty((m.npoints, m.nlines), dtype=torch.float64, device=self.dm.device)
+            Je = torch.empty_like(Jl)
+        _abi.check(_abi.lib().mgx_nlte_iteration_stateq(self.h, _abi.ptr(J), _abi.ptr(lambda_diag),
+                                                        *self._eoi(), conv, _abi.ptr(Jl),
+                                                        _abi.ptr(Je), self.dm.stream_ptr()))
+        return (self._conv(conv), Jl, Je) if want_J else self._conv(conv)
+
+    def ng(self, order):
+        conv = (_abi.MgxNlteConv * len(self.nlev))()
+        _abi.check(_abi.lib().mgx_nlte_iteration_ng(self.h, order, *self._eoi(), conv,
+                                                    self.dm.stream_ptr()))
+        return self._conv(conv)
+
+    def trial_ng(self, order):
+        conv = (_abi.MgxNlteConv * len(self.nlev))()
+        _abi.check(_abi.lib().mgx_nlte_trial_ng(self.h, order, conv, self.dm.stream_ptr()))
+        return self._conv(conv)
+
+    def populations(self):
+        """(populations, population_tot): species blocks of (P, nlev) and (nlspecs, P)."""
+        torch = _require_gpu()
+        m = self.dm.model
+        pop = torch.empty(sum(self.nlev) * m.npoints, dtype=torch.float64, device=self.dm.device)
+        tot = torch.empty((m.nlspecs, m.npoints), dtype=torch.float64, device=self.dm.device)
+        _abi.check(_abi.lib().mgx_nlte_get_populations(self.h, _abi.ptr(pop), _abi.ptr(tot),
+                                                       self.dm.stream_ptr()))
+        return pop, tot
+
+    def n_residuals(self):
+        n = np.zeros(1, dtype=np.uint32)
+        _abi.check(_abi.lib().mgx_nlte_history(self.h, n.ctypes.data))
+        return int(n[0])
+
+
+def compute_level_populations_sparse(dm, opts, lp, use_ng=True, max_iterations=100):
+    """Model::compute_level_populations_sparse on the device (the native loop of
+    libmgx): returns (iterations, error_mean, error_max)."""
+    m = dm.model
+    err_mean = np.zeros(max(1, max_iterations) * m.nlspecs)
+    err_max = np.zeros_like(err_mean)
+    it = np.zeros(1, dtype=np.int32)
+    t = dm.tensors
+    _abi.check(_abi.lib().mgx_compute_level_populations_sparse(
+        C.byref(dm.view), C.byref(opts), lp.h, 1 if use_ng else 0, int(max_iterations),
+        _abi.ptr(t["emissivity"]), _abi.ptr(t["opacity"]), _abi.ptr(t["interp_crit"]),
+        it.ctypes.data, err_mean.ctypes.data, err_max.ctypes.data, dm.stream_ptr()))
+    n = int(it[0]) * m.nlspecs
+    return int(it[0]), err_mean[:n].tolist(), err_max[:n].tolist()

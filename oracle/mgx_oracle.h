@@ -42,6 +42,38 @@ int oracle_image_adjoint(const mgx_model *m, const mgx_options *op, const double
 /* 1: interpolate_log in double (smooth; for finite-difference checks), 0: as the reference */
 void oracle_set_smooth_interp(int on);
 
+/* Sparse level-population update in long double (mgx_oracle_nlte.c).  The
+   handle copies every array it is given; m is a host model view (sizes,
+   temperature, quad_weights, ray_weight).  Emissivity, opacity and the
+   interpolation criterion outputs are (P, L) and may be NULL.  Return codes:
+   0, -1 allocation, -2 no populations set, -3 Ng order beyond the history. */
+typedef struct oracle_nlte oracle_nlte;
+oracle_nlte *oracle_nlte_create(const mgx_model *m, uint32_t nlspecs, const mgx_linedata *ld,
+                                const uint32_t *ncolpar, const mgx_colldata *colpar,
+                                uint32_t nspec, const double *abundance,
+                                const mgx_nlte_params *prm);
+void oracle_nlte_destroy(oracle_nlte *h);
+int oracle_nlte_set_lte(oracle_nlte *h, double *eps, double *kap, double *ic);
+int oracle_nlte_set_populations(oracle_nlte *h, const double *pop, double *eps, double *kap,
+                                double *ic);
+/* J_lin from the plain solve's u (hnrays, P, F), J_eff with lambda_diag (P, L) */
+int oracle_nlte_compute_Jeff(oracle_nlte *h, const double *u, const double *lambda_diag);
+/* J_lin given as (P, L) */
+int oracle_nlte_set_Jeff(oracle_nlte *h, const double *Jlin, const double *lambda_diag);
+int oracle_nlte_stateq(oracle_nlte *h, const double *lambda_diag, double *eps, double *kap,
+                       double *ic);
+int oracle_nlte_ng(oracle_nlte *h, uint32_t order, double *eps, double *kap, double *ic);
+int oracle_nlte_trial_ng(oracle_nlte *h, uint32_t order);
+/* populations and trial populations (species blocks of (P, nlev)), J_lin, J_eff (P, L) */
+int oracle_nlte_get(const oracle_nlte *h, double *pop, double *trial, double *Jlin, double *Jeff);
+/* conv: (nlspecs, 3) fraction_not_converged, relative_change_mean, relative_change_max */
+int oracle_nlte_stats(const oracle_nlte *h, double *conv, uint32_t *n_residuals);
+/* one point's statistical-equilibrium system (column-major nlev x nlev) and right side */
+int oracle_nlte_stateq_system(const oracle_nlte *h, const double *lambda_diag, uint32_t l,
+                              uint32_t p, double *M, double *y);
+/* long-double column-pivoted QR solve of an n x n column-major system */
+int oracle_cpqr_solve(int n, const double *a_colmajor, const double *b, double *x);
+
 float oracle_expf(float x);
 float oracle_expm1f(float x);
 float oracle_logf(float x);

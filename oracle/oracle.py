@@ -157,3 +157,185 @@ def image_adjoint(model, opts, raydir, nx, ny, I_bar, freqs=None, nthreads=0, ki
                                           ny, ib.ctypes.data, de.ctypes.data, dk.ctypes.data,
                                           nthreads), "image_adjoint")
     return de, dk
+
+
+# ---------------------------------------------------------------------------
+# sparse level populations (mgx_oracle_nlte.c)
+
+
+def _nlte_sigs(h):
+    vp = C.c_void_p
+    h.oracle_nlte_create.restype = vp
+    h.oracle_nlte_create.argtypes = [vp, C.c_uint32, vp, vp, vp, C.c_uint32, vp, vp]
+    h.oracle_nlte_destroy.restype = None
+    h.oracle_nlte_destroy.argtypes = [vp]
+    for f, n in (("set_lte", 3), ("set_populations", 4), ("compute_Jeff", 2), ("set_Jeff", 2),
+                 ("stateq", 4), ("get", 4), ("stats", 2)):
+        fn = getattr(h, "oracle_nlte_" + f)
+        fn.argtypes = [vp] * (n + 1)
+    h.oracle_nlte_ng.argtypes = [vp, C.c_uint32, vp, vp, vp]
+    h.oracle_nlte_trial_ng.argtypes = [vp, C.c_uint32]
+    h.oracle_nlte_stateq_system.argtypes = [vp, vp, C.c_uint32, C.c_uint32, vp, vp]
+    h.oracle_cpqr_solve.argtypes = [C.c_int, vp, vp, vp]
+
+
+def cpqr_solve(A, b, kind="cr"):
+    """x = A^-1 b by the long-double column-pivoted Householder QR."""
+    A = np.asarray(A, dtype=np.float64)
+    n = A.shape[0]
+    a = np.ascontiguousarray(A.T)  # column-major
+    bb = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros(n)
+    h = lib(kind)
+    _nlte_sigs(h)
+    _check(h.oracle_cpqr_solve(n, a.ctypes.data, bb.ctypes.data, x.ctypes.data), "cpqr_solve")
+    return x
+
+
+class NLTE:
+    """Long-double level-population state of a ModelData (the oracle's
+    LineProducingSpecies set).  Writes the model's emissivity, opacity and
+    interp_crit arrays in place, as Lines::set_emissivity_and_opacity does."""
+
+    def __init__(self, model, params, kind="cr"):
+        from magritte_amd import _abi  # noqa: F401  (ctypes structs)
+
+        self.model = model
+        self.params = params
+        self.h_lib = lib(kind)
+        _nlte_sigs(self.h_lib)
+        self._view = model.view()
+        ld, self._keep_ld = model._ld_array()
+        ncol, cp, self._keep_cp = model._colpar_arrays()
+        self._ncol, self._cp = ncol, cp
+        self.h = self.h_lib.oracle_nlte_create(C.addressof(self._view), model.nlspecs, ld,
+                                               ncol.ctypes.data, cp, model.nspec,
+                                               model.abundance.ctypes.data, C.addressof(params))
+        if not self.h:
+            raise RuntimeError("oracle_nlte_create failed")
+        self.nlev = [d["nlev"] for d in model.linedata]
+
+    def close(self):
+        if self.h:
+            self.h_lib.oracle_nlte_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _eoi(self):
+        m = self.model
+        return m.emissivity.ctypes.data, m.opacity.ctypes.data, m.interp_crit.ctypes.data
+
+    def set_lte(self):
+        _check(self.h_lib.oracle_nlte_set_lte(self.h, *self._eoi()), "nlte_set_lte")
+
+    def set_populations(self, pop):
+        pop = np.ascontiguousarray(pop, dtype=np.float64)
+        _check(self.h_lib.oracle_nlte_set_populations(self.h, pop.ctypes.data, *self._eoi()),
+               "nlte_set_populations")
+
+    def compute_Jeff(self, u, lam):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        _check(self.h_lib.oracle_nlte_compute_Jeff(self.h, u.ctypes.data, lam.ctypes.data),
+               "nlte_compute_Jeff")
+
+    def set_Jeff(self, Jlin, lam):
+        Jlin = np.ascontiguousarray(Jlin, dtype=np.float64)
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        _check(self.h_lib.oracle_nlte_set_Jeff(self.h, Jlin.ctypes.data, lam.ctypes.data),
+               "nlte_set_Jeff")
+
+    def stateq(self, lam):
+        self._lam = np.ascontiguousarray(lam, dtype=np.float64)
+        _check(self.h_lib.oracle_nlte_stateq(self.h, self._lam.ctypes.data, *self._eoi()),
+               "nlte_stateq")
+
+    def ng(self, order):
+        _check(self.h_lib.oracle_nlte_ng(self.h, order, *self._eoi()), "nlte_ng")
+
+    def trial_ng(self, order):
+        _check(self.h_lib.oracle_nlte_trial_ng(self.h, order), "nlte_trial_ng")
+
+    def populations(self):
+        n = sum(self.nlev) * self.model.npoints
+        pop = np.zeros(n)
+        trial = np.zeros(n)
+        P, L = self.model.npoints, self.model.nlines
+        Jlin = np.zeros((P, L))
+        Jeff = np.zeros((P, L))
+        _check(self.h_lib.oracle_nlte_get(self.h, pop.ctypes.data, trial.ctypes.data,
+                                          Jlin.ctypes.data, Jeff.ctypes.data), "nlte_get")
+        return dict(population=pop, trial=trial, J_lin=Jlin, J_eff=Jeff)
+
+    def stats(self):
+        conv = np.zeros((self.model.nlspecs, 3))
+        nres = np.zeros(self.model.nlspecs, dtype=np.uint32)
+        _check(self.h_lib.oracle_nlte_stats(self.h, conv.ctypes.data, nres.ctypes.data),
+               "nlte_stats")
+        return conv, nres
+
+    def system(self, lam, l, p):
+        n = self.nlev[l]
+        M = np.zeros(n * n)
+        y = np.zeros(n)
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        _check(self.h_lib.oracle_nlte_stateq_system(self.h, lam.ctypes.data, l, p, M.ctypes.data,
+                                                    y.ctypes.data), "nlte_stateq_system")
+        return M.reshape(n, n).T.copy(), y
+
+
+def ng_decision(nlte, params, use_ng, prior):
+    """Model::ng_acceleration_criterion<Default / Adaptive> (model.cpp:580-645)."""
+    skip = params.Ng_acceleration_remove_N_its
+    if params.use_adaptive_Ng_acceleration:
+        if not use_ng or prior < params.adaptive_Ng_acceleration_min_order + skip:
+            return False, 0
+        col = 2 if params.adaptive_Ng_acceleration_use_max_criterion else 1
+        curr = float(nlte.stats()[0][:, col].sum())
+        nlte.trial_ng(prior - skip)
+        trial = float(nlte.stats()[0][:, col].sum())
+        if trial < curr or prior == params.Ng_acceleration_mem_limit + skip:
+            return True, prior - skip
+        return False, 0
+    if not use_ng:
+        return False, 0
+    if prior == params.Ng_acceleration_mem_limit + skip:
+        return True, params.Ng_acceleration_mem_limit
+    return False, 0
+
+
+def compute_level_populations_sparse(model, opts, params, use_ng=True, max_iterations=100,
+                                     nlte=None, nthreads=0, kind="cr"):
+    """Model::compute_level_populations_sparse (model.cpp:756-850) on the CPU:
+    each iteration the plain formal solve (u and the Lambda diagonal), then
+    compute_Jeff_sparse and the statistical-equilibrium update; Ng steps per
+    the criterion.  Starts from LTE unless an NLTE state is given.  Returns
+    (nlte, iterations, error_mean, error_max)."""
+    if nlte is None:
+        nlte = NLTE(model, params, kind)
+        nlte.set_lte()
+    iteration = 0
+    normal = 0
+    err_mean, err_max = [], []
+    some_not_converged = True
+    while some_not_converged and iteration < max_iterations:
+        use_step, order = ng_decision(nlte, params, use_ng, normal)
+        if use_step:
+            nlte.ng(order)
+            normal = 0
+            continue
+        iteration += 1
+        some_not_converged = False
+        u, _, lam = solve(model, opts, with_u=True, nthreads=nthreads, kind=kind)
+        nlte.compute_Jeff(u, lam)
+        nlte.stateq(lam)
+        normal += 1
+        conv = nlte.stats()[0]
+        for l in range(model.nlspecs):
+            err_mean.append(float(conv[l, 1]))
+            err_max.append(float(conv[l, 2]))
+            if conv[l, 0] > 1.0 - params.convergence_fraction:
+                some_not_converged = True
+    return nlte, iteration, err_mean, err_max

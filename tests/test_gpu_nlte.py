@@ -1,0 +1,238 @@
+"""GPU level populations (mgx_nlte_*, mgx_compute_level_populations_sparse)
+against the CPU oracle (oracle/mgx_oracle_nlte.c, long double), on the same
+inputs:
+
+* one statistical-equilibrium step from the same J and Lambda diagonal:
+  J_lin, J_eff, populations, line emissivity / opacity / interpolation
+  criterion within 1e-10 relative (populations below 1e-14 of the species
+  total compared absolutely at that floor; the opacity, a difference of two
+  terms that nearly cancel close to LTE, relative to the larger term);
+  convergence statistics;
+* the negative-population correction and the inversion -> LTE rule, driven
+  by unphysical negative J;
+* Ng acceleration and the adaptive trial after a history of updates;
+* the whole non-LTE loop (GPU formal solves + GPU updates) against the
+  oracle's loop: the same iteration count and populations within 1e-8
+  relative (the two loops' formal solves differ at the 1e-12 level and the
+  difference is carried through every iteration);
+* the reference's van Zadelhoff 1a benchmark against RATRAN within the
+  reference's bounds (0.12 / 0.13).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from magritte_amd import _abi, models
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LAMDA = os.path.join(ROOT, "tests", "golden", "lamda_test.txt")
+HH4PI = models.HH / (4.0 * np.pi)
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _close(got, ref, rtol, what, floor=0.0):
+    got = np.asarray(got, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    err = np.abs(got - ref)
+    bad = err > rtol * np.abs(ref) + floor
+    worst = float(np.max(err / (np.abs(ref) + floor / rtol + 1e-300)))
+    _say(f"  {what}: worst {worst:.3e}")
+    assert not bad.any(), f"{what}: {int(bad.sum())} entries off, worst {worst:.3e}"
+
+
+def _Jlin(m, J):
+    Q = m.nquads
+    w = m.quad_weights
+    out = np.empty((m.npoints, m.nlines))
+    for l in range(m.nlines):
+        s = int(m.line_species[l])
+        out[:, l] = (J[:, l * Q:(l + 1) * Q] * w[s]).sum(axis=1)
+    return out
+
+
+def _pair(m, prm):
+    """A fresh device state and oracle state of model m, both at LTE."""
+    from magritte_amd import solver
+    from oracle import oracle
+
+    dm = solver.DeviceModel(m)
+    lp = solver.LevelPopulations(dm, prm)
+    lp.set_lte()
+    nl = oracle.NLTE(m, prm)
+    nl.set_lte()
+    return dm, lp, nl
+
+
+def _check_state(m, dm, lp, nl, what, rtol=1e-10):
+    pop, tot = lp.populations()
+    ref = nl.populations()["population"]
+    off = 0
+    for s, n in enumerate(lp.nlev):
+        blk = slice(off, off + n * m.npoints)
+        ptot = np.repeat(tot.cpu().numpy()[s], n)
+        _close(pop.cpu().numpy()[blk], ref[blk], rtol, f"{what} populations[{s}]",
+               floor=1e-14 * ptot)
+        off += n * m.npoints
+    # the opacity is the difference nu h/4pi (n_j Ba - n_i Bs): its error scales with
+    # the two terms, not with their difference (close to LTE they cancel to ~1e-3)
+    scale = np.zeros((m.npoints, m.nlines))
+    off = 0
+    for s, d in enumerate(m.linedata):
+        n = d["nlev"]
+        x = ref[off:off + n * m.npoints].reshape(m.npoints, n)
+        lid = int(np.flatnonzero(m.line_species == s)[0])
+        for k in range(d["nrad"]):
+            scale[:, lid + k] = d["frequency"][k] * HH4PI * (np.abs(x[:, d["jrad"][k]] * d["Ba"][k])
+                                                             + np.abs(x[:, d["irad"][k]] * d["Bs"][k]))
+        off += n * m.npoints
+    ptot = np.stack([m.abundance[:, d["num"]] for d in m.linedata for _ in range(d["nrad"])], 1)
+    _close(dm.tensors["emissivity"].cpu().numpy(), m.emissivity, rtol, f"{what} emissivity")
+    _close(dm.tensors["opacity"].cpu().numpy(), m.opacity, rtol, f"{what} opacity",
+           floor=rtol * scale)
+    _close(dm.tensors["interp_crit"].cpu().numpy(), m.interp_crit, rtol, f"{what} interp_crit",
+           floor=rtol * scale / ptot)
+
+
+def _fresh_wind():
+    return models.model_wind(n_edge=10, seed=models.seed_for(1, 8))
+
+
+@pytest.mark.parametrize("which", ["line", "wind"])
+def test_stateq_matches_oracle(which):
+    from magritte_amd import solver
+
+    m = models.model_line_1d() if which == "line" else _fresh_wind()
+    prm = _abi.default_nlte_params()
+    _say(f"[stateq] model={which} P={m.npoints} nlev={m.linedata[0]['nlev']}")
+    dm, lp, nl = _pair(m, prm)
+    _check_state(m, dm, lp, nl, "LTE", rtol=1e-13)
+    out = solver.feautrier_forward(dm, m.options(), with_u=False)
+    J, lam = out["J"], out["lambda_diag"]
+    conv, Jl, Je = lp.stateq(J, lam, want_J=True)
+    Jn, lamn = J.cpu().numpy(), lam.cpu().numpy()
+    Jlin = _Jlin(m, Jn)
+    nl.set_Jeff(Jlin, lamn)
+    nl.stateq(lamn)
+    ref = nl.populations()
+    _close(Jl.cpu().numpy(), Jlin, 1e-13, "J_lin")
+    _close(Je.cpu().numpy(), ref["J_eff"], 1e-12, "J_eff", floor=1e-14 * np.abs(Jlin).max())
+    _check_state(m, dm, lp, nl, "SE")
+    rconv = nl.stats()[0]
+    for s in range(m.nlspecs):
+        w = 1.0 / (m.npoints * lp.nlev[s])
+        assert abs(conv[s][0] - rconv[s, 0]) <= 2 * w, (conv[s], rconv[s])
+        assert abs(conv[s][1] - rconv[s, 1]) <= 1e-8 * rconv[s, 1]
+        assert abs(conv[s][2] - rconv[s, 2]) <= 1e-8 * rconv[s, 2]
+    assert lp.n_residuals() == 1
+
+
+@pytest.mark.parametrize("inv_frac", [-1.0, 1.01])
+def test_negative_populations_match_oracle(inv_frac):
+    import torch
+
+    m = models.model_wind(n_edge=8, seed=models.seed_for(1, 9), nlev=6)
+    prm = _abi.default_nlte_params(population_inversion_fraction=inv_frac)
+    _say(f"[negative] inv_frac={inv_frac} P={m.npoints}")
+    dm, lp, nl = _pair(m, prm)
+    rng = np.random.default_rng(21)
+    Jl = rng.uniform(-1e-16, 1e-16, (m.npoints, m.nlines))
+    J = np.repeat(Jl, m.nquads, axis=1)
+    lam = np.zeros((m.npoints, m.nlines))
+    lp.stateq(torch.from_numpy(J).to(dm.device), torch.from_numpy(lam).to(dm.device))
+    nl.set_Jeff(_Jlin(m, J), lam)
+    nl.stateq(lam)
+    _check_state(m, dm, lp, nl, "corrected")
+    pop, _ = lp.populations()
+    assert (pop.cpu().numpy() >= 0).all()
+
+
+def test_ng_and_trial_match_oracle():
+    import torch
+
+    m = models.model_wind(n_edge=8, seed=models.seed_for(1, 10), nlev=8)
+    prm = _abi.default_nlte_params()
+    _say(f"[ng] P={m.npoints}")
+    dm, lp, nl = _pair(m, prm)
+    rng = np.random.default_rng(22)
+    lam = rng.uniform(0.0, 0.3, (m.npoints, m.nlines))
+    lam_d = torch.from_numpy(lam).to(dm.device)
+    nu = m.line
+    for it in range(4):
+        T = 30.0 + 10.0 * it
+        B = 2.0 * models.HH * nu**3 / models.CC**2 / np.expm1(models.HH * nu / (models.KB * T))
+        Jl = np.tile(B, (m.npoints, 1)) * rng.uniform(0.8, 1.2, (m.npoints, m.nlines))
+        J = np.repeat(Jl, m.nquads, axis=1)
+        lp.stateq(torch.from_numpy(J).to(dm.device), lam_d)
+        nl.set_Jeff(_Jlin(m, J), lam)
+        nl.stateq(lam)
+    _check_state(m, dm, lp, nl, "after 4 updates")
+    for order in (2, 3, 4):
+        _say(f"[ng] trial order {order}")
+        conv = lp.trial_ng(order)
+        nl.trial_ng(order)
+        rconv = nl.stats()[0]
+        assert abs(conv[0][2] - rconv[0, 2]) <= 1e-6 * rconv[0, 2], (conv, rconv)
+    _say("[ng] step order 4")
+    lp.ng(4)
+    nl.ng(4)
+    _check_state(m, dm, lp, nl, "Ng", rtol=1e-9)
+    assert lp.n_residuals() == 0
+
+
+def test_ng_order_beyond_history_is_an_error():
+    from magritte_amd import solver
+
+    m = models.model_line_1d(npoints=20)
+    dm = solver.DeviceModel(m)
+    lp = solver.LevelPopulations(dm)
+    lp.set_lte()
+    with pytest.raises(_abi.MgxError, match="greater than data stored"):
+        lp.ng(3)
+
+
+@pytest.mark.parametrize("which", ["line", "wind"])
+def test_level_population_loop_matches_oracle(which):
+    from magritte_amd import solver
+    from oracle import oracle
+
+    m = models.model_line_1d() if which == "line" else _fresh_wind()
+    m_ref = models.model_line_1d() if which == "line" else _fresh_wind()
+    prm = _abi.default_nlte_params()
+    _say(f"[loop] model={which} P={m.npoints}")
+    dm = solver.DeviceModel(m)
+    lp = solver.LevelPopulations(dm, prm)
+    lp.set_lte()
+    it, em, ex = solver.compute_level_populations_sparse(dm, m.options(), lp, True, 60)
+    nl, it_r, em_r, ex_r = oracle.compute_level_populations_sparse(m_ref, m_ref.options(), prm,
+                                                                   True, 60, nthreads=16)
+    _say(f"  iterations gpu {it} oracle {it_r}; last max change {ex[-1]:.3e} / {ex_r[-1]:.3e}")
+    assert it == it_r
+    pop, tot = lp.populations()
+    ref = nl.populations()["population"]
+    ptot = np.repeat(tot.cpu().numpy()[0], lp.nlev[0])
+    _close(pop.cpu().numpy(), ref, 1e-8, "populations", floor=1e-14 * ptot)
+    _close(np.array(em), np.array(em_r), 1e-4, "error_mean history",
+           floor=1e-3 * prm.pop_prec)
+
+
+def test_van_zadelhoff_1a_gpu():
+    from test_oracle_nlte import _ratran_errors
+
+    from magritte_amd import solver
+
+    m = models.model_vanzadelhoff_1d(LAMDA, "a")
+    _say(f"[van Zadelhoff 1a] P={m.npoints} nrays={m.nrays}")
+    dm = solver.DeviceModel(m)
+    lp = solver.LevelPopulations(dm)
+    lp.set_lte()
+    it, em, ex = solver.compute_level_populations_sparse(dm, m.options(), lp, True, 100)
+    pop, _ = lp.populations()
+    e0, e1 = _ratran_errors(m, pop.cpu().numpy().reshape(m.npoints, 2))
+    _say(f"  {it} iterations, max rel err vs RATRAN {e0:.4f} / {e1:.4f}")
+    assert e0 < 0.12 and e1 < 0.13
