@@ -510,6 +510,22 @@ def delaunay_neighbors(pos):
     return (n_nbs.astype(np.uint32), nbs.astype(np.uint32))
 
 
+def grf_clumps(n_edge, sigma=0.5, corr_length=1.0 / 20.0, seed=None):
+    """Log-normal clump factors exp(sigma g - sigma^2/2) on the n_edge^3 grid (the
+    point order of jittered_grid), g a unit-variance Gaussian random field with a
+    Gaussian correlation of length corr_length (fraction of the box): seeded
+    white noise filtered in Fourier space."""
+    rng = np.random.Generator(np.random.PCG64(seed_for(4) if seed is None else seed))
+    w = rng.standard_normal((n_edge, n_edge, n_edge))
+    k = 2.0 * np.pi * np.fft.fftfreq(n_edge)  # per grid spacing
+    ell = corr_length * n_edge                # correlation length in grid spacings
+    kx, ky, kz = np.meshgrid(k, k, k, indexing="ij")
+    filt = np.exp(-0.25 * (kx**2 + ky**2 + kz**2) * ell**2)
+    g = np.real(np.fft.ifftn(np.fft.fftn(w) * filt))
+    g = (g - g.mean()) / g.std()
+    return np.exp(sigma * g - 0.5 * sigma**2).ravel()
+
+
 def model_wind(n_edge=32, half_size=1.0e14, nrays=12, nquads=31, seed=None, nlev=10,
                B=57.6e9, dipole=0.11, v_out=15.0e3, v_rot=0.0, vturb=150.0, clumps=None,
                temp_scale=None):
