@@ -95,6 +95,7 @@ struct KParams {
     uint32_t cap;         /* LDS walk capacity (steps per half ray) */
     uint32_t nslots;      /* number of 32-lane line slots = L * ceil(Q/32) */
     uint32_t slots_per_line;
+    uint32_t j_lds;       /* forward solve: 1 = J accumulated in LDS, 0 = in the output row */
     uint32_t *status;     /* [0] error bits, [1] walk capacity needed, [2] adjoint segments needed */
 };
 
@@ -683,10 +684,34 @@ __host__ __device__ inline size_t walk_lds_bytes(uint32_t cap)
     return 8 * ((2 * (size_t)cap * 4 + 7) / 8) + 4 * (size_t)cap * 8 + 4 * (size_t)cap * 2;
 }
 
-/* dynamic LDS of the solve kernel: walk buffers + J (F) and Lambda (L) accumulators */
-__host__ __device__ inline size_t solve_lds_bytes(uint32_t cap, uint32_t F, uint32_t L)
+/* dynamic LDS of the solve kernel: walk buffers, the Lambda (L) accumulator and, when
+   j_lds, the J (F) accumulator */
+__host__ __device__ inline size_t solve_lds_bytes(uint32_t cap, uint32_t F, uint32_t L, bool j_lds)
 {
-    return ((walk_lds_bytes(cap) + 7) / 8) * 8 + 8 * ((size_t)F + L);
+    return ((walk_lds_bytes(cap) + 7) / 8) * 8 + 8 * (size_t)L + (j_lds ? 8 * (size_t)F : 0);
+}
+
+/* J stays in LDS while the whole LDS of a workgroup fits the share of a CU's
+   160 KiB that the register-limited occupancy (MGX_SOLVE_WAVES_PER_SIMD waves on
+   each of 4 SIMDs) leaves it; beyond that (many lines, F ~ 10^3) LDS would lower
+   the occupancy, and J is accumulated in the origin's row of the output instead
+   (only that origin's wave writes it) */
+inline bool j_in_lds(uint32_t cap, uint32_t F, uint32_t L)
+{
+    const size_t share = (160 * 1024) / (4 * MGX_SOLVE_WAVES_PER_SIMD);
+    const size_t stat = MGX_ERF_NINT * (MGX_ERF_DEG + 1) * sizeof(double);
+    return solve_lds_bytes(cap, F, L, true) + stat <= share;
+}
+
+/* J(o, f) += c for one ray pair: the first ray pair of a full pass stores (the sum
+   then starts from c, as 0 + c), later ones add; ray pairs of an origin split over
+   several waves add atomically into a cleared row */
+__device__ inline void j_add(double *Jrow, uint32_t f, double c, bool first, bool atomic)
+{
+    if (atomic)
+        unsafeAtomicAdd(&Jrow[f], c);
+    else
+        Jrow[f] = first ? c : Jrow[f] + c;
 }
 
 /*
@@ -768,13 +793,15 @@ __device__ inline void sweep_half(const KParams &kp, const double *s_erf, const 
 /*
  * Ray pairs rr0..rr1-1 of origin o (the body of feautrier_kernel): walk, per-
  * pass interpolation counts, two-sided elimination per lane, u stored when
- * given, J and the Lambda diagonal added into the LDS accumulators s_J (F) and
- * s_L (L).  Returns 0, or an ERR_* code after flagging it in kp.status.
+ * given, J added into the origin's row Jrow (F; atomically when j_atomic) and
+ * the Lambda diagonal into the LDS accumulator s_L (L).  Returns 0, or an
+ * ERR_* code after flagging it in kp.status.
  */
 template <int APPROX, int PREC>
 __device__ __forceinline__ int solve_origin(const KParams &kp, uint32_t o, uint32_t rr0,
                                             uint32_t rr1, unsigned char *smem, const double *s_erf,
-                                            double *u, bool with_lam, double *s_J, double *s_L)
+                                            double *u, bool with_lam, double *s_J, double *Jrow,
+                                            bool j_atomic, double *s_L)
 {
     const mgx_model &m = kp.m;
     const uint32_t lane = threadIdx.x;
@@ -836,7 +863,10 @@ __device__ __forceinline__ int solve_origin(const KParams &kp, uint32_t o, uint3
                 double uc;
                 if (S0 == 0 && S1 == 0) {
                     uc = boundary_intensity(kp, o, freq);
-                    s_J[f] = s_J[f] + 2.0 * w * uc;
+                    if (s_J)
+                        s_J[f] = s_J[f] + 2.0 * w * uc;
+                    else
+                        j_add(Jrow, f, 2.0 * w * uc, rr == rr0, j_atomic);
                 } else {
                     PointEC ec_cent = {0.0, 0.0, false};
                     Sweep bw, fw;
@@ -887,7 +917,10 @@ __device__ __forceinline__ int solve_origin(const KParams &kp, uint32_t o, uint3
                         uc = fma(A_l, fw.Y, Sl) * (1.0 + fw.X) * den;
                         Lc = (1.0 + fw.X) * den;
                     }
-                    s_J[f] = s_J[f] + uc * 2.0 * w;
+                    if (s_J)
+                        s_J[f] = s_J[f] + uc * 2.0 * w;
+                    else
+                        j_add(Jrow, f, uc * 2.0 * w, rr == rr0, j_atomic);
                     /* ---- Lambda diagonal (update_Lambda, solver.tpp:1761-1800) ---- */
                     if (with_lam && m.f_in_line[f]) {
                         const uint32_t spec = m.f_species[f];
@@ -950,12 +983,14 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_SOLVE_WAVES_PER_SIMD) void feautrier_
     const uint32_t o = blockIdx.x;
     const uint32_t lane = threadIdx.x;
     const uint32_t F = m.nfreqs, L = m.nlines;
-    double *s_J = reinterpret_cast<double *>(smem + ((walk_lds_bytes(kp.cap) + 7) / 8) * 8);
-    double *s_L = s_J + F;
-    for (uint32_t i = lane; i < F + L; i += MGX_WAVE) s_J[i] = 0.0;
-    if (solve_origin<APPROX, PREC>(kp, o, 0, m.hnrays, smem, s_erf, u, lam != nullptr, s_J, s_L))
+    double *s_L = reinterpret_cast<double *>(smem + ((walk_lds_bytes(kp.cap) + 7) / 8) * 8);
+    double *s_J = kp.j_lds ? s_L + L : nullptr;
+    for (uint32_t i = lane; i < L + (s_J ? F : 0); i += MGX_WAVE) s_L[i] = 0.0;
+    if (solve_origin<APPROX, PREC>(kp, o, 0, m.hnrays, smem, s_erf, u, lam != nullptr, s_J,
+                                   J + (size_t)o * F, false, s_L))
         return;
-    for (uint32_t i = lane; i < F; i += MGX_WAVE) J[(size_t)o * F + i] = s_J[i];
+    if (s_J)
+        for (uint32_t i = lane; i < F; i += MGX_WAVE) J[(size_t)o * F + i] = s_J[i];
     if (lam)
         for (uint32_t i = lane; i < L; i += MGX_WAVE) lam[(size_t)o * L + i] = s_L[i];
 }
@@ -997,22 +1032,26 @@ __global__ __launch_bounds__(MGX_WAVE, WPS) void feautrier_batch_kernel(
         const BatchItem it = items[w];
         const KParams &kp = kps[it.model];
         const uint32_t F = kp.m.nfreqs, L = kp.m.nlines;
-        double *s_J = reinterpret_cast<double *>(smem + ((walk_lds_bytes(kp.cap) + 7) / 8) * 8);
-        double *s_L = s_J + F;
-        for (uint32_t i = lane; i < F + L; i += MGX_WAVE) s_J[i] = 0.0;
+        double *s_L = reinterpret_cast<double *>(smem + ((walk_lds_bytes(kp.cap) + 7) / 8) * 8);
+        double *s_J = kp.j_lds ? s_L + L : nullptr;
+        for (uint32_t i = lane; i < L + (s_J ? F : 0); i += MGX_WAVE) s_L[i] = 0.0;
         double *Jm = J[it.model];
         double *Lm = lam ? lam[it.model] : nullptr;
+        const size_t o = it.origin;
+        const bool whole = it.rr0 == 0 && it.rr1 == kp.m.hnrays;
         if (solve_origin<APPROX, PREC>(kp, it.origin, it.rr0, it.rr1, smem, s_erf, nullptr,
-                                       Lm != nullptr, s_J, s_L))
+                                       Lm != nullptr, s_J, Jm + o * F, !whole, s_L))
             return;
         __syncthreads();
-        const size_t o = it.origin;
-        if (it.rr0 == 0 && it.rr1 == kp.m.hnrays) {
-            for (uint32_t i = lane; i < F; i += MGX_WAVE) Jm[o * F + i] = s_J[i];
+        if (whole) {
+            if (s_J)
+                for (uint32_t i = lane; i < F; i += MGX_WAVE) Jm[o * F + i] = s_J[i];
             if (Lm)
                 for (uint32_t i = lane; i < L; i += MGX_WAVE) Lm[o * L + i] = s_L[i];
         } else {
-            for (uint32_t i = lane; i < F; i += MGX_WAVE) unsafeAtomicAdd(&Jm[o * F + i], s_J[i]);
+            if (s_J)
+                for (uint32_t i = lane; i < F; i += MGX_WAVE)
+                    unsafeAtomicAdd(&Jm[o * F + i], s_J[i]);
             if (Lm)
                 for (uint32_t i = lane; i < L; i += MGX_WAVE)
                     unsafeAtomicAdd(&Lm[o * L + i], s_L[i]);
@@ -2824,6 +2863,7 @@ int launch_with_capacity(hipStream_t st, DevBuffers &b, KParams &kp, Launch laun
     uint32_t cap = 128;
     for (;;) {
         kp.cap = cap;
+        kp.j_lds = j_in_lds(cap, kp.m.nfreqs, kp.m.nlines) ? 1 : 0;
         HIP_TRY(hipMemsetAsync(b.status, 0, 4 * sizeof(uint32_t), st));
         HIP_TRY(hipMemcpyAsync(b.kp_dev, &kp, sizeof(KParams), hipMemcpyHostToDevice, st));
         launch(walk_lds_bytes(cap));
@@ -3079,9 +3119,11 @@ uint32_t batch_grid(uint32_t wps, size_t lds)
     return persistent_waves(feautrier_batch_kernel<A, PR, 3>, lds, 0xffffffffu);
 }
 
+bool batch_j_lds(const mgx_batch &B) { return j_in_lds(B.cap, B.maxF, B.maxL); }
+
 size_t batch_lds(const mgx_batch &B)
 {
-    return walk_lds_bytes(B.cap) + 8 + 8 * ((size_t)B.maxF + B.maxL);
+    return walk_lds_bytes(B.cap) + 8 + 8 * (size_t)B.maxL + (batch_j_lds(B) ? 8 * (size_t)B.maxF : 0);
 }
 
 /* persistent grid of the current configuration (resident waves, at most n_items) */
@@ -3162,7 +3204,10 @@ int batch_upload(mgx_batch &B, hipStream_t st)
     B.n_items = (uint32_t)it.size();
     HIP_TRY(hipMemcpyAsync(B.items_dev, it.data(), it.size() * sizeof(BatchItem),
                            hipMemcpyHostToDevice, st));
-    for (auto &k : B.kps) k.cap = B.cap;
+    for (auto &k : B.kps) {
+        k.cap = B.cap;
+        k.j_lds = batch_j_lds(B) ? 1 : 0;
+    }
     HIP_TRY(hipMemcpyAsync(B.kps_dev, B.kps.data(), B.kps.size() * sizeof(KParams),
                            hipMemcpyHostToDevice, st));
     HIP_TRY(hipStreamSynchronize(st));
@@ -3222,7 +3267,7 @@ int batch_execute(mgx_batch &B, hipStream_t st)
             return mgx::fail(MGX_E_NO_NEXT, "a ray from an interior point found no valid next point");
         if (!(h[0] & ERR_CAP)) return MGX_OK;
         uint32_t max_cap = kMaxCap;
-        const size_t extra = 8 + 8 * ((size_t)B.maxF + B.maxL);
+        const size_t extra = 8 + 8 * (size_t)B.maxL;
         while (max_cap > 16 && walk_lds_bytes(max_cap) + extra > kLdsBudget) max_cap -= 16;
         if (B.cap >= max_cap)
             return mgx::fail(MGX_E_RAY_LENGTH, "a ray needs more than %u walk steps", max_cap);
@@ -3252,7 +3297,7 @@ int mgx_feautrier_forward_device(const mgx_model *model, const mgx_options *opts
     if (int e = prepare(model, opts, st, b, kp)) return e;
     return launch_with_capacity(st, b, kp, [&](size_t) {
         const dim3 grid(model->npoints), block(MGX_WAVE);
-        const size_t lds = solve_lds_bytes(kp.cap, model->nfreqs, model->nlines);
+        const size_t lds = solve_lds_bytes(kp.cap, model->nfreqs, model->nlines, kp.j_lds != 0);
         timed(st, [&] {
             const bool fp64 = opts->precision == MGX_PREC_FP64;
             switch (opts->approx) {
@@ -3281,7 +3326,7 @@ int mgx_feautrier_forward_device(const mgx_model *model, const mgx_options *opts
                                        grid, block, lds, st, b.kp_dev, u, J, lambda_diag);
             }
         });
-    }, 8 * ((size_t)model->nfreqs + model->nlines));
+    }, 8 * (size_t)model->nlines);
 }
 
 
