@@ -58,14 +58,43 @@ namespace {
 
 enum { ERR_CAP = 1, ERR_NO_NEXT = 2, ERR_OUT = 4, ERR_CAPE = 8 };
 
-/* global-address-space load (the pointers live in a struct, so the compiler
-   would otherwise emit flat loads) */
+/* global-address-space load (the pointers live in a struct, so the compiler would
+   otherwise emit flat loads, which also count against the LDS wait counter and so
+   serialise the gathers with LDS traffic).  Records are read as 16-, 8- or 4-byte
+   words through address-space-1 pointers: a whole-struct copy through such a
+   pointer still becomes a flat load. */
 #define MGX_GLOBAL __attribute__((address_space(1)))
 template <typename T>
 __device__ __forceinline__ T gld(const T *p)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
-    return *(const MGX_GLOBAL T *)p;
+    static_assert(sizeof(T) % 4 == 0, "gld reads whole 32-bit words");
+    if constexpr (sizeof(T) % 16 == 0) {
+        typedef unsigned int w4 __attribute__((ext_vector_type(4)));
+        const MGX_GLOBAL w4 *q = (const MGX_GLOBAL w4 *)p;
+        w4 tmp[sizeof(T) / 16];
+#pragma unroll
+        for (unsigned i = 0; i < sizeof(T) / 16; i++) tmp[i] = q[i];
+        T r;
+        __builtin_memcpy(&r, tmp, sizeof(T));
+        return r;
+    } else if constexpr (sizeof(T) % 8 == 0) {
+        const MGX_GLOBAL unsigned long long *q = (const MGX_GLOBAL unsigned long long *)p;
+        unsigned long long tmp[sizeof(T) / 8];
+#pragma unroll
+        for (unsigned i = 0; i < sizeof(T) / 8; i++) tmp[i] = q[i];
+        T r;
+        __builtin_memcpy(&r, tmp, sizeof(T));
+        return r;
+    } else {
+        const MGX_GLOBAL unsigned *q = (const MGX_GLOBAL unsigned *)p;
+        unsigned tmp[sizeof(T) / 4];
+#pragma unroll
+        for (unsigned i = 0; i < sizeof(T) / 4; i++) tmp[i] = q[i];
+        T r;
+        __builtin_memcpy(&r, tmp, sizeof(T));
+        return r;
+    }
 #else
     return *p;
 #endif
@@ -112,7 +141,7 @@ __device__ inline const double *ray_dir(const mgx_model &m, uint32_t o, uint32_t
 
 __device__ inline double ray_weight(const mgx_model &m, uint32_t o, uint32_t r)
 {
-    return m.adaptive_directions ? m.ray_weight[(size_t)o * m.nrays + r] : m.ray_weight[r];
+    return m.adaptive_directions ? gld(&m.ray_weight[(size_t)o * m.nrays + r]) : gld(&m.ray_weight[r]);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -354,9 +383,9 @@ __device__ inline double boundary_intensity(const KParams &kp, uint32_t p, doubl
 {
     const mgx_model &m = kp.m;
     const uint32_t b = gld(&kp.pidx[p]).p2b;
-    const int32_t bc = (b < m.nboundary) ? m.boundary_condition[b] : MGX_BC_CMB;
+    const int32_t bc = (b < m.nboundary) ? gld(&m.boundary_condition[b]) : MGX_BC_CMB;
     if (bc == MGX_BC_ZERO) return 0.0;
-    if (bc == MGX_BC_THERMAL) return planck(m.boundary_temperature[b], freq);
+    if (bc == MGX_BC_THERMAL) return planck(gld(&m.boundary_temperature[b]), freq);
     return planck(MGX_T_CMB, freq);
 }
 
@@ -365,7 +394,7 @@ __device__ inline uint32_t lower_bound_d(const double *a, uint32_t n, double v)
     uint32_t lo = 0, len = n;
     while (len > 0) {
         const uint32_t h = len / 2;
-        if (a[lo + h] < v) {
+        if (gld(&a[lo + h]) < v) {
             lo = lo + h + 1;
             len = len - h - 1;
         } else {
@@ -380,7 +409,7 @@ __device__ inline uint32_t upper_bound_d(const double *a, uint32_t n, double v)
     uint32_t lo = 0, len = n;
     while (len > 0) {
         const uint32_t h = len / 2;
-        if (!(v < a[lo + h])) {
+        if (!(v < gld(&a[lo + h]))) {
             lo = lo + h + 1;
             len = len - h - 1;
         } else {
@@ -406,11 +435,11 @@ __device__ inline OwnLine own_line(const mgx_model &m, uint32_t l)
     OwnLine r;
     uint32_t g = 0;
     for (uint32_t s = 0; s < m.nlines; s++)
-        if (m.sorted_line_map[s] == l) g = s;
+        if (gld(&m.sorted_line_map[s]) == l) g = s;
     r.g = g;
-    r.a_g = m.sorted_line[g];
-    r.a_lo = (g > 0) ? m.sorted_line[g - 1] : -__builtin_inf();
-    r.a_hi = (g + 1 < m.nlines) ? m.sorted_line[g + 1] : __builtin_inf();
+    r.a_g = gld(&m.sorted_line[g]);
+    r.a_lo = (g > 0) ? gld(&m.sorted_line[g - 1]) : -__builtin_inf();
+    r.a_hi = (g + 1 < m.nlines) ? gld(&m.sorted_line[g + 1]) : __builtin_inf();
     return r;
 }
 
@@ -452,8 +481,8 @@ __device__ void eta_chi_interp(const KParams &kp, const OwnLine &ol, uint32_t p1
         uint32_t l;
         double diff;
         if (APPROX == MGX_APPROX_CLOSE_LINES) {
-            l = m.sorted_line_map[s];
-            diff = freq - m.sorted_line[s];
+            l = gld(&m.sorted_line_map[s]);
+            diff = freq - gld(&m.sorted_line[s]);
         } else {
             l = s;
             diff = freq - m.line[l];
@@ -528,10 +557,10 @@ __device__ void S_dtau_line_integrated(const KParams &kp, const double *erf_tab,
         line_window(m, ol, left_freq - W, right_freq + W, lo, hi);
     }
     for (uint32_t s = lo; s < hi; s++) {
-        const uint32_t l = (APPROX == MGX_APPROX_CLOSE_LINES) ? m.sorted_line_map[s] : s;
+        const uint32_t l = (APPROX == MGX_APPROX_CLOSE_LINES) ? gld(&m.sorted_line_map[s]) : s;
         const double4 c = lp(kp, cp, l), cI = lp(kp, cpi, l);
         const double4 n = lp(kp, np, l), nI = lp(kp, npi, l);
-        const double line_dtau = dtau_single_line<PREC>(kp, erf_tab, c, cI, n, nI, m.line[l],
+        const double line_dtau = dtau_single_line<PREC>(kp, erf_tab, c, cI, n, nI, gld(&m.line[l]),
                                                         currfreq, nextfreq, curr_interp,
                                                         next_interp, dZ);
         const double Sc = interp_log<PREC>(c.w, cI.w, curr_interp);
@@ -817,7 +846,7 @@ __device__ __forceinline__ int solve_origin(const KParams &kp, uint32_t o, uint3
     const size_t PF = (size_t)P * F;
 
     for (uint32_t rr = rr0; rr < rr1; rr++) {
-        const uint32_t ar = m.antipod[rr];
+        const uint32_t ar = gld(&m.antipod[rr]);
         uint32_t S0, S1;
         double sh00, sh01;
         int e = walk_half(kp, o, rr, s_nxt0, s_dZ0, s_sh0, S0, sh00);
@@ -859,7 +888,7 @@ __device__ __forceinline__ int solve_origin(const KParams &kp, uint32_t o, uint3
                 const OwnLine ol = own_line(m, l);
                 const uint32_t f = l * Q + z;
                 const size_t of = (size_t)o * F + f;
-                const double freq = m.nu[of];
+                const double freq = gld(&m.nu[of]);
                 double uc;
                 if (S0 == 0 && S1 == 0) {
                     uc = boundary_intensity(kp, o, freq);
@@ -923,21 +952,21 @@ __device__ __forceinline__ int solve_origin(const KParams &kp, uint32_t o, uint3
                         j_add(Jrow, f, uc * 2.0 * w, rr == rr0, j_atomic);
                     /* ---- Lambda diagonal (update_Lambda, solver.tpp:1761-1800) ---- */
                     if (with_lam && m.f_in_line[f]) {
-                        const uint32_t spec = m.f_species[f];
-                        const uint32_t k = m.f_k[f];
-                        const double freq_line = m.line[l];
+                        const uint32_t spec = gld(&m.f_species[f]);
+                        const uint32_t k = gld(&m.f_k[f]);
+                        const double freq_line = gld(&m.line[l]);
                         const double w_ang = 2.0 * w;
                         const double constante =
-                            m.line_A[l] * m.quad_weights[(size_t)spec * Q + m.f_z[f]] * w_ang;
+                            gld(&m.line_A[l]) * gld(&m.quad_weights[(size_t)spec * Q + gld(&m.f_z[f])]) * w_ang;
                         double eta, chi;
                         eta_chi_interp<APPROX, PREC>(kp, ol, o, o, 0.0, k, freq_line, eta, chi);
                         const double inverse_chi = rcp(chi);
                         const double frq = freq * 1.0;
-                        const double inv_mass = m.species_inverse_mass[spec];
+                        const double inv_mass = gld(&m.species_inverse_mass[spec]);
                         const double width =
                             freq_line * sqrt(MGX_TWO_KB_OVER_AMU_CC_SQUARED * inv_mass *
-                                                 m.temperature[o] +
-                                             m.vturb2[o]);
+                                                 gld(&m.temperature[o]) +
+                                             gld(&m.vturb2[o]));
                         const double inverse_width = 1.0 / width;
                         const double se = inverse_width * (frq - freq_line);
                         const double ex = se * se;
@@ -2467,7 +2496,7 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_ADJ_WAVES_PER_SIMD) void feautrier_ad
 
     for (uint32_t o = blockIdx.x; o < P; o += gridDim.x) {
         for (uint32_t rr = 0; rr < m.hnrays; rr++) {
-            const uint32_t ar = m.antipod[rr];
+            const uint32_t ar = gld(&m.antipod[rr]);
             uint32_t S0, S1;
             double sh00, sh01;
             __syncthreads();
@@ -2508,7 +2537,7 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_ADJ_WAVES_PER_SIMD) void feautrier_ad
                 const OwnLine ol = own_line(m, l);
                 const uint32_t f = l * Q + z;
                 const size_t of = (size_t)o * F + f;
-                const double freq = m.nu[of];
+                const double freq = gld(&m.nu[of]);
                 const double ubar = (z0 < Q) ? 2.0 * w * J_bar[of] : 0.0;
                 PointECT ec_cent = {0.0, 0.0, 0.0, false};
                 Sweep bw, fw;
