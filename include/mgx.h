@@ -135,6 +135,22 @@ int mgx_feautrier_forward_device(const mgx_model *model_dev, const mgx_options *
                                  double *u, double *J, double *lambda_diag, void *stream);
 
 /*
+ * Order-0 short-characteristics formal solve on the device: fills I
+ * (nrays, P, F) [may be NULL; I(rr) and I(antipod(rr)) are the intensities
+ * arriving at each point from the rr and the antipod side], u = (I_f + I_b)/2
+ * and v = (I_f - I_b)/2 (hnrays, P, F) [may be NULL], J (P, F) and
+ * lambda_diag (P, L) [may be NULL: the elements of both directions].
+ * min_dtau is Parameters::min_dtau (default 1e-15).  Replaces
+ * Model::compute_radiation_field_shortchar_order_0 (src/model/model.cpp:313-351)
+ * -> Solver::solve_shortchar_order_0<approx, adaptive> (src/solver/solver.tpp:
+ * 151-235, 1565-1757).  A direction without a segment adds no Lambda element
+ * (the reference reads an uninitialised optical depth there).
+ */
+int mgx_shortchar_order_0_device(const mgx_model *model_dev, const mgx_options *opts,
+                                 double min_dtau, double *I, double *u, double *v, double *J,
+                                 double *lambda_diag, void *stream);
+
+/*
  * Traced point counts n_tot(rr, o, l) = last + 1 - first for every ray pair,
  * origin and line, written as (hnrays, P, L) uint32.
  * Same trace as Solver::trace_ray_for_line (src/solver/solver.tpp:774-800,

@@ -124,6 +124,8 @@ SIGNATURES = [
     ("mgx_version", C.c_char_p, []),
     ("mgx_feautrier_forward_device", C.c_int,
      [C.POINTER(MgxModel), C.POINTER(MgxOptions), _VP, _VP, _VP, _VP]),
+    ("mgx_shortchar_order_0_device", C.c_int,
+     [C.POINTER(MgxModel), C.POINTER(MgxOptions), C.c_double, _VP, _VP, _VP, _VP, _VP, _VP]),
     ("mgx_trace_counts_device", C.c_int,
      [C.POINTER(MgxModel), C.POINTER(MgxOptions), _VP, _VP]),
     ("mgx_trace_rays_device", C.c_int,

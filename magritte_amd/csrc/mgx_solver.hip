@@ -820,17 +820,107 @@ __device__ inline void sweep_half(const KParams &kp, const double *s_erf, const 
 }
 
 /*
+ * One direction of the order-0 short-characteristics solve
+ * (solve_shortchar_order_0_ray_forward / _backward, solver.tpp:1565-1757): from
+ * the outer end of a half ray to the origin, I <- e^-dt I + expm1(-dt)/dt
+ * (S_c - S_n (1 + dt)) + S_c - S_n per segment with dt = max(min_dtau, dtau)
+ * and the reference's float expf / expm1f.  Segments run from the outer entry
+ * (current) to the inner one (next) on both halves, as the reference calls
+ * compute_source_dtau.  dt_last receives the last segment's dt (0 when the half
+ * has no segment: I is then the boundary intensity at the origin).
+ */
+template <int APPROX, int PREC>
+__device__ inline double shortchar_half(const KParams &kp, const double *s_erf, const OwnLine &ol,
+                                        uint32_t S, uint32_t o, double shift0,
+                                        const uint32_t *s_nxt, const double *s_dZ,
+                                        const double *s_sh, const uint16_t *mtab, uint32_t l,
+                                        double freq, double min_dtau, double &dt_last,
+                                        PointEC &ec_centre)
+{
+    dt_last = 0.0;
+    if (S == 0) return boundary_intensity(kp, o, freq);
+    HalfIter it;
+    it.s = (int)S - 1;
+    hi_load_step(kp, it, l, o, shift0, s_nxt, s_dZ, s_sh, mtab);
+    Entry outer = hi_entry(it, o);
+    PointEC ec_out = {0.0, 0.0, false};
+    double I = boundary_intensity(kp, outer.nr, freq * outer.shift);
+    double dt = 0.0;
+    for (;;) {
+        const double dz = hi_advance(kp, it, l, o, shift0, s_nxt, s_dZ, s_sh, mtab);
+        Entry inner = hi_entry(it, o);
+        PointEC ec_in = (it.s < 0) ? ec_centre : PointEC{0.0, 0.0, false};
+        double dtau, S_c, S_n;
+        segment<APPROX, PREC>(kp, s_erf, ol, outer, inner, dz, freq, l, ec_out, ec_in, dtau, S_c,
+                              S_n);
+        dt = (min_dtau < dtau) ? dtau : min_dtau; /* std::max(min_dtau, dtau) */
+        const float mdt = (float)(-dt);
+        I = (double)mgx_expf(mdt) * I + (double)mgx_expm1f(mdt) / dt * (S_c - S_n * (1.0 + dt)) +
+            S_c - S_n;
+        outer = inner;
+        ec_out = ec_in;
+        if (it.s < 0) break;
+    }
+    ec_centre = ec_out;
+    dt_last = dt;
+    return I;
+}
+
+/* The factors of a Lambda element at the origin that do not depend on the
+   solve (update_Lambda, solver.tpp:1777-1803, and the short-characteristics
+   rays, :1618-1637): the element is constante * frq * phi * X * inverse_chi. */
+struct LamPoint {
+    double constante, frq, phi, inverse_chi;
+};
+
+template <int APPROX, int PREC>
+__device__ inline LamPoint lambda_point(const KParams &kp, const OwnLine &ol, uint32_t o,
+                                        uint32_t l, uint32_t f, double freq, double w_ang)
+{
+    const mgx_model &m = kp.m;
+    const uint32_t Q = m.nquads;
+    const uint32_t spec = gld(&m.f_species[f]);
+    const uint32_t k = gld(&m.f_k[f]);
+    const double freq_line = gld(&m.line[l]);
+    LamPoint r;
+    r.constante = gld(&m.line_A[l]) * gld(&m.quad_weights[(size_t)spec * Q + gld(&m.f_z[f])]) * w_ang;
+    double eta, chi;
+    eta_chi_interp<APPROX, PREC>(kp, ol, o, o, 0.0, k, freq_line, eta, chi);
+    r.inverse_chi = rcp(chi);
+    r.frq = freq * 1.0;
+    const double inv_mass = gld(&m.species_inverse_mass[spec]);
+    const double width = freq_line * sqrt(MGX_TWO_KB_OVER_AMU_CC_SQUARED * inv_mass *
+                                              gld(&m.temperature[o]) +
+                                          gld(&m.vturb2[o]));
+    const double inverse_width = 1.0 / width;
+    const double se = inverse_width * (r.frq - freq_line);
+    const double ex = se * se;
+    r.phi = inverse_width * MGX_INVERSE_SQRT_PI * (double)f_exp<PREC>((float)(-ex));
+    return r;
+}
+
+/* which formal solution solve_origin runs */
+enum { METHOD_FEAUTRIER = 0, METHOD_SHORTCHAR = 1 };
+
+/* extra outputs of the short-characteristics solve */
+struct ShortcharOut {
+    double *I;       /* (nrays, P, F) or NULL */
+    double *v;       /* (hnrays, P, F) or NULL */
+    double min_dtau; /* Parameters::min_dtau */
+};
+
+/*
  * Ray pairs rr0..rr1-1 of origin o (the body of feautrier_kernel): walk, per-
  * pass interpolation counts, two-sided elimination per lane, u stored when
  * given, J added into the origin's row Jrow (F; atomically when j_atomic) and
  * the Lambda diagonal into the LDS accumulator s_L (L).  Returns 0, or an
  * ERR_* code after flagging it in kp.status.
  */
-template <int APPROX, int PREC>
+template <int APPROX, int PREC, int METHOD = METHOD_FEAUTRIER>
 __device__ __forceinline__ int solve_origin(const KParams &kp, uint32_t o, uint32_t rr0,
                                             uint32_t rr1, unsigned char *smem, const double *s_erf,
                                             double *u, bool with_lam, double *s_J, double *Jrow,
-                                            bool j_atomic, double *s_L)
+                                            bool j_atomic, double *s_L, ShortcharOut sc = {})
 {
     const mgx_model &m = kp.m;
     const uint32_t lane = threadIdx.x;
@@ -889,6 +979,50 @@ __device__ __forceinline__ int solve_origin(const KParams &kp, uint32_t o, uint3
                 const uint32_t f = l * Q + z;
                 const size_t of = (size_t)o * F + f;
                 const double freq = gld(&m.nu[of]);
+                if constexpr (METHOD == METHOD_SHORTCHAR) {
+                    /* solve_shortchar_order_0 (solver.tpp:186-224): both directions,
+                       u, v, J and a Lambda element per direction */
+                    double If, Ib, uc, vc;
+                    if (S0 == 0 && S1 == 0) {
+                        If = Ib = uc = boundary_intensity(kp, o, freq);
+                        vc = 0.0;
+                    } else {
+                        PointEC ec_cent = {0.0, 0.0, false};
+                        double dtf, dtb;
+                        If = shortchar_half<APPROX, PREC>(kp, s_erf, ol, S0, o, sh00, s_nxt0, s_dZ0,
+                                                          s_sh0, s_m + (jj * 2 + 0) * cap, l, freq,
+                                                          sc.min_dtau, dtf, ec_cent);
+                        Ib = shortchar_half<APPROX, PREC>(kp, s_erf, ol, S1, o, sh01, s_nxt1, s_dZ1,
+                                                          s_sh1, s_m + (jj * 2 + 1) * cap, l, freq,
+                                                          sc.min_dtau, dtb, ec_cent);
+                        uc = 0.5 * (If + Ib);
+                        vc = 0.5 * (If - Ib);
+                        if (with_lam) {
+                            /* a direction without a segment adds nothing (the reference
+                               reads an uninitialised dtau there) */
+                            const LamPoint lq = lambda_point<APPROX, PREC>(kp, ol, o, l, f, freq, w);
+                            if (S0 > 0) {
+                                const double factor = (double)mgx_expm1f((float)(-dtf)) / dtf;
+                                Lval += lq.constante * lq.frq * lq.phi * (factor + 1.0) * lq.inverse_chi;
+                            }
+                            if (S1 > 0) {
+                                const double factor = (double)mgx_expm1f((float)(-dtb)) / dtb;
+                                Lval += lq.constante * lq.frq * lq.phi * (factor + 1.0) * lq.inverse_chi;
+                            }
+                        }
+                    }
+                    const double wt = w * 2.0;
+                    if (s_J)
+                        s_J[f] = s_J[f] + wt * uc;
+                    else
+                        j_add(Jrow, f, wt * uc, rr == rr0, j_atomic);
+                    if (sc.I) {
+                        sc.I[(size_t)rr * PF + of] = If;
+                        sc.I[(size_t)ar * PF + of] = Ib;
+                    }
+                    if (u) u[(size_t)rr * PF + of] = uc;
+                    if (sc.v) sc.v[(size_t)rr * PF + of] = vc;
+                } else {
                 double uc;
                 if (S0 == 0 && S1 == 0) {
                     uc = boundary_intensity(kp, o, freq);
@@ -952,30 +1086,12 @@ __device__ __forceinline__ int solve_origin(const KParams &kp, uint32_t o, uint3
                         j_add(Jrow, f, uc * 2.0 * w, rr == rr0, j_atomic);
                     /* ---- Lambda diagonal (update_Lambda, solver.tpp:1761-1800) ---- */
                     if (with_lam && m.f_in_line[f]) {
-                        const uint32_t spec = gld(&m.f_species[f]);
-                        const uint32_t k = gld(&m.f_k[f]);
-                        const double freq_line = gld(&m.line[l]);
-                        const double w_ang = 2.0 * w;
-                        const double constante =
-                            gld(&m.line_A[l]) * gld(&m.quad_weights[(size_t)spec * Q + gld(&m.f_z[f])]) * w_ang;
-                        double eta, chi;
-                        eta_chi_interp<APPROX, PREC>(kp, ol, o, o, 0.0, k, freq_line, eta, chi);
-                        const double inverse_chi = rcp(chi);
-                        const double frq = freq * 1.0;
-                        const double inv_mass = gld(&m.species_inverse_mass[spec]);
-                        const double width =
-                            freq_line * sqrt(MGX_TWO_KB_OVER_AMU_CC_SQUARED * inv_mass *
-                                                 gld(&m.temperature[o]) +
-                                             gld(&m.vturb2[o]));
-                        const double inverse_width = 1.0 / width;
-                        const double se = inverse_width * (frq - freq_line);
-                        const double ex = se * se;
-                        const double phi = inverse_width * MGX_INVERSE_SQRT_PI *
-                                           (double)f_exp<PREC>((float)(-ex));
-                        Lval = constante * frq * phi * Lc * inverse_chi;
+                        const LamPoint lq = lambda_point<APPROX, PREC>(kp, ol, o, l, f, freq, 2.0 * w);
+                        Lval = lq.constante * lq.frq * lq.phi * Lc * lq.inverse_chi;
                     }
                 }
                 if (u) u[(size_t)rr * PF + of] = uc;
+                }
             }
             /* per-line sum of the Lambda contributions over the 32 lanes of a slot;
                lane 0 then adds both slots of the pass in order (they may share a line) */
@@ -1017,6 +1133,34 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_SOLVE_WAVES_PER_SIMD) void feautrier_
     for (uint32_t i = lane; i < L + (s_J ? F : 0); i += MGX_WAVE) s_L[i] = 0.0;
     if (solve_origin<APPROX, PREC>(kp, o, 0, m.hnrays, smem, s_erf, u, lam != nullptr, s_J,
                                    J + (size_t)o * F, false, s_L))
+        return;
+    if (s_J)
+        for (uint32_t i = lane; i < F; i += MGX_WAVE) J[(size_t)o * F + i] = s_J[i];
+    if (lam)
+        for (uint32_t i = lane; i < L; i += MGX_WAVE) lam[(size_t)o * L + i] = s_L[i];
+}
+
+/* Order-0 short characteristics (Solver::solve_shortchar_order_0, solver.tpp:151-235):
+   the same grid, walk and segments as feautrier_kernel */
+template <int APPROX, int PREC>
+__global__ __launch_bounds__(MGX_WAVE, MGX_SOLVE_WAVES_PER_SIMD) void shortchar_kernel(
+    const KParams *__restrict__ kpp, ShortcharOut sc, double *u, double *J, double *lam)
+{
+    const KParams &kp = *kpp;
+    extern __shared__ unsigned char smem[];
+    __shared__ double s_erf[MGX_ERF_NINT * (MGX_ERF_DEG + 1)];
+    for (uint32_t i = threadIdx.x; i < MGX_ERF_NINT * (MGX_ERF_DEG + 1); i += MGX_WAVE)
+        s_erf[i] = MGX_ERF_TAB[i];
+    const mgx_model &m = kp.m;
+    const uint32_t o = blockIdx.x;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t F = m.nfreqs, L = m.nlines;
+    double *s_L = reinterpret_cast<double *>(smem + ((walk_lds_bytes(kp.cap) + 7) / 8) * 8);
+    double *s_J = kp.j_lds ? s_L + L : nullptr;
+    for (uint32_t i = lane; i < L + (s_J ? F : 0); i += MGX_WAVE) s_L[i] = 0.0;
+    if (solve_origin<APPROX, PREC, METHOD_SHORTCHAR>(kp, o, 0, m.hnrays, smem, s_erf, u,
+                                                     lam != nullptr, s_J, J + (size_t)o * F,
+                                                     false, s_L, sc))
         return;
     if (s_J)
         for (uint32_t i = lane; i < F; i += MGX_WAVE) J[(size_t)o * F + i] = s_J[i];
@@ -3353,6 +3497,53 @@ int mgx_feautrier_forward_device(const mgx_model *model, const mgx_options *opts
                 else
                     hipLaunchKernelGGL((feautrier_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_MIXED>),
                                        grid, block, lds, st, b.kp_dev, u, J, lambda_diag);
+            }
+        });
+    }, 8 * (size_t)model->nlines);
+}
+
+int mgx_shortchar_order_0_device(const mgx_model *model, const mgx_options *opts, double min_dtau,
+                                 double *I, double *u, double *v, double *J, double *lambda_diag,
+                                 void *stream)
+{
+    mgx::clear_error();
+    if (int e = check_view(model, opts)) return e;
+    if (!J) return mgx::fail(MGX_E_ARG, "J is required");
+    if (!(min_dtau > 0.0)) return mgx::fail(MGX_E_ARG, "min_dtau must be positive");
+    hipStream_t st = (hipStream_t)stream;
+    DevBuffers b;
+    KParams kp;
+    if (int e = prepare(model, opts, st, b, kp)) return e;
+    const ShortcharOut sc{I, v, min_dtau};
+    return launch_with_capacity(st, b, kp, [&](size_t) {
+        const dim3 grid(model->npoints), block(MGX_WAVE);
+        const size_t lds = solve_lds_bytes(kp.cap, model->nfreqs, model->nlines, kp.j_lds != 0);
+        timed(st, [&] {
+            const bool fp64 = opts->precision == MGX_PREC_FP64;
+            switch (opts->approx) {
+            case MGX_APPROX_NONE:
+                if (fp64)
+                    hipLaunchKernelGGL((shortchar_kernel<MGX_APPROX_NONE, MGX_PREC_FP64>), grid,
+                                       block, lds, st, b.kp_dev, sc, u, J, lambda_diag);
+                else
+                    hipLaunchKernelGGL((shortchar_kernel<MGX_APPROX_NONE, MGX_PREC_MIXED>), grid,
+                                       block, lds, st, b.kp_dev, sc, u, J, lambda_diag);
+                break;
+            case MGX_APPROX_ONE_LINE:
+                if (fp64)
+                    hipLaunchKernelGGL((shortchar_kernel<MGX_APPROX_ONE_LINE, MGX_PREC_FP64>), grid,
+                                       block, lds, st, b.kp_dev, sc, u, J, lambda_diag);
+                else
+                    hipLaunchKernelGGL((shortchar_kernel<MGX_APPROX_ONE_LINE, MGX_PREC_MIXED>),
+                                       grid, block, lds, st, b.kp_dev, sc, u, J, lambda_diag);
+                break;
+            default:
+                if (fp64)
+                    hipLaunchKernelGGL((shortchar_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_FP64>),
+                                       grid, block, lds, st, b.kp_dev, sc, u, J, lambda_diag);
+                else
+                    hipLaunchKernelGGL((shortchar_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_MIXED>),
+                                       grid, block, lds, st, b.kp_dev, sc, u, J, lambda_diag);
             }
         });
     }, 8 * (size_t)model->nlines);

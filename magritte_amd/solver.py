@@ -54,6 +54,24 @@ def feautrier_forward(dm, opts, with_u=True, with_lambda=True, out=None, view=No
     return out
 
 
+def shortchar_forward(dm, opts, min_dtau=1.0e-15, with_I=True, with_uv=True, with_lambda=True):
+    """Order-0 short characteristics (Model::compute_radiation_field_shortchar_order_0):
+    dict(I=(nrays,P,F), u=(hnrays,P,F), v=(hnrays,P,F), J=(P,F), lambda_diag=(P,L))."""
+    torch = _require_gpu()
+    m = dm.model
+    dev = dm.device
+    f64 = dict(dtype=torch.float64, device=dev)
+    out = {"I": torch.empty((m.nrays, m.npoints, m.nfreqs), **f64) if with_I else None,
+           "u": torch.empty((m.hnrays, m.npoints, m.nfreqs), **f64) if with_uv else None,
+           "v": torch.empty((m.hnrays, m.npoints, m.nfreqs), **f64) if with_uv else None,
+           "J": torch.empty((m.npoints, m.nfreqs), **f64),
+           "lambda_diag": torch.empty((m.npoints, m.nlines), **f64) if with_lambda else None}
+    _abi.check(_abi.lib().mgx_shortchar_order_0_device(
+        C.byref(dm.view), C.byref(opts), float(min_dtau), _abi.ptr(out["I"]), _abi.ptr(out["u"]),
+        _abi.ptr(out["v"]), _abi.ptr(out["J"]), _abi.ptr(out["lambda_diag"]), dm.stream_ptr()))
+    return out
+
+
 def trace_counts(dm, opts):
     torch = _require_gpu()
     m = dm.model

@@ -894,6 +894,144 @@ done:
     return err;
 }
 
+/* ------------------------------------------------------------------------- */
+/* order-0 short characteristics (solver.tpp:151-235, 1565-1757)             */
+
+/* solve_shortchar_order_0_ray_forward (forward = 1) / _backward: the intensity
+   arriving at the centre from the first (last) traced entry, and the
+   direction's Lambda element in *L (*have_L = 0 when the direction has no
+   segment: the reference then reads an uninitialised dtau) */
+static double shortchar_ray(const mgx_model *m, const mgx_options *op, const Scratch *s,
+                            uint32_t o, uint32_t rr, uint32_t f, double min_dtau, int forward,
+                            double *L, int *have_L)
+{
+    const double freq = m->nu[(size_t)o * m->nfreqs + f];
+    const uint32_t l = m->f_line[f];
+    const uint32_t *nr = s->nr, *nri = s->nr_interp;
+    const double *dZ = s->dZ, *shift = s->shift, *ifac = s->interp_factor;
+    double chi_c = 0.0, chi_n = 0.0, source_c = 0.0, source_n = 0.0, dtau = 0.0;
+    int cco = 1, any = 0;
+    const uint32_t bdy = forward ? s->first : s->last;
+    double I = boundary_intensity(m, nr[bdy], freq * shift[bdy]);
+    const long n_seg = forward ? (long)s->centre - (long)s->first : (long)s->last - (long)s->centre;
+    for (long q = 0; q < n_seg; q++) {
+        /* forward: idx = first + 1 + q, (crt, nxt) = (idx - 1, idx);
+           backward: indexp1 = last - q, (crt, nxt) = (indexp1, indexp1 - 1) */
+        const uint32_t c = forward ? s->first + (uint32_t)q : s->last - (uint32_t)q;
+        const uint32_t n = forward ? c + 1 : c - 1;
+        const uint32_t dz = forward ? c : n; /* dZ[idx - 1] / dZ[indexp1 - 1] */
+        chi_c = chi_n;
+        source_c = source_n;
+        compute_source_dtau(m, op, nr[c], nr[n], nri[c], nri[n], l, freq * shift[c],
+                            freq * shift[n], shift[c], shift[n], dZ[dz], ifac[c], ifac[n], &cco,
+                            &dtau, &chi_c, &chi_n, &source_c, &source_n);
+        dtau = (min_dtau < dtau) ? dtau : min_dtau; /* std::max(min_dtau, dtau) */
+        I = (double)O_EXPF((float)(-dtau)) * I +
+            (double)O_EXPM1F((float)(-dtau)) / dtau * (source_c - source_n * (1.0 + dtau)) +
+            source_c - source_n;
+        any = 1;
+    }
+    *have_L = any;
+    if (any) {
+        const double factor = (double)O_EXPM1F((float)(-dtau)) / dtau;
+        const uint32_t spec = m->f_species[f];
+        const uint32_t k = m->f_k[f];
+        const uint32_t z = m->f_z[f];
+        const uint32_t lg = m->f_line[f];
+        const double w_ang = ray_weight(m, o, rr);
+        const double freq_line = m->line[lg];
+        const double invr_mass = m->species_inverse_mass[spec];
+        const double constante =
+            m->line_A[lg] * m->quad_weights[(size_t)spec * m->nquads + z] * w_ang;
+        double eta, chi;
+        get_eta_and_chi(m, op, o, k, freq_line, &eta, &chi);
+        const double inverse_chi = 1.0 / chi;
+        const double phi = thermo_profile(m, invr_mass, o, freq_line, freq);
+        *L = constante * freq * phi * (factor + 1.0) * inverse_chi;
+    }
+    return I;
+}
+
+int oracle_shortchar_order_0(const mgx_model *m, const mgx_options *op, double min_dtau,
+                             double *I, double *u, double *v, double *J, double *lambda_diag,
+                             int nthreads)
+{
+    int err = check_model(m, op);
+    if (err) return err;
+    const int nt = default_threads(nthreads);
+#ifdef _OPENMP
+    const int saved = omp_get_max_threads();
+    omp_set_num_threads(nt);
+#endif
+    const uint32_t length = 2 * get_ray_lengths_max(m, op, &err) + 1;
+    if (err) goto done;
+    const size_t PF = (size_t)m->npoints * m->nfreqs;
+    memset(J, 0, PF * sizeof(double));
+    if (lambda_diag) memset(lambda_diag, 0, (size_t)m->npoints * m->nlines * sizeof(double));
+#pragma omp parallel
+    {
+        Scratch s;
+        int serr = scratch_init(&s, length);
+        for (uint32_t rr = 0; rr < m->hnrays; rr++) {
+            const uint32_t ar = m->antipod[rr];
+#pragma omp for schedule(static)
+            for (uint32_t o = 0; o < m->npoints; o++) {
+                if (serr) continue;
+                const double wt = ray_weight(m, o, rr) * 2.0;
+                s.nr[s.centre] = o;
+                s.shift[s.centre] = 1.0;
+                for (uint32_t l = 0; l < m->nlines; l++) {
+                    int terr = 0;
+                    s.first = trace_ray_for_line(m, op, &s, l, o, rr, -1, s.centre - 1,
+                                                 s.centre - 1, &terr) + 1;
+                    s.last = trace_ray_for_line(m, op, &s, l, o, ar, +1, s.centre + 1, s.centre,
+                                                &terr) - 1;
+                    s.n_tot = (s.last + 1) - s.first;
+                    if (terr) serr = terr;
+                    for (uint32_t q = 0; q < m->nquads; q++) {
+                        const uint32_t f = l * m->nquads + q;
+                        const size_t of = (size_t)o * m->nfreqs + f;
+                        double If, Ib, uu, vv;
+                        if (s.n_tot > 1) {
+                            double Lf = 0.0, Lb = 0.0;
+                            int hf = 0, hb = 0;
+                            If = shortchar_ray(m, op, &s, o, rr, f, min_dtau, 1, &Lf, &hf);
+                            Ib = shortchar_ray(m, op, &s, o, rr, f, min_dtau, 0, &Lb, &hb);
+                            uu = 0.5 * (If + Ib);
+                            vv = 0.5 * (If - Ib);
+                            if (lambda_diag) {
+                                const size_t li = (size_t)o * m->nlines + m->f_line[f];
+                                if (hf) lambda_diag[li] += Lf;
+                                if (hb) lambda_diag[li] += Lb;
+                            }
+                        } else {
+                            If = Ib = uu = boundary_intensity(m, o, m->nu[of]);
+                            vv = 0.0;
+                        }
+                        if (I) {
+                            I[(size_t)rr * PF + of] = If;
+                            I[(size_t)ar * PF + of] = Ib;
+                        }
+                        if (u) u[(size_t)rr * PF + of] = uu;
+                        if (v) v[(size_t)rr * PF + of] = vv;
+                        J[of] += wt * uu;
+                    }
+                }
+            }
+        }
+#pragma omp critical
+        {
+            if (serr && !err) err = serr;
+        }
+        scratch_free(&s);
+    }
+done:
+#ifdef _OPENMP
+    omp_set_num_threads(saved);
+#endif
+    return err;
+}
+
 int oracle_trace_counts(const mgx_model *m, const mgx_options *op, uint32_t *n_tot, int nthreads)
 {
     int err = check_model(m, op);

@@ -15,6 +15,11 @@ extern "C" {
 /* Model::compute_radiation_field_feautrier_order_2 (setup + solve). u may be NULL. */
 int oracle_solve_feautrier_order_2(const mgx_model *m, const mgx_options *op, double *u,
                                    double *J, double *lambda_diag, int nthreads);
+/* Model::compute_radiation_field_shortchar_order_0: I (nrays, P, F), u and v
+   (hnrays, P, F), J (P, F), lambda_diag (P, L); I, u, v, lambda_diag may be NULL. */
+int oracle_shortchar_order_0(const mgx_model *m, const mgx_options *op, double min_dtau,
+                             double *I, double *u, double *v, double *J, double *lambda_diag,
+                             int nthreads);
 /* n_tot(rr, o, l) for every ray pair, origin and line, (hnrays, P, L). */
 int oracle_trace_counts(const mgx_model *m, const mgx_options *op, uint32_t *n_tot, int nthreads);
 /* expanded list of one (rr, o, l); same layout as mgx_trace_rays_device. */

@@ -65,6 +65,24 @@ def solve(model, opts, with_u=True, nthreads=0, kind="cr"):
     return u, J, lam
 
 
+def shortchar(model, opts, min_dtau=1.0e-15, nthreads=0, kind="cr"):
+    """Model::compute_radiation_field_shortchar_order_0: (I (nrays, P, F), u, v
+    (hnrays, P, F), J (P, F), lambda_diag (P, L))."""
+    m = model.view()
+    P, F, L, R = model.npoints, model.nfreqs, model.nlines, model.hnrays
+    I = np.zeros((model.nrays, P, F))
+    u = np.zeros((R, P, F))
+    v = np.zeros((R, P, F))
+    J = np.zeros((P, F))
+    lam = np.zeros((P, L))
+    h = lib(kind)
+    h.oracle_shortchar_order_0.argtypes = [C.c_void_p, C.c_void_p, C.c_double] + [C.c_void_p] * 5 + [C.c_int]
+    _check(h.oracle_shortchar_order_0(C.addressof(m), C.addressof(opts), min_dtau, I.ctypes.data,
+                                      u.ctypes.data, v.ctypes.data, J.ctypes.data, lam.ctypes.data,
+                                      nthreads), "shortchar")
+    return I, u, v, J, lam
+
+
 def trace_counts(model, opts, nthreads=0, kind="cr"):
     m = model.view()
     n = np.zeros((model.hnrays, model.npoints, model.nlines), dtype=np.uint32)

@@ -1,0 +1,88 @@
+"""The oracle's order-0 short-characteristics solver (Solver::solve_shortchar_order_0,
+solver.tpp:151-235, 1565-1757), pinned by:
+
+* the all-constant slab: with constant source function and opacity every step of
+  the order-0 recurrence is exact, I = S + (I_bdy - S) exp(-chi x) from each end
+  (the closed form of the reference's all_constant_single_ray benchmark), to the
+  float exp/expm1 rounding the reference's recurrence carries;
+* u and v of the two directions, J against the weighted u, and u against the
+  second-order Feautrier u where both are accurate (an optically thin slab);
+* the Lambda element per direction, A w_z w nu phi (1 + expm1f(-dtau)/dtau) / chi
+  with the optical depth of the step into the centre, in closed form on the slab.
+"""
+import math
+import os
+
+import numpy as np
+
+from magritte_amd import models
+from oracle import oracle
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _slab(**kw):
+    ld = models.read_lamda(os.path.join(HERE, "golden", "lamda_test.txt"), num=0)
+    return models.model_constant_slab(linedata=ld, **kw), ld
+
+
+def _closed_form(m, ld, T=4.5, n=1.0e3):
+    w = ld["weight"] * np.exp(-ld["energy"] / (models.KB * T))
+    pop = w / w.sum() * n
+    frq = ld["frequency"][0]
+    width = frq * math.sqrt(2.0 * models.KB * T / (models.AMU * models.CC_SQUARED) * ld["inverse_mass"])
+    phi = 1.0 / (width * math.sqrt(math.pi))
+    hv4pi = models.HH * frq / (4.0 * math.pi)
+    eta = hv4pi * ld["A"][0] * pop[1] * phi
+    chi = hv4pi * (pop[0] * ld["Ba"][0] - pop[1] * ld["Bs"][0]) * phi
+    src = eta / chi
+    bdy = 2.0 * models.HH / models.CC_SQUARED * frq**3 / math.expm1(models.HH * frq / (models.KB * models.T_CMB))
+    x = m.position[:, 0]
+    I0 = src + (bdy - src) * np.exp(-chi * x)
+    I1 = src + (bdy - src) * np.exp(-chi * (x[-1] - x))
+    return I0, I1, chi
+
+
+def test_constant_slab_closed_form():
+    m, ld = _slab()
+    I, u, v, J, lam = oracle.shortchar(m, m.options())
+    I0, I1, chi = _closed_form(m, ld)
+    assert 0.1 < chi * m.position[-1, 0] < 1e3
+    # ray 0 points along -x: its traced half ends at x = 0, so I(0) arrives from there
+    e0 = np.max(np.abs(I[0, :, 0] - I0) / I0)
+    e1 = np.max(np.abs(I[1, :, 0] - I1) / I1)
+    assert e0 < 1e-5 and e1 < 1e-5, (e0, e1)
+    assert np.allclose(u[0, :, 0], 0.5 * (I[0, :, 0] + I[1, :, 0]), rtol=1e-15, atol=0)
+    assert np.allclose(v[0, :, 0], 0.5 * (I[0, :, 0] - I[1, :, 0]), rtol=1e-15, atol=1e-300)
+    assert np.allclose(J[:, 0], 2.0 * 0.5 * u[0, :, 0], rtol=1e-15, atol=0)
+
+
+def test_thin_slab_agrees_with_feautrier():
+    # an optically thin slab (low density): both methods converge to the same u
+    m, ld = _slab(n_species=1.0e-3)
+    _, u_sc, _, _, _ = oracle.shortchar(m, m.options())
+    u_f, _, _ = oracle.solve(m, m.options())
+    assert np.max(np.abs(u_sc - u_f) / np.abs(u_f)) < 1e-6
+
+
+def test_lambda_element_closed_form():
+    """Interior points of the constant slab: each direction adds
+    A w_z w nu phi (1 + expm1f(-d)/d) / chi with d = chi dx of the step into the
+    centre (the float expm1 of the reference, pinned exhaustively in test_fmath)."""
+    m, ld = _slab(n_species=1.0e-3)
+    _, _, _, _, lam = oracle.shortchar(m, m.options())
+    T = 4.5
+    w = ld["weight"] * np.exp(-ld["energy"] / (models.KB * T))
+    pop = w / w.sum() * 1.0e-3
+    frq = ld["frequency"][0]
+    iw = 1.0 / (frq * math.sqrt(2.0 * models.KB * T / (models.AMU * models.CC_SQUARED) * ld["inverse_mass"]))
+    phi = iw * 0.5 * (2.0 / math.sqrt(math.pi))
+    hv4pi = models.HH * frq / (4.0 * math.pi)
+    kap = max(hv4pi * (pop[0] * ld["Ba"][0] - pop[1] * ld["Bs"][0]), 1.0e-13)  # min_line_opacity
+    chi = kap * phi + 1.0e-26
+    dx = m.position[1, 0] - m.position[0, 0]
+    d = chi * dx
+    em1 = float(oracle.lib().oracle_expm1f(np.float32(-d)))
+    Ldir = ld["A"][0] * 1.0 * 0.5 * frq * phi * (em1 / d + 1.0) / chi
+    inner = slice(1, m.npoints - 1)
+    assert np.allclose(lam[inner, 0], 2.0 * Ldir, rtol=1e-9, atol=0), (lam[inner, 0][:3], 2 * Ldir)
