@@ -151,6 +151,18 @@ int mgx_shortchar_order_0_device(const mgx_model *model_dev, const mgx_options *
                                  double *lambda_diag, void *stream);
 
 /*
+ * Second-order Feautrier solve with v on the device: u and v (hnrays, P, F),
+ * v = -du/dtau at each point by the second-order difference over its two
+ * optical-depth increments (the boundary difference at a ray's end).
+ * Replaces Model::compute_radiation_field_feautrier_order_2_uv
+ * (src/model/model.cpp:395-433) -> Solver::solve_feautrier_order_2_uv
+ * (src/solver/solver.tpp:238-295, 2767-2921); like the reference it computes
+ * neither J nor Lambda.
+ */
+int mgx_feautrier_order_2_uv_device(const mgx_model *model_dev, const mgx_options *opts,
+                                    double *u, double *v, void *stream);
+
+/*
  * Traced point counts n_tot(rr, o, l) = last + 1 - first for every ray pair,
  * origin and line, written as (hnrays, P, L) uint32.
  * Same trace as Solver::trace_ray_for_line (src/solver/solver.tpp:774-800,

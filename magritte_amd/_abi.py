@@ -126,6 +126,8 @@ SIGNATURES = [
      [C.POINTER(MgxModel), C.POINTER(MgxOptions), _VP, _VP, _VP, _VP]),
     ("mgx_shortchar_order_0_device", C.c_int,
      [C.POINTER(MgxModel), C.POINTER(MgxOptions), C.c_double, _VP, _VP, _VP, _VP, _VP, _VP]),
+    ("mgx_feautrier_order_2_uv_device", C.c_int,
+     [C.POINTER(MgxModel), C.POINTER(MgxOptions), _VP, _VP, _VP]),
     ("mgx_trace_counts_device", C.c_int,
      [C.POINTER(MgxModel), C.POINTER(MgxOptions), _VP, _VP]),
     ("mgx_trace_rays_device", C.c_int,

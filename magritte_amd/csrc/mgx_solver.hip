@@ -601,7 +601,7 @@ struct PointEC {
  * source values of the two rows; uses/updates the caches of the two entries.
  */
 template <int APPROX, int PREC>
-__device__ inline void segment(const KParams &kp, const double *erf_tab, const OwnLine &ol,
+__device__ inline bool segment(const KParams &kp, const double *erf_tab, const OwnLine &ol,
                                const Entry &cu, const Entry &nx, double dZ, double freq,
                                uint32_t line, PointEC &ec_cu, PointEC &ec_nx, double &dtau,
                                double &S_cu, double &S_nx)
@@ -614,6 +614,7 @@ __device__ inline void segment(const KParams &kp, const double *erf_tab, const O
     if (dshift_abs > dshift_max) {
         S_dtau_line_integrated<APPROX, PREC>(kp, erf_tab, ol, cu.nr, nx.nr, cu.nri, nx.nri, line,
                                              cf, nf, cu.ifac, nx.ifac, dZ, dtau, S_cu, S_nx);
+        return false;
     } else {
         if (!ec_cu.valid) {
             eta_chi_interp<APPROX, PREC>(kp, ol, cu.nr, cu.nri, cu.ifac, line, cf, ec_cu.eta,
@@ -628,6 +629,7 @@ __device__ inline void segment(const KParams &kp, const double *erf_tab, const O
         S_cu = ec_cu.eta * rcp(ec_cu.chi);
         S_nx = ec_nx.eta * rcp(ec_nx.chi);
         dtau = 0.5 * (ec_cu.chi + ec_nx.chi) * dZ;
+        return true;
     }
 }
 
@@ -759,6 +761,9 @@ struct Sweep {
     double prev_dtau;  /* depth of the last segment crossed (the innermost one at the end) */
     double S_inner;    /* source of the innermost entry's row from the last segment */
     double S_prev_row; /* source value carried to the next row (backward half) */
+    double chi_outer;  /* opacity at the outer entry of the last segment (when trapezoid) */
+    double dz_last;    /* length of the last segment */
+    bool last_trap;    /* the last segment took the trapezoid branch */
     bool started;
 };
 
@@ -785,7 +790,10 @@ __device__ inline void sweep_half(const KParams &kp, const double *s_erf, const 
         PointEC &ec_cu = backward ? ec_in : ec_out;
         PointEC &ec_nx = backward ? ec_out : ec_in;
         double dtau, S_cu, S_nx;
-        segment<APPROX, PREC>(kp, s_erf, ol, cu, nx, dz, freq, l, ec_cu, ec_nx, dtau, S_cu, S_nx);
+        sw.last_trap =
+            segment<APPROX, PREC>(kp, s_erf, ol, cu, nx, dz, freq, l, ec_cu, ec_nx, dtau, S_cu, S_nx);
+        sw.chi_outer = ec_out.chi;
+        sw.dz_last = dz;
         const double S_out = backward ? S_nx : S_cu; /* row "outer" from this segment */
         const double S_in = backward ? S_cu : S_nx;
         /* row "outer": the boundary row first, then interior rows; the backward
@@ -900,13 +908,13 @@ __device__ inline LamPoint lambda_point(const KParams &kp, const OwnLine &ol, ui
 }
 
 /* which formal solution solve_origin runs */
-enum { METHOD_FEAUTRIER = 0, METHOD_SHORTCHAR = 1 };
+enum { METHOD_FEAUTRIER = 0, METHOD_SHORTCHAR = 1, METHOD_UV = 2 };
 
-/* extra outputs of the short-characteristics solve */
+/* extra outputs of the short-characteristics and u/v solves */
 struct ShortcharOut {
-    double *I;       /* (nrays, P, F) or NULL */
+    double *I;       /* (nrays, P, F) or NULL (short characteristics) */
     double *v;       /* (hnrays, P, F) or NULL */
-    double min_dtau; /* Parameters::min_dtau */
+    double min_dtau; /* Parameters::min_dtau (short characteristics) */
 };
 
 /*
@@ -1024,12 +1032,15 @@ __device__ __forceinline__ int solve_origin(const KParams &kp, uint32_t o, uint3
                     if (sc.v) sc.v[(size_t)rr * PF + of] = vc;
                 } else {
                 double uc;
+                double vc = 0.0; /* v at the centre (METHOD_UV) */
                 if (S0 == 0 && S1 == 0) {
                     uc = boundary_intensity(kp, o, freq);
-                    if (s_J)
-                        s_J[f] = s_J[f] + 2.0 * w * uc;
-                    else
-                        j_add(Jrow, f, 2.0 * w * uc, rr == rr0, j_atomic);
+                    if constexpr (METHOD != METHOD_UV) {
+                        if (s_J)
+                            s_J[f] = s_J[f] + 2.0 * w * uc;
+                        else
+                            j_add(Jrow, f, 2.0 * w * uc, rr == rr0, j_atomic);
+                    }
                 } else {
                     PointEC ec_cent = {0.0, 0.0, false};
                     Sweep bw, fw;
@@ -1069,6 +1080,34 @@ __device__ __forceinline__ int solve_origin(const KParams &kp, uint32_t o, uint3
                         const double rden = rcp(FF + GP);
                         uc = fma(Su, 1.0 + FF, bw.Y) * rden;
                         Lc = iC_c * rden;
+                        if constexpr (METHOD == METHOD_UV) {
+                            if (S0 == 0) {
+                                /* the origin is the first row: v = I_bdy - u there */
+                                vc = boundary_intensity(kp, o, freq) - uc;
+                            } else {
+                                /* solve_feautrier_order_2_uv (solver.tpp:2882-2919): u one row to
+                                   either side (the back substitution u_n = Y_n + XI_n u_c of
+                                   each half) and -du/dtau at the centre by the second-order
+                                   difference over the two optical depths */
+                                const double um = fw.Y + uc * fw.XI;
+                                const double up = bw.Y + uc * bw.XI;
+                                const double dtaumin = fw.prev_dtau;
+                                /* the reference recomputes both depths with one pair of
+                                   opacity variables: when both segments take the trapezoid
+                                   branch, the second reuses the opacity of row c-1 for its
+                                   current point (solver.tpp:2898-2908) */
+                                const double dtauplus =
+                                    (fw.last_trap && bw.last_trap)
+                                        ? 0.5 * (fw.chi_outer + bw.chi_outer) * bw.dz_last
+                                        : dtau_c;
+                                const double coeffmin =
+                                    -dtauplus / (dtaumin * dtaumin + dtaumin * dtauplus);
+                                const double coeffplus =
+                                    dtaumin / (dtaumin * dtauplus + dtauplus * dtauplus);
+                                const double coeffzero = -coeffmin - coeffplus;
+                                vc = -(coeffmin * um + coeffplus * up + coeffzero * uc);
+                            }
+                        }
                     } else { /* the origin is the last row */
                         const double I_l = boundary_intensity(kp, o, freq);
                         const double idt = rcp(fw.prev_dtau);
@@ -1079,18 +1118,22 @@ __device__ __forceinline__ int solve_origin(const KParams &kp, uint32_t o, uint3
                         const double Sl = fma(2.0 * I_l, idt, fw.S_inner);
                         uc = fma(A_l, fw.Y, Sl) * (1.0 + fw.X) * den;
                         Lc = (1.0 + fw.X) * den;
+                        if constexpr (METHOD == METHOD_UV) vc = uc - I_l; /* v = u - I_bdy at the last row */
                     }
-                    if (s_J)
-                        s_J[f] = s_J[f] + uc * 2.0 * w;
-                    else
-                        j_add(Jrow, f, uc * 2.0 * w, rr == rr0, j_atomic);
+                    if constexpr (METHOD != METHOD_UV) {
+                        if (s_J)
+                            s_J[f] = s_J[f] + uc * 2.0 * w;
+                        else
+                            j_add(Jrow, f, uc * 2.0 * w, rr == rr0, j_atomic);
+                    }
                     /* ---- Lambda diagonal (update_Lambda, solver.tpp:1761-1800) ---- */
-                    if (with_lam && m.f_in_line[f]) {
+                    if (METHOD != METHOD_UV && with_lam && m.f_in_line[f]) {
                         const LamPoint lq = lambda_point<APPROX, PREC>(kp, ol, o, l, f, freq, 2.0 * w);
                         Lval = lq.constante * lq.frq * lq.phi * Lc * lq.inverse_chi;
                     }
                 }
                 if (u) u[(size_t)rr * PF + of] = uc;
+                if (METHOD == METHOD_UV && sc.v) sc.v[(size_t)rr * PF + of] = vc;
                 }
             }
             /* per-line sum of the Lambda contributions over the 32 lanes of a slot;
@@ -1166,6 +1209,24 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_SOLVE_WAVES_PER_SIMD) void shortchar_
         for (uint32_t i = lane; i < F; i += MGX_WAVE) J[(size_t)o * F + i] = s_J[i];
     if (lam)
         for (uint32_t i = lane; i < L; i += MGX_WAVE) lam[(size_t)o * L + i] = s_L[i];
+}
+
+/* Feautrier order 2 with v (Solver::solve_feautrier_order_2_uv, solver.tpp:238-295,
+   2767-2921): u and v only */
+template <int APPROX, int PREC>
+__global__ __launch_bounds__(MGX_WAVE, MGX_SOLVE_WAVES_PER_SIMD) void feautrier_uv_kernel(
+    const KParams *__restrict__ kpp, double *u, double *v)
+{
+    const KParams &kp = *kpp;
+    extern __shared__ unsigned char smem[];
+    __shared__ double s_erf[MGX_ERF_NINT * (MGX_ERF_DEG + 1)];
+    for (uint32_t i = threadIdx.x; i < MGX_ERF_NINT * (MGX_ERF_DEG + 1); i += MGX_WAVE)
+        s_erf[i] = MGX_ERF_TAB[i];
+    const uint32_t o = blockIdx.x;
+    double *s_L = reinterpret_cast<double *>(smem + ((walk_lds_bytes(kp.cap) + 7) / 8) * 8);
+    ShortcharOut sc{nullptr, v, 0.0};
+    solve_origin<APPROX, PREC, METHOD_UV>(kp, o, 0, kp.m.hnrays, smem, s_erf, u, false, nullptr,
+                                          nullptr, false, s_L, sc);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -3544,6 +3605,50 @@ int mgx_shortchar_order_0_device(const mgx_model *model, const mgx_options *opts
                 else
                     hipLaunchKernelGGL((shortchar_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_MIXED>),
                                        grid, block, lds, st, b.kp_dev, sc, u, J, lambda_diag);
+            }
+        });
+    }, 8 * (size_t)model->nlines);
+}
+
+int mgx_feautrier_order_2_uv_device(const mgx_model *model, const mgx_options *opts, double *u,
+                                    double *v, void *stream)
+{
+    mgx::clear_error();
+    if (int e = check_view(model, opts)) return e;
+    if (!u || !v) return mgx::fail(MGX_E_ARG, "u and v are required");
+    hipStream_t st = (hipStream_t)stream;
+    DevBuffers b;
+    KParams kp;
+    if (int e = prepare(model, opts, st, b, kp)) return e;
+    return launch_with_capacity(st, b, kp, [&](size_t) {
+        const dim3 grid(model->npoints), block(MGX_WAVE);
+        const size_t lds = solve_lds_bytes(kp.cap, model->nfreqs, model->nlines, false);
+        timed(st, [&] {
+            const bool fp64 = opts->precision == MGX_PREC_FP64;
+            switch (opts->approx) {
+            case MGX_APPROX_NONE:
+                if (fp64)
+                    hipLaunchKernelGGL((feautrier_uv_kernel<MGX_APPROX_NONE, MGX_PREC_FP64>), grid,
+                                       block, lds, st, b.kp_dev, u, v);
+                else
+                    hipLaunchKernelGGL((feautrier_uv_kernel<MGX_APPROX_NONE, MGX_PREC_MIXED>), grid,
+                                       block, lds, st, b.kp_dev, u, v);
+                break;
+            case MGX_APPROX_ONE_LINE:
+                if (fp64)
+                    hipLaunchKernelGGL((feautrier_uv_kernel<MGX_APPROX_ONE_LINE, MGX_PREC_FP64>),
+                                       grid, block, lds, st, b.kp_dev, u, v);
+                else
+                    hipLaunchKernelGGL((feautrier_uv_kernel<MGX_APPROX_ONE_LINE, MGX_PREC_MIXED>),
+                                       grid, block, lds, st, b.kp_dev, u, v);
+                break;
+            default:
+                if (fp64)
+                    hipLaunchKernelGGL((feautrier_uv_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_FP64>),
+                                       grid, block, lds, st, b.kp_dev, u, v);
+                else
+                    hipLaunchKernelGGL((feautrier_uv_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_MIXED>),
+                                       grid, block, lds, st, b.kp_dev, u, v);
             }
         });
     }, 8 * (size_t)model->nlines);

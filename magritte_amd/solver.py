@@ -72,6 +72,18 @@ def shortchar_forward(dm, opts, min_dtau=1.0e-15, with_I=True, with_uv=True, wit
     return out
 
 
+def feautrier_uv(dm, opts):
+    """Model::compute_radiation_field_feautrier_order_2_uv: dict(u, v), each (hnrays, P, F)."""
+    torch = _require_gpu()
+    m = dm.model
+    f64 = dict(dtype=torch.float64, device=dm.device)
+    out = {"u": torch.empty((m.hnrays, m.npoints, m.nfreqs), **f64),
+           "v": torch.empty((m.hnrays, m.npoints, m.nfreqs), **f64)}
+    _abi.check(_abi.lib().mgx_feautrier_order_2_uv_device(
+        C.byref(dm.view), C.byref(opts), _abi.ptr(out["u"]), _abi.ptr(out["v"]), dm.stream_ptr()))
+    return out
+
+
 def trace_counts(dm, opts):
     torch = _require_gpu()
     m = dm.model

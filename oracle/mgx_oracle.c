@@ -895,6 +895,163 @@ done:
 }
 
 /* ------------------------------------------------------------------------- */
+/* Feautrier with v (solver.tpp:238-295, 2767-2921)                          */
+
+/* solve_feautrier_order_2_uv(o, f): the elimination of solve_feautrier_order_2,
+   back substitution down to the centre (and one row past it), and v at the
+   centre from the second-order difference of u over the two neighbouring
+   optical depths (or the boundary difference when the centre is an end) */
+static void solve_feautrier_order_2_uv_of(const mgx_model *m, const mgx_options *op, Scratch *s,
+                                          uint32_t o, uint32_t f, double *u_c, double *v_c,
+                                          double *v_s)
+{
+    const double freq = m->nu[(size_t)o * m->nfreqs + f];
+    const uint32_t l = m->f_line[f];
+    double dtau_c, term_c, dtau_n, term_n, chi_c = 0.0, chi_n = 0.0;
+    const uint32_t first = s->first, last = s->last, centre = s->centre;
+    const uint32_t *nr = s->nr, *nri = s->nr_interp;
+    const double *dZ = s->dZ, *shift = s->shift, *ifac = s->interp_factor;
+    double *Su = s->Su, *A = s->A, *C = s->C, *iA = s->inverse_A, *iC = s->inverse_C;
+    double *FF = s->FF, *FI = s->FI;
+    int cco = 1;
+
+    compute_source_dtau(m, op, nr[first], nr[first + 1], nri[first], nri[first + 1], l,
+                        freq * shift[first], freq * shift[first + 1], shift[first],
+                        shift[first + 1], dZ[first], ifac[first], ifac[first + 1], &cco, &dtau_n,
+                        &chi_c, &chi_n, &term_c, &term_n);
+    const double inverse_dtau_f = 1.0 / dtau_n;
+    C[first] = 2.0 * inverse_dtau_f * inverse_dtau_f;
+    iC[first] = 1.0 / C[first];
+    const double Bf_min_Cf = 1.0 + 2.0 * inverse_dtau_f;
+    const double Bf = Bf_min_Cf + C[first];
+    const double I_bdy_f = boundary_intensity(m, nr[first], freq * shift[first]);
+    Su[first] = term_c + 2.0 * I_bdy_f * inverse_dtau_f;
+    Su[first] /= Bf;
+    FF[first] = 0.5 * Bf_min_Cf * dtau_n * dtau_n;
+    FI[first] = 1.0 / (1.0 + FF[first]);
+    for (uint32_t n = first + 1; n < last; n++) {
+        term_c = term_n;
+        dtau_c = dtau_n;
+        chi_c = chi_n;
+        compute_source_dtau(m, op, nr[n], nr[n + 1], nri[n], nri[n + 1], l, freq * shift[n],
+                            freq * shift[n + 1], shift[n], shift[n + 1], dZ[n], ifac[n],
+                            ifac[n + 1], &cco, &dtau_n, &chi_c, &chi_n, &term_c, &term_n);
+        const double dtau_avg = 0.5 * (dtau_c + dtau_n);
+        iA[n] = dtau_avg * dtau_c;
+        iC[n] = dtau_avg * dtau_n;
+        A[n] = 1.0 / iA[n];
+        C[n] = 1.0 / iC[n];
+        Su[n] = term_c;
+        FF[n] = (A[n] * FF[n - 1] * FI[n - 1] + 1.0) * iC[n];
+        FI[n] = 1.0 / (1.0 + FF[n]);
+        Su[n] = (A[n] * Su[n - 1] + Su[n]) * FI[n] * iC[n];
+    }
+    const double inverse_dtau_l = 1.0 / dtau_n;
+    A[last] = 2.0 * inverse_dtau_l * inverse_dtau_l;
+    const double Bl_min_Al = 1.0 + 2.0 * inverse_dtau_l;
+    const double Bl = Bl_min_Al + A[last];
+    const double denominator = 1.0 / (Bl * FF[last - 1] + Bl_min_Al);
+    const double I_bdy_l = boundary_intensity(m, nr[last], freq * shift[last]);
+    Su[last] = term_n + 2.0 * I_bdy_l * inverse_dtau_l;
+    Su[last] = (A[last] * Su[last - 1] + Su[last]) * (1.0 + FF[last - 1]) * denominator;
+    if (centre < last) {
+        for (long n = (long)last - 1; n >= (long)centre; n--) Su[n] += Su[n + 1] * FI[n];
+    } else {
+        *u_c = Su[centre];
+        *v_c = Su[last] - I_bdy_l;
+        *v_s = fabs(Su[last]) + fabs(I_bdy_l);
+        return;
+    }
+    if (centre > first) {
+        Su[centre - 1] += Su[centre] * FI[centre - 1];
+        cco = 1;
+        compute_source_dtau(m, op, nr[centre - 1], nr[centre], nri[centre - 1], nri[centre], l,
+                            freq * shift[centre - 1], freq * shift[centre], shift[centre - 1],
+                            shift[centre], dZ[centre - 1], ifac[centre - 1], ifac[centre], &cco,
+                            &dtau_n, &chi_c, &chi_n, &term_c, &term_n);
+        const double dtaumin = dtau_n;
+        compute_source_dtau(m, op, nr[centre], nr[centre + 1], nri[centre], nri[centre + 1], l,
+                            freq * shift[centre], freq * shift[centre + 1], shift[centre],
+                            shift[centre + 1], dZ[centre], ifac[centre], ifac[centre + 1], &cco,
+                            &dtau_n, &chi_c, &chi_n, &term_c, &term_n);
+        const double dtauplus = dtau_n;
+        const double coeffmin = -dtauplus / (dtaumin * dtaumin + dtaumin * dtauplus);
+        const double coeffplus = dtaumin / (dtaumin * dtauplus + dtauplus * dtauplus);
+        const double coeffzero = -coeffmin - coeffplus;
+        *u_c = Su[centre];
+        *v_c = -(coeffmin * Su[centre - 1] + coeffplus * Su[centre + 1] + coeffzero * Su[centre]);
+        *v_s = fabs(coeffmin * Su[centre - 1]) + fabs(coeffplus * Su[centre + 1]) +
+               fabs(coeffzero * Su[centre]);
+    } else {
+        *u_c = Su[centre];
+        *v_c = I_bdy_f - Su[first];
+        *v_s = fabs(I_bdy_f) + fabs(Su[first]);
+    }
+}
+
+int oracle_feautrier_order_2_uv(const mgx_model *m, const mgx_options *op, double *u, double *v,
+                                double *v_scale, int nthreads)
+{
+    int err = check_model(m, op);
+    if (err) return err;
+    const int nt = default_threads(nthreads);
+#ifdef _OPENMP
+    const int saved = omp_get_max_threads();
+    omp_set_num_threads(nt);
+#endif
+    const uint32_t length = 2 * get_ray_lengths_max(m, op, &err) + 1;
+    if (err) goto done;
+    const size_t PF = (size_t)m->npoints * m->nfreqs;
+#pragma omp parallel
+    {
+        Scratch s;
+        int serr = scratch_init(&s, length);
+        for (uint32_t rr = 0; rr < m->hnrays; rr++) {
+            const uint32_t ar = m->antipod[rr];
+#pragma omp for schedule(static)
+            for (uint32_t o = 0; o < m->npoints; o++) {
+                if (serr) continue;
+                s.nr[s.centre] = o;
+                s.shift[s.centre] = 1.0;
+                for (uint32_t l = 0; l < m->nlines; l++) {
+                    int terr = 0;
+                    s.first = trace_ray_for_line(m, op, &s, l, o, rr, -1, s.centre - 1,
+                                                 s.centre - 1, &terr) + 1;
+                    s.last = trace_ray_for_line(m, op, &s, l, o, ar, +1, s.centre + 1, s.centre,
+                                                &terr) - 1;
+                    s.n_tot = (s.last + 1) - s.first;
+                    if (terr) serr = terr;
+                    for (uint32_t q = 0; q < m->nquads; q++) {
+                        const uint32_t f = l * m->nquads + q;
+                        const size_t of = (size_t)o * m->nfreqs + f;
+                        double uc, vc, vs = 0.0;
+                        if (s.n_tot > 1) {
+                            solve_feautrier_order_2_uv_of(m, op, &s, o, f, &uc, &vc, &vs);
+                        } else {
+                            uc = boundary_intensity(m, o, m->nu[of]);
+                            vc = 0.0;
+                        }
+                        u[(size_t)rr * PF + of] = uc;
+                        v[(size_t)rr * PF + of] = vc;
+                        if (v_scale) v_scale[(size_t)rr * PF + of] = vs;
+                    }
+                }
+            }
+        }
+#pragma omp critical
+        {
+            if (serr && !err) err = serr;
+        }
+        scratch_free(&s);
+    }
+done:
+#ifdef _OPENMP
+    omp_set_num_threads(saved);
+#endif
+    return err;
+}
+
+/* ------------------------------------------------------------------------- */
 /* order-0 short characteristics (solver.tpp:151-235, 1565-1757)             */
 
 /* solve_shortchar_order_0_ray_forward (forward = 1) / _backward: the intensity

@@ -15,6 +15,12 @@ extern "C" {
 /* Model::compute_radiation_field_feautrier_order_2 (setup + solve). u may be NULL. */
 int oracle_solve_feautrier_order_2(const mgx_model *m, const mgx_options *op, double *u,
                                    double *J, double *lambda_diag, int nthreads);
+/* Model::compute_radiation_field_feautrier_order_2_uv: u and v (hnrays, P, F).
+   v_scale (may be NULL) receives the sum of the absolute terms v is formed from
+   (|coeffmin u_c-1| + |coeffplus u_c+1| + |coeffzero u_c|, or |I_bdy| + |u| at an
+   end): the size the rounding of v is relative to. */
+int oracle_feautrier_order_2_uv(const mgx_model *m, const mgx_options *op, double *u, double *v,
+                                double *v_scale, int nthreads);
 /* Model::compute_radiation_field_shortchar_order_0: I (nrays, P, F), u and v
    (hnrays, P, F), J (P, F), lambda_diag (P, L); I, u, v, lambda_diag may be NULL. */
 int oracle_shortchar_order_0(const mgx_model *m, const mgx_options *op, double min_dtau,

@@ -83,6 +83,24 @@ def shortchar(model, opts, min_dtau=1.0e-15, nthreads=0, kind="cr"):
     return I, u, v, J, lam
 
 
+def feautrier_uv(model, opts, nthreads=0, kind="cr", with_scale=False):
+    """Model::compute_radiation_field_feautrier_order_2_uv: (u, v), each (hnrays, P, F).
+    with_scale=True also returns v_scale, the sum of the absolute terms each v is
+    formed from (v is a difference of neighbouring u's and cancels)."""
+    m = model.view()
+    P, F, R = model.npoints, model.nfreqs, model.hnrays
+    u = np.zeros((R, P, F))
+    v = np.zeros((R, P, F))
+    vs = np.zeros((R, P, F))
+    h = lib(kind)
+    h.oracle_feautrier_order_2_uv.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                              C.c_void_p, C.c_int]
+    _check(h.oracle_feautrier_order_2_uv(C.addressof(m), C.addressof(opts), u.ctypes.data,
+                                         v.ctypes.data, vs.ctypes.data, nthreads),
+           "feautrier_uv")
+    return (u, v, vs) if with_scale else (u, v)
+
+
 def trace_counts(model, opts, nthreads=0, kind="cr"):
     m = model.view()
     n = np.zeros((model.hnrays, model.npoints, model.nlines), dtype=np.uint32)

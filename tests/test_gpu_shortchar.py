@@ -71,3 +71,29 @@ def test_shortchar_rejects_nonpositive_min_dtau(line_model):
 
     with pytest.raises(_abi.MgxError, match="min_dtau"):
         solver.shortchar_forward(solver.DeviceModel(line_model), line_model.options(), min_dtau=0.0)
+
+
+@pytest.mark.parametrize("which,approx", [("line_model", _abi.APPROX_CLOSE_LINES),
+                                          ("small_wind", _abi.APPROX_CLOSE_LINES),
+                                          ("small_wind", _abi.APPROX_ONE_LINE),
+                                          ("vgrad_model", _abi.APPROX_CLOSE_LINES)])
+def test_feautrier_uv_matches_oracle(which, approx, request):
+    """mgx_feautrier_order_2_uv_device against the oracle's solve_feautrier_order_2_uv:
+    u within 1e-10; v within 1e-10 of itself plus 1e-12 of the sum of the absolute
+    terms it is formed from (coeffmin u_c-1, coeffplus u_c+1, coeffzero u_c).  v is
+    a second-order difference of u and cancels: in a uniform stretch of the 1D chain
+    the terms exceed |v| by 1e12 and more, so v there is rounding noise in the
+    reference as well (the two-sided elimination reaches u one row either side of
+    the centre through other roundings than the reference's back substitution).
+    On the wind, the depth after the centre reuses the opacity of the row before
+    it when both segments take the trapezoid branch, as the reference's does."""
+    from magritte_amd import solver
+    from oracle import oracle
+
+    m = request.getfixturevalue(which)
+    op = m.options(approx=approx)
+    print(f"[uv] model={which} approx={approx} P={m.npoints}", flush=True)
+    out = solver.feautrier_uv(solver.DeviceModel(m), op)
+    u, v, v_scale = oracle.feautrier_uv(m, op, nthreads=16, with_scale=True)
+    _close(out["u"].cpu().numpy(), u, 1e-10, "u")
+    _close(out["v"].cpu().numpy(), v, 1e-10, "v", floor=1e-12 * v_scale)

@@ -86,3 +86,23 @@ def test_lambda_element_closed_form():
     Ldir = ld["A"][0] * 1.0 * 0.5 * frq * phi * (em1 / d + 1.0) / chi
     inner = slice(1, m.npoints - 1)
     assert np.allclose(lam[inner, 0], 2.0 * Ldir, rtol=1e-9, atol=0), (lam[inner, 0][:3], 2 * Ldir)
+
+
+def test_feautrier_uv_on_the_slab():
+    """solve_feautrier_order_2_uv (solver.tpp:2767-2921): u is the plain method's u to
+    the bit, and v, the second-order difference -du/dtau at the centre, is the
+    flux-like (I_f - I_b)/2 of the closed form to the discretisation error of the
+    slab's steps (the reference's own Feautrier bound here is 1.7e-4)."""
+    m, ld = _slab()
+    u, v = oracle.feautrier_uv(m, m.options())
+    uf, _, _ = oracle.solve(m, m.options())
+    assert np.array_equal(u, uf)
+    I0, I1, _ = _closed_form(m, ld)
+    err = np.abs(v[0, :, 0] - 0.5 * (I0 - I1)) / (0.5 * (I0 + I1))
+    assert err.max() < 5e-4, err.max()
+    # the end points take the boundary difference: v = I_bdy - u at the rr-side end
+    # (I_bdy with the float expm1 of the reference's Planck function: 1e-7)
+    bdy = 2.0 * models.HH / models.CC_SQUARED * ld["frequency"][0] ** 3 / math.expm1(
+        models.HH * ld["frequency"][0] / (models.KB * models.T_CMB))
+    assert abs(v[0, 0, 0] - (bdy - u[0, 0, 0])) <= 1e-7 * abs(u[0, 0, 0])
+    assert abs(v[0, -1, 0] - (u[0, -1, 0] - bdy)) <= 1e-7 * abs(u[0, -1, 0])
