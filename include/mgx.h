@@ -202,6 +202,48 @@ int mgx_trace_rays_device(const mgx_model *model_dev, const mgx_options *opts, u
 int mgx_image_device(const mgx_model *model_dev, const mgx_options *opts, const double *raydir,
                      uint32_t nx, uint32_t ny, double *I, double *ImX, double *ImY, void *stream);
 
+/*
+ * Optical-depth image on the projection surface: the same pixel rays as
+ * mgx_image_device, tau(pix, f) = the sum of the optical-depth increments
+ * along the ray at the start point's frequencies (0 for a ray of one point).
+ * Replaces Model::compute_image_optical_depth_new(raydir, Nx, Ny)
+ * (src/model/model.cpp:1102-1127) -> Solver::image_optical_depth_new_imager
+ * (src/solver/solver.tpp:687-736) -> image_optical_depth(o, f) (:2718-2758).
+ * Outputs and errors as mgx_image_device.
+ */
+int mgx_image_optical_depth_device(const mgx_model *model_dev, const mgx_options *opts,
+                                   const double *raydir, uint32_t nx, uint32_t ny, double *tau,
+                                   double *ImX, double *ImY, void *stream);
+
+/*
+ * The old imager: an image at every model point along model ray rr
+ * (0 <= rr < nrays).  For each point the ray pair (rr, antipod) is traced with
+ * the Rest-frame shifts (2 - shift on rays >= hnrays) and the all-lines
+ * interpolation count, the Feautrier system is eliminated from the rr end to
+ * the antipod end, and I(p, f) = 2 Su[last] - I_bdy(nr[last], nu(p, f)), the
+ * boundary term at the unshifted frequency (I_bdy(p, nu) for a ray of one
+ * point).  Outputs (device): I (P, F), ImX, ImY (P), the points' coordinates
+ * on the plane normal to the ray.  Replaces Model::compute_image(ray_nr)
+ * (src/model/model.cpp:957-976) -> Solver::image_feautrier_order_2(model, rr)
+ * (src/solver/solver.tpp:543-574, 742-767, 2463-2560) and
+ * Image::set_coordinates_all_model_points (src/model/image/image.cpp:154-212).
+ * Errors: OneLine -> MGX_E_UNSUPPORTED; adaptive directions ->
+ * MGX_E_UNSUPPORTED; 1D model with ray rr != (0,1,0) -> MGX_E_ARG.
+ */
+int mgx_image_ray_device(const mgx_model *model_dev, const mgx_options *opts, uint32_t rr,
+                         double *I, double *ImX, double *ImY, void *stream);
+
+/*
+ * Optical-depth image of the old imager: tau(p, f), the optical depth of the
+ * whole ray pair through p along ray rr (0 for a ray of one point).  Replaces
+ * Model::compute_image_optical_depth(ray_nr) (src/model/model.cpp:1061-1076)
+ * -> Solver::image_optical_depth(model, rr) (src/solver/solver.tpp:656-686,
+ * 2718-2758).  Outputs and errors as mgx_image_ray_device.
+ */
+int mgx_image_optical_depth_ray_device(const mgx_model *model_dev, const mgx_options *opts,
+                                       uint32_t rr, double *tau, double *ImX, double *ImY,
+                                       void *stream);
+
 /* ------------------------------------------------------------------------ */
 /* Adjoint (reverse mode).  No counterpart in the reference: the gradients   */
 /* of the formal solve with respect to the line emissivity and opacity       */

@@ -136,6 +136,13 @@ SIGNATURES = [
     ("mgx_image_device", C.c_int,
      [C.POINTER(MgxModel), C.POINTER(MgxOptions), _VP, C.c_uint32, C.c_uint32, _VP, _VP, _VP,
       _VP]),
+    ("mgx_image_optical_depth_device", C.c_int,
+     [C.POINTER(MgxModel), C.POINTER(MgxOptions), _VP, C.c_uint32, C.c_uint32, _VP, _VP, _VP,
+      _VP]),
+    ("mgx_image_ray_device", C.c_int,
+     [C.POINTER(MgxModel), C.POINTER(MgxOptions), C.c_uint32, _VP, _VP, _VP, _VP]),
+    ("mgx_image_optical_depth_ray_device", C.c_int,
+     [C.POINTER(MgxModel), C.POINTER(MgxOptions), C.c_uint32, _VP, _VP, _VP, _VP]),
     ("mgx_feautrier_adjoint_device", C.c_int,
      [C.POINTER(MgxModel), C.POINTER(MgxOptions), _VP, _VP, _VP, _VP]),
     ("mgx_image_adjoint_device", C.c_int,

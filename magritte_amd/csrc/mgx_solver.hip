@@ -165,6 +165,28 @@ __device__ inline double shift_comoving(const KParams &kp, const double *origin,
     return 1.0 + dot3(dv, dir);
 }
 
+/* Geometry::get_shift<Rest, false>(o, r, c, Z) (geometry.tpp:213-221, 405-415,
+   429-465): the shift the old imager walks with, reversed (2 - shift) on the
+   second half of the ray set */
+__device__ inline double shift_rest_ray(const KParams &kp, const double *origin, const double *dir,
+                                        uint32_t c, double Z, bool reverse)
+{
+    const PointRec pc = gld(&kp.pts[c]);
+    double sh;
+    if (kp.m.spherical_symmetry) {
+        if (pc.x == 0.0) {
+            sh = 1.0;
+        } else {
+            const double Rcos_plus_Z = dot3(origin, dir) + Z;
+            sh = 1.0 + pc.vx * Rcos_plus_Z / pc.x;
+        }
+    } else {
+        const double v[3] = {pc.vx, pc.vy, pc.vz};
+        sh = 1.0 + dot3(v, dir);
+    }
+    return reverse ? 2.0 - sh : sh;
+}
+
 /* ------------------------------------------------------------------------- */
 /* next point on the ray, wave-cooperative                                    */
 
@@ -261,7 +283,9 @@ __device__ inline bool on_boundary(const KParams &kp, uint32_t p)
  * solver.tpp:774-800) into LDS: step s goes from crt_s (o for s = 0) to
  * s_nxt[s] over distance s_dZ[s]; s_shift[s] is the shift of s_nxt[s].
  * Returns 0 or an ERR_* code; nsteps and shift0 (shift of o) are set.
+ * REST selects the old imager's trace_ray<Rest, false> shifts (:742-767).
  */
+template <bool REST = false>
 __device__ int walk_half(const KParams &kp, uint32_t o, uint32_t r, uint32_t *s_nxt, double *s_dZ,
                          double *s_shift, uint32_t &nsteps, double &shift0)
 {
@@ -275,7 +299,9 @@ __device__ int walk_half(const KParams &kp, uint32_t o, uint32_t r, uint32_t *s_
     double Z = 0.0, dZ = 0.0;
     uint32_t crt = o;
     nsteps = 0;
-    shift0 = shift_comoving(kp, origin, ovel, dir, o, 0.0);
+    const bool reverse = r >= m.hnrays;
+    shift0 = REST ? shift_rest_ray(kp, origin, dir, o, 0.0, reverse)
+                  : shift_comoving(kp, origin, ovel, dir, o, 0.0);
     for (;;) {
         const uint32_t nxt = m.spherical_symmetry ? next_spherical(kp, origin, dir, crt, Z, dZ)
                                                   : next_general(kp, origin, dir, crt, Z, dZ);
@@ -284,7 +310,8 @@ __device__ int walk_half(const KParams &kp, uint32_t o, uint32_t r, uint32_t *s_
             return ERR_NO_NEXT;
         }
         if (nsteps >= kp.cap) return ERR_CAP;
-        const double sh = shift_comoving(kp, origin, ovel, dir, nxt, Z);
+        const double sh = REST ? shift_rest_ray(kp, origin, dir, nxt, Z, reverse)
+                               : shift_comoving(kp, origin, ovel, dir, nxt, Z);
         if (lane == 0) {
             s_nxt[nsteps] = nxt;
             s_dZ[nsteps] = dZ;
@@ -1665,6 +1692,53 @@ __device__ inline double shift_rest(const mgx_model &m, const double *origin, co
     return 1.0 + dot3(&m.velocity[3 * (size_t)c], rd);
 }
 
+/* image kinds: the intensity (2 Su[last] - I_bdy) or the optical depth along the ray */
+enum { IMG_INTENSITY = 0, IMG_OPTICAL_DEPTH = 1 };
+
+/* Forward elimination of a whole ray, one row at a time
+   (image_feautrier_order_2, solver.tpp:2463-2560).  fe_row takes each row from
+   the first on, with the depth of the segment leaving it and its source there;
+   fe_last closes the last row and returns Su[last]. */
+struct FwdElim {
+    double FF, FI, Su, prev_dtau;
+    bool started;
+};
+
+__device__ inline void fe_row(const KParams &kp, FwdElim &fe, double dtau, double S_row,
+                              const Entry &row, double freq)
+{
+    if (!fe.started) {
+        const double I_f = boundary_intensity(kp, row.nr, freq * row.shift);
+        const double idt = rcp(dtau);
+        const double Bf_min_Cf = fma(2.0, idt, 1.0);
+        const double Bf = fma(2.0 * idt, idt, Bf_min_Cf);
+        fe.Su = fma(2.0 * I_f, idt, S_row) * rcp(Bf);
+        fe.FF = 0.5 * Bf_min_Cf * dtau * dtau;
+        fe.FI = rcp(1.0 + fe.FF);
+        fe.started = true;
+    } else {
+        const double dtau_avg = 0.5 * (fe.prev_dtau + dtau);
+        const double iA = dtau_avg * fe.prev_dtau;
+        const double iC = dtau_avg * dtau;
+        const double A = rcp(iA);
+        fe.FF = fma(A * fe.FF, fe.FI, 1.0) * iC;
+        fe.FI = rcp(1.0 + fe.FF);
+        fe.Su = fma(A, fe.Su, S_row) * fe.FI * iC;
+    }
+    fe.prev_dtau = dtau;
+}
+
+__device__ inline double fe_last(const FwdElim &fe, double S_last, double I_l)
+{
+    const double idt = rcp(fe.prev_dtau);
+    const double A_l = 2.0 * idt * idt;
+    const double Bl_min_Al = fma(2.0, idt, 1.0);
+    const double Bl = Bl_min_Al + A_l;
+    const double denominator = rcp(fma(Bl, fe.FF, Bl_min_Al));
+    const double Sl = fma(2.0 * I_l, idt, S_last);
+    return fma(A_l, fe.Su, Sl) * (1.0 + fe.FF) * denominator;
+}
+
 /* all-lines interpolation count (InterpHelper::get_n_interp, :10-46) */
 __device__ inline uint32_t n_interp_all(const KParams &kp, uint32_t c, uint32_t n)
 {
@@ -1679,7 +1753,7 @@ __device__ inline uint32_t n_interp_all(const KParams &kp, uint32_t c, uint32_t 
     return (k > 1) ? k : 1;
 }
 
-template <int APPROX, int PREC>
+template <int APPROX, int PREC, int KIND>
 __global__ __launch_bounds__(MGX_WAVE) void image_kernel(const KParams *__restrict__ kpp, const ImageFrame *frp,
                                                          double *I, double *ImX, double *ImY)
 {
@@ -1795,7 +1869,9 @@ __global__ __launch_bounds__(MGX_WAVE) void image_kernel(const KParams *__restri
         const double nu_ip = m.nu[(size_t)ip * F + f];
         double Ival;
         if (S == 0) {
-            Ival = boundary_intensity(kp, ip, nu_ip);
+            /* n_tot = 1: the boundary intensity, or no optical depth
+               (image_optical_depth_new_imager, solver.tpp:703-729) */
+            Ival = (KIND == IMG_OPTICAL_DEPTH) ? 0.0 : boundary_intensity(kp, ip, nu_ip);
         } else {
             const double freq = nu_ip;
             const uint32_t line = m.f_line[f];
@@ -1807,8 +1883,9 @@ __global__ __launch_bounds__(MGX_WAVE) void image_kernel(const KParams *__restri
             hi_load_step(kp, it, line, ip, shift0, s_nxt, s_dZ, s_sh, s_m);
             Entry outer = hi_entry(it, ip);
             PointEC ec_out = {0.0, 0.0, false};
-            double FF = 0.0, FI = 0.0, Su = 0.0, prev_dtau = 0.0, prev_Snext = 0.0;
-            bool is_first = true;
+            FwdElim fe;
+            fe.started = false;
+            double prev_Snext = 0.0, tau = 0.0;
             for (;;) {
                 const double dz = hi_advance(kp, it, line, ip, shift0, s_nxt, s_dZ, s_sh, s_m);
                 Entry inner = hi_entry(it, ip);
@@ -1817,42 +1894,186 @@ __global__ __launch_bounds__(MGX_WAVE) void image_kernel(const KParams *__restri
                 double dtau, S_out, S_in;
                 segment<APPROX, PREC>(kp, s_erf, ol, outer, inner, dz, freq, line, ec_out, ec_in,
                                       dtau, S_out, S_in);
-                if (is_first) {
-                    const double I_f = boundary_intensity(kp, outer.nr, freq * outer.shift);
-                    const double idt = rcp(dtau);
-                    const double Bf_min_Cf = fma(2.0, idt, 1.0);
-                    const double Bf = fma(2.0 * idt, idt, Bf_min_Cf);
-                    Su = fma(2.0 * I_f, idt, S_out) * rcp(Bf);
-                    FF = 0.5 * Bf_min_Cf * dtau * dtau;
-                    FI = rcp(1.0 + FF);
-                    is_first = false;
-                } else {
-                    const double dtau_avg = 0.5 * (prev_dtau + dtau);
-                    const double iA = dtau_avg * prev_dtau;
-                    const double iC = dtau_avg * dtau;
-                    const double A = rcp(iA);
-                    FF = fma(A * FF, FI, 1.0) * iC;
-                    FI = rcp(1.0 + FF);
-                    Su = fma(A, Su, S_out) * FI * iC;
-                }
-                prev_dtau = dtau;
+                if constexpr (KIND == IMG_OPTICAL_DEPTH)
+                    tau += dtau; /* image_optical_depth (solver.tpp:2718-2758) */
+                else
+                    fe_row(kp, fe, dtau, S_out, outer, freq);
                 prev_Snext = S_in;
                 outer = inner;
                 ec_out = ec_in;
                 if (it.s < 0) break;
             }
-            /* last row at the start point (image_feautrier_order_2, solver.tpp:2546-2559) */
-            const double I_l = boundary_intensity(kp, ip, freq * shift_centre);
-            const double idt = rcp(prev_dtau);
-            const double A_l = 2.0 * idt * idt;
-            const double Bl_min_Al = fma(2.0, idt, 1.0);
-            const double Bl = Bl_min_Al + A_l;
-            const double denominator = rcp(fma(Bl, FF, Bl_min_Al));
-            const double Sl = fma(2.0 * I_l, idt, prev_Snext);
-            const double Su_last = fma(A_l, Su, Sl) * (1.0 + FF) * denominator;
-            Ival = 2.0 * Su_last - boundary_intensity(kp, ip, nu_ip);
+            if constexpr (KIND == IMG_OPTICAL_DEPTH) {
+                Ival = tau;
+            } else {
+                /* last row at the start point (image_feautrier_order_2, solver.tpp:2546-2559) */
+                const double I_l = boundary_intensity(kp, ip, freq * shift_centre);
+                Ival = 2.0 * fe_last(fe, prev_Snext, I_l) - boundary_intensity(kp, ip, nu_ip);
+            }
         }
         I[(size_t)pix * F + f] = Ival;
+    }
+}
+
+/*
+ * The old imager at every model point along ray rr (Solver::image_feautrier_order_2
+ * and Solver::image_optical_depth, solver.tpp:543-574, 656-686).  One wave per
+ * point o: the ray pair (rr, antipod) is walked with the Rest-frame shifts of
+ * trace_ray<Rest, false> (reversed on rays >= hnrays) and the all-lines
+ * interpolation count of every step; lanes take frequencies and run over the
+ * whole ray from the rr end (first) to the antipod end (last): the rr half
+ * inward, the origin, then the antipod half outward.  KIND selects
+ * I(o, f) = 2 Su[last] - B(nr[last], nu(o, f)) (the boundary term at the
+ * unshifted frequency, as the reference) or the summed optical depth.  ImX, ImY:
+ * the point's coordinates on the image plane
+ * (Image::set_coordinates_all_model_points, image.cpp:154-212).
+ */
+template <int APPROX, int PREC, int KIND>
+__global__ __launch_bounds__(MGX_WAVE) void image_ray_kernel(const KParams *__restrict__ kpp,
+                                                             uint32_t rr, double *I, double *ImX,
+                                                             double *ImY)
+{
+    const KParams &kp = *kpp;
+    extern __shared__ unsigned char smem[];
+    __shared__ double s_erf[MGX_ERF_NINT * (MGX_ERF_DEG + 1)];
+    for (uint32_t i = threadIdx.x; i < MGX_ERF_NINT * (MGX_ERF_DEG + 1); i += MGX_WAVE)
+        s_erf[i] = MGX_ERF_TAB[i];
+    const mgx_model &m = kp.m;
+    const uint32_t o = blockIdx.x;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t cap = kp.cap;
+    const uint32_t F = m.nfreqs;
+    uint32_t *s_nxt0 = reinterpret_cast<uint32_t *>(smem);
+    uint32_t *s_nxt1 = s_nxt0 + cap;
+    double *s_dZ0 = reinterpret_cast<double *>(smem + 8 * ((2 * cap * 4 + 7) / 8));
+    double *s_dZ1 = s_dZ0 + cap;
+    double *s_sh0 = s_dZ1 + cap;
+    double *s_sh1 = s_sh0 + cap;
+    uint16_t *s_m0 = reinterpret_cast<uint16_t *>(s_sh1 + cap);
+    uint16_t *s_m1 = s_m0 + cap;
+
+    if (lane == 0) {
+        const PointRec po = gld(&kp.pts[o]);
+        double X = po.x, Y = 0.0;
+        if (m.dimension == 3) {
+            const double rx = m.ray_direction[3 * (size_t)rr + 0];
+            const double ry = m.ray_direction[3 * (size_t)rr + 1];
+            const double rz = m.ray_direction[3 * (size_t)rr + 2];
+            const double denominator = sqrt(rx * rx + ry * ry);
+            if (denominator >= 1.0e-9) {
+                const double inverse_denominator = 1.0 / denominator;
+                const double ix = ry * inverse_denominator, iy = -rx * inverse_denominator;
+                const double jx = rx * rz * inverse_denominator, jy = ry * rz * inverse_denominator;
+                const double jz = -denominator;
+                X = ix * po.x + iy * po.y;
+                Y = jx * po.x + jy * po.y + jz * po.z;
+            } else {
+                Y = po.y;
+            }
+        } else if (m.dimension != 1) {
+            X = 0.0;
+        }
+        ImX[o] = X;
+        ImY[o] = Y;
+    }
+
+    const uint32_t ar = gld(&m.antipod[rr]);
+    uint32_t S0, S1;
+    double sh00, sh01;
+    int e = walk_half<true>(kp, o, rr, s_nxt0, s_dZ0, s_sh0, S0, sh00);
+    if (!e) e = walk_half<true>(kp, o, ar, s_nxt1, s_dZ1, s_sh1, S1, sh01);
+    if (e) {
+        if (lane == 0) {
+            atomicOr(&kp.status[0], (uint32_t)e);
+            if (e == ERR_CAP) atomicMax(&kp.status[1], kp.cap + 1);
+        }
+        return;
+    }
+    __syncthreads();
+    for (uint32_t i = lane; i < S0 + S1; i += MGX_WAVE) {
+        const uint32_t h = (i < S0) ? 0 : 1;
+        const uint32_t st = h ? i - S0 : i;
+        const uint32_t *nx = h ? s_nxt1 : s_nxt0;
+        const uint32_t crt = (st == 0) ? o : nx[st - 1];
+        const uint32_t mm = n_interp_all(kp, crt, nx[st]);
+        if (mm > 0xffffu) atomicOr(&kp.status[0], (uint32_t)ERR_CAP);
+        (h ? s_m1 : s_m0)[st] = (uint16_t)(mm > 0xffffu ? 0xffffu : mm);
+    }
+    __syncthreads();
+
+    const uint32_t last_nr = (S1 > 0) ? s_nxt1[S1 - 1] : o;
+    for (uint32_t f = lane; f < F; f += MGX_WAVE) {
+        const double freq = m.nu[(size_t)o * F + f];
+        double val;
+        if (S0 + S1 == 0) {
+            /* n_tot = 1 */
+            val = (KIND == IMG_OPTICAL_DEPTH) ? 0.0 : boundary_intensity(kp, o, freq);
+        } else {
+            const uint32_t line = m.f_line[f];
+            const uint32_t ub_idx = upper_bound_d(m.sorted_line, m.nlines, freq);
+            const OwnLine ol = own_line(m, m.sorted_line_map[ub_idx > 0 ? ub_idx - 1 : 0]);
+            FwdElim fe;
+            fe.started = false;
+            double tau = 0.0, S_last = 0.0;
+            Entry prev;
+            PointEC ec_prev = {0.0, 0.0, false};
+            /* one segment from prev to nx: row prev of the elimination, or its depth */
+            auto cross = [&](const Entry &nx, double dz) {
+                PointEC ec_nx = {0.0, 0.0, false};
+                double dtau, S_cu, S_nx;
+                segment<APPROX, PREC>(kp, s_erf, ol, prev, nx, dz, freq, line, ec_prev, ec_nx,
+                                      dtau, S_cu, S_nx);
+                if constexpr (KIND == IMG_OPTICAL_DEPTH)
+                    tau += dtau;
+                else
+                    fe_row(kp, fe, dtau, S_cu, prev, freq);
+                S_last = S_nx;
+                prev = nx;
+                ec_prev = ec_nx;
+            };
+            if (S0 > 0) { /* the rr half from its far end inward to the origin */
+                HalfIter it;
+                it.s = (int)S0 - 1;
+                hi_load_step(kp, it, line, o, sh00, s_nxt0, s_dZ0, s_sh0, s_m0);
+                prev = hi_entry(it, o);
+                for (;;) {
+                    const double dz = hi_advance(kp, it, line, o, sh00, s_nxt0, s_dZ0, s_sh0, s_m0);
+                    Entry nx = hi_entry(it, o);
+                    if (it.s < 0) nx.shift = sh00;
+                    cross(nx, dz);
+                    if (it.s < 0) break;
+                }
+            } else {
+                prev.nr = o;
+                prev.nri = o;
+                prev.ifac = 0.0;
+                prev.shift = sh00;
+            }
+            if (S1 > 0) { /* the antipod half from the origin outward */
+                HalfIter it;
+                it.s = 0;
+                hi_load_step(kp, it, line, o, sh01, s_nxt1, s_dZ1, s_sh1, s_m1);
+                it.i = 1;
+                for (;;) {
+                    cross(hi_entry(it, o), it.dZ);
+                    if (it.i < it.m_s) {
+                        it.i++;
+                    } else {
+                        it.s++;
+                        if (it.s >= (int)S1) break;
+                        hi_load_step(kp, it, line, o, sh01, s_nxt1, s_dZ1, s_sh1, s_m1);
+                        it.i = 1;
+                    }
+                }
+            }
+            if constexpr (KIND == IMG_OPTICAL_DEPTH) {
+                val = tau;
+            } else {
+                const double I_l = boundary_intensity(kp, last_nr, freq * prev.shift);
+                val = 2.0 * fe_last(fe, S_last, I_l) - boundary_intensity(kp, last_nr, freq);
+            }
+        }
+        I[(size_t)o * F + f] = val;
     }
 }
 
@@ -3513,6 +3734,67 @@ int batch_execute(mgx_batch &B, hipStream_t st)
 
 } // namespace
 
+namespace {
+
+/* argument checks of the old imager (Model::compute_image / compute_image_optical_depth,
+   model.cpp:957-976, 1061-1076; the 1D rule of the Image constructor, image.cpp:12-26) and
+   the launch of image_ray_kernel */
+template <int KIND>
+int run_image_ray(const mgx_model *model, const mgx_options *opts, uint32_t rr, double *out,
+                  double *ImX, double *ImY, hipStream_t st)
+{
+    if (!model || !opts || !out || !ImX || !ImY) return mgx::fail(MGX_E_ARG, "null argument");
+    if (opts->approx == MGX_APPROX_ONE_LINE)
+        return mgx::fail(MGX_E_UNSUPPORTED, "One line approximation is not supported for imaging.");
+    if (opts->approx < 0 || opts->approx > 2) return mgx::fail(MGX_E_ARG, "unknown approximation");
+    if (opts->precision != MGX_PREC_FP64 && opts->precision != MGX_PREC_MIXED)
+        return mgx::fail(MGX_E_ARG, "unknown precision mode");
+    if (!model->position || !model->velocity || !model->nu || !model->ray_direction ||
+        !model->antipod || !model->point2boundary || !model->interp_crit || !model->sorted_line ||
+        !model->emissivity)
+        return mgx::fail(MGX_E_ARG, "null array in model view");
+    if (model->npoints == 0 || model->nfreqs == 0 || model->nlines == 0)
+        return mgx::fail(MGX_E_ARG, "empty model");
+    if (rr >= model->nrays) return mgx::fail(MGX_E_ARG, "ray %u out of range", rr);
+    if (model->adaptive_directions)
+        return mgx::fail(MGX_E_UNSUPPORTED,
+                         "the imager walks the model's global ray directions (no adaptive directions)");
+    if (model->dimension == 1) {
+        double d[3];
+        HIP_TRY(hipMemcpyAsync(d, model->ray_direction + 3 * (size_t)rr, sizeof d,
+                               hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        if (d[0] != 0.0 || d[1] != 1.0 || d[2] != 0.0)
+            return mgx::fail(MGX_E_ARG, "In 1D, the image ray has to be (0,1,0)");
+    }
+    DevBuffers b;
+    KParams kp;
+    if (int e = prepare(model, opts, st, b, kp)) return e;
+    return launch_with_capacity(st, b, kp, [&](size_t lds) {
+        const dim3 grid(model->npoints), block(MGX_WAVE);
+        timed(st, [&] {
+            const bool fp64 = opts->precision == MGX_PREC_FP64;
+            if (opts->approx == MGX_APPROX_NONE) {
+                if (fp64)
+                    hipLaunchKernelGGL((image_ray_kernel<MGX_APPROX_NONE, MGX_PREC_FP64, KIND>), grid,
+                                       block, lds, st, b.kp_dev, rr, out, ImX, ImY);
+                else
+                    hipLaunchKernelGGL((image_ray_kernel<MGX_APPROX_NONE, MGX_PREC_MIXED, KIND>), grid,
+                                       block, lds, st, b.kp_dev, rr, out, ImX, ImY);
+            } else {
+                if (fp64)
+                    hipLaunchKernelGGL((image_ray_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_FP64, KIND>),
+                                       grid, block, lds, st, b.kp_dev, rr, out, ImX, ImY);
+                else
+                    hipLaunchKernelGGL((image_ray_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_MIXED, KIND>),
+                                       grid, block, lds, st, b.kp_dev, rr, out, ImX, ImY);
+            }
+        });
+    });
+}
+
+} // namespace
+
 extern "C" {
 
 const char *mgx_last_error(void) { return mgx::error_buffer(); }
@@ -3675,21 +3957,76 @@ int mgx_image_device(const mgx_model *model, const mgx_options *opts, const doub
             const bool fp64 = opts->precision == MGX_PREC_FP64;
             if (opts->approx == MGX_APPROX_NONE) {
                 if (fp64)
-                    hipLaunchKernelGGL((image_kernel<MGX_APPROX_NONE, MGX_PREC_FP64>), grid, block,
+                    hipLaunchKernelGGL((image_kernel<MGX_APPROX_NONE, MGX_PREC_FP64, IMG_INTENSITY>), grid, block,
                                        lds, st, b.kp_dev, dfr, I, ImX, ImY);
                 else
-                    hipLaunchKernelGGL((image_kernel<MGX_APPROX_NONE, MGX_PREC_MIXED>), grid, block,
+                    hipLaunchKernelGGL((image_kernel<MGX_APPROX_NONE, MGX_PREC_MIXED, IMG_INTENSITY>), grid, block,
                                        lds, st, b.kp_dev, dfr, I, ImX, ImY);
             } else {
                 if (fp64)
-                    hipLaunchKernelGGL((image_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_FP64>), grid,
+                    hipLaunchKernelGGL((image_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_FP64, IMG_INTENSITY>), grid,
                                        block, lds, st, b.kp_dev, dfr, I, ImX, ImY);
                 else
-                    hipLaunchKernelGGL((image_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_MIXED>), grid,
+                    hipLaunchKernelGGL((image_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_MIXED, IMG_INTENSITY>), grid,
                                        block, lds, st, b.kp_dev, dfr, I, ImX, ImY);
             }
         });
     });
+}
+
+int mgx_image_optical_depth_device(const mgx_model *model, const mgx_options *opts,
+                                   const double *raydir, uint32_t nx, uint32_t ny, double *tau,
+                                   double *ImX, double *ImY, void *stream)
+{
+    mgx::clear_error();
+    if (!tau || !ImX || !ImY) return mgx::fail(MGX_E_ARG, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    DevBuffers b;
+    KParams kp;
+    ImageFrame *dfr = nullptr;
+    if (int e = image_setup(model, opts, raydir, nx, ny, st, b, kp, dfr)) return e;
+    struct Free {
+        ImageFrame *p;
+        ~Free() { (void)hipFree(p); }
+    } free_fr{dfr};
+    return launch_with_capacity(st, b, kp, [&](size_t lds) {
+        const dim3 grid(nx * ny), block(MGX_WAVE);
+        timed(st, [&] {
+            const bool fp64 = opts->precision == MGX_PREC_FP64;
+            if (opts->approx == MGX_APPROX_NONE) {
+                if (fp64)
+                    hipLaunchKernelGGL((image_kernel<MGX_APPROX_NONE, MGX_PREC_FP64, IMG_OPTICAL_DEPTH>),
+                                       grid, block, lds, st, b.kp_dev, dfr, tau, ImX, ImY);
+                else
+                    hipLaunchKernelGGL((image_kernel<MGX_APPROX_NONE, MGX_PREC_MIXED, IMG_OPTICAL_DEPTH>),
+                                       grid, block, lds, st, b.kp_dev, dfr, tau, ImX, ImY);
+            } else {
+                if (fp64)
+                    hipLaunchKernelGGL(
+                        (image_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_FP64, IMG_OPTICAL_DEPTH>), grid,
+                        block, lds, st, b.kp_dev, dfr, tau, ImX, ImY);
+                else
+                    hipLaunchKernelGGL(
+                        (image_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_MIXED, IMG_OPTICAL_DEPTH>), grid,
+                        block, lds, st, b.kp_dev, dfr, tau, ImX, ImY);
+            }
+        });
+    });
+}
+
+int mgx_image_ray_device(const mgx_model *model, const mgx_options *opts, uint32_t rr, double *I,
+                         double *ImX, double *ImY, void *stream)
+{
+    mgx::clear_error();
+    return run_image_ray<IMG_INTENSITY>(model, opts, rr, I, ImX, ImY, (hipStream_t)stream);
+}
+
+int mgx_image_optical_depth_ray_device(const mgx_model *model, const mgx_options *opts,
+                                       uint32_t rr, double *tau, double *ImX, double *ImY,
+                                       void *stream)
+{
+    mgx::clear_error();
+    return run_image_ray<IMG_OPTICAL_DEPTH>(model, opts, rr, tau, ImX, ImY, (hipStream_t)stream);
 }
 
 int mgx_feautrier_adjoint_device(const mgx_model *model, const mgx_options *opts,

@@ -199,6 +199,48 @@ def image_new(dm, opts, raydir, nx, ny, freqs, view=None):
     return I, ImX, ImY
 
 
+def image_optical_depth_new(dm, opts, raydir, nx, ny, freqs, view=None):
+    """Optical-depth image on the projection surface (Model::compute_image_optical_depth_new):
+    (tau (nx*ny, F), ImX, ImY) as torch tensors, the pixel rays of image_new."""
+    torch = _require_gpu()
+    dev = dm.device
+    F = freqs["nu"].shape[1]
+    if view is None:
+        view, keep = image_view(dm, freqs)
+    tau = torch.empty((nx * ny, F), dtype=torch.float64, device=dev)
+    ImX = torch.empty(nx * ny, dtype=torch.float64, device=dev)
+    ImY = torch.empty(nx * ny, dtype=torch.float64, device=dev)
+    rd = np.ascontiguousarray(raydir, dtype=np.float64)
+    _abi.check(_abi.lib().mgx_image_optical_depth_device(
+        C.byref(view), C.byref(opts), rd.ctypes.data, nx, ny, _abi.ptr(tau), _abi.ptr(ImX),
+        _abi.ptr(ImY), dm.stream_ptr()))
+    return tau, ImX, ImY
+
+
+def image_ray(dm, opts, rr, freqs=None, optical_depth=False, view=None):
+    """The old imager (Model::compute_image(rr), or compute_image_optical_depth(rr) with
+    optical_depth=True): an image at every model point along model ray rr.  Returns
+    (I or tau (P, F), ImX, ImY) as torch tensors; ``freqs`` is an image frequency grid
+    (default: the model's own frequencies)."""
+    torch = _require_gpu()
+    m = dm.model
+    dev = dm.device
+    if view is None:
+        if freqs is None:
+            view = dm.view
+        else:
+            view, keep = image_view(dm, freqs)
+    F = view.nfreqs
+    out = torch.empty((m.npoints, F), dtype=torch.float64, device=dev)
+    ImX = torch.empty(m.npoints, dtype=torch.float64, device=dev)
+    ImY = torch.empty(m.npoints, dtype=torch.float64, device=dev)
+    fn = (_abi.lib().mgx_image_optical_depth_ray_device if optical_depth
+          else _abi.lib().mgx_image_ray_device)
+    _abi.check(fn(C.byref(view), C.byref(opts), int(rr), _abi.ptr(out), _abi.ptr(ImX),
+                  _abi.ptr(ImY), dm.stream_ptr()))
+    return out, ImX, ImY
+
+
 class BatchPlan:
     """Many models solved together in one persistent launch (mgx_batch_*): the
     models' work items go through a single atomic queue, runs replay a captured

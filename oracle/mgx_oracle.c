@@ -1637,6 +1637,195 @@ int oracle_image_new(const mgx_model *m, const mgx_options *op, const double *ra
     return 0;
 }
 
+/* image_optical_depth(model, o, f) :2718-2758: the optical depth of the traced ray */
+static double optical_depth_of(const mgx_model *m, const mgx_options *op, Scratch *s, uint32_t o,
+                               uint32_t f)
+{
+    const double freq = m->nu[(size_t)o * m->nfreqs + f];
+    const uint32_t l = m->f_line[f];
+    double term_c, dtau_n, term_n, chi_c = 0.0, chi_n = 0.0;
+    const uint32_t first = s->first, last = s->last;
+    const uint32_t *nr = s->nr, *nri = s->nr_interp;
+    const double *dZ = s->dZ, *shift = s->shift, *ifac = s->interp_factor;
+    int cco = 1;
+    double tau = 0.0;
+    compute_source_dtau(m, op, nr[first], nr[first + 1], nri[first], nri[first + 1], l,
+                        freq * shift[first], freq * shift[first + 1], shift[first],
+                        shift[first + 1], dZ[first], ifac[first], ifac[first + 1], &cco, &dtau_n,
+                        &chi_c, &chi_n, &term_c, &term_n);
+    tau += dtau_n;
+    for (uint32_t n = first + 1; n < last; n++) {
+        term_c = term_n;
+        chi_c = chi_n;
+        compute_source_dtau(m, op, nr[n], nr[n + 1], nri[n], nri[n + 1], l, freq * shift[n],
+                            freq * shift[n + 1], shift[n], shift[n + 1], dZ[n], ifac[n],
+                            ifac[n + 1], &cco, &dtau_n, &chi_c, &chi_n, &term_c, &term_n);
+        tau += dtau_n;
+    }
+    return tau;
+}
+
+/* image_optical_depth_new_imager :687-736: the pixel rays of oracle_image_new, tau(pix, f) */
+int oracle_image_optical_depth_new(const mgx_model *m, const mgx_options *op,
+                                   const double *raydir_in, uint32_t nx, uint32_t ny, double *tau,
+                                   double *ImX, double *ImY, int nthreads)
+{
+    int err = check_model(m, op);
+    if (err) return err;
+    double rd[3];
+    ImageFrame fr;
+    uint32_t start_bdy;
+    err = image_prepare(m, op, raydir_in, nx, ny, rd, &fr, ImX, ImY, &start_bdy);
+    if (err) return err;
+    const int nt = default_threads(nthreads);
+#ifdef _OPENMP
+    const int saved = omp_get_max_threads();
+    omp_set_num_threads(nt);
+#endif
+    const size_t npix = (size_t)nx * ny;
+    const uint32_t length = image_scratch_length(m, op, &fr, rd, ImX, ImY, npix, start_bdy);
+#pragma omp parallel
+    {
+        Scratch s;
+        const int serr = scratch_init(&s, length);
+#pragma omp for schedule(static)
+        for (size_t pix = 0; pix < npix; pix++) {
+            if (serr) continue;
+            const uint32_t cb = image_trace_pixel(m, op, &s, &fr, rd, ImX[pix], ImY[pix], start_bdy);
+            for (uint32_t f = 0; f < m->nfreqs; f++)
+                tau[pix * m->nfreqs + f] = (s.n_tot > 1) ? optical_depth_of(m, op, &s, cb, f) : 0.0;
+        }
+        scratch_free(&s);
+    }
+#ifdef _OPENMP
+    omp_set_num_threads(saved);
+#endif
+    return 0;
+}
+
+/* Geometry::get_shift<Rest, false>(o, r, c, Z) :429-465, reversed (2 - shift) on the rays
+   r >= hnrays, as the old imager walks them */
+static double get_shift_rest_ray(const mgx_model *m, uint32_t o, uint32_t r, uint32_t c, double Z)
+{
+    const double sh = shift_rest(m, &m->position[3 * (size_t)o], ray_dir(m, o, r), c, Z);
+    return (r >= m->hnrays) ? 2.0 - sh : sh;
+}
+
+/* trace_ray<Rest, false> :742-767 with set_data's all-lines interpolation count
+   (:1003-1043).  Returns id1; *err set on an invalid next point. */
+static uint32_t trace_ray_rest(const mgx_model *m, const mgx_options *op, Scratch *s, uint32_t o,
+                               uint32_t r, int increment, uint32_t id1, uint32_t id2, int *err)
+{
+    double Z = 0.0, dZ = 0.0;
+    uint32_t nxt = get_next(m, o, r, o, &Z, &dZ);
+    if (nxt < m->npoints) {
+        uint32_t crt = o;
+        double shift_crt = get_shift_rest_ray(m, o, r, crt, 0.0);
+        double shift_nxt = get_shift_rest_ray(m, o, r, nxt, Z);
+        set_data_all(m, op, s, crt, nxt, shift_crt, shift_nxt, dZ, increment, &id1, &id2);
+        while (not_on_boundary(m, nxt)) {
+            crt = nxt;
+            shift_crt = shift_nxt;
+            nxt = get_next(m, o, r, nxt, &Z, &dZ);
+            if (nxt >= m->npoints) {
+                *err = MGX_E_NO_NEXT;
+                return id1;
+            }
+            shift_nxt = get_shift_rest_ray(m, o, r, nxt, Z);
+            set_data_all(m, op, s, crt, nxt, shift_crt, shift_nxt, dZ, increment, &id1, &id2);
+        }
+    }
+    return id1;
+}
+
+/*
+ * The old imager along model ray rr (image_feautrier_order_2(model, rr) :543-574 and
+ * image_optical_depth(model, rr) :656-686), with the coordinates of
+ * Image::set_coordinates_all_model_points (image.cpp:154-212).  optical_depth = 0:
+ * out(p, f) = 2 Su[last] - I_bdy(nr[last], nu(p, f)); 1: the ray's optical depth.
+ */
+int oracle_image_ray(const mgx_model *m, const mgx_options *op, uint32_t rr, int optical_depth,
+                     double *out, double *ImX, double *ImY, int nthreads)
+{
+    int err = check_model(m, op);
+    if (err) return err;
+    if (rr >= m->nrays) return MGX_E_ARG;
+    if (op->approx == MGX_APPROX_ONE_LINE || m->adaptive_directions) return MGX_E_UNSUPPORTED;
+    const double *rd = &m->ray_direction[3 * (size_t)rr];
+    if (m->dimension == 1 && (rd[0] != 0.0 || rd[1] != 1.0 || rd[2] != 0.0)) return MGX_E_ARG;
+    const double rx = rd[0], ry = rd[1], rz = rd[2];
+    const double denominator = sqrt(rx * rx + ry * ry);
+    const double inverse_denominator = 1.0 / denominator;
+    const double ix = ry * inverse_denominator, iy = -rx * inverse_denominator;
+    const double jx = rx * rz * inverse_denominator, jy = ry * rz * inverse_denominator;
+    const double jz = -denominator;
+    for (uint32_t p = 0; p < m->npoints; p++) {
+        const double *x = &m->position[3 * (size_t)p];
+        if (m->dimension == 1) {
+            ImX[p] = x[0];
+            ImY[p] = 0.0;
+        } else if (m->dimension == 3 && denominator >= 1.0e-9) {
+            ImX[p] = ix * x[0] + iy * x[1];
+            ImY[p] = jx * x[0] + jy * x[1] + jz * x[2];
+        } else if (m->dimension == 3) {
+            ImX[p] = x[0];
+            ImY[p] = x[1];
+        } else {
+            ImX[p] = 0.0;
+            ImY[p] = 0.0;
+        }
+    }
+    const int nt = default_threads(nthreads);
+#ifdef _OPENMP
+    const int saved = omp_get_max_threads();
+    omp_set_num_threads(nt);
+#endif
+    const uint32_t length = 2 * get_ray_lengths_max(m, op, &err) + 1;
+    const uint32_t ar = m->antipod[rr];
+    const uint32_t F = m->nfreqs;
+    if (err) goto done;
+#pragma omp parallel
+    {
+        Scratch s;
+        int serr = scratch_init(&s, length);
+#pragma omp for schedule(static)
+        for (uint32_t o = 0; o < m->npoints; o++) {
+            if (serr) continue;
+            int terr = 0;
+            s.nr[s.centre] = o;
+            s.shift[s.centre] = get_shift_rest_ray(m, o, rr, o, 0.0);
+            s.first = trace_ray_rest(m, op, &s, o, rr, -1, s.centre - 1, s.centre - 1, &terr) + 1;
+            s.last = trace_ray_rest(m, op, &s, o, ar, +1, s.centre + 1, s.centre, &terr) - 1;
+            s.n_tot = (s.last + 1) - s.first;
+            if (terr) {
+                serr = terr;
+                continue;
+            }
+            for (uint32_t f = 0; f < F; f++) {
+                const double nu_o = m->nu[(size_t)o * F + f];
+                double val;
+                if (s.n_tot > 1)
+                    val = optical_depth ? optical_depth_of(m, op, &s, o, f)
+                                        : 2.0 * image_solve_of(m, op, &s, o, f) -
+                                              boundary_intensity(m, s.nr[s.last], nu_o);
+                else
+                    val = optical_depth ? 0.0 : boundary_intensity(m, o, nu_o);
+                out[(size_t)o * F + f] = val;
+            }
+        }
+#pragma omp critical
+        {
+            if (serr && !err) err = serr;
+        }
+        scratch_free(&s);
+    }
+done:
+#ifdef _OPENMP
+    omp_set_num_threads(saved);
+#endif
+    return err;
+}
+
 /*
  * The tridiagonal system behind one (rr, o, f) solve, for the brute-force
  * checks: dtau[j] of the segment between list entries j and j+1 and the row

@@ -36,6 +36,15 @@ int oracle_trace_ray(const mgx_model *m, const mgx_options *op, uint32_t rr, uin
 /* Model::compute_image_new(raydir, nx, ny): I (nx*ny, F), ImX, ImY (nx*ny). */
 int oracle_image_new(const mgx_model *m, const mgx_options *op, const double *raydir, uint32_t nx,
                      uint32_t ny, double *I, double *ImX, double *ImY, int nthreads);
+/* Model::compute_image_optical_depth_new: tau (nx*ny, F) on the pixel rays of
+   oracle_image_new, ImX, ImY (nx*ny). */
+int oracle_image_optical_depth_new(const mgx_model *m, const mgx_options *op,
+                                   const double *raydir, uint32_t nx, uint32_t ny, double *tau,
+                                   double *ImX, double *ImY, int nthreads);
+/* Model::compute_image(rr) (optical_depth = 0) or compute_image_optical_depth(rr) (1):
+   out (P, F) and the points' image coordinates ImX, ImY (P). */
+int oracle_image_ray(const mgx_model *m, const mgx_options *op, uint32_t rr, int optical_depth,
+                     double *out, double *ImX, double *ImY, int nthreads);
 uint32_t oracle_closest_bdy_point(const mgx_model *m, const double *raydir);
 /* tridiagonal system of one (rr, o, f) solve: dtau (n-1), S (n), I_bdy (2), u_c, L_c */
 int oracle_feautrier_system(const mgx_model *m, const mgx_options *op, uint32_t rr, uint32_t o,

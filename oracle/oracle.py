@@ -140,6 +140,42 @@ def image_new(model, opts, raydir, nx, ny, freqs=None, nthreads=0, kind="cr"):
     return I, ImX, ImY
 
 
+def image_optical_depth_new(model, opts, raydir, nx, ny, freqs=None, nthreads=0, kind="cr"):
+    """Model::compute_image_optical_depth_new: (tau (nx*ny, F), ImX, ImY)."""
+    F = model.nfreqs if freqs is None else freqs["nu"].shape[1]
+    m = model.view(freqs, nfreqs=F)
+    rd = np.ascontiguousarray(raydir, dtype=np.float64)
+    tau = np.zeros((nx * ny, F))
+    ImX = np.zeros(nx * ny)
+    ImY = np.zeros(nx * ny)
+    h = lib(kind)
+    h.oracle_image_optical_depth_new.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                                 C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                 C.c_int]
+    _check(h.oracle_image_optical_depth_new(C.addressof(m), C.addressof(opts), rd.ctypes.data, nx,
+                                            ny, tau.ctypes.data, ImX.ctypes.data, ImY.ctypes.data,
+                                            nthreads), "image_optical_depth_new")
+    return tau, ImX, ImY
+
+
+def image_ray(model, opts, rr, freqs=None, optical_depth=False, nthreads=0, kind="cr"):
+    """Model::compute_image(rr) (or compute_image_optical_depth(rr) with
+    optical_depth=True), the old imager at every model point: (out (P, F), ImX, ImY)."""
+    F = model.nfreqs if freqs is None else freqs["nu"].shape[1]
+    m = model.view(freqs, nfreqs=F)
+    P = model.npoints
+    out = np.zeros((P, F))
+    ImX = np.zeros(P)
+    ImY = np.zeros(P)
+    h = lib(kind)
+    h.oracle_image_ray.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_void_p,
+                                   C.c_void_p, C.c_void_p, C.c_int]
+    _check(h.oracle_image_ray(C.addressof(m), C.addressof(opts), rr, int(bool(optical_depth)),
+                              out.ctypes.data, ImX.ctypes.data, ImY.ctypes.data, nthreads),
+           "image_ray")
+    return out, ImX, ImY
+
+
 def feautrier_system(model, opts, rr, o, f, capacity=4096, kind="cr"):
     """dtau (n-1), S (n), I_bdy (2), centre, u_c and L_c of one (rr, o, f) solve."""
     m = model.view()
