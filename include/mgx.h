@@ -49,6 +49,7 @@ enum {
 enum { MGX_APPROX_NONE = 0, MGX_APPROX_ONE_LINE = 1, MGX_APPROX_CLOSE_LINES = 2 };
 enum { MGX_BC_ZERO = 0, MGX_BC_THERMAL = 1, MGX_BC_CMB = 2 };
 enum { MGX_PREC_FP64 = 0, MGX_PREC_MIXED = 1 };
+enum { MGX_MAX_OFF_DIAG = 4 }; /* largest n_off_diag of mgx_feautrier_forward_lambda_device */
 
 /*
  * Model view.  Mirrors the data the reference's Solver reads from Model&
@@ -133,6 +134,26 @@ const char *mgx_version(void);
  */
 int mgx_feautrier_forward_device(const mgx_model *model_dev, const mgx_options *opts,
                                  double *u, double *J, double *lambda_diag, void *stream);
+
+/*
+ * The plain-method formal solve with the general Lambda operator: u
+ * (hnrays,P,F) [may be NULL] and J (P,F) as mgx_feautrier_forward_device, and
+ * for each receiving point p and line l the list of approximated-Lambda
+ * elements (emitting point, value): lam_count (P,L), lam_nr and lam_val
+ * (P,L,cap), entries [0, lam_count) in the reference's order, the diagonal
+ * element (p itself) first.  With opts->n_off_diag = m the list holds the
+ * diagonal and the elements of the m rows before and after p on every ray
+ * (an interpolated row counts for its step's next point); a capacity of
+ * 1 + 2 m hnrays always suffices.  Replaces Solver::solve_feautrier_order_2
+ * with n_off_diag > 0 (src/solver/solver.tpp:459-540, 2319-2355),
+ * Solver::update_Lambda (:1761-1850) and Lambda::add_element
+ * (src/model/lines/lineProducingSpecies/lambda/lambda.tpp).  Errors:
+ * n_off_diag > MGX_MAX_OFF_DIAG -> MGX_E_UNSUPPORTED; a list longer than cap ->
+ * MGX_E_ARG.
+ */
+int mgx_feautrier_forward_lambda_device(const mgx_model *model_dev, const mgx_options *opts,
+                                        double *u, double *J, uint32_t cap, uint32_t *lam_count,
+                                        uint32_t *lam_nr, double *lam_val, void *stream);
 
 /*
  * Order-0 short-characteristics formal solve on the device: fills I

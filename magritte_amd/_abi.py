@@ -19,6 +19,7 @@ MGX_E_UNSUPPORTED = -5
 APPROX_NONE, APPROX_ONE_LINE, APPROX_CLOSE_LINES = 0, 1, 2
 BC_ZERO, BC_THERMAL, BC_CMB = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
+MAX_OFF_DIAG = 4  # largest n_off_diag of mgx_feautrier_forward_lambda_device
 
 _pd = C.POINTER(C.c_double)
 _pu = C.POINTER(C.c_uint32)
@@ -136,6 +137,8 @@ SIGNATURES = [
     ("mgx_image_device", C.c_int,
      [C.POINTER(MgxModel), C.POINTER(MgxOptions), _VP, C.c_uint32, C.c_uint32, _VP, _VP, _VP,
       _VP]),
+    ("mgx_feautrier_forward_lambda_device", C.c_int,
+     [C.POINTER(MgxModel), C.POINTER(MgxOptions), _VP, _VP, C.c_uint32, _VP, _VP, _VP, _VP]),
     ("mgx_image_optical_depth_device", C.c_int,
      [C.POINTER(MgxModel), C.POINTER(MgxOptions), _VP, C.c_uint32, C.c_uint32, _VP, _VP, _VP,
       _VP]),

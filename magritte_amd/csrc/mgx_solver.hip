@@ -56,7 +56,7 @@
 
 namespace {
 
-enum { ERR_CAP = 1, ERR_NO_NEXT = 2, ERR_OUT = 4, ERR_CAPE = 8 };
+enum { ERR_CAP = 1, ERR_NO_NEXT = 2, ERR_OUT = 4, ERR_CAPE = 8, ERR_LAMCAP = 16 };
 
 /* global-address-space load (the pointers live in a struct, so the compiler would
    otherwise emit flat loads, which also count against the LDS wait counter and so
@@ -794,12 +794,44 @@ struct Sweep {
     bool started;
 };
 
-template <int APPROX, int PREC>
+/*
+ * The last NB rows a half sweep passed (index 0 = the row next to the origin):
+ * h = dY/dS of the row and g = dY/dY_outer (the elimination factors of the
+ * row's source and of the row before it), with the row's point and shift.
+ * The off-diagonal Lambda bands are products of these: the entry of the
+ * inverse of the Feautrier system between the origin and the j-th row on a
+ * side is (du_c/dY_1) g_1 ... g_(j-1) h_j.
+ */
+template <int NB>
+struct BandRing {
+    double h[NB > 0 ? NB : 1], g[NB > 0 ? NB : 1], sh[NB > 0 ? NB : 1];
+    uint32_t nr[NB > 0 ? NB : 1];
+    uint32_t n;
+};
+
+template <int NB>
+__device__ inline void ring_push(BandRing<NB> &r, double h, double g, const Entry &e)
+{
+#pragma unroll
+    for (int i = NB - 1; i > 0; i--) {
+        r.h[i] = r.h[i - 1];
+        r.g[i] = r.g[i - 1];
+        r.sh[i] = r.sh[i - 1];
+        r.nr[i] = r.nr[i - 1];
+    }
+    r.h[0] = h;
+    r.g[0] = g;
+    r.sh[0] = e.shift;
+    r.nr[0] = e.nr;
+    r.n = (r.n < (uint32_t)NB) ? r.n + 1 : (uint32_t)NB;
+}
+
+template <int APPROX, int PREC, int NB = 0>
 __device__ inline void sweep_half(const KParams &kp, const double *s_erf, const OwnLine &ol,
                                   bool backward, uint32_t S, uint32_t o, double shift0,
                                   const uint32_t *s_nxt, const double *s_dZ, const double *s_sh,
                                   const uint16_t *mtab, uint32_t l, double freq, Sweep &sw,
-                                  PointEC &ec_centre)
+                                  PointEC &ec_centre, BandRing<NB> *ring = nullptr)
 {
     HalfIter it;
     it.s = (int)S - 1;
@@ -831,10 +863,12 @@ __device__ inline void sweep_half(const KParams &kp, const double *s_erf, const 
             const double idt = rcp(dtau);
             const double B_min = fma(2.0, idt, 1.0);
             const double B = fma(2.0 * idt, idt, B_min);
-            sw.Y = fma(2.0 * I_b, idt, S_row) * rcp(B);
+            const double iB = rcp(B);
+            sw.Y = fma(2.0 * I_b, idt, S_row) * iB;
             sw.X = 0.5 * B_min * dtau * dtau;
             sw.XI = rcp(1.0 + sw.X);
             sw.started = true;
+            if constexpr (NB > 0) ring_push<NB>(*ring, iB, 0.0, outer);
         } else {
             const double avg = 0.5 * (sw.prev_dtau + dtau);
             const double q_out = avg * sw.prev_dtau;
@@ -843,6 +877,10 @@ __device__ inline void sweep_half(const KParams &kp, const double *s_erf, const 
             sw.X = fma(r * sw.X, sw.XI, 1.0) * q_in;
             sw.XI = rcp(1.0 + sw.X);
             sw.Y = fma(r, sw.Y, S_row) * sw.XI * q_in;
+            if constexpr (NB > 0) {
+                const double hq = sw.XI * q_in;
+                ring_push<NB>(*ring, hq, r * hq, outer);
+            }
         }
         sw.prev_dtau = dtau;
         sw.S_inner = S_in;
@@ -951,11 +989,42 @@ struct ShortcharOut {
  * the Lambda diagonal into the LDS accumulator s_L (L).  Returns 0, or an
  * ERR_* code after flagging it in kp.status.
  */
-template <int APPROX, int PREC, int METHOD = METHOD_FEAUTRIER>
+/* the general Lambda operator (n_off_diag > 0): for each receiving point and line a list
+   of at most cap (emitting point, value) entries, the diagonal element first */
+struct LamBands {
+    uint32_t *count; /* (P, L) */
+    uint32_t *nr;    /* (P, L, cap) */
+    double *val;     /* (P, L, cap) */
+    uint32_t cap, noff;
+};
+
+/* Lambda::add_element (lambda.tpp): add to the entry of point q, else append */
+__device__ inline void lam_merge(const KParams &kp, const LamBands &lb, size_t pl, uint32_t q,
+                                 double v)
+{
+    const size_t base = pl * lb.cap;
+    const uint32_t n = lb.count[pl];
+    for (uint32_t i = 0; i < n; i++) {
+        if (lb.nr[base + i] == q) {
+            lb.val[base + i] += v;
+            return;
+        }
+    }
+    if (n >= lb.cap) {
+        atomicOr(&kp.status[0], (uint32_t)ERR_LAMCAP);
+        return;
+    }
+    lb.nr[base + n] = q;
+    lb.val[base + n] = v;
+    lb.count[pl] = n + 1;
+}
+
+template <int APPROX, int PREC, int METHOD = METHOD_FEAUTRIER, int NB = 0>
 __device__ __forceinline__ int solve_origin(const KParams &kp, uint32_t o, uint32_t rr0,
                                             uint32_t rr1, unsigned char *smem, const double *s_erf,
                                             double *u, bool with_lam, double *s_J, double *Jrow,
-                                            bool j_atomic, double *s_L, ShortcharOut sc = {})
+                                            bool j_atomic, double *s_L, ShortcharOut sc = {},
+                                            LamBands lb = {})
 {
     const mgx_model &m = kp.m;
     const uint32_t lane = threadIdx.x;
@@ -1009,6 +1078,16 @@ __device__ __forceinline__ int solve_origin(const KParams &kp, uint32_t o, uint3
             const uint32_t z = (slot % kp.slots_per_line) * 32 + (lane & 31);
             const bool active = (slot < kp.nslots) && (z < Q);
             double Lval = 0.0;
+            /* band elements of this lane, rr side (lo) and antipod side (up), j = 0 .. NB-1 */
+            double Llo[NB > 0 ? NB : 1], Lup[NB > 0 ? NB : 1];
+            uint32_t qlo[NB > 0 ? NB : 1], qup[NB > 0 ? NB : 1];
+            uint32_t nlo = 0, nup = 0;
+            bool lam_added = false; /* update_Lambda ran for this lane (n_tot > 1, in the line) */
+#pragma unroll
+            for (int j = 0; j < (NB > 0 ? NB : 1); j++) {
+                Llo[j] = Lup[j] = 0.0;
+                qlo[j] = qup[j] = 0;
+            }
             if (active) {
                 const OwnLine ol = own_line(m, l);
                 const uint32_t f = l * Q + z;
@@ -1072,15 +1151,18 @@ __device__ __forceinline__ int solve_origin(const KParams &kp, uint32_t o, uint3
                     PointEC ec_cent = {0.0, 0.0, false};
                     Sweep bw, fw;
                     bw.started = fw.started = false;
+                    BandRing<NB> ring_b, ring_f;
+                    ring_b.n = ring_f.n = 0;
                     if (S1 > 0)
-                        sweep_half<APPROX, PREC>(kp, s_erf, ol, true, S1, o, sh01, s_nxt1, s_dZ1,
-                                                 s_sh1, s_m + (jj * 2 + 1) * cap, l, freq, bw,
-                                                 ec_cent);
+                        sweep_half<APPROX, PREC, NB>(kp, s_erf, ol, true, S1, o, sh01, s_nxt1,
+                                                     s_dZ1, s_sh1, s_m + (jj * 2 + 1) * cap, l,
+                                                     freq, bw, ec_cent, &ring_b);
                     if (S0 > 0)
-                        sweep_half<APPROX, PREC>(kp, s_erf, ol, false, S0, o, sh00, s_nxt0, s_dZ0,
-                                                 s_sh0, s_m + (jj * 2 + 0) * cap, l, freq, fw,
-                                                 ec_cent);
+                        sweep_half<APPROX, PREC, NB>(kp, s_erf, ol, false, S0, o, sh00, s_nxt0,
+                                                     s_dZ0, s_sh0, s_m + (jj * 2 + 0) * cap, l,
+                                                     freq, fw, ec_cent, &ring_f);
                     double Lc;
+                    double du_fw = 0.0, du_bw = 0.0; /* du_c / dY of each half's last row */
                     if (S1 > 0) {
                         /* row of the origin: forward quantities at the origin, joined with
                            the backward ones of its antipod-side neighbour */
@@ -1102,11 +1184,14 @@ __device__ __forceinline__ int solve_origin(const KParams &kp, uint32_t o, uint3
                             FF = fma(r * fw.X, fw.XI, 1.0) * iC_c;
                             const double FI = rcp(1.0 + FF);
                             Su = fma(r, fw.Y, S_c) * FI * iC_c;
+                            du_fw = r * iC_c; /* times rden below */
                         }
                         const double GP = bw.X * bw.XI;
                         const double rden = rcp(FF + GP);
                         uc = fma(Su, 1.0 + FF, bw.Y) * rden;
                         Lc = iC_c * rden;
+                        du_fw *= rden;
+                        du_bw = rden;
                         if constexpr (METHOD == METHOD_UV) {
                             if (S0 == 0) {
                                 /* the origin is the first row: v = I_bdy - u there */
@@ -1145,6 +1230,7 @@ __device__ __forceinline__ int solve_origin(const KParams &kp, uint32_t o, uint3
                         const double Sl = fma(2.0 * I_l, idt, fw.S_inner);
                         uc = fma(A_l, fw.Y, Sl) * (1.0 + fw.X) * den;
                         Lc = (1.0 + fw.X) * den;
+                        du_fw = A_l * Lc;
                         if constexpr (METHOD == METHOD_UV) vc = uc - I_l; /* v = u - I_bdy at the last row */
                     }
                     if constexpr (METHOD != METHOD_UV) {
@@ -1152,6 +1238,41 @@ __device__ __forceinline__ int solve_origin(const KParams &kp, uint32_t o, uint3
                             s_J[f] = s_J[f] + uc * 2.0 * w;
                         else
                             j_add(Jrow, f, uc * 2.0 * w, rr == rr0, j_atomic);
+                    }
+                    /* ---- off-diagonal bands (update_Lambda, solver.tpp:1808-1850): the
+                       entry of the inverse between the origin and the j-th row on each
+                       side, an element at that row's point, its opacity at the line
+                       centre and its profile at nu(point, f) times the row's shift ---- */
+                    if constexpr (NB > 0) {
+                        if (with_lam && m.f_in_line[f]) {
+                            lam_added = true;
+                            nlo = ring_f.n < lb.noff ? ring_f.n : lb.noff;
+                            nup = ring_b.n < lb.noff ? ring_b.n : lb.noff;
+                            double af = du_fw, ab = du_bw;
+#pragma unroll
+                            for (int j = 0; j < NB; j++) {
+                                if ((uint32_t)j < nlo) {
+                                    const uint32_t q = ring_f.nr[j];
+                                    const double fq = gld(&m.nu[(size_t)q * F + f]) * ring_f.sh[j];
+                                    const LamPoint lq =
+                                        lambda_point<APPROX, PREC>(kp, ol, q, l, f, fq, 2.0 * w);
+                                    Llo[j] = lq.constante * lq.frq * lq.phi * (af * ring_f.h[j]) *
+                                             lq.inverse_chi;
+                                    qlo[j] = q;
+                                }
+                                if ((uint32_t)j < nup) {
+                                    const uint32_t q = ring_b.nr[j];
+                                    const double fq = gld(&m.nu[(size_t)q * F + f]) * ring_b.sh[j];
+                                    const LamPoint lq =
+                                        lambda_point<APPROX, PREC>(kp, ol, q, l, f, fq, 2.0 * w);
+                                    Lup[j] = lq.constante * lq.frq * lq.phi * (ab * ring_b.h[j]) *
+                                             lq.inverse_chi;
+                                    qup[j] = q;
+                                }
+                                af *= ring_f.g[j];
+                                ab *= ring_b.g[j];
+                            }
+                        }
                     }
                     /* ---- Lambda diagonal (update_Lambda, solver.tpp:1761-1800) ---- */
                     if (METHOD != METHOD_UV && with_lam && m.f_in_line[f]) {
@@ -1176,6 +1297,63 @@ __device__ __forceinline__ int solve_origin(const KParams &kp, uint32_t o, uint3
                         if (slot_h >= kp.nslots) break;
                         const uint32_t ll = slot_h / kp.slots_per_line;
                         s_L[ll] = s_L[ll] + (h ? v_hi : v);
+                    }
+                }
+                /* band elements: the same sums, then lane 0 merges them into the lists of
+                   (o, line) in the reference's order (lower then upper, band by band) */
+                if constexpr (NB > 0) {
+                    const uint64_t added = __ballot(lam_added);
+                    /* rows and points are the same on every lane of a slot that ran
+                       update_Lambda, zero elsewhere: take the largest */
+#pragma unroll
+                    for (int off = 16; off >= 1; off >>= 1) {
+                        const uint32_t a = __shfl_xor(nlo, off, MGX_WAVE), b = __shfl_xor(nup, off, MGX_WAVE);
+                        nlo = a > nlo ? a : nlo;
+                        nup = b > nup ? b : nup;
+                    }
+                    const uint32_t nlo_hi = __shfl(nlo, 32, MGX_WAVE), nup_hi = __shfl(nup, 32, MGX_WAVE);
+#pragma unroll
+                    for (int j = 0; j < NB; j++) {
+                        double a = Llo[j], b = Lup[j];
+                        uint32_t qa = qlo[j], qb = qup[j];
+#pragma unroll
+                        for (int off = 16; off >= 1; off >>= 1) {
+                            a += __shfl_xor(a, off, MGX_WAVE);
+                            b += __shfl_xor(b, off, MGX_WAVE);
+                            const uint32_t ta = __shfl_xor(qa, off, MGX_WAVE), tb = __shfl_xor(qb, off, MGX_WAVE);
+                            qa = ta > qa ? ta : qa;
+                            qb = tb > qb ? tb : qb;
+                        }
+                        Llo[j] = a;
+                        Lup[j] = b;
+                        qlo[j] = qa;
+                        qup[j] = qb;
+                    }
+                    double Llo_hi[NB], Lup_hi[NB];
+                    uint32_t qlo_hi[NB], qup_hi[NB];
+#pragma unroll
+                    for (int j = 0; j < NB; j++) {
+                        Llo_hi[j] = __shfl(Llo[j], 32, MGX_WAVE);
+                        Lup_hi[j] = __shfl(Lup[j], 32, MGX_WAVE);
+                        qlo_hi[j] = __shfl(qlo[j], 32, MGX_WAVE);
+                        qup_hi[j] = __shfl(qup[j], 32, MGX_WAVE);
+                    }
+                    if (lane == 0) {
+                        for (uint32_t h = 0; h < 2; h++) {
+                            const uint32_t slot_h = slot0 + h;
+                            if (slot_h >= kp.nslots) break;
+                            const size_t pl = (size_t)o * m.nlines + slot_h / kp.slots_per_line;
+                            const uint32_t n_lo = h ? nlo_hi : nlo, n_up = h ? nup_hi : nup;
+                            if (((added >> (32 * h)) & 0xffffffffull) == 0) continue;
+                            lam_merge(kp, lb, pl, o, h ? v_hi : v); /* the diagonal element first */
+#pragma unroll
+                            for (int j = 0; j < NB; j++) {
+                                if ((uint32_t)j < n_lo)
+                                    lam_merge(kp, lb, pl, h ? qlo_hi[j] : qlo[j], h ? Llo_hi[j] : Llo[j]);
+                                if ((uint32_t)j < n_up)
+                                    lam_merge(kp, lb, pl, h ? qup_hi[j] : qup[j], h ? Lup_hi[j] : Lup[j]);
+                            }
+                        }
                     }
                 }
             }
@@ -1208,6 +1386,34 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_SOLVE_WAVES_PER_SIMD) void feautrier_
         for (uint32_t i = lane; i < F; i += MGX_WAVE) J[(size_t)o * F + i] = s_J[i];
     if (lam)
         for (uint32_t i = lane; i < L; i += MGX_WAVE) lam[(size_t)o * L + i] = s_L[i];
+}
+
+/* The forward solve with the general Lambda operator: the bands of up to MGX_MAX_OFF_DIAG
+   rows on either side of each origin merged into per-(point, line) lists.  Origins are
+   not split over waves, so each list is written by one lane of one wave. */
+template <int APPROX, int PREC>
+__global__ __launch_bounds__(MGX_WAVE, 2) void feautrier_bands_kernel(
+    const KParams *__restrict__ kpp, double *u, double *J, LamBands lb)
+{
+    const KParams &kp = *kpp;
+    extern __shared__ unsigned char smem[];
+    __shared__ double s_erf[MGX_ERF_NINT * (MGX_ERF_DEG + 1)];
+    for (uint32_t i = threadIdx.x; i < MGX_ERF_NINT * (MGX_ERF_DEG + 1); i += MGX_WAVE)
+        s_erf[i] = MGX_ERF_TAB[i];
+    const mgx_model &m = kp.m;
+    const uint32_t o = blockIdx.x;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t F = m.nfreqs, L = m.nlines;
+    double *s_L = reinterpret_cast<double *>(smem + ((walk_lds_bytes(kp.cap) + 7) / 8) * 8);
+    double *s_J = kp.j_lds ? s_L + L : nullptr;
+    for (uint32_t i = lane; i < L + (s_J ? F : 0); i += MGX_WAVE) s_L[i] = 0.0;
+    if (lane == 0)
+        for (uint32_t l = 0; l < L; l++) lb.count[(size_t)o * L + l] = 0;
+    if (solve_origin<APPROX, PREC, METHOD_FEAUTRIER, MGX_MAX_OFF_DIAG>(
+            kp, o, 0, m.hnrays, smem, s_erf, u, true, s_J, J + (size_t)o * F, false, s_L, {}, lb))
+        return;
+    if (s_J)
+        for (uint32_t i = lane; i < F; i += MGX_WAVE) J[(size_t)o * F + i] = s_J[i];
 }
 
 /* Order-0 short characteristics (Solver::solve_shortchar_order_0, solver.tpp:151-235):
@@ -3304,6 +3510,7 @@ int read_status(hipStream_t st, const DevBuffers &b, uint32_t *need, uint32_t *n
     if (h[0] & ERR_NO_NEXT)
         return mgx::fail(MGX_E_NO_NEXT, "a ray from an interior point found no valid next point");
     if (h[0] & ERR_OUT) return mgx::fail(MGX_E_RAY_LENGTH, "a traced list exceeds the output capacity");
+    if (h[0] & ERR_LAMCAP) return mgx::fail(MGX_E_ARG, "a Lambda list exceeds its capacity");
     if (h[0] & ERR_CAP) return MGX_E_RAY_LENGTH;
     return MGX_OK;
 }
@@ -3840,6 +4047,58 @@ int mgx_feautrier_forward_device(const mgx_model *model, const mgx_options *opts
                 else
                     hipLaunchKernelGGL((feautrier_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_MIXED>),
                                        grid, block, lds, st, b.kp_dev, u, J, lambda_diag);
+            }
+        });
+    }, 8 * (size_t)model->nlines);
+}
+
+int mgx_feautrier_forward_lambda_device(const mgx_model *model, const mgx_options *opts,
+                                        double *u, double *J, uint32_t cap, uint32_t *lam_count,
+                                        uint32_t *lam_nr, double *lam_val, void *stream)
+{
+    mgx::clear_error();
+    if (!opts) return mgx::fail(MGX_E_ARG, "null options");
+    if (opts->n_off_diag > MGX_MAX_OFF_DIAG)
+        return mgx::fail(MGX_E_UNSUPPORTED, "n_off_diag > %d is not supported", MGX_MAX_OFF_DIAG);
+    mgx_options op = *opts;
+    op.n_off_diag = 0; /* the rest of the checks as the diagonal solve */
+    if (int e = check_view(model, &op)) return e;
+    if (!J || !lam_count || !lam_nr || !lam_val) return mgx::fail(MGX_E_ARG, "null argument");
+    if (cap < 1) return mgx::fail(MGX_E_ARG, "the Lambda lists need a capacity of at least 1");
+    hipStream_t st = (hipStream_t)stream;
+    DevBuffers b;
+    KParams kp;
+    if (int e = prepare(model, &op, st, b, kp)) return e;
+    const LamBands lb{lam_count, lam_nr, lam_val, cap, opts->n_off_diag};
+    return launch_with_capacity(st, b, kp, [&](size_t) {
+        const dim3 grid(model->npoints), block(MGX_WAVE);
+        const size_t lds = solve_lds_bytes(kp.cap, model->nfreqs, model->nlines, kp.j_lds != 0);
+        timed(st, [&] {
+            const bool fp64 = op.precision == MGX_PREC_FP64;
+            switch (op.approx) {
+            case MGX_APPROX_NONE:
+                if (fp64)
+                    hipLaunchKernelGGL((feautrier_bands_kernel<MGX_APPROX_NONE, MGX_PREC_FP64>), grid,
+                                       block, lds, st, b.kp_dev, u, J, lb);
+                else
+                    hipLaunchKernelGGL((feautrier_bands_kernel<MGX_APPROX_NONE, MGX_PREC_MIXED>), grid,
+                                       block, lds, st, b.kp_dev, u, J, lb);
+                break;
+            case MGX_APPROX_ONE_LINE:
+                if (fp64)
+                    hipLaunchKernelGGL((feautrier_bands_kernel<MGX_APPROX_ONE_LINE, MGX_PREC_FP64>),
+                                       grid, block, lds, st, b.kp_dev, u, J, lb);
+                else
+                    hipLaunchKernelGGL((feautrier_bands_kernel<MGX_APPROX_ONE_LINE, MGX_PREC_MIXED>),
+                                       grid, block, lds, st, b.kp_dev, u, J, lb);
+                break;
+            default:
+                if (fp64)
+                    hipLaunchKernelGGL((feautrier_bands_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_FP64>),
+                                       grid, block, lds, st, b.kp_dev, u, J, lb);
+                else
+                    hipLaunchKernelGGL((feautrier_bands_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_MIXED>),
+                                       grid, block, lds, st, b.kp_dev, u, J, lb);
             }
         });
     }, 8 * (size_t)model->nlines);

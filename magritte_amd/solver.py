@@ -54,6 +54,28 @@ def feautrier_forward(dm, opts, with_u=True, with_lambda=True, out=None, view=No
     return out
 
 
+def feautrier_forward_lambda(dm, opts, cap=None, with_u=False):
+    """The formal solve with the general Lambda operator (opts.n_off_diag bands on either
+    side, up to _abi.MAX_OFF_DIAG): dict(u, J, count (P, L), nr (P, L, cap), val (P, L, cap));
+    the list of (p, l) is nr[p, l, :count], val[p, l, :count], diagonal element first.
+    cap defaults to 1 + 2 n_off_diag hnrays, which always suffices."""
+    torch = _require_gpu()
+    m = dm.model
+    dev = dm.device
+    P, L = m.npoints, m.nlines
+    cap = 1 + 2 * int(opts.n_off_diag) * m.hnrays if cap is None else int(cap)
+    out = {"u": torch.empty((m.hnrays, P, m.nfreqs), dtype=torch.float64, device=dev)
+           if with_u else None,
+           "J": torch.empty((P, m.nfreqs), dtype=torch.float64, device=dev),
+           "count": torch.empty((P, L), dtype=torch.int32, device=dev),
+           "nr": torch.zeros((P, L, cap), dtype=torch.int32, device=dev),
+           "val": torch.zeros((P, L, cap), dtype=torch.float64, device=dev)}
+    _abi.check(_abi.lib().mgx_feautrier_forward_lambda_device(
+        C.byref(dm.view), C.byref(opts), _abi.ptr(out["u"]), _abi.ptr(out["J"]), cap,
+        _abi.ptr(out["count"]), _abi.ptr(out["nr"]), _abi.ptr(out["val"]), dm.stream_ptr()))
+    return out
+
+
 def shortchar_forward(dm, opts, min_dtau=1.0e-15, with_I=True, with_uv=True, with_lambda=True):
     """Order-0 short characteristics (Model::compute_radiation_field_shortchar_order_0):
     dict(I=(nrays,P,F), u=(hnrays,P,F), v=(hnrays,P,F), J=(P,F), lambda_diag=(P,L))."""

@@ -294,6 +294,10 @@ typedef struct {
     uint32_t *nr, *nr_interp;
     double *dZ, *shift, *interp_factor;
     double *Su, *A, *C, *inverse_A, *inverse_C, *FF, *FI, *GG, *GP, *L_diag;
+    /* off-diagonal Lambda bands (n_off_diag > 0): GI (length) and L_upper,
+       L_lower (n_off_diag x length, row-major) */
+    uint32_t noff;
+    double *GI, *L_upper, *L_lower;
 } Scratch;
 
 static int scratch_init(Scratch *s, uint32_t length)
@@ -313,8 +317,21 @@ static int scratch_init(Scratch *s, uint32_t length)
     return ok ? 0 : -1;
 }
 
+/* the band arrays of Solver::setup (:50-56) for n_off_diag > 0 */
+static int scratch_bands(Scratch *s, uint32_t noff)
+{
+    s->noff = noff;
+    s->GI = calloc(s->length, sizeof(double));
+    s->L_upper = calloc((size_t)noff * s->length, sizeof(double));
+    s->L_lower = calloc((size_t)noff * s->length, sizeof(double));
+    return (s->GI && s->L_upper && s->L_lower) ? 0 : -1;
+}
+
 static void scratch_free(Scratch *s)
 {
+    free(s->GI);
+    free(s->L_upper);
+    free(s->L_lower);
     free(s->nr);
     free(s->nr_interp);
     double *d[] = {s->dZ, s->shift, s->interp_factor, s->Su, s->A, s->C, s->inverse_A,
@@ -764,6 +781,35 @@ static void solve_feautrier_order_2_of(const mgx_model *m, const mgx_options *op
     Su[last] = term_n + 2.0 * I_bdy_l * inverse_dtau_l;
     Su[last] = (A[last] * Su[last - 1] + Su[last]) * (1.0 + FF[last - 1]) * denominator;
 
+    if (op->n_off_diag > 0) {
+        /* all of L_diag and the bands of the inverse (:2319-2355) */
+        const uint32_t M = op->n_off_diag, L0 = s->length;
+        double *GI = s->GI, *Lu = s->L_upper, *Ll = s->L_lower;
+        GG[last] = 0.5 * Bl_min_Al * dtau_n * dtau_n;
+        GI[last] = 1.0 / (1.0 + GG[last]);
+        GP[last] = GG[last] * GI[last];
+        L_diag[last] = (1.0 + FF[last - 1]) / (Bl_min_Al + Bl * FF[last - 1]);
+        for (long n = (long)last - 1; n > (long)first; n--) {
+            Su[n] += Su[n + 1] * FI[n];
+            GG[n] = (C[n] * GP[n + 1] + 1.0) * iA[n];
+            GI[n] = 1.0 / (1.0 + GG[n]);
+            GP[n] = GG[n] * GI[n];
+            L_diag[n] = iC[n] / (FF[n] + GP[n + 1]);
+        }
+        Su[first] += Su[first + 1] * FI[first];
+        L_diag[first] = (1.0 + GG[first + 1]) / (Bf_min_Cf + Bf * GG[first + 1]);
+        for (long n = (long)last - 1; n >= (long)first; n--) {
+            Lu[n + 1] = L_diag[n + 1] * FI[n];
+            Ll[n] = L_diag[n] * GI[n + 1];
+        }
+        for (uint32_t mm = 1; mm < M && mm < s->n_tot - 1; mm++) {
+            for (long n = (long)last - 1 - mm; n >= (long)first; n--) {
+                Lu[(size_t)mm * L0 + n + mm + 1] = Lu[(size_t)(mm - 1) * L0 + n + mm + 1] * FI[n];
+                Ll[(size_t)mm * L0 + n] = Ll[(size_t)(mm - 1) * L0 + n] * GI[n + mm + 1];
+            }
+        }
+        return;
+    }
     if (centre < last) {
         GG[last] = 0.5 * Bl_min_Al * dtau_n * dtau_n;
         GP[last] = GG[last] / (1.0 + GG[last]);
@@ -804,6 +850,78 @@ static void update_lambda(const mgx_model *m, const mgx_options *op, const Scrat
     lambda_diag[(size_t)p * m->nlines + lg] += L;
 }
 
+/* Lambda (lambda.tpp): per receiving point p and line lg a list of (emitting point, value)
+   with capacity cap; add_element adds to an existing entry of the same point, else appends */
+typedef struct {
+    uint32_t cap;
+    uint32_t *count, *nr;
+    double *val;
+} LamList;
+
+static int lam_add(LamList *ll, size_t pl, uint32_t nr, double L)
+{
+    const uint32_t n = ll->count[pl];
+    for (uint32_t i = 0; i < n; i++) {
+        if (ll->nr[pl * ll->cap + i] == nr) {
+            ll->val[pl * ll->cap + i] += L;
+            return 0;
+        }
+    }
+    if (n >= ll->cap) return MGX_E_ARG;
+    ll->nr[pl * ll->cap + n] = nr;
+    ll->val[pl * ll->cap + n] = L;
+    ll->count[pl] = n + 1;
+    return 0;
+}
+
+/* update_Lambda<approx,adaptive>(rr, f) with the off-diagonal bands (:1761-1850): the
+   diagonal element, then for m = 0 .. n_off_diag-1 the entries m+1 before and after the
+   centre, each at its own point nr[n] (an interpolated entry counts for its step's next
+   point), its opacity at the line centre with the transition index as line index, and its
+   profile at nu(nr[n], f) shift[n] */
+static int update_lambda_list(const mgx_model *m, const mgx_options *op, const Scratch *s,
+                              uint32_t rr, uint32_t f, LamList *ll)
+{
+    if (!m->f_in_line[f]) return 0;
+    const uint32_t c = s->centre, first = s->first, last = s->last;
+    const uint32_t p = s->nr[c];
+    const double w_ang = 2.0 * ray_weight(m, p, rr);
+    const uint32_t spec = m->f_species[f];
+    const uint32_t k = m->f_k[f];
+    const uint32_t z = m->f_z[f];
+    const uint32_t lg = m->f_line[f];
+    const double freq_line = m->line[lg];
+    const double invr_mass = m->species_inverse_mass[spec];
+    const double constante = m->line_A[lg] * m->quad_weights[(size_t)spec * m->nquads + z] * w_ang;
+    const size_t pl = (size_t)p * m->nlines + lg;
+    double eta, chi;
+    get_eta_and_chi(m, op, p, k, freq_line, &eta, &chi);
+    double frq = m->nu[(size_t)p * m->nfreqs + f] * s->shift[c];
+    double phi = thermo_profile(m, invr_mass, p, freq_line, frq);
+    int err = lam_add(ll, pl, p, constante * frq * phi * s->L_diag[c] * (1.0 / chi));
+    for (uint32_t mm = 0; mm < op->n_off_diag && mm + 1 < s->n_tot; mm++) {
+        if (c >= first + mm + 1) {
+            const uint32_t n = c - mm - 1, q = s->nr[n];
+            get_eta_and_chi(m, op, q, k, freq_line, &eta, &chi);
+            frq = m->nu[(size_t)q * m->nfreqs + f] * s->shift[n];
+            phi = thermo_profile(m, invr_mass, q, freq_line, frq);
+            const double L =
+                constante * frq * phi * s->L_lower[(size_t)mm * s->length + n] * (1.0 / chi);
+            if (!err) err = lam_add(ll, pl, q, L);
+        }
+        if (c + mm + 1 <= last) {
+            const uint32_t n = c + mm + 1, q = s->nr[n];
+            get_eta_and_chi(m, op, q, k, freq_line, &eta, &chi);
+            frq = m->nu[(size_t)q * m->nfreqs + f] * s->shift[n];
+            phi = thermo_profile(m, invr_mass, q, freq_line, frq);
+            const double L =
+                constante * frq * phi * s->L_upper[(size_t)mm * s->length + n] * (1.0 / chi);
+            if (!err) err = lam_add(ll, pl, q, L);
+        }
+    }
+    return err;
+}
+
 /* ------------------------------------------------------------------------- */
 /* public entry points                                                       */
 
@@ -825,11 +943,10 @@ static int default_threads(int nthreads)
 #endif
 }
 
-int oracle_solve_feautrier_order_2(const mgx_model *m, const mgx_options *op, double *u, double *J,
-                                   double *lambda_diag, int nthreads)
+static int solve_common(const mgx_model *m, const mgx_options *op, double *u, double *J,
+                        double *lambda_diag, LamList *ll, int nthreads)
 {
-    int err = check_model(m, op);
-    if (err) return err;
+    int err = 0;
     const int nt = default_threads(nthreads);
 #ifdef _OPENMP
     const int saved = omp_get_max_threads();
@@ -842,11 +959,13 @@ int oracle_solve_feautrier_order_2(const mgx_model *m, const mgx_options *op, do
     const size_t PF = (size_t)m->npoints * m->nfreqs;
     memset(J, 0, PF * sizeof(double));
     if (lambda_diag) memset(lambda_diag, 0, (size_t)m->npoints * m->nlines * sizeof(double));
+    if (ll) memset(ll->count, 0, (size_t)m->npoints * m->nlines * sizeof(uint32_t));
 
 #pragma omp parallel
     {
         Scratch s;
         int serr = scratch_init(&s, length);
+        if (!serr && op->n_off_diag > 0) serr = scratch_bands(&s, op->n_off_diag);
         for (uint32_t rr = 0; rr < m->hnrays; rr++) {
             const uint32_t ar = m->antipod[rr];
 #pragma omp for schedule(static)
@@ -872,6 +991,10 @@ int oracle_solve_feautrier_order_2(const mgx_model *m, const mgx_options *op, do
                             if (u) u[(size_t)rr * PF + of] = uc;
                             J[of] += uc * 2.0 * ray_weight(m, o, rr);
                             if (lambda_diag) update_lambda(m, op, &s, rr, f, lambda_diag);
+                            if (ll) {
+                                const int le = update_lambda_list(m, op, &s, rr, f, ll);
+                                if (le) serr = le;
+                            }
                         } else {
                             const double ub = boundary_intensity(m, o, m->nu[of]);
                             if (u) u[(size_t)rr * PF + of] = ub;
@@ -892,6 +1015,24 @@ done:
     omp_set_num_threads(saved);
 #endif
     return err;
+}
+
+int oracle_solve_feautrier_order_2(const mgx_model *m, const mgx_options *op, double *u, double *J,
+                                   double *lambda_diag, int nthreads)
+{
+    int err = check_model(m, op);
+    if (err) return err;
+    return solve_common(m, op, u, J, lambda_diag, NULL, nthreads);
+}
+
+int oracle_solve_feautrier_order_2_lambda(const mgx_model *m, const mgx_options *op, double *u,
+                                          double *J, uint32_t cap, uint32_t *lam_count,
+                                          uint32_t *lam_nr, double *lam_val, int nthreads)
+{
+    if (!m || !op || !J || !lam_count || !lam_nr || !lam_val || cap == 0) return MGX_E_ARG;
+    if (m->npoints == 0 || m->nfreqs == 0) return MGX_E_ARG;
+    LamList ll = {cap, lam_count, lam_nr, lam_val};
+    return solve_common(m, op, u, J, NULL, &ll, nthreads);
 }
 
 /* ------------------------------------------------------------------------- */

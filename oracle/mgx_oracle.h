@@ -15,6 +15,13 @@ extern "C" {
 /* Model::compute_radiation_field_feautrier_order_2 (setup + solve). u may be NULL. */
 int oracle_solve_feautrier_order_2(const mgx_model *m, const mgx_options *op, double *u,
                                    double *J, double *lambda_diag, int nthreads);
+/* The same solve with the general Lambda operator (n_off_diag >= 0): for each
+   receiving point p and line l a list of at most cap (emitting point, value)
+   entries in the reference's order, lam_count (P, L), lam_nr and lam_val
+   (P, L, cap); MGX_E_ARG if a list would exceed cap. */
+int oracle_solve_feautrier_order_2_lambda(const mgx_model *m, const mgx_options *op, double *u,
+                                          double *J, uint32_t cap, uint32_t *lam_count,
+                                          uint32_t *lam_nr, double *lam_val, int nthreads);
 /* Model::compute_radiation_field_feautrier_order_2_uv: u and v (hnrays, P, F).
    v_scale (may be NULL) receives the sum of the absolute terms v is formed from
    (|coeffmin u_c-1| + |coeffplus u_c+1| + |coeffzero u_c|, or |I_bdy| + |u| at an

@@ -83,6 +83,29 @@ def shortchar(model, opts, min_dtau=1.0e-15, nthreads=0, kind="cr"):
     return I, u, v, J, lam
 
 
+def solve_lambda(model, opts, cap=None, with_u=False, nthreads=0, kind="cr"):
+    """The plain-method solve with the general Lambda operator (opts.n_off_diag >= 0):
+    returns (u or None, J, count (P, L), nr (P, L, cap), val (P, L, cap)); the list of
+    (p, l) is nr[p, l, :count], val[p, l, :count] in the reference's order
+    (Lambda::add_element).  cap defaults to 1 + 2 n_off_diag hnrays."""
+    m = model.view()
+    P, F, L, R = model.npoints, model.nfreqs, model.nlines, model.hnrays
+    cap = 1 + 2 * int(opts.n_off_diag) * R if cap is None else cap
+    u = np.zeros((R, P, F)) if with_u else None
+    J = np.zeros((P, F))
+    cnt = np.zeros((P, L), dtype=np.uint32)
+    nr = np.zeros((P, L, cap), dtype=np.uint32)
+    val = np.zeros((P, L, cap))
+    h = lib(kind)
+    h.oracle_solve_feautrier_order_2_lambda.argtypes = [
+        C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
+        C.c_void_p, C.c_int]
+    _check(h.oracle_solve_feautrier_order_2_lambda(
+        C.addressof(m), C.addressof(opts), u.ctypes.data if with_u else None, J.ctypes.data,
+        cap, cnt.ctypes.data, nr.ctypes.data, val.ctypes.data, nthreads), "solve_lambda")
+    return u, J, cnt, nr, val
+
+
 def feautrier_uv(model, opts, nthreads=0, kind="cr", with_scale=False):
     """Model::compute_radiation_field_feautrier_order_2_uv: (u, v), each (hnrays, P, F).
     with_scale=True also returns v_scale, the sum of the absolute terms each v is

@@ -21,6 +21,8 @@ import pytest
 from magritte_amd import models
 from oracle import oracle
 
+import _exact
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
@@ -57,49 +59,12 @@ def test_constant_slab_closed_form():
 
 def _exact_tridiag(sys_):
     """u and diag(T^-1) at the centre of the reference's second-order Feautrier system."""
-    mp.mp.dps = 50
-    n = sys_["n"]
-    dt = [mp.mpf(float(v)) for v in sys_["dtau"]]
-    S = [mp.mpf(float(v)) for v in sys_["S"]]
-    If, Il = (mp.mpf(float(v)) for v in sys_["I_bdy"])
-    a = [mp.mpf(0)] * n  # sub-diagonal (coefficient of u_{j-1})
-    b = [mp.mpf(0)] * n
-    c = [mp.mpf(0)] * n  # super-diagonal
-    r = [mp.mpf(0)] * n
-    Cf = 2 / dt[0] ** 2
-    b[0] = 1 + 2 / dt[0] + Cf
-    c[0] = -Cf
-    r[0] = S[0] + 2 * If / dt[0]
-    for j in range(1, n - 1):
-        avg = (dt[j - 1] + dt[j]) / 2
-        A = 1 / (avg * dt[j - 1])
-        C = 1 / (avg * dt[j])
-        a[j], b[j], c[j], r[j] = -A, 1 + A + C, -C, S[j]
-    Al = 2 / dt[n - 2] ** 2
-    a[n - 1] = -Al
-    b[n - 1] = 1 + 2 / dt[n - 2] + Al
-    r[n - 1] = S[n - 1] + 2 * Il / dt[n - 2]
-
-    def solve(rhs):
-        cp = [mp.mpf(0)] * n
-        dp = [mp.mpf(0)] * n
-        cp[0] = c[0] / b[0]
-        dp[0] = rhs[0] / b[0]
-        for j in range(1, n):
-            den = b[j] - a[j] * cp[j - 1]
-            cp[j] = c[j] / den if j < n - 1 else 0
-            dp[j] = (rhs[j] - a[j] * dp[j - 1]) / den
-        x = [mp.mpf(0)] * n
-        x[n - 1] = dp[n - 1]
-        for j in range(n - 2, -1, -1):
-            x[j] = dp[j] - cp[j] * x[j + 1]
-        return x
-
+    a, b, c, r = _exact.system(sys_)
     cidx = sys_["centre"]
-    u = solve(r)[cidx]
-    e = [mp.mpf(0)] * n
+    u = _exact.solve((a, b, c), r)[cidx]
+    e = [mp.mpf(0)] * len(b)
     e[cidx] = mp.mpf(1)
-    lam = solve(e)[cidx]
+    lam = _exact.solve((a, b, c), e)[cidx]
     return float(u), float(lam)
 
 
