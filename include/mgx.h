@@ -527,6 +527,27 @@ int mgx_nlte_iteration_stateq(mgx_nlte *h, const double *J, const double *lambda
 int mgx_nlte_iteration_ng(mgx_nlte *h, uint32_t order, double *emissivity, double *opacity,
                           double *interp_crit, mgx_nlte_conv *conv, void *stream);
 
+/*
+ * Lines::iteration_using_statistical_equilibrium (lines.cpp:124-135) with
+ * Model::compute_Jeff (model.cpp:514-545): J_eff from J (P,F) and the general
+ * Lambda lists of mgx_feautrier_forward_lambda_device (capacity cap), then the
+ * global rate equations of all points
+ * (LineProducingSpecies::update_using_statistical_equilibrium,
+ * lineProducingSpecies.tpp:377-581), the negative-population correction and the
+ * convergence statistics.  The reference factorises the whole sparse system
+ * (SparseLU in long double); here block-Jacobi sweeps solve every point's block
+ * (elimination with compensated refinement) with the other points' populations
+ * on the right-hand side until the largest change is at the rounding level;
+ * sweeps (may be NULL) receives their number.  With n_off_diag = 0 the system
+ * is block diagonal and one sweep is exact.  Fails with MGX_E_UNSUPPORTED if
+ * the sweeps do not converge within 100.
+ */
+int mgx_nlte_iteration_stateq_global(mgx_nlte *h, const double *J, uint32_t cap,
+                                     const uint32_t *lam_count, const uint32_t *lam_nr,
+                                     const double *lam_val, double *emissivity, double *opacity,
+                                     double *interp_crit, mgx_nlte_conv *conv, double *Jlin,
+                                     double *Jeff, uint32_t *sweeps, void *stream);
+
 /* Lines::trial_iteration_using_adaptive_Ng_acceleration (lines.cpp:119-124):
  * the extrapolation into the trial populations and check_for_convergence_trial
  * (lineProducingSpecies.tpp:167-201); the populations are unchanged. */
@@ -553,6 +574,18 @@ int mgx_compute_level_populations_sparse(const mgx_model *model_dev, const mgx_o
                                          double *opacity, double *interp_crit,
                                          int32_t *iterations, double *error_mean,
                                          double *error_max, void *stream);
+
+/*
+ * Model::compute_level_populations(use_Ng_acceleration, max_niterations)
+ * (model.cpp:653-744): as mgx_compute_level_populations_sparse, with the
+ * formal solve of mgx_feautrier_forward_lambda_device (opts->n_off_diag bands,
+ * up to MGX_MAX_OFF_DIAG) and mgx_nlte_iteration_stateq_global.
+ */
+int mgx_compute_level_populations(const mgx_model *model_dev, const mgx_options *opts,
+                                  mgx_nlte *h, int use_ng, int64_t max_niterations,
+                                  double *emissivity, double *opacity, double *interp_crit,
+                                  int32_t *iterations, double *error_mean, double *error_max,
+                                  void *stream);
 
 /*
  * Workload statistics of the trace, for the algorithmic-byte model of the

@@ -185,11 +185,17 @@ SIGNATURES = [
     ("mgx_nlte_get_populations", C.c_int, [_VP, _VP, _VP, _VP]),
     ("mgx_nlte_iteration_stateq", C.c_int,
      [_VP, _VP, _VP, _VP, _VP, _VP, C.POINTER(MgxNlteConv), _VP, _VP, _VP]),
+    ("mgx_nlte_iteration_stateq_global", C.c_int,
+     [_VP, _VP, C.c_uint32, _VP, _VP, _VP, _VP, _VP, _VP, C.POINTER(MgxNlteConv), _VP, _VP, _VP,
+      _VP]),
     ("mgx_nlte_iteration_ng", C.c_int,
      [_VP, C.c_uint32, _VP, _VP, _VP, C.POINTER(MgxNlteConv), _VP]),
     ("mgx_nlte_trial_ng", C.c_int, [_VP, C.c_uint32, C.POINTER(MgxNlteConv), _VP]),
     ("mgx_nlte_history", C.c_int, [_VP, _VP]),
     ("mgx_compute_level_populations_sparse", C.c_int,
+     [C.POINTER(MgxModel), C.POINTER(MgxOptions), _VP, C.c_int, C.c_int64, _VP, _VP, _VP,
+      _VP, _VP, _VP, _VP]),
+    ("mgx_compute_level_populations", C.c_int,
      [C.POINTER(MgxModel), C.POINTER(MgxOptions), _VP, C.c_int, C.c_int64, _VP, _VP, _VP,
       _VP, _VP, _VP, _VP]),
     ("mgx_trace_stats_device", C.c_int, [C.POINTER(MgxModel), C.POINTER(MgxOptions), _VP, _VP]),

@@ -273,13 +273,31 @@ __host__ __device__ inline size_t stateq_lds_doubles(uint32_t n, uint32_t maxnra
     return 2 * (size_t)n * n + n + 2 * (size_t)maxnrad + 2 * (size_t)n;
 }
 
+/*
+ * Inputs of the global statistical equilibrium (update_using_statistical_equilibrium,
+ * lineProducingSpecies.tpp:377-523): the general Lambda lists, the block-Jacobi iterate
+ * (population layout) and the diagonal terms the reference's approximated-Lambda elements
+ * put on the rows of other points (lambda_extra_kernel).  Each sweep solves every
+ * point's block with the other points' populations on the right-hand side.
+ */
+struct GlobalIn {
+    const uint32_t *count, *nr; /* (P, L), (P, L, cap) */
+    const double *val;          /* (P, L, cap) */
+    uint32_t cap;
+    const double *x_it;         /* iterate */
+    const double *dextra;       /* diagonal terms from other points' lists */
+    double *x_out;              /* this sweep's solution */
+    unsigned long long *dmax;   /* largest |x_out - x_it| / n_tot (bits of a positive double) */
+};
+
+template <bool GLOBAL>
 __global__ __launch_bounds__(NL_WAVE) void stateq_kernel(NView v, const double *__restrict__ J,
                                                          const double *__restrict__ lam,
                                                          const double *__restrict__ old_all,
                                                          double *cur_all, double *eps,
                                                          double *kap, double *ic,
                                                          double *Jlin_out, double *Jeff_out,
-                                                         double *part)
+                                                         double *part, GlobalIn g)
 {
     extern __shared__ double sm[];
     const uint32_t p = blockIdx.x, s = blockIdx.y, lane = threadIdx.x;
@@ -303,7 +321,19 @@ __global__ __launch_bounds__(NL_WAVE) void stateq_kernel(NView v, const double *
         double jl = 0.0;
         for (uint32_t z = 0; z < v.Q; z++) jl += w[z] * Jp[z];
         const double ni = old[S.irad[k]], nj = old[S.jrad[k]];
-        const double diff = lam[(size_t)p * v.L + lid] * ni;
+        double diff;
+        if constexpr (GLOBAL) {
+            /* Model::compute_Jeff (model.cpp:514-545): the list's elements times the upper
+               population at their emitting points */
+            const size_t pl = (size_t)p * v.L + lid;
+            diff = 0.0;
+            for (uint32_t m = 0; m < g.count[pl]; m++) {
+                const uint32_t q = g.nr[pl * g.cap + m];
+                diff += g.val[pl * g.cap + m] * old_all[S.pop_off + (size_t)q * n + S.irad[k]];
+            }
+        } else {
+            diff = lam[(size_t)p * v.L + lid] * ni;
+        }
         const double je = jl - MGX_HH_OVER_FOUR_PI * diff;
         jeff[k] = je;
         kold[k] = line_opacity(S, k, ni, nj, v.min_o);
@@ -315,6 +345,7 @@ __global__ __launch_bounds__(NL_WAVE) void stateq_kernel(NView v, const double *
 
     /* the rate matrix; lane r accumulates row r (column-major A0[c n + r]) */
     const uint32_t r = lane;
+    double yoff = 0.0; /* right-hand side of row r from other points (global mode) */
     if (r < n) {
         for (uint32_t k = 0; k < nrad; k++) {
             const uint32_t I = S.irad[k], Jl = S.jrad[k];
@@ -332,10 +363,28 @@ __global__ __launch_bounds__(NL_WAVE) void stateq_kernel(NView v, const double *
         }
         for (uint32_t k = 0; k < nrad; k++) {
             const uint32_t I = S.irad[k], Jl = S.jrad[k];
-            const double v_IJ = -lam[(size_t)p * v.L + S.lid0 + k] * kold[k] / S.freq[k];
-            if (r == Jl) A0[(size_t)I * n + r] += v_IJ;
-            if (r == I) A0[(size_t)I * n + r] -= v_IJ;
+            if constexpr (GLOBAL) {
+                /* elements at this point stay in the block; one at another point q couples
+                   row (p, lower) to that point's upper level: moved to the right-hand side
+                   with the iterate (its diagonal term on q's row is in dextra) */
+                const size_t pl = (size_t)p * v.L + S.lid0 + k;
+                for (uint32_t m = 0; m < g.count[pl]; m++) {
+                    const uint32_t q = g.nr[pl * g.cap + m];
+                    const double v_IJ = -g.val[pl * g.cap + m] * kold[k] / S.freq[k];
+                    if (q == p) {
+                        if (r == Jl) A0[(size_t)I * n + r] += v_IJ;
+                        if (r == I) A0[(size_t)I * n + r] -= v_IJ;
+                    } else if (r == Jl) {
+                        yoff -= v_IJ * g.x_it[S.pop_off + (size_t)q * n + I];
+                    }
+                }
+            } else {
+                const double v_IJ = -lam[(size_t)p * v.L + S.lid0 + k] * kold[k] / S.freq[k];
+                if (r == Jl) A0[(size_t)I * n + r] += v_IJ;
+                if (r == I) A0[(size_t)I * n + r] -= v_IJ;
+            }
         }
+        if constexpr (GLOBAL) A0[(size_t)r * n + r] += g.dextra[S.pop_off + (size_t)p * n + r];
         const double T = v.temperature[p];
         for (uint32_t c = 0; c < S.ncolpar; c++) {
             const NCol C = v.col[S.col0 + c];
@@ -371,7 +420,7 @@ __global__ __launch_bounds__(NL_WAVE) void stateq_kernel(NView v, const double *
     }
     __syncthreads();
     for (uint32_t i = lane; i < n * n; i += NL_WAVE) LU[i] = A0[i];
-    const double y0 = (r == n - 1) ? ptot : 0.0;
+    const double y0 = (r == n - 1) ? ptot : yoff;
     double yv = y0;
     __syncthreads();
 
@@ -440,7 +489,45 @@ __global__ __launch_bounds__(NL_WAVE) void stateq_kernel(NView v, const double *
         x += dx;
     }
     __syncthreads();
+    if constexpr (GLOBAL) {
+        double d = 0.0;
+        if (lane < n) {
+            const size_t i = S.pop_off + (size_t)p * n + lane;
+            g.x_out[i] = x;
+            d = fabs(x - g.x_it[i]) / ptot;
+            if (!(d == d)) d = __builtin_inf();
+        }
+        for (int off = 32; off >= 1; off >>= 1) {
+            const double o2 = __shfl_xor(d, off);
+            d = o2 > d ? o2 : d;
+        }
+        if (lane == 0) atomicMax(g.dmax, (unsigned long long)__double_as_longlong(d));
+        return;
+    }
     finish_point(v, S, p, s, x, old, cur_p, pv, eps, kap, ic, part);
+}
+
+/* the diagonal terms an approximated-Lambda element of the list of (p, k) at another point
+   q puts on row (q, upper) (lineProducingSpecies.tpp:455-458): + Ls opacity(p, k) / nu */
+__global__ void lambda_extra_kernel(NView v, GlobalIn g, const double *__restrict__ old_all,
+                                    double *dextra)
+{
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x, s = blockIdx.y;
+    if (p >= v.P) return;
+    const NSpec S = v.spec[s];
+    const uint32_t n = S.nlev;
+    const double *old = old_all + S.pop_off + (size_t)p * n;
+    for (uint32_t k = 0; k < S.nrad; k++) {
+        const uint32_t I = S.irad[k];
+        if (I == n - 1) continue;
+        const double kold = line_opacity(S, k, old[I], old[S.jrad[k]], v.min_o);
+        const size_t pl = (size_t)p * v.L + S.lid0 + k;
+        for (uint32_t m = 0; m < g.count[pl]; m++) {
+            const uint32_t q = g.nr[pl * g.cap + m];
+            if (q == p) continue;
+            atomicAdd(&dextra[S.pop_off + (size_t)q * n + I], g.val[pl * g.cap + m] * kold / S.freq[k]);
+        }
+    }
 }
 
 /* negative correction, convergence and line data after an Ng step */
@@ -767,6 +854,10 @@ struct mgx_nlte {
     double *gram_partial = nullptr;
     double *J = nullptr, *lambda = nullptr; /* driver buffers */
     size_t J_elems = 0, lambda_elems = 0;
+    double *gx = nullptr; /* global update: two iterates and the extra diagonal (3 nvec), dmax */
+    uint32_t *lam_count = nullptr, *lam_nr = nullptr; /* driver's Lambda lists */
+    double *lam_val = nullptr;
+    size_t lam_elems = 0, lam_count_elems = 0;
     hipStream_t stream = nullptr;
     std::vector<mgx_nlte_conv> conv;
 };
@@ -953,6 +1044,22 @@ void release(mgx_nlte *h)
     (void)hipFree(h->gram_partial);
     (void)hipFree(h->J);
     (void)hipFree(h->lambda);
+    (void)hipFree(h->gx);
+    (void)hipFree(h->lam_count);
+    (void)hipFree(h->lam_nr);
+    (void)hipFree(h->lam_val);
+}
+
+/* grow-only device buffer */
+template <typename T>
+int ensure(T *&p, size_t &have, size_t want)
+{
+    if (have >= want) return MGX_OK;
+    (void)hipFree(p);
+    p = nullptr;
+    NL_TRY(hipMalloc(&p, want * sizeof(T)));
+    have = want;
+    return MGX_OK;
 }
 
 } // namespace
@@ -1207,10 +1314,71 @@ int mgx_nlte_iteration_stateq(mgx_nlte *h, const double *J, const double *lam, d
     if (int e = push_history(h, st)) return e;
     const double *old = h->ring[h->hist.back()];
     const size_t lds = point_lds(h, true);
-    if (int e = set_lds_limit((const void *)stateq_kernel, lds)) return e;
-    hipLaunchKernelGGL(stateq_kernel, dim3(h->P, h->nlspecs), dim3(NL_WAVE), lds, st,
-                       make_view(h), J, lam, old, h->cur, eps, kap, ic, Jlin, Jeff, h->part);
+    if (int e = set_lds_limit((const void *)stateq_kernel<false>, lds)) return e;
+    hipLaunchKernelGGL(stateq_kernel<false>, dim3(h->P, h->nlspecs), dim3(NL_WAVE), lds, st,
+                       make_view(h), J, lam, old, h->cur, eps, kap, ic, Jlin, Jeff, h->part,
+                       GlobalIn{});
     NL_TRY(hipGetLastError());
+    if (int e = collect_conv(h, st)) return e;
+    if (conv) std::memcpy(conv, h->conv.data(), sizeof(mgx_nlte_conv) * h->nlspecs);
+    return MGX_OK;
+}
+
+int mgx_nlte_iteration_stateq_global(mgx_nlte *h, const double *J, uint32_t cap,
+                                     const uint32_t *lam_count, const uint32_t *lam_nr,
+                                     const double *lam_val, double *eps, double *kap, double *ic,
+                                     mgx_nlte_conv *conv, double *Jlin, double *Jeff,
+                                     uint32_t *sweeps, void *stream)
+{
+    mgx::clear_error();
+    if (!h || !J || !lam_count || !lam_nr || !lam_val || !eps || !kap || !ic)
+        return mgx::fail(MGX_E_ARG, "null argument");
+    if (cap < 1) return mgx::fail(MGX_E_ARG, "the Lambda lists need a capacity of at least 1");
+    if (h->hist.empty()) return mgx::fail(MGX_E_ARG, "no populations set (call mgx_nlte_set_lte first)");
+    hipStream_t st = (hipStream_t)stream;
+    if (!h->gx) NL_TRY(hipMalloc(&h->gx, (3 * h->nvec + 1) * sizeof(double)));
+    if (int e = push_history(h, st)) return e;
+    const double *old = h->ring[h->hist.back()];
+    double *xa = h->gx, *xb = h->gx + h->nvec, *dx = h->gx + 2 * h->nvec;
+    unsigned long long *dmax = reinterpret_cast<unsigned long long *>(h->gx + 3 * h->nvec);
+    NL_TRY(hipMemcpyAsync(xa, old, h->nvec * sizeof(double), hipMemcpyDeviceToDevice, st));
+    NL_TRY(hipMemsetAsync(dx, 0, h->nvec * sizeof(double), st));
+    GlobalIn g{lam_count, lam_nr, lam_val, cap, nullptr, dx, nullptr, dmax};
+    hipLaunchKernelGGL(lambda_extra_kernel, dim3((h->P + 63) / 64, h->nlspecs), dim3(64), 0, st,
+                       make_view(h), g, old, dx);
+    NL_TRY(hipGetLastError());
+    const size_t lds = point_lds(h, true);
+    if (int e = set_lds_limit((const void *)stateq_kernel<true>, lds)) return e;
+    /* block-Jacobi sweeps until the largest change is at the rounding level of the point
+       solves (below 1e-15 of the point's total, or no longer decreasing) */
+    double prev = __builtin_inf();
+    uint32_t it = 0;
+    const uint32_t max_sweeps = 100;
+    for (;;) {
+        g.x_it = xa;
+        g.x_out = xb;
+        NL_TRY(hipMemsetAsync(dmax, 0, sizeof(unsigned long long), st));
+        hipLaunchKernelGGL(stateq_kernel<true>, dim3(h->P, h->nlspecs), dim3(NL_WAVE), lds, st,
+                           make_view(h), J, nullptr, old, h->cur, eps, kap, ic,
+                           it == 0 ? Jlin : nullptr, it == 0 ? Jeff : nullptr, h->part, g);
+        NL_TRY(hipGetLastError());
+        unsigned long long bits = 0;
+        NL_TRY(hipMemcpyAsync(&bits, dmax, sizeof bits, hipMemcpyDeviceToHost, st));
+        NL_TRY(hipStreamSynchronize(st));
+        double d;
+        std::memcpy(&d, &bits, sizeof d);
+        std::swap(xa, xb);
+        it++;
+        if (d <= 1.0e-15 || (it >= 3 && d >= prev && d <= 1.0e-12)) break;
+        if (it >= max_sweeps || !(d < 1.0e300))
+            return mgx::fail(MGX_E_UNSUPPORTED,
+                             "the global statistical equilibrium did not converge by block-Jacobi "
+                             "sweeps (largest change %.3e of the total after %u sweeps)", d, it);
+        prev = d;
+    }
+    if (sweeps) *sweeps = it;
+    NL_TRY(hipMemcpyAsync(h->cur, xa, h->nvec * sizeof(double), hipMemcpyDeviceToDevice, st));
+    if (int e = launch_finish(h, old, eps, kap, ic, st)) return e;
     if (int e = collect_conv(h, st)) return e;
     if (conv) std::memcpy(conv, h->conv.data(), sizeof(mgx_nlte_conv) * h->nlspecs);
     return MGX_OK;
@@ -1281,32 +1449,34 @@ int mgx_nlte_trial_ng(mgx_nlte *h, uint32_t order, mgx_nlte_conv *conv, void *st
     return MGX_OK;
 }
 
-int mgx_compute_level_populations_sparse(const mgx_model *m, const mgx_options *opts, mgx_nlte *h,
-                                         int use_ng, int64_t max_niterations, double *eps,
-                                         double *kap, double *ic, int32_t *iterations,
-                                         double *error_mean, double *error_max, void *stream)
+} // extern "C"
+
+namespace {
+
+/* Model::compute_level_populations_sparse (model.cpp:756-850) or, with global,
+   Model::compute_level_populations (model.cpp:653-744) */
+int level_populations_loop(const mgx_model *m, const mgx_options *opts, mgx_nlte *h, int use_ng,
+                           int64_t max_niterations, double *eps, double *kap, double *ic,
+                           int32_t *iterations, double *error_mean, double *error_max,
+                           hipStream_t st, bool global)
 {
-    mgx::clear_error();
     if (!m || !opts || !h || !eps || !kap || !ic) return mgx::fail(MGX_E_ARG, "null argument");
     if (eps != m->emissivity || kap != m->opacity || ic != m->interp_crit)
         return mgx::fail(MGX_E_ARG, "emissivity, opacity and interp_crit must be the model view's arrays");
     if (m->npoints != h->P || m->nlines != h->L || m->nfreqs != h->F)
         return mgx::fail(MGX_E_ARG, "model view does not match the level-population handle");
-    if (opts->n_off_diag != 0)
+    if (!global && opts->n_off_diag != 0)
         return mgx::fail(MGX_E_ARG, "parameters->n_off_diag != 0 so cannot use sparse update.");
-    hipStream_t st = (hipStream_t)stream;
     const size_t nJ = (size_t)h->P * h->F, nL = (size_t)h->P * h->L;
-    if (h->J_elems < nJ) {
-        (void)hipFree(h->J);
-        h->J = nullptr;
-        NL_TRY(hipMalloc(&h->J, nJ * sizeof(double)));
-        h->J_elems = nJ;
-    }
-    if (h->lambda_elems < nL) {
-        (void)hipFree(h->lambda);
-        h->lambda = nullptr;
-        NL_TRY(hipMalloc(&h->lambda, nL * sizeof(double)));
-        h->lambda_elems = nL;
+    const uint32_t cap = 1 + 2 * opts->n_off_diag * m->hnrays;
+    if (int e = ensure(h->J, h->J_elems, nJ)) return e;
+    if (global) {
+        size_t have_n = h->lam_elems;
+        if (int e = ensure(h->lam_val, h->lam_elems, nL * cap)) return e;
+        if (int e = ensure(h->lam_nr, have_n, nL * cap)) return e;
+        if (int e = ensure(h->lam_count, h->lam_count_elems, nL)) return e;
+    } else {
+        if (int e = ensure(h->lambda, h->lambda_elems, nL)) return e;
     }
     const mgx_nlte_params &pr = h->prm;
     const uint32_t skip = pr.Ng_acceleration_remove_N_its;
@@ -1349,10 +1519,20 @@ int mgx_compute_level_populations_sparse(const mgx_model *m, const mgx_options *
         }
         iteration++;
         some_not_converged = false;
-        if (int e = mgx_feautrier_forward_device(m, opts, nullptr, h->J, h->lambda, st)) return e;
-        if (int e = mgx_nlte_iteration_stateq(h, h->J, h->lambda, eps, kap, ic, cv.data(), nullptr,
-                                              nullptr, st))
-            return e;
+        if (global) {
+            if (int e = mgx_feautrier_forward_lambda_device(m, opts, nullptr, h->J, cap,
+                                                            h->lam_count, h->lam_nr, h->lam_val, st))
+                return e;
+            if (int e = mgx_nlte_iteration_stateq_global(h, h->J, cap, h->lam_count, h->lam_nr,
+                                                         h->lam_val, eps, kap, ic, cv.data(),
+                                                         nullptr, nullptr, nullptr, st))
+                return e;
+        } else {
+            if (int e = mgx_feautrier_forward_device(m, opts, nullptr, h->J, h->lambda, st)) return e;
+            if (int e = mgx_nlte_iteration_stateq(h, h->J, h->lambda, eps, kap, ic, cv.data(),
+                                                  nullptr, nullptr, st))
+                return e;
+        }
         normal++;
         for (uint32_t s = 0; s < h->nlspecs; s++) {
             if (error_mean) error_mean[nerr] = cv[s].relative_change_mean;
@@ -1364,6 +1544,32 @@ int mgx_compute_level_populations_sparse(const mgx_model *m, const mgx_options *
     }
     if (iterations) *iterations = iteration;
     return MGX_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int mgx_compute_level_populations_sparse(const mgx_model *m, const mgx_options *opts, mgx_nlte *h,
+                                         int use_ng, int64_t max_niterations, double *eps,
+                                         double *kap, double *ic, int32_t *iterations,
+                                         double *error_mean, double *error_max, void *stream)
+{
+    mgx::clear_error();
+    return level_populations_loop(m, opts, h, use_ng, max_niterations, eps, kap, ic, iterations,
+                                  error_mean, error_max, (hipStream_t)stream, false);
+}
+
+int mgx_compute_level_populations(const mgx_model *m, const mgx_options *opts, mgx_nlte *h,
+                                  int use_ng, int64_t max_niterations, double *eps, double *kap,
+                                  double *ic, int32_t *iterations, double *error_mean,
+                                  double *error_max, void *stream)
+{
+    mgx::clear_error();
+    if (opts && opts->n_off_diag > MGX_MAX_OFF_DIAG)
+        return mgx::fail(MGX_E_UNSUPPORTED, "n_off_diag > %d is not supported", MGX_MAX_OFF_DIAG);
+    return level_populations_loop(m, opts, h, use_ng, max_niterations, eps, kap, ic, iterations,
+                                  error_mean, error_max, (hipStream_t)stream, true);
 }
 
 } // extern "C"

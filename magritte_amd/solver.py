@@ -385,6 +385,26 @@ class LevelPopulations:
                                                         _abi.ptr(Je), self.dm.stream_ptr()))
         return (self._conv(conv), Jl, Je) if want_J else self._conv(conv)
 
+    def stateq_global(self, J, lam, want_J=False):
+        """One global statistical-equilibrium iteration (Lines::iteration_using_statistical_
+        equilibrium with Model::compute_Jeff) from J (P, F) and the Lambda lists ``lam`` of
+        feautrier_forward_lambda (a dict with count, nr, val); returns the convergence
+        statistics per species and the number of block-Jacobi sweeps (and J_lin, J_eff)."""
+        torch = _require_gpu()
+        m = self.dm.model
+        conv = (_abi.MgxNlteConv * len(self.nlev))()
+        sweeps = np.zeros(1, dtype=np.uint32)
+        Jl = Je = None
+        if want_J:
+            Jl = torch.empty((m.npoints, m.nlines), dtype=torch.float64, device=self.dm.device)
+            Je = torch.empty_like(Jl)
+        _abi.check(_abi.lib().mgx_nlte_iteration_stateq_global(
+            self.h, _abi.ptr(J), int(lam["nr"].shape[2]), _abi.ptr(lam["count"]),
+            _abi.ptr(lam["nr"]), _abi.ptr(lam["val"]), *self._eoi(), conv, _abi.ptr(Jl),
+            _abi.ptr(Je), sweeps.ctypes.data, self.dm.stream_ptr()))
+        out = (self._conv(conv), int(sweeps[0]))
+        return out + (Jl, Je) if want_J else out
+
     def ng(self, order):
         conv = (_abi.MgxNlteConv * len(self.nlev))()
         _abi.check(_abi.lib().mgx_nlte_iteration_ng(self.h, order, *self._eoi(), conv,
@@ -410,6 +430,23 @@ class LevelPopulations:
         n = np.zeros(1, dtype=np.uint32)
         _abi.check(_abi.lib().mgx_nlte_history(self.h, n.ctypes.data))
         return int(n[0])
+
+
+def compute_level_populations(dm, opts, lp, use_ng=True, max_iterations=100):
+    """Model::compute_level_populations on the device: the loop with the general Lambda
+    operator (opts.n_off_diag bands) and the global update; returns (iterations,
+    error_mean, error_max)."""
+    m = dm.model
+    err_mean = np.zeros(max(1, max_iterations) * m.nlspecs)
+    err_max = np.zeros_like(err_mean)
+    it = np.zeros(1, dtype=np.int32)
+    t = dm.tensors
+    _abi.check(_abi.lib().mgx_compute_level_populations(
+        C.byref(dm.view), C.byref(opts), lp.h, 1 if use_ng else 0, int(max_iterations),
+        _abi.ptr(t["emissivity"]), _abi.ptr(t["opacity"]), _abi.ptr(t["interp_crit"]),
+        it.ctypes.data, err_mean.ctypes.data, err_max.ctypes.data, dm.stream_ptr()))
+    n = int(it[0]) * m.nlspecs
+    return int(it[0]), err_mean[:n].tolist(), err_max[:n].tolist()
 
 
 def compute_level_populations_sparse(dm, opts, lp, use_ng=True, max_iterations=100):

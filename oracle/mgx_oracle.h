@@ -98,6 +98,19 @@ int oracle_nlte_stats(const oracle_nlte *h, double *conv, uint32_t *n_residuals)
 /* one point's statistical-equilibrium system (column-major nlev x nlev) and right side */
 int oracle_nlte_stateq_system(const oracle_nlte *h, const double *lambda_diag, uint32_t l,
                               uint32_t p, double *M, double *y);
+/* Model::compute_Jeff with the general Lambda operator: J (P, F) and the lists of
+   oracle_solve_feautrier_order_2_lambda */
+int oracle_nlte_compute_Jeff_general(oracle_nlte *h, const double *J, uint32_t cap,
+                                     const uint32_t *lam_count, const uint32_t *lam_nr,
+                                     const double *lam_val);
+/* the global statistical-equilibrium update of all points (dense long-double solve) */
+int oracle_nlte_stateq_global(oracle_nlte *h, uint32_t cap, const uint32_t *lam_count,
+                              const uint32_t *lam_nr, const double *lam_val, double *eps,
+                              double *kap, double *ic);
+/* its system for species l: M ((P nlev)^2, column-major) and y */
+int oracle_nlte_stateq_global_system(const oracle_nlte *h, uint32_t cap,
+                                     const uint32_t *lam_count, const uint32_t *lam_nr,
+                                     const double *lam_val, uint32_t l, double *M, double *y);
 /* long-double column-pivoted QR solve of an n x n column-major system */
 int oracle_cpqr_solve(int n, const double *a_colmajor, const double *b, double *x);
 

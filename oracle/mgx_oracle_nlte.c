@@ -632,6 +632,108 @@ static void assemble_point(const oracle_nlte *h, const ospec *s, const double *l
     y[n - 1] = s->pop_tot[p];
 }
 
+/* the general Lambda operator: per (point, global line) at most cap (point, value) entries */
+typedef struct {
+    uint32_t cap;
+    const uint32_t *count, *nr;
+    const double *val;
+} lamview;
+
+/* Model::compute_Jeff (model.cpp:514-545): J_lin = sum_z w_z J(p, line bin z), J_eff =
+   J_lin - h/4pi sum over the list of (p, k) of Ls n_upper(emitting point) */
+int oracle_nlte_compute_Jeff_general(oracle_nlte *h, const double *J, uint32_t cap,
+                                     const uint32_t *lam_count, const uint32_t *lam_nr,
+                                     const double *lam_val)
+{
+    const uint32_t P = h->P, F = h->F, Q = h->Q;
+    for (uint32_t l = 0; l < h->nlspecs; l++) {
+        ospec *s = &h->s[l];
+        for (uint32_t p = 0; p < P; p++)
+            for (uint32_t k = 0; k < s->nrad; k++) {
+                const uint32_t lid = s->lid0 + k;
+                double Jlin = 0.0;
+                for (uint32_t z = 0; z < Q; z++)
+                    Jlin += h->quad_weights[(size_t)l * Q + z] * J[(size_t)p * F + (size_t)lid * Q + z];
+                const size_t pl = (size_t)p * h->L + lid;
+                double diff = 0.0;
+                for (uint32_t m = 0; m < lam_count[pl]; m++) {
+                    const uint32_t q = lam_nr[pl * cap + m];
+                    diff = (double)(diff + lam_val[pl * cap + m] * s->pop[(size_t)q * s->nlev + s->irad[k]]);
+                }
+                s->Jlin[(size_t)p * s->nrad + k] = Jlin;
+                s->Jeff[(size_t)p * s->nrad + k] = Jlin - N_HH_OVER_FOUR_PI * diff;
+            }
+    }
+    return 0;
+}
+
+/* the global rate equations of update_using_statistical_equilibrium
+   (lineProducingSpecies.tpp:377-523): triplets summed into a dense column-major
+   (P nlev)^2 long-double matrix in the order they are pushed; rows of a point's last level
+   skip every term and hold the normalisation.  An approximated-Lambda element of the list
+   of (p, k) at emitting point q couples row (p, lower) to column (q, upper) and adds to the
+   diagonal of row (q, upper), as the reference does. */
+static void assemble_global(const oracle_nlte *h, const ospec *s, const lamview *lv, ld *M,
+                            ld *y, double *Cd, double *Ce)
+{
+    const uint32_t n = s->nlev;
+    const size_t N = (size_t)h->P * n;
+    for (size_t i = 0; i < N * N; i++) M[i] = 0.0L;
+    for (size_t i = 0; i < N; i++) y[i] = 0.0L;
+#define G_(r, c) M[(size_t)(c) * N + (r)]
+    for (uint32_t p = 0; p < h->P; p++) {
+        const size_t b = (size_t)p * n;
+        for (uint32_t k = 0; k < s->nrad; k++) {
+            const double Jeff = s->Jeff[(size_t)p * s->nrad + k];
+            const ld v_IJ = s->A[k] + s->Bs[k] * Jeff;
+            const ld v_JI = s->Ba[k] * Jeff;
+            const size_t I = b + s->irad[k], J = b + s->jrad[k];
+            if (s->jrad[k] != n - 1) {
+                G_(J, I) += v_IJ;
+                G_(J, J) -= v_JI;
+            }
+            if (s->irad[k] != n - 1) {
+                G_(I, J) += v_JI;
+                G_(I, I) -= v_IJ;
+            }
+        }
+        for (uint32_t k = 0; k < s->nrad; k++) {
+            const double freq = s->frequency[k];
+            const size_t pl = (size_t)p * h->L + s->lid0 + k;
+            for (uint32_t m = 0; m < lv->count[pl]; m++) {
+                const uint32_t q = lv->nr[pl * lv->cap + m];
+                const ld v_IJ = -lv->val[pl * lv->cap + m] * get_opacity(h, s, p, k) / freq;
+                const size_t I = (size_t)q * n + s->irad[k], J = b + s->jrad[k];
+                if (s->jrad[k] != n - 1) G_(J, I) += v_IJ;
+                if (s->irad[k] != n - 1) G_(I, I) -= v_IJ;
+            }
+        }
+        for (uint32_t c = 0; c < s->ncolpar; c++) {
+            const ocol *o = &s->cp[c];
+            double abn = h->abundance[(size_t)p * h->nspec + o->num_col_partner];
+            const double tmp = h->temperature[p];
+            abn = ortho_para(o->orth_or_para, tmp, abn);
+            collision_rates(o, tmp, Cd, Ce);
+            for (uint32_t k = 0; k < o->ncol; k++) {
+                const ld v_IJ = Cd[k] * abn;
+                const ld v_JI = Ce[k] * abn;
+                const size_t I = b + o->icol[k], J = b + o->jcol[k];
+                if (o->jcol[k] != n - 1) {
+                    G_(J, I) += v_IJ;
+                    G_(J, J) -= v_JI;
+                }
+                if (o->icol[k] != n - 1) {
+                    G_(I, J) += v_JI;
+                    G_(I, I) -= v_IJ;
+                }
+            }
+        }
+        for (uint32_t i = 0; i < n; i++) G_(b + n - 1, b + i) += 1.0L;
+        y[b + n - 1] = s->pop_tot[p];
+    }
+#undef G_
+}
+
 static uint32_t max_ncol(const ospec *s)
 {
     uint32_t mc = 1;
@@ -810,6 +912,62 @@ int oracle_nlte_stateq(oracle_nlte *h, const double *lambda_diag, double *eps, d
     }
     set_emissivity_and_opacity(h, eps, kap, ic);
     return 0;
+}
+
+/* Lines::iteration_using_statistical_equilibrium (lines.cpp:124-135) with the global
+   update_using_statistical_equilibrium: one long-double solve of all points' rate
+   equations (the reference: SparseLU; here the column-pivoted QR of the dense matrix, for
+   the small models of the tests), then the negative-population correction and the
+   convergence statistics */
+int oracle_nlte_stateq_global(oracle_nlte *h, uint32_t cap, const uint32_t *lam_count,
+                              const uint32_t *lam_nr, const double *lam_val, double *eps,
+                              double *kap, double *ic)
+{
+    const lamview lv = {cap, lam_count, lam_nr, lam_val};
+    for (uint32_t l = 0; l < h->nlspecs; l++) {
+        ospec *s = &h->s[l];
+        const size_t N = (size_t)h->P * s->nlev;
+        if (s->npops == 0) return -2;
+        memcpy(s->prev1, s->pop, N * sizeof(ld));
+        if (push_history(s, N)) return -1;
+        const uint32_t mc = max_ncol(s);
+        ld *M = malloc(N * N * sizeof(ld)), *y = malloc(N * sizeof(ld)), *x = malloc(N * sizeof(ld));
+        double *Cd = malloc(mc * sizeof(double)), *Ce = malloc(mc * sizeof(double));
+        int err = (!M || !y || !x || !Cd || !Ce) ? -1 : 0;
+        if (!err) {
+            assemble_global(h, s, &lv, M, y, Cd, Ce);
+            err = oracle_cpqr_solve_ld((int)N, M, y, x);
+            if (!err) memcpy(s->pop, x, N * sizeof(ld));
+        }
+        free(M); free(y); free(x); free(Cd); free(Ce);
+        if (err) return err;
+        correct_negative_populations(h, s);
+        check_for_convergence(h, s);
+    }
+    set_emissivity_and_opacity(h, eps, kap, ic);
+    return 0;
+}
+
+/* the global system of species l for tests: M ((P nlev)^2, column-major) and y */
+int oracle_nlte_stateq_global_system(const oracle_nlte *h, uint32_t cap,
+                                     const uint32_t *lam_count, const uint32_t *lam_nr,
+                                     const double *lam_val, uint32_t l, double *M_out,
+                                     double *y_out)
+{
+    const lamview lv = {cap, lam_count, lam_nr, lam_val};
+    const ospec *s = &h->s[l];
+    const size_t N = (size_t)h->P * s->nlev;
+    const uint32_t mc = max_ncol(s);
+    ld *M = malloc(N * N * sizeof(ld)), *y = malloc(N * sizeof(ld));
+    double *Cd = malloc(mc * sizeof(double)), *Ce = malloc(mc * sizeof(double));
+    int err = (!M || !y || !Cd || !Ce) ? -1 : 0;
+    if (!err) {
+        assemble_global(h, s, &lv, M, y, Cd, Ce);
+        for (size_t i = 0; i < N * N; i++) M_out[i] = (double)M[i];
+        for (size_t i = 0; i < N; i++) y_out[i] = (double)y[i];
+    }
+    free(M); free(y); free(Cd); free(Ce);
+    return err;
 }
 
 static int ng_coefficients(const ospec *s, size_t n, uint32_t order, ld *coef)

@@ -272,6 +272,9 @@ def _nlte_sigs(h):
     h.oracle_nlte_trial_ng.argtypes = [vp, C.c_uint32]
     h.oracle_nlte_stateq_system.argtypes = [vp, vp, C.c_uint32, C.c_uint32, vp, vp]
     h.oracle_cpqr_solve.argtypes = [C.c_int, vp, vp, vp]
+    h.oracle_nlte_compute_Jeff_general.argtypes = [vp, vp, C.c_uint32, vp, vp, vp]
+    h.oracle_nlte_stateq_global.argtypes = [vp, C.c_uint32, vp, vp, vp, vp, vp, vp]
+    h.oracle_nlte_stateq_global_system.argtypes = [vp, C.c_uint32, vp, vp, vp, C.c_uint32, vp, vp]
 
 
 def cpqr_solve(A, b, kind="cr"):
@@ -347,6 +350,34 @@ class NLTE:
         _check(self.h_lib.oracle_nlte_stateq(self.h, self._lam.ctypes.data, *self._eoi()),
                "nlte_stateq")
 
+    def compute_Jeff_general(self, J, lists):
+        """Model::compute_Jeff: J (P, F) and the Lambda lists (count, nr, val) of
+        solve_lambda."""
+        J = np.ascontiguousarray(J, dtype=np.float64)
+        cnt, nr, val = (np.ascontiguousarray(a) for a in lists)
+        _check(self.h_lib.oracle_nlte_compute_Jeff_general(
+            self.h, J.ctypes.data, nr.shape[2], cnt.ctypes.data, nr.ctypes.data, val.ctypes.data),
+            "nlte_compute_Jeff_general")
+
+    def stateq_global(self, lists):
+        """Lines::iteration_using_statistical_equilibrium: the global update of all points."""
+        self._lists = tuple(np.ascontiguousarray(a) for a in lists)
+        cnt, nr, val = self._lists
+        _check(self.h_lib.oracle_nlte_stateq_global(self.h, nr.shape[2], cnt.ctypes.data,
+                                                    nr.ctypes.data, val.ctypes.data, *self._eoi()),
+               "nlte_stateq_global")
+
+    def global_system(self, lists, l):
+        """The global rate equations of species l: (M (N, N), y), N = P nlev."""
+        cnt, nr, val = (np.ascontiguousarray(a) for a in lists)
+        N = self.nlev[l] * self.model.npoints
+        M = np.zeros(N * N)
+        y = np.zeros(N)
+        _check(self.h_lib.oracle_nlte_stateq_global_system(
+            self.h, nr.shape[2], cnt.ctypes.data, nr.ctypes.data, val.ctypes.data, l,
+            M.ctypes.data, y.ctypes.data), "nlte_stateq_global_system")
+        return M.reshape(N, N).T.copy(), y
+
     def ng(self, order):
         _check(self.h_lib.oracle_nlte_ng(self.h, order, *self._eoi()), "nlte_ng")
 
@@ -399,6 +430,40 @@ def ng_decision(nlte, params, use_ng, prior):
     if prior == params.Ng_acceleration_mem_limit + skip:
         return True, params.Ng_acceleration_mem_limit
     return False, 0
+
+
+def compute_level_populations(model, opts, params, use_ng=True, max_iterations=100, nlte=None,
+                              nthreads=0, kind="cr"):
+    """Model::compute_level_populations (model.cpp:653-744) on the CPU: each iteration the
+    formal solve with the general Lambda operator (opts.n_off_diag bands), compute_Jeff and
+    the global statistical-equilibrium update; Ng steps per the criterion.  Starts from LTE
+    unless an NLTE state is given.  Returns (nlte, iterations, error_mean, error_max)."""
+    if nlte is None:
+        nlte = NLTE(model, params, kind)
+        nlte.set_lte()
+    iteration = 0
+    normal = 0
+    err_mean, err_max = [], []
+    some_not_converged = True
+    while some_not_converged and iteration < max_iterations:
+        use_step, order = ng_decision(nlte, params, use_ng, normal)
+        if use_step:
+            nlte.ng(order)
+            normal = 0
+            continue
+        iteration += 1
+        some_not_converged = False
+        _, J, cnt, nr, val = solve_lambda(model, opts, nthreads=nthreads, kind=kind)
+        nlte.compute_Jeff_general(J, (cnt, nr, val))
+        nlte.stateq_global((cnt, nr, val))
+        normal += 1
+        conv = nlte.stats()[0]
+        for l in range(model.nlspecs):
+            err_mean.append(float(conv[l, 1]))
+            err_max.append(float(conv[l, 2]))
+            if conv[l, 0] > 1.0 - params.convergence_fraction:
+                some_not_converged = True
+    return nlte, iteration, err_mean, err_max
 
 
 def compute_level_populations_sparse(model, opts, params, use_ng=True, max_iterations=100,
