@@ -36,12 +36,17 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--workload", default="wind32",
-                    choices=["wind32", "batch", "image-gradient", "nlte"],
+                    choices=["wind32", "batch", "image-gradient", "nlte", "siblings"],
                     help="wind32: the single-model solve (default); batch: many winds in one "
                          "persistent launch against per-model launches; image-gradient: the "
                          "gradient of an image loss through the adjoint; nlte: non-LTE level "
-                         "populations of a clumpy wind with a 41-level rotor")
+                         "populations of a clumpy wind with a 41-level rotor; siblings: the "
+                         "other per-ray solvers (short characteristics, u/v, Lambda bands, "
+                         "old imager, optical-depth images) on the 32^3 wind")
     ap.add_argument("--max-iterations", type=int, default=60)
+    ap.add_argument("--n-off-diag", type=int, default=0,
+                    help="nlte workload: > 0 runs the non-sparse loop with this many Lambda "
+                         "bands and the global statistical equilibrium")
     ap.add_argument("--n-models", type=int, default=64)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 512 MiB three times (scale of rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -330,9 +335,16 @@ def run_nlte(args):
     lp = solver.LevelPopulations(dm, prm)
     lp.set_lte()
     opts = m.options()
+    noff = args.n_off_diag
+    opts.n_off_diag = noff
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    it, em, ex = solver.compute_level_populations_sparse(dm, opts, lp, True, args.max_iterations)
+    loop = solver.compute_level_populations if noff > 0 else solver.compute_level_populations_sparse
+    # with bands the reference's global rate equations put each off-point element's loss on
+    # the emitting point's row; on this wind its loop diverges from the second iteration on
+    # (J_eff < 0 by orders of magnitude), so the path is measured over its first iteration
+    max_it = 1 if noff > 0 else args.max_iterations
+    it, em, ex = loop(dm, opts, lp, True, max_it)
     torch.cuda.synchronize()
     t_loop = time.perf_counter() - t1
     print(f"# loop: {it} iterations in {t_loop:.2f}s, last max change {ex[-1]:.3e}",
@@ -341,18 +353,25 @@ def run_nlte(args):
     lib = _abi.lib()
     lib.mgx_set_kernel_timing(1)
     t2 = time.perf_counter()
-    out = solver.feautrier_forward(dm, opts, with_u=False)
+    if noff > 0:
+        out = solver.feautrier_forward_lambda(dm, opts)
+    else:
+        out = solver.feautrier_forward(dm, opts, with_u=False)
     torch.cuda.synchronize()
     t_fs = time.perf_counter() - t2
     fs_k = lib.mgx_last_kernel_ms()
     pop0, _ = lp.populations()
     pop0 = pop0.cpu().numpy()
     ms = []
-    for rep in range(3):  # the update twice from the same state: restore it in between
+    sweeps = None
+    for rep in range(3):  # the update from the same state: restore it in between
         lp.set_populations(pop0)
         torch.cuda.synchronize()
         t3 = time.perf_counter()
-        conv = lp.stateq(out["J"], out["lambda_diag"])
+        if noff > 0:
+            conv, sweeps = lp.stateq_global(out["J"], out)
+        else:
+            conv = lp.stateq(out["J"], out["lambda_diag"])
         torch.cuda.synchronize()
         ms.append((time.perf_counter() - t3) * 1e3)
     lib.mgx_set_kernel_timing(0)
@@ -361,6 +380,9 @@ def run_nlte(args):
     # bytes the update must move: J (P, F) and Lambda (P, L) in, old populations in,
     # new populations and emissivity / opacity / interpolation criterion out
     se_bytes = 8 * (P * m.nfreqs + P * L + 2 * P * nlev + 3 * P * L)
+    if noff > 0:  # the lists replace the diagonal: count, points and values of every list
+        cap = out["nr"].shape[2]
+        se_bytes += 8 * P * L * (2 * cap + 1) - 8 * P * L
     res = {"metric": "non-LTE statistical-equilibrium update, point*level/s",
            "value": round(P * nlev / (t_se * 1e-3), 1), "unit": "point*level/s",
            "n_gpus": 1, "higher_is_better": True, "data": "synthetic", "dtype": "f64",
@@ -370,15 +392,21 @@ def run_nlte(args):
                       "iterations": it, "loop_s": round(t_loop, 3),
                       "loop_s_per_iteration": round(t_loop / max(1, it), 4),
                       "last_max_change": ex[-1] if ex else None,
-                      "converged": bool(it < args.max_iterations),
+                      "converged": bool(it < max_it),
                       "formal_solve_ms": round(t_fs * 1e3, 3), "formal_solve_kernel_ms": round(fs_k, 3),
                       "stateq_ms": round(t_se, 3), "stateq_ms_all": [round(x, 3) for x in ms],
-                      "stateq_fraction_of_iteration": round(t_se / (t_se + t_fs * 1e3), 4)},
+                      "stateq_fraction_of_iteration": round(t_se / (t_se + t_fs * 1e3), 4),
+                      "n_off_diag": noff,
+                      "loop": "compute_level_populations (global)" if noff > 0
+                              else "compute_level_populations_sparse",
+                      "block_jacobi_sweeps": sweeps},
            "roofline": {"bound": "hbm", "achieved": round(se_bytes / (t_se * 1e-3) / 1e9, 1),
                         "peak": 8000.0, "unit": "GB/s",
                         "frac": round(se_bytes / (t_se * 1e-3) / 8e12, 5), "traffic": None,
                         "kernel": "stateq_kernel (with its host launch and statistics read-back)"}}
-    if not args.no_cpu_baseline:
+    if noff > 0:
+        res["cpu_baseline"] = None  # the oracle's global update is a dense solve (small models)
+    if not args.no_cpu_baseline and noff == 0:
         from oracle import oracle
 
         # the oracle's long-double update of the same state from the same J and Lambda
@@ -423,6 +451,67 @@ def run_nlte(args):
     print(json.dumps(res), flush=True)
 
 
+def run_siblings(args):
+    """The sibling per-ray solvers on the default 32^3 wind (FP64, CloseLines): the plain
+    forward solve for reference, order-0 short characteristics, the Feautrier u/v solve, the
+    forward solve with two Lambda bands, the old imager along ray 0 and its optical depth
+    (every point), and the 256^2 x 31 optical-depth image of the projection surface.  Kernel
+    times from HIP events, the best of `steps` after `warmup` calls; throughput in the unit of
+    each solver's output."""
+    import numpy as np
+    import torch
+
+    from magritte_amd import _abi, models, solver
+
+    m = models.model_wind(n_edge=args.n_edge)
+    dm = solver.DeviceModel(m, "cuda:0")
+    lib = _abi.lib()
+    lib.mgx_set_kernel_timing(1)
+    op = m.options()
+    opb = m.options()
+    opb.n_off_diag = 2
+    nu0 = m.line[0]
+    fr = m.image_frequencies(nu0 * (1 - 1e-4), nu0 * (1 + 1e-4), 31)
+    rd = np.array([0.3, -0.5, 0.81])
+    U = m.hnrays * m.npoints * m.nfreqs
+    cases = [
+        ("feautrier_order_2", lambda: solver.feautrier_forward(dm, op, with_u=False), U, "ray*point*freq/s"),
+        ("shortchar_order_0", lambda: solver.shortchar_forward(dm, op, with_I=False), U, "ray*point*freq/s"),
+        ("feautrier_order_2_uv", lambda: solver.feautrier_uv(dm, op), U, "ray*point*freq/s"),
+        ("feautrier_order_2_lambda_bands_2", lambda: solver.feautrier_forward_lambda(dm, opb), U,
+         "ray*point*freq/s"),
+        ("image_old_ray0", lambda: solver.image_ray(dm, op, 0), m.npoints * m.nfreqs, "point*freq/s"),
+        ("image_optical_depth_ray0", lambda: solver.image_ray(dm, op, 0, optical_depth=True),
+         m.npoints * m.nfreqs, "point*freq/s"),
+        ("image_optical_depth_256x256", lambda: solver.image_optical_depth_new(dm, op, rd, 256, 256, fr),
+         256 * 256 * 31, "pixel*freq/s"),
+        ("image_new_256x256", lambda: solver.image_new(dm, op, rd, 256, 256, fr), 256 * 256 * 31,
+         "pixel*freq/s"),
+    ]
+    var = {}
+    for name, fn, units, unit in cases:
+        for _ in range(args.warmup):
+            fn()
+        ks = []
+        for _ in range(max(1, args.steps)):
+            fn()
+            torch.cuda.synchronize()
+            ks.append(lib.mgx_last_kernel_ms())
+        k = min(ks)
+        var[name] = {"kernel_ms": round(k, 3), "value": round(units / (k * 1e-3), 1), "unit": unit}
+        print(f"# {name}: {k:.3f} ms", file=sys.stderr, flush=True)
+    lib.mgx_set_kernel_timing(0)
+    base = var["feautrier_order_2"]["kernel_ms"]
+    for v in var.values():
+        v["cost_vs_feautrier"] = round(v["kernel_ms"] / base, 3)
+    res = {"metric": "sibling per-ray solvers, kernel time", "value": var["shortchar_order_0"]["value"],
+           "unit": "ray*point*freq/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "higher_is_better": True, "data": "synthetic", "dtype": "f64",
+           "config": {"workload": f"{args.n_edge}^3 jittered spherical wind, 12 rays, 9 lines x 31 bins, "
+                                  "FP64 CloseLines", "solvers": var}}
+    print(json.dumps(res), flush=True)
+
+
 def main():
     args = parse()
     if args.workload == "nlte":
@@ -436,6 +525,12 @@ def main():
         return
     if args.workload == "image-gradient":
         run_image_gradient(args)
+        return
+    if args.workload == "siblings":
+        if args.gpus != 1:
+            print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+            sys.exit(2)
+        run_siblings(args)
         return
     if args.workload == "batch":
         if args.gpus != 1:

@@ -101,3 +101,25 @@ def test_nonsparse_loop_converges_like_the_sparse_one():
     nl, it1, em, ex = oracle.compute_level_populations(m, _opts(m, 2), prm, True, 60)
     assert it1 < 60 and it1 <= it0 + 1, (it0, it1)
     assert ex[-1] < 1e-5
+
+
+def test_reference_band_loop_diverges_on_a_clumpy_wind():
+    """A property of the reference's method, kept in the restatement: with bands, the
+    global rate equations add each off-point element's loss term to the diagonal of the
+    emitting point's row (lineProducingSpecies.tpp:455-458).  On a clumpy rotating wind the
+    loop then diverges: by the third iteration J_eff is negative by orders of magnitude and
+    the largest relative change stays at its bound of 2 (populations flip sign and are
+    corrected).  The device's block-Jacobi sweeps stop with an error there."""
+    clumps = models.grf_clumps(5, sigma=0.5, corr_length=1.0 / 20.0)
+    m = models.model_wind(n_edge=5, nrays=48, nlev=10, v_rot=5.0e3, clumps=clumps)
+    nl = oracle.NLTE(m, _abi.default_nlte_params(pop_prec=1e-4))
+    nl.set_lte()
+    worst = 0.0
+    for it in range(3):
+        _, J, cnt, nr, val = oracle.solve_lambda(m, _opts(m, 2))
+        nl.compute_Jeff_general(J, (cnt, nr, val))
+        pj = nl.populations()
+        worst = min(worst, float(np.min(pj["J_eff"] / pj["J_lin"])))
+        nl.stateq_global((cnt, nr, val))
+    assert worst < -1e3, worst
+    assert nl.stats()[0][0, 2] > 1.0
