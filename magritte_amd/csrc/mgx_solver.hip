@@ -998,25 +998,34 @@ struct LamBands {
     uint32_t cap, noff;
 };
 
-/* Lambda::add_element (lambda.tpp): add to the entry of point q, else append */
+/* Lambda::add_element (lambda.tpp) by the whole wave (q, v uniform): the lanes compare 64
+   entries of the list at a time; the first entry of point q gets v added, else lane 0
+   appends.  Only this wave writes the list; the fence orders lane 0's writes before the
+   next merge's reads by the other lanes. */
 __device__ inline void lam_merge(const KParams &kp, const LamBands &lb, size_t pl, uint32_t q,
-                                 double v)
+                                 double v, uint32_t lane)
 {
     const size_t base = pl * lb.cap;
     const uint32_t n = lb.count[pl];
-    for (uint32_t i = 0; i < n; i++) {
-        if (lb.nr[base + i] == q) {
-            lb.val[base + i] += v;
+    for (uint32_t i0 = 0; i0 < n; i0 += MGX_WAVE) {
+        const uint32_t i = i0 + lane;
+        const uint64_t hit = __ballot(i < n && lb.nr[base + i] == q);
+        if (hit) {
+            if (lane == (uint32_t)__ffsll((unsigned long long)hit) - 1) lb.val[base + i] += v;
+            __threadfence_block();
             return;
         }
     }
-    if (n >= lb.cap) {
-        atomicOr(&kp.status[0], (uint32_t)ERR_LAMCAP);
-        return;
+    if (lane == 0) {
+        if (n >= lb.cap) {
+            atomicOr(&kp.status[0], (uint32_t)ERR_LAMCAP);
+        } else {
+            lb.nr[base + n] = q;
+            lb.val[base + n] = v;
+            lb.count[pl] = n + 1;
+        }
     }
-    lb.nr[base + n] = q;
-    lb.val[base + n] = v;
-    lb.count[pl] = n + 1;
+    __threadfence_block();
 }
 
 template <int APPROX, int PREC, int METHOD = METHOD_FEAUTRIER, int NB = 0>
@@ -1329,30 +1338,23 @@ __device__ __forceinline__ int solve_origin(const KParams &kp, uint32_t o, uint3
                         qlo[j] = qa;
                         qup[j] = qb;
                     }
-                    double Llo_hi[NB], Lup_hi[NB];
-                    uint32_t qlo_hi[NB], qup_hi[NB];
+                    /* every lane takes slot h's values from its first lane (0 or 32) */
+                    for (uint32_t h = 0; h < 2; h++) {
+                        const uint32_t slot_h = slot0 + h;
+                        if (slot_h >= kp.nslots) break;
+                        if (((added >> (32 * h)) & 0xffffffffull) == 0) continue;
+                        const int src = 32 * (int)h;
+                        const size_t pl = (size_t)o * m.nlines + slot_h / kp.slots_per_line;
+                        const uint32_t n_lo = __shfl(nlo, src, MGX_WAVE), n_up = __shfl(nup, src, MGX_WAVE);
+                        lam_merge(kp, lb, pl, o, __shfl(v, src, MGX_WAVE), lane); /* diagonal first */
 #pragma unroll
-                    for (int j = 0; j < NB; j++) {
-                        Llo_hi[j] = __shfl(Llo[j], 32, MGX_WAVE);
-                        Lup_hi[j] = __shfl(Lup[j], 32, MGX_WAVE);
-                        qlo_hi[j] = __shfl(qlo[j], 32, MGX_WAVE);
-                        qup_hi[j] = __shfl(qup[j], 32, MGX_WAVE);
-                    }
-                    if (lane == 0) {
-                        for (uint32_t h = 0; h < 2; h++) {
-                            const uint32_t slot_h = slot0 + h;
-                            if (slot_h >= kp.nslots) break;
-                            const size_t pl = (size_t)o * m.nlines + slot_h / kp.slots_per_line;
-                            const uint32_t n_lo = h ? nlo_hi : nlo, n_up = h ? nup_hi : nup;
-                            if (((added >> (32 * h)) & 0xffffffffull) == 0) continue;
-                            lam_merge(kp, lb, pl, o, h ? v_hi : v); /* the diagonal element first */
-#pragma unroll
-                            for (int j = 0; j < NB; j++) {
-                                if ((uint32_t)j < n_lo)
-                                    lam_merge(kp, lb, pl, h ? qlo_hi[j] : qlo[j], h ? Llo_hi[j] : Llo[j]);
-                                if ((uint32_t)j < n_up)
-                                    lam_merge(kp, lb, pl, h ? qup_hi[j] : qup[j], h ? Lup_hi[j] : Lup[j]);
-                            }
+                        for (int j = 0; j < NB; j++) {
+                            if ((uint32_t)j < n_lo)
+                                lam_merge(kp, lb, pl, __shfl(qlo[j], src, MGX_WAVE),
+                                          __shfl(Llo[j], src, MGX_WAVE), lane);
+                            if ((uint32_t)j < n_up)
+                                lam_merge(kp, lb, pl, __shfl(qup[j], src, MGX_WAVE),
+                                          __shfl(Lup[j], src, MGX_WAVE), lane);
                         }
                     }
                 }
