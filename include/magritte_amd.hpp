@@ -3,9 +3,11 @@
  * names (magritte::Model, magritte::Solver, Parameters, Radiation, Image), for
  * a maintainer who replaces the reference's implementation of
  *   Model::compute_radiation_field_feautrier_order_2()   src/model/model.hpp:59
- *   Model::compute_image_new(...)                        src/model/model.hpp:74-80
- *   Solver::setup / solve_feautrier_order_2 / image_feautrier_order_2_new_imager
- *                                                        src/solver/solver.hpp:81-205
+ *   Model::compute_radiation_field_shortchar_order_0(),
+ *          compute_radiation_field_feautrier_order_2_uv() src/model/model.hpp:60, 111
+ *   Model::compute_image(...), compute_image_new(...), compute_image_optical_depth(...),
+ *          compute_image_optical_depth_new(...)          src/model/model.hpp:72-92
+ *   Solver::setup / solve_* / image_*                    src/solver/solver.hpp:81-205
  * The model's arrays are the ones the path reads (the mgx_model view of
  * include/mgx.h), held on the host; the class keeps device mirrors and
  * calls the C ABI.  Errors are thrown as std::runtime_error with the
@@ -40,21 +42,37 @@ struct Parameters {
     bool sum_opacity_emissivity_over_all_lines = false;
     Size n_off_diag = 0;
     Real min_opacity = 1.0e-26;
+    Real min_dtau = 1.0e-15; /* short characteristics */
     Real max_distance_opacity_contribution = 10.0;
     Real max_interpolation_diff = 1.4;
     bool mixed_precision = false; /* MGX_PREC_MIXED instead of MGX_PREC_FP64 */
 };
 
-/* src/model/radiation/radiation.hpp: u (hnrays, P, F) and J (P, F), row-major */
+/* src/model/radiation/radiation.hpp: I (nrays, P, F; short characteristics), u and v
+   (hnrays, P, F), J (P, F), row-major; empty when the last solve did not fill them */
 struct Radiation {
-    std::vector<Real> u, J;
-    Size hnrays = 0, npoints = 0, nfreqs = 0; /* shapes of the last solve */
+    std::vector<Real> I, u, v, J;
+    Size nrays = 0, hnrays = 0, npoints = 0, nfreqs = 0; /* shapes of the last solve */
 };
 
-/* src/model/image/image.hpp (intensity images of the new imager): I (nx*ny, F)
-   with pixel index y + ny*x, pixel coordinates ImX, ImY */
+/* LineProducingSpecies::lambda (lambda.hpp) of every line with n_off_diag > 0: for each
+   receiving point p and line l, count(p, l) entries (emitting point, value) at
+   nr / val[(p L + l) cap + i], the diagonal element first */
+struct LambdaLists {
+    Size cap = 0;
+    std::vector<uint32_t> count, nr;
+    std::vector<Real> val;
+};
+
+/* src/model/image/image.hpp: I (npix, F) of the intensity or the optical depth, with the
+   image-plane coordinates ImX, ImY (npix).  Projection-surface images (the new imager) have
+   npix = nx * ny pixels (index y + ny*x); images at all model points (the old imager) have
+   npix = P and nx = P, ny = 1. */
+enum ImageType { Intensity = 0, OpticalDepth = 1 };
 struct Image {
-    Size nx = 0, ny = 0, nfreqs = 0;
+    ImageType type = Intensity;
+    int64_t ray_nr = -1; /* the model ray of an image along one, else -1 */
+    Size nx = 0, ny = 0, npix = 0, nfreqs = 0;
     Vector3D direction{0.0, 0.0, 0.0};
     std::vector<Real> I, ImX, ImY, freqs;
 };
@@ -64,6 +82,7 @@ class Model {
     Parameters parameters;
     Radiation radiation;
     std::vector<Real> lambda_diag; /* (P, L): the diagonal Lambda entries of every line */
+    LambdaLists lambda;            /* the general operator (n_off_diag > 0) */
     std::vector<Image> images;
 
     Model() = default;
@@ -78,8 +97,20 @@ class Model {
     void set_center(const Vector3D &c) { center = c; }
     void set_max_inverse_mass(Real v) { max_inverse_mass = v; }
 
-    /* src/model/model.cpp:353-390 */
+    /* src/model/model.cpp:353-390; with n_off_diag > 0 also the Lambda lists */
     int compute_radiation_field_feautrier_order_2();
+    /* src/model/model.cpp:313-351 */
+    int compute_radiation_field_shortchar_order_0();
+    /* src/model/model.cpp:395-433 */
+    int compute_radiation_field_feautrier_order_2_uv();
+    /* the old imager at all model points along ray ray_nr, model.cpp:957-976, 1061-1076 */
+    int compute_image(Size ray_nr);
+    int compute_image_optical_depth(Size ray_nr);
+    /* optical depth on the projection surface, model.cpp:1081-1127 */
+    int compute_image_optical_depth_new(Size ray_nr);
+    int compute_image_optical_depth_new(Size ray_nr, Size Nxpix, Size Nypix);
+    int compute_image_optical_depth_new(Real rx, Real ry, Real rz, Size Nxpix, Size Nypix);
+    int compute_image_optical_depth_new(const Vector3D raydir, Size Nxpix, Size Nypix);
     /* src/model/model.cpp:987-1035 */
     int compute_image_new(Size ray_nr);
     int compute_image_new(Size ray_nr, Size Nxpix, Size Nypix);
@@ -108,8 +139,14 @@ class Solver {
   public:
     void setup(Model &model);
     void solve_feautrier_order_2(Model &model);
+    void solve_shortchar_order_0(Model &model);
+    void solve_feautrier_order_2_uv(Model &model);
     void image_feautrier_order_2_new_imager(Model &model, const Vector3D raydir, Size Nxpix,
                                             Size Nypix);
+    void image_optical_depth_new_imager(Model &model, const Vector3D raydir, Size Nxpix,
+                                        Size Nypix);
+    void image_feautrier_order_2(Model &model, Size rr);
+    void image_optical_depth(Model &model, Size rr);
 };
 
 } // namespace magritte

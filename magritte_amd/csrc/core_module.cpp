@@ -2,8 +2,11 @@
  * pybind11 module magritte_amd.core: the hot-path subset of the reference's
  * Python surface (src/bindings/pybindings.cpp) over magritte::Model:
  *   Model.compute_radiation_field_feautrier_order_2()          (:185-188)
- *   Model.compute_image_new(ray_nr) / (ray_nr, Nx, Ny) /
- *        (rx, ry, rz, Nx, Ny)                                   (:227-240)
+ *   Model.compute_radiation_field_shortchar_order_0()           (:202-204)
+ *   Model.compute_radiation_field_feautrier_order_2_uv()        (:195-197)
+ *   Model.compute_image(ray_nr), compute_image_optical_depth(ray_nr) (:225, :241)
+ *   Model.compute_image_new / compute_image_optical_depth_new(ray_nr) / (ray_nr, Nx, Ny) /
+ *        (rx, ry, rz, Nx, Ny)                                   (:227-258)
  *   Model.parameters knobs of the path                          (:277-327)
  *   model.radiation.u (hnrays, P, F), .J (P, F) and model.images[i].I,
  *   .ImX, .ImY as numpy arrays through the buffer protocol      (:109-118, :735-800)
@@ -25,9 +28,11 @@ using magritte::Size;
 
 namespace {
 
-py::array view_of(std::vector<double> &v, std::vector<py::ssize_t> shape, py::handle base)
+template <typename T>
+py::array view_of(std::vector<T> &v, std::vector<py::ssize_t> shape, py::handle base)
 {
-    return py::array(py::dtype::of<double>(), shape, v.data(), base);
+    if (v.empty()) return py::array(py::dtype::of<T>(), std::vector<py::ssize_t>{0});
+    return py::array(py::dtype::of<T>(), shape, v.data(), base);
 }
 
 } // namespace
@@ -54,6 +59,7 @@ PYBIND11_MODULE(core, m)
                        &Parameters::sum_opacity_emissivity_over_all_lines)
         .def_readwrite("n_off_diag", &Parameters::n_off_diag)
         .def_readwrite("min_opacity", &Parameters::min_opacity)
+        .def_readwrite("min_dtau", &Parameters::min_dtau)
         .def_readwrite("max_distance_opacity_contribution",
                        &Parameters::max_distance_opacity_contribution)
         .def_readwrite("max_interpolation_diff", &Parameters::max_interpolation_diff)
@@ -65,29 +71,47 @@ PYBIND11_MODULE(core, m)
                                    Radiation &r = self.cast<Radiation &>();
                                    return view_of(r.u, {r.hnrays, r.npoints, r.nfreqs}, self);
                                })
-        .def_property_readonly("J", [](py::object self) {
+        .def_property_readonly("J",
+                               [](py::object self) {
+                                   Radiation &r = self.cast<Radiation &>();
+                                   return view_of(r.J, {r.npoints, r.nfreqs}, self);
+                               })
+        .def_property_readonly("I",
+                               [](py::object self) {
+                                   Radiation &r = self.cast<Radiation &>();
+                                   return view_of(r.I, {r.nrays, r.npoints, r.nfreqs}, self);
+                               })
+        .def_property_readonly("v", [](py::object self) {
             Radiation &r = self.cast<Radiation &>();
-            return view_of(r.J, {r.npoints, r.nfreqs}, self);
+            return view_of(r.v, {r.hnrays, r.npoints, r.nfreqs}, self);
         });
 
     py::class_<Image>(m, "Image")
         .def_readonly("nfreqs", &Image::nfreqs)
         .def_readonly("nx", &Image::nx)
         .def_readonly("ny", &Image::ny)
+        .def_readonly("npix", &Image::npix)
+        .def_readonly("ray_nr", &Image::ray_nr)
+        .def_property_readonly("imageType",
+                               [](const Image &im) {
+                                   return std::string(im.type == magritte::OpticalDepth
+                                                          ? "OpticalDepth"
+                                                          : "Intensity");
+                               })
         .def_property_readonly("I",
                                [](py::object self) {
                                    Image &im = self.cast<Image &>();
-                                   return view_of(im.I, {(py::ssize_t)im.nx * im.ny, im.nfreqs}, self);
+                                   return view_of(im.I, {(py::ssize_t)im.npix, im.nfreqs}, self);
                                })
         .def_property_readonly("ImX",
                                [](py::object self) {
                                    Image &im = self.cast<Image &>();
-                                   return view_of(im.ImX, {(py::ssize_t)im.nx * im.ny}, self);
+                                   return view_of(im.ImX, {(py::ssize_t)im.npix}, self);
                                })
         .def_property_readonly("ImY",
                                [](py::object self) {
                                    Image &im = self.cast<Image &>();
-                                   return view_of(im.ImY, {(py::ssize_t)im.nx * im.ny}, self);
+                                   return view_of(im.ImY, {(py::ssize_t)im.npix}, self);
                                })
         .def_property_readonly("freqs",
                                [](py::object self) {
@@ -110,6 +134,29 @@ PYBIND11_MODULE(core, m)
                                                   {mo.parameters.npoints, mo.parameters.nlines},
                                                   self);
                                })
+        .def_property_readonly("lambda_count",
+                               [](py::object self) {
+                                   Model &mo = self.cast<Model &>();
+                                   return view_of(mo.lambda.count,
+                                                  {mo.parameters.npoints, mo.parameters.nlines},
+                                                  self);
+                               })
+        .def_property_readonly("lambda_nr",
+                               [](py::object self) {
+                                   Model &mo = self.cast<Model &>();
+                                   return view_of(mo.lambda.nr,
+                                                  {mo.parameters.npoints, mo.parameters.nlines,
+                                                   mo.lambda.cap},
+                                                  self);
+                               })
+        .def_property_readonly("lambda_val",
+                               [](py::object self) {
+                                   Model &mo = self.cast<Model &>();
+                                   return view_of(mo.lambda.val,
+                                                  {mo.parameters.npoints, mo.parameters.nlines,
+                                                   mo.lambda.cap},
+                                                  self);
+                               })
         .def("set_array",
              [](Model &mo, const std::string &name, py::buffer b) {
                  py::buffer_info info = b.request();
@@ -122,6 +169,26 @@ PYBIND11_MODULE(core, m)
              &Model::compute_radiation_field_feautrier_order_2,
              "Compute the radiation field (u, J, Lambda diagonal) with the second-order "
              "Feautrier solver.")
+        .def("compute_radiation_field_shortchar_order_0",
+             &Model::compute_radiation_field_shortchar_order_0,
+             "Compute the radiation field (I both ways, u, v, J, Lambda diagonal) with the "
+             "order-0 short-characteristics solver.")
+        .def("compute_radiation_field_feautrier_order_2_uv",
+             &Model::compute_radiation_field_feautrier_order_2_uv,
+             "Compute u and v with the second-order Feautrier solver.")
+        .def("compute_image", &Model::compute_image,
+             "Old imager: the intensity at every model point along ray ray_nr.")
+        .def("compute_image_optical_depth", &Model::compute_image_optical_depth,
+             "Old imager: the optical depth through every model point along ray ray_nr.")
+        .def("compute_image_optical_depth_new",
+             (int (Model::*)(Size)) & Model::compute_image_optical_depth_new,
+             "Optical-depth image along ray ray_nr, 256 x 256 pixels.")
+        .def("compute_image_optical_depth_new",
+             (int (Model::*)(Size, Size, Size)) & Model::compute_image_optical_depth_new,
+             "Optical-depth image along ray ray_nr with Nxpix x Nypix pixels.")
+        .def("compute_image_optical_depth_new",
+             (int (Model::*)(Real, Real, Real, Size, Size)) & Model::compute_image_optical_depth_new,
+             "Optical-depth image along (rx, ry, rz) with Nxpix x Nypix pixels.")
         .def("compute_image_new", (int (Model::*)(Size)) & Model::compute_image_new,
              "New imager along ray ray_nr, 256 x 256 pixels.")
         .def("compute_image_new", (int (Model::*)(Size, Size, Size)) & Model::compute_image_new,

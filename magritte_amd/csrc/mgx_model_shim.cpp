@@ -2,7 +2,11 @@
  * magritte::Model / magritte::Solver (include/magritte_amd.hpp): the reference's
  * C++ entry points of the formal-solver path on top of the C ABI.
  *   Model::compute_radiation_field_feautrier_order_2   src/model/model.cpp:353-390
- *   Model::compute_image_new (all overloads)           src/model/model.cpp:987-1035
+ *   Model::compute_radiation_field_shortchar_order_0   src/model/model.cpp:313-351
+ *   Model::compute_radiation_field_feautrier_order_2_uv src/model/model.cpp:395-433
+ *   Model::compute_image, compute_image_optical_depth  src/model/model.cpp:957-976, 1061-1076
+ *   Model::compute_image_new, compute_image_optical_depth_new (all overloads)
+ *                                                      src/model/model.cpp:987-1035, 1081-1127
  *   Model::compute_spectral_discretisation(min,max,n)  src/model/model.cpp:257-298
  */
 #include <hip/hip_runtime_api.h>
@@ -187,17 +191,94 @@ int Model::compute_radiation_field_feautrier_order_2()
     return 0;
 }
 
-int Model::compute_image_new(Size ray_nr) { return compute_image_new(ray_nr, 256, 256); }
-
-int Model::compute_image_new(Size ray_nr, Size Nxpix, Size Nypix)
+int Model::compute_radiation_field_shortchar_order_0()
 {
-    /* geometry.rays.get_direction<false>(0, ray_nr) */
-    const auto &d = get_array("ray_direction");
+    Solver solver;
+    solver.setup(*this);
+    solver.solve_shortchar_order_0(*this);
+    return 0;
+}
+
+int Model::compute_radiation_field_feautrier_order_2_uv()
+{
+    Solver solver;
+    solver.setup(*this);
+    solver.solve_feautrier_order_2_uv(*this);
+    return 0;
+}
+
+int Model::compute_image(Size ray_nr)
+{
+    if (parameters.one_line_approximation)
+        throw std::runtime_error("One line approximation is currently not supported for imaging.");
+    Solver solver;
+    solver.image_feautrier_order_2(*this, ray_nr);
+    return 0;
+}
+
+int Model::compute_image_optical_depth(Size ray_nr)
+{
+    if (parameters.one_line_approximation)
+        throw std::runtime_error("One line approximation is currently not supported for imaging.");
+    Solver solver;
+    solver.image_optical_depth(*this, ray_nr);
+    return 0;
+}
+
+namespace {
+
+/* geometry.rays.get_direction<false>(0, ray_nr) */
+Vector3D ray_direction_of(const Model &model, Size ray_nr)
+{
+    const auto &d = model.get_array("ray_direction");
     if ((size_t)(ray_nr + 1) * 3 * sizeof(double) > d.size())
         throw std::runtime_error("ray_nr out of range");
     Vector3D r;
     std::memcpy(r.data(), d.data() + (size_t)ray_nr * 3 * sizeof(double), sizeof r);
-    return compute_image_new(r, Nxpix, Nypix);
+    return r;
+}
+
+void check_image_direction(const Model &model, const Vector3D &raydir)
+{
+    /* model.cpp:1009-1024: the checks come before any work */
+    if (raydir[0] * raydir[0] + raydir[1] * raydir[1] + raydir[2] * raydir[2] == 0.0)
+        throw std::runtime_error(
+            "The given ray direction vector does not point in a direction. Please use a "
+            "non-zero (normed) direction vector to generate an image.");
+    if (model.parameters.one_line_approximation)
+        throw std::runtime_error("One line approximation is not supported for imaging.");
+}
+
+} // namespace
+
+int Model::compute_image_optical_depth_new(Size ray_nr)
+{
+    return compute_image_optical_depth_new(ray_nr, 256, 256);
+}
+
+int Model::compute_image_optical_depth_new(Size ray_nr, Size Nxpix, Size Nypix)
+{
+    return compute_image_optical_depth_new(ray_direction_of(*this, ray_nr), Nxpix, Nypix);
+}
+
+int Model::compute_image_optical_depth_new(Real rx, Real ry, Real rz, Size Nxpix, Size Nypix)
+{
+    return compute_image_optical_depth_new(Vector3D{rx, ry, rz}, Nxpix, Nypix);
+}
+
+int Model::compute_image_optical_depth_new(const Vector3D raydir, Size Nxpix, Size Nypix)
+{
+    check_image_direction(*this, raydir);
+    Solver solver;
+    solver.image_optical_depth_new_imager(*this, raydir, Nxpix, Nypix);
+    return 0;
+}
+
+int Model::compute_image_new(Size ray_nr) { return compute_image_new(ray_nr, 256, 256); }
+
+int Model::compute_image_new(Size ray_nr, Size Nxpix, Size Nypix)
+{
+    return compute_image_new(ray_direction_of(*this, ray_nr), Nxpix, Nypix);
 }
 
 int Model::compute_image_new(Real rx, Real ry, Real rz, Size Nxpix, Size Nypix)
@@ -207,13 +288,7 @@ int Model::compute_image_new(Real rx, Real ry, Real rz, Size Nxpix, Size Nypix)
 
 int Model::compute_image_new(const Vector3D raydir, Size Nxpix, Size Nypix)
 {
-    /* model.cpp:1009-1024: the checks come before any work */
-    if (raydir[0] * raydir[0] + raydir[1] * raydir[1] + raydir[2] * raydir[2] == 0.0)
-        throw std::runtime_error(
-            "The given ray direction vector does not point in a direction. Please use a "
-            "non-zero (normed) direction vector to generate an image.");
-    if (parameters.one_line_approximation)
-        throw std::runtime_error("One line approximation is not supported for imaging.");
+    check_image_direction(*this, raydir);
     Solver solver;
     solver.image_feautrier_order_2_new_imager(*this, raydir, Nxpix, Nypix);
     return 0;
@@ -254,20 +329,182 @@ void Solver::setup(Model &model)
     (void)model.device_view();
 }
 
+namespace {
+
+void set_shapes(Radiation &r, const mgx_model &v)
+{
+    r.nrays = v.nrays;
+    r.hnrays = v.hnrays;
+    r.npoints = v.npoints;
+    r.nfreqs = v.nfreqs;
+}
+
+/* a device buffer of 32-bit integers, freed at scope end */
+struct DevOutU32 {
+    uint32_t *p = nullptr;
+    explicit DevOutU32(size_t n)
+    {
+        if (n) hip_check(hipMalloc(&p, n * sizeof(uint32_t)), "hipMalloc");
+    }
+    ~DevOutU32()
+    {
+        if (p) (void)hipFree(p);
+    }
+    void copy_to(std::vector<uint32_t> &v, size_t n) const
+    {
+        v.resize(n);
+        if (n) hip_check(hipMemcpy(v.data(), p, n * sizeof(uint32_t), hipMemcpyDeviceToHost), "copy");
+    }
+};
+
+} // namespace
+
 void Solver::solve_feautrier_order_2(Model &model)
 {
     const mgx_model &v = model.device_view();
     const mgx_options o = model.options();
     const size_t PF = (size_t)v.npoints * v.nfreqs;
     const size_t U = (size_t)v.hnrays * PF;
-    DevOut u(U), J(PF), lam((size_t)v.npoints * v.nlines);
-    check(mgx_feautrier_forward_device(&v, &o, u.p, J.p, lam.p, nullptr));
+    const size_t PL = (size_t)v.npoints * v.nlines;
+    DevOut u(U), J(PF);
+    model.radiation.I.clear();
+    model.radiation.v.clear();
+    if (o.n_off_diag == 0) {
+        DevOut lam(PL);
+        check(mgx_feautrier_forward_device(&v, &o, u.p, J.p, lam.p, nullptr));
+        lam.copy_to(model.lambda_diag, PL);
+        model.lambda = LambdaLists{};
+    } else {
+        /* the general operator: lists of the diagonal and the n_off_diag bands */
+        const Size cap = 1 + 2 * o.n_off_diag * v.hnrays;
+        DevOutU32 cnt(PL), nr(PL * cap);
+        DevOut val(PL * cap);
+        check(mgx_feautrier_forward_lambda_device(&v, &o, u.p, J.p, cap, cnt.p, nr.p, val.p,
+                                                  nullptr));
+        LambdaLists &L = model.lambda;
+        L.cap = cap;
+        cnt.copy_to(L.count, PL);
+        nr.copy_to(L.nr, PL * cap);
+        val.copy_to(L.val, PL * cap);
+        model.lambda_diag.assign(PL, 0.0);
+        for (size_t i = 0; i < PL; i++)
+            if (L.count[i]) model.lambda_diag[i] = L.val[i * cap];
+    }
     u.copy_to(model.radiation.u, U);
     J.copy_to(model.radiation.J, PF);
-    model.radiation.hnrays = v.hnrays;
-    model.radiation.npoints = v.npoints;
-    model.radiation.nfreqs = v.nfreqs;
-    lam.copy_to(model.lambda_diag, (size_t)v.npoints * v.nlines);
+    set_shapes(model.radiation, v);
+}
+
+void Solver::solve_shortchar_order_0(Model &model)
+{
+    const mgx_model &v = model.device_view();
+    const mgx_options o = model.options();
+    const size_t PF = (size_t)v.npoints * v.nfreqs;
+    const size_t PL = (size_t)v.npoints * v.nlines;
+    DevOut I((size_t)v.nrays * PF), u((size_t)v.hnrays * PF), vv((size_t)v.hnrays * PF), J(PF),
+        lam(PL);
+    check(mgx_shortchar_order_0_device(&v, &o, model.parameters.min_dtau, I.p, u.p, vv.p, J.p,
+                                       lam.p, nullptr));
+    I.copy_to(model.radiation.I, (size_t)v.nrays * PF);
+    u.copy_to(model.radiation.u, (size_t)v.hnrays * PF);
+    vv.copy_to(model.radiation.v, (size_t)v.hnrays * PF);
+    J.copy_to(model.radiation.J, PF);
+    lam.copy_to(model.lambda_diag, PL);
+    set_shapes(model.radiation, v);
+}
+
+void Solver::solve_feautrier_order_2_uv(Model &model)
+{
+    const mgx_model &v = model.device_view();
+    const mgx_options o = model.options();
+    const size_t U = (size_t)v.hnrays * v.npoints * v.nfreqs;
+    DevOut u(U), vv(U);
+    check(mgx_feautrier_order_2_uv_device(&v, &o, u.p, vv.p, nullptr));
+    u.copy_to(model.radiation.u, U);
+    vv.copy_to(model.radiation.v, U);
+    model.radiation.I.clear();
+    model.radiation.J.clear();
+    set_shapes(model.radiation, v);
+}
+
+namespace {
+
+/* the image frequencies: the grid at the first point (uniform over points) */
+void set_image_freqs(const Model &model, Image &img, Size F)
+{
+    const auto &nu = model.get_array("nu");
+    img.freqs.resize(F);
+    std::memcpy(img.freqs.data(), nu.data(), F * sizeof(double));
+}
+
+} // namespace
+
+void Solver::image_feautrier_order_2(Model &model, Size rr)
+{
+    const mgx_model &v = model.device_view();
+    const mgx_options o = model.options();
+    const size_t P = v.npoints;
+    DevOut I(P * v.nfreqs), X(P), Y(P);
+    check(mgx_image_ray_device(&v, &o, rr, I.p, X.p, Y.p, nullptr));
+    Image img;
+    img.type = Intensity;
+    img.ray_nr = rr;
+    img.nx = v.npoints;
+    img.ny = 1;
+    img.npix = v.npoints;
+    img.nfreqs = v.nfreqs;
+    img.direction = ray_direction_of(model, rr);
+    I.copy_to(img.I, P * v.nfreqs);
+    X.copy_to(img.ImX, P);
+    Y.copy_to(img.ImY, P);
+    set_image_freqs(model, img, v.nfreqs);
+    model.images.push_back(std::move(img));
+}
+
+void Solver::image_optical_depth(Model &model, Size rr)
+{
+    const mgx_model &v = model.device_view();
+    const mgx_options o = model.options();
+    const size_t P = v.npoints;
+    DevOut T(P * v.nfreqs), X(P), Y(P);
+    check(mgx_image_optical_depth_ray_device(&v, &o, rr, T.p, X.p, Y.p, nullptr));
+    Image img;
+    img.type = OpticalDepth;
+    img.ray_nr = rr;
+    img.nx = v.npoints;
+    img.ny = 1;
+    img.npix = v.npoints;
+    img.nfreqs = v.nfreqs;
+    img.direction = ray_direction_of(model, rr);
+    T.copy_to(img.I, P * v.nfreqs);
+    X.copy_to(img.ImX, P);
+    Y.copy_to(img.ImY, P);
+    set_image_freqs(model, img, v.nfreqs);
+    model.images.push_back(std::move(img));
+}
+
+void Solver::image_optical_depth_new_imager(Model &model, const Vector3D raydir, Size Nxpix,
+                                            Size Nypix)
+{
+    const mgx_model &v = model.device_view();
+    const mgx_options o = model.options();
+    const size_t npix = (size_t)Nxpix * Nypix;
+    DevOut T(npix * v.nfreqs), X(npix), Y(npix);
+    check(mgx_image_optical_depth_device(&v, &o, raydir.data(), Nxpix, Nypix, T.p, X.p, Y.p,
+                                         nullptr));
+    Image img;
+    img.type = OpticalDepth;
+    img.nx = Nxpix;
+    img.ny = Nypix;
+    img.npix = (Size)npix;
+    img.nfreqs = v.nfreqs;
+    const Real n = std::sqrt(raydir[0] * raydir[0] + raydir[1] * raydir[1] + raydir[2] * raydir[2]);
+    for (int i = 0; i < 3; i++) img.direction[i] = raydir[i] / n;
+    T.copy_to(img.I, npix * v.nfreqs);
+    X.copy_to(img.ImX, npix);
+    Y.copy_to(img.ImY, npix);
+    set_image_freqs(model, img, v.nfreqs);
+    model.images.push_back(std::move(img));
 }
 
 void Solver::image_feautrier_order_2_new_imager(Model &model, const Vector3D raydir, Size Nxpix,
@@ -279,18 +516,17 @@ void Solver::image_feautrier_order_2_new_imager(Model &model, const Vector3D ray
     DevOut I(npix * v.nfreqs), X(npix), Y(npix);
     check(mgx_image_device(&v, &o, raydir.data(), Nxpix, Nypix, I.p, X.p, Y.p, nullptr));
     Image img;
+    img.type = Intensity;
     img.nx = Nxpix;
     img.ny = Nypix;
+    img.npix = (Size)npix;
     img.nfreqs = v.nfreqs;
     const Real n = std::sqrt(raydir[0] * raydir[0] + raydir[1] * raydir[1] + raydir[2] * raydir[2]);
     for (int i = 0; i < 3; i++) img.direction[i] = raydir[i] / n;
     I.copy_to(img.I, npix * v.nfreqs);
     X.copy_to(img.ImX, npix);
     Y.copy_to(img.ImY, npix);
-    /* image frequencies: the grid at the first point (uniform over points) */
-    const auto &nu = model.get_array("nu");
-    img.freqs.resize(v.nfreqs);
-    std::memcpy(img.freqs.data(), nu.data(), v.nfreqs * sizeof(double));
+    set_image_freqs(model, img, v.nfreqs);
     model.images.push_back(std::move(img));
 }
 

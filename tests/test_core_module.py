@@ -52,3 +52,51 @@ def test_core_model_matches_device_solver(small_wind):
     np.testing.assert_array_equal(np.array(img.ImX), X.cpu().numpy())
     cm.restore_line_grid()
     assert cm.parameters.nfreqs == small_wind.nfreqs
+
+
+@pytest.mark.gpu
+def test_core_sibling_solvers_match_device_calls(small_wind):
+    """The C++ Model's short characteristics, u/v, old imager, optical-depth images and
+    general Lambda lists (n_off_diag > 0) give the device entry points' results bit for bit."""
+    import torch  # noqa: F401  (one HIP runtime: torch's)
+
+    from magritte_amd import solver
+
+    m = small_wind
+    dm = solver.DeviceModel(m)
+    op = m.options()
+    cm = m.to_core()
+    cm.compute_radiation_field_shortchar_order_0()
+    sc = solver.shortchar_forward(dm, op)
+    for k in ("I", "u", "v", "J"):
+        np.testing.assert_array_equal(np.array(getattr(cm.radiation, k)), sc[k].cpu().numpy())
+    np.testing.assert_array_equal(np.array(cm.lambda_diag), sc["lambda_diag"].cpu().numpy())
+    cm.compute_radiation_field_feautrier_order_2_uv()
+    uv = solver.feautrier_uv(dm, op)
+    np.testing.assert_array_equal(np.array(cm.radiation.v), uv["v"].cpu().numpy())
+    assert np.array(cm.radiation.J).size == 0
+    for od, name in ((False, "compute_image"), (True, "compute_image_optical_depth")):
+        getattr(cm, name)(8)
+        img = cm.images[-1]
+        ref, X, _ = solver.image_ray(dm, op, 8, optical_depth=od)
+        assert img.imageType == ("OpticalDepth" if od else "Intensity") and img.ray_nr == 8
+        np.testing.assert_array_equal(np.array(img.I), ref.cpu().numpy())
+        np.testing.assert_array_equal(np.array(img.ImX), X.cpu().numpy())
+    nu0 = m.line[0]
+    cm.compute_spectral_discretisation(nu0 * (1 - 2e-5), nu0 * (1 + 2e-5), 31)
+    cm.compute_image_optical_depth_new(0.3, -0.5, 0.81, 12, 10)
+    fr = m.image_frequencies(nu0 * (1 - 2e-5), nu0 * (1 + 2e-5), 31)
+    tau, _, _ = solver.image_optical_depth_new(dm, op, np.array([0.3, -0.5, 0.81]), 12, 10, fr)
+    np.testing.assert_array_equal(np.array(cm.images[-1].I), tau.cpu().numpy())
+    cm.restore_line_grid()
+    cm.parameters.n_off_diag = 2
+    cm.compute_radiation_field_feautrier_order_2()
+    op2 = m.options()
+    op2.n_off_diag = 2
+    lam = solver.feautrier_forward_lambda(dm, op2)
+    cnt = np.array(cm.lambda_count)
+    np.testing.assert_array_equal(cnt, lam["count"].cpu().numpy().view(np.uint32))
+    used = np.arange(np.array(cm.lambda_val).shape[2])[None, None, :] < cnt[:, :, None]
+    np.testing.assert_array_equal(np.array(cm.lambda_nr)[used],
+                                  lam["nr"].cpu().numpy().view(np.uint32)[used])
+    np.testing.assert_array_equal(np.array(cm.lambda_val)[used], lam["val"].cpu().numpy()[used])
