@@ -48,6 +48,8 @@ def parse():
                     help="nlte workload: > 0 runs the non-sparse loop with this many Lambda "
                          "bands and the global statistical equilibrium")
     ap.add_argument("--n-models", type=int, default=64)
+    ap.add_argument("--image-size", type=int, default=256,
+                    help="image-gradient workload: pixels per image side")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 512 MiB three times (scale of rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
@@ -209,8 +211,9 @@ def run_batch(args):
 
 
 def run_image_gradient(args):
-    """Gradient of L = |I - I*|^2 for a 256^2 x 31 image of the 32^3 wind with
-    respect to the line emissivity and opacity (P, L), through the adjoint
+    """Gradient of L = |I - I*|^2 for a 256^2 x 31 image (--image-size) of the 32^3
+    wind with respect to the line emissivity and opacity (P, L), and with respect to
+    the log-density and temperature of every point, through the adjoint
     kernel behind torch.autograd; the target I* is the image of the same wind
     with density and temperature perturbed by seeded 10% log-normal factors.
     Reports forward and adjoint time, central differences of the GPU forward
@@ -229,7 +232,7 @@ def run_image_gradient(args):
     nu0 = m.line[0]
     fr = m.image_frequencies(nu0 * (1 - 1e-4), nu0 * (1 + 1e-4), 31)
     rd = np.array([0.3, -0.5, 0.81])
-    nx = ny = 256
+    nx = ny = args.image_size
     dm, dmt = solver.DeviceModel(m, "cuda:0"), solver.DeviceModel(mt, "cuda:0")
     lib = _abi.lib()
     lib.mgx_set_kernel_timing(1)
@@ -292,6 +295,7 @@ def run_image_gradient(args):
     for k, (a, b) in zip(("eps", "kap"), zip(grads["mixed"], grads["fp64"])):
         sig = np.abs(b) > 1e-3 * np.abs(b).max()
         mix_err[k] = float(np.max(np.abs(a[sig] - b[sig]) / np.abs(b[sig])))
+    lnT = _image_gradient_lnT(m, dm, Istar, rd, nx, ny, fr, rng, lib)
     U_img = nx * ny * fr["nu"].shape[1]
     line = {"metric": "image gradient (adjoint) ray.point.freq/s", "unit": "pixel*freq/s",
             "value": round(U_img / (res["mixed"]["adjoint_kernel_ms"] * 1e-3), 1),
@@ -304,8 +308,72 @@ def run_image_gradient(args):
                                     "max_rel_err": float(errs.max()),
                                     "median_rel_err": float(np.median(errs)),
                                     "frac_below_1e-4": float(np.mean(errs < 1e-4))},
-                       "mixed_vs_fp64_grad_max_rel_err": mix_err}}
+                       "mixed_vs_fp64_grad_max_rel_err": mix_err,
+                       "wrt_log_density_temperature": lnT}}
     print(json.dumps(line), flush=True)
+
+
+def _image_gradient_lnT(m, dm, Istar, rd, nx, ny, fr, rng, lib):
+    """The same loss as a function of the log-density and temperature of every point
+    (LTE populations and Doppler widths through autograd.LineThermo; the adjoint with
+    the inverse-width output): forward and adjoint times in both precisions, and
+    central differences of the FP64 forward at 16 points per variable, interpolated
+    entries off (step 1e-6 for log n; 1e-4 relative for T, whose float profile
+    values would put rounding steps into a quotient of step 1e-6)."""
+    import numpy as np
+    import torch
+
+    from magritte_amd import _abi, autograd
+
+    th = autograd.LineThermo(m, dm.device)
+    ln0, T0 = th.log_density(), th.temperature()
+    res = {}
+    for prec, name in ((_abi.PREC_FP64, "fp64"), (_abi.PREC_MIXED, "mixed")):
+        op = m.options(precision=prec)
+        for rep in range(2):  # second pass timed (warm)
+            ln = ln0.clone().requires_grad_(True)
+            T = T0.clone().requires_grad_(True)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            I = autograd.image_log_density_temperature(ln, T, th, dm, op, rd, nx, ny, fr)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            fwd_k = lib.mgx_last_kernel_ms()
+            ((I - Istar) ** 2).sum().backward()
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            adj_k = lib.mgx_last_kernel_ms()
+        res[name] = {"forward_ms": round((t1 - t0) * 1e3, 3), "forward_kernel_ms": round(fwd_k, 3),
+                     "backward_ms": round((t2 - t1) * 1e3, 3), "adjoint_kernel_ms": round(adj_k, 3),
+                     "cost_vs_forward": round(adj_k / fwd_k, 3)}
+    op = m.options()
+    op.max_interpolation_diff = 1.0e30
+    ln = ln0.clone().requires_grad_(True)
+    T = T0.clone().requires_grad_(True)
+    I0 = autograd.image_log_density_temperature(ln, T, th, dm, op, rd, nx, ny, fr)
+    ((I0 - Istar) ** 2).sum().backward()
+    I0 = I0.detach()
+    fd = {}
+    for which, g, x0, h in (("log_n", ln.grad, ln0, 1e-6), ("T", T.grad, T0, 1e-4)):
+        g = g.cpu().numpy()
+        cand = np.flatnonzero(np.abs(g) > 1e-3 * np.abs(g).max())
+        errs = []
+        for p in rng.choice(cand, min(16, cand.size), replace=False):
+            step = h if which == "log_n" else h * float(x0[p])
+            Is = []
+            for sgn in (1.0, -1.0):
+                x = x0.clone()
+                x[int(p)] += sgn * step
+                args = (x, T0) if which == "log_n" else (ln0, x)
+                with torch.no_grad():
+                    e, k, w = th(*args)
+                Is.append(autograd.image_width(e, k, w, dm, op, rd, nx, ny, fr))
+            dL = ((Is[0] - Is[1]) * (Is[0] + Is[1] - 2.0 * Istar)).sum().item()
+            errs.append(abs(g[p] - dL / (2 * step)) / abs(dL / (2 * step)))
+        errs = np.array(errs)
+        fd[which] = {"points": int(len(errs)), "step": h, "max_rel_err": float(errs.max()),
+                     "median_rel_err": float(np.median(errs))}
+    return {"populations": "lte", "variants": res, "fd_check": fd}
 
 
 def run_nlte(args):

@@ -293,6 +293,23 @@ int mgx_image_adjoint_device(const mgx_model *model_dev, const mgx_options *opts
                              const double *raydir, uint32_t nx, uint32_t ny, const double *I_bar,
                              double *d_eps, double *d_kap, void *stream);
 
+/*
+ * The two products above with a third output, d_inverse_width (P, L), device:
+ * the gradient with respect to model_dev->inverse_width, through which the
+ * temperature enters the line profile (Lines::set_inverse_width,
+ * src/model/lines/lines.tpp:45-60).  The Gaussian profile and the difference
+ * of error functions of compute_dtau_single_line (solver.tpp:2374-2455) are
+ * differentiated as the smooth functions (the reference evaluates them with
+ * float expf / erff).  d_eps and d_kap are the same as without it.
+ */
+int mgx_feautrier_adjoint_width_device(const mgx_model *model_dev, const mgx_options *opts,
+                                       const double *J_bar, double *d_eps, double *d_kap,
+                                       double *d_inverse_width, void *stream);
+int mgx_image_adjoint_width_device(const mgx_model *model_dev, const mgx_options *opts,
+                                   const double *raydir, uint32_t nx, uint32_t ny,
+                                   const double *I_bar, double *d_eps, double *d_kap,
+                                   double *d_inverse_width, void *stream);
+
 /* Frees the adjoint's device workspace (it is re-allocated on the next call). */
 void mgx_release_workspace(void);
 

@@ -151,6 +151,11 @@ SIGNATURES = [
     ("mgx_image_adjoint_device", C.c_int,
      [C.POINTER(MgxModel), C.POINTER(MgxOptions), _VP, C.c_uint32, C.c_uint32, _VP, _VP, _VP,
       _VP]),
+    ("mgx_feautrier_adjoint_width_device", C.c_int,
+     [C.POINTER(MgxModel), C.POINTER(MgxOptions), _VP, _VP, _VP, _VP, _VP]),
+    ("mgx_image_adjoint_width_device", C.c_int,
+     [C.POINTER(MgxModel), C.POINTER(MgxOptions), _VP, C.c_uint32, C.c_uint32, _VP, _VP, _VP,
+      _VP, _VP]),
     ("mgx_release_workspace", None, []),
     ("mgx_batch_create", C.c_int,
      [C.POINTER(MgxModel), C.c_uint32, C.POINTER(MgxOptions), _VP, _VP, C.POINTER(_VP), _VP]),

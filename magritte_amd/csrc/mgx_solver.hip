@@ -2523,58 +2523,74 @@ struct GradSink {
     uint32_t width;  /* lanes summed together (32 or 64), 0: no reduction */
     uint32_t p_crt, p_nxt;
     double e_crt, k_crt, e_nxt, k_nxt;
+    double *g_iw;    /* inverse-width gradient (IW instantiations only) */
+    double w_crt, w_nxt;
 };
 
-__device__ inline void sink_put(const GradSink &g, uint32_t p, uint32_t l, double de, double dk)
+/* IW: the sink also carries the gradient with respect to the inverse line widths */
+template <bool IW = false>
+__device__ inline void sink_put(const GradSink &g, uint32_t p, uint32_t l, double de, double dk,
+                                double dw = 0.0)
 {
     const size_t i = (size_t)p * g.L + l;
     if (de != 0.0) unsafeAtomicAdd(&g.g_eps[i], de);
     if (dk != 0.0) unsafeAtomicAdd(&g.g_kap[i], dk);
+    if (IW && dw != 0.0) unsafeAtomicAdd(&g.g_iw[i], dw);
 }
 
-__device__ inline void sink_add(GradSink &g, uint32_t p, uint32_t l, double de, double dk)
+template <bool IW = false>
+__device__ inline void sink_add(GradSink &g, uint32_t p, uint32_t l, double de, double dk,
+                                double dw = 0.0)
 {
     if (l != g.line) {
-        sink_put(g, p, l, de, dk);
+        sink_put<IW>(g, p, l, de, dk, dw);
     } else if (p == g.p_nxt) {
         g.e_nxt += de;
         g.k_nxt += dk;
+        if (IW) g.w_nxt += dw;
     } else {
         g.e_crt += de;
         g.k_crt += dk;
+        if (IW) g.w_crt += dw;
     }
 }
 
 /* all lanes of a reduction group call this together */
-__device__ inline void sink_flush(const GradSink &g, uint32_t p, double e, double k)
+template <bool IW = false>
+__device__ inline void sink_flush(const GradSink &g, uint32_t p, double e, double k, double w)
 {
     if (g.width == 0) {
-        sink_put(g, p, g.line, e, k);
+        sink_put<IW>(g, p, g.line, e, k, w);
         return;
     }
     for (uint32_t off = g.width >> 1; off >= 1; off >>= 1) {
         e += __shfl_xor(e, (int)off, MGX_WAVE);
         k += __shfl_xor(k, (int)off, MGX_WAVE);
+        if (IW) w += __shfl_xor(w, (int)off, MGX_WAVE);
     }
-    if ((threadIdx.x & (g.width - 1)) == 0) sink_put(g, p, g.line, e, k);
+    if ((threadIdx.x & (g.width - 1)) == 0) sink_put<IW>(g, p, g.line, e, k, w);
 }
 
 /* the walk moves outward to a step whose points are (crt = old nxt, nxt) */
+template <bool IW = false>
 __device__ inline void sink_step(GradSink &g, uint32_t nxt)
 {
-    sink_flush(g, g.p_crt, g.e_crt, g.k_crt);
+    sink_flush<IW>(g, g.p_crt, g.e_crt, g.k_crt, g.w_crt);
     g.p_crt = g.p_nxt;
     g.e_crt = g.e_nxt;
     g.k_crt = g.k_nxt;
+    g.w_crt = g.w_nxt;
     g.p_nxt = nxt;
     g.e_nxt = 0.0;
     g.k_nxt = 0.0;
+    g.w_nxt = 0.0;
 }
 
+template <bool IW = false>
 __device__ inline void sink_finish(GradSink &g)
 {
-    sink_flush(g, g.p_crt, g.e_crt, g.k_crt);
-    sink_flush(g, g.p_nxt, g.e_nxt, g.k_nxt);
+    sink_flush<IW>(g, g.p_crt, g.e_crt, g.k_crt, g.w_crt);
+    sink_flush<IW>(g, g.p_nxt, g.e_nxt, g.k_nxt, g.w_nxt);
 }
 
 /* partial derivatives of interpolate_log(a, b, t) = a^(1-t) b^t with value r */
@@ -2595,14 +2611,64 @@ __device__ inline void grad_put(double *g_eps, double *g_kap, uint32_t L, uint32
     if (dk != 0.0) unsafeAtomicAdd(&g_kap[i], dk);
 }
 
-/* reverse of eta_chi_interp: eta_bar d(eta) + chi_bar d(chi), added directly.
+/* reverse of interpolate_linear(iw(p), iw(pi), t): partials 1 - t and t */
+__device__ inline void iw_put(double *g_iw, uint32_t L, uint32_t p, uint32_t pi, uint32_t l,
+                              double t, double bar)
+{
+    const double a = (1.0 - t) * bar;
+    if (a != 0.0) unsafeAtomicAdd(&g_iw[(size_t)p * L + l], a);
+    if (t != 0.0) {
+        const double b = t * bar;
+        if (b != 0.0) unsafeAtomicAdd(&g_iw[(size_t)pi * L + l], b);
+    }
+}
+
+/* d(gaussian(w, diff))/dw = exp(-x^2) (1 - 2 x^2) / sqrt(pi), x = w diff: the
+   derivative of the smooth profile, in double */
+__device__ inline double gaussian_dw(double w, double diff)
+{
+    const double x = w * diff;
+    return MGX_INVERSE_SQRT_PI * exp(-x * x) * (1.0 - 2.0 * x * x);
+}
+
+/* d(compute_dtau_single_line)/d(avg_iw) at fixed frequencies, of the smooth
+   function: dz avg_op d(profile)/dw when the two frequencies are equal, else the
+   derivative of dz avg_op (erf(np) - erf(cp)) / (2 (cf - nf)), np = (lf - nf) w,
+   cp = (lf - cf) w: dz avg_op (np exp(-np^2) - cp exp(-cp^2)) / (sqrt(pi) (np - cp)) */
+__device__ inline double dtau_line_dw(double avg_iw, double avg_op, double linefreq, double cf,
+                                      double nf, double dz)
+{
+    if (cf == nf) return dz * avg_op * gaussian_dw(avg_iw, cf - linefreq);
+    const double np = (linefreq - nf) * avg_iw, cp = (linefreq - cf) * avg_iw;
+    return dz * avg_op * MGX_INVERSE_SQRT_PI * (np * exp(-np * np) - cp * exp(-cp * cp)) /
+           (np - cp);
+}
+
+/* the inverse-width part of the reverse of compute_dtau_single_line; bar is the
+   adjoint of the line's optical depth */
+template <int PREC>
+__device__ inline void dtau_iw_bwd(double *g_iw, uint32_t L, uint32_t cp, uint32_t cpi,
+                                   uint32_t np, uint32_t npi, uint32_t l, double4 c, double4 cI,
+                                   double4 n, double4 nI, double linefreq, double cf, double nf,
+                                   double ct, double nt, double dz, double bar)
+{
+    if (bar == 0.0) return;
+    const double avg_iw = (interp_linear(c.x, cI.x, ct) + interp_linear(n.x, nI.x, nt)) / 2.0;
+    const double avg_op = (interp_log<PREC>(c.z, cI.z, ct) + interp_log<PREC>(n.z, nI.z, nt)) / 2.0;
+    const double wb = 0.5 * bar * dtau_line_dw(avg_iw, avg_op, linefreq, cf, nf, dz);
+    iw_put(g_iw, L, cp, cpi, l, ct, wb);
+    iw_put(g_iw, L, np, npi, l, nt, wb);
+}
+
+/* reverse of eta_chi_interp: eta_bar d(eta) + chi_bar d(chi), added directly
+   (and, with IW, the inverse widths' part into g_iw).
    A rare path of the reverse pass (interpolated entries, several lines in the
    window): kept out of line, with every argument by value so that the
    caller's registers stay out of memory. */
-template <int APPROX, int PREC>
+template <int APPROX, int PREC, bool IW = false>
 __device__ __attribute__((noinline)) void eta_chi_interp_bwd(
     const KParams *kpp, OwnLine ol, uint32_t p1, uint32_t p2, double t, uint32_t ll, double freq,
-    double eta_bar, double chi_bar, double *g_eps, double *g_kap)
+    double eta_bar, double chi_bar, double *g_eps, double *g_kap, double *g_iw)
 {
     const KParams &kp = *kpp;
     const mgx_model &m = kp.m;
@@ -2618,13 +2684,17 @@ __device__ __attribute__((noinline)) void eta_chi_interp_bwd(
         const uint32_t l = (APPROX == MGX_APPROX_CLOSE_LINES) ? m.sorted_line_map[s] : s;
         const double diff = freq - m.line[l];
         const double4 a = lp(kp, p1, l), b = lp(kp, p2, l);
-        const double prof = gaussian<PREC>(interp_linear(a.x, b.x, t), diff);
+        const double iw = interp_linear(a.x, b.x, t);
+        const double prof = gaussian<PREC>(iw, diff);
+        const double ei = interp_log<PREC>(a.y, b.y, t), ki = interp_log<PREC>(a.z, b.z, t);
         double de1, de2, dk1, dk2;
-        interp_log_partials(a.y, b.y, t, interp_log<PREC>(a.y, b.y, t), de1, de2);
-        interp_log_partials(a.z, b.z, t, interp_log<PREC>(a.z, b.z, t), dk1, dk2);
+        interp_log_partials(a.y, b.y, t, ei, de1, de2);
+        interp_log_partials(a.z, b.z, t, ki, dk1, dk2);
         const double pe = prof * eta_bar, pk = prof * chi_bar;
         grad_put(g_eps, g_kap, m.nlines, p1, l, pe * de1, pk * dk1);
         if (t != 0.0) grad_put(g_eps, g_kap, m.nlines, p2, l, pe * de2, pk * dk2);
+        /* eta = prof eps_i, chi = chi_min + prof kap_i */
+        if (IW) iw_put(g_iw, m.nlines, p1, p2, l, t, gaussian_dw(iw, diff) * (eta_bar * ei + chi_bar * ki));
     }
 }
 
@@ -2667,12 +2737,12 @@ __device__ inline void line_end_bwd(double *g_eps, double *g_kap, uint32_t L, ui
 
 /* reverse of S_dtau_line_integrated (the general case: out of line, arguments by
    value, gradients added directly) */
-template <int APPROX, int PREC>
+template <int APPROX, int PREC, bool IW = false>
 __device__ __attribute__((noinline)) void S_dtau_line_integrated_bwd(
     const KParams *kpp, const double *erf_tab, OwnLine ol, uint32_t cp, uint32_t np, uint32_t cpi,
     uint32_t npi, uint32_t lineidx, double currfreq, double nextfreq, double curr_interp,
     double next_interp, double dZ, double dtau_bar, double Sc_bar, double Sn_bar, double *g_eps,
-    double *g_kap)
+    double *g_kap, double *g_iw)
 {
     const KParams &kp = *kpp;
     const mgx_model &m = kp.m;
@@ -2685,6 +2755,9 @@ __device__ __attribute__((noinline)) void S_dtau_line_integrated_bwd(
                                                       next_interp, dZ);
         line_end_bwd<PREC>(g_eps, g_kap, L, cp, cpi, lineidx, curr_interp, c, cI, 0.5 * dop * dtau_bar, Sc_bar);
         line_end_bwd<PREC>(g_eps, g_kap, L, np, npi, lineidx, next_interp, n, nI, 0.5 * dop * dtau_bar, Sn_bar);
+        if (IW)
+            dtau_iw_bwd<PREC>(g_iw, L, cp, cpi, np, npi, lineidx, c, cI, n, nI, m.line[lineidx],
+                              currfreq, nextfreq, curr_interp, next_interp, dZ, dtau_bar);
         return;
     }
     uint32_t lo = 0, hi = m.nlines;
@@ -2743,12 +2816,14 @@ __device__ __attribute__((noinline)) void S_dtau_line_integrated_bwd(
         }
         const double rd = rcp(d);
         const double sSc_bar = Sc_bar * rd, sSn_bar = Sn_bar * rd;
-        const double sum_Sc = line_dtau * Sc, sum_Sn = line_dtau * Sn;
-        const double sdt_bar =
-            clamped ? 0.0 : dtau_bar - sSc_bar * (sum_Sc * rd) - sSn_bar * (sum_Sn * rd);
-        const double ld_bar = sdt_bar + sSc_bar * Sc + sSn_bar * Sn;
+        /* one line: unclamped, S = S_l exactly and the adjoint of the line depth is
+           dtau_bar; clamped, the depth is the bound and S scales with the line depth */
+        const double ld_bar = clamped ? sSc_bar * Sc + sSn_bar * Sn : dtau_bar;
         line_end_bwd<PREC>(g_eps, g_kap, L, cp, cpi, l, curr_interp, c, cI, 0.5 * dop * ld_bar, sSc_bar * line_dtau);
         line_end_bwd<PREC>(g_eps, g_kap, L, np, npi, l, next_interp, n, nI, 0.5 * dop * ld_bar, sSn_bar * line_dtau);
+        if (IW)
+            dtau_iw_bwd<PREC>(g_iw, L, cp, cpi, np, npi, l, c, cI, n, nI, linefreq, currfreq,
+                              nextfreq, curr_interp, next_interp, dZ, ld_bar);
         return;
     }
     /* forward sums (the same operations as S_dtau_line_integrated) */
@@ -2791,6 +2866,9 @@ __device__ __attribute__((noinline)) void S_dtau_line_integrated_bwd(
                                                       nextfreq, curr_interp, next_interp, dZ);
         line_end_bwd<PREC>(g_eps, g_kap, L, cp, cpi, l, curr_interp, c, cI, 0.5 * dop * ld_bar, sSc_bar * line_dtau);
         line_end_bwd<PREC>(g_eps, g_kap, L, np, npi, l, next_interp, n, nI, 0.5 * dop * ld_bar, sSn_bar * line_dtau);
+        if (IW)
+            dtau_iw_bwd<PREC>(g_iw, L, cp, cpi, np, npi, l, c, cI, n, nI, m.line[l], currfreq,
+                              nextfreq, curr_interp, next_interp, dZ, ld_bar);
     }
 }
 
@@ -2834,17 +2912,23 @@ __device__ inline void entry_eta_chi(const KParams &kp, const OwnLine &ol, const
 }
 
 /* back-propagates an entry's collected (eta, chi) adjoints */
-template <int APPROX, int PREC>
+template <int APPROX, int PREC, bool IW = false>
 __device__ inline void entry_bwd(const KParams &kp, const OwnLine &ol, const Entry &e, double freq,
                                  uint32_t line, uint32_t lown, const EntryBar &eb, GradSink &g)
 {
     if (eb.eta_bar == 0.0 && eb.chi_bar == 0.0) return;
     if (eb.phi == eb.phi) {
-        /* eta = phi eps, chi = chi_min + phi kappa at this entry's own point */
-        sink_add(g, e.nr, lown, eb.phi * eb.eta_bar, eb.phi * eb.chi_bar);
+        /* eta = phi eps, chi = chi_min + phi kappa at this entry's own point, phi the
+           profile of its inverse width (phi = 0: no line in the window) */
+        double dw = 0.0;
+        if (IW && eb.phi != 0.0) {
+            const double4 a = lp(kp, e.nr, lown);
+            dw = gaussian_dw(a.x, freq * e.shift - ol.a_g) * (a.y * eb.eta_bar + a.z * eb.chi_bar);
+        }
+        sink_add<IW>(g, e.nr, lown, eb.phi * eb.eta_bar, eb.phi * eb.chi_bar, dw);
     } else {
-        eta_chi_interp_bwd<APPROX, PREC>(&kp, ol, e.nr, e.nri, e.ifac, line, freq * e.shift,
-                                         eb.eta_bar, eb.chi_bar, g.g_eps, g.g_kap);
+        eta_chi_interp_bwd<APPROX, PREC, IW>(&kp, ol, e.nr, e.nri, e.ifac, line, freq * e.shift,
+                                             eb.eta_bar, eb.chi_bar, g.g_eps, g.g_kap, g.g_iw);
     }
 }
 
@@ -2855,7 +2939,7 @@ __device__ inline void entry_bwd(const KParams &kp, const OwnLine &ol, const Ent
  * d dtau / d mean-opacity when g >= 0, else by recomputation); trapezoid
  * segments add the adjoints of (eta, chi) at their two entries to ecu / enx.
  */
-template <int APPROX, int PREC>
+template <int APPROX, int PREC, bool IW = false>
 __device__ void segment_bwd(const KParams &kp, const double *erf_tab, const OwnLine &ol,
                             const Entry &cu, const Entry &nx, double dZ, double freq,
                             uint32_t line, uint32_t lown, double d, double kind, double dtau_bar,
@@ -2867,21 +2951,26 @@ __device__ void segment_bwd(const KParams &kp, const double *erf_tab, const OwnL
         const double4 c = lp(kp, cu.nr, lown), n = lp(kp, nx.nr, lown);
         const double rd = rcp(d);
         const double sSc_bar = Sc_bar * rd, sSn_bar = Sn_bar * rd;
-        const double sdt_bar = dtau_bar - sSc_bar * (fma(d, c.w, 0.0) * rd) -
-                               sSn_bar * (fma(d, n.w, 0.0) * rd);
-        const double ld_bar = sdt_bar + sSc_bar * c.w + sSn_bar * n.w;
+        /* the segment's sources are the line's own (S = d S_l / d): the adjoint of the
+           line depth is dtau_bar (the source terms cancel exactly) */
+        const double ld_bar = dtau_bar;
         const double op_bar = 0.5 * kind * ld_bar;
         const double rc = sSc_bar * d * rcp(c.z), rn = sSn_bar * d * rcp(n.z);
-        sink_add(g, cu.nr, lown, rc, op_bar - rc * c.w);
-        sink_add(g, nx.nr, lown, rn, op_bar - rn * n.w);
+        double wb = 0.0;
+        if (IW && ld_bar != 0.0)
+            wb = 0.5 * ld_bar * dtau_line_dw((c.x + n.x) / 2.0, (n.z + c.z) / 2.0, ol.a_g,
+                                             freq * cu.shift, freq * nx.shift, dZ);
+        sink_add<IW>(g, cu.nr, lown, rc, op_bar - rc * c.w, wb);
+        sink_add<IW>(g, nx.nr, lown, rn, op_bar - rn * n.w, wb);
         return;
     }
     if (kind == MGX_SEG_ERF_EMPTY) return;
     const double cf = freq * cu.shift, nf = freq * nx.shift;
     if (kind == MGX_SEG_ERF_GENERIC) {
-        S_dtau_line_integrated_bwd<APPROX, PREC>(&kp, erf_tab, ol, cu.nr, nx.nr, cu.nri, nx.nri,
-                                                 line, cf, nf, cu.ifac, nx.ifac, dZ, dtau_bar,
-                                                 Sc_bar, Sn_bar, g.g_eps, g.g_kap);
+        S_dtau_line_integrated_bwd<APPROX, PREC, IW>(&kp, erf_tab, ol, cu.nr, nx.nr, cu.nri,
+                                                     nx.nri, line, cf, nf, cu.ifac, nx.ifac, dZ,
+                                                     dtau_bar, Sc_bar, Sn_bar, g.g_eps, g.g_kap,
+                                                     g.g_iw);
         return;
     }
     entry_eta_chi<APPROX, PREC>(kp, ol, cu, freq, line, lown, ecu);
@@ -2934,7 +3023,7 @@ struct RevCentre {
     PointECT ec;       /* the centre entry's cache from the forward sweeps */
 };
 
-template <int APPROX, int PREC>
+template <int APPROX, int PREC, bool IW = false>
 __device__ __forceinline__ void reverse_half(const KParams &kp, const double *s_erf, const OwnLine &ol,
                              bool backward, uint32_t S, uint32_t o, double shift0,
                              double shift_centre, const uint32_t *s_nxt, const double *s_dZ,
@@ -2956,6 +3045,7 @@ __device__ __forceinline__ void reverse_half(const KParams &kp, const double *s_
     g.p_crt = o;
     g.p_nxt = it.nxt;
     g.e_crt = g.k_crt = g.e_nxt = g.k_nxt = 0.0;
+    g.w_crt = g.w_nxt = 0.0;
 
     double u_in = rc.u, mu_in = rc.mu, lam_in = rc.lam, dbar = rc.dbar;
     EntryBar ein = {rc.ec.eta, rc.ec.chi, rc.ec.valid ? rc.ec.phi : __builtin_nan(""), 0.0, 0.0,
@@ -2991,17 +3081,17 @@ __device__ __forceinline__ void reverse_half(const KParams &kp, const double *s_
         EntryBar eout = {0.0, 0.0, phi, 0.0, 0.0, false};
         if (!backward) {
             const double Sn_bar = (k == K - 1 && rc.centre_last) ? rc.lam : 0.0;
-            segment_bwd<APPROX, PREC>(kp, s_erf, ol, outer, inner, dz, freq, l, lown, d, kind,
+            segment_bwd<APPROX, PREC, IW>(kp, s_erf, ol, outer, inner, dz, freq, l, lown, d, kind,
                                       dbar, lam_k, Sn_bar, g, eout, ein);
         } else {
             const double Sn_bar = (k == 0) ? lam_k : 0.0;
-            segment_bwd<APPROX, PREC>(kp, s_erf, ol, inner, outer, dz, freq, l, lown, d, kind,
+            segment_bwd<APPROX, PREC, IW>(kp, s_erf, ol, inner, outer, dz, freq, l, lown, d, kind,
                                       dbar, lam_in, Sn_bar, g, ein, eout);
         }
         /* the inner entry has all its contributions now */
-        entry_bwd<APPROX, PREC>(kp, ol, inner, freq, l, lown, ein, g);
+        entry_bwd<APPROX, PREC, IW>(kp, ol, inner, freq, l, lown, ein, g);
         if (k == 0) {
-            entry_bwd<APPROX, PREC>(kp, ol, outer, freq, l, lown, eout, g);
+            entry_bwd<APPROX, PREC, IW>(kp, ol, outer, freq, l, lown, eout, g);
             break;
         }
         /* move outward */
@@ -3022,10 +3112,10 @@ __device__ __forceinline__ void reverse_half(const KParams &kp, const double *s_
             it.s++;
             hi_load_step(kp, it, l, o, shift0, s_nxt, s_dZ, s_sh, mtab);
             it.i = 1;
-            sink_step(g, it.nxt);
+            sink_step<IW>(g, it.nxt);
         }
     }
-    sink_finish(g);
+    sink_finish<IW>(g);
 }
 
 /*
@@ -3105,10 +3195,10 @@ __device__ inline CentreRow centre_row(const KParams &kp, uint32_t o, double fre
  * the last quadrature bin repeat it with a zero seed so every slot reduces its
  * gradients over 32 lanes.
  */
-template <int APPROX, int PREC>
+template <int APPROX, int PREC, bool IW>
 __global__ __launch_bounds__(MGX_WAVE, MGX_ADJ_WAVES_PER_SIMD) void feautrier_adjoint_kernel(
     const KParams *__restrict__ kpp, const double *__restrict__ J_bar, double *g_eps,
-    double *g_kap, double *scratch, uint32_t capE)
+    double *g_kap, double *g_iw, double *scratch, uint32_t capE)
 {
     const KParams &kp = *kpp;
     extern __shared__ unsigned char smem[];
@@ -3196,18 +3286,18 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_ADJ_WAVES_PER_SIMD) void feautrier_ad
                 const CentreRow c = centre_row<PREC>(kp, o, freq, 1.0, S0 > 0, S1 > 0, fw, bw);
                 const double mu_c = ubar * c.xc * rcp(c.Dc);
                 const double lam_c = ubar * c.xc;
-                GradSink g{g_eps, g_kap, m.nlines, l, 32, 0, 0, 0.0, 0.0, 0.0, 0.0};
+                GradSink g{g_eps, g_kap, m.nlines, l, 32, 0, 0, 0.0, 0.0, 0.0, 0.0, g_iw, 0.0, 0.0};
                 if (S0 > 0) {
                     const RevCentre rc{c.u, mu_c, lam_c, -lam_c * c.dd_fw, S1 == 0, ec_cent};
-                    reverse_half<APPROX, PREC>(kp, s_erf, ol, false, S0, o, sh00, 1.0, s_nxt0,
-                                               s_dZ0, s_sh0, s_m + (jj * 2 + 0) * cap, l, freq,
-                                               st0, rc, g);
+                    reverse_half<APPROX, PREC, IW>(kp, s_erf, ol, false, S0, o, sh00, 1.0, s_nxt0,
+                                                   s_dZ0, s_sh0, s_m + (jj * 2 + 0) * cap, l,
+                                                   freq, st0, rc, g);
                 }
                 if (S1 > 0) {
                     const RevCentre rc{c.u, mu_c, lam_c, -lam_c * c.dd_bw, false, ec_cent};
-                    reverse_half<APPROX, PREC>(kp, s_erf, ol, true, S1, o, sh01, 1.0, s_nxt1,
-                                               s_dZ1, s_sh1, s_m + (jj * 2 + 1) * cap, l, freq,
-                                               st1, rc, g);
+                    reverse_half<APPROX, PREC, IW>(kp, s_erf, ol, true, S1, o, sh01, 1.0, s_nxt1,
+                                                   s_dZ1, s_sh1, s_m + (jj * 2 + 1) * cap, l,
+                                                   freq, st1, rc, g);
                 }
             }
         }
@@ -3221,10 +3311,10 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_ADJ_WAVES_PER_SIMD) void feautrier_ad
  * (lanes past F repeat the last frequency with a zero seed).  I = 2 u_last -
  * I_bdy, so the seed of u at the start point is 2 I_bar.
  */
-template <int APPROX, int PREC>
+template <int APPROX, int PREC, bool IW>
 __global__ __launch_bounds__(MGX_WAVE, MGX_ADJ_WAVES_PER_SIMD) void image_adjoint_kernel(
     const KParams *__restrict__ kpp, const ImageFrame *frp, const double *__restrict__ I_bar,
-    double *g_eps, double *g_kap, double *scratch, uint32_t capE)
+    double *g_eps, double *g_kap, double *g_iw, double *scratch, uint32_t capE)
 {
     const KParams &kp = *kpp;
     extern __shared__ unsigned char smem[];
@@ -3356,10 +3446,11 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_ADJ_WAVES_PER_SIMD) void image_adjoin
             const double lam_c = ubar * c.xc;
             const uint32_t l0 = __shfl(own_l, 0, MGX_WAVE);
             const bool uniform = __all(own_l == l0);
-            GradSink g{g_eps, g_kap, m.nlines, own_l, uniform ? 64u : 0u, 0, 0, 0.0, 0.0, 0.0, 0.0};
+            GradSink g{g_eps, g_kap, m.nlines, own_l, uniform ? 64u : 0u, 0, 0, 0.0, 0.0, 0.0, 0.0,
+                       g_iw, 0.0, 0.0};
             const RevCentre rc{c.u, mu_c, lam_c, -lam_c * c.dd_fw, true, ec};
-            reverse_half<APPROX, PREC>(kp, s_erf, ol, false, S, ip, shift0, shift_centre, s_nxt,
-                                       s_dZ, s_sh, s_m, line, freq, st, rc, g);
+            reverse_half<APPROX, PREC, IW>(kp, s_erf, ol, false, S, ip, shift0, shift_centre,
+                                           s_nxt, s_dZ, s_sh, s_m, line, freq, st, rc, g);
         }
     }
 }
@@ -3622,7 +3713,8 @@ uint32_t persistent_waves(K kernel, size_t lds, uint32_t work)
  */
 template <typename Launch, typename Grid>
 int adjoint_launch(hipStream_t st, DevBuffers &b, KParams &kp, double *d_eps, double *d_kap,
-                   size_t PL, size_t per_wave_segments_factor, Grid grid_of, Launch launch)
+                   double *d_iw, size_t PL, size_t per_wave_segments_factor, Grid grid_of,
+                   Launch launch)
 {
     uint32_t cap = 128;
     uint32_t capE = g_capE_hint;
@@ -3636,6 +3728,7 @@ int adjoint_launch(hipStream_t st, DevBuffers &b, KParams &kp, double *d_eps, do
         HIP_TRY(hipMemsetAsync(b.status, 0, 4 * sizeof(uint32_t), st));
         HIP_TRY(hipMemsetAsync(d_eps, 0, PL * sizeof(double), st));
         HIP_TRY(hipMemsetAsync(d_kap, 0, PL * sizeof(double), st));
+        if (d_iw) HIP_TRY(hipMemsetAsync(d_iw, 0, PL * sizeof(double), st));
         HIP_TRY(hipMemcpyAsync(b.kp_dev, &kp, sizeof(KParams), hipMemcpyHostToDevice, st));
         launch(lds, grid, capE);
         HIP_TRY(hipGetLastError());
@@ -3664,21 +3757,29 @@ int adjoint_launch(hipStream_t st, DevBuffers &b, KParams &kp, double *d_eps, do
     }
 }
 
-template <int A, int PR>
-int run_J_adjoint(hipStream_t st, DevBuffers &b, KParams &kp, const double *J_bar, double *d_eps,
-                  double *d_kap)
+template <int A, int PR, bool IW>
+int run_J_adjoint_k(hipStream_t st, DevBuffers &b, KParams &kp, const double *J_bar,
+                    double *d_eps, double *d_kap, double *d_iw)
 {
-    auto kern = feautrier_adjoint_kernel<A, PR>;
+    auto kern = feautrier_adjoint_kernel<A, PR, IW>;
     const uint32_t P = kp.m.npoints;
     return adjoint_launch(
-        st, b, kp, d_eps, d_kap, (size_t)P * kp.m.nlines, 2,
+        st, b, kp, d_eps, d_kap, d_iw, (size_t)P * kp.m.nlines, 2,
         [&](size_t lds) { return persistent_waves(kern, lds, P); },
         [&](size_t lds, uint32_t grid, uint32_t capE) {
             timed(st, [&] {
                 hipLaunchKernelGGL(kern, dim3(grid), dim3(MGX_WAVE), lds, st, b.kp_dev, J_bar, d_eps,
-                                   d_kap, g_ws, capE);
+                                   d_kap, d_iw, g_ws, capE);
             });
         });
+}
+
+template <int A, int PR>
+int run_J_adjoint(hipStream_t st, DevBuffers &b, KParams &kp, const double *J_bar, double *d_eps,
+                  double *d_kap, double *d_iw)
+{
+    return d_iw ? run_J_adjoint_k<A, PR, true>(st, b, kp, J_bar, d_eps, d_kap, d_iw)
+                : run_J_adjoint_k<A, PR, false>(st, b, kp, J_bar, d_eps, d_kap, nullptr);
 }
 
 
@@ -4290,8 +4391,8 @@ int mgx_image_optical_depth_ray_device(const mgx_model *model, const mgx_options
     return run_image_ray<IMG_OPTICAL_DEPTH>(model, opts, rr, tau, ImX, ImY, (hipStream_t)stream);
 }
 
-int mgx_feautrier_adjoint_device(const mgx_model *model, const mgx_options *opts,
-                                 const double *J_bar, double *d_eps, double *d_kap, void *stream)
+static int J_adjoint_entry(const mgx_model *model, const mgx_options *opts, const double *J_bar,
+                           double *d_eps, double *d_kap, double *d_iw, void *stream)
 {
     mgx::clear_error();
     if (int e = check_view(model, opts)) return e;
@@ -4304,20 +4405,38 @@ int mgx_feautrier_adjoint_device(const mgx_model *model, const mgx_options *opts
     const bool fp64 = opts->precision == MGX_PREC_FP64;
     switch (opts->approx) {
     case MGX_APPROX_NONE:
-        return fp64 ? run_J_adjoint<MGX_APPROX_NONE, MGX_PREC_FP64>(st, b, kp, J_bar, d_eps, d_kap)
-                    : run_J_adjoint<MGX_APPROX_NONE, MGX_PREC_MIXED>(st, b, kp, J_bar, d_eps, d_kap);
+        return fp64 ? run_J_adjoint<MGX_APPROX_NONE, MGX_PREC_FP64>(st, b, kp, J_bar, d_eps, d_kap, d_iw)
+                    : run_J_adjoint<MGX_APPROX_NONE, MGX_PREC_MIXED>(st, b, kp, J_bar, d_eps, d_kap, d_iw);
     case MGX_APPROX_ONE_LINE:
-        return fp64 ? run_J_adjoint<MGX_APPROX_ONE_LINE, MGX_PREC_FP64>(st, b, kp, J_bar, d_eps, d_kap)
-                    : run_J_adjoint<MGX_APPROX_ONE_LINE, MGX_PREC_MIXED>(st, b, kp, J_bar, d_eps, d_kap);
+        return fp64 ? run_J_adjoint<MGX_APPROX_ONE_LINE, MGX_PREC_FP64>(st, b, kp, J_bar, d_eps, d_kap, d_iw)
+                    : run_J_adjoint<MGX_APPROX_ONE_LINE, MGX_PREC_MIXED>(st, b, kp, J_bar, d_eps, d_kap, d_iw);
     default:
-        return fp64 ? run_J_adjoint<MGX_APPROX_CLOSE_LINES, MGX_PREC_FP64>(st, b, kp, J_bar, d_eps, d_kap)
-                    : run_J_adjoint<MGX_APPROX_CLOSE_LINES, MGX_PREC_MIXED>(st, b, kp, J_bar, d_eps, d_kap);
+        return fp64 ? run_J_adjoint<MGX_APPROX_CLOSE_LINES, MGX_PREC_FP64>(st, b, kp, J_bar, d_eps, d_kap, d_iw)
+                    : run_J_adjoint<MGX_APPROX_CLOSE_LINES, MGX_PREC_MIXED>(st, b, kp, J_bar, d_eps, d_kap, d_iw);
     }
 }
 
-int mgx_image_adjoint_device(const mgx_model *model, const mgx_options *opts, const double *raydir,
-                             uint32_t nx, uint32_t ny, const double *I_bar, double *d_eps,
-                             double *d_kap, void *stream)
+int mgx_feautrier_adjoint_device(const mgx_model *model, const mgx_options *opts,
+                                 const double *J_bar, double *d_eps, double *d_kap, void *stream)
+{
+    return J_adjoint_entry(model, opts, J_bar, d_eps, d_kap, nullptr, stream);
+}
+
+int mgx_feautrier_adjoint_width_device(const mgx_model *model, const mgx_options *opts,
+                                       const double *J_bar, double *d_eps, double *d_kap,
+                                       double *d_inverse_width, void *stream)
+{
+    if (!d_inverse_width) {
+        mgx::clear_error();
+        return mgx::fail(MGX_E_ARG, "null argument");
+    }
+    return J_adjoint_entry(model, opts, J_bar, d_eps, d_kap, d_inverse_width, stream);
+}
+
+static int image_adjoint_entry(const mgx_model *model, const mgx_options *opts,
+                               const double *raydir, uint32_t nx, uint32_t ny,
+                               const double *I_bar, double *d_eps, double *d_kap, double *d_iw,
+                               void *stream)
 {
     mgx::clear_error();
     if (!I_bar || !d_eps || !d_kap) return mgx::fail(MGX_E_ARG, "null argument");
@@ -4335,20 +4454,47 @@ int mgx_image_adjoint_device(const mgx_model *model, const mgx_options *opts, co
     const bool fp64 = opts->precision == MGX_PREC_FP64;
     auto run = [&](auto kern) {
         return adjoint_launch(
-            st, b, kp, d_eps, d_kap, PL, 1,
+            st, b, kp, d_eps, d_kap, d_iw, PL, 1,
             [&](size_t lds) { return persistent_waves(kern, lds, nx * ny); },
             [&](size_t lds, uint32_t grid, uint32_t capE) {
                 timed(st, [&] {
                     hipLaunchKernelGGL(kern, dim3(grid), dim3(MGX_WAVE), lds, st, b.kp_dev, dfr,
-                                       I_bar, d_eps, d_kap, g_ws, capE);
+                                       I_bar, d_eps, d_kap, d_iw, g_ws, capE);
                 });
             });
     };
+    if (d_iw) {
+        if (opts->approx == MGX_APPROX_NONE)
+            return fp64 ? run(image_adjoint_kernel<MGX_APPROX_NONE, MGX_PREC_FP64, true>)
+                        : run(image_adjoint_kernel<MGX_APPROX_NONE, MGX_PREC_MIXED, true>);
+        return fp64 ? run(image_adjoint_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_FP64, true>)
+                    : run(image_adjoint_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_MIXED, true>);
+    }
     if (opts->approx == MGX_APPROX_NONE)
-        return fp64 ? run(image_adjoint_kernel<MGX_APPROX_NONE, MGX_PREC_FP64>)
-                    : run(image_adjoint_kernel<MGX_APPROX_NONE, MGX_PREC_MIXED>);
-    return fp64 ? run(image_adjoint_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_FP64>)
-                : run(image_adjoint_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_MIXED>);
+        return fp64 ? run(image_adjoint_kernel<MGX_APPROX_NONE, MGX_PREC_FP64, false>)
+                    : run(image_adjoint_kernel<MGX_APPROX_NONE, MGX_PREC_MIXED, false>);
+    return fp64 ? run(image_adjoint_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_FP64, false>)
+                : run(image_adjoint_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_MIXED, false>);
+}
+
+int mgx_image_adjoint_device(const mgx_model *model, const mgx_options *opts, const double *raydir,
+                             uint32_t nx, uint32_t ny, const double *I_bar, double *d_eps,
+                             double *d_kap, void *stream)
+{
+    return image_adjoint_entry(model, opts, raydir, nx, ny, I_bar, d_eps, d_kap, nullptr, stream);
+}
+
+int mgx_image_adjoint_width_device(const mgx_model *model, const mgx_options *opts,
+                                   const double *raydir, uint32_t nx, uint32_t ny,
+                                   const double *I_bar, double *d_eps, double *d_kap,
+                                   double *d_inverse_width, void *stream)
+{
+    if (!d_inverse_width) {
+        mgx::clear_error();
+        return mgx::fail(MGX_E_ARG, "null argument");
+    }
+    return image_adjoint_entry(model, opts, raydir, nx, ny, I_bar, d_eps, d_kap, d_inverse_width,
+                               stream);
 }
 
 void mgx_release_workspace(void)

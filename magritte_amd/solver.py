@@ -154,9 +154,11 @@ def trace_stats(dm, opts):
     return tuple(int(x) for x in st)
 
 
-def feautrier_adjoint(dm, opts, J_bar, view=None):
+def feautrier_adjoint(dm, opts, J_bar, view=None, want_iw=False):
     """(d_eps, d_kap), each (P, L): the gradient of sum(J_bar * J) with respect to
-    the line emissivity and opacity (mgx_feautrier_adjoint_device)."""
+    the line emissivity and opacity (mgx_feautrier_adjoint_device); with want_iw
+    also d_iw, the gradient with respect to the inverse line widths
+    (mgx_feautrier_adjoint_width_device)."""
     torch = _require_gpu()
     m = dm.model
     jb = J_bar.to(device=dm.device, dtype=torch.float64).contiguous()
@@ -165,6 +167,12 @@ def feautrier_adjoint(dm, opts, J_bar, view=None):
     de = torch.empty((m.npoints, m.nlines), dtype=torch.float64, device=dm.device)
     dk = torch.empty_like(de)
     v = dm.view if view is None else view
+    if want_iw:
+        dw = torch.empty_like(de)
+        _abi.check(_abi.lib().mgx_feautrier_adjoint_width_device(
+            C.byref(v), C.byref(opts), _abi.ptr(jb), _abi.ptr(de), _abi.ptr(dk), _abi.ptr(dw),
+            dm.stream_ptr()))
+        return de, dk, dw
     _abi.check(_abi.lib().mgx_feautrier_adjoint_device(
         C.byref(v), C.byref(opts), _abi.ptr(jb), _abi.ptr(de), _abi.ptr(dk), dm.stream_ptr()))
     return de, dk
@@ -183,8 +191,9 @@ def image_view(dm, freqs, arrays=None):
     return m.view(t, nfreqs=freqs["nu"].shape[1]), t
 
 
-def image_adjoint(dm, opts, raydir, nx, ny, freqs, I_bar, view=None):
-    """(d_eps, d_kap): the gradient of sum(I_bar * I) for image_new's I."""
+def image_adjoint(dm, opts, raydir, nx, ny, freqs, I_bar, view=None, want_iw=False):
+    """(d_eps, d_kap[, d_iw]): the gradient of sum(I_bar * I) for image_new's I
+    (mgx_image_adjoint_device, or mgx_image_adjoint_width_device with want_iw)."""
     torch = _require_gpu()
     m = dm.model
     F = freqs["nu"].shape[1]
@@ -196,6 +205,12 @@ def image_adjoint(dm, opts, raydir, nx, ny, freqs, I_bar, view=None):
     de = torch.empty((m.npoints, m.nlines), dtype=torch.float64, device=dm.device)
     dk = torch.empty_like(de)
     rd = np.ascontiguousarray(raydir, dtype=np.float64)
+    if want_iw:
+        dw = torch.empty_like(de)
+        _abi.check(_abi.lib().mgx_image_adjoint_width_device(
+            C.byref(view), C.byref(opts), rd.ctypes.data, nx, ny, _abi.ptr(ib), _abi.ptr(de),
+            _abi.ptr(dk), _abi.ptr(dw), dm.stream_ptr()))
+        return de, dk, dw
     _abi.check(_abi.lib().mgx_image_adjoint_device(
         C.byref(view), C.byref(opts), rd.ctypes.data, nx, ny, _abi.ptr(ib), _abi.ptr(de),
         _abi.ptr(dk), dm.stream_ptr()))

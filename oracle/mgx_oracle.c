@@ -445,10 +445,33 @@ static uint32_t get_ray_lengths_max(const mgx_model *m, const mgx_options *op, i
 /* ------------------------------------------------------------------------- */
 /* line profile, Planck, boundary (solver.tpp :1106-1142, thermodynamics.tpp) */
 
+/* Finite-difference aid of the adjoint checks with respect to the inverse line
+   widths: with it set, the line profile and the error-function difference of
+   the line-integrated optical depth are evaluated in double (the reference's
+   float expf and erff put steps of ~6e-8 relative into a difference quotient).
+   Off by default. */
+static int g_smooth_profile = 0;
+
 static inline double gaussian(double inverse_width, double diff)
 {
     const double sqrt_exp = inverse_width * diff;
+    if (g_smooth_profile) return inverse_width * C_INVERSE_SQRT_PI * exp(-sqrt_exp * sqrt_exp);
     return inverse_width * C_INVERSE_SQRT_PI * (double)O_EXPF((float)(-sqrt_exp * sqrt_exp));
+}
+
+/* erf(next_pos) - erf(curr_pos) of compute_dtau_single_line: the difference of
+   two float erff values, rounded to float, as the reference takes it */
+static inline double erf_difference(double next_pos, double curr_pos)
+{
+    if (g_smooth_profile) {
+        /* in a wing both values are near +-1: take the difference of the
+           complementary functions there, which keeps its relative accuracy */
+        if (next_pos >= 0.0 && curr_pos >= 0.0) return erfc(curr_pos) - erfc(next_pos);
+        if (next_pos <= 0.0 && curr_pos <= 0.0) return erfc(-next_pos) - erfc(-curr_pos);
+        return erf(next_pos) - erf(curr_pos);
+    }
+    const float d = O_ERFF((float)next_pos) - O_ERFF((float)curr_pos);
+    return (double)d;
 }
 
 static inline double planck(double temp, double freq)
@@ -628,8 +651,8 @@ static double compute_dtau_single_line(const mgx_model *m, const mgx_options *op
     const double curr_pos = (linefreq - curr_freq) * avg_iw;
     const double diff_pos = next_pos - curr_pos;
     const double avg_op = (op_n + op_c) / 2.0;
-    const float erf_diff = O_ERFF((float)next_pos) - O_ERFF((float)curr_pos);
-    const double erfterm = avg_op / diff_pos / 2.0 * (double)erf_diff;
+    const double erf_diff = erf_difference(next_pos, curr_pos);
+    const double erfterm = avg_op / diff_pos / 2.0 * erf_diff;
     return dz * (avg_iw * erfterm + op->min_opacity);
 }
 
@@ -2029,9 +2052,11 @@ int oracle_feautrier_system(const mgx_model *m, const mgx_options *op, uint32_t 
 /* emissivity, and depend on the opacity only through the discrete counts).   */
 
 void oracle_set_smooth_interp(int on) { g_smooth_interp = on; }
+void oracle_set_smooth_profile(int on) { g_smooth_profile = on; }
 
 typedef struct {
     double *eps, *kap; /* (P, L) accumulators */
+    double *iw;        /* (P, L) accumulator of the inverse widths, or NULL */
     uint32_t L;
 } Grad;
 
@@ -2039,6 +2064,35 @@ static inline void grad_add(Grad *g, uint32_t p, uint32_t l, double de, double d
 {
     g->eps[(size_t)p * g->L + l] += de;
     g->kap[(size_t)p * g->L + l] += dk;
+}
+
+/* reverse of interpolate_linear(iw(p), iw(pi), t) (partials 1 - t and t) */
+static inline void iw_bwd(Grad *g, uint32_t p, uint32_t pi, uint32_t l, double t, double bar)
+{
+    if (!g->iw) return;
+    g->iw[(size_t)p * g->L + l] += (1.0 - t) * bar;
+    if (t != 0.0) g->iw[(size_t)pi * g->L + l] += t * bar;
+}
+
+/* d(gaussian(w, diff))/dw = exp(-x^2) (1 - 2 x^2) / sqrt(pi), x = w diff: the
+   derivative of the smooth profile, in double */
+static inline double gaussian_dw(double w, double diff)
+{
+    const double x = w * diff;
+    return C_INVERSE_SQRT_PI * exp(-x * x) * (1.0 - 2.0 * x * x);
+}
+
+/* d(compute_dtau_single_line)/d(avg_iw) at fixed frequencies, of the smooth
+   function: dz avg_op d(profile)/dw when the two frequencies are equal, else
+   dz avg_op (erf(np) - erf(cp)) / (2 (cf - nf)) with np = (lf - nf) w and
+   cp = (lf - cf) w, whose derivative is
+   dz avg_op (np exp(-np^2) - cp exp(-cp^2)) / (sqrt(pi) (np - cp)) */
+static double dtau_line_dw(double avg_iw, double avg_op, double linefreq, double cf, double nf,
+                           double dz)
+{
+    if (cf == nf) return dz * avg_op * gaussian_dw(avg_iw, cf - linefreq);
+    const double np = (linefreq - nf) * avg_iw, cp = (linefreq - cf) * avg_iw;
+    return dz * avg_op * C_INVERSE_SQRT_PI * (np * exp(-np * np) - cp * exp(-cp * cp)) / (np - cp);
 }
 
 /* reverse of get_eta_and_chi_interp: eta_bar d(eta) + chi_bar d(chi) */
@@ -2064,11 +2118,35 @@ static void eta_chi_interp_bwd(const mgx_model *m, const mgx_options *op, uint32
         const double e1 = m->emissivity[LIDX(m, p1, l)], e2 = m->emissivity[LIDX(m, p2, l)];
         const double k1 = m->opacity[LIDX(m, p1, l)], k2 = m->opacity[LIDX(m, p2, l)];
         double de1, de2, dk1, dk2;
-        interp_log_partials(e1, e2, t, interp_log(e1, e2, t), &de1, &de2);
-        interp_log_partials(k1, k2, t, interp_log(k1, k2, t), &dk1, &dk2);
+        const double ei = interp_log(e1, e2, t), ki = interp_log(k1, k2, t);
+        interp_log_partials(e1, e2, t, ei, &de1, &de2);
+        interp_log_partials(k1, k2, t, ki, &dk1, &dk2);
         grad_add(g, p1, l, prof * eta_bar * de1, prof * chi_bar * dk1);
         if (t != 0.0) grad_add(g, p2, l, prof * eta_bar * de2, prof * chi_bar * dk2);
+        /* eta = prof eps_i, chi = chi_min + prof kap_i */
+        iw_bwd(g, p1, p2, l, t, gaussian_dw(iw, diff) * (eta_bar * ei + chi_bar * ki));
     }
+}
+
+/* the inverse-width part of the reverse of compute_dtau_single_line: bar is the
+   adjoint of the line's optical depth */
+static void dtau_line_iw_bwd(const mgx_model *m, uint32_t ci, uint32_t ni, uint32_t cii,
+                             uint32_t nii, uint32_t l, double curr_freq, double next_freq,
+                             double curr_interp, double next_interp, double dz, double bar,
+                             Grad *g)
+{
+    if (!g->iw || bar == 0.0) return;
+    const double iw_c =
+        interp_linear(m->inverse_width[LIDX(m, ci, l)], m->inverse_width[LIDX(m, cii, l)], curr_interp);
+    const double iw_n =
+        interp_linear(m->inverse_width[LIDX(m, ni, l)], m->inverse_width[LIDX(m, nii, l)], next_interp);
+    const double avg_iw = (iw_c + iw_n) / 2.0;
+    const double op_c = interp_log(m->opacity[LIDX(m, ci, l)], m->opacity[LIDX(m, cii, l)], curr_interp);
+    const double op_n = interp_log(m->opacity[LIDX(m, ni, l)], m->opacity[LIDX(m, nii, l)], next_interp);
+    const double w_bar =
+        0.5 * bar * dtau_line_dw(avg_iw, (op_c + op_n) / 2.0, m->line[l], curr_freq, next_freq, dz);
+    iw_bwd(g, ci, cii, l, curr_interp, w_bar);
+    iw_bwd(g, ni, nii, l, next_interp, w_bar);
 }
 
 /* d(compute_dtau_single_line)/d(avg_op) */
@@ -2088,8 +2166,8 @@ static double dtau_single_line_dop(const mgx_model *m, const mgx_options *op, ui
     const double next_pos = (linefreq - next_freq) * avg_iw;
     const double curr_pos = (linefreq - curr_freq) * avg_iw;
     const double diff_pos = next_pos - curr_pos;
-    const float erf_diff = O_ERFF((float)next_pos) - O_ERFF((float)curr_pos);
-    return dz * (avg_iw * ((double)erf_diff / diff_pos / 2.0));
+    const double erf_diff = erf_difference(next_pos, curr_pos);
+    return dz * (avg_iw * (erf_diff / diff_pos / 2.0));
 }
 
 /* reverse of interpolate_log(kappa(p), kappa(pi), t) */
@@ -2134,6 +2212,8 @@ static void S_dtau_line_integrated_bwd(const mgx_model *m, const mgx_options *op
         opacity_bwd(m, np, npi, lineidx, next_interp, 0.5 * dop * dtau_bar, g);
         line_source_bwd(m, cp, cpi, lineidx, curr_interp, Sc_bar, g);
         line_source_bwd(m, np, npi, lineidx, next_interp, Sn_bar, g);
+        dtau_line_iw_bwd(m, cp, np, cpi, npi, lineidx, currfreq, nextfreq, curr_interp,
+                         next_interp, dZ, dtau_bar, g);
         return;
     }
     uint32_t lo = 0, hi = m->nlines;
@@ -2179,13 +2259,19 @@ static void S_dtau_line_integrated_bwd(const mgx_model *m, const mgx_options *op
                                                           nextfreq, curr_interp, next_interp, dZ);
         const double Sc = line_source(m, cp, cpi, l, curr_interp);
         const double Sn = line_source(m, np, npi, l, next_interp);
-        const double ld_bar = sdt_bar + sSc_bar * Sc + sSn_bar * Sn;
+        /* adjoint of the line's optical depth: dtau_bar + (S_l - S) S_bar / d per side.
+           With one line and no clamp S = S_l exactly, and the two source terms, which
+           can exceed dtau_bar by many orders, cancel: take dtau_bar itself. */
+        const double ld_bar = (hi == lo + 1 && !clamped) ? dtau_bar
+                                                         : sdt_bar + sSc_bar * Sc + sSn_bar * Sn;
         const double dop = dtau_single_line_dop(m, op, cp, np, cpi, npi, l, currfreq, nextfreq,
                                                 curr_interp, next_interp, dZ);
         opacity_bwd(m, cp, cpi, l, curr_interp, 0.5 * dop * ld_bar, g);
         opacity_bwd(m, np, npi, l, next_interp, 0.5 * dop * ld_bar, g);
         line_source_bwd(m, cp, cpi, l, curr_interp, sSc_bar * line_dtau, g);
         line_source_bwd(m, np, npi, l, next_interp, sSn_bar * line_dtau, g);
+        dtau_line_iw_bwd(m, cp, np, cpi, npi, l, currfreq, nextfreq, curr_interp, next_interp,
+                         dZ, ld_bar, g);
     }
 }
 
@@ -2359,27 +2445,29 @@ static void ray_adjoint(const mgx_model *m, const mgx_options *op, const Scratch
     }
 }
 
-static int grad_threads_init(Grad *g, uint32_t P, uint32_t L)
+static int grad_threads_init(Grad *g, uint32_t P, uint32_t L, int want_iw)
 {
     g->L = L;
     g->eps = calloc((size_t)P * L, sizeof(double));
     g->kap = calloc((size_t)P * L, sizeof(double));
-    return (g->eps && g->kap) ? 0 : -1;
+    g->iw = want_iw ? calloc((size_t)P * L, sizeof(double)) : NULL;
+    return (g->eps && g->kap && (g->iw || !want_iw)) ? 0 : -1;
 }
 
-static void grad_reduce(const Grad *g, double *d_eps, double *d_kap, size_t PL)
+static void grad_reduce(const Grad *g, double *d_eps, double *d_kap, double *d_iw, size_t PL)
 {
 #pragma omp critical
     {
         for (size_t i = 0; i < PL; i++) {
             d_eps[i] += g->eps[i];
             d_kap[i] += g->kap[i];
+            if (d_iw) d_iw[i] += g->iw[i];
         }
     }
 }
 
 int oracle_J_adjoint(const mgx_model *m, const mgx_options *op, const double *J_bar,
-                     double *d_eps, double *d_kap, int nthreads)
+                     double *d_eps, double *d_kap, double *d_iw, int nthreads)
 {
     int err = check_model(m, op);
     if (err) return err;
@@ -2393,14 +2481,15 @@ int oracle_J_adjoint(const mgx_model *m, const mgx_options *op, const double *J_
     if (err) goto done;
     memset(d_eps, 0, PL * sizeof(double));
     memset(d_kap, 0, PL * sizeof(double));
+    if (d_iw) memset(d_iw, 0, PL * sizeof(double));
 #pragma omp parallel
     {
         Scratch s;
         AdjWork w;
-        Grad g;
+        Grad g = {0};
         int serr = scratch_init(&s, length);
         if (!serr) serr = adjwork_init(&w, length);
-        if (!serr) serr = grad_threads_init(&g, m->npoints, m->nlines);
+        if (!serr) serr = grad_threads_init(&g, m->npoints, m->nlines, d_iw != NULL);
         for (uint32_t rr = 0; rr < m->hnrays; rr++) {
             const uint32_t ar = m->antipod[rr];
 #pragma omp for schedule(static)
@@ -2428,7 +2517,7 @@ int oracle_J_adjoint(const mgx_model *m, const mgx_options *op, const double *J_
                 }
             }
         }
-        if (!serr) grad_reduce(&g, d_eps, d_kap, PL);
+        if (!serr) grad_reduce(&g, d_eps, d_kap, d_iw, PL);
 #pragma omp critical
         {
             if (serr && !err) err = serr;
@@ -2437,6 +2526,7 @@ int oracle_J_adjoint(const mgx_model *m, const mgx_options *op, const double *J_
         adjwork_free(&w);
         free(g.eps);
         free(g.kap);
+        free(g.iw);
     }
 done:
 #ifdef _OPENMP
@@ -2447,7 +2537,7 @@ done:
 
 int oracle_image_adjoint(const mgx_model *m, const mgx_options *op, const double *raydir_in,
                          uint32_t nx, uint32_t ny, const double *I_bar, double *d_eps,
-                         double *d_kap, int nthreads)
+                         double *d_kap, double *d_iw, int nthreads)
 {
     int err = check_model(m, op);
     if (err) return err;
@@ -2472,15 +2562,16 @@ int oracle_image_adjoint(const mgx_model *m, const mgx_options *op, const double
     if (!err) {
         memset(d_eps, 0, PL * sizeof(double));
         memset(d_kap, 0, PL * sizeof(double));
+        if (d_iw) memset(d_iw, 0, PL * sizeof(double));
         const uint32_t length = image_scratch_length(m, op, &fr, rd, ImX, ImY, npix, start_bdy);
 #pragma omp parallel
         {
             Scratch s;
             AdjWork w;
-            Grad g;
+            Grad g = {0};
             int serr = scratch_init(&s, length);
             if (!serr) serr = adjwork_init(&w, length);
-            if (!serr) serr = grad_threads_init(&g, m->npoints, m->nlines);
+            if (!serr) serr = grad_threads_init(&g, m->npoints, m->nlines, d_iw != NULL);
 #pragma omp for schedule(static)
             for (size_t pix = 0; pix < npix; pix++) {
                 if (serr) continue;
@@ -2492,7 +2583,7 @@ int oracle_image_adjoint(const mgx_model *m, const mgx_options *op, const double
                     if (ib != 0.0) ray_adjoint(m, op, &s, &w, cb, f, s.last, 2.0 * ib, &g);
                 }
             }
-            if (!serr) grad_reduce(&g, d_eps, d_kap, PL);
+            if (!serr) grad_reduce(&g, d_eps, d_kap, d_iw, PL);
 #pragma omp critical
             {
                 if (serr && !err) err = serr;
@@ -2501,6 +2592,7 @@ int oracle_image_adjoint(const mgx_model *m, const mgx_options *op, const double
             adjwork_free(&w);
             free(g.eps);
             free(g.kap);
+            free(g.iw);
         }
     }
 #ifdef _OPENMP

@@ -58,16 +58,21 @@ int oracle_feautrier_system(const mgx_model *m, const mgx_options *op, uint32_t 
                             uint32_t f, uint32_t capacity, uint32_t *n_out, uint32_t *centre_out,
                             double *dtau, double *S, double *I_bdy, double *u_c, double *L_c);
 
-/* Vector-Jacobian products (adjoints) with respect to emissivity and opacity
-   (P, L): d_eps = (dJ/d eps)^T J_bar and d_kap likewise; the image variant
-   takes I_bar (nx*ny, F).  The outputs are overwritten. */
+/* Vector-Jacobian products (adjoints) with respect to emissivity, opacity and
+   the inverse line widths (P, L): d_eps = (dJ/d eps)^T J_bar and d_kap, d_iw
+   likewise (d_iw may be NULL); the image variant takes I_bar (nx*ny, F).  The
+   outputs are overwritten.  The inverse-width derivative is that of the smooth
+   profile and error function (the reference evaluates them in float). */
 int oracle_J_adjoint(const mgx_model *m, const mgx_options *op, const double *J_bar,
-                     double *d_eps, double *d_kap, int nthreads);
+                     double *d_eps, double *d_kap, double *d_iw, int nthreads);
 int oracle_image_adjoint(const mgx_model *m, const mgx_options *op, const double *raydir,
                          uint32_t nx, uint32_t ny, const double *I_bar, double *d_eps,
-                         double *d_kap, int nthreads);
+                         double *d_kap, double *d_iw, int nthreads);
 /* 1: interpolate_log in double (smooth; for finite-difference checks), 0: as the reference */
 void oracle_set_smooth_interp(int on);
+/* 1: line profile and error-function difference in double (smooth in the inverse
+   widths; for finite-difference checks), 0: as the reference */
+void oracle_set_smooth_profile(int on);
 
 /* Sparse level-population update in long double (mgx_oracle_nlte.c).  The
    handle copies every array it is given; m is a host model view (sizes,

@@ -36,11 +36,12 @@ def lib(kind="cr"):
             fn = getattr(h, "oracle_" + f)
             fn.argtypes = [C.c_float]
             fn.restype = C.c_float
-        h.oracle_J_adjoint.argtypes = [vp, vp, vp, vp, vp, C.c_int]
-        h.oracle_image_adjoint.argtypes = [vp, vp, vp, C.c_uint32, C.c_uint32, vp, vp, vp,
+        h.oracle_J_adjoint.argtypes = [vp, vp, vp, vp, vp, vp, C.c_int]
+        h.oracle_image_adjoint.argtypes = [vp, vp, vp, C.c_uint32, C.c_uint32, vp, vp, vp, vp,
                                            C.c_int]
-        h.oracle_set_smooth_interp.argtypes = [C.c_int]
-        h.oracle_set_smooth_interp.restype = None
+        for f in ("oracle_set_smooth_interp", "oracle_set_smooth_profile"):
+            getattr(h, f).argtypes = [C.c_int]
+            getattr(h, f).restype = None
         h.oracle_powf.argtypes = [C.c_float, C.c_float]
         h.oracle_powf.restype = C.c_float
         _libs[kind] = h
@@ -225,22 +226,33 @@ def set_smooth_interp(on, kind="cr"):
     lib(kind).oracle_set_smooth_interp(1 if on else 0)
 
 
-def J_adjoint(model, opts, J_bar, nthreads=0, kind="cr"):
+def set_smooth_profile(on, kind="cr"):
+    """Evaluate the line profile and the error-function difference of the
+    line-integrated optical depth in double (smooth in the inverse widths), for
+    finite-difference checks of the adjoint; off restores the float expf/erff."""
+    lib(kind).oracle_set_smooth_profile(1 if on else 0)
+
+
+def J_adjoint(model, opts, J_bar, nthreads=0, kind="cr", want_iw=False):
     """(d_eps, d_kap), each (P, L): the gradient of sum(J_bar * J) with respect to
-    the line emissivity and opacity."""
+    the line emissivity and opacity; with want_iw also d_iw, the gradient with
+    respect to the inverse line widths."""
     m = model.view()
     P, L = model.npoints, model.nlines
     jb = np.ascontiguousarray(J_bar, dtype=np.float64)
     assert jb.shape == (P, model.nfreqs)
     de = np.zeros((P, L))
     dk = np.zeros((P, L))
+    dw = np.zeros((P, L)) if want_iw else None
     _check(lib(kind).oracle_J_adjoint(C.addressof(m), C.addressof(opts), jb.ctypes.data,
-                                      de.ctypes.data, dk.ctypes.data, nthreads), "J_adjoint")
-    return de, dk
+                                      de.ctypes.data, dk.ctypes.data,
+                                      dw.ctypes.data if want_iw else None, nthreads), "J_adjoint")
+    return (de, dk, dw) if want_iw else (de, dk)
 
 
-def image_adjoint(model, opts, raydir, nx, ny, I_bar, freqs=None, nthreads=0, kind="cr"):
-    """(d_eps, d_kap): the gradient of sum(I_bar * I) for the image_new image."""
+def image_adjoint(model, opts, raydir, nx, ny, I_bar, freqs=None, nthreads=0, kind="cr",
+                  want_iw=False):
+    """(d_eps, d_kap[, d_iw]): the gradient of sum(I_bar * I) for the image_new image."""
     F = model.nfreqs if freqs is None else freqs["nu"].shape[1]
     m = model.view(freqs, nfreqs=F)
     rd = np.ascontiguousarray(raydir, dtype=np.float64)
@@ -248,10 +260,12 @@ def image_adjoint(model, opts, raydir, nx, ny, I_bar, freqs=None, nthreads=0, ki
     assert ib.shape == (nx * ny, F)
     de = np.zeros((model.npoints, model.nlines))
     dk = np.zeros((model.npoints, model.nlines))
+    dw = np.zeros((model.npoints, model.nlines)) if want_iw else None
     _check(lib(kind).oracle_image_adjoint(C.addressof(m), C.addressof(opts), rd.ctypes.data, nx,
                                           ny, ib.ctypes.data, de.ctypes.data, dk.ctypes.data,
-                                          nthreads), "image_adjoint")
-    return de, dk
+                                          dw.ctypes.data if want_iw else None, nthreads),
+           "image_adjoint")
+    return (de, dk, dw) if want_iw else (de, dk)
 
 
 # ---------------------------------------------------------------------------
