@@ -50,6 +50,9 @@ def parse():
     ap.add_argument("--n-models", type=int, default=64)
     ap.add_argument("--image-size", type=int, default=256,
                     help="image-gradient workload: pixels per image side")
+    ap.add_argument("--nlte-image-size", type=int, default=512,
+                    help="nlte workload: pixels per side of the image whose gradient with "
+                         "respect to (log n, T) follows the loop (0: none)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 512 MiB three times (scale of rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
@@ -516,7 +519,68 @@ def run_nlte(args):
                                "cores": 1, "kind": "port",
                                "sample": "one long-double statistical-equilibrium update of "
                                          "every point (the oracle, single thread)"}
+    if noff == 0 and args.nlte_image_size > 0:
+        res["config"]["image_gradient"] = _nlte_image_gradient(m, dm, lp, args.nlte_image_size)
     print(json.dumps(res), flush=True)
+
+
+def _nlte_image_gradient(m, dm, lp, n_pix):
+    """After the loop: the gradient of the L2 distance between an n_pix^2 x 31 image of the
+    converged model (one line, along a seeded direction) and the image of the same model
+    with 10% log-normal perturbations of density and temperature, with respect to the
+    log-density and temperature of every point.  The level fractions are held at their
+    converged non-LTE values (LineThermo populations='fixed'), so the temperature acts
+    through the Doppler widths and the density scales the populations."""
+    import numpy as np
+    import torch
+
+    from magritte_amd import _abi, autograd, models
+
+    P = m.npoints
+    nlev = m.linedata[0]["nlev"]
+    pop, tot = lp.populations()
+    frac = (pop.reshape(P, nlev) / tot[0][:, None]).cpu().numpy()
+    th = autograd.LineThermo(m, dm.device, populations="fixed", fractions=[frac])
+    ln0, T0 = th.log_density(), th.temperature()
+    with torch.no_grad():
+        e0, k0, w0 = th(ln0, T0)
+    # the map reproduces the loop's line data at the model's own density and temperature
+    match = max(float(((e0 - dm.tensors["emissivity"]).abs() / dm.tensors["emissivity"]).max()),
+                float(((k0 - dm.tensors["opacity"]).abs() / dm.tensors["opacity"]).max()))
+    rng = np.random.Generator(np.random.PCG64(models.seed_for(4)))
+    rd = rng.standard_normal(3)
+    rd /= np.linalg.norm(rd)
+    nu0 = m.line[3]
+    fr = m.image_frequencies(nu0 * (1 - 1e-4), nu0 * (1 + 1e-4), 31)
+    op = m.options(precision=_abi.PREC_MIXED)
+    pn = torch.from_numpy(0.1 * rng.standard_normal(P)).to(dm.device)
+    pt = torch.from_numpy(np.exp(0.1 * rng.standard_normal(P))).to(dm.device)
+    with torch.no_grad():
+        Istar = autograd.image_log_density_temperature(ln0 + pn, T0 * pt, th, dm, op, rd, n_pix,
+                                                       n_pix, fr)
+    lib = _abi.lib()
+    lib.mgx_set_kernel_timing(1)
+    for rep in range(2):  # second pass timed (warm)
+        ln = ln0.clone().requires_grad_(True)
+        T = T0.clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        I = autograd.image_log_density_temperature(ln, T, th, dm, op, rd, n_pix, n_pix, fr)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        fwd_k = lib.mgx_last_kernel_ms()
+        ((I - Istar) ** 2).sum().backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        adj_k = lib.mgx_last_kernel_ms()
+    lib.mgx_set_kernel_timing(0)
+    return {"image": f"{n_pix}x{n_pix}x31 around line 3, mixed precision",
+            "populations": "converged non-LTE level fractions held fixed",
+            "line_data_match_rel": match,
+            "forward_ms": round((t1 - t0) * 1e3, 3), "forward_kernel_ms": round(fwd_k, 3),
+            "backward_ms": round((t2 - t1) * 1e3, 3), "adjoint_kernel_ms": round(adj_k, 3),
+            "cost_vs_forward": round(adj_k / fwd_k, 3),
+            "grad_norm_log_n": float(ln.grad.norm()), "grad_norm_T": float(T.grad.norm())}
 
 
 def run_siblings(args):
