@@ -46,6 +46,55 @@ struct Parameters {
     Real max_distance_opacity_contribution = 10.0;
     Real max_interpolation_diff = 1.4;
     bool mixed_precision = false; /* MGX_PREC_MIXED instead of MGX_PREC_FP64 */
+    /* the level-population loop's knobs (parameters.hpp:76-104; mgx_nlte_params) */
+    Real pop_prec = 1.0e-6;
+    Real min_rel_pop_for_convergence = 1.0e-10;
+    Real population_inversion_fraction = -1.0;
+    Real min_line_emissivity = 1.0e-50;
+    Real min_line_opacity = 1.0e-13;
+    Real convergence_fraction = 0.995;
+    bool use_adaptive_Ng_acceleration = true;
+    Size Ng_acceleration_mem_limit = 9;
+    Size adaptive_Ng_acceleration_min_order = 2;
+    bool adaptive_Ng_acceleration_use_max_criterion = true;
+    Size Ng_acceleration_remove_N_its = 0;
+};
+
+/* CollisionPartner (src/model/lines/lineProducingSpecies/linedata/collisionPartner/
+   collisionPartner.hpp:7-33): Cd, Ce are (ntmp, ncol) row-major */
+struct CollisionPartner {
+    Size num_col_partner = 0;  /* abundance column of the partner */
+    char orth_or_para_H2 = 'n'; /* 'n', 'o' or 'p' */
+    std::vector<Size> icol, jcol;
+    std::vector<Real> tmp, Cd, Ce;
+};
+
+/* Linedata (linedata.hpp) of one line-producing species */
+struct Linedata {
+    Size num = 0; /* species column in the abundance table */
+    Size nlev = 0, nrad = 0;
+    Real inverse_mass = 0.0;
+    std::vector<Real> energy, weight;              /* (nlev) */
+    std::vector<Size> irad, jrad;                  /* (nrad) upper / lower level */
+    std::vector<Real> A, Bs, Ba, frequency;        /* (nrad) */
+    std::vector<CollisionPartner> colpar;
+};
+
+/* Lambda (src/model/lines/lineProducingSpecies/lambda/lambda.hpp): for every point p
+   and transition k of the species the approximated-Lambda elements Ls[p][k][i] and
+   their emitting points nr[p][k][i], the diagonal element first */
+struct LambdaNested {
+    std::vector<std::vector<std::vector<Real>>> Ls;
+    std::vector<std::vector<std::vector<Size>>> nr;
+};
+
+/* LineProducingSpecies (lineProducingSpecies.hpp): the line data, the level
+   populations, (P, nlev) row-major, with their totals (P), and the Lambda operator of
+   the last formal solve (filled by compute_radiation_field_feautrier_order_2) */
+struct LineProducingSpecies {
+    Linedata linedata;
+    std::vector<Real> population, population_tot;
+    LambdaNested lambda;
 };
 
 /* src/model/radiation/radiation.hpp: I (nrays, P, F; short characteristics), u and v
@@ -84,6 +133,12 @@ class Model {
     std::vector<Real> lambda_diag; /* (P, L): the diagonal Lambda entries of every line */
     LambdaLists lambda;            /* the general operator (n_off_diag > 0) */
     std::vector<Image> images;
+    /* lines.lineProducingSpecies, chemistry.species.abundance (P, nspec) and the loop's
+       per-iteration relative changes (Model::error_mean / error_max, species innermost) */
+    std::vector<LineProducingSpecies> lspecs;
+    std::vector<Real> abundance;
+    Size nspec = 0;
+    std::vector<Real> error_mean, error_max;
 
     Model() = default;
     Model(const Model &) = delete;
@@ -121,11 +176,23 @@ class Model {
     int compute_spectral_discretisation(Real nu_min, Real nu_max, Size n);
     void restore_line_grid();
 
+    /* Model::compute_LTE_level_populations (model.cpp:302-308): LTE populations of every
+       species and the line emissivity, opacity and interpolation criterion */
+    int compute_LTE_level_populations();
+    /* Model::compute_level_populations(use_Ng_acceleration, max_niterations)
+       (model.cpp:654-744; the general Lambda operator with n_off_diag bands and the
+       global statistical equilibrium) and Model::compute_level_populations_sparse
+       (model.cpp:756-850): from the current populations (LTE when none are set);
+       returns the number of statistical-equilibrium iterations */
+    int compute_level_populations(bool use_Ng_acceleration, long max_niterations);
+    int compute_level_populations_sparse(bool use_Ng_acceleration, long max_niterations);
+
     mgx_options options() const;
     /* the device view (uploads stale arrays first) */
     const mgx_model &device_view();
 
   private:
+    int level_population_loop(bool sparse, bool use_Ng_acceleration, long max_niterations);
     std::map<std::string, std::vector<unsigned char>> host, saved_line_grid;
     std::map<std::string, void *> dev;
     std::map<std::string, bool> stale;

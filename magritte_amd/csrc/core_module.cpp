@@ -15,11 +15,21 @@
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
+#include <pybind11/stl_bind.h>
 
 #include "magritte_amd.hpp"
 
+/* species and collision partners are bound as lists that alias the model's own
+   (model.lspecs[0].linedata.A = ... changes the model, as lineProducingSpecies does) */
+PYBIND11_MAKE_OPAQUE(std::vector<magritte::LineProducingSpecies>)
+PYBIND11_MAKE_OPAQUE(std::vector<magritte::CollisionPartner>)
+
 namespace py = pybind11;
+using magritte::CollisionPartner;
 using magritte::Image;
+using magritte::LambdaNested;
+using magritte::Linedata;
+using magritte::LineProducingSpecies;
 using magritte::Model;
 using magritte::Parameters;
 using magritte::Radiation;
@@ -63,7 +73,76 @@ PYBIND11_MODULE(core, m)
         .def_readwrite("max_distance_opacity_contribution",
                        &Parameters::max_distance_opacity_contribution)
         .def_readwrite("max_interpolation_diff", &Parameters::max_interpolation_diff)
-        .def_readwrite("mixed_precision", &Parameters::mixed_precision);
+        .def_readwrite("mixed_precision", &Parameters::mixed_precision)
+        .def_readwrite("pop_prec", &Parameters::pop_prec)
+        .def_readwrite("min_rel_pop_for_convergence", &Parameters::min_rel_pop_for_convergence)
+        .def_readwrite("population_inversion_fraction", &Parameters::population_inversion_fraction)
+        .def_readwrite("min_line_emissivity", &Parameters::min_line_emissivity)
+        .def_readwrite("min_line_opacity", &Parameters::min_line_opacity)
+        .def_readwrite("convergence_fraction", &Parameters::convergence_fraction)
+        .def_readwrite("use_adaptive_Ng_acceleration", &Parameters::use_adaptive_Ng_acceleration)
+        .def_readwrite("Ng_acceleration_mem_limit", &Parameters::Ng_acceleration_mem_limit)
+        .def_readwrite("adaptive_Ng_acceleration_min_order",
+                       &Parameters::adaptive_Ng_acceleration_min_order)
+        .def_readwrite("adaptive_Ng_acceleration_use_max_criterion",
+                       &Parameters::adaptive_Ng_acceleration_use_max_criterion)
+        .def_readwrite("Ng_acceleration_remove_N_its", &Parameters::Ng_acceleration_remove_N_its);
+
+    py::class_<CollisionPartner>(m, "CollisionPartner")
+        .def(py::init<>())
+        .def_readwrite("num_col_partner", &CollisionPartner::num_col_partner)
+        .def_readwrite("orth_or_para_H2", &CollisionPartner::orth_or_para_H2)
+        .def_readwrite("icol", &CollisionPartner::icol)
+        .def_readwrite("jcol", &CollisionPartner::jcol)
+        .def_readwrite("tmp", &CollisionPartner::tmp)
+        .def_readwrite("Cd", &CollisionPartner::Cd)
+        .def_readwrite("Ce", &CollisionPartner::Ce);
+    py::bind_vector<std::vector<CollisionPartner>>(m, "VectorCollisionPartner");
+
+    py::class_<Linedata>(m, "Linedata")
+        .def(py::init<>())
+        .def_readwrite("num", &Linedata::num)
+        .def_readwrite("nlev", &Linedata::nlev)
+        .def_readwrite("nrad", &Linedata::nrad)
+        .def_readwrite("inverse_mass", &Linedata::inverse_mass)
+        .def_readwrite("energy", &Linedata::energy)
+        .def_readwrite("weight", &Linedata::weight)
+        .def_readwrite("irad", &Linedata::irad)
+        .def_readwrite("jrad", &Linedata::jrad)
+        .def_readwrite("A", &Linedata::A)
+        .def_readwrite("Bs", &Linedata::Bs)
+        .def_readwrite("Ba", &Linedata::Ba)
+        .def_readwrite("frequency", &Linedata::frequency)
+        .def_readwrite("colpar", &Linedata::colpar);
+
+    /* lspec.Lambda.Ls / .nr as nested lists [point][transition][element] (pybindings.cpp:570-580) */
+    py::class_<LambdaNested>(m, "Lambda")
+        .def_readonly("Ls", &LambdaNested::Ls)
+        .def_readonly("nr", &LambdaNested::nr);
+
+    py::class_<LineProducingSpecies>(m, "LineProducingSpecies")
+        .def(py::init<>())
+        .def_readwrite("linedata", &LineProducingSpecies::linedata)
+        .def_readonly("Lambda", &LineProducingSpecies::lambda)
+        .def_property(
+            "population",
+            [](py::object self) {
+                LineProducingSpecies &s = self.cast<LineProducingSpecies &>();
+                return view_of(s.population, {(py::ssize_t)s.population.size()}, self);
+            },
+            [](LineProducingSpecies &s, py::array_t<double, py::array::c_style | py::array::forcecast> a) {
+                s.population.assign(a.data(), a.data() + a.size());
+            })
+        .def_property(
+            "population_tot",
+            [](py::object self) {
+                LineProducingSpecies &s = self.cast<LineProducingSpecies &>();
+                return view_of(s.population_tot, {(py::ssize_t)s.population_tot.size()}, self);
+            },
+            [](LineProducingSpecies &s, py::array_t<double, py::array::c_style | py::array::forcecast> a) {
+                s.population_tot.assign(a.data(), a.data() + a.size());
+            });
+    py::bind_vector<std::vector<LineProducingSpecies>>(m, "VectorLineProducingSpecies");
 
     py::class_<Radiation>(m, "Radiation")
         .def_property_readonly("u",
@@ -199,5 +278,39 @@ PYBIND11_MODULE(core, m)
         .def("compute_spectral_discretisation",
              (int (Model::*)(Real, Real, Size)) & Model::compute_spectral_discretisation,
              "Uniform image frequency grid nu_min..nu_max with n points.")
-        .def("restore_line_grid", &Model::restore_line_grid);
+        .def("restore_line_grid", &Model::restore_line_grid)
+        .def_readwrite("lspecs", &Model::lspecs,
+                       "lines.lineProducingSpecies: line data, populations and Lambda")
+        .def_property(
+            "abundance",
+            [](py::object self) {
+                Model &mo = self.cast<Model &>();
+                return view_of(mo.abundance, {mo.parameters.npoints, mo.nspec}, self);
+            },
+            [](Model &mo, py::array_t<double, py::array::c_style | py::array::forcecast> a) {
+                if (a.ndim() != 2) throw std::runtime_error("abundance must be (npoints, nspec)");
+                mo.nspec = (Size)a.shape(1);
+                mo.abundance.assign(a.data(), a.data() + a.size());
+            },
+            "chemistry.species.abundance (npoints, nspec)")
+        .def(
+            "get_array",
+            [](const Model &mo, const std::string &name) {
+                const auto &v = mo.get_array(name);
+                return py::bytes(reinterpret_cast<const char *>(v.data()), v.size());
+            },
+            "The bytes of a model array (as set, or as the level-population loop left it).")
+        .def_readonly("error_mean", &Model::error_mean)
+        .def_readonly("error_max", &Model::error_max)
+        .def("compute_LTE_level_populations", &Model::compute_LTE_level_populations,
+             "LTE level populations and the line emissivity, opacity, interpolation criterion.")
+        .def("compute_level_populations", &Model::compute_level_populations,
+             py::arg("use_Ng_acceleration"), py::arg("max_niterations"),
+             "Non-LTE level populations: formal solve with the general Lambda operator "
+             "(parameters.n_off_diag bands) and the global statistical equilibrium per "
+             "iteration, with Ng acceleration; returns the number of iterations.")
+        .def("compute_level_populations_sparse", &Model::compute_level_populations_sparse,
+             py::arg("use_Ng_acceleration"), py::arg("max_niterations"),
+             "Non-LTE level populations with the Lambda diagonal and the per-point "
+             "statistical equilibrium; returns the number of iterations.");
 }

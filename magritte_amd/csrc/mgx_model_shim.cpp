@@ -331,6 +331,40 @@ void Solver::setup(Model &model)
 
 namespace {
 
+/* lspec.lambda of every species from the model-wide lists (or the diagonal): the
+   reference's Lambda::add_element order, line l of the model being transition
+   line_k[l] of species line_species[l] */
+void fill_species_lambda(Model &model)
+{
+    const Parameters &p = model.parameters;
+    if (model.lspecs.size() != p.nlspecs) return;
+    const auto &ls_b = model.get_array("line_species");
+    const auto &lk_b = model.get_array("line_k");
+    const uint32_t *lsp = reinterpret_cast<const uint32_t *>(ls_b.data());
+    const uint32_t *lk = reinterpret_cast<const uint32_t *>(lk_b.data());
+    const size_t P = p.npoints, L = p.nlines;
+    for (auto &s : model.lspecs) {
+        s.lambda.Ls.assign(P, std::vector<std::vector<Real>>(s.linedata.nrad));
+        s.lambda.nr.assign(P, std::vector<std::vector<Size>>(s.linedata.nrad));
+    }
+    const LambdaLists &G = model.lambda;
+    for (size_t q = 0; q < P; q++)
+        for (size_t l = 0; l < L; l++) {
+            LineProducingSpecies &s = model.lspecs.at(lsp[l]);
+            if (lk[l] >= s.linedata.nrad) throw std::runtime_error("line_k out of range");
+            auto &Ls = s.lambda.Ls[q][lk[l]];
+            auto &nr = s.lambda.nr[q][lk[l]];
+            if (G.cap == 0) {
+                Ls.assign(1, model.lambda_diag[q * L + l]);
+                nr.assign(1, (Size)q);
+            } else {
+                const size_t i = q * L + l, n = G.count[i];
+                Ls.assign(G.val.begin() + i * G.cap, G.val.begin() + i * G.cap + n);
+                nr.assign(G.nr.begin() + i * G.cap, G.nr.begin() + i * G.cap + n);
+            }
+        }
+}
+
 void set_shapes(Radiation &r, const mgx_model &v)
 {
     r.nrays = v.nrays;
@@ -393,6 +427,7 @@ void Solver::solve_feautrier_order_2(Model &model)
     u.copy_to(model.radiation.u, U);
     J.copy_to(model.radiation.J, PF);
     set_shapes(model.radiation, v);
+    fill_species_lambda(model);
 }
 
 void Solver::solve_shortchar_order_0(Model &model)
@@ -528,6 +563,213 @@ void Solver::image_feautrier_order_2_new_imager(Model &model, const Vector3D ray
     Y.copy_to(img.ImY, npix);
     set_image_freqs(model, img, v.nfreqs);
     model.images.push_back(std::move(img));
+}
+
+/* ------------------------------------------------------------------------- */
+/* level populations                                                         */
+
+namespace {
+
+/* the C ABI's view of the species' line and collision data (pointers into lspecs) */
+struct NlteTables {
+    std::vector<mgx_linedata> ld;
+    std::vector<uint32_t> ncolpar;
+    std::vector<mgx_colldata> colpar;
+    size_t npop = 0; /* sum over species of P * nlev */
+};
+
+NlteTables nlte_tables(const Model &model)
+{
+    const Parameters &p = model.parameters;
+    if (model.lspecs.size() != p.nlspecs)
+        throw std::runtime_error("lspecs must hold parameters.nlspecs line-producing species");
+    if (model.abundance.size() != (size_t)p.npoints * model.nspec)
+        throw std::runtime_error("abundance must be (npoints, nspec)");
+    NlteTables t;
+    for (const LineProducingSpecies &s : model.lspecs) {
+        const Linedata &d = s.linedata;
+        if (d.energy.size() != d.nlev || d.weight.size() != d.nlev || d.irad.size() != d.nrad ||
+            d.jrad.size() != d.nrad || d.A.size() != d.nrad || d.Bs.size() != d.nrad ||
+            d.Ba.size() != d.nrad || d.frequency.size() != d.nrad)
+            throw std::runtime_error("line data arrays do not match nlev / nrad");
+        mgx_linedata x;
+        std::memset(&x, 0, sizeof x);
+        x.num = d.num;
+        x.nlev = d.nlev;
+        x.nrad = d.nrad;
+        x.inverse_mass = d.inverse_mass;
+        x.energy = d.energy.data();
+        x.weight = d.weight.data();
+        x.irad = d.irad.data();
+        x.jrad = d.jrad.data();
+        x.A = d.A.data();
+        x.Bs = d.Bs.data();
+        x.Ba = d.Ba.data();
+        x.frequency = d.frequency.data();
+        t.ld.push_back(x);
+        t.ncolpar.push_back((uint32_t)d.colpar.size());
+        for (const CollisionPartner &c : d.colpar) {
+            const size_t ntmp = c.tmp.size(), ncol = c.icol.size();
+            if (c.jcol.size() != ncol || c.Cd.size() != ntmp * ncol || c.Ce.size() != ntmp * ncol)
+                throw std::runtime_error("collision data arrays do not match ntmp / ncol");
+            mgx_colldata y;
+            std::memset(&y, 0, sizeof y);
+            y.num_col_partner = c.num_col_partner;
+            y.orth_or_para_H2 = c.orth_or_para_H2;
+            y.ntmp = (uint32_t)ntmp;
+            y.ncol = (uint32_t)ncol;
+            y.icol = c.icol.data();
+            y.jcol = c.jcol.data();
+            y.tmp = c.tmp.data();
+            y.Cd = c.Cd.data();
+            y.Ce = c.Ce.data();
+            t.colpar.push_back(y);
+        }
+        t.npop += (size_t)p.npoints * d.nlev;
+    }
+    return t;
+}
+
+mgx_nlte_params nlte_params(const Parameters &p)
+{
+    mgx_nlte_params q;
+    mgx_nlte_default_params(&q);
+    q.pop_prec = p.pop_prec;
+    q.min_rel_pop_for_convergence = p.min_rel_pop_for_convergence;
+    q.population_inversion_fraction = p.population_inversion_fraction;
+    q.min_line_emissivity = p.min_line_emissivity;
+    q.min_line_opacity = p.min_line_opacity;
+    q.convergence_fraction = p.convergence_fraction;
+    q.use_adaptive_Ng_acceleration = p.use_adaptive_Ng_acceleration ? 1 : 0;
+    q.Ng_acceleration_mem_limit = p.Ng_acceleration_mem_limit;
+    q.adaptive_Ng_acceleration_min_order = p.adaptive_Ng_acceleration_min_order;
+    q.adaptive_Ng_acceleration_use_max_criterion = p.adaptive_Ng_acceleration_use_max_criterion ? 1 : 0;
+    q.Ng_acceleration_remove_N_its = p.Ng_acceleration_remove_N_its;
+    return q;
+}
+
+/* the populations of every species, concatenated in species order */
+std::vector<double> joined_populations(const Model &model, size_t npop)
+{
+    std::vector<double> pop;
+    pop.reserve(npop);
+    for (const LineProducingSpecies &s : model.lspecs)
+        pop.insert(pop.end(), s.population.begin(), s.population.end());
+    return pop;
+}
+
+void split_populations(Model &model, const std::vector<double> &pop,
+                       const std::vector<double> &ptot)
+{
+    const size_t P = model.parameters.npoints;
+    size_t off = 0;
+    for (size_t s = 0; s < model.lspecs.size(); s++) {
+        LineProducingSpecies &ls = model.lspecs[s];
+        const size_t n = P * ls.linedata.nlev;
+        ls.population.assign(pop.begin() + off, pop.begin() + off + n);
+        ls.population_tot.assign(ptot.begin() + s * P, ptot.begin() + (s + 1) * P);
+        off += n;
+    }
+}
+
+struct NlteHandle {
+    mgx_nlte *h = nullptr;
+    ~NlteHandle()
+    {
+        if (h) mgx_nlte_destroy(h);
+    }
+};
+
+} // namespace
+
+int Model::compute_LTE_level_populations()
+{
+    const NlteTables t = nlte_tables(*this);
+    const Parameters &p = parameters;
+    const size_t P = p.npoints, PL = (size_t)p.npoints * p.nlines;
+    const auto &T = get_array("temperature");
+    if (T.size() != P * sizeof(double)) throw std::runtime_error("temperature must be (npoints)");
+    std::vector<double> pop(t.npop), ptot((size_t)p.nlspecs * P), em(PL), op(PL), ic(PL);
+    check(mgx_compute_lte(p.npoints, nspec, abundance.data(),
+                          reinterpret_cast<const double *>(T.data()), p.nlspecs, t.ld.data(),
+                          p.min_line_emissivity, p.min_line_opacity, pop.data(), ptot.data(),
+                          em.data(), op.data()));
+    check(mgx_compute_interp_crit(p.npoints, p.nlspecs, t.ld.data(), op.data(), ptot.data(),
+                                  ic.data()));
+    set_array("emissivity", em.data(), PL * sizeof(double));
+    set_array("opacity", op.data(), PL * sizeof(double));
+    set_array("interp_crit", ic.data(), PL * sizeof(double));
+    split_populations(*this, pop, ptot);
+    return 0;
+}
+
+int Model::level_population_loop(bool sparse, bool use_Ng_acceleration, long max_niterations)
+{
+    const Parameters &p = parameters;
+    NlteTables t = nlte_tables(*this);
+    bool have = true;
+    for (const LineProducingSpecies &s : lspecs)
+        have = have && s.population.size() == (size_t)p.npoints * s.linedata.nlev;
+    if (!have) compute_LTE_level_populations();
+    const mgx_model &v = device_view();
+    const mgx_options o = options();
+    const mgx_nlte_params q = nlte_params(p);
+    const size_t P = p.npoints, PL = (size_t)p.npoints * p.nlines;
+    DevOut ab(abundance.size()), pd(t.npop), pt((size_t)p.nlspecs * P);
+    hip_check(hipMemcpy(ab.p, abundance.data(), abundance.size() * sizeof(double),
+                        hipMemcpyHostToDevice),
+              "hipMemcpy");
+    const std::vector<double> pop0 = joined_populations(*this, t.npop);
+    hip_check(hipMemcpy(pd.p, pop0.data(), t.npop * sizeof(double), hipMemcpyHostToDevice),
+              "hipMemcpy");
+    NlteHandle h;
+    check(mgx_nlte_create(&v, p.nlspecs, t.ld.data(), t.ncolpar.data(),
+                          t.colpar.empty() ? nullptr : t.colpar.data(), nspec, ab.p, &q, &h.h,
+                          nullptr));
+    /* the line data the loop rewrites are the device mirrors the view points to */
+    double *em = const_cast<double *>(v.emissivity);
+    double *op = const_cast<double *>(v.opacity);
+    double *ic = const_cast<double *>(v.interp_crit);
+    check(mgx_nlte_set_populations(h.h, pd.p, em, op, ic, nullptr));
+    const size_t cap = (size_t)(max_niterations > 0 ? max_niterations : 1) * p.nlspecs;
+    std::vector<double> emean(cap), emax(cap);
+    int32_t it = 0;
+    if (sparse)
+        check(mgx_compute_level_populations_sparse(&v, &o, h.h, use_Ng_acceleration ? 1 : 0,
+                                                   max_niterations, em, op, ic, &it,
+                                                   emean.data(), emax.data(), nullptr));
+    else
+        check(mgx_compute_level_populations(&v, &o, h.h, use_Ng_acceleration ? 1 : 0,
+                                            max_niterations, em, op, ic, &it, emean.data(),
+                                            emax.data(), nullptr));
+    check(mgx_nlte_get_populations(h.h, pd.p, pt.p, nullptr));
+    std::vector<double> pop, ptot, line;
+    pd.copy_to(pop, t.npop);
+    pt.copy_to(ptot, (size_t)p.nlspecs * P);
+    split_populations(*this, pop, ptot);
+    /* the host arrays take the loop's line data (re-uploaded on the next solve) */
+    const char *names[] = {"emissivity", "opacity", "interp_crit"};
+    double *arrays[] = {em, op, ic};
+    for (int i = 0; i < 3; i++) {
+        line.resize(PL);
+        hip_check(hipMemcpy(line.data(), arrays[i], PL * sizeof(double), hipMemcpyDeviceToHost),
+                  "copy");
+        set_array(names[i], line.data(), PL * sizeof(double));
+    }
+    const size_t n = (size_t)it * p.nlspecs;
+    error_mean.assign(emean.begin(), emean.begin() + (n < cap ? n : cap));
+    error_max.assign(emax.begin(), emax.begin() + (n < cap ? n : cap));
+    return it;
+}
+
+int Model::compute_level_populations(bool use_Ng_acceleration, long max_niterations)
+{
+    return level_population_loop(false, use_Ng_acceleration, max_niterations);
+}
+
+int Model::compute_level_populations_sparse(bool use_Ng_acceleration, long max_niterations)
+{
+    return level_population_loop(true, use_Ng_acceleration, max_niterations);
 }
 
 } // namespace magritte

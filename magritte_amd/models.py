@@ -453,6 +453,35 @@ class ModelData:
             cm.set_array(k, np.ascontiguousarray(getattr(self, k)))
         cm.set_center(*[float(x) for x in self.center])
         cm.set_max_inverse_mass(float(self.max_inverse_mass))
+        # chemistry and the line-producing species (the level-population loop's inputs)
+        p.min_line_emissivity = self.params["min_line_emissivity"]
+        p.min_line_opacity = self.params["min_line_opacity"]
+        cm.abundance = np.ascontiguousarray(self.abundance)
+        off = 0
+        for s, d in enumerate(self.linedata):
+            ls = core.LineProducingSpecies()
+            ld = ls.linedata
+            ld.num, ld.nlev, ld.nrad = int(d["num"]), int(d["nlev"]), int(d["nrad"])
+            ld.inverse_mass = float(d["inverse_mass"])
+            for f in ("energy", "weight", "A", "Bs", "Ba", "frequency"):
+                setattr(ld, f, [float(x) for x in d[f]])
+            for f in ("irad", "jrad"):
+                setattr(ld, f, [int(x) for x in d[f]])
+            for c in d.get("colpar", []):
+                cp = core.CollisionPartner()
+                cp.num_col_partner = int(c["num_col_partner"])
+                cp.orth_or_para_H2 = c["orth_or_para"]
+                cp.icol, cp.jcol = [int(x) for x in c["icol"]], [int(x) for x in c["jcol"]]
+                cp.tmp = [float(x) for x in c["tmp"]]
+                cp.Cd = [float(x) for x in np.ravel(c["Cd"])]
+                cp.Ce = [float(x) for x in np.ravel(c["Ce"])]
+                ld.colpar.append(cp)
+            n = self.npoints * int(d["nlev"])
+            if getattr(self, "population", None) is not None:
+                ls.population = self.population[off:off + n]
+                ls.population_tot = self.population_tot[s]
+            off += n
+            cm.lspecs.append(ls)
         return cm
 
     def to_device(self, device="cuda"):

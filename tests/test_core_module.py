@@ -100,3 +100,65 @@ def test_core_sibling_solvers_match_device_calls(small_wind):
     np.testing.assert_array_equal(np.array(cm.lambda_nr)[used],
                                   lam["nr"].cpu().numpy().view(np.uint32)[used])
     np.testing.assert_array_equal(np.array(cm.lambda_val)[used], lam["val"].cpu().numpy()[used])
+
+
+def test_core_lte_level_populations(small_wind):
+    """Model.compute_LTE_level_populations on the host producers: the species'
+    populations and the line data equal the model generator's LTE arrays."""
+    m = small_wind
+    cm = m.to_core()
+    assert len(cm.lspecs) == m.nlspecs and cm.abundance.shape == m.abundance.shape
+    cm.lspecs[0].population = np.zeros(0)  # forget the copied populations
+    cm.compute_LTE_level_populations()
+    n = m.npoints * m.linedata[0]["nlev"]
+    np.testing.assert_array_equal(np.array(cm.lspecs[0].population), m.population[:n])
+    np.testing.assert_array_equal(np.array(cm.lspecs[0].population_tot), m.population_tot[0])
+    for k in ("emissivity", "opacity", "interp_crit"):
+        got = np.frombuffer(bytes(cm.get_array(k)), dtype=np.float64).reshape(m.npoints, m.nlines)
+        np.testing.assert_array_equal(got, getattr(m, k))
+    cm.lspecs[0].linedata.A = [1.0]  # the species list aliases the model's
+    with pytest.raises(RuntimeError, match="nlev / nrad"):
+        cm.compute_LTE_level_populations()
+
+
+@pytest.mark.gpu
+def test_core_level_populations_match_device_loop():
+    """Model.compute_level_populations_sparse / compute_level_populations (n_off_diag 2)
+    on the C++ Model give the Python device loop's iterations, errors and populations,
+    and lspec.Lambda holds the model-wide Lambda lists in the reference's nesting."""
+    import torch  # noqa: F401  (one HIP runtime: torch's)
+
+    from magritte_amd import _abi, models, solver
+
+    for noff in (0, 2):
+        m = models.model_line_1d()
+        cm = m.to_core()
+        cm.parameters.n_off_diag = noff
+        prm = _abi.default_nlte_params()
+        if noff == 0:
+            it_c = cm.compute_level_populations_sparse(True, 60)
+        else:
+            it_c = cm.compute_level_populations(True, 60)
+        dm = solver.DeviceModel(m)
+        lp = solver.LevelPopulations(dm, prm)
+        lp.set_lte()
+        op = m.options()
+        op.n_off_diag = noff
+        loop = solver.compute_level_populations if noff else solver.compute_level_populations_sparse
+        it, em, ex = loop(dm, op, lp, True, 60)
+        print(f"[core level populations] n_off_diag={noff} iterations {it_c} / {it}", flush=True)
+        assert it_c == it
+        np.testing.assert_allclose(np.array(cm.error_max), ex, rtol=1e-12, atol=0)
+        pop, tot = lp.populations()
+        np.testing.assert_allclose(np.array(cm.lspecs[0].population), pop.cpu().numpy(),
+                                   rtol=1e-12, atol=1e-14 * float(tot.max()))
+        got = np.frombuffer(bytes(cm.get_array("opacity")), dtype=np.float64)
+        np.testing.assert_allclose(got, dm.tensors["opacity"].cpu().numpy().ravel(), rtol=1e-12)
+    # Lambda of the last formal solve in lspec.Lambda.Ls / nr
+    cm.compute_radiation_field_feautrier_order_2()
+    lam = cm.lspecs[0].Lambda
+    cnt, nr, val = np.array(cm.lambda_count), np.array(cm.lambda_nr), np.array(cm.lambda_val)
+    for p in (0, 7, 50, 99):
+        c = int(cnt[p, 0])
+        assert list(lam.nr[p][0]) == list(nr[p, 0, :c])
+        np.testing.assert_array_equal(lam.Ls[p][0], val[p, 0, :c])
