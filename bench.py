@@ -36,11 +36,12 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--workload", default="wind32",
-                    choices=["wind32", "batch", "image-gradient", "nlte", "siblings"],
+                    choices=["wind32", "batch", "image-gradient", "nlte", "nlte-full", "siblings"],
                     help="wind32: the single-model solve (default); batch: many winds in one "
                          "persistent launch against per-model launches; image-gradient: the "
                          "gradient of an image loss through the adjoint; nlte: non-LTE level "
-                         "populations of a clumpy wind with a 41-level rotor; siblings: the "
+                         "populations of a clumpy wind with a 41-level rotor; nlte-full: one "
+                         "iteration of it at full size (--n-edge 100 --nrays 192); siblings: the "
                          "other per-ray solvers (short characteristics, u/v, Lambda bands, "
                          "old imager, optical-depth images) on the 32^3 wind")
     ap.add_argument("--max-iterations", type=int, default=60)
@@ -50,6 +51,8 @@ def parse():
     ap.add_argument("--n-models", type=int, default=64)
     ap.add_argument("--image-size", type=int, default=256,
                     help="image-gradient workload: pixels per image side")
+    ap.add_argument("--nrays", type=int, default=48,
+                    help="nlte workload: HEALPix rays (12 * nside^2; 192 is the full configuration)")
     ap.add_argument("--nlte-image-size", type=int, default=512,
                     help="nlte workload: pixels per side of the image whose gradient with "
                          "respect to (log n, T) follows the loop (0: none)")
@@ -381,7 +384,8 @@ def _image_gradient_lnT(m, dm, Istar, rd, nx, ny, fr, rng, lib):
 
 def run_nlte(args):
     """Non-LTE level populations of a clumpy rotating wind with a 41-level rotor
-    (the large non-LTE configuration, scaled to n_edge^3 points and 48 rays):
+    (the large non-LTE configuration at n_edge^3 points and --nrays rays; the full one
+    is 100^3 and 192):
     the native sparse loop (formal solve with the Lambda diagonal, J_eff,
     statistical equilibrium, adaptive Ng) from LTE to a 1e-4 population change.
     Reports the loop, then one more iteration split into its formal solve and
@@ -393,7 +397,7 @@ def run_nlte(args):
     from magritte_amd import _abi, models, solver
 
     n_edge = args.n_edge
-    nrays = 48
+    nrays = args.nrays
     t0 = time.time()
     clumps = models.grf_clumps(n_edge, sigma=0.5, corr_length=1.0 / 20.0)
     m = models.model_wind(n_edge=n_edge, nrays=nrays, nlev=41, v_rot=5.0e3, clumps=clumps)
@@ -524,7 +528,114 @@ def run_nlte(args):
     print(json.dumps(res), flush=True)
 
 
-def _nlte_image_gradient(m, dm, lp, n_pix):
+def _pair_view(m, dm, rr):
+    """The model's device view restricted to ray pair rr (its two directions and
+    weights, antipods 1 and 0): a formal solve of it gives that pair's part of J and
+    of the Lambda diagonal, which add up over the pairs."""
+    import numpy as np
+    import torch
+
+    d = m.ray_direction.reshape(-1, 3)
+    ar = int(m.antipod[rr])
+    t = dict(dm.tensors)
+    t["ray_direction"] = torch.from_numpy(np.ascontiguousarray([d[rr], d[ar]])).to(dm.device)
+    t["ray_weight"] = torch.from_numpy(np.array([m.ray_weight[rr], m.ray_weight[ar]])).to(dm.device)
+    t["antipod"] = torch.from_numpy(np.array([1, 0], dtype=np.uint32)).to(dm.device)
+    v = m.view(t)
+    v.nrays, v.hnrays = 2, 1
+    return v, t
+
+
+def run_nlte_full(args):
+    """One iteration of the large non-LTE configuration at its full size (n_edge^3
+    points, --nrays rays; 100^3 and 192 as configured): LTE populations, the formal
+    solve with the Lambda diagonal, the statistical-equilibrium update, then the
+    512^2 image gradient with respect to (log n, T).  The formal solve is launched one
+    ray pair at a time (each launch covers every origin) so that progress is printed
+    between launches; the pairs' J and Lambda add up to the single launch's, which a
+    12^3 check at the start confirms."""
+    import numpy as np
+    import torch
+
+    from magritte_amd import _abi, models, solver
+
+    lib = _abi.lib()
+    # the pairwise sum against one launch, on a small wind
+    ms = models.model_wind(n_edge=12, nrays=48, nlev=41, v_rot=5.0e3,
+                           clumps=models.grf_clumps(12, sigma=0.5, corr_length=1.0 / 20.0))
+    dms = solver.DeviceModel(ms, "cuda:0")
+    ref = solver.feautrier_forward(dms, ms.options(), with_u=False)
+    Js = torch.zeros_like(ref["J"])
+    Ls = torch.zeros_like(ref["lambda_diag"])
+    for rr in range(ms.hnrays):
+        v, keep = _pair_view(ms, dms, rr)
+        o = solver.feautrier_forward(dms, ms.options(), with_u=False, view=v)
+        Js += o["J"]
+        Ls += o["lambda_diag"]
+    pair_err = max(float(((Js - ref["J"]).abs() / ref["J"].abs().clamp_min(1e-300)).max()),
+                   float(((Ls - ref["lambda_diag"]).abs() / ref["lambda_diag"].abs()).max()))
+    print(f"# pairwise vs single launch (12^3, 48 rays): {pair_err:.2e}", file=sys.stderr, flush=True)
+    del dms, ref, Js, Ls
+
+    t0 = time.time()
+    clumps = models.grf_clumps(args.n_edge, sigma=0.5, corr_length=1.0 / 20.0)
+    m = models.model_wind(n_edge=args.n_edge, nrays=args.nrays, nlev=41, v_rot=5.0e3,
+                          clumps=clumps)
+    P, L, F = m.npoints, m.nlines, m.nfreqs
+    print(f"# model built in {time.time() - t0:.1f}s: P={P} lines={L} hnrays={m.hnrays} F={F}",
+          file=sys.stderr, flush=True)
+    dm = solver.DeviceModel(m, "cuda:0")
+    lp = solver.LevelPopulations(dm, _abi.default_nlte_params(pop_prec=1.0e-4))
+    lp.set_lte()
+    opts = m.options()
+    stats = solver.trace_stats(dm, opts)
+    J = torch.zeros((P, F), dtype=torch.float64, device=dm.device)
+    lam = torch.zeros((P, L), dtype=torch.float64, device=dm.device)
+    lib.mgx_set_kernel_timing(1)
+    k_ms = 0.0
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for rr in range(m.hnrays):
+        v, keep = _pair_view(m, dm, rr)
+        o = solver.feautrier_forward(dm, opts, with_u=False, view=v)
+        J += o["J"]
+        lam += o["lambda_diag"]
+        torch.cuda.synchronize()
+        k_ms += lib.mgx_last_kernel_ms()
+        del o
+        print(f"# ray pair {rr + 1}/{m.hnrays}: {time.perf_counter() - t1:.1f}s", file=sys.stderr,
+              flush=True)
+    t_fs = time.perf_counter() - t1
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    conv = lp.stateq(J, lam)
+    torch.cuda.synchronize()
+    t_se = time.perf_counter() - t2
+    lib.mgx_set_kernel_timing(0)
+    print(f"# statistical equilibrium {t_se * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    U = m.hnrays * P * F
+    res = {"metric": "Feautrier ray.point.freq/s (forward, large non-LTE configuration)",
+           "value": round(U / (k_ms * 1e-3), 1), "unit": "ray*point*freq/s", "n_gpus": 1,
+           "higher_is_better": True, "data": "synthetic", "dtype": "f64",
+           "config": {"workload": f"{args.n_edge}^3 clumpy rotating wind (sigma 0.5 log-normal GRF "
+                                  f"clumps, 5 km/s rotation + 15 km/s outflow), 41-level rotor, "
+                                  f"{args.nrays} rays, {m.nquads} bins per line, LTE start",
+                      "npoints": P, "hnrays": m.hnrays, "nfreqs": F,
+                      "trace_stats": {"walk_steps": stats[0], "neighbours": stats[1],
+                                      "traced_entries": stats[2], "interp_entries": stats[3]},
+                      "formal_solve_kernel_ms": round(k_ms, 1), "formal_solve_wall_s": round(t_fs, 2),
+                      "launches": f"one per ray pair ({m.hnrays}); pairwise vs single launch "
+                                  f"on 12^3: {pair_err:.1e}",
+                      "stateq_ms": round(t_se * 1e3, 1),
+                      "stateq_point_level_per_s": round(P * 41 / t_se, 1),
+                      "first_iteration_max_change": conv[0][2] if conv else None}}
+    if args.nlte_image_size > 0:
+        res["config"]["image_gradient"] = _nlte_image_gradient(
+            m, dm, lp, args.nlte_image_size, "level fractions after the first update held fixed")
+    print(json.dumps(res), flush=True)
+
+
+def _nlte_image_gradient(m, dm, lp, n_pix, populations="converged non-LTE level fractions held fixed"):
     """After the loop: the gradient of the L2 distance between an n_pix^2 x 31 image of the
     converged model (one line, along a seeded direction) and the image of the same model
     with 10% log-normal perturbations of density and temperature, with respect to the
@@ -575,7 +686,7 @@ def _nlte_image_gradient(m, dm, lp, n_pix):
         adj_k = lib.mgx_last_kernel_ms()
     lib.mgx_set_kernel_timing(0)
     return {"image": f"{n_pix}x{n_pix}x31 around line 3, mixed precision",
-            "populations": "converged non-LTE level fractions held fixed",
+            "populations": populations,
             "line_data_match_rel": match,
             "forward_ms": round((t1 - t0) * 1e3, 3), "forward_kernel_ms": round(fwd_k, 3),
             "backward_ms": round((t2 - t1) * 1e3, 3), "adjoint_kernel_ms": round(adj_k, 3),
@@ -652,6 +763,12 @@ def main():
             sys.exit(2)
         run_nlte(args)
         return
+    if args.workload == "nlte-full":
+        if args.gpus != 1:
+            print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+            sys.exit(2)
+        run_nlte_full(args)
+        return
     if args.calibrate_copy:
         calibrate_copy()
         return
@@ -672,7 +789,8 @@ def main():
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                          "(the multi-GPU sharding by ray pair is described in DESIGN.md)"}))
+                          "(the path would shard by ray pair with one all-reduce of J and "
+                          "Lambda per formal solve; not built)"}))
         sys.exit(2)
     import numpy as np
 

@@ -1378,6 +1378,12 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_SOLVE_WAVES_PER_SIMD) void feautrier_
     const uint32_t o = blockIdx.x;
     const uint32_t lane = threadIdx.x;
     const uint32_t F = m.nfreqs, L = m.nlines;
+    /* a walk of this launch has outgrown the walk buffer: the host relaunches the whole
+       solve with a larger one, so the origins not started yet skip their work */
+    if (__builtin_amdgcn_readfirstlane(
+            __hip_atomic_load(&kp.status[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) &
+        ERR_CAP)
+        return;
     double *s_L = reinterpret_cast<double *>(smem + ((walk_lds_bytes(kp.cap) + 7) / 8) * 8);
     double *s_J = kp.j_lds ? s_L + L : nullptr;
     for (uint32_t i = lane; i < L + (s_J ? F : 0); i += MGX_WAVE) s_L[i] = 0.0;
@@ -1406,6 +1412,12 @@ __global__ __launch_bounds__(MGX_WAVE, 2) void feautrier_bands_kernel(
     const uint32_t o = blockIdx.x;
     const uint32_t lane = threadIdx.x;
     const uint32_t F = m.nfreqs, L = m.nlines;
+    /* a walk of this launch has outgrown the walk buffer: the host relaunches the whole
+       solve with a larger one, so the origins not started yet skip their work */
+    if (__builtin_amdgcn_readfirstlane(
+            __hip_atomic_load(&kp.status[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) &
+        ERR_CAP)
+        return;
     double *s_L = reinterpret_cast<double *>(smem + ((walk_lds_bytes(kp.cap) + 7) / 8) * 8);
     double *s_J = kp.j_lds ? s_L + L : nullptr;
     for (uint32_t i = lane; i < L + (s_J ? F : 0); i += MGX_WAVE) s_L[i] = 0.0;
@@ -1433,6 +1445,12 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_SOLVE_WAVES_PER_SIMD) void shortchar_
     const uint32_t o = blockIdx.x;
     const uint32_t lane = threadIdx.x;
     const uint32_t F = m.nfreqs, L = m.nlines;
+    /* a walk of this launch has outgrown the walk buffer: the host relaunches the whole
+       solve with a larger one, so the origins not started yet skip their work */
+    if (__builtin_amdgcn_readfirstlane(
+            __hip_atomic_load(&kp.status[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) &
+        ERR_CAP)
+        return;
     double *s_L = reinterpret_cast<double *>(smem + ((walk_lds_bytes(kp.cap) + 7) / 8) * 8);
     double *s_J = kp.j_lds ? s_L + L : nullptr;
     for (uint32_t i = lane; i < L + (s_J ? F : 0); i += MGX_WAVE) s_L[i] = 0.0;
