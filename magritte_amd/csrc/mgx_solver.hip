@@ -3237,6 +3237,12 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_ADJ_WAVES_PER_SIMD) void feautrier_ad
     double *st_wave = scratch + (size_t)blockIdx.x * 2 * capE * MGX_TAPE_WORDS * MGX_WAVE;
 
     for (uint32_t o = blockIdx.x; o < P; o += gridDim.x) {
+        /* a buffer of this launch overflowed: the host clears the gradients and
+           relaunches, so the remaining origins are skipped */
+        if (__builtin_amdgcn_readfirstlane(__hip_atomic_load(&kp.status[0], __ATOMIC_RELAXED,
+                                                             __HIP_MEMORY_SCOPE_AGENT)) &
+            (ERR_CAP | ERR_CAPE))
+            return;
         for (uint32_t rr = 0; rr < m.hnrays; rr++) {
             const uint32_t ar = gld(&m.antipod[rr]);
             uint32_t S0, S1;
@@ -3353,6 +3359,12 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_ADJ_WAVES_PER_SIMD) void image_adjoin
     const double *rd = fr.rd;
 
     for (uint32_t pix = blockIdx.x; pix < npix; pix += gridDim.x) {
+        /* a buffer of this launch overflowed: the host clears the gradients and
+           relaunches, so the remaining pixels are skipped */
+        if (__builtin_amdgcn_readfirstlane(__hip_atomic_load(&kp.status[0], __ATOMIC_RELAXED,
+                                                             __HIP_MEMORY_SCOPE_AGENT)) &
+            (ERR_CAP | ERR_CAPE))
+            return;
         const uint32_t x_idx = pix / fr.ny, y_idx = pix % fr.ny;
         const double xloc = fr.min_x + fr.deltax / 2.0 + x_idx * fr.deltax;
         const double yloc = fr.min_y + fr.deltay / 2.0 + y_idx * fr.deltay;
