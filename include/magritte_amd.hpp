@@ -186,13 +186,16 @@ class Model {
        returns the number of statistical-equilibrium iterations */
     int compute_level_populations(bool use_Ng_acceleration, long max_niterations);
     int compute_level_populations_sparse(bool use_Ng_acceleration, long max_niterations);
+    /* Model::compute_level_populations_shortchar (model.cpp:859-935): the loop with the
+       order-0 short-characteristics formal solve (parameters.min_dtau) */
+    int compute_level_populations_shortchar(bool use_Ng_acceleration, long max_niterations);
 
     mgx_options options() const;
     /* the device view (uploads stale arrays first) */
     const mgx_model &device_view();
 
   private:
-    int level_population_loop(bool sparse, bool use_Ng_acceleration, long max_niterations);
+    int level_population_loop(int kind, bool use_Ng_acceleration, long max_niterations);
     std::map<std::string, std::vector<unsigned char>> host, saved_line_grid;
     std::map<std::string, void *> dev;
     std::map<std::string, bool> stale;

@@ -605,6 +605,19 @@ int mgx_compute_level_populations(const mgx_model *model_dev, const mgx_options 
                                   void *stream);
 
 /*
+ * Model::compute_level_populations_shortchar(use_Ng_acceleration, max_niterations)
+ * (model.cpp:859-935): as mgx_compute_level_populations_sparse with the formal
+ * solve of mgx_shortchar_order_0_device (Parameters::min_dtau).  Its Lambda is
+ * diagonal, so the reference's compute_Jeff and global statistical equilibrium
+ * reduce to the per-point update of mgx_nlte_iteration_stateq.
+ */
+int mgx_compute_level_populations_shortchar(const mgx_model *model_dev, const mgx_options *opts,
+                                            mgx_nlte *h, int use_ng, int64_t max_niterations,
+                                            double min_dtau, double *emissivity, double *opacity,
+                                            double *interp_crit, int32_t *iterations,
+                                            double *error_mean, double *error_max, void *stream);
+
+/*
  * Workload statistics of the trace, for the algorithmic-byte model of the
  * solve: stats[0] = walk steps S (both halves, all ray pairs and origins),
  * stats[1] = neighbours examined (sum of n_neighbors over the points stepped

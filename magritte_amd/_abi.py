@@ -203,6 +203,9 @@ SIGNATURES = [
     ("mgx_compute_level_populations", C.c_int,
      [C.POINTER(MgxModel), C.POINTER(MgxOptions), _VP, C.c_int, C.c_int64, _VP, _VP, _VP,
       _VP, _VP, _VP, _VP]),
+    ("mgx_compute_level_populations_shortchar", C.c_int,
+     [C.POINTER(MgxModel), C.POINTER(MgxOptions), _VP, C.c_int, C.c_int64, C.c_double, _VP, _VP,
+      _VP, _VP, _VP, _VP, _VP]),
     ("mgx_trace_stats_device", C.c_int, [C.POINTER(MgxModel), C.POINTER(MgxOptions), _VP, _VP]),
     ("mgx_set_kernel_timing", None, [C.c_int]),
     ("mgx_last_kernel_ms", C.c_double, []),

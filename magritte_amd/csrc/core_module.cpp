@@ -312,5 +312,9 @@ PYBIND11_MODULE(core, m)
         .def("compute_level_populations_sparse", &Model::compute_level_populations_sparse,
              py::arg("use_Ng_acceleration"), py::arg("max_niterations"),
              "Non-LTE level populations with the Lambda diagonal and the per-point "
-             "statistical equilibrium; returns the number of iterations.");
+             "statistical equilibrium; returns the number of iterations.")
+        .def("compute_level_populations_shortchar", &Model::compute_level_populations_shortchar,
+             py::arg("use_Ng_acceleration"), py::arg("max_niterations"),
+             "Non-LTE level populations with the order-0 short-characteristics formal solve; "
+             "returns the number of iterations.");
 }

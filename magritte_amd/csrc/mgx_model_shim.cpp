@@ -703,7 +703,8 @@ int Model::compute_LTE_level_populations()
     return 0;
 }
 
-int Model::level_population_loop(bool sparse, bool use_Ng_acceleration, long max_niterations)
+/* kind: 0 the general loop, 1 the sparse loop, 2 the short-characteristics loop */
+int Model::level_population_loop(int kind, bool use_Ng_acceleration, long max_niterations)
 {
     const Parameters &p = parameters;
     NlteTables t = nlte_tables(*this);
@@ -734,10 +735,14 @@ int Model::level_population_loop(bool sparse, bool use_Ng_acceleration, long max
     const size_t cap = (size_t)(max_niterations > 0 ? max_niterations : 1) * p.nlspecs;
     std::vector<double> emean(cap), emax(cap);
     int32_t it = 0;
-    if (sparse)
+    if (kind == 1)
         check(mgx_compute_level_populations_sparse(&v, &o, h.h, use_Ng_acceleration ? 1 : 0,
                                                    max_niterations, em, op, ic, &it,
                                                    emean.data(), emax.data(), nullptr));
+    else if (kind == 2)
+        check(mgx_compute_level_populations_shortchar(&v, &o, h.h, use_Ng_acceleration ? 1 : 0,
+                                                      max_niterations, p.min_dtau, em, op, ic,
+                                                      &it, emean.data(), emax.data(), nullptr));
     else
         check(mgx_compute_level_populations(&v, &o, h.h, use_Ng_acceleration ? 1 : 0,
                                             max_niterations, em, op, ic, &it, emean.data(),
@@ -764,12 +769,17 @@ int Model::level_population_loop(bool sparse, bool use_Ng_acceleration, long max
 
 int Model::compute_level_populations(bool use_Ng_acceleration, long max_niterations)
 {
-    return level_population_loop(false, use_Ng_acceleration, max_niterations);
+    return level_population_loop(0, use_Ng_acceleration, max_niterations);
 }
 
 int Model::compute_level_populations_sparse(bool use_Ng_acceleration, long max_niterations)
 {
-    return level_population_loop(true, use_Ng_acceleration, max_niterations);
+    return level_population_loop(1, use_Ng_acceleration, max_niterations);
+}
+
+int Model::compute_level_populations_shortchar(bool use_Ng_acceleration, long max_niterations)
+{
+    return level_population_loop(2, use_Ng_acceleration, max_niterations);
 }
 
 } // namespace magritte

@@ -1455,11 +1455,18 @@ namespace {
 
 /* Model::compute_level_populations_sparse (model.cpp:756-850) or, with global,
    Model::compute_level_populations (model.cpp:653-744) */
+/* the formal solve of each iteration: the plain method with the Lambda diagonal (sparse
+   update), with the general Lambda lists (global update), or order-0 short characteristics
+   (whose Lambda is diagonal, so compute_Jeff and the global update reduce to the per-point
+   update) */
+enum LoopKind { LOOP_SPARSE = 0, LOOP_GLOBAL = 1, LOOP_SHORTCHAR = 2 };
+
 int level_populations_loop(const mgx_model *m, const mgx_options *opts, mgx_nlte *h, int use_ng,
                            int64_t max_niterations, double *eps, double *kap, double *ic,
                            int32_t *iterations, double *error_mean, double *error_max,
-                           hipStream_t st, bool global)
+                           hipStream_t st, int kind, double min_dtau = 0.0)
 {
+    const bool global = kind == LOOP_GLOBAL;
     if (!m || !opts || !h || !eps || !kap || !ic) return mgx::fail(MGX_E_ARG, "null argument");
     if (eps != m->emissivity || kap != m->opacity || ic != m->interp_crit)
         return mgx::fail(MGX_E_ARG, "emissivity, opacity and interp_crit must be the model view's arrays");
@@ -1528,7 +1535,13 @@ int level_populations_loop(const mgx_model *m, const mgx_options *opts, mgx_nlte
                                                          nullptr, nullptr, nullptr, st))
                 return e;
         } else {
-            if (int e = mgx_feautrier_forward_device(m, opts, nullptr, h->J, h->lambda, st)) return e;
+            if (kind == LOOP_SHORTCHAR) {
+                if (int e = mgx_shortchar_order_0_device(m, opts, min_dtau, nullptr, nullptr,
+                                                         nullptr, h->J, h->lambda, st))
+                    return e;
+            } else if (int e = mgx_feautrier_forward_device(m, opts, nullptr, h->J, h->lambda, st)) {
+                return e;
+            }
             if (int e = mgx_nlte_iteration_stateq(h, h->J, h->lambda, eps, kap, ic, cv.data(),
                                                   nullptr, nullptr, st))
                 return e;
@@ -1557,7 +1570,19 @@ int mgx_compute_level_populations_sparse(const mgx_model *m, const mgx_options *
 {
     mgx::clear_error();
     return level_populations_loop(m, opts, h, use_ng, max_niterations, eps, kap, ic, iterations,
-                                  error_mean, error_max, (hipStream_t)stream, false);
+                                  error_mean, error_max, (hipStream_t)stream, LOOP_SPARSE);
+}
+
+int mgx_compute_level_populations_shortchar(const mgx_model *m, const mgx_options *opts,
+                                            mgx_nlte *h, int use_ng, int64_t max_niterations,
+                                            double min_dtau, double *eps, double *kap, double *ic,
+                                            int32_t *iterations, double *error_mean,
+                                            double *error_max, void *stream)
+{
+    mgx::clear_error();
+    return level_populations_loop(m, opts, h, use_ng, max_niterations, eps, kap, ic, iterations,
+                                  error_mean, error_max, (hipStream_t)stream, LOOP_SHORTCHAR,
+                                  min_dtau);
 }
 
 int mgx_compute_level_populations(const mgx_model *m, const mgx_options *opts, mgx_nlte *h,
@@ -1569,7 +1594,7 @@ int mgx_compute_level_populations(const mgx_model *m, const mgx_options *opts, m
     if (opts && opts->n_off_diag > MGX_MAX_OFF_DIAG)
         return mgx::fail(MGX_E_UNSUPPORTED, "n_off_diag > %d is not supported", MGX_MAX_OFF_DIAG);
     return level_populations_loop(m, opts, h, use_ng, max_niterations, eps, kap, ic, iterations,
-                                  error_mean, error_max, (hipStream_t)stream, true);
+                                  error_mean, error_max, (hipStream_t)stream, LOOP_GLOBAL);
 }
 
 } // extern "C"

@@ -464,6 +464,25 @@ def compute_level_populations(dm, opts, lp, use_ng=True, max_iterations=100):
     return int(it[0]), err_mean[:n].tolist(), err_max[:n].tolist()
 
 
+def compute_level_populations_shortchar(dm, opts, lp, use_ng=True, max_iterations=100,
+                                        min_dtau=1.0e-15):
+    """Model::compute_level_populations_shortchar on the device: the loop with the
+    order-0 short-characteristics formal solve; returns (iterations, error_mean,
+    error_max)."""
+    m = dm.model
+    err_mean = np.zeros(max(1, max_iterations) * m.nlspecs)
+    err_max = np.zeros_like(err_mean)
+    it = np.zeros(1, dtype=np.int32)
+    t = dm.tensors
+    _abi.check(_abi.lib().mgx_compute_level_populations_shortchar(
+        C.byref(dm.view), C.byref(opts), lp.h, 1 if use_ng else 0, int(max_iterations),
+        float(min_dtau), _abi.ptr(t["emissivity"]), _abi.ptr(t["opacity"]),
+        _abi.ptr(t["interp_crit"]), it.ctypes.data, err_mean.ctypes.data, err_max.ctypes.data,
+        dm.stream_ptr()))
+    n = int(it[0]) * m.nlspecs
+    return int(it[0]), err_mean[:n].tolist(), err_max[:n].tolist()
+
+
 def compute_level_populations_sparse(dm, opts, lp, use_ng=True, max_iterations=100):
     """Model::compute_level_populations_sparse on the device (the native loop of
     libmgx): returns (iterations, error_mean, error_max)."""

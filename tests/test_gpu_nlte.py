@@ -292,3 +292,44 @@ def test_van_zadelhoff_1a_gpu():
     e0, e1 = _ratran_errors(m, pop.cpu().numpy().reshape(m.npoints, 2))
     _say(f"  {it} iterations, max rel err vs RATRAN {e0:.4f} / {e1:.4f}")
     assert e0 < 0.12 and e1 < 0.13
+
+
+def test_shortchar_loop_is_the_shortchar_solve_and_the_update():
+    """Model::compute_level_populations_shortchar (model.cpp:859-935) on the device: without
+    Ng, the native loop's iterations (at most three) equal as many rounds of the order-0
+    short-characteristics solve (J, Lambda diagonal) and the per-point update, the form
+    compute_Jeff and the global update take for a diagonal Lambda; the C++ Model's
+    method gives the Python loop's iterations and populations with Ng."""
+    import torch  # noqa: F401  (one HIP runtime: torch's)
+
+    from magritte_amd import solver
+
+    m = models.model_line_1d()
+    prm = _abi.default_nlte_params()
+    dm = solver.DeviceModel(m)
+    lp = solver.LevelPopulations(dm, prm)
+    lp.set_lte()
+    it, em, ex = solver.compute_level_populations_shortchar(dm, m.options(), lp, False, 3)
+    _say(f"[shortchar loop] {it} iterations, max changes {ex}")
+    assert 1 <= it <= 3
+    dm2 = solver.DeviceModel(m)
+    lp2 = solver.LevelPopulations(dm2, prm)
+    lp2.set_lte()
+    for _ in range(it):
+        out = solver.shortchar_forward(dm2, m.options())
+        conv = lp2.stateq(out["J"], out["lambda_diag"])
+    assert conv[0][2] == ex[-1]
+    p1, _ = lp.populations()
+    p2, _ = lp2.populations()
+    np.testing.assert_array_equal(p1.cpu().numpy(), p2.cpu().numpy())
+    # with Ng, through the C++ Model
+    dm3 = solver.DeviceModel(m)
+    lp3 = solver.LevelPopulations(dm3, prm)
+    lp3.set_lte()
+    it_py, _, ex_py = solver.compute_level_populations_shortchar(dm3, m.options(), lp3, True, 60)
+    cm = models.model_line_1d().to_core()
+    it_c = cm.compute_level_populations_shortchar(True, 60)
+    _say(f"  with Ng: python {it_py}, C++ Model {it_c}")
+    assert it_c == it_py and it_py < 60
+    np.testing.assert_allclose(np.array(cm.error_max), ex_py, rtol=1e-12, atol=0)
+
