@@ -574,6 +574,19 @@ int mgx_nlte_trial_ng(mgx_nlte *h, uint32_t order, mgx_nlte_conv *conv, void *st
 int mgx_nlte_history(const mgx_nlte *h, uint32_t *n_residuals);
 
 /*
+ * The Lambda of the last formal solve an iteration loop ran (what the
+ * reference's lspec.lambda holds after compute_level_populations*,
+ * lineProducingSpecies.hpp:35): after mgx_compute_level_populations the
+ * lists -- *cap entries per (point, line), with *count, *nr and *val set and
+ * *diag NULL -- and after the sparse or short-characteristics loop the
+ * diagonal (P, L) in *diag with *cap = 0 and the list pointers NULL.  Device
+ * memory owned by the handle, valid until the next loop call or
+ * mgx_nlte_destroy.  MGX_E_ARG if no loop has run a formal solve yet.
+ */
+int mgx_nlte_last_lambda(const mgx_nlte *h, uint32_t *cap, const uint32_t **count,
+                         const uint32_t **nr, const double **val, const double **diag);
+
+/*
  * Model::compute_level_populations_sparse(use_Ng_acceleration, max_niterations)
  * (model.cpp:756-850) with the Default / Adaptive Ng criteria
  * (model.cpp:580-645): per iteration a formal solve
@@ -631,8 +644,9 @@ int mgx_trace_stats_device(const mgx_model *model_dev, const mgx_options *opts,
 /*
  * Kernel timing: when enabled, the device entry points record HIP events
  * around their main kernel on the caller's stream; mgx_last_kernel_ms()
- * returns the duration of the last one (ms).  No counterpart in the reference
- * (measurement aid).
+ * returns the duration of the last call's main kernel (ms), summed over the
+ * launches that call made (a launch whose walk outgrew its buffer is repeated
+ * with a larger one).  No counterpart in the reference (measurement aid).
  */
 void mgx_set_kernel_timing(int enable);
 double mgx_last_kernel_ms(void);

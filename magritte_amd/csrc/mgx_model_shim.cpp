@@ -76,6 +76,13 @@ struct DevOut {
     }
 };
 
+template <class T>
+void copy_from_device(std::vector<T> &v, const T *src, size_t n)
+{
+    v.resize(n);
+    if (n) hip_check(hipMemcpy(v.data(), src, n * sizeof(T), hipMemcpyDeviceToHost), "copy");
+}
+
 } // namespace
 
 Model::~Model()
@@ -446,6 +453,9 @@ void Solver::solve_shortchar_order_0(Model &model)
     J.copy_to(model.radiation.J, PF);
     lam.copy_to(model.lambda_diag, PL);
     set_shapes(model.radiation, v);
+    /* its Lambda is diagonal whatever n_off_diag says (solver.tpp:164-166, 1651, 1754) */
+    model.lambda = LambdaLists{};
+    fill_species_lambda(model);
 }
 
 void Solver::solve_feautrier_order_2_uv(Model &model)
@@ -748,6 +758,27 @@ int Model::level_population_loop(int kind, bool use_Ng_acceleration, long max_ni
                                             max_niterations, em, op, ic, &it, emean.data(),
                                             emax.data(), nullptr));
     check(mgx_nlte_get_populations(h.h, pd.p, pt.p, nullptr));
+    if (it > 0) {
+        /* lspec.lambda: the Lambda of the loop's last formal solve */
+        uint32_t lcap = 0;
+        const uint32_t *lcount = nullptr, *lnr = nullptr;
+        const double *lval = nullptr, *ldiag = nullptr;
+        check(mgx_nlte_last_lambda(h.h, &lcap, &lcount, &lnr, &lval, &ldiag));
+        if (lcap == 0) {
+            lambda = LambdaLists{};
+            copy_from_device(lambda_diag, ldiag, PL);
+        } else {
+            lambda = LambdaLists{};
+            lambda.cap = lcap;
+            copy_from_device(lambda.count, lcount, PL);
+            copy_from_device(lambda.nr, lnr, PL * lcap);
+            copy_from_device(lambda.val, lval, PL * lcap);
+            lambda_diag.assign(PL, 0.0);
+            for (size_t i = 0; i < PL; i++)
+                if (lambda.count[i]) lambda_diag[i] = lambda.val[i * lcap];
+        }
+        fill_species_lambda(*this);
+    }
     std::vector<double> pop, ptot, line;
     pd.copy_to(pop, t.npop);
     pt.copy_to(ptot, (size_t)p.nlspecs * P);

@@ -44,6 +44,7 @@
 #include "mgx_constants.h"
 #include "mgx_error.h"
 #include "mgx_fmath.h"
+#include "mgx_fmath_ref.h"
 
 #define NL_TRY(expr)                                                                           \
     do {                                                                                       \
@@ -390,7 +391,7 @@ __global__ __launch_bounds__(NL_WAVE) void stateq_kernel(NView v, const double *
             const NCol C = v.col[S.col0 + c];
             double abn = v.abundance[(size_t)p * v.nspec + C.num];
             if (C.op != 'n') {
-                const double frac_H2_para = 1.0 / (1.0 + 9.0 * mgx_expf((float)(-170.5 / T)));
+                const double frac_H2_para = 1.0 / (1.0 + 9.0 * mgx_ref_expf((float)(-170.5 / T)));
                 if (C.op == 'o') abn *= (1.0 - frac_H2_para);
                 if (C.op == 'p') abn *= frac_H2_para;
             }
@@ -858,6 +859,8 @@ struct mgx_nlte {
     uint32_t *lam_count = nullptr, *lam_nr = nullptr; /* driver's Lambda lists */
     double *lam_val = nullptr;
     size_t lam_elems = 0, lam_count_elems = 0;
+    int last_loop = -1;     /* kind of the loop whose formal solve filled J and Lambda last */
+    uint32_t last_cap = 0;  /* its list capacity (general loop) */
     hipStream_t stream = nullptr;
     std::vector<mgx_nlte_conv> conv;
 };
@@ -1472,7 +1475,9 @@ int level_populations_loop(const mgx_model *m, const mgx_options *opts, mgx_nlte
         return mgx::fail(MGX_E_ARG, "emissivity, opacity and interp_crit must be the model view's arrays");
     if (m->npoints != h->P || m->nlines != h->L || m->nfreqs != h->F)
         return mgx::fail(MGX_E_ARG, "model view does not match the level-population handle");
-    if (!global && opts->n_off_diag != 0)
+    /* the sparse update needs the diagonal Lambda; short characteristics
+       ignore n_off_diag (their Lambda is always diagonal, model.cpp:859-935) */
+    if (kind == LOOP_SPARSE && opts->n_off_diag != 0)
         return mgx::fail(MGX_E_ARG, "parameters->n_off_diag != 0 so cannot use sparse update.");
     const size_t nJ = (size_t)h->P * h->F, nL = (size_t)h->P * h->L;
     const uint32_t cap = 1 + 2 * opts->n_off_diag * m->hnrays;
@@ -1546,6 +1551,8 @@ int level_populations_loop(const mgx_model *m, const mgx_options *opts, mgx_nlte
                                                   nullptr, nullptr, st))
                 return e;
         }
+        h->last_loop = kind;
+        h->last_cap = global ? cap : 0;
         normal++;
         for (uint32_t s = 0; s < h->nlspecs; s++) {
             if (error_mean) error_mean[nerr] = cv[s].relative_change_mean;
@@ -1562,6 +1569,21 @@ int level_populations_loop(const mgx_model *m, const mgx_options *opts, mgx_nlte
 } // namespace
 
 extern "C" {
+
+int mgx_nlte_last_lambda(const mgx_nlte *h, uint32_t *cap, const uint32_t **count,
+                         const uint32_t **nr, const double **val, const double **diag)
+{
+    mgx::clear_error();
+    if (!h || !cap || !count || !nr || !val || !diag) return mgx::fail(MGX_E_ARG, "null argument");
+    if (h->last_loop < 0) return mgx::fail(MGX_E_ARG, "no iteration loop has run a formal solve");
+    *cap = h->last_cap;
+    const bool lists = h->last_loop == LOOP_GLOBAL;
+    *count = lists ? h->lam_count : nullptr;
+    *nr = lists ? h->lam_nr : nullptr;
+    *val = lists ? h->lam_val : nullptr;
+    *diag = lists ? nullptr : h->lambda;
+    return MGX_OK;
+}
 
 int mgx_compute_level_populations_sparse(const mgx_model *m, const mgx_options *opts, mgx_nlte *h,
                                          int use_ng, int64_t max_niterations, double *eps,

@@ -25,8 +25,9 @@
  *        owning wave.
  *
  * Numerics: FP64 throughout, no FMA contraction (-ffp-contract=off), and the
- * float transcendentals of mgx_fmath.h, so the traced lists and the source /
- * optical-depth terms are bit-identical to the CPU restatement.
+ * reference's float transcendentals (the C library's expf, expm1f, logf,
+ * powf and erff, restated in mgx_fmath_ref.h), so the traced lists and the
+ * source / optical-depth terms are those the reference computes.
  */
 #include <hip/hip_runtime.h>
 
@@ -43,6 +44,7 @@
 #include "mgx_constants.h"
 #include "mgx_error.h"
 #include "mgx_fmath.h"
+#include "mgx_fmath_ref.h"
 
 #define MGX_WAVE 64
 /* minimum waves per SIMD requested for the solve kernel (register budget) */
@@ -56,7 +58,9 @@
 
 namespace {
 
-enum { ERR_CAP = 1, ERR_NO_NEXT = 2, ERR_OUT = 4, ERR_CAPE = 8, ERR_LAMCAP = 16 };
+/* kernel status bits: walk buffer full (relaunch with a larger one), no valid next point,
+   output list full, adjoint tape full, Lambda list full, interpolation count above 0xffff */
+enum { ERR_CAP = 1, ERR_NO_NEXT = 2, ERR_OUT = 4, ERR_CAPE = 8, ERR_LAMCAP = 16, ERR_INTERP = 32 };
 
 /* global-address-space load (the pointers live in a struct, so the compiler would
    otherwise emit flat loads, which also count against the LDS wait counter and so
@@ -332,35 +336,36 @@ __device__ inline uint32_t n_interp_for_line(const KParams &kp, uint32_t l, uint
     const uint32_t L = kp.m.nlines;
     const double curr_source = gld(&kp.m.interp_crit[(size_t)c * L + l]);
     const double next_source = gld(&kp.m.interp_crit[(size_t)n * L + l]);
-    const double source_diff = (double)fabsf(mgx_logf((float)(next_source / curr_source)));
+    const double source_diff = (double)fabsf(mgx_ref_logf((float)(next_source / curr_source)));
     const uint32_t k = (uint32_t)ceil(source_diff / kp.log_max_interp);
     return (k > 1) ? k : 1;
 }
 
-/* float transcendentals: correctly rounded (mgx_fmath.h) in FP64 mode, the
-   hardware/ROCm-libm single-precision functions in mixed mode */
+/* float transcendentals: in FP64 mode the reference's own (the C library's
+   expf and powf, restated in mgx_fmath_ref.h), in mixed mode the hardware
+   single-precision functions */
 template <int PREC>
 __device__ inline float f_exp(float x)
 {
-    if (PREC == MGX_PREC_FP64) return mgx_expf(x);
+    if (PREC == MGX_PREC_FP64) return mgx_ref_expf(x);
     return __expf(x);
 }
 
 template <int PREC>
 __device__ inline float f_pow(float x, float y)
 {
-    if (PREC == MGX_PREC_FP64) return mgx_powf(x, y);
+    if (PREC == MGX_PREC_FP64) return mgx_ref_powf(x, y);
     return __powf(x, y);
 }
 
-/* erf stays correctly rounded in both modes: the reference takes the
+/* erf is the reference's erff in both modes: the reference takes the
    difference of two float erf values (compute_dtau_single_line), which cancels
    in the line wings, so a one-ulp change of either value moves u by up to
    ~1e-3 relative there */
 template <int PREC>
-__device__ inline float f_erf(float x, const double *tab)
+__device__ inline float f_erf(float x, const double *)
 {
-    return mgx_erff_tab(x, tab);
+    return mgx_ref_erff(x);
 }
 
 __device__ inline double interp_linear(double a, double b, double t)
@@ -403,7 +408,7 @@ __device__ inline double gaussian(double inverse_width, double diff)
 __device__ inline double planck(double temp, double freq)
 {
     return MGX_TWO_HH_OVER_CC_SQUARED * (freq * freq * freq) /
-           (double)mgx_expm1f((float)(MGX_HH_OVER_KB * freq / temp));
+           (double)mgx_ref_expm1f((float)(MGX_HH_OVER_KB * freq / temp));
 }
 
 __device__ inline double boundary_intensity(const KParams &kp, uint32_t p, double freq)
@@ -928,7 +933,7 @@ __device__ inline double shortchar_half(const KParams &kp, const double *s_erf, 
                               S_n);
         dt = (min_dtau < dtau) ? dtau : min_dtau; /* std::max(min_dtau, dtau) */
         const float mdt = (float)(-dt);
-        I = (double)mgx_expf(mdt) * I + (double)mgx_expm1f(mdt) / dt * (S_c - S_n * (1.0 + dt)) +
+        I = (double)mgx_ref_expf(mdt) * I + (double)mgx_ref_expm1f(mdt) / dt * (S_c - S_n * (1.0 + dt)) +
             S_c - S_n;
         outer = inner;
         ec_out = ec_in;
@@ -1076,7 +1081,7 @@ __device__ __forceinline__ int solve_origin(const KParams &kp, uint32_t o, uint3
                 const uint32_t *nx = h ? s_nxt1 : s_nxt0;
                 const uint32_t crt = (st == 0) ? o : nx[st - 1];
                 const uint32_t mm = n_interp_for_line(kp, j ? lB : lA, crt, nx[st]);
-                if (mm > 0xffffu) atomicOr(&kp.status[0], (uint32_t)ERR_CAP);
+                if (mm > 0xffffu) atomicOr(&kp.status[0], (uint32_t)ERR_INTERP);
                 s_m[(j * 2 + h) * cap + st] = (uint16_t)(mm > 0xffffu ? 0xffffu : mm);
             }
             __syncthreads();
@@ -1125,11 +1130,11 @@ __device__ __forceinline__ int solve_origin(const KParams &kp, uint32_t o, uint3
                                reads an uninitialised dtau there) */
                             const LamPoint lq = lambda_point<APPROX, PREC>(kp, ol, o, l, f, freq, w);
                             if (S0 > 0) {
-                                const double factor = (double)mgx_expm1f((float)(-dtf)) / dtf;
+                                const double factor = (double)mgx_ref_expm1f((float)(-dtf)) / dtf;
                                 Lval += lq.constante * lq.frq * lq.phi * (factor + 1.0) * lq.inverse_chi;
                             }
                             if (S1 > 0) {
-                                const double factor = (double)mgx_expm1f((float)(-dtb)) / dtb;
+                                const double factor = (double)mgx_ref_expm1f((float)(-dtb)) / dtb;
                                 Lval += lq.constante * lq.frq * lq.phi * (factor + 1.0) * lq.inverse_chi;
                             }
                         }
@@ -1371,9 +1376,7 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_SOLVE_WAVES_PER_SIMD) void feautrier_
 {
     const KParams &kp = *kpp;
     extern __shared__ unsigned char smem[];
-    __shared__ double s_erf[MGX_ERF_NINT * (MGX_ERF_DEG + 1)];
-    for (uint32_t i = threadIdx.x; i < MGX_ERF_NINT * (MGX_ERF_DEG + 1); i += MGX_WAVE)
-        s_erf[i] = MGX_ERF_TAB[i];
+    const double *s_erf = nullptr; /* erf needs no table */
     const mgx_model &m = kp.m;
     const uint32_t o = blockIdx.x;
     const uint32_t lane = threadIdx.x;
@@ -1405,9 +1408,7 @@ __global__ __launch_bounds__(MGX_WAVE, 2) void feautrier_bands_kernel(
 {
     const KParams &kp = *kpp;
     extern __shared__ unsigned char smem[];
-    __shared__ double s_erf[MGX_ERF_NINT * (MGX_ERF_DEG + 1)];
-    for (uint32_t i = threadIdx.x; i < MGX_ERF_NINT * (MGX_ERF_DEG + 1); i += MGX_WAVE)
-        s_erf[i] = MGX_ERF_TAB[i];
+    const double *s_erf = nullptr; /* erf needs no table */
     const mgx_model &m = kp.m;
     const uint32_t o = blockIdx.x;
     const uint32_t lane = threadIdx.x;
@@ -1438,9 +1439,7 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_SOLVE_WAVES_PER_SIMD) void shortchar_
 {
     const KParams &kp = *kpp;
     extern __shared__ unsigned char smem[];
-    __shared__ double s_erf[MGX_ERF_NINT * (MGX_ERF_DEG + 1)];
-    for (uint32_t i = threadIdx.x; i < MGX_ERF_NINT * (MGX_ERF_DEG + 1); i += MGX_WAVE)
-        s_erf[i] = MGX_ERF_TAB[i];
+    const double *s_erf = nullptr; /* erf needs no table */
     const mgx_model &m = kp.m;
     const uint32_t o = blockIdx.x;
     const uint32_t lane = threadIdx.x;
@@ -1472,9 +1471,7 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_SOLVE_WAVES_PER_SIMD) void feautrier_
 {
     const KParams &kp = *kpp;
     extern __shared__ unsigned char smem[];
-    __shared__ double s_erf[MGX_ERF_NINT * (MGX_ERF_DEG + 1)];
-    for (uint32_t i = threadIdx.x; i < MGX_ERF_NINT * (MGX_ERF_DEG + 1); i += MGX_WAVE)
-        s_erf[i] = MGX_ERF_TAB[i];
+    const double *s_erf = nullptr; /* erf needs no table */
     const uint32_t o = blockIdx.x;
     double *s_L = reinterpret_cast<double *>(smem + ((walk_lds_bytes(kp.cap) + 7) / 8) * 8);
     ShortcharOut sc{nullptr, v, 0.0};
@@ -1505,10 +1502,8 @@ __global__ __launch_bounds__(MGX_WAVE, WPS) void feautrier_batch_kernel(
     uint32_t *counter, double *const *__restrict__ J, double *const *__restrict__ lam)
 {
     extern __shared__ unsigned char smem[];
-    __shared__ double s_erf[MGX_ERF_NINT * (MGX_ERF_DEG + 1)];
+    const double *s_erf = nullptr; /* erf needs no table */
     __shared__ uint32_t s_item;
-    for (uint32_t i = threadIdx.x; i < MGX_ERF_NINT * (MGX_ERF_DEG + 1); i += MGX_WAVE)
-        s_erf[i] = MGX_ERF_TAB[i];
     const uint32_t lane = threadIdx.x;
     for (;;) {
         __syncthreads();
@@ -1971,7 +1966,7 @@ __device__ inline uint32_t n_interp_all(const KParams &kp, uint32_t c, uint32_t 
     const uint32_t L = kp.m.nlines;
     double mx = 0.0;
     for (uint32_t l = 0; l < L; l++) {
-        const double d = (double)fabsf(mgx_logf((float)(kp.m.interp_crit[(size_t)n * L + l] /
+        const double d = (double)fabsf(mgx_ref_logf((float)(kp.m.interp_crit[(size_t)n * L + l] /
                                                         kp.m.interp_crit[(size_t)c * L + l])));
         if (l == 0 || mx < d) mx = d;
     }
@@ -1985,9 +1980,7 @@ __global__ __launch_bounds__(MGX_WAVE) void image_kernel(const KParams *__restri
 {
     const KParams &kp = *kpp;
     extern __shared__ unsigned char smem[];
-    __shared__ double s_erf[MGX_ERF_NINT * (MGX_ERF_DEG + 1)];
-    for (uint32_t i = threadIdx.x; i < MGX_ERF_NINT * (MGX_ERF_DEG + 1); i += MGX_WAVE)
-        s_erf[i] = MGX_ERF_TAB[i];
+    const double *s_erf = nullptr; /* erf needs no table */
     const mgx_model &m = kp.m;
     const ImageFrame fr = *frp;
     const uint32_t pix = blockIdx.x;
@@ -2086,7 +2079,7 @@ __global__ __launch_bounds__(MGX_WAVE) void image_kernel(const KParams *__restri
     for (uint32_t i = lane; i < S; i += MGX_WAVE) {
         const uint32_t crt = (i == 0) ? ip : s_nxt[i - 1];
         const uint32_t mm = n_interp_all(kp, crt, s_nxt[i]);
-        if (mm > 0xffffu) atomicOr(&kp.status[0], (uint32_t)ERR_CAP);
+        if (mm > 0xffffu) atomicOr(&kp.status[0], (uint32_t)ERR_INTERP);
         s_m[i] = (uint16_t)(mm > 0xffffu ? 0xffffu : mm);
     }
     __syncthreads();
@@ -2161,9 +2154,7 @@ __global__ __launch_bounds__(MGX_WAVE) void image_ray_kernel(const KParams *__re
 {
     const KParams &kp = *kpp;
     extern __shared__ unsigned char smem[];
-    __shared__ double s_erf[MGX_ERF_NINT * (MGX_ERF_DEG + 1)];
-    for (uint32_t i = threadIdx.x; i < MGX_ERF_NINT * (MGX_ERF_DEG + 1); i += MGX_WAVE)
-        s_erf[i] = MGX_ERF_TAB[i];
+    const double *s_erf = nullptr; /* erf needs no table */
     const mgx_model &m = kp.m;
     const uint32_t o = blockIdx.x;
     const uint32_t lane = threadIdx.x;
@@ -2222,7 +2213,7 @@ __global__ __launch_bounds__(MGX_WAVE) void image_ray_kernel(const KParams *__re
         const uint32_t *nx = h ? s_nxt1 : s_nxt0;
         const uint32_t crt = (st == 0) ? o : nx[st - 1];
         const uint32_t mm = n_interp_all(kp, crt, nx[st]);
-        if (mm > 0xffffu) atomicOr(&kp.status[0], (uint32_t)ERR_CAP);
+        if (mm > 0xffffu) atomicOr(&kp.status[0], (uint32_t)ERR_INTERP);
         (h ? s_m1 : s_m0)[st] = (uint16_t)(mm > 0xffffu ? 0xffffu : mm);
     }
     __syncthreads();
@@ -3220,9 +3211,7 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_ADJ_WAVES_PER_SIMD) void feautrier_ad
 {
     const KParams &kp = *kpp;
     extern __shared__ unsigned char smem[];
-    __shared__ double s_erf[MGX_ERF_NINT * (MGX_ERF_DEG + 1)];
-    for (uint32_t i = threadIdx.x; i < MGX_ERF_NINT * (MGX_ERF_DEG + 1); i += MGX_WAVE)
-        s_erf[i] = MGX_ERF_TAB[i];
+    const double *s_erf = nullptr; /* erf needs no table */
     const mgx_model &m = kp.m;
     const uint32_t lane = threadIdx.x;
     const uint32_t cap = kp.cap;
@@ -3272,7 +3261,7 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_ADJ_WAVES_PER_SIMD) void feautrier_ad
                     const uint32_t *nx = h ? s_nxt1 : s_nxt0;
                     const uint32_t crt = (stp == 0) ? o : nx[stp - 1];
                     const uint32_t mm = n_interp_for_line(kp, j ? lB : lA, crt, nx[stp]);
-                    if (mm > 0xffffu) atomicOr(&kp.status[0], (uint32_t)ERR_CAP);
+                    if (mm > 0xffffu) atomicOr(&kp.status[0], (uint32_t)ERR_INTERP);
                     s_m[(j * 2 + h) * cap + stp] = (uint16_t)(mm > 0xffffu ? 0xffffu : mm);
                 }
                 __syncthreads();
@@ -3342,9 +3331,7 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_ADJ_WAVES_PER_SIMD) void image_adjoin
 {
     const KParams &kp = *kpp;
     extern __shared__ unsigned char smem[];
-    __shared__ double s_erf[MGX_ERF_NINT * (MGX_ERF_DEG + 1)];
-    for (uint32_t i = threadIdx.x; i < MGX_ERF_NINT * (MGX_ERF_DEG + 1); i += MGX_WAVE)
-        s_erf[i] = MGX_ERF_TAB[i];
+    const double *s_erf = nullptr; /* erf needs no table */
     const mgx_model &m = kp.m;
     const ImageFrame fr = *frp;
     const uint32_t lane = threadIdx.x;
@@ -3443,7 +3430,7 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_ADJ_WAVES_PER_SIMD) void image_adjoin
         for (uint32_t i = lane; i < S; i += MGX_WAVE) {
             const uint32_t crt = (i == 0) ? ip : s_nxt[i - 1];
             const uint32_t mm = n_interp_all(kp, crt, s_nxt[i]);
-            if (mm > 0xffffu) atomicOr(&kp.status[0], (uint32_t)ERR_CAP);
+            if (mm > 0xffffu) atomicOr(&kp.status[0], (uint32_t)ERR_INTERP);
             s_m[i] = (uint16_t)(mm > 0xffffu ? 0xffffu : mm);
         }
         __syncthreads();
@@ -3533,8 +3520,10 @@ struct DevBuffers {
     PointIdx *pidx = nullptr;
     uint32_t *status = nullptr;
     KParams *kp_dev = nullptr; /* device copy of the launch parameters */
+    bool owned = true;         /* false: the memory belongs to someone else */
     ~DevBuffers()
     {
+        if (!owned) return;
         if (kp_dev) (void)hipFree(kp_dev);
         if (lpack) (void)hipFree(lpack);
         if (pts) (void)hipFree(pts);
@@ -3543,11 +3532,20 @@ struct DevBuffers {
     }
 };
 
+/* entry points whose Lambda output is the diagonal only: with n_off_diag > 0
+   the reference also stores the bands, which mgx_feautrier_forward_lambda_device
+   returns */
+int check_lambda_diagonal(const mgx_options *op)
+{
+    if (op && op->n_off_diag != 0)
+        return mgx::fail(MGX_E_UNSUPPORTED,
+                         "n_off_diag > 0: the Lambda bands come from mgx_feautrier_forward_lambda_device");
+    return 0;
+}
+
 int check_view(const mgx_model *m, const mgx_options *op)
 {
     if (!m || !op) return mgx::fail(MGX_E_ARG, "null model or options");
-    if (op->n_off_diag != 0)
-        return mgx::fail(MGX_E_UNSUPPORTED, "n_off_diag > 0 is not supported (only the Lambda diagonal)");
     if (op->approx < 0 || op->approx > 2) return mgx::fail(MGX_E_ARG, "unknown approximation");
     if (op->precision != MGX_PREC_FP64 && op->precision != MGX_PREC_MIXED)
         return mgx::fail(MGX_E_ARG, "unknown precision mode");
@@ -3596,14 +3594,11 @@ void timed(hipStream_t st, Fn fn)
     (void)hipEventDestroy(b);
 }
 
-int prepare(const mgx_model *m, const mgx_options *op, hipStream_t st, DevBuffers &b, KParams &kp)
+/* fills the packed tables of b (already allocated) and the launch parameters */
+int prepare_tables(const mgx_model *m, const mgx_options *op, hipStream_t st, DevBuffers &b,
+                   KParams &kp)
 {
     const size_t PL = (size_t)m->npoints * m->nlines;
-    HIP_TRY(hipMalloc(&b.lpack, PL * sizeof(double4)));
-    HIP_TRY(hipMalloc(&b.pts, (size_t)m->npoints * sizeof(PointRec)));
-    HIP_TRY(hipMalloc(&b.pidx, (size_t)m->npoints * sizeof(PointIdx)));
-    HIP_TRY(hipMalloc(&b.status, 4 * sizeof(uint32_t)));
-    HIP_TRY(hipMalloc(&b.kp_dev, sizeof(KParams)));
     HIP_TRY(hipMemsetAsync(b.status, 0, 4 * sizeof(uint32_t), st));
     const size_t n = PL > m->npoints ? PL : m->npoints;
     hipLaunchKernelGGL(pack_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, *m,
@@ -3614,12 +3609,24 @@ int prepare(const mgx_model *m, const mgx_options *op, hipStream_t st, DevBuffer
     kp.lpack = b.lpack;
     kp.pts = b.pts;
     kp.pidx = b.pidx;
-    kp.log_max_interp = (double)mgx_logf((float)op->max_interpolation_diff);
+    kp.log_max_interp = (double)mgx_ref_logf((float)op->max_interpolation_diff);
     kp.slots_per_line = (m->nquads + 31) / 32;
     kp.nslots = m->nlines * kp.slots_per_line;
     kp.status = b.status;
     return MGX_OK;
 }
+
+int prepare(const mgx_model *m, const mgx_options *op, hipStream_t st, DevBuffers &b, KParams &kp)
+{
+    const size_t PL = (size_t)m->npoints * m->nlines;
+    HIP_TRY(hipMalloc(&b.lpack, PL * sizeof(double4)));
+    HIP_TRY(hipMalloc(&b.pts, (size_t)m->npoints * sizeof(PointRec)));
+    HIP_TRY(hipMalloc(&b.pidx, (size_t)m->npoints * sizeof(PointIdx)));
+    HIP_TRY(hipMalloc(&b.status, 4 * sizeof(uint32_t)));
+    HIP_TRY(hipMalloc(&b.kp_dev, sizeof(KParams)));
+    return prepare_tables(m, op, st, b, kp);
+}
+
 
 /* reads the status words; returns MGX_OK, or an error (needed capacity in *need;
    segments needed by the adjoint's scratch in *need_e) */
@@ -3632,6 +3639,10 @@ int read_status(hipStream_t st, const DevBuffers &b, uint32_t *need, uint32_t *n
     if (need_e) *need_e = (h[0] & ERR_CAPE) ? (h[2] ? h[2] : 1) : 0;
     if (h[0] & ERR_NO_NEXT)
         return mgx::fail(MGX_E_NO_NEXT, "a ray from an interior point found no valid next point");
+    if (h[0] & ERR_INTERP)
+        return mgx::fail(MGX_E_UNSUPPORTED,
+                         "a segment needs more than 65535 interpolation points (the interpolation "
+                         "criterion changes by more than max_interpolation_diff^65535 between neighbours)");
     if (h[0] & ERR_OUT) return mgx::fail(MGX_E_RAY_LENGTH, "a traced list exceeds the output capacity");
     if (h[0] & ERR_LAMCAP) return mgx::fail(MGX_E_ARG, "a Lambda list exceeds its capacity");
     if (h[0] & ERR_CAP) return MGX_E_RAY_LENGTH;
@@ -3646,12 +3657,19 @@ int launch_with_capacity(hipStream_t st, DevBuffers &b, KParams &kp, Launch laun
                          size_t extra_lds = 0)
 {
     uint32_t cap = 128;
+    double total_ms = 0.0; /* kernel time summed over the relaunches */
+    struct Report {
+        double &t;
+        ~Report() { g_last_kernel_ms = t; }
+    } report{total_ms};
     for (;;) {
         kp.cap = cap;
         kp.j_lds = j_in_lds(cap, kp.m.nfreqs, kp.m.nlines) ? 1 : 0;
         HIP_TRY(hipMemsetAsync(b.status, 0, 4 * sizeof(uint32_t), st));
         HIP_TRY(hipMemcpyAsync(b.kp_dev, &kp, sizeof(KParams), hipMemcpyHostToDevice, st));
+        g_last_kernel_ms = 0.0;
         launch(walk_lds_bytes(cap));
+        total_ms += g_last_kernel_ms;
         HIP_TRY(hipGetLastError());
         uint32_t need = 0;
         const int e = read_status(st, b, &need);
@@ -4060,6 +4078,9 @@ int batch_execute(mgx_batch &B, hipStream_t st)
         HIP_TRY(hipStreamSynchronize(st));
         if (h[0] & ERR_NO_NEXT)
             return mgx::fail(MGX_E_NO_NEXT, "a ray from an interior point found no valid next point");
+        if (h[0] & ERR_INTERP)
+            return mgx::fail(MGX_E_UNSUPPORTED,
+                             "a segment needs more than 65535 interpolation points");
         if (!(h[0] & ERR_CAP)) return MGX_OK;
         uint32_t max_cap = kMaxCap;
         const size_t extra = 8 + 8 * (size_t)B.maxL;
@@ -4141,16 +4162,13 @@ const char *mgx_last_error(void) { return mgx::error_buffer(); }
 
 const char *mgx_version(void) { return "magritte_amd 0.1.0 (gfx950)"; }
 
-int mgx_feautrier_forward_device(const mgx_model *model, const mgx_options *opts, double *u,
-                                 double *J, double *lambda_diag, void *stream)
+namespace {
+
+/* the plain-method solve on prepared buffers (shared by the allocating and the
+   no-allocation entry points) */
+int forward_launch(const mgx_model *model, const mgx_options *opts, double *u, double *J,
+                   double *lambda_diag, hipStream_t st, DevBuffers &b, KParams &kp)
 {
-    mgx::clear_error();
-    if (int e = check_view(model, opts)) return e;
-    if (!J) return mgx::fail(MGX_E_ARG, "J is required");
-    hipStream_t st = (hipStream_t)stream;
-    DevBuffers b;
-    KParams kp;
-    if (int e = prepare(model, opts, st, b, kp)) return e;
     return launch_with_capacity(st, b, kp, [&](size_t) {
         const dim3 grid(model->npoints), block(MGX_WAVE);
         const size_t lds = solve_lds_bytes(kp.cap, model->nfreqs, model->nlines, kp.j_lds != 0);
@@ -4184,6 +4202,23 @@ int mgx_feautrier_forward_device(const mgx_model *model, const mgx_options *opts
         });
     }, 8 * (size_t)model->nlines);
 }
+
+} // namespace
+
+int mgx_feautrier_forward_device(const mgx_model *model, const mgx_options *opts, double *u,
+                                 double *J, double *lambda_diag, void *stream)
+{
+    mgx::clear_error();
+    if (int e = check_view(model, opts)) return e;
+    if (int e = check_lambda_diagonal(opts)) return e;
+    if (!J) return mgx::fail(MGX_E_ARG, "J is required");
+    hipStream_t st = (hipStream_t)stream;
+    DevBuffers b;
+    KParams kp;
+    if (int e = prepare(model, opts, st, b, kp)) return e;
+    return forward_launch(model, opts, u, J, lambda_diag, st, b, kp);
+}
+
 
 int mgx_feautrier_forward_lambda_device(const mgx_model *model, const mgx_options *opts,
                                         double *u, double *J, uint32_t cap, uint32_t *lam_count,
@@ -4615,6 +4650,7 @@ int mgx_batch_create(const mgx_model *models, uint32_t n, const mgx_options *opt
     *out = nullptr;
     for (uint32_t i = 0; i < n; i++) {
         if (int e = check_view(&models[i], opts)) return e;
+        if (int e = check_lambda_diagonal(opts)) return e;
         if (!J[i]) return mgx::fail(MGX_E_ARG, "J of model %u is null", i);
         if (lambda_diag && !lambda_diag[i]) return mgx::fail(MGX_E_ARG, "lambda of model %u is null", i);
     }

@@ -58,19 +58,34 @@
 #endif
 
 #include "mgx_fmath.h"
+#include "mgx_fmath_ref.h"
 
-#ifdef MGX_ORACLE_GLIBC
+/* The float transcendentals, by build:
+ *   MGX_ORACLE_GLIBC  the C library's expf, expm1f, logf, powf, erff: the
+ *                     reference's own calls (liboracle_glibc.so);
+ *   MGX_ORACLE_CR     the correctly rounded functions of mgx_fmath.h, kept to
+ *                     measure how much the results depend on these last bits
+ *                     (liboracle_cr.so);
+ *   neither           the restatements of the C library's functions in
+ *                     mgx_fmath_ref.h that the kernels evaluate (liboracle.so). */
+#if defined(MGX_ORACLE_GLIBC)
 #define O_EXPF expf
 #define O_EXPM1F expm1f
 #define O_LOGF logf
 #define O_POWF powf
 #define O_ERFF erff
-#else
+#elif defined(MGX_ORACLE_CR)
 #define O_EXPF mgx_expf
 #define O_EXPM1F mgx_expm1f
 #define O_LOGF mgx_logf
 #define O_POWF mgx_powf
 #define O_ERFF mgx_erff
+#else
+#define O_EXPF mgx_ref_expf
+#define O_EXPM1F mgx_ref_expm1f
+#define O_LOGF mgx_ref_logf
+#define O_POWF mgx_ref_powf
+#define O_ERFF mgx_ref_erff
 #endif
 
 /* src/tools/constants.hpp */
@@ -1277,6 +1292,14 @@ int oracle_shortchar_order_0(const mgx_model *m, const mgx_options *op, double m
                              double *I, double *u, double *v, double *J, double *lambda_diag,
                              int nthreads)
 {
+    /* the reference's short characteristics ignore n_off_diag: their Lambda is
+       the diagonal (solver.tpp:151-235) */
+    mgx_options op0;
+    if (op) {
+        op0 = *op;
+        op0.n_off_diag = 0;
+        op = &op0;
+    }
     int err = check_model(m, op);
     if (err) return err;
     const int nt = default_threads(nthreads);

@@ -36,12 +36,15 @@
 #include <string.h>
 
 #include "mgx_fmath.h"
+#include "mgx_fmath_ref.h"
 #include "mgx_oracle.h"
 
-#ifdef MGX_ORACLE_GLIBC
+#if defined(MGX_ORACLE_GLIBC)
 #define N_EXPF expf
-#else
+#elif defined(MGX_ORACLE_CR)
 #define N_EXPF mgx_expf
+#else
+#define N_EXPF mgx_ref_expf
 #endif
 
 typedef long double ld;

@@ -13,11 +13,23 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _libs = {}
 
 
-def lib(kind="cr"):
-    """kind='cr': correctly rounded float functions (shared with the kernels);
-    kind='glibc': the C library's expf/expm1f/logf/powf/erff."""
+# The float transcendentals of each build:
+#   'glibc'  the C library's expf/expm1f/logf/powf/erff -- the reference's own
+#            calls; the default, so parity is checked against the reference's
+#            arithmetic;
+#   'ref'    the restatements of those functions that the kernels evaluate
+#            (mgx_fmath_ref.h), equal to the C library's on every argument
+#            checked (tests/test_fmath.py);
+#   'cr'     correctly rounded functions (mgx_fmath.h): how far the results
+#            move when only these last bits change.
+DEFAULT_KIND = "glibc"
+_LIB_NAMES = {"glibc": "liboracle_glibc.so", "ref": "liboracle.so", "cr": "liboracle_cr.so"}
+
+
+def lib(kind=DEFAULT_KIND):
+    """The oracle library with the float functions of `kind` ('glibc', 'ref', 'cr')."""
     if kind not in _libs:
-        name = "liboracle.so" if kind == "cr" else "liboracle_glibc.so"
+        name = _LIB_NAMES[kind]
         path = os.path.join(_HERE, name)
         if not os.path.exists(path):
             raise RuntimeError(f"{path} missing: run `make -C oracle` (or __graft_entry__.build())")
@@ -53,7 +65,7 @@ def _check(code, what):
         raise RuntimeError(f"oracle {what} failed with code {code}")
 
 
-def solve(model, opts, with_u=True, nthreads=0, kind="cr"):
+def solve(model, opts, with_u=True, nthreads=0, kind=DEFAULT_KIND):
     """Returns (u, J, lambda_diag) of the reference plain method."""
     m = model.view()
     P, F, L, R = model.npoints, model.nfreqs, model.nlines, model.hnrays
@@ -66,7 +78,7 @@ def solve(model, opts, with_u=True, nthreads=0, kind="cr"):
     return u, J, lam
 
 
-def shortchar(model, opts, min_dtau=1.0e-15, nthreads=0, kind="cr"):
+def shortchar(model, opts, min_dtau=1.0e-15, nthreads=0, kind=DEFAULT_KIND):
     """Model::compute_radiation_field_shortchar_order_0: (I (nrays, P, F), u, v
     (hnrays, P, F), J (P, F), lambda_diag (P, L))."""
     m = model.view()
@@ -84,7 +96,7 @@ def shortchar(model, opts, min_dtau=1.0e-15, nthreads=0, kind="cr"):
     return I, u, v, J, lam
 
 
-def solve_lambda(model, opts, cap=None, with_u=False, nthreads=0, kind="cr"):
+def solve_lambda(model, opts, cap=None, with_u=False, nthreads=0, kind=DEFAULT_KIND):
     """The plain-method solve with the general Lambda operator (opts.n_off_diag >= 0):
     returns (u or None, J, count (P, L), nr (P, L, cap), val (P, L, cap)); the list of
     (p, l) is nr[p, l, :count], val[p, l, :count] in the reference's order
@@ -107,7 +119,7 @@ def solve_lambda(model, opts, cap=None, with_u=False, nthreads=0, kind="cr"):
     return u, J, cnt, nr, val
 
 
-def feautrier_uv(model, opts, nthreads=0, kind="cr", with_scale=False):
+def feautrier_uv(model, opts, nthreads=0, kind=DEFAULT_KIND, with_scale=False):
     """Model::compute_radiation_field_feautrier_order_2_uv: (u, v), each (hnrays, P, F).
     with_scale=True also returns v_scale, the sum of the absolute terms each v is
     formed from (v is a difference of neighbouring u's and cancels)."""
@@ -125,7 +137,7 @@ def feautrier_uv(model, opts, nthreads=0, kind="cr", with_scale=False):
     return (u, v, vs) if with_scale else (u, v)
 
 
-def trace_counts(model, opts, nthreads=0, kind="cr"):
+def trace_counts(model, opts, nthreads=0, kind=DEFAULT_KIND):
     m = model.view()
     n = np.zeros((model.hnrays, model.npoints, model.nlines), dtype=np.uint32)
     _check(lib(kind).oracle_trace_counts(C.addressof(m), C.addressof(opts), n.ctypes.data,
@@ -133,7 +145,7 @@ def trace_counts(model, opts, nthreads=0, kind="cr"):
     return n
 
 
-def trace_ray(model, opts, rr, o, l, capacity=4096, kind="cr"):
+def trace_ray(model, opts, rr, o, l, capacity=4096, kind=DEFAULT_KIND):
     m = model.view()
     cnt = np.zeros(1, dtype=np.uint32)
     cen = np.zeros(1, dtype=np.uint32)
@@ -151,7 +163,7 @@ def trace_ray(model, opts, rr, o, l, capacity=4096, kind="cr"):
                 interp_factor=fa[:n], dZ=dz[:n])
 
 
-def image_new(model, opts, raydir, nx, ny, freqs=None, nthreads=0, kind="cr"):
+def image_new(model, opts, raydir, nx, ny, freqs=None, nthreads=0, kind=DEFAULT_KIND):
     F = model.nfreqs if freqs is None else freqs["nu"].shape[1]
     m = model.view(freqs, nfreqs=F)
     rd = np.ascontiguousarray(raydir, dtype=np.float64)
@@ -164,7 +176,7 @@ def image_new(model, opts, raydir, nx, ny, freqs=None, nthreads=0, kind="cr"):
     return I, ImX, ImY
 
 
-def image_optical_depth_new(model, opts, raydir, nx, ny, freqs=None, nthreads=0, kind="cr"):
+def image_optical_depth_new(model, opts, raydir, nx, ny, freqs=None, nthreads=0, kind=DEFAULT_KIND):
     """Model::compute_image_optical_depth_new: (tau (nx*ny, F), ImX, ImY)."""
     F = model.nfreqs if freqs is None else freqs["nu"].shape[1]
     m = model.view(freqs, nfreqs=F)
@@ -182,7 +194,7 @@ def image_optical_depth_new(model, opts, raydir, nx, ny, freqs=None, nthreads=0,
     return tau, ImX, ImY
 
 
-def image_ray(model, opts, rr, freqs=None, optical_depth=False, nthreads=0, kind="cr"):
+def image_ray(model, opts, rr, freqs=None, optical_depth=False, nthreads=0, kind=DEFAULT_KIND):
     """Model::compute_image(rr) (or compute_image_optical_depth(rr) with
     optical_depth=True), the old imager at every model point: (out (P, F), ImX, ImY)."""
     F = model.nfreqs if freqs is None else freqs["nu"].shape[1]
@@ -200,7 +212,7 @@ def image_ray(model, opts, rr, freqs=None, optical_depth=False, nthreads=0, kind
     return out, ImX, ImY
 
 
-def feautrier_system(model, opts, rr, o, f, capacity=4096, kind="cr"):
+def feautrier_system(model, opts, rr, o, f, capacity=4096, kind=DEFAULT_KIND):
     """dtau (n-1), S (n), I_bdy (2), centre, u_c and L_c of one (rr, o, f) solve."""
     m = model.view()
     n = np.zeros(1, dtype=np.uint32)
@@ -219,21 +231,21 @@ def feautrier_system(model, opts, rr, o, f, capacity=4096, kind="cr"):
                 L_c=float(lc[0]))
 
 
-def set_smooth_interp(on, kind="cr"):
+def set_smooth_interp(on, kind=DEFAULT_KIND):
     """Evaluate interpolate_log in double (a smooth function of its end values),
     for finite-difference checks of the adjoint; off restores the reference's
     float powf."""
     lib(kind).oracle_set_smooth_interp(1 if on else 0)
 
 
-def set_smooth_profile(on, kind="cr"):
+def set_smooth_profile(on, kind=DEFAULT_KIND):
     """Evaluate the line profile and the error-function difference of the
     line-integrated optical depth in double (smooth in the inverse widths), for
     finite-difference checks of the adjoint; off restores the float expf/erff."""
     lib(kind).oracle_set_smooth_profile(1 if on else 0)
 
 
-def J_adjoint(model, opts, J_bar, nthreads=0, kind="cr", want_iw=False):
+def J_adjoint(model, opts, J_bar, nthreads=0, kind=DEFAULT_KIND, want_iw=False):
     """(d_eps, d_kap), each (P, L): the gradient of sum(J_bar * J) with respect to
     the line emissivity and opacity; with want_iw also d_iw, the gradient with
     respect to the inverse line widths."""
@@ -250,7 +262,7 @@ def J_adjoint(model, opts, J_bar, nthreads=0, kind="cr", want_iw=False):
     return (de, dk, dw) if want_iw else (de, dk)
 
 
-def image_adjoint(model, opts, raydir, nx, ny, I_bar, freqs=None, nthreads=0, kind="cr",
+def image_adjoint(model, opts, raydir, nx, ny, I_bar, freqs=None, nthreads=0, kind=DEFAULT_KIND,
                   want_iw=False):
     """(d_eps, d_kap[, d_iw]): the gradient of sum(I_bar * I) for the image_new image."""
     F = model.nfreqs if freqs is None else freqs["nu"].shape[1]
@@ -291,7 +303,7 @@ def _nlte_sigs(h):
     h.oracle_nlte_stateq_global_system.argtypes = [vp, C.c_uint32, vp, vp, vp, C.c_uint32, vp, vp]
 
 
-def cpqr_solve(A, b, kind="cr"):
+def cpqr_solve(A, b, kind=DEFAULT_KIND):
     """x = A^-1 b by the long-double column-pivoted Householder QR."""
     A = np.asarray(A, dtype=np.float64)
     n = A.shape[0]
@@ -309,7 +321,7 @@ class NLTE:
     LineProducingSpecies set).  Writes the model's emissivity, opacity and
     interp_crit arrays in place, as Lines::set_emissivity_and_opacity does."""
 
-    def __init__(self, model, params, kind="cr"):
+    def __init__(self, model, params, kind=DEFAULT_KIND):
         from magritte_amd import _abi  # noqa: F401  (ctypes structs)
 
         self.model = model
@@ -447,7 +459,7 @@ def ng_decision(nlte, params, use_ng, prior):
 
 
 def compute_level_populations(model, opts, params, use_ng=True, max_iterations=100, nlte=None,
-                              nthreads=0, kind="cr"):
+                              nthreads=0, kind=DEFAULT_KIND):
     """Model::compute_level_populations (model.cpp:653-744) on the CPU: each iteration the
     formal solve with the general Lambda operator (opts.n_off_diag bands), compute_Jeff and
     the global statistical-equilibrium update; Ng steps per the criterion.  Starts from LTE
@@ -481,7 +493,7 @@ def compute_level_populations(model, opts, params, use_ng=True, max_iterations=1
 
 
 def compute_level_populations_sparse(model, opts, params, use_ng=True, max_iterations=100,
-                                     nlte=None, nthreads=0, kind="cr"):
+                                     nlte=None, nthreads=0, kind=DEFAULT_KIND):
     """Model::compute_level_populations_sparse (model.cpp:756-850) on the CPU:
     each iteration the plain formal solve (u and the Lambda diagonal), then
     compute_Jeff_sparse and the statistical-equilibrium update; Ng steps per
@@ -505,6 +517,47 @@ def compute_level_populations_sparse(model, opts, params, use_ng=True, max_itera
         u, _, lam = solve(model, opts, with_u=True, nthreads=nthreads, kind=kind)
         nlte.compute_Jeff(u, lam)
         nlte.stateq(lam)
+        normal += 1
+        conv = nlte.stats()[0]
+        for l in range(model.nlspecs):
+            err_mean.append(float(conv[l, 1]))
+            err_max.append(float(conv[l, 2]))
+            if conv[l, 0] > 1.0 - params.convergence_fraction:
+                some_not_converged = True
+    return nlte, iteration, err_mean, err_max
+
+
+def compute_level_populations_shortchar(model, opts, params, use_ng=True, max_iterations=100,
+                                        min_dtau=1.0e-15, nlte=None, nthreads=0,
+                                        kind=DEFAULT_KIND):
+    """Model::compute_level_populations_shortchar (model.cpp:859-935) on the CPU: each
+    iteration the order-0 short-characteristics solve (solver.tpp:151-235), whose Lambda
+    is the diagonal whatever opts.n_off_diag says, then Model::compute_Jeff and the
+    global statistical-equilibrium update over those one-element lists, as the reference
+    does; Ng steps per the criterion.  Starts from LTE unless an NLTE state is given.
+    Returns (nlte, iterations, error_mean, error_max)."""
+    if nlte is None:
+        nlte = NLTE(model, params, kind)
+        nlte.set_lte()
+    P, L = model.npoints, model.nlines
+    cnt = np.ones((P, L), dtype=np.uint32)
+    nr = np.repeat(np.arange(P, dtype=np.uint32), L).reshape(P, L, 1)
+    iteration = 0
+    normal = 0
+    err_mean, err_max = [], []
+    some_not_converged = True
+    while some_not_converged and iteration < max_iterations:
+        use_step, order = ng_decision(nlte, params, use_ng, normal)
+        if use_step:
+            nlte.ng(order)
+            normal = 0
+            continue
+        iteration += 1
+        some_not_converged = False
+        _, _, _, J, lam = shortchar(model, opts, min_dtau=min_dtau, nthreads=nthreads, kind=kind)
+        lists = (cnt, nr, np.ascontiguousarray(lam.reshape(P, L, 1)))
+        nlte.compute_Jeff_general(J, lists)
+        nlte.stateq_global(lists)
         normal += 1
         conv = nlte.stats()[0]
         for l in range(model.nlspecs):

@@ -63,7 +63,12 @@ def test_adjoint_entry_points_reject_null_arguments(line_model):
     rd = (C.c_double * 3)(0.0, 1.0, 0.0)
     rc = lib.mgx_image_adjoint_device(C.byref(v), C.byref(op), rd, 4, 4, None, None, None, None)
     assert rc == _abi.MGX_E_ARG
-    bad = line_model.options()
-    bad.n_off_diag = 1
-    rc = lib.mgx_feautrier_adjoint_device(C.byref(v), C.byref(bad), None, None, None, None)
-    assert rc == _abi.MGX_E_UNSUPPORTED
+    # J and its gradient do not depend on the Lambda bands: n_off_diag is accepted,
+    # and the null arguments are still what is reported
+    bands = line_model.options()
+    bands.n_off_diag = 1
+    rc = lib.mgx_feautrier_adjoint_device(C.byref(v), C.byref(bands), None, None, None, None)
+    assert rc == _abi.MGX_E_ARG and b"null argument" in lib.mgx_last_error()
+    # the plain forward solve returns the diagonal only and points to the bands' entry
+    rc = lib.mgx_feautrier_forward_device(C.byref(v), C.byref(bands), None, None, None, None)
+    assert rc == _abi.MGX_E_UNSUPPORTED and b"mgx_feautrier_forward_lambda_device" in lib.mgx_last_error()

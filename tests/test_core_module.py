@@ -154,6 +154,20 @@ def test_core_level_populations_match_device_loop():
                                    rtol=1e-12, atol=1e-14 * float(tot.max()))
         got = np.frombuffer(bytes(cm.get_array("opacity")), dtype=np.float64)
         np.testing.assert_allclose(got, dm.tensors["opacity"].cpu().numpy().ravel(), rtol=1e-12)
+        # after the loop, lspec.Lambda holds its last formal solve's Lambda: one diagonal
+        # element per (point, line) for the sparse loop, the bands for the general one
+        lam = cm.lspecs[0].Lambda
+        lens = [len(lam.Ls[p][0]) for p in range(m.npoints)]
+        assert all(len(lam.nr[p][0]) == lens[p] for p in range(m.npoints))
+        if noff == 0:
+            assert lens == [1] * m.npoints
+            assert [list(lam.nr[p][0]) for p in range(m.npoints)] == [[p] for p in range(m.npoints)]
+            np.testing.assert_array_equal([lam.Ls[p][0][0] for p in range(m.npoints)],
+                                          np.array(cm.lambda_diag)[:, 0])
+        else:
+            assert max(lens) == 1 + 2 * noff
+            np.testing.assert_array_equal([lam.Ls[p][0][0] for p in range(m.npoints)],
+                                          np.array(cm.lambda_val)[:, 0, 0])
     # Lambda of the last formal solve in lspec.Lambda.Ls / nr
     cm.compute_radiation_field_feautrier_order_2()
     lam = cm.lspecs[0].Lambda
@@ -162,3 +176,31 @@ def test_core_level_populations_match_device_loop():
         c = int(cnt[p, 0])
         assert list(lam.nr[p][0]) == list(nr[p, 0, :c])
         np.testing.assert_array_equal(lam.Ls[p][0], val[p, 0, :c])
+
+
+@pytest.mark.gpu
+def test_core_shortchar_lambda_replaces_the_previous_lists(small_wind):
+    """After a Feautrier solve with Lambda bands, the short-characteristics solve leaves
+    its own diagonal Lambda in lspec.Lambda (solver.tpp:164-166, 1651, 1754), whatever
+    n_off_diag says, and n_off_diag does not change its results."""
+    import torch  # noqa: F401  (one HIP runtime: torch's)
+
+    m = small_wind
+    cm = m.to_core()
+    cm.parameters.n_off_diag = 2
+    cm.compute_radiation_field_feautrier_order_2()
+    assert max(len(x) for x in cm.lspecs[0].Lambda.Ls[5]) > 1
+    cm.compute_radiation_field_shortchar_order_0()
+    J2 = np.array(cm.radiation.J).copy()
+    diag = np.array(cm.lambda_diag)
+    lam = cm.lspecs[0].Lambda
+    line_k = np.frombuffer(bytes(cm.get_array("line_k")), dtype=np.uint32)
+    for p in (0, 5, m.npoints // 2, m.npoints - 1):
+        for l in range(m.nlines):
+            k = int(line_k[l])
+            assert list(lam.nr[p][k]) == [p]
+            assert list(lam.Ls[p][k]) == [diag[p, l]]
+    cm.parameters.n_off_diag = 0
+    cm.compute_radiation_field_shortchar_order_0()
+    np.testing.assert_array_equal(np.array(cm.radiation.J), J2)
+    np.testing.assert_array_equal(np.array(cm.lambda_diag), diag)

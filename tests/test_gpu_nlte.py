@@ -277,6 +277,53 @@ def test_level_population_loop_matches_oracle(which):
            floor=1e-3 * prm.pop_prec)
 
 
+@pytest.mark.parametrize("which,use_ng,max_it", [("line", True, 60), ("wind", False, 2)])
+def test_shortchar_loop_matches_oracle(which, use_ng, max_it):
+    """Model::compute_level_populations_shortchar (model.cpp:859-935) against the oracle's
+    restatement (its short-characteristics solve, then compute_Jeff and the global update
+    over the one-element Lambda lists): same iterations, populations within 1e-8; the loop
+    ignores n_off_diag (n_off_diag = 2 gives the n_off_diag = 0 bits).  The 1D line runs
+    to convergence with Ng.  On the 6^3 wind this iteration amplifies any difference by
+    10^2-10^4 per step (the oracle's own global and per-point updates, 2e-12 apart after
+    one step, are 1e-6 apart after three), so it is compared over two iterations."""
+    from magritte_amd import solver
+    from oracle import oracle
+
+    def fresh():
+        # the oracle's global update is a dense long-double solve of all points' rate
+        # equations: keep P * nlev small (a 6^3 wind with 5 levels: 1080 unknowns)
+        if which == "line":
+            return models.model_line_1d()
+        return models.model_wind(n_edge=6, nrays=12, seed=models.seed_for(1, 7), nlev=5)
+
+    m, m_ref = fresh(), fresh()
+    prm = _abi.default_nlte_params()
+    _say(f"[shortchar loop vs oracle] model={which} P={m.npoints} Ng={use_ng} max {max_it}")
+    res = []
+    for noff in (0, 2):
+        dm = solver.DeviceModel(m if noff == 0 else fresh())
+        lp = solver.LevelPopulations(dm, prm)
+        lp.set_lte()
+        op = m.options()
+        op.n_off_diag = noff
+        it, em, ex = solver.compute_level_populations_shortchar(dm, op, lp, use_ng, max_it)
+        pop, tot = lp.populations()
+        res.append((it, ex, pop.cpu().numpy(), tot.cpu().numpy()))
+    assert res[0][0] == res[1][0]
+    np.testing.assert_array_equal(res[0][2], res[1][2])
+    it, ex, pop, tot = res[0]
+    op_r = m_ref.options()
+    op_r.n_off_diag = 2
+    nl, it_r, em_r, ex_r = oracle.compute_level_populations_shortchar(m_ref, op_r, prm, use_ng,
+                                                                      max_it, nthreads=16)
+    _say(f"  iterations gpu {it} oracle {it_r}; last max change {ex[-1]:.3e} / {ex_r[-1]:.3e}")
+    assert it == it_r
+    ref = nl.populations()["population"]
+    ptot = np.repeat(tot[0], len(pop) // len(tot[0]))
+    _close(pop, ref, 1e-8, "populations", floor=1e-14 * ptot)
+    _close(np.array(ex), np.array(ex_r), 1e-4, "error_max history", floor=1e-3 * prm.pop_prec)
+
+
 def test_van_zadelhoff_1a_gpu():
     from test_oracle_nlte import _ratran_errors
 

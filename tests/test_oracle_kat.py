@@ -209,10 +209,10 @@ def test_velocity_gradient_erf_branch_closed_form(mwf, bound):
 
 
 def test_reference_is_sensitive_to_the_erff_rounding(small_wind):
-    """Documents why the kernels keep the correctly rounded erf in every
-    precision mode: the reference forms erf(b) - erf(a) in float, so the same
-    solve with glibc's erff (not correctly rounded) instead of the correctly
-    rounded one moves J by ~1e-4 relative, while the traced counts do not move."""
+    """Documents why the kernels evaluate the C library's own float functions:
+    the reference forms erf(b) - erf(a) in float, so the same solve with
+    correctly rounded functions instead of glibc's moves J by ~1e-4 relative,
+    while the traced counts do not move."""
     m = small_wind
     op = m.options()
     _, J1, _ = oracle.solve(m, op, with_u=False, kind="cr")
@@ -276,3 +276,33 @@ def test_new_imager_closed_form(mwf, bound):
     im_a = np.array([[I_im(f, r) for f in nu] for r in rs])
     err = np.abs(rel(im, im_a))[:-1]
     assert np.max(err) < bound, np.max(err)
+
+
+@pytest.mark.parametrize("approx", [0, 1, 2])
+def test_restated_functions_reproduce_the_c_library_solve(small_wind, approx):
+    """The oracle built on the restated float functions (the code the kernels
+    run) and the one calling the C library give the same u, J and Lambda bit for
+    bit, and the same traced counts, on a wind with every branch in use."""
+    m = small_wind
+    op = m.options()
+    op.approx = approx
+    a = oracle.solve(m, op, kind="ref")
+    b = oracle.solve(m, op, kind="glibc")
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+    np.testing.assert_array_equal(oracle.trace_counts(m, op, kind="ref"),
+                                  oracle.trace_counts(m, op, kind="glibc"))
+
+
+def test_restated_functions_reproduce_the_c_library_erf_branch():
+    """Same on the velocity-gradient benchmark with max_width_fraction 0.10,
+    where every segment takes the erf branch and J depends most on erff's bits."""
+    ld = models.read_lamda(os.path.join(HERE, "golden", "lamda_test.txt"), num=0)
+    m = models.model_velocity_gradient_1d(ld, max_width_fraction=0.10)
+    op = m.options()
+    a = oracle.solve(m, op, kind="ref")
+    b = oracle.solve(m, op, kind="glibc")
+    c = oracle.solve(m, op, kind="cr")
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+    assert np.max(np.abs(a[1] - c[1]) / np.abs(b[1])) > 1e-9
