@@ -8,10 +8,18 @@ elimination, u, J and the Lambda diagonal) of the 32^3 jittered spherical wind
 (32768 points, Delaunay neighbours, 12 HEALPix rays, 10-level linear rotor,
 9 lines x 31 quadrature bins), with every input already resident in HBM.
 
-The JSON line also carries the roofline of the dominant kernel (algorithmic
-bytes per launch / kernel time from HIP events on the launch stream) and the
-CPU baseline: the C restatement of the reference solver (oracle/) timed on
-this host's cores on the same model.
+The JSON line also carries:
+- both precisions (FP64 with the reference's float functions; mixed), each
+  with the device-to-host copy of J timed apart, and the adjoint's cost;
+- the roofline of the dominant kernel (algorithmic bytes per launch / kernel
+  time from HIP events on the launch stream; HBM traffic and VALU issue from
+  the rocprofv3 counters recorded under profiles/);
+- image_gradient: the 256^2 x 31 image-loss gradient of the same wind
+  through the adjoint (forward / adjoint cost, central-difference check);
+- batched: 64 winds of 2e3..1.3e5 points in one persistent launch against
+  per-model launches;
+- the CPU baseline: the C restatement of the reference solver (oracle/), with
+  the C library's float functions, timed on this host on the same model.
 """
 import argparse
 import json
@@ -35,6 +43,10 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="default workload: skip the batched 64-wind comparison")
+    ap.add_argument("--no-image-gradient", action="store_true",
+                    help="default workload: skip the 256^2 image-gradient sub-benchmark")
     ap.add_argument("--workload", default="wind32",
                     choices=["wind32", "batch", "image-gradient", "nlte", "nlte-full", "siblings"],
                     help="wind32: the single-model solve (default); batch: many winds in one "
@@ -140,7 +152,17 @@ def run_variant(dm, model, precision, steps, warmup):
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     lib.mgx_set_kernel_timing(0)
-    return 1e3 * elapsed / steps, float(np.mean(kernel_ms)), out["J"].cpu().numpy()
+    # the reference hands J back in host memory: time that copy apart (pinned buffer)
+    hJ = torch.empty(out["J"].shape, dtype=torch.float64, pin_memory=True)
+    d2h = []
+    for _ in range(5):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        hJ.copy_(out["J"], non_blocking=True)
+        torch.cuda.synchronize()
+        d2h.append(time.perf_counter() - t1)
+    return (1e3 * elapsed / steps, float(np.mean(kernel_ms)), hJ.numpy().copy(),
+            1e3 * float(np.median(d2h)))
 
 
 def calibrate_copy():
@@ -158,27 +180,19 @@ def calibrate_copy():
     print(json.dumps({"copy_bytes": n * 8, "copies": 3}), flush=True)
 
 
-def run_batch(args):
-    """Batched forward solve of n winds (point counts log-uniform in [2e3, 1.3e5],
-    12/48/192 rays, the same rotor): per-model launches against one persistent
-    launch with the autotuned tiling; value = total u values per second."""
+def batch_compare(ms, dms, precisions, steps):
+    """Per-model launches against one persistent launch with the autotuned tiling
+    for each precision in `precisions` ('mixed', 'fp64'); returns a dict per
+    precision: plain_ms, batch_ms, speedup, tuned, value, J_rel_diff_batch_vs_plain."""
     import numpy as np
     import torch
 
-    from magritte_amd import _abi, models, solver
+    from magritte_amd import _abi, solver
 
-    t0 = time.time()
-    specs = models.batch_wind_specs(n_models=args.n_models)
-    ms = []
-    for k, (e, r, sd) in enumerate(specs):
-        ms.append(models.model_wind(n_edge=e, nrays=r, seed=sd))
-        print(f"# model {k}: n_edge={e} P={e**3} nrays={r}", file=sys.stderr, flush=True)
-    dms = [solver.DeviceModel(m, "cuda:0") for m in ms]
     U = sum(m.hnrays * m.npoints * m.nfreqs for m in ms)
-    print(f"# {len(ms)} models built in {time.time() - t0:.1f}s, U={U:.3e}", file=sys.stderr,
-          flush=True)
     res = {}
-    for prec, name in ((_abi.PREC_MIXED, "mixed"), (_abi.PREC_FP64, "fp64")):
+    for name in precisions:
+        prec = _abi.PREC_MIXED if name == "mixed" else _abi.PREC_FP64
         op = ms[0].options(precision=prec)
         outs = [None] * len(ms)
         for k, dm in enumerate(dms):  # warm-up
@@ -189,12 +203,12 @@ def run_batch(args):
             solver.feautrier_forward(dm, op, with_u=False, with_lambda=False, out=outs[k])
         torch.cuda.synchronize()
         t_plain = time.perf_counter() - t1
-        print(f"# {name}: per-model launches {t_plain * 1e3:.1f} ms", file=sys.stderr, flush=True)
+        print(f"# batch {name}: per-model launches {t_plain * 1e3:.1f} ms", file=sys.stderr,
+              flush=True)
         plan = solver.BatchPlan(dms, op)
         best = plan.tune(sample_stride=16)
         plan.run()  # warm-up (graph capture)
         torch.cuda.synchronize()
-        steps = max(1, args.steps // 5)
         t2 = time.perf_counter()
         for _ in range(steps):
             Js = plan.run()
@@ -203,10 +217,38 @@ def run_batch(args):
         err = max(float(np.max(np.abs(J.cpu().numpy() - o["J"].cpu().numpy()) /
                                np.abs(o["J"].cpu().numpy()))) for J, o in zip(Js, outs))
         plan.close()
+        del outs, Js
         res[name] = {"plain_ms": round(t_plain * 1e3, 2), "batch_ms": round(t_batch * 1e3, 2),
                      "speedup": round(t_plain / t_batch, 3), "tuned": best,
                      "value": round(U / t_batch, 1), "J_rel_diff_batch_vs_plain": err}
-        print(f"# {name}: batch {t_batch * 1e3:.1f} ms, tuned {best}", file=sys.stderr, flush=True)
+        print(f"# batch {name}: batch {t_batch * 1e3:.1f} ms, tuned {best}", file=sys.stderr,
+              flush=True)
+    return res
+
+
+def build_batch_models(n_models):
+    """The C2 winds, built in forked processes (before the GPU is touched)."""
+    from magritte_amd import models
+
+    t0 = time.time()
+    specs = models.batch_wind_specs(n_models=n_models)
+    workers = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16), 16)
+    ms = models.batch_winds(specs, workers=workers)
+    print(f"# {len(ms)} batch models built in {time.time() - t0:.1f}s with {workers} processes, "
+          f"P total {sum(m.npoints for m in ms)}", file=sys.stderr, flush=True)
+    return ms
+
+
+def run_batch(args):
+    """Batched forward solve of n winds (point counts log-uniform in [2e3, 1.3e5],
+    12/48/192 rays, the same rotor): per-model launches against one persistent
+    launch with the autotuned tiling; value = total u values per second."""
+    from magritte_amd import solver
+
+    ms = build_batch_models(args.n_models)
+    dms = [solver.DeviceModel(m, "cuda:0") for m in ms]
+    U = sum(m.hnrays * m.npoints * m.nfreqs for m in ms)
+    res = batch_compare(ms, dms, ("mixed", "fp64"), max(1, args.steps // 5))
     line = {"metric": "Feautrier ray.point.freq/s (batched forward)", "value": res["mixed"]["value"],
             "unit": "ray*point*freq/s", "n_gpus": 1, "steps": max(1, args.steps // 5),
             "higher_is_better": True, "dtype": "f32+f64", "data": "synthetic",
@@ -216,33 +258,39 @@ def run_batch(args):
     print(json.dumps(line), flush=True)
 
 
-def run_image_gradient(args):
-    """Gradient of L = |I - I*|^2 for a 256^2 x 31 image (--image-size) of the 32^3
-    wind with respect to the line emissivity and opacity (P, L), and with respect to
-    the log-density and temperature of every point, through the adjoint
-    kernel behind torch.autograd; the target I* is the image of the same wind
-    with density and temperature perturbed by seeded 10% log-normal factors.
-    Reports forward and adjoint time, central differences of the GPU forward
-    (step 1e-6 relative) at 64 random (p, l), and mixed against FP64 gradients."""
+def image_gradient_target(m, n_edge, rng):
+    """The C3 target: the same wind with density and temperature perturbed by
+    seeded 10% log-normal factors."""
     import numpy as np
-    import torch
 
-    from magritte_amd import _abi, autograd, models, solver
+    from magritte_amd import models
 
-    rng = np.random.Generator(np.random.PCG64(models.seed_for(3)))
-    m = models.model_wind(n_edge=args.n_edge)
     P = m.npoints
     pert_n = np.exp(0.1 * rng.standard_normal(P))
     pert_t = np.exp(0.1 * rng.standard_normal(P))
-    mt = models.model_wind(n_edge=args.n_edge, clumps=pert_n, temp_scale=pert_t)
+    return models.model_wind(n_edge=n_edge, clumps=pert_n, temp_scale=pert_t)
+
+
+def image_gradient_core(m, dm, mt, nx, ny, rng, fd_points=64):
+    """Gradient of L = |I - I*|^2 for an nx x ny x 31 image of model m with respect
+    to the line emissivity and opacity (P, L), through the adjoint kernel behind
+    torch.autograd, I* the image of mt.  Returns (variants, fd_check,
+    mixed_vs_fp64, context): forward and adjoint time per precision, central
+    differences of the GPU forward (step 1e-6 relative) at fd_points random (p, l),
+    and mixed against FP64 gradients."""
+    import numpy as np
+    import torch
+
+    from magritte_amd import _abi, autograd, solver
+
     nu0 = m.line[0]
     fr = m.image_frequencies(nu0 * (1 - 1e-4), nu0 * (1 + 1e-4), 31)
     rd = np.array([0.3, -0.5, 0.81])
-    nx = ny = args.image_size
-    dm, dmt = solver.DeviceModel(m, "cuda:0"), solver.DeviceModel(mt, "cuda:0")
+    dmt = solver.DeviceModel(mt, "cuda:0")
     lib = _abi.lib()
     lib.mgx_set_kernel_timing(1)
     Istar, _, _ = solver.image_new(dmt, mt.options(), rd, nx, ny, fr)
+    del dmt
     res = {}
     grads = {}
     for prec, name in ((_abi.PREC_FP64, "fp64"), (_abi.PREC_MIXED, "mixed")):
@@ -284,7 +332,7 @@ def run_image_gradient(args):
     base_k = dm.tensors["opacity"]
     for which, g in (("eps", ge), ("kap", gk)):
         cand = np.flatnonzero(np.abs(g) > 1e-3 * np.abs(g).max())
-        for i in rng.choice(cand, 32, replace=False):
+        for i in rng.choice(cand, fd_points // 2, replace=False):
             p, l = divmod(int(i), m.nlines)
             Is = []
             for sgn in (1.0, -1.0):
@@ -301,6 +349,33 @@ def run_image_gradient(args):
     for k, (a, b) in zip(("eps", "kap"), zip(grads["mixed"], grads["fp64"])):
         sig = np.abs(b) > 1e-3 * np.abs(b).max()
         mix_err[k] = float(np.max(np.abs(a[sig] - b[sig]) / np.abs(b[sig])))
+    fd_check = {"points": int(len(errs)), "step": h, "interpolation": "off",
+                "max_rel_err": float(errs.max()), "median_rel_err": float(np.median(errs)),
+                "frac_below_1e-4": float(np.mean(errs < 1e-4))}
+    lib.mgx_set_kernel_timing(0)
+    return res, fd_check, mix_err, (Istar, rd, fr)
+
+
+def run_image_gradient(args):
+    """Gradient of L = |I - I*|^2 for a 256^2 x 31 image (--image-size) of the 32^3
+    wind with respect to the line emissivity and opacity (P, L), and with respect to
+    the log-density and temperature of every point, through the adjoint
+    kernel behind torch.autograd; the target I* is the image of the same wind
+    with density and temperature perturbed by seeded 10% log-normal factors.
+    Reports forward and adjoint time, central differences of the GPU forward
+    (step 1e-6 relative) at 64 random (p, l), and mixed against FP64 gradients."""
+    import numpy as np
+
+    from magritte_amd import _abi, models, solver
+
+    rng = np.random.Generator(np.random.PCG64(models.seed_for(3)))
+    m = models.model_wind(n_edge=args.n_edge)
+    mt = image_gradient_target(m, args.n_edge, rng)
+    nx = ny = args.image_size
+    dm = solver.DeviceModel(m, "cuda:0")
+    res, fd_check, mix_err, (Istar, rd, fr) = image_gradient_core(m, dm, mt, nx, ny, rng)
+    lib = _abi.lib()
+    lib.mgx_set_kernel_timing(1)
     lnT = _image_gradient_lnT(m, dm, Istar, rd, nx, ny, fr, rng, lib)
     U_img = nx * ny * fr["nu"].shape[1]
     line = {"metric": "image gradient (adjoint) ray.point.freq/s", "unit": "pixel*freq/s",
@@ -309,11 +384,7 @@ def run_image_gradient(args):
             "config": {"workload": f"{args.n_edge}^3 wind, {nx}x{ny} image at 31 frequencies, "
                                    f"L2 loss against 10% log-normal n/T perturbation",
                        "variants": res,
-                       "fd_check": {"points": int(len(errs)), "step": h,
-                                    "interpolation": "off",
-                                    "max_rel_err": float(errs.max()),
-                                    "median_rel_err": float(np.median(errs)),
-                                    "frac_below_1e-4": float(np.mean(errs < 1e-4))},
+                       "fd_check": fd_check,
                        "mixed_vs_fp64_grad_max_rel_err": mix_err,
                        "wrt_log_density_temperature": lnT}}
     print(json.dumps(line), flush=True)
@@ -756,6 +827,73 @@ def run_siblings(args):
     print(json.dumps(res), flush=True)
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(model, U, args):
+    """The C restatement of the reference solver with the C library's float
+    functions (the reference's arithmetic), timed on this host: one warm-up and
+    five timed solves, the median reported.  Threads: this process's CPU share
+    (OMP_NUM_THREADS when set, which the GPU boxes set to the one-GPU share;
+    else all cores).  Returns (summary, J of that solve, J of the same solve
+    with correctly rounded float functions)."""
+    import numpy as np
+
+    from oracle import oracle
+
+    ncpu = os.cpu_count() or 1
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = args.cpu_threads or (min(ncpu, share) if share > 0 else ncpu)
+    times = []
+    J = None
+    for k in range(6):
+        t1 = time.perf_counter()
+        _, J, _ = oracle.solve(model, model.options(), with_u=False, nthreads=threads, kind="glibc")
+        dt = time.perf_counter() - t1
+        print(f"# CPU solve {k} ({'warm-up' if k == 0 else 'timed'}): {dt:.2f}s, {threads} threads",
+              file=sys.stderr, flush=True)
+        if k > 0:
+            times.append(dt)
+    tmed = float(np.median(times))
+    _, J_cr, _ = oracle.solve(model, model.options(), with_u=False, nthreads=threads, kind="cr")
+    cpu = {"value": round(U / tmed, 1), "unit": "ray*point*freq/s", "cores": threads,
+           "kind": "port",
+           "sample": f"full solves of the same {model.npoints}-point wind (setup pass + trace + "
+                     f"solve + J + Lambda) by the C restatement with the C library's float "
+                     f"functions: median of 5 after 1 warm-up, {threads} OpenMP threads",
+           "host": {"nproc": ncpu, "cpu_model": _cpu_model(), "omp_num_threads": share or None},
+           "runs_s": [round(t, 3) for t in times], "median_s": round(tmed, 3),
+           "J_rel_diff_cr_vs_glibc": float(np.max(np.abs(J_cr - J) /
+                                                  np.maximum(np.abs(J), 1e-300)))}
+    return cpu, J, J_cr
+
+
+def read_valu(workload_key, kernel_ms):
+    """VALU issue of the dominant kernel from rocprofv3 counters recorded in
+    profiles/counters.json (SQ_INSTS_VALU and SQ_INSTS_SALU per dispatch, wave
+    instructions): busy = SQ_INSTS_VALU x 4 cycles / (kernel time x 256 CUs x 4
+    SIMDs x 2.4 GHz); 4 cycles is the issue time of a wave64 FP64 VALU
+    instruction, so this is the FP64-rate bound of the issue fraction."""
+    path = os.path.join(ROOT, "profiles", "counters.json")
+    try:
+        with open(path) as fh:
+            d = json.load(fh)[workload_key]
+    except Exception:
+        return None
+    peak = 256 * 4 * 2.4e9 / 4.0  # wave64 VALU instructions per second at 4 cycles each
+    busy = d["SQ_INSTS_VALU"] / (kernel_ms * 1e-3) / peak
+    return {"insts_per_launch": d["SQ_INSTS_VALU"], "salu_insts_per_launch": d.get("SQ_INSTS_SALU"),
+            "issue_peak_per_s": peak, "frac": round(busy, 4), "source": d.get("source")}
+
+
 def main():
     args = parse()
     if args.workload == "nlte":
@@ -795,23 +933,33 @@ def main():
         sys.exit(2)
     import numpy as np
 
-    from magritte_amd import models, solver
+    from magritte_amd import models
 
     rank = int(os.environ.get("RANK", "0"))
+    # the CPU-side model generation first, so the worker processes fork before the GPU is used
+    batch_models = None if args.no_batch else build_batch_models(args.n_models)
     t0 = time.time()
+    rng = np.random.Generator(np.random.PCG64(models.seed_for(3)))
     model = models.model_wind(n_edge=args.n_edge)
+    target = None if args.no_image_gradient else image_gradient_target(model, args.n_edge, rng)
+    print(f"# wind model built in {time.time() - t0:.1f}s: P={model.npoints} "
+          f"hnrays={model.hnrays} F={model.nfreqs}", file=sys.stderr, flush=True)
+
+    from magritte_amd import solver
+
     dm = solver.DeviceModel(model, "cuda:0")
     P, F, L, R = model.npoints, model.nfreqs, model.nlines, model.hnrays
     U = R * P * F
-    print(f"# model built in {time.time() - t0:.1f}s: P={P} hnrays={R} F={F}", file=sys.stderr,
-          flush=True)
 
     variants = {}
     for prec in ("fp64", "mixed"):
-        ms, kms, J = run_variant(dm, model, prec, args.steps, args.warmup)
+        ms, kms, J, d2h_ms = run_variant(dm, model, prec, args.steps, args.warmup)
         variants[prec] = {"ms_per_step": round(ms, 4), "kernel_ms": round(kms, 4),
-                          "value": round(U / (ms * 1e-3), 1), "J": J}
-        print(f"# {prec}: {ms:.3f} ms/step, kernel {kms:.3f} ms", file=sys.stderr, flush=True)
+                          "value": round(U / (ms * 1e-3), 1), "J": J,
+                          "d2h_J_ms": round(d2h_ms, 4),
+                          "value_with_d2h_J": round(U / ((ms + d2h_ms) * 1e-3), 1)}
+        print(f"# {prec}: {ms:.3f} ms/step, kernel {kms:.3f} ms, J to host {d2h_ms:.3f} ms",
+              file=sys.stderr, flush=True)
     adjoint = {}
     if not args.no_adjoint:
         for prec in ("fp64", "mixed"):
@@ -821,34 +969,44 @@ def main():
                              "cost_vs_forward": round(ams / variants[prec]["ms_per_step"], 3)}
             print(f"# adjoint {prec}: {ams:.3f} ms/step, kernel {akms:.3f} ms", file=sys.stderr,
                   flush=True)
-
     stats = solver.trace_stats(dm, model.options())
+
+    image_gradient = None
+    if target is not None:
+        n = args.image_size
+        res, fd_check, mix_err, _ = image_gradient_core(model, dm, target, n, n, rng)
+        image_gradient = {
+            "workload": f"{args.n_edge}^3 wind, {n}x{n} image at 31 frequencies, L2 loss "
+                        f"against the image of the wind with 10% log-normal n/T perturbations",
+            "value": round(n * n * 31 / (res["mixed"]["adjoint_kernel_ms"] * 1e-3), 1),
+            "unit": "pixel*freq/s (adjoint)", "variants": res, "fd_check": fd_check,
+            "mixed_vs_fp64_grad_max_rel_err": mix_err}
+        print(f"# image gradient {n}^2: adjoint/forward mixed "
+              f"{res['mixed']['cost_vs_forward']}, fp64 {res['fp64']['cost_vs_forward']}",
+              file=sys.stderr, flush=True)
+        del target
+
+    batched = None
+    if batch_models is not None:
+        dms = [solver.DeviceModel(m, "cuda:0") for m in batch_models]
+        Ub = sum(m.hnrays * m.npoints * m.nfreqs for m in batch_models)
+        res = batch_compare(batch_models, dms, ("mixed", "fp64"), 1)
+        batched = {"workload": f"{len(batch_models)} jittered spherical winds, P log-uniform "
+                               f"[2e3, 1.3e5], 12/48/192 HEALPix rays, 9 lines x 31 bins",
+                   "U": Ub, "value": res["mixed"]["value"], "unit": "ray*point*freq/s",
+                   "variants": res,
+                   "speedup_vs_plain_fp64": round(res["fp64"]["plain_ms"] /
+                                                  res["mixed"]["batch_ms"], 3)}
+        del dms, batch_models
+
     cpu = None
     if not args.no_cpu_baseline and rank == 0:
-        from oracle import oracle
-
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
-        # the reference's own float functions (glibc expf/expm1f/logf/powf/erff)
-        t1 = time.perf_counter()
-        _, J_glibc, _ = oracle.solve(model, model.options(), with_u=False, nthreads=threads,
-                                     kind="glibc")
-        tc = time.perf_counter() - t1
-        # the same solve with correctly rounded float functions: how much J depends on
-        # the last bits of the C library's expf/erff
-        _, J_cr, _ = oracle.solve(model, model.options(), with_u=False, nthreads=threads,
-                                  kind="cr")
-        J_ref = J_glibc
-        cpu = {"value": round(U / tc, 1), "unit": "ray*point*freq/s", "cores": threads,
-               "kind": "port",
-               "sample": f"one full solve of the same wind{args.n_edge}^3 model "
-                         f"(setup pass + trace + solve + J + Lambda) by the C restatement "
-                         f"with the C library's float functions, {tc:.2f}s, "
-                         f"{threads} OpenMP threads",
-               "J_rel_diff_cr_vs_glibc": float(np.max(np.abs(J_cr - J_glibc) /
-                                                      np.maximum(np.abs(J_glibc), 1e-300)))}
+        cpu, J_glibc, J_cr = cpu_baseline(model, U, args)
         for v in variants.values():
             v["J_rel_err_vs_cpu_fp64"] = float(
-                np.max(np.abs(v["J"] - J_ref) / np.maximum(np.abs(J_ref), 1e-300)))
+                np.max(np.abs(v["J"] - J_glibc) / np.maximum(np.abs(J_glibc), 1e-300)))
+            v["J_rel_err_vs_cpu_fp64_correctly_rounded"] = float(
+                np.max(np.abs(v["J"] - J_cr) / np.maximum(np.abs(J_cr), 1e-300)))
     for v in variants.values():
         del v["J"]
 
@@ -865,7 +1023,7 @@ def main():
                 "traffic": read_traffic(workload), "kernel": "feautrier_kernel",
                 "kernel_ms": hv["kernel_ms"], "alg_bytes": int(B),
                 "walk_steps": stats[0], "neighbours": stats[1], "traced_entries": stats[2],
-                "interp_entries": stats[3]}
+                "interp_entries": stats[3], "valu": read_valu(workload, hv["kernel_ms"])}
     line = {
         "metric": "Feautrier ray.point.freq/s (forward)",
         "value": hv["value"],
@@ -882,7 +1040,8 @@ def main():
         "config": {"workload": f"{args.n_edge}^3 jittered spherical wind, Delaunay neighbours, "
                                f"12 HEALPix rays, 10-level linear rotor, 9 lines x 31 bins",
                    "npoints": P, "hnrays": R, "nfreqs": F, "precision": head,
-                   "approx": "CloseLines", "variants": variants, "adjoint": adjoint},
+                   "approx": "CloseLines", "variants": variants, "adjoint": adjoint,
+                   "image_gradient": image_gradient, "batched": batched},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

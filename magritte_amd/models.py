@@ -597,9 +597,27 @@ def batch_wind_specs(n_models=64, p_min=2.0e3, p_max=1.3e5, nrays_choices=(12, 4
             for i, (lp, r) in enumerate(zip(logp, nr))]
 
 
-def batch_winds(specs):
-    """The wind models of batch_wind_specs (same rotor as model_wind)."""
-    return [model_wind(n_edge=e, nrays=r, seed=sd) for e, r, sd in specs]
+def _wind_from_spec(spec):
+    e, r, sd = spec
+    return model_wind(n_edge=e, nrays=r, seed=sd)
+
+
+def batch_winds(specs, workers=1):
+    """The wind models of batch_wind_specs (same rotor as model_wind), in order.
+    workers > 1 builds them in that many forked processes (the Delaunay
+    triangulation is single-threaded); fork before the process uses the GPU."""
+    if workers <= 1 or len(specs) < 2:
+        return [_wind_from_spec(sp) for sp in specs]
+    import multiprocessing as mp
+
+    # largest first, so the pool's tail is short
+    order = sorted(range(len(specs)), key=lambda i: -specs[i][0])
+    with mp.get_context("fork").Pool(min(workers, len(specs))) as pool:
+        built = pool.map(_wind_from_spec, [specs[i] for i in order], chunksize=1)
+    out = [None] * len(specs)
+    for i, m in zip(order, built):
+        out[i] = m
+    return out
 
 
 # LAMDA collision-partner codes -> (name, ortho/para flag)

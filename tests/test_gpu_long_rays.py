@@ -90,3 +90,22 @@ def test_too_long_chain_fails_cleanly():
     ms = models.model_line_1d(npoints=50, nquads=3)
     out = solver.feautrier_forward(solver.DeviceModel(ms), ms.options())
     assert np.isfinite(out["J"].cpu().numpy()).all()
+
+
+def test_interpolation_count_overflow_is_its_own_error():
+    """A segment needing more than 65535 interpolation points (here: the criterion
+    doubles at one point while max_interpolation_diff is 1 + 1e-6, so
+    ceil(log 2 / log(1 + 1e-6)) = 693147) fails at once with its own message,
+    not as a walk-buffer overflow; the library stays usable."""
+    from magritte_amd import solver
+
+    m = models.model_line_1d(nquads=3)
+    m.interp_crit[50, :] *= 2.0
+    op = m.options()
+    op.max_interpolation_diff = 1.0 + 1e-6
+    print("[interpolation overflow] line P=100, one criterion jump of 2", flush=True)
+    dm = solver.DeviceModel(m)
+    with pytest.raises(_abi.MgxError, match="65535 interpolation points"):
+        solver.feautrier_forward(dm, op)
+    out = solver.feautrier_forward(dm, m.options())
+    assert np.isfinite(out["J"].cpu().numpy()).all()
