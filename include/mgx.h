@@ -136,6 +136,26 @@ int mgx_feautrier_forward_device(const mgx_model *model_dev, const mgx_options *
                                  double *u, double *J, double *lambda_diag, void *stream);
 
 /*
+ * The plain-method solve of mgx_feautrier_forward_device (same kernels, same
+ * results bit for bit) without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.
+ * Its scratch (status words, launch parameters and the packed point and line
+ * tables: 256-byte-aligned pieces of 16 B, the parameter block, 80 B per point
+ * and 32 B per (point, line)) lives in a static device array of
+ * MGX_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; a model
+ * needing more fails with MGX_E_ARG (e.g. 64 MiB holds the 32^3 wind with 9
+ * lines, 12 MB).  The caller passes the inputs and outputs as for
+ * mgx_feautrier_forward_device.  Every call on a GPU shares that GPU's array,
+ * which keeps the last call's contents: the calls are serialised by a lock,
+ * and work that overlaps on one GPU through other means races.  No kernel
+ * timing (it would create events).
+ */
+#define MGX_NOALLOC_SCRATCH_BYTES (64u << 20)
+int mgx_feautrier_forward_noalloc(const mgx_model *model_dev, const mgx_options *opts, double *u,
+                                  double *J, double *lambda_diag, void *stream);
+
+/*
  * The plain-method formal solve with the general Lambda operator: u
  * (hnrays,P,F) [may be NULL] and J (P,F) as mgx_feautrier_forward_device, and
  * for each receiving point p and line l the list of approximated-Lambda

@@ -125,6 +125,8 @@ SIGNATURES = [
     ("mgx_version", C.c_char_p, []),
     ("mgx_feautrier_forward_device", C.c_int,
      [C.POINTER(MgxModel), C.POINTER(MgxOptions), _VP, _VP, _VP, _VP]),
+    ("mgx_feautrier_forward_noalloc", C.c_int,
+     [C.POINTER(MgxModel), C.POINTER(MgxOptions), _VP, _VP, _VP, _VP]),
     ("mgx_shortchar_order_0_device", C.c_int,
      [C.POINTER(MgxModel), C.POINTER(MgxOptions), C.c_double, _VP, _VP, _VP, _VP, _VP, _VP]),
     ("mgx_feautrier_order_2_uv_device", C.c_int,

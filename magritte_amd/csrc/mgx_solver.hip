@@ -3520,7 +3520,7 @@ struct DevBuffers {
     PointIdx *pidx = nullptr;
     uint32_t *status = nullptr;
     KParams *kp_dev = nullptr; /* device copy of the launch parameters */
-    bool owned = true;         /* false: the memory belongs to someone else */
+    bool owned = true;         /* false: carved out of the no-allocation scratch */
     ~DevBuffers()
     {
         if (!owned) return;
@@ -3571,12 +3571,13 @@ const uint32_t kMaxCap = 1280; /* 1280 steps per half: 60 KB of dynamic LDS */
 
 bool g_timing = false;
 double g_last_kernel_ms = 0.0;
+thread_local bool t_no_events = false; /* set by the no-allocation entry point */
 
 /* launches fn() between two events on st when timing is enabled */
 template <typename Fn>
 void timed(hipStream_t st, Fn fn)
 {
-    if (!g_timing) {
+    if (!g_timing || t_no_events) {
         fn();
         return;
     }
@@ -3627,6 +3628,36 @@ int prepare(const mgx_model *m, const mgx_options *op, hipStream_t st, DevBuffer
     return prepare_tables(m, op, st, b, kp);
 }
 
+/* ------------------------------------------------------------------------- */
+/* the no-allocation scratch                                                 */
+
+/* Device memory of mgx_feautrier_forward_noalloc: the status words, the launch
+   parameters and the packed point and line tables live here instead of in
+   per-call allocations.  Each GPU holds its own copy for the life of the
+   process; host code reaches it only through the address hipGetSymbolAddress
+   returns on that GPU. */
+static __device__ __align__(256) unsigned char scratch[MGX_NOALLOC_SCRATCH_BYTES];
+
+std::mutex g_scratch_mutex; /* guards g_scratch_ptr and serialises the calls that use it */
+constexpr int kMaxDevices = 64;
+unsigned char *g_scratch_ptr[kMaxDevices] = {};
+
+/* the current GPU's copy of scratch (looked up once per GPU); g_scratch_mutex held */
+int scratch_base(unsigned char **out)
+{
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return mgx::fail(MGX_E_ARG, "device index %d out of range", dev);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return mgx::fail(MGX_E_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<unsigned char *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return MGX_OK;
+}
 
 /* reads the status words; returns MGX_OK, or an error (needed capacity in *need;
    segments needed by the adjoint's scratch in *need_e) */
@@ -4219,6 +4250,54 @@ int mgx_feautrier_forward_device(const mgx_model *model, const mgx_options *opts
     return forward_launch(model, opts, u, J, lambda_diag, st, b, kp);
 }
 
+static size_t noalloc_scratch_bytes(const mgx_model *model)
+{
+    if (!model) return 0;
+    const size_t P = model->npoints, PL = (size_t)model->npoints * model->nlines;
+    const auto rnd = [](size_t n) { return (n + 255) & ~(size_t)255; };
+    return rnd(4 * sizeof(uint32_t)) + rnd(sizeof(KParams)) + rnd(P * sizeof(PointRec)) +
+           rnd(P * sizeof(PointIdx)) + rnd(PL * sizeof(double4));
+}
+
+int mgx_feautrier_forward_noalloc(const mgx_model *model, const mgx_options *opts, double *u,
+                                  double *J, double *lambda_diag, void *stream)
+{
+    mgx::clear_error();
+    if (int e = check_view(model, opts)) return e;
+    if (int e = check_lambda_diagonal(opts)) return e;
+    if (!J) return mgx::fail(MGX_E_ARG, "J is required");
+    const size_t need = noalloc_scratch_bytes(model);
+    if (need > (size_t)MGX_NOALLOC_SCRATCH_BYTES)
+        return mgx::fail(MGX_E_ARG,
+                         "the model needs %zu bytes of scratch (npoints %u, nlines %u); the "
+                         "no-allocation solve holds %zu (use mgx_feautrier_forward_device)",
+                         need, model->npoints, model->nlines, (size_t)MGX_NOALLOC_SCRATCH_BYTES);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    unsigned char *base = nullptr;
+    if (int e = scratch_base(&base)) return e;
+    const size_t P = model->npoints, PL = (size_t)model->npoints * model->nlines;
+    size_t off = 0;
+    const auto take = [&](size_t bytes) {
+        unsigned char *p = base + off;
+        off += (bytes + 255) & ~(size_t)255;
+        return p;
+    };
+    DevBuffers b;
+    b.owned = false;
+    b.status = reinterpret_cast<uint32_t *>(take(4 * sizeof(uint32_t)));
+    b.kp_dev = reinterpret_cast<KParams *>(take(sizeof(KParams)));
+    b.pts = reinterpret_cast<PointRec *>(take(P * sizeof(PointRec)));
+    b.pidx = reinterpret_cast<PointIdx *>(take(P * sizeof(PointIdx)));
+    b.lpack = reinterpret_cast<double4 *>(take(PL * sizeof(double4)));
+    struct NoEvents {
+        NoEvents() { t_no_events = true; }
+        ~NoEvents() { t_no_events = false; }
+    } no_events;
+    hipStream_t st = (hipStream_t)stream;
+    KParams kp;
+    if (int e = prepare_tables(model, opts, st, b, kp)) return e;
+    return forward_launch(model, opts, u, J, lambda_diag, st, b, kp);
+}
 
 int mgx_feautrier_forward_lambda_device(const mgx_model *model, const mgx_options *opts,
                                         double *u, double *J, uint32_t cap, uint32_t *lam_count,
