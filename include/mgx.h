@@ -669,6 +669,20 @@ int mgx_trace_stats_device(const mgx_model *model_dev, const mgx_options *opts,
  * with a larger one).  No counterpart in the reference (measurement aid).
  */
 void mgx_set_kernel_timing(int enable);
+
+/*
+ * Verification aid (no counterpart in the reference): evaluates the device
+ * build of one of the reference's float functions as this library restates
+ * them -- fn 0 expf, 1 expm1f, 2 erff, 3 logf, 4 powf(x, y) -- on every float
+ * argument with bits in [first, first + nchunks * 2^chunk_log2) and writes
+ * per chunk the wrapping sum of a 64-bit digest of (argument bits, result
+ * bits) into digest (device, nchunks entries), NaN results canonicalised.
+ * Equal digests to the C library's (computed on the host the same way) show
+ * the GPU gives the C library's bits on the whole range.  Synchronises the
+ * stream.  MGX_E_ARG for an unknown fn or a range beyond 2^32.
+ */
+int mgx_fmath_digest(int fn, float y, uint64_t first, uint32_t nchunks, uint32_t chunk_log2,
+                     uint64_t *digest, void *stream);
 double mgx_last_kernel_ms(void);
 
 #ifdef __cplusplus

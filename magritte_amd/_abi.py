@@ -199,6 +199,8 @@ SIGNATURES = [
      [_VP, C.c_uint32, _VP, _VP, _VP, C.POINTER(MgxNlteConv), _VP]),
     ("mgx_nlte_trial_ng", C.c_int, [_VP, C.c_uint32, C.POINTER(MgxNlteConv), _VP]),
     ("mgx_nlte_history", C.c_int, [_VP, _VP]),
+    ("mgx_fmath_digest", C.c_int, [C.c_int, C.c_float, C.c_uint64, C.c_uint32, C.c_uint32, _VP,
+                                   _VP]),
     ("mgx_nlte_last_lambda", C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP]),
     ("mgx_compute_level_populations_sparse", C.c_int,
      [C.POINTER(MgxModel), C.POINTER(MgxOptions), _VP, C.c_int, C.c_int64, _VP, _VP, _VP,

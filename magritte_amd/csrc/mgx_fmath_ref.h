@@ -75,6 +75,25 @@ MGX_FM_CONST uint64_t MGX_REF_EXP2_TAB[32] = {
     0x3feee89f995ad3adULL, 0x3feeff76f2fb5e47ULL, 0x3fef199bdd85529cULL, 0x3fef3720dcef9069ULL,
     0x3fef5818dcfba487ULL, 0x3fef7c97337b9b5fULL, 0x3fefa4afa2a490daULL, 0x3fefd0765b6e4540ULL};
 
+/*
+ * Where the tables are read.  On the GPU a per-lane index into constant memory
+ * is a global load in the middle of a dependent chain, so device code reads
+ * copies in LDS: every kernel that evaluates expf, erff, logf or powf calls
+ * mgx_ref_tables_init() (below) as its first statement, in all threads of the
+ * block (it ends with a barrier).  Host code reads the constant tables.
+ */
+#if defined(__HIP_DEVICE_COMPILE__)
+static __shared__ uint64_t mgx_ref_lds_exp2[32];
+static __shared__ double mgx_ref_lds_log[16][3];
+#define MGX_REF_EXP2_AT(i) mgx_ref_lds_exp2[(i)]
+#define MGX_REF_LOG_AT(i, j) mgx_ref_lds_log[(i)][(j)]
+#else
+#define MGX_REF_EXP2_AT(i) MGX_REF_EXP2_TAB[(i)]
+#define MGX_REF_LOG_AT(i, j) MGX_REF_LOG_TAB[(i)][(j)]
+#endif
+/* LDS the tables take in each workgroup of a kernel that uses them */
+#define MGX_REF_LDS_TABLE_BYTES (32 * 8 + 16 * 3 * 8)
+
 /* Cubic for 2^(r/32), r in [-1/2, 1/2]: coefficients of r^3, r^2, r (the
  * minimax fit of optimized-routines' exp2f, scaled by 1/32^3, 1/32^2, 1/32). */
 #define MGX_REF_EXPF_C0 (0x1.c6af84b912394p-5 / 32 / 32 / 32)
@@ -84,49 +103,55 @@ MGX_FM_CONST uint64_t MGX_REF_EXP2_TAB[32] = {
 #define MGX_REF_EXPF_SHIFT 0x1.8p+52
 
 /* glibc expf; FUSED selects the build with fused multiply-adds (the exported
- * symbol on x86-64 CPUs with FMA) or the baseline build (erff's internal call). */
+ * symbol on x86-64 CPUs with FMA) or the baseline build (erff's internal call).
+ * The main path runs for every argument and the results of |x| >= 88, inf and
+ * nan are selected afterwards (the same values as glibc's early returns; on
+ * the GPU this avoids divergent branches around a path no lane takes). */
 MGX_FM_HD float mgx_ref_expf_impl(float x, int fused)
 {
     const double xd = (double)x;
-    const uint32_t abstop = (mgx_rf_asuint(x) >> 20) & 0x7ff;
-    if (abstop >= (mgx_rf_asuint(88.0f) >> 20)) {
-        /* |x| >= 88 or nan */
-        if (mgx_rf_asuint(x) == 0xff800000u) return 0.0f;
-        if (abstop >= 0x7f8) return x + x;
-        if (x > 0x1.62e42ep6f) return mgx_rf_asfloat(0x7f800000u);
-        if (x < -0x1.9fe368p6f) return 0.0f;
-    }
-    double kd, r;
+    const uint32_t ux = mgx_rf_asuint(x);
+    const uint32_t abstop = (ux >> 20) & 0x7ff;
+    double kd, r, zz = 0.0;
+    uint64_t ki;
     if (fused) {
         /* both uses of x*InvLn2N are sums, so the FMA build fuses them */
         kd = MGX_FMA(MGX_REF_EXPF_INVLN2N, xd, MGX_REF_EXPF_SHIFT);
-        const uint64_t ki = mgx_fm_bits(kd);
+        ki = mgx_fm_bits(kd);
         kd -= MGX_REF_EXPF_SHIFT;
         r = MGX_FMA(MGX_REF_EXPF_INVLN2N, xd, -kd);
-        uint64_t t = MGX_REF_EXP2_TAB[ki % 32];
-        t += ki << 47;
-        const double s = mgx_fm_from_bits(t);
-        const double z = MGX_FMA(MGX_REF_EXPF_C0, r, MGX_REF_EXPF_C1);
-        const double r2 = r * r;
-        double y = MGX_FMA(MGX_REF_EXPF_C2, r, 1.0);
-        y = MGX_FMA(z, r2, y);
-        y = y * s;
-        return (float)y;
+    } else {
+        zz = MGX_REF_EXPF_INVLN2N * xd;
+        kd = zz + MGX_REF_EXPF_SHIFT;
+        ki = mgx_fm_bits(kd);
+        kd -= MGX_REF_EXPF_SHIFT;
+        r = zz - kd;
     }
-    const double zz = MGX_REF_EXPF_INVLN2N * xd;
-    kd = zz + MGX_REF_EXPF_SHIFT;
-    const uint64_t ki = mgx_fm_bits(kd);
-    kd -= MGX_REF_EXPF_SHIFT;
-    r = zz - kd;
-    uint64_t t = MGX_REF_EXP2_TAB[ki % 32];
+    uint64_t t = MGX_REF_EXP2_AT(ki % 32);
     t += ki << 47;
     const double s = mgx_fm_from_bits(t);
-    const double z = MGX_REF_EXPF_C0 * r + MGX_REF_EXPF_C1;
+    double z, y;
     const double r2 = r * r;
-    double y = MGX_REF_EXPF_C2 * r + 1.0;
-    y = z * r2 + y;
+    if (fused) {
+        z = MGX_FMA(MGX_REF_EXPF_C0, r, MGX_REF_EXPF_C1);
+        y = MGX_FMA(MGX_REF_EXPF_C2, r, 1.0);
+        y = MGX_FMA(z, r2, y);
+    } else {
+        z = MGX_REF_EXPF_C0 * r + MGX_REF_EXPF_C1;
+        y = MGX_REF_EXPF_C2 * r + 1.0;
+        y = z * r2 + y;
+    }
     y = y * s;
-    return (float)y;
+    float res = (float)y;
+    /* |x| >= 88 or nan: -inf -> 0, inf/nan -> x + x, overflow -> inf, underflow -> 0 */
+    if (abstop >= 0x42b) {
+        res = (ux == 0xff800000u) ? 0.0f
+            : (abstop >= 0x7f8)   ? x + x
+            : (x > 0x1.62e42ep6f) ? mgx_rf_asfloat(0x7f800000u)
+            : (x < -0x1.9fe368p6f) ? 0.0f
+                                   : res;
+    }
+    return res;
 }
 
 MGX_FM_HD float mgx_ref_expf(float x) { return mgx_ref_expf_impl(x, 1); }
@@ -209,6 +234,27 @@ MGX_FM_HD float mgx_ref_expm1f(float x)
     return y;
 }
 
+/* a / b rounded to float.  Host: the IEEE division.  Device: v_rcp_f32, one
+ * Newton step and a residual correction (Markstein's sequence: q0 = a y,
+ * r = a - b q0 exactly by FMA, q = q0 + r y) instead of the compiler's scaled
+ * division, valid where neither the quotient nor the operands come near the
+ * float overflow or underflow thresholds, as in erff's divisions.  That it
+ * rounds every one of erff's quotients as the IEEE division does is checked on
+ * all 2^32 arguments of erff (mgx_fmath_digest, tests/test_gpu_fmath.py). */
+MGX_FM_HD float mgx_ref_fdiv(float a, float b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    float y = __builtin_amdgcn_rcpf(b);
+    const float e = __builtin_fmaf(-b, y, 1.0f);
+    y = __builtin_fmaf(e, y, y);
+    const float q = a * y;
+    const float r = __builtin_fmaf(-b, q, a);
+    return __builtin_fmaf(r, y, q);
+#else
+    return a / b;
+#endif
+}
+
 /* fdlibm s_erff.c */
 MGX_FM_HD float mgx_ref_erff(float x)
 {
@@ -252,22 +298,23 @@ MGX_FM_HD float mgx_ref_erff(float x)
         z = x * x;
         r = pp0 + z * (pp1 + z * (pp2 + z * (pp3 + z * pp4)));
         s = one + z * (qq1 + z * (qq2 + z * (qq3 + z * (qq4 + z * qq5))));
-        y = r / s;
+        y = mgx_ref_fdiv(r, s);
         return x + x * y;
     }
     if (ix < 0x3fa00000u) {          /* 0.84375 <= |x| < 1.25 */
         s = fabsf(x) - one;
         P = pa0 + s * (pa1 + s * (pa2 + s * (pa3 + s * (pa4 + s * (pa5 + s * pa6)))));
         Q = one + s * (qa1 + s * (qa2 + s * (qa3 + s * (qa4 + s * (qa5 + s * qa6)))));
-        if (!neg) return erx + P / Q;
-        return -erx - P / Q;
+        const float pq = mgx_ref_fdiv(P, Q);
+        if (!neg) return erx + pq;
+        return -erx - pq;
     }
     if (ix >= 0x40c00000u) {         /* |x| >= 6 */
         if (!neg) return one - tiny;
         return tiny - one;
     }
     x = fabsf(x);
-    s = one / (x * x);
+    s = mgx_ref_fdiv(one, x * x);
     if (ix < 0x4036db6eu) {          /* |x| < 1/0.35 */
         R = ra0 + s * (ra1 + s * (ra2 + s * (ra3 + s * (ra4 + s * (ra5 + s * (ra6 + s * ra7))))));
         S = one + s * (sa1 + s * (sa2 + s * (sa3 + s * (sa4 + s * (sa5 + s * (sa6 + s * (sa7 + s * sa8)))))));
@@ -276,9 +323,10 @@ MGX_FM_HD float mgx_ref_erff(float x)
         S = one + s * (sb1 + s * (sb2 + s * (sb3 + s * (sb4 + s * (sb5 + s * (sb6 + s * sb7))))));
     }
     z = mgx_rf_asfloat(mgx_rf_asuint(x) & 0xfffff000u);
-    r = mgx_ref_expf_impl(-z * z - 0.5625f, 0) * mgx_ref_expf_impl((z - x) * (z + x) + R / S, 0);
-    if (!neg) return one - r / x;
-    return r / x - one;
+    r = mgx_ref_expf_impl(-z * z - 0.5625f, 0) * mgx_ref_expf_impl((z - x) * (z + x) + mgx_ref_fdiv(R, S), 0);
+    const float rx = mgx_ref_fdiv(r, x);
+    if (!neg) return one - rx;
+    return rx - one;
 }
 
 /* {1/c, log(c), log2(c)} for the 16 subintervals of [0x1.66p-1, 0x1.66p0). */
@@ -302,6 +350,19 @@ MGX_FM_CONST double MGX_REF_LOG_TAB[16][3] = {
 
 #define MGX_REF_LN2 0x1.62e42fefa39efp-1
 
+#if defined(__HIPCC__)
+/* copies the tables into LDS (device code; see MGX_REF_EXP2_AT) */
+__device__ static inline void mgx_ref_tables_init()
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    for (uint32_t k = threadIdx.x; k < 32; k += blockDim.x) mgx_ref_lds_exp2[k] = MGX_REF_EXP2_TAB[k];
+    for (uint32_t k = threadIdx.x; k < 48; k += blockDim.x)
+        mgx_ref_lds_log[k / 3][k % 3] = MGX_REF_LOG_TAB[k / 3][k % 3];
+    __syncthreads();
+#endif
+}
+#endif
+
 /* glibc logf (the FMA build) */
 MGX_FM_HD float mgx_ref_logf(float x)
 {
@@ -319,7 +380,7 @@ MGX_FM_HD float mgx_ref_logf(float x)
     const int i = (int)((tmp >> 19) % 16);
     const int k = (int32_t)tmp >> 23;
     const uint32_t iz = ix - (tmp & 0xff800000u);
-    const double invc = MGX_REF_LOG_TAB[i][0], logc = MGX_REF_LOG_TAB[i][1];
+    const double invc = MGX_REF_LOG_AT(i, 0), logc = MGX_REF_LOG_AT(i, 1);
     const double z = (double)mgx_rf_asfloat(iz);
     const double r = MGX_FMA(z, invc, -1.0);
     const double y0 = MGX_FMA((double)k, MGX_REF_LN2, logc);
@@ -339,7 +400,7 @@ MGX_FM_HD double mgx_ref_powf_log2(uint32_t ix)
     const uint32_t top = tmp & 0xff800000u;
     const uint32_t iz = ix - top;
     const int k = (int32_t)top >> 23;
-    const double invc = MGX_REF_LOG_TAB[i][0], logc = MGX_REF_LOG_TAB[i][2];
+    const double invc = MGX_REF_LOG_AT(i, 0), logc = MGX_REF_LOG_AT(i, 2);
     const double z = (double)mgx_rf_asfloat(iz);
     const double r = MGX_FMA(z, invc, -1.0);
     const double y0 = logc + (double)k;
@@ -362,7 +423,7 @@ MGX_FM_HD float mgx_ref_powf_exp2(double xd, uint64_t sign_bias)
     const uint64_t ki = mgx_fm_bits(kd);
     kd -= shift;
     const double r = xd - kd;
-    uint64_t t = MGX_REF_EXP2_TAB[ki % 32];
+    uint64_t t = MGX_REF_EXP2_AT(ki % 32);
     t += (ki + sign_bias) << 47;
     const double s = mgx_fm_from_bits(t);
     const double z = MGX_FMA(0x1.c6af84b912394p-5, r, 0x1.ebfce50fac4f3p-3);
@@ -430,4 +491,34 @@ MGX_FM_HD float mgx_ref_powf(float x, float y)
             return (sign_bias ? -0x1p-95f : 0x1p-95f) * 0x1p-95f;
     }
     return mgx_ref_powf_exp2(ylogx, sign_bias);
+}
+
+/* The restated functions by number, for the exhaustive comparisons (powf takes
+ * the exponent y; NaN results are canonicalised so that payloads do not count). */
+enum { MGX_FN_EXPF = 0, MGX_FN_EXPM1F = 1, MGX_FN_ERFF = 2, MGX_FN_LOGF = 3, MGX_FN_POWF = 4 };
+
+MGX_FM_HD uint32_t mgx_ref_eval_bits(int fn, uint32_t xbits, float y)
+{
+    const float x = mgx_rf_asfloat(xbits);
+    float r;
+    switch (fn) {
+    case MGX_FN_EXPF: r = mgx_ref_expf(x); break;
+    case MGX_FN_EXPM1F: r = mgx_ref_expm1f(x); break;
+    case MGX_FN_ERFF: r = mgx_ref_erff(x); break;
+    case MGX_FN_LOGF: r = mgx_ref_logf(x); break;
+    default: r = mgx_ref_powf(x, y); break;
+    }
+    return (r != r) ? 0x7fc00000u : mgx_rf_asuint(r);
+}
+
+/* Order-independent digest of one (argument, result) pair: the 64-bit
+ * finaliser of SplitMix64 applied to (argument bits << 32 | result bits); a
+ * chunk's digest is the wrapping sum over its arguments. */
+MGX_FM_HD uint64_t mgx_fm_digest(uint32_t xbits, uint32_t rbits)
+{
+    uint64_t z = ((uint64_t)xbits << 32) | rbits;
+    z += 0x9e3779b97f4a7c15ULL;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
 }

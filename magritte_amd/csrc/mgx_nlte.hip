@@ -300,6 +300,7 @@ __global__ __launch_bounds__(NL_WAVE) void stateq_kernel(NView v, const double *
                                                          double *Jlin_out, double *Jeff_out,
                                                          double *part, GlobalIn g)
 {
+    mgx_ref_tables_init();
     extern __shared__ double sm[];
     const uint32_t p = blockIdx.x, s = blockIdx.y, lane = threadIdx.x;
     const NSpec S = v.spec[s];

@@ -759,11 +759,15 @@ __host__ __device__ inline size_t solve_lds_bytes(uint32_t cap, uint32_t F, uint
    each of 4 SIMDs) leaves it; beyond that (many lines, F ~ 10^3) LDS would lower
    the occupancy, and J is accumulated in the origin's row of the output instead
    (only that origin's wave writes it) */
+/* room for the static LDS of a kernel: the float functions' tables
+   (MGX_REF_LDS_TABLE_BYTES) and a few words */
+constexpr size_t kStaticLds = 2048;
+static_assert(MGX_REF_LDS_TABLE_BYTES + 64 <= kStaticLds, "static LDS estimate");
+
 inline bool j_in_lds(uint32_t cap, uint32_t F, uint32_t L)
 {
     const size_t share = (160 * 1024) / (4 * MGX_SOLVE_WAVES_PER_SIMD);
-    const size_t stat = MGX_ERF_NINT * (MGX_ERF_DEG + 1) * sizeof(double);
-    return solve_lds_bytes(cap, F, L, true) + stat <= share;
+    return solve_lds_bytes(cap, F, L, true) + kStaticLds <= share;
 }
 
 /* J(o, f) += c for one ray pair: the first ray pair of a full pass stores (the sum
@@ -1374,6 +1378,7 @@ template <int APPROX, int PREC>
 __global__ __launch_bounds__(MGX_WAVE, MGX_SOLVE_WAVES_PER_SIMD) void feautrier_kernel(
     const KParams *__restrict__ kpp, double *u, double *J, double *lam)
 {
+    mgx_ref_tables_init();
     const KParams &kp = *kpp;
     extern __shared__ unsigned char smem[];
     const double *s_erf = nullptr; /* erf needs no table */
@@ -1406,6 +1411,7 @@ template <int APPROX, int PREC>
 __global__ __launch_bounds__(MGX_WAVE, 2) void feautrier_bands_kernel(
     const KParams *__restrict__ kpp, double *u, double *J, LamBands lb)
 {
+    mgx_ref_tables_init();
     const KParams &kp = *kpp;
     extern __shared__ unsigned char smem[];
     const double *s_erf = nullptr; /* erf needs no table */
@@ -1437,6 +1443,7 @@ template <int APPROX, int PREC>
 __global__ __launch_bounds__(MGX_WAVE, MGX_SOLVE_WAVES_PER_SIMD) void shortchar_kernel(
     const KParams *__restrict__ kpp, ShortcharOut sc, double *u, double *J, double *lam)
 {
+    mgx_ref_tables_init();
     const KParams &kp = *kpp;
     extern __shared__ unsigned char smem[];
     const double *s_erf = nullptr; /* erf needs no table */
@@ -1469,6 +1476,7 @@ template <int APPROX, int PREC>
 __global__ __launch_bounds__(MGX_WAVE, MGX_SOLVE_WAVES_PER_SIMD) void feautrier_uv_kernel(
     const KParams *__restrict__ kpp, double *u, double *v)
 {
+    mgx_ref_tables_init();
     const KParams &kp = *kpp;
     extern __shared__ unsigned char smem[];
     const double *s_erf = nullptr; /* erf needs no table */
@@ -1501,6 +1509,7 @@ __global__ __launch_bounds__(MGX_WAVE, WPS) void feautrier_batch_kernel(
     const KParams *__restrict__ kps, const BatchItem *__restrict__ items, uint32_t n_items,
     uint32_t *counter, double *const *__restrict__ J, double *const *__restrict__ lam)
 {
+    mgx_ref_tables_init();
     extern __shared__ unsigned char smem[];
     const double *s_erf = nullptr; /* erf needs no table */
     __shared__ uint32_t s_item;
@@ -1560,6 +1569,7 @@ __global__ void fill_f64_kernel(double *p, size_t n)
 /* one wave per (rr, o): n_tot for every line */
 __global__ __launch_bounds__(MGX_WAVE) void trace_counts_kernel(const KParams *__restrict__ kpp, uint32_t *n_tot)
 {
+    mgx_ref_tables_init();
     const KParams &kp = *kpp;
     extern __shared__ unsigned char smem[];
     const mgx_model &m = kp.m;
@@ -1596,6 +1606,7 @@ __global__ __launch_bounds__(MGX_WAVE) void trace_counts_kernel(const KParams *_
 /* one wave per (rr, o): walk and trace statistics (S, neighbours, T, T_interp) */
 __global__ __launch_bounds__(MGX_WAVE) void trace_stats_kernel(const KParams *__restrict__ kpp, unsigned long long *stats)
 {
+    mgx_ref_tables_init();
     const KParams &kp = *kpp;
     extern __shared__ unsigned char smem[];
     const mgx_model &m = kp.m;
@@ -1658,6 +1669,7 @@ __global__ __launch_bounds__(MGX_WAVE) void trace_rays_kernel(
     uint32_t capacity, uint32_t *count, uint32_t *centre_idx, uint32_t *nr, uint32_t *nr_interp,
     double *shift, double *interp_factor, double *dZ)
 {
+    mgx_ref_tables_init();
     const KParams &kp = *kpp;
     extern __shared__ unsigned char smem[];
     const mgx_model &m = kp.m;
@@ -1778,6 +1790,7 @@ __device__ inline bool lex_less(double a, uint32_t ka, double b, uint32_t kb)
    the closest boundary point along the ray direction */
 __global__ __launch_bounds__(256) void image_frame_kernel(mgx_model m, ImageFrame *out)
 {
+    mgx_ref_tables_init();
     __shared__ double s_min_x[256], s_max_x[256], s_min_y[256], s_max_y[256], s_pv[256];
     __shared__ uint32_t s_pk[256], s_pp[256];
     ImageFrame fr = *out; /* rd, nx, ny set by the host */
@@ -1978,6 +1991,7 @@ template <int APPROX, int PREC, int KIND>
 __global__ __launch_bounds__(MGX_WAVE) void image_kernel(const KParams *__restrict__ kpp, const ImageFrame *frp,
                                                          double *I, double *ImX, double *ImY)
 {
+    mgx_ref_tables_init();
     const KParams &kp = *kpp;
     extern __shared__ unsigned char smem[];
     const double *s_erf = nullptr; /* erf needs no table */
@@ -2152,6 +2166,7 @@ __global__ __launch_bounds__(MGX_WAVE) void image_ray_kernel(const KParams *__re
                                                              uint32_t rr, double *I, double *ImX,
                                                              double *ImY)
 {
+    mgx_ref_tables_init();
     const KParams &kp = *kpp;
     extern __shared__ unsigned char smem[];
     const double *s_erf = nullptr; /* erf needs no table */
@@ -3209,6 +3224,7 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_ADJ_WAVES_PER_SIMD) void feautrier_ad
     const KParams *__restrict__ kpp, const double *__restrict__ J_bar, double *g_eps,
     double *g_kap, double *g_iw, double *scratch, uint32_t capE)
 {
+    mgx_ref_tables_init();
     const KParams &kp = *kpp;
     extern __shared__ unsigned char smem[];
     const double *s_erf = nullptr; /* erf needs no table */
@@ -3329,6 +3345,7 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_ADJ_WAVES_PER_SIMD) void image_adjoin
     const KParams *__restrict__ kpp, const ImageFrame *frp, const double *__restrict__ I_bar,
     double *g_eps, double *g_kap, double *g_iw, double *scratch, uint32_t capE)
 {
+    mgx_ref_tables_init();
     const KParams &kp = *kpp;
     extern __shared__ unsigned char smem[];
     const double *s_erf = nullptr; /* erf needs no table */
@@ -3680,8 +3697,8 @@ int read_status(hipStream_t st, const DevBuffers &b, uint32_t *need, uint32_t *n
     return MGX_OK;
 }
 
-/* LDS per workgroup available to dynamic allocations (64 KiB minus the erf table) */
-const size_t kLdsBudget = 65536 - MGX_ERF_NINT * (MGX_ERF_DEG + 1) * sizeof(double);
+/* LDS per workgroup available to dynamic allocations: 64 KiB less the static LDS */
+const size_t kLdsBudget = 65536 - kStaticLds;
 
 template <typename Launch>
 int launch_with_capacity(hipStream_t st, DevBuffers &b, KParams &kp, Launch launch,

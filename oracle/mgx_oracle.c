@@ -2627,6 +2627,46 @@ int oracle_image_adjoint(const mgx_model *m, const mgx_options *op, const double
 }
 
 /* exported transcendentals, for the tests that pin them */
+/* per-chunk digests of (argument, result) over float arguments [first, first +
+   nchunks 2^chunk_log2), with this build's float functions (mgx_fmath_ref.h's
+   mgx_fm_digest; NaN results canonicalised): the host side of the device
+   comparison mgx_fmath_digest */
+static uint32_t o_eval_bits(int fn, uint32_t xb, float y)
+{
+    float x, r;
+    memcpy(&x, &xb, 4);
+    switch (fn) {
+    case MGX_FN_EXPF: r = O_EXPF(x); break;
+    case MGX_FN_EXPM1F: r = O_EXPM1F(x); break;
+    case MGX_FN_ERFF: r = O_ERFF(x); break;
+    case MGX_FN_LOGF: r = O_LOGF(x); break;
+    default: r = O_POWF(x, y); break;
+    }
+    if (r != r) return 0x7fc00000u;
+    uint32_t rb;
+    memcpy(&rb, &r, 4);
+    return rb;
+}
+
+int oracle_fmath_digest(int fn, float y, uint64_t first, uint32_t nchunks, uint32_t chunk_log2,
+                        uint64_t *digest, int nthreads)
+{
+    if (fn < MGX_FN_EXPF || fn > MGX_FN_POWF || !digest || chunk_log2 > 32) return MGX_E_ARG;
+    if (first + ((uint64_t)nchunks << chunk_log2) > (1ull << 32)) return MGX_E_ARG;
+    const int nt = default_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
+    for (int64_t c = 0; c < (int64_t)nchunks; c++) {
+        const uint64_t base = first + ((uint64_t)c << chunk_log2);
+        uint64_t acc = 0;
+        for (uint64_t i = 0; i < (1ull << chunk_log2); i++) {
+            const uint32_t xb = (uint32_t)(base + i);
+            acc += mgx_fm_digest(xb, o_eval_bits(fn, xb, y));
+        }
+        digest[c] = acc;
+    }
+    return 0;
+}
+
 float oracle_expf(float x) { return O_EXPF(x); }
 float oracle_expm1f(float x) { return O_EXPM1F(x); }
 float oracle_logf(float x) { return O_LOGF(x); }

@@ -119,6 +119,8 @@ int oracle_nlte_stateq_global_system(const oracle_nlte *h, uint32_t cap,
 /* long-double column-pivoted QR solve of an n x n column-major system */
 int oracle_cpqr_solve(int n, const double *a_colmajor, const double *b, double *x);
 
+int oracle_fmath_digest(int fn, float y, uint64_t first, uint32_t nchunks, uint32_t chunk_log2,
+                        uint64_t *digest, int nthreads);
 float oracle_expf(float x);
 float oracle_expm1f(float x);
 float oracle_logf(float x);

@@ -566,3 +566,19 @@ def compute_level_populations_shortchar(model, opts, params, use_ng=True, max_it
             if conv[l, 0] > 1.0 - params.convergence_fraction:
                 some_not_converged = True
     return nlte, iteration, err_mean, err_max
+
+
+FN_EXPF, FN_EXPM1F, FN_ERFF, FN_LOGF, FN_POWF = range(5)
+
+
+def fmath_digest(fn, first, nchunks, chunk_log2, y=0.0, nthreads=0, kind=DEFAULT_KIND):
+    """Per-chunk digests of (argument, result) of one float function of this build
+    over the float arguments [first, first + nchunks 2^chunk_log2): the host side
+    of the device check mgx_fmath_digest."""
+    h = lib(kind)
+    h.oracle_fmath_digest.argtypes = [C.c_int, C.c_float, C.c_uint64, C.c_uint32, C.c_uint32,
+                                      C.c_void_p, C.c_int]
+    out = np.zeros(nchunks, dtype=np.uint64)
+    _check(h.oracle_fmath_digest(fn, y, first, nchunks, chunk_log2, out.ctypes.data, nthreads),
+           "fmath_digest")
+    return out

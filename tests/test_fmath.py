@@ -149,3 +149,16 @@ def test_c_library_float_functions_are_not_correctly_rounded(fn):
     assert max(d) <= 1
     if fn in ("erff", "expm1f"):
         assert sum(d) > 0
+
+
+def test_digests_of_the_restated_and_library_builds_agree():
+    """The digests the GPU comparison uses: equal for the restated and the C
+    library's erff and powf over 2^22 arguments around 1, different for the
+    correctly rounded erff (so a one-ulp difference changes a digest)."""
+    first, n, cl = 0x3f000000, 64, 16
+    for fn, y in ((oracle.FN_ERFF, 0.0), (oracle.FN_POWF, 0.375)):
+        g = oracle.fmath_digest(fn, first, n, cl, y=y, kind="glibc")
+        r = oracle.fmath_digest(fn, first, n, cl, y=y, kind="ref")
+        np.testing.assert_array_equal(g, r)
+    c = oracle.fmath_digest(oracle.FN_ERFF, first, n, cl, kind="cr")
+    assert (c != oracle.fmath_digest(oracle.FN_ERFF, first, n, cl, kind="glibc")).all()
