@@ -66,6 +66,21 @@ def linedata_linear_rotor(B, dipole_debye, nlev, mass=28.0, num=1):
     return _linedata(num, mass, energy, weight, irad, jrad, A)
 
 
+def linedata_blended(freq, separations, A=1.0e-4, mass=28.0, num=1):
+    """A ladder species whose lines overlap: level k+1 lies h*nu_k above level k,
+    with nu_0 = freq and nu_k = freq * (1 + separations[k-1]) (relative
+    separations, e.g. a few Doppler widths); transitions k+1 -> k with Einstein
+    A (scalar or one per line) and weights 2k+1.  Lines in ascending frequency."""
+    nus = [freq] + [freq * (1.0 + s) for s in separations]
+    energy = np.concatenate([[0.0], np.cumsum([HH * nu for nu in nus])])
+    nlev = len(energy)
+    weight = 2.0 * np.arange(nlev, dtype=np.float64) + 1.0
+    irad = list(range(1, nlev))
+    jrad = list(range(0, nlev - 1))
+    Av = [A] * len(irad) if np.isscalar(A) else list(A)
+    return _linedata(num, mass, energy, weight, irad, jrad, Av)
+
+
 def _linedata(num, mass, energy, weight, irad, jrad, A):
     irad = np.asarray(irad, dtype=np.uint32)
     jrad = np.asarray(jrad, dtype=np.uint32)
@@ -177,6 +192,37 @@ def _rotation_matrix(n):
     return np.array([n1, n2, n3]).T
 
 
+def random_rotations(n, seed):
+    """n seeded rotation matrices, uniform over SO(3) (unit quaternions)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    q = rng.standard_normal((n, 4))
+    q /= np.linalg.norm(q, axis=1)[:, None]
+    w, x, y, z = q.T
+    R = np.empty((n, 3, 3))
+    R[:, 0, 0] = 1 - 2 * (y * y + z * z)
+    R[:, 0, 1] = 2 * (x * y - z * w)
+    R[:, 0, 2] = 2 * (x * z + y * w)
+    R[:, 1, 0] = 2 * (x * y + z * w)
+    R[:, 1, 1] = 1 - 2 * (x * x + z * z)
+    R[:, 1, 2] = 2 * (y * z - x * w)
+    R[:, 2, 0] = 2 * (x * z - y * w)
+    R[:, 2, 1] = 2 * (y * z + x * w)
+    R[:, 2, 2] = 1 - 2 * (x * x + y * y)
+    return R
+
+
+def rotated_directions(rotations, directions):
+    """directions (nrays, 3) rotated by each of rotations (n, 3, 3): (n, nrays, 3),
+    with -d mapped to exactly -(R d), so antipodes stay antipodes."""
+    d = np.asarray(directions, dtype=np.float64)
+    out = np.empty((len(rotations), len(d), 3))
+    for i in range(3):
+        out[:, :, i] = (rotations[:, i, 0][:, None] * d[None, :, 0]
+                        + rotations[:, i, 1][:, None] * d[None, :, 1]
+                        + rotations[:, i, 2][:, None] * d[None, :, 2])
+    return out
+
+
 def uniform_rays(nrays, first_ray=(1.0, 0.0, 0.0)):
     """HEALPix directions rotated so ray 0 points along first_ray (setup.py:174-223)."""
     nside = int(round(np.sqrt(nrays / 12)))
@@ -251,6 +297,31 @@ class ModelData:
         lo = np.minimum(lo, self.position.min(axis=0))
         hi = np.maximum(hi, self.position.max(axis=0))
         self.center = (lo + hi) / 2
+
+    def set_adaptive_directions(self, directions, weights):
+        """Per-point ray directions (Rays::use_adaptive_directions, rays.cpp:14-17,
+        37-50): directions (P, nrays, 3) and weights (P, nrays), stored point-major
+        as the reference reads them ((P*nrays, 3) and (P*nrays)).  The antipod
+        table comes from the first point's directions (rays.cpp:52-61: it is
+        assumed the same for every point)."""
+        P = self.npoints
+        d = np.asarray(directions, dtype=np.float64)
+        w = np.asarray(weights, dtype=np.float64)
+        assert d.shape[0] == P and d.shape[2] == 3 and w.shape == d.shape[:2]
+        self.adaptive_directions = 1
+        self.ray_direction = np.ascontiguousarray(d.reshape(-1, 3))
+        self.ray_weight = np.ascontiguousarray(w.reshape(-1))
+        self.nrays = d.shape[1]
+        self.hnrays = self.nrays // 2
+        anti = np.zeros(self.nrays, dtype=np.uint32)
+        d0 = d[0]
+        for r1 in range(self.nrays):
+            for r2 in range(self.nrays):
+                t = d0[r1] + d0[r2]
+                if t[0] * t[0] + t[1] * t[1] + t[2] * t[2] < 1.0e-9:
+                    anti[r1] = r2
+        self.antipod = anti
+        return self
 
     # -- physics ----------------------------------------------------------
     def set_thermo(self, temperature, vturb, abundance):
