@@ -599,24 +599,6 @@ def run_nlte(args):
     print(json.dumps(res), flush=True)
 
 
-def _pair_view(m, dm, rr):
-    """The model's device view restricted to ray pair rr (its two directions and
-    weights, antipods 1 and 0): a formal solve of it gives that pair's part of J and
-    of the Lambda diagonal, which add up over the pairs."""
-    import numpy as np
-    import torch
-
-    d = m.ray_direction.reshape(-1, 3)
-    ar = int(m.antipod[rr])
-    t = dict(dm.tensors)
-    t["ray_direction"] = torch.from_numpy(np.ascontiguousarray([d[rr], d[ar]])).to(dm.device)
-    t["ray_weight"] = torch.from_numpy(np.array([m.ray_weight[rr], m.ray_weight[ar]])).to(dm.device)
-    t["antipod"] = torch.from_numpy(np.array([1, 0], dtype=np.uint32)).to(dm.device)
-    v = m.view(t)
-    v.nrays, v.hnrays = 2, 1
-    return v, t
-
-
 def run_nlte_full(args):
     """One iteration of the large non-LTE configuration at its full size (n_edge^3
     points, --nrays rays; 100^3 and 192 as configured): LTE populations, the formal
@@ -639,7 +621,7 @@ def run_nlte_full(args):
     Js = torch.zeros_like(ref["J"])
     Ls = torch.zeros_like(ref["lambda_diag"])
     for rr in range(ms.hnrays):
-        v, keep = _pair_view(ms, dms, rr)
+        v, keep = solver.ray_pair_view(dms, rr)
         o = solver.feautrier_forward(dms, ms.options(), with_u=False, view=v)
         Js += o["J"]
         Ls += o["lambda_diag"]
@@ -668,7 +650,7 @@ def run_nlte_full(args):
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     for rr in range(m.hnrays):
-        v, keep = _pair_view(m, dm, rr)
+        v, keep = solver.ray_pair_view(dm, rr)
         o = solver.feautrier_forward(dm, opts, with_u=False, view=v)
         J += o["J"]
         lam += o["lambda_diag"]

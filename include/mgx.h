@@ -136,6 +136,19 @@ int mgx_feautrier_forward_device(const mgx_model *model_dev, const mgx_options *
                                  double *u, double *J, double *lambda_diag, void *stream);
 
 /*
+ * The memory-sparse formal solve: J_line (P, L) receives, for every point o and
+ * line l (transition k of species s, l = nrad_cum[s] + k), the line-integrated
+ * mean intensity sum_z w_z J(o, l nquads + z) with the quadrature weights of
+ * species s -- lspec.J(o, k) -- and lambda_diag (P, L) [may be NULL] the Lambda
+ * diagonal; neither u nor J (P, F) is stored.  The view must carry the line
+ * frequency grid.  Replaces Solver::solve_feautrier_order_2_sparse
+ * (src/solver/solver.tpp:297-363) behind Model::compute_radiation_field_
+ * feautrier_order_2_sparse (src/model/model.cpp:474-515).
+ */
+int mgx_feautrier_sparse_device(const mgx_model *model_dev, const mgx_options *opts,
+                                double *J_line, double *lambda_diag, void *stream);
+
+/*
  * The plain-method solve of mgx_feautrier_forward_device (same kernels, same
  * results bit for bit) without any device allocation.  No counterpart in the
  * reference: it is for callers that cannot allocate -- code under HIP graph
@@ -360,12 +373,28 @@ int mgx_batch_create(const mgx_model *models, uint32_t n, const mgx_options *opt
 int mgx_batch_configure(mgx_batch *batch, uint32_t rays_per_item, uint32_t order, uint32_t wps,
                         int use_graph, void *stream);
 
-/* Autotuner: times the candidate tilings on every sample_stride-th work item
- * and keeps the fastest; best (3 words, may be NULL) receives it.  Leaves the
- * outputs undefined until the next mgx_batch_run. */
+/* Autotuner: times the candidate tilings of the solve (without the images) on
+ * every sample_stride-th work item and keeps the fastest; best (3 words, may be
+ * NULL) receives it.  Leaves the outputs undefined until the next mgx_batch_run. */
 int mgx_batch_tune(mgx_batch *batch, uint32_t sample_stride, uint32_t *best, void *stream);
 
-/* Runs the batch (line tables, the persistent kernel; synchronises). */
+/*
+ * Adds an image of every model to the batch (Model::compute_image_new,
+ * model.cpp:1008-1035, for each model; replaces any earlier images): model i is
+ * imaged on nx x ny pixels along raydirs[3i..3i+2] at the frequencies of
+ * image_views[i] -- the view models[i] with nu, f_* and nfreqs replaced by the
+ * image's frequency grid (every geometry and line array must be the same
+ * pointer as in models[i]) -- into I[i] (nx*ny, F_i), ImX[i] and ImY[i] (nx*ny).
+ * mgx_batch_run then launches one persistent kernel over the pixels of all the
+ * images after the solve, in the same graph.  Not with the one-line
+ * approximation (as the reference).
+ */
+int mgx_batch_set_images(mgx_batch *batch, const mgx_model *image_views, const double *raydirs,
+                         uint32_t nx, uint32_t ny, double *const *I, double *const *ImX,
+                         double *const *ImY, void *stream);
+
+/* Runs the batch (line tables, the persistent kernel, the images if set;
+ * synchronises). */
 int mgx_batch_run(mgx_batch *batch, void *stream);
 
 /* Duration (ms) of the last run when kernel timing is enabled. */

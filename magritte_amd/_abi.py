@@ -125,6 +125,8 @@ SIGNATURES = [
     ("mgx_version", C.c_char_p, []),
     ("mgx_feautrier_forward_device", C.c_int,
      [C.POINTER(MgxModel), C.POINTER(MgxOptions), _VP, _VP, _VP, _VP]),
+    ("mgx_feautrier_sparse_device", C.c_int,
+     [C.POINTER(MgxModel), C.POINTER(MgxOptions), _VP, _VP, _VP]),
     ("mgx_feautrier_forward_noalloc", C.c_int,
      [C.POINTER(MgxModel), C.POINTER(MgxOptions), _VP, _VP, _VP, _VP]),
     ("mgx_shortchar_order_0_device", C.c_int,
@@ -164,6 +166,8 @@ SIGNATURES = [
     ("mgx_batch_configure", C.c_int,
      [_VP, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, _VP]),
     ("mgx_batch_tune", C.c_int, [_VP, C.c_uint32, _VP, _VP]),
+    ("mgx_batch_set_images", C.c_int,
+     [_VP, C.POINTER(MgxModel), _VP, C.c_uint32, C.c_uint32, _VP, _VP, _VP, _VP]),
     ("mgx_batch_run", C.c_int, [_VP, _VP]),
     ("mgx_batch_last_ms", C.c_double, [_VP]),
     ("mgx_batch_destroy", None, [_VP]),

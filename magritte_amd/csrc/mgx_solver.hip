@@ -1374,9 +1374,13 @@ __device__ __forceinline__ int solve_origin(const KParams &kp, uint32_t o, uint3
     return 0;
 }
 
+/* One origin per wave.  With Jline set (the sparse solve) the origin's J stays in
+   LDS (or in its row of J, a scratch) and only its line integrals are written:
+   Jline(o, l) = sum_z w_z J(o, l Q + z) with the quadrature weights of line l's
+   species (lspec.J of Solver::solve_feautrier_order_2_sparse, solver.tpp:340-357). */
 template <int APPROX, int PREC>
 __global__ __launch_bounds__(MGX_WAVE, MGX_SOLVE_WAVES_PER_SIMD) void feautrier_kernel(
-    const KParams *__restrict__ kpp, double *u, double *J, double *lam)
+    const KParams *__restrict__ kpp, double *u, double *J, double *lam, double *Jline = nullptr)
 {
     mgx_ref_tables_init();
     const KParams &kp = *kpp;
@@ -1396,10 +1400,21 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_SOLVE_WAVES_PER_SIMD) void feautrier_
     double *s_J = kp.j_lds ? s_L + L : nullptr;
     for (uint32_t i = lane; i < L + (s_J ? F : 0); i += MGX_WAVE) s_L[i] = 0.0;
     if (solve_origin<APPROX, PREC>(kp, o, 0, m.hnrays, smem, s_erf, u, lam != nullptr, s_J,
-                                   J + (size_t)o * F, false, s_L))
+                                   s_J ? nullptr : J + (size_t)o * F, false, s_L))
         return;
-    if (s_J)
+    if (Jline) {
+        __syncthreads();
+        const double *Jo = s_J ? s_J : J + (size_t)o * F;
+        const uint32_t Q = m.nquads;
+        for (uint32_t l = lane; l < L; l += MGX_WAVE) {
+            const double *w = &m.quad_weights[(size_t)m.line_species[l] * Q];
+            double acc = 0.0;
+            for (uint32_t z = 0; z < Q; z++) acc += w[z] * Jo[(size_t)l * Q + z];
+            Jline[(size_t)o * L + l] = acc;
+        }
+    } else if (s_J) {
         for (uint32_t i = lane; i < F; i += MGX_WAVE) J[(size_t)o * F + i] = s_J[i];
+    }
     if (lam)
         for (uint32_t i = lane; i < L; i += MGX_WAVE) lam[(size_t)o * L + i] = s_L[i];
 }
@@ -1987,17 +2002,18 @@ __device__ inline uint32_t n_interp_all(const KParams &kp, uint32_t c, uint32_t 
     return (k > 1) ? k : 1;
 }
 
+/* One pixel of the projection-surface imager (the pixel loop bodies of
+   Solver::image_feautrier_order_2_new_imager and image_optical_depth_new_imager,
+   solver.tpp:590-620, 701-729): the wave walks the
+   pixel's ray into the LDS walk buffer smem (walk_lds_bytes(kp.cap) bytes) and its
+   lanes take the frequencies.  Returns false when the walk needs more than kp.cap
+   steps (the status words then hold ERR_CAP and the capacity needed). */
 template <int APPROX, int PREC, int KIND>
-__global__ __launch_bounds__(MGX_WAVE) void image_kernel(const KParams *__restrict__ kpp, const ImageFrame *frp,
-                                                         double *I, double *ImX, double *ImY)
+__device__ bool image_pixel(const KParams &kp, const ImageFrame &fr, uint32_t pix,
+                            unsigned char *smem, double *I, double *ImX, double *ImY)
 {
-    mgx_ref_tables_init();
-    const KParams &kp = *kpp;
-    extern __shared__ unsigned char smem[];
     const double *s_erf = nullptr; /* erf needs no table */
     const mgx_model &m = kp.m;
-    const ImageFrame fr = *frp;
-    const uint32_t pix = blockIdx.x;
     const uint32_t lane = threadIdx.x;
     const uint32_t cap = kp.cap;
     uint32_t *s_nxt = reinterpret_cast<uint32_t *>(smem);
@@ -2067,7 +2083,7 @@ __global__ __launch_bounds__(MGX_WAVE) void image_kernel(const KParams *__restri
                         atomicOr(&kp.status[0], (uint32_t)ERR_CAP);
                         atomicMax(&kp.status[1], cap + 1);
                     }
-                    return;
+                    return false;
                 }
                 if (lane == 0) {
                     s_nxt[S] = nxt;
@@ -2145,6 +2161,67 @@ __global__ __launch_bounds__(MGX_WAVE) void image_kernel(const KParams *__restri
             }
         }
         I[(size_t)pix * F + f] = Ival;
+    }
+    return true;
+}
+
+/* one wave per pixel of one model's image */
+template <int APPROX, int PREC, int KIND>
+__global__ __launch_bounds__(MGX_WAVE) void image_kernel(const KParams *__restrict__ kpp, const ImageFrame *frp,
+                                                         double *I, double *ImX, double *ImY)
+{
+    mgx_ref_tables_init();
+    extern __shared__ unsigned char smem[];
+    const ImageFrame fr = *frp;
+    (void)image_pixel<APPROX, PREC, KIND>(*kpp, fr, blockIdx.x, smem, I, ImX, ImY);
+}
+
+/*
+ * The images of a batch of models in one persistent launch: model i's pixels are
+ * cut into chunks of kImageChunk consecutive pixels, the chunks of all models form
+ * one work list (models by decreasing walk length, so the long rays start first),
+ * and a wave takes the next chunk from an atomic counter and images its pixels
+ * with image_pixel.  chunk_start[s] is the first chunk of the s-th model of that
+ * order, slot_model[s] the model (n_slots + 1 entries of chunk_start).
+ */
+constexpr uint32_t kImageChunk = 4;
+
+template <int APPROX, int PREC>
+__global__ __launch_bounds__(MGX_WAVE) void image_batch_kernel(
+    const KParams *__restrict__ kps, const ImageFrame *__restrict__ frames,
+    const uint32_t *__restrict__ chunk_start, const uint32_t *__restrict__ slot_model,
+    uint32_t n_slots, uint32_t *counter, double *const *__restrict__ I,
+    double *const *__restrict__ ImX, double *const *__restrict__ ImY)
+{
+    mgx_ref_tables_init();
+    extern __shared__ unsigned char smem[];
+    __shared__ uint32_t s_item;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t n_chunks = chunk_start[n_slots];
+    for (;;) {
+        __syncthreads();
+        if (lane == 0) s_item = atomicAdd(counter, 1u);
+        __syncthreads();
+        const uint32_t w = s_item;
+        if (w >= n_chunks) break;
+        /* the slot s with chunk_start[s] <= w < chunk_start[s + 1] */
+        uint32_t lo = 0, hi = n_slots;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (chunk_start[mid] <= w) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t mi = slot_model[lo];
+        const KParams &kp = kps[mi];
+        const ImageFrame fr = frames[mi];
+        const uint32_t npix = fr.nx * fr.ny;
+        const uint32_t p0 = (w - chunk_start[lo]) * kImageChunk;
+        const uint32_t p1 = (p0 + kImageChunk < npix) ? p0 + kImageChunk : npix;
+        for (uint32_t pix = p0; pix < p1; pix++) {
+            if (!image_pixel<APPROX, PREC, IMG_INTENSITY>(kp, fr, pix, smem, I[mi], ImX[mi], ImY[mi]))
+                return;
+            __syncthreads();
+        }
     }
 }
 
@@ -3910,10 +3987,32 @@ struct mgx_batch {
     hipGraphExec_t exec = nullptr;
     int use_graph = 1;
     double last_ms = 0.0;
+    /* images (mgx_batch_set_images): one per model when n_img == n */
+    uint32_t n_img = 0, n_chunks = 0;
+    bool images_on = true; /* off while the tuner times the solve alone */
+    std::vector<KParams> img_kps;
+    KParams *img_kps_dev = nullptr;
+    ImageFrame *frames_dev = nullptr;
+    uint32_t *chunk_start_dev = nullptr, *slot_model_dev = nullptr, *img_counter = nullptr;
+    double **I_dev = nullptr, **ImX_dev = nullptr, **ImY_dev = nullptr;
+    void free_images()
+    {
+        for (void *p : {(void *)img_kps_dev, (void *)frames_dev, (void *)chunk_start_dev,
+                        (void *)slot_model_dev, (void *)img_counter, (void *)I_dev, (void *)ImX_dev,
+                        (void *)ImY_dev})
+            if (p) (void)hipFree(p);
+        img_kps_dev = nullptr;
+        frames_dev = nullptr;
+        chunk_start_dev = slot_model_dev = img_counter = nullptr;
+        I_dev = ImX_dev = ImY_dev = nullptr;
+        img_kps.clear();
+        n_img = n_chunks = 0;
+    }
     ~mgx_batch()
     {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
+        free_images();
         for (auto *b : bufs) delete b;
         if (kps_dev) (void)hipFree(kps_dev);
         if (items_dev) (void)hipFree(items_dev);
@@ -4010,11 +4109,48 @@ uint32_t batch_grid_size(const mgx_batch &B, uint32_t n_items)
     return grid;
 }
 
+bool batch_images(const mgx_batch &B) { return B.n_img == B.n && B.images_on; }
+
+/* persistent grid of the image kernel (resident waves, at most the chunk count) */
+uint32_t image_grid_size(const mgx_batch &B)
+{
+    const size_t lds = walk_lds_bytes(B.cap);
+    const uint32_t work = B.n_chunks ? B.n_chunks : 1;
+    const bool fp64 = B.op.precision == MGX_PREC_FP64;
+    if (B.op.approx == MGX_APPROX_NONE)
+        return fp64 ? persistent_waves(image_batch_kernel<MGX_APPROX_NONE, MGX_PREC_FP64>, lds, work)
+                    : persistent_waves(image_batch_kernel<MGX_APPROX_NONE, MGX_PREC_MIXED>, lds, work);
+    return fp64 ? persistent_waves(image_batch_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_FP64>, lds, work)
+                : persistent_waves(image_batch_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_MIXED>, lds, work);
+}
+
+void launch_image_batch(const mgx_batch &B, hipStream_t st, uint32_t grid)
+{
+    const size_t lds = walk_lds_bytes(B.cap);
+    const bool fp64 = B.op.precision == MGX_PREC_FP64;
+    const dim3 g(grid), blk(MGX_WAVE);
+#define MGX_IMG_ARGS B.img_kps_dev, B.frames_dev, B.chunk_start_dev, B.slot_model_dev, B.n_img, \
+                     B.img_counter, B.I_dev, B.ImX_dev, B.ImY_dev
+    if (B.op.approx == MGX_APPROX_NONE) {
+        if (fp64)
+            hipLaunchKernelGGL((image_batch_kernel<MGX_APPROX_NONE, MGX_PREC_FP64>), g, blk, lds, st, MGX_IMG_ARGS);
+        else
+            hipLaunchKernelGGL((image_batch_kernel<MGX_APPROX_NONE, MGX_PREC_MIXED>), g, blk, lds, st, MGX_IMG_ARGS);
+    } else {
+        if (fp64)
+            hipLaunchKernelGGL((image_batch_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_FP64>), g, blk, lds, st, MGX_IMG_ARGS);
+        else
+            hipLaunchKernelGGL((image_batch_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_MIXED>), g, blk, lds, st, MGX_IMG_ARGS);
+    }
+#undef MGX_IMG_ARGS
+}
+
 /* the launches of one execution: line tables, output clearing (split items),
    counter and status reset, the persistent kernel over n_items items on `grid`
-   waves (no device queries: the sequence is also captured into a graph) */
+   waves, then (when images are set) the persistent image kernel on `igrid` waves
+   (no device queries: the sequence is also captured into a graph) */
 void batch_enqueue(const mgx_batch &B, hipStream_t st, uint32_t n_items, bool clear_outputs,
-                   uint32_t grid)
+                   uint32_t grid, uint32_t igrid)
 {
     for (uint32_t mi = 0; mi < B.n; mi++) {
         const mgx_model &m = B.views[mi];
@@ -4049,6 +4185,10 @@ void batch_enqueue(const mgx_batch &B, hipStream_t st, uint32_t n_items, bool cl
         fp64 ? launch_batch_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_FP64>(B, B.cfg.wps, g, lds, st, n_items)
              : launch_batch_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_MIXED>(B, B.cfg.wps, g, lds, st, n_items);
     }
+    if (batch_images(B)) {
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, st, B.img_counter, 1u, 0u);
+        launch_image_batch(B, st, igrid);
+    }
 }
 
 /* uploads the work list of the current configuration and the per-model
@@ -4071,6 +4211,11 @@ int batch_upload(mgx_batch &B, hipStream_t st)
     }
     HIP_TRY(hipMemcpyAsync(B.kps_dev, B.kps.data(), B.kps.size() * sizeof(KParams),
                            hipMemcpyHostToDevice, st));
+    if (B.n_img) {
+        for (auto &k : B.img_kps) k.cap = B.cap;
+        HIP_TRY(hipMemcpyAsync(B.img_kps_dev, B.img_kps.data(), B.img_kps.size() * sizeof(KParams),
+                               hipMemcpyHostToDevice, st));
+    }
     HIP_TRY(hipStreamSynchronize(st));
     return MGX_OK;
 }
@@ -4098,6 +4243,7 @@ int batch_execute(mgx_batch &B, hipStream_t st)
     for (;;) {
         const bool clear = batch_split(B);
         const uint32_t grid = batch_grid_size(B, B.n_items);
+        const uint32_t igrid = batch_images(B) ? image_grid_size(B) : 0;
         if (B.use_graph) {
             if (!B.exec) {
                 /* capture on a private stream (the caller's may be the null stream) */
@@ -4105,7 +4251,7 @@ int batch_execute(mgx_batch &B, hipStream_t st)
                 HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
                 hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
                 if (e == hipSuccess) {
-                    batch_enqueue(B, cs, B.n_items, clear, grid);
+                    batch_enqueue(B, cs, B.n_items, clear, grid, igrid);
                     e = hipStreamEndCapture(cs, &B.graph);
                 }
                 if (e == hipSuccess) e = hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0);
@@ -4117,7 +4263,7 @@ int batch_execute(mgx_batch &B, hipStream_t st)
             }
             timed(st, [&] { (void)hipGraphLaunch(B.exec, st); });
         } else {
-            timed(st, [&] { batch_enqueue(B, st, B.n_items, clear, grid); });
+            timed(st, [&] { batch_enqueue(B, st, B.n_items, clear, grid, igrid); });
         }
         HIP_TRY(hipGetLastError());
         B.last_ms = g_last_kernel_ms;
@@ -4215,7 +4361,8 @@ namespace {
 /* the plain-method solve on prepared buffers (shared by the allocating and the
    no-allocation entry points) */
 int forward_launch(const mgx_model *model, const mgx_options *opts, double *u, double *J,
-                   double *lambda_diag, hipStream_t st, DevBuffers &b, KParams &kp)
+                   double *lambda_diag, hipStream_t st, DevBuffers &b, KParams &kp,
+                   double *Jline = nullptr)
 {
     return launch_with_capacity(st, b, kp, [&](size_t) {
         const dim3 grid(model->npoints), block(MGX_WAVE);
@@ -4226,26 +4373,26 @@ int forward_launch(const mgx_model *model, const mgx_options *opts, double *u, d
             case MGX_APPROX_NONE:
                 if (fp64)
                     hipLaunchKernelGGL((feautrier_kernel<MGX_APPROX_NONE, MGX_PREC_FP64>), grid,
-                                       block, lds, st, b.kp_dev, u, J, lambda_diag);
+                                       block, lds, st, b.kp_dev, u, J, lambda_diag, Jline);
                 else
                     hipLaunchKernelGGL((feautrier_kernel<MGX_APPROX_NONE, MGX_PREC_MIXED>), grid,
-                                       block, lds, st, b.kp_dev, u, J, lambda_diag);
+                                       block, lds, st, b.kp_dev, u, J, lambda_diag, Jline);
                 break;
             case MGX_APPROX_ONE_LINE:
                 if (fp64)
                     hipLaunchKernelGGL((feautrier_kernel<MGX_APPROX_ONE_LINE, MGX_PREC_FP64>), grid,
-                                       block, lds, st, b.kp_dev, u, J, lambda_diag);
+                                       block, lds, st, b.kp_dev, u, J, lambda_diag, Jline);
                 else
                     hipLaunchKernelGGL((feautrier_kernel<MGX_APPROX_ONE_LINE, MGX_PREC_MIXED>),
-                                       grid, block, lds, st, b.kp_dev, u, J, lambda_diag);
+                                       grid, block, lds, st, b.kp_dev, u, J, lambda_diag, Jline);
                 break;
             default:
                 if (fp64)
                     hipLaunchKernelGGL((feautrier_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_FP64>),
-                                       grid, block, lds, st, b.kp_dev, u, J, lambda_diag);
+                                       grid, block, lds, st, b.kp_dev, u, J, lambda_diag, Jline);
                 else
                     hipLaunchKernelGGL((feautrier_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_MIXED>),
-                                       grid, block, lds, st, b.kp_dev, u, J, lambda_diag);
+                                       grid, block, lds, st, b.kp_dev, u, J, lambda_diag, Jline);
             }
         });
     }, 8 * (size_t)model->nlines);
@@ -4265,6 +4412,43 @@ int mgx_feautrier_forward_device(const mgx_model *model, const mgx_options *opts
     KParams kp;
     if (int e = prepare(model, opts, st, b, kp)) return e;
     return forward_launch(model, opts, u, J, lambda_diag, st, b, kp);
+}
+
+int mgx_feautrier_sparse_device(const mgx_model *model, const mgx_options *opts, double *J_line,
+                                double *lambda_diag, void *stream)
+{
+    mgx::clear_error();
+    if (int e = check_view(model, opts)) return e;
+    if (int e = check_lambda_diagonal(opts)) return e;
+    if (!J_line) return mgx::fail(MGX_E_ARG, "J_line is required");
+    if (!model->line_species) return mgx::fail(MGX_E_ARG, "null array in model view");
+    hipStream_t st = (hipStream_t)stream;
+    /* the line sums read bin z of line l at l nquads + z: the line grid's order */
+    {
+        std::vector<uint32_t> fl(model->nfreqs), fz(model->nfreqs);
+        if (hipMemcpyAsync(fl.data(), model->f_line, fl.size() * 4, hipMemcpyDeviceToHost, st) !=
+                hipSuccess ||
+            hipMemcpyAsync(fz.data(), model->f_z, fz.size() * 4, hipMemcpyDeviceToHost, st) !=
+                hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            return mgx::fail(MGX_E_HIP, "reading the frequency map failed");
+        for (uint32_t f = 0; f < model->nfreqs; f++)
+            if (fl[f] != f / model->nquads || fz[f] != f % model->nquads)
+                return mgx::fail(MGX_E_UNSUPPORTED,
+                                 "the sparse solve needs the line frequency grid (bin z of line l at l*nquads+z)");
+    }
+    DevBuffers b;
+    KParams kp;
+    if (int e = prepare(model, opts, st, b, kp)) return e;
+    /* J of an origin stays in LDS; when it does not fit, its row of a scratch J */
+    double *Jtmp = nullptr;
+    struct Free {
+        double *&p;
+        ~Free() { if (p) (void)hipFree(p); }
+    } free_tmp{Jtmp};
+    if (!j_in_lds(kMaxCap, model->nfreqs, model->nlines))
+        HIP_TRY(hipMalloc(&Jtmp, (size_t)model->npoints * model->nfreqs * sizeof(double)));
+    return forward_launch(model, opts, nullptr, Jtmp, lambda_diag, st, b, kp, J_line);
 }
 
 static size_t noalloc_scratch_bytes(const mgx_model *model)
@@ -4807,6 +4991,108 @@ int mgx_batch_run(mgx_batch *B, void *stream)
     return batch_execute(*B, (hipStream_t)stream);
 }
 
+int mgx_batch_set_images(mgx_batch *B, const mgx_model *image_views, const double *raydirs,
+                         uint32_t nx, uint32_t ny, double *const *I, double *const *ImX,
+                         double *const *ImY, void *stream)
+{
+    mgx::clear_error();
+    if (!B || !image_views || !raydirs || !I || !ImX || !ImY) return mgx::fail(MGX_E_ARG, "null argument");
+    if (nx == 0 || ny == 0) return mgx::fail(MGX_E_ARG, "the image needs at least one pixel");
+    if (B->op.approx == MGX_APPROX_ONE_LINE)
+        return mgx::fail(MGX_E_UNSUPPORTED, "One line approximation is not supported for imaging.");
+    const uint64_t npix = (uint64_t)nx * ny;
+    const uint64_t chunks_per_model = (npix + kImageChunk - 1) / kImageChunk;
+    if (npix > 0xffffffffull || chunks_per_model * B->n > 0xffffffffull)
+        return mgx::fail(MGX_E_ARG, "too many pixels in the batch's images");
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<ImageFrame> frames(B->n);
+    for (uint32_t i = 0; i < B->n; i++) {
+        const mgx_model &v = image_views[i], &m = B->views[i];
+        if (!I[i] || !ImX[i] || !ImY[i]) return mgx::fail(MGX_E_ARG, "image outputs of model %u are null", i);
+        if (v.npoints != m.npoints || v.nlines != m.nlines || v.nboundary != m.nboundary ||
+            v.dimension != m.dimension || v.spherical_symmetry != m.spherical_symmetry ||
+            v.max_inverse_mass != m.max_inverse_mass)
+            return mgx::fail(MGX_E_ARG, "image view %u does not describe batch model %u", i, i);
+        /* the image shares the solve's packed point and line tables */
+        if (v.position != m.position || v.velocity != m.velocity || v.n_neighbors != m.n_neighbors ||
+            v.cum_n_neighbors != m.cum_n_neighbors || v.neighbors != m.neighbors ||
+            v.point2boundary != m.point2boundary || v.boundary2point != m.boundary2point ||
+            v.emissivity != m.emissivity || v.opacity != m.opacity ||
+            v.inverse_width != m.inverse_width || v.dshift_max != m.dshift_max ||
+            v.temperature != m.temperature || v.vturb2 != m.vturb2 || v.interp_crit != m.interp_crit)
+            return mgx::fail(MGX_E_ARG, "image view %u must share the geometry and line arrays of batch model %u", i, i);
+        if (v.nfreqs == 0 || !v.nu || !v.f_line || !v.sorted_line || !v.sorted_line_map ||
+            !v.boundary_condition || !v.boundary_temperature)
+            return mgx::fail(MGX_E_ARG, "image view %u has no frequencies or boundary data", i);
+        const double *rd = &raydirs[3 * (size_t)i];
+        const double sq = rd[0] * rd[0] + rd[1] * rd[1] + rd[2] * rd[2];
+        if (sq == 0.0)
+            return mgx::fail(MGX_E_ARG, "The given ray direction vector does not point in a "
+                                        "direction. Please use a non-zero (normed) direction vector "
+                                        "to generate an image.");
+        ImageFrame fr;
+        std::memset(&fr, 0, sizeof fr);
+        const double inv = 1 / std::sqrt(sq);
+        for (int k = 0; k < 3; k++) fr.rd[k] = rd[k] * inv;
+        if (v.dimension == 1 && (fr.rd[0] != 0.0 || fr.rd[1] != 1.0 || fr.rd[2] != 0.0))
+            return mgx::fail(MGX_E_ARG, "In 1D, the image ray has to be (0,1,0)");
+        fr.nx = nx;
+        fr.ny = ny;
+        frames[i] = fr;
+    }
+    batch_drop_graph(*B);
+    B->free_images();
+    const uint32_t n = B->n;
+    if (hipMalloc(&B->img_kps_dev, n * sizeof(KParams)) != hipSuccess ||
+        hipMalloc(&B->frames_dev, n * sizeof(ImageFrame)) != hipSuccess ||
+        hipMalloc(&B->chunk_start_dev, (n + 1) * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&B->slot_model_dev, n * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&B->img_counter, sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&B->I_dev, n * sizeof(double *)) != hipSuccess ||
+        hipMalloc(&B->ImX_dev, n * sizeof(double *)) != hipSuccess ||
+        hipMalloc(&B->ImY_dev, n * sizeof(double *)) != hipSuccess) {
+        B->free_images();
+        return mgx::fail(MGX_E_HIP, "batch image allocation failed");
+    }
+    /* models by decreasing walk length (the cube root of the point count) */
+    std::vector<uint32_t> order(n);
+    for (uint32_t i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        return B->views[a].npoints > B->views[b].npoints;
+    });
+    std::vector<uint32_t> start(n + 1);
+    for (uint32_t s_ = 0; s_ <= n; s_++) start[s_] = (uint32_t)(chunks_per_model * s_);
+    B->img_kps.resize(n);
+    for (uint32_t i = 0; i < n; i++) {
+        B->img_kps[i] = B->kps[i];
+        B->img_kps[i].m = image_views[i];
+        B->img_kps[i].cap = B->cap;
+    }
+    auto up = [&](void *dst, const void *src, size_t bytes) {
+        return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st);
+    };
+    hipError_t e = up(B->frames_dev, frames.data(), n * sizeof(ImageFrame));
+    if (e == hipSuccess) e = up(B->chunk_start_dev, start.data(), (n + 1) * sizeof(uint32_t));
+    if (e == hipSuccess) e = up(B->slot_model_dev, order.data(), n * sizeof(uint32_t));
+    if (e == hipSuccess) e = up(B->img_kps_dev, B->img_kps.data(), n * sizeof(KParams));
+    if (e == hipSuccess) e = up(B->I_dev, I, n * sizeof(double *));
+    if (e == hipSuccess) e = up(B->ImX_dev, ImX, n * sizeof(double *));
+    if (e == hipSuccess) e = up(B->ImY_dev, ImY, n * sizeof(double *));
+    for (uint32_t i = 0; i < n && e == hipSuccess; i++) {
+        hipLaunchKernelGGL(image_frame_kernel, dim3(1), dim3(256), 0, st, image_views[i],
+                           B->frames_dev + i);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        B->free_images();
+        return mgx::fail(MGX_E_HIP, "batch image set-up: %s", hipGetErrorString(e));
+    }
+    B->n_chunks = (uint32_t)(chunks_per_model * n);
+    B->n_img = n;
+    return MGX_OK;
+}
+
 int mgx_batch_tune(mgx_batch *B, uint32_t sample_stride, uint32_t *best, void *stream)
 {
     mgx::clear_error();
@@ -4817,6 +5103,11 @@ int mgx_batch_tune(mgx_batch *B, uint32_t sample_stride, uint32_t *best, void *s
                                  {4, 0, 3}, {2, 0, 3}, {1, 0, 3}, {4, 0, 2}};
     const bool saved_timing = g_timing;
     g_timing = true;
+    B->images_on = false; /* the candidates tile the solve; the images are timed apart */
+    struct Restore {
+        mgx_batch *b;
+        ~Restore() { b->images_on = true; }
+    } restore{B};
     double best_ms = 1e300;
     BatchConfig best_cfg = B->cfg;
     const int saved_graph = B->use_graph;

@@ -54,6 +54,30 @@ def feautrier_forward(dm, opts, with_u=True, with_lambda=True, out=None, view=No
     return out
 
 
+def ray_pair_view(dm, rr):
+    """(view, tensors): the resident model's view restricted to ray pair rr -- its two
+    directions and weights, antipods 1 and 0.  A formal solve of the view gives that
+    pair's u (as ray 0) and its part of J and of the Lambda diagonal, which add up over
+    the pairs to the full solve's (Solver::solve_feautrier_order_2 adds the pairs'
+    contributions one pair at a time, solver.tpp:475-520); keep ``tensors`` alive while
+    the view is used.  Models with per-point (adaptive) directions are not supported."""
+    torch = _require_gpu()
+    m = dm.model
+    if m.adaptive_directions:
+        raise ValueError("ray_pair_view: per-point ray directions are not supported")
+    if not 0 <= rr < m.hnrays:
+        raise ValueError(f"ray_pair_view: ray pair {rr} out of range (hnrays {m.hnrays})")
+    d = m.ray_direction.reshape(-1, 3)
+    ar = int(m.antipod[rr])
+    t = dict(dm.tensors)
+    t["ray_direction"] = torch.from_numpy(np.ascontiguousarray([d[rr], d[ar]])).to(dm.device)
+    t["ray_weight"] = torch.from_numpy(np.array([m.ray_weight[rr], m.ray_weight[ar]])).to(dm.device)
+    t["antipod"] = torch.from_numpy(np.array([1, 0], dtype=np.uint32)).to(dm.device)
+    v = m.view(t)
+    v.nrays, v.hnrays = 2, 1
+    return v, t
+
+
 def feautrier_forward_lambda(dm, opts, cap=None, with_u=False):
     """The formal solve with the general Lambda operator (opts.n_off_diag bands on either
     side, up to _abi.MAX_OFF_DIAG): dict(u, J, count (P, L), nr (P, L, cap), val (P, L, cap));
@@ -314,8 +338,37 @@ class BatchPlan:
                                              self.dms[0].stream_ptr()))
         return dict(rays_per_item=int(best[0]), order=int(best[1]), wps=int(best[2]))
 
+    def set_images(self, raydirs, nx, ny, freqs):
+        """Adds an nx x ny image of every model (mgx_batch_set_images): model i along
+        raydirs[i] at the frequency grid freqs[i] (ModelData.image_frequencies);
+        run() then computes them after the solve.  The images are self.images, a
+        list of (I (nx*ny, F_i), ImX, ImY) tensors."""
+        torch = _require_gpu()
+        n = len(self.dms)
+        if len(raydirs) != n or len(freqs) != n:
+            raise ValueError("one ray direction and one frequency grid per model")
+        dev = self.dms[0].device
+        views, self._img_keep = [], []
+        self.images = []
+        for dm, fr in zip(self.dms, freqs):
+            v, keep = image_view(dm, fr)
+            views.append(v)
+            self._img_keep.append(keep)
+            F = fr["nu"].shape[1]
+            self.images.append((torch.empty((nx * ny, F), dtype=torch.float64, device=dev),
+                                torch.empty(nx * ny, dtype=torch.float64, device=dev),
+                                torch.empty(nx * ny, dtype=torch.float64, device=dev)))
+        self._img_views = (_abi.MgxModel * n)(*views)
+        self._rd = np.ascontiguousarray(np.asarray(raydirs, dtype=np.float64).reshape(n, 3))
+        ptrs = [(C.c_void_p * n)(*[im[k].data_ptr() for im in self.images]) for k in range(3)]
+        self._img_ptrs = ptrs
+        _abi.check(_abi.lib().mgx_batch_set_images(self._h, self._img_views, self._rd.ctypes.data,
+                                                   nx, ny, ptrs[0], ptrs[1], ptrs[2],
+                                                   self.dms[0].stream_ptr()))
+
     def run(self):
-        """Solves every model; returns the list of J tensors (and Lambda, if kept)."""
+        """Solves every model (and images them, after set_images); returns the list of
+        J tensors (and Lambda, if kept)."""
         _abi.check(_abi.lib().mgx_batch_run(self._h, self.dms[0].stream_ptr()))
         return self.J if self.lam is None else (self.J, self.lam)
 

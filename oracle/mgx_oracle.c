@@ -457,6 +457,21 @@ static uint32_t get_ray_lengths_max(const mgx_model *m, const mgx_options *op, i
     return mx;
 }
 
+/* scratch length for the one ray pair (rr, antipod) from origin o: the all-lines
+   interpolation counts of its two halves bound every line's, so a scratch centred
+   at the longer half holds the pair (the single-ray functions below use it in
+   place of the maximum over all rays and origins, which costs a walk of every
+   ray of the model) */
+static uint32_t pair_length(const mgx_model *m, const mgx_options *op, uint32_t o, uint32_t rr,
+                            int *err)
+{
+    int le = 0;
+    const uint32_t a = get_ray_length(m, op, o, rr, &le);
+    const uint32_t b = get_ray_length(m, op, o, m->antipod[rr], &le);
+    if (le) *err = le;
+    return 2 * (a > b ? a : b) + 1;
+}
+
 /* ------------------------------------------------------------------------- */
 /* line profile, Planck, boundary (solver.tpp :1106-1142, thermodynamics.tpp) */
 
@@ -1427,8 +1442,7 @@ int oracle_trace_ray(const mgx_model *m, const mgx_options *op, uint32_t rr, uin
 {
     int err = check_model(m, op);
     if (err) return err;
-    /* a scratch with room for any ray of this model */
-    const uint32_t length = 2 * get_ray_lengths_max(m, op, &err) + 1;
+    const uint32_t length = pair_length(m, op, o, rr, &err);
     if (err) return err;
     Scratch s;
     if (scratch_init(&s, length)) return MGX_E_ARG;
@@ -2025,7 +2039,7 @@ int oracle_feautrier_system(const mgx_model *m, const mgx_options *op, uint32_t 
 {
     int err = check_model(m, op);
     if (err) return err;
-    const uint32_t length = 2 * get_ray_lengths_max(m, op, &err) + 1;
+    const uint32_t length = pair_length(m, op, o, rr, &err);
     if (err) return err;
     Scratch s;
     if (scratch_init(&s, length)) return MGX_E_ARG;
