@@ -1,17 +1,26 @@
 /*
- * C++ face of the formal-solver path with the reference's class and member
- * names (magritte::Model, magritte::Solver, Parameters, Radiation, Image), for
- * a maintainer who replaces the reference's implementation of
+ * C++ face of the formal-solver path with the reference's object model and
+ * method names (magritte::Model and its parameters / geometry / chemistry /
+ * thermodynamics / lines / radiation members, magritte::Solver, Image), for a
+ * maintainer who replaces the reference's implementation of
  *   Model::compute_radiation_field_feautrier_order_2()   src/model/model.hpp:59
+ *   Model::compute_radiation_field_feautrier_order_2_sparse()
  *   Model::compute_radiation_field_shortchar_order_0(),
  *          compute_radiation_field_feautrier_order_2_uv() src/model/model.hpp:60, 111
  *   Model::compute_image(...), compute_image_new(...), compute_image_optical_depth(...),
  *          compute_image_optical_depth_new(...)          src/model/model.hpp:72-92
+ *   Model::compute_inverse_line_widths / compute_spectral_discretisation /
+ *          compute_LTE_level_populations / compute_level_populations*
  *   Solver::setup / solve_* / image_*                    src/solver/solver.hpp:81-205
- * The model's arrays are the ones the path reads (the mgx_model view of
- * include/mgx.h), held on the host; the class keeps device mirrors and
- * calls the C ABI.  Errors are thrown as std::runtime_error with the
- * library's message, as the reference throws them.
+ * The model is built in memory as the reference's is (parameters.set_*, .set()
+ * on the geometry and thermodynamics vectors, the line data of every
+ * line-producing species); the helper arrays the reference derives when it
+ * reads a model (Points::read, Rays::read, Boundary::read, Lines::read:
+ * cum_n_neighbors, the bounding-box centre, antipodes, point2boundary, nrad_cum,
+ * the sorted line list, nlines, hnrays, nboundary) are derived here before every
+ * solve.  The class keeps device mirrors of the arrays and calls the C ABI of
+ * include/mgx.h.  Errors are thrown as std::runtime_error with the library's
+ * message, as the reference throws them.
  */
 #pragma once
 
@@ -30,34 +39,168 @@ using Real = double;
 using Size = uint32_t;
 using Vector3D = std::array<Real, 3>;
 
-/* The knobs of src/model/parameters/parameters.hpp that the path reads, plus the
-   precision mode of this implementation. */
-struct Parameters {
-    Size npoints = 0, nrays = 0, hnrays = 0, nboundary = 0;
-    Size nlines = 0, nlspecs = 0, nquads = 0, nfreqs = 0;
-    Size dimension = 3;
-    bool spherical_symmetry = false;
-    bool adaptive_ray = false;
-    bool one_line_approximation = false;
-    bool sum_opacity_emissivity_over_all_lines = false;
+/* Vector<T> (src/tools/types.hpp): a std::vector with the reference's set() */
+template <typename T>
+struct Vector {
+    std::vector<T> vec;
+    void set(const std::vector<T> &v) { vec = v; }
+    void resize(size_t n) { vec.resize(n); }
+    size_t size() const { return vec.size(); }
+    T &operator[](size_t i) { return vec[i]; }
+    const T &operator[](size_t i) const { return vec[i]; }
+};
+
+/* Matrix<T>: row-major (nrows, ncols) */
+template <typename T>
+struct Matrix : Vector<T> {
+    size_t nrows = 0, ncols = 0;
+    void resize(size_t r, size_t c)
+    {
+        nrows = r;
+        ncols = c;
+        this->vec.assign(r * c, T());
+    }
+    void set(const std::vector<std::vector<T>> &rows)
+    {
+        nrows = rows.size();
+        ncols = nrows ? rows[0].size() : 0;
+        this->vec.clear();
+        this->vec.reserve(nrows * ncols);
+        for (const auto &r : rows) {
+            if (r.size() != ncols) throw std::runtime_error("Matrix::set: ragged rows");
+            this->vec.insert(this->vec.end(), r.begin(), r.end());
+        }
+    }
+    T &operator()(size_t r, size_t c) { return this->vec[r * ncols + c]; }
+    const T &operator()(size_t r, size_t c) const { return this->vec[r * ncols + c]; }
+};
+
+/* Parameters (src/model/parameters/parameters.hpp:10-140): the counts and switches
+   are set once (set_x; a second set with another value throws, as SetOnce does) and
+   read with x(); the numerical knobs are plain members with the reference's defaults,
+   plus the precision mode of this implementation. */
+class Parameters {
+  public:
+#define MGX_SET_ONCE(type, x)                                                                \
+  public:                                                                                    \
+    void set_##x(const type value)                                                           \
+    {                                                                                        \
+        if (x##_set_ && x##_ != value)                                                       \
+            throw std::runtime_error("Tried to overwrite SetOnce object with another value " \
+                                     "(" #x ").");                                           \
+        x##_ = value;                                                                        \
+        x##_set_ = true;                                                                     \
+    }                                                                                        \
+    type x() const { return x##_; }                                                          \
+    bool x##_is_set() const { return x##_set_; }                                             \
+                                                                                             \
+  private:                                                                                   \
+    type x##_{};                                                                             \
+    bool x##_set_ = false;
+
+    MGX_SET_ONCE(std::string, model_name)
+    MGX_SET_ONCE(Size, dimension)
+    MGX_SET_ONCE(Size, npoints)
+    MGX_SET_ONCE(Size, nrays)
+    MGX_SET_ONCE(Size, hnrays)
+    MGX_SET_ONCE(Size, nboundary)
+    MGX_SET_ONCE(Size, nspecs)
+    MGX_SET_ONCE(Size, nlspecs)
+    MGX_SET_ONCE(Size, nlines)
+    MGX_SET_ONCE(Size, nquads)
+    MGX_SET_ONCE(bool, spherical_symmetry)
+    MGX_SET_ONCE(bool, use_scattering)
+    MGX_SET_ONCE(bool, adaptive_ray_tracing)
+#undef MGX_SET_ONCE
+
+  public:
+    /* nfreqs changes between the line grid and image grids (a mutable parameter) */
+    void set_nfreqs(Size value) { nfreqs_ = value; }
+    Size nfreqs() const { return nfreqs_; }
+    std::string version() const { return "magritte_amd " + std::string(mgx_version()); }
+
     Size n_off_diag = 0;
-    Real min_opacity = 1.0e-26;
-    Real min_dtau = 1.0e-15; /* short characteristics */
-    Real max_distance_opacity_contribution = 10.0;
-    Real max_interpolation_diff = 1.4;
-    bool mixed_precision = false; /* MGX_PREC_MIXED instead of MGX_PREC_FP64 */
-    /* the level-population loop's knobs (parameters.hpp:76-104; mgx_nlte_params) */
-    Real pop_prec = 1.0e-6;
-    Real min_rel_pop_for_convergence = 1.0e-10;
-    Real population_inversion_fraction = -1.0;
-    Real min_line_emissivity = 1.0e-50;
-    Real min_line_opacity = 1.0e-13;
+    Real max_width_fraction = 0.35;
     Real convergence_fraction = 0.995;
+    Real min_rel_pop_for_convergence = 1.0e-10;
+    Real pop_prec = 1.0e-6;
+    Real min_opacity = 1.0e-26;
+    Real min_line_opacity = 1.0e-13;
+    Real min_line_emissivity = 1.0e-50;
+    Real min_dtau = 1.0e-15; /* short characteristics */
+    Real max_interpolation_diff = 1.4;
+    Real population_inversion_fraction = -1.0;
+    bool store_intensities = false;
     bool use_adaptive_Ng_acceleration = true;
     Size Ng_acceleration_mem_limit = 9;
     Size adaptive_Ng_acceleration_min_order = 2;
     bool adaptive_Ng_acceleration_use_max_criterion = true;
     Size Ng_acceleration_remove_N_its = 0;
+    bool one_line_approximation = false;
+    bool sum_opacity_emissivity_over_all_lines = false;
+    Real max_distance_opacity_contribution = 10.0;
+    bool mixed_precision = false; /* MGX_PREC_MIXED instead of MGX_PREC_FP64 */
+
+  private:
+    Size nfreqs_ = 0;
+};
+
+/* Geometry (src/model/geometry/): points, rays and boundary */
+struct Points {
+    Vector<Vector3D> position, velocity; /* (P) [m], [fraction of c] */
+    Vector<Size> neighbors, n_neighbors; /* linearised lists and their lengths */
+    Vector<Size> cum_n_neighbors;        /* derived (Points::read) */
+    Vector3D center{0.0, 0.0, 0.0};      /* derived bounding-box centre (Points::read) */
+};
+
+struct Rays {
+    Vector<Vector3D> direction; /* (nrays), or (P nrays) with adaptive directions */
+    Vector<Real> weight;        /* (nrays), or (P nrays) */
+    Vector<Size> antipod;       /* derived when not set (Rays::read) */
+    bool use_adaptive_directions = false;
+    Size get_antipod_index(Size r) const { return antipod.vec.at(r); }
+};
+
+enum BoundaryCondition { Zero = 0, Thermal = 1, CMB = 2 };
+
+struct Boundary {
+    Vector<Size> boundary2point;
+    Vector<Size> point2boundary; /* derived (Boundary::read) */
+    Vector<Real> boundary_temperature;
+    std::vector<BoundaryCondition> boundary_condition;
+    void set_boundary_condition(Size b, BoundaryCondition c)
+    {
+        if (b >= boundary_condition.size()) boundary_condition.resize(b + 1, Zero);
+        boundary_condition[b] = c;
+    }
+    BoundaryCondition get_boundary_condition(Size b) const { return boundary_condition.at(b); }
+};
+
+struct Geometry {
+    Points points;
+    Rays rays;
+    Boundary boundary;
+};
+
+/* Chemistry (src/model/chemistry/species/species.hpp): abundance (P, nspecs) */
+struct Species {
+    std::vector<std::string> symbol;
+    std::vector<std::vector<Real>> abundance;
+};
+struct Chemistry {
+    Species species;
+};
+
+/* Thermodynamics (src/model/thermodynamics/) */
+struct Temperature {
+    Vector<Real> gas; /* [K] */
+};
+struct Turbulence {
+    Vector<Real> vturb2; /* (v_turb / c)^2 */
+};
+struct Thermodynamics {
+    Temperature temperature;
+    Turbulence turbulence;
 };
 
 /* CollisionPartner (src/model/lines/lineProducingSpecies/linedata/collisionPartner/
@@ -72,12 +215,18 @@ struct CollisionPartner {
 /* Linedata (linedata.hpp) of one line-producing species */
 struct Linedata {
     Size num = 0; /* species column in the abundance table */
+    std::string sym;
     Size nlev = 0, nrad = 0;
     Real inverse_mass = 0.0;
     std::vector<Real> energy, weight;              /* (nlev) */
     std::vector<Size> irad, jrad;                  /* (nrad) upper / lower level */
     std::vector<Real> A, Bs, Ba, frequency;        /* (nrad) */
     std::vector<CollisionPartner> colpar;
+};
+
+/* Quadrature (quadrature.hpp): Gauss-Hermite roots and weights of the line profile */
+struct Quadrature {
+    Vector<Real> roots, weights;
 };
 
 /* Lambda (src/model/lines/lineProducingSpecies/lambda/lambda.hpp): for every point p
@@ -88,18 +237,43 @@ struct LambdaNested {
     std::vector<std::vector<std::vector<Size>>> nr;
 };
 
-/* LineProducingSpecies (lineProducingSpecies.hpp): the line data, the level
-   populations, (P, nlev) row-major, with their totals (P), and the Lambda operator of
-   the last formal solve (filled by compute_radiation_field_feautrier_order_2) */
+/* LineProducingSpecies (lineProducingSpecies.hpp): the line data and quadrature, the
+   level populations, (P, nlev) row-major, with their totals (P), the Lambda operator of
+   the last formal solve, and J (P, nrad), the line-integrated mean intensity of the
+   sparse solve */
 struct LineProducingSpecies {
     Linedata linedata;
+    Quadrature quadrature;
     std::vector<Real> population, population_tot;
     LambdaNested lambda;
+    Matrix<Real> J;
+    Size index(Size p, Size i) const { return p * linedata.nlev + i; }
+};
+
+/* Lines (src/model/lines/lines.hpp): the species, and the line list and per-(point,
+   line) data derived from them */
+struct Lines {
+    std::vector<LineProducingSpecies> lineProducingSpecies;
+    Vector<Real> line;                          /* (L) line centre frequencies */
+    Vector<Size> nrad_cum;                      /* (nlspecs) */
+    Matrix<Real> emissivity, opacity, inverse_width; /* (P, L) */
+    void resize_LineProducingSpecies(Size n) { lineProducingSpecies.resize(n); }
+    Size line_index(Size s, Size k) const { return nrad_cum.vec.at(s) + k; }
+};
+
+/* Frequencies (src/model/radiation/frequencies/frequencies.hpp): nu (P, F) and the
+   line, species, transition and quadrature index of every bin */
+struct Frequencies {
+    Matrix<Real> nu;
+    std::vector<Size> corresponding_line, corresponding_l_for_spec, corresponding_k_for_tran,
+        corresponding_z_for_line;
+    std::vector<uint8_t> appears_in_line_integral;
 };
 
 /* src/model/radiation/radiation.hpp: I (nrays, P, F; short characteristics), u and v
    (hnrays, P, F), J (P, F), row-major; empty when the last solve did not fill them */
 struct Radiation {
+    Frequencies frequencies;
     std::vector<Real> I, u, v, J;
     Size nrays = 0, hnrays = 0, npoints = 0, nfreqs = 0; /* shapes of the last solve */
 };
@@ -129,15 +303,16 @@ struct Image {
 class Model {
   public:
     Parameters parameters;
+    Geometry geometry;
+    Chemistry chemistry;
+    Thermodynamics thermodynamics;
+    Lines lines;
     Radiation radiation;
+    std::vector<Image> images;
     std::vector<Real> lambda_diag; /* (P, L): the diagonal Lambda entries of every line */
     LambdaLists lambda;            /* the general operator (n_off_diag > 0) */
-    std::vector<Image> images;
-    /* lines.lineProducingSpecies, chemistry.species.abundance (P, nspec) and the loop's
-       per-iteration relative changes (Model::error_mean / error_max, species innermost) */
-    std::vector<LineProducingSpecies> lspecs;
-    std::vector<Real> abundance;
-    Size nspec = 0;
+    /* the loop's per-iteration relative changes (Model::error_mean / error_max, species
+       innermost) */
     std::vector<Real> error_mean, error_max;
 
     Model() = default;
@@ -145,15 +320,23 @@ class Model {
     Model &operator=(const Model &) = delete;
     ~Model();
 
-    /* Host arrays by the field names of mgx_model (position, velocity, n_neighbors,
-       ..., nu, f_line, ...).  Setting an array marks the device mirror stale. */
-    void set_array(const std::string &name, const void *data, size_t bytes);
-    const std::vector<unsigned char> &get_array(const std::string &name) const;
-    void set_center(const Vector3D &c) { center = c; }
-    void set_max_inverse_mass(Real v) { max_inverse_mass = v; }
+    /* Lines::set_inverse_width (lines.tpp:45-60) */
+    int compute_inverse_line_widths();
+    /* Model::compute_spectral_discretisation() (model.cpp:70-124): the Gauss-Hermite line
+       grid; (nu_min, nu_max[, n]) (model.cpp:257-298): a uniform image grid of n bins
+       (parameters.nfreqs when n is not given), replacing the line grid until the line grid
+       is computed again */
+    int compute_spectral_discretisation();
+    int compute_spectral_discretisation(Real nu_min, Real nu_max);
+    int compute_spectral_discretisation(Real nu_min, Real nu_max, Size n);
+    /* Model::compute_LTE_level_populations (model.cpp:302-308): LTE populations of every
+       species and the line emissivity, opacity and interpolation criterion */
+    int compute_LTE_level_populations();
 
     /* src/model/model.cpp:353-390; with n_off_diag > 0 also the Lambda lists */
     int compute_radiation_field_feautrier_order_2();
+    /* src/model/model.cpp:474-515: lspec.J (P, nrad) and Lambda, no u or J (P, F) */
+    int compute_radiation_field_feautrier_order_2_sparse();
     /* src/model/model.cpp:313-351 */
     int compute_radiation_field_shortchar_order_0();
     /* src/model/model.cpp:395-433 */
@@ -171,14 +354,7 @@ class Model {
     int compute_image_new(Size ray_nr, Size Nxpix, Size Nypix);
     int compute_image_new(Real rx, Real ry, Real rz, Size Nxpix, Size Nypix);
     int compute_image_new(const Vector3D raydir, Size Nxpix, Size Nypix);
-    /* Model::compute_spectral_discretisation(nu_min, nu_max, n) (model.cpp:257-298):
-       a uniform image grid; the line grid is restored by restore_line_grid(). */
-    int compute_spectral_discretisation(Real nu_min, Real nu_max, Size n);
-    void restore_line_grid();
 
-    /* Model::compute_LTE_level_populations (model.cpp:302-308): LTE populations of every
-       species and the line emissivity, opacity and interpolation criterion */
-    int compute_LTE_level_populations();
     /* Model::compute_level_populations(use_Ng_acceleration, max_niterations)
        (model.cpp:654-744; the general Lambda operator with n_off_diag bands and the
        global statistical equilibrium) and Model::compute_level_populations_sparse
@@ -191,15 +367,22 @@ class Model {
     int compute_level_populations_shortchar(bool use_Ng_acceleration, long max_niterations);
 
     mgx_options options() const;
-    /* the device view (uploads stale arrays first) */
+    /* the derived helper arrays of the reference's readers (cum_n_neighbors, centre,
+       antipodes, point2boundary, nrad_cum, line list, nlines, hnrays, nboundary) */
+    void derive();
+    /* the device view (derives, packs and uploads changed arrays first) */
     const mgx_model &device_view();
+    /* the packed bytes of one array of the device view, by its mgx_model field name */
+    const std::vector<unsigned char> &get_array(const std::string &name);
 
   private:
     int level_population_loop(int kind, bool use_Ng_acceleration, long max_niterations);
-    std::map<std::string, std::vector<unsigned char>> host, saved_line_grid;
+    void pack();
+    void put(const std::string &name, const void *data, size_t bytes);
+    std::map<std::string, std::vector<unsigned char>> host;
     std::map<std::string, void *> dev;
+    std::map<std::string, size_t> dev_bytes;
     std::map<std::string, bool> stale;
-    Vector3D center{0.0, 0.0, 0.0};
     Real max_inverse_mass = 0.0;
     mgx_model view{};
 };
@@ -209,6 +392,7 @@ class Solver {
   public:
     void setup(Model &model);
     void solve_feautrier_order_2(Model &model);
+    void solve_feautrier_order_2_sparse(Model &model);
     void solve_shortchar_order_0(Model &model);
     void solve_feautrier_order_2_uv(Model &model);
     void image_feautrier_order_2_new_imager(Model &model, const Vector3D raydir, Size Nxpix,

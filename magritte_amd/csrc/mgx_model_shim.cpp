@@ -1,16 +1,24 @@
 /*
  * magritte::Model / magritte::Solver (include/magritte_amd.hpp): the reference's
  * C++ entry points of the formal-solver path on top of the C ABI.
+ *   Model::compute_inverse_line_widths                 src/model/model.cpp:60-66
+ *   Model::compute_spectral_discretisation (all)       src/model/model.cpp:70-161, 257-298
+ *   Model::compute_LTE_level_populations               src/model/model.cpp:302-308
  *   Model::compute_radiation_field_feautrier_order_2   src/model/model.cpp:353-390
+ *   Model::compute_radiation_field_feautrier_order_2_sparse  src/model/model.cpp:474-515
  *   Model::compute_radiation_field_shortchar_order_0   src/model/model.cpp:313-351
  *   Model::compute_radiation_field_feautrier_order_2_uv src/model/model.cpp:395-433
  *   Model::compute_image, compute_image_optical_depth  src/model/model.cpp:957-976, 1061-1076
  *   Model::compute_image_new, compute_image_optical_depth_new (all overloads)
  *                                                      src/model/model.cpp:987-1035, 1081-1127
- *   Model::compute_spectral_discretisation(min,max,n)  src/model/model.cpp:257-298
+ *   Model::compute_level_populations*                  src/model/model.cpp:654-935
+ * The helper arrays the reference derives when it reads a model are derived by
+ * Model::derive (Points::read points.cpp:29-57, Rays::read rays.cpp:22-61,
+ * Boundary::read boundary.cpp:9-44, Lines::read lines.cpp:22-68).
  */
 #include <hip/hip_runtime_api.h>
 
+#include <cfloat>
 #include <cmath>
 #include <cstring>
 
@@ -39,8 +47,6 @@ const Field kFields[] = {
     {"f_k", 4},              {"f_z", 4},               {"f_in_line", 1},
 };
 
-const char *kFreqFields[] = {"nu", "f_line", "f_species", "f_k", "f_z", "f_in_line"};
-
 void check(int code)
 {
     if (code != MGX_OK) throw std::runtime_error(mgx_last_error());
@@ -49,6 +55,11 @@ void check(int code)
 void hip_check(hipError_t e, const char *what)
 {
     if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void require(bool ok, const std::string &what)
+{
+    if (!ok) throw std::runtime_error(what);
 }
 
 template <typename T>
@@ -83,6 +94,79 @@ void copy_from_device(std::vector<T> &v, const T *src, size_t n)
     if (n) hip_check(hipMemcpy(v.data(), src, n * sizeof(T), hipMemcpyDeviceToHost), "copy");
 }
 
+/* the C ABI's view of the species' line and collision data (pointers into the model) */
+struct NlteTables {
+    std::vector<mgx_linedata> ld;
+    std::vector<uint32_t> ncolpar;
+    std::vector<mgx_colldata> colpar;
+    size_t npop = 0; /* sum over species of P * nlev */
+};
+
+NlteTables nlte_tables(const Model &model)
+{
+    const auto &species = model.lines.lineProducingSpecies;
+    require(!species.empty(), "the model has no line-producing species");
+    NlteTables t;
+    for (const LineProducingSpecies &s : species) {
+        const Linedata &d = s.linedata;
+        require(d.energy.size() == d.nlev && d.weight.size() == d.nlev && d.irad.size() == d.nrad &&
+                    d.jrad.size() == d.nrad && d.A.size() == d.nrad && d.Bs.size() == d.nrad &&
+                    d.Ba.size() == d.nrad && d.frequency.size() == d.nrad,
+                "line data arrays do not match nlev / nrad");
+        mgx_linedata x;
+        std::memset(&x, 0, sizeof x);
+        x.num = d.num;
+        x.nlev = d.nlev;
+        x.nrad = d.nrad;
+        x.inverse_mass = d.inverse_mass;
+        x.energy = d.energy.data();
+        x.weight = d.weight.data();
+        x.irad = d.irad.data();
+        x.jrad = d.jrad.data();
+        x.A = d.A.data();
+        x.Bs = d.Bs.data();
+        x.Ba = d.Ba.data();
+        x.frequency = d.frequency.data();
+        t.ld.push_back(x);
+        t.ncolpar.push_back((uint32_t)d.colpar.size());
+        for (const CollisionPartner &c : d.colpar) {
+            const size_t ntmp = c.tmp.size(), ncol = c.icol.size();
+            require(c.jcol.size() == ncol && c.Cd.size() == ntmp * ncol && c.Ce.size() == ntmp * ncol,
+                    "collision data arrays do not match ntmp / ncol");
+            mgx_colldata y;
+            std::memset(&y, 0, sizeof y);
+            y.num_col_partner = c.num_col_partner;
+            y.orth_or_para_H2 = c.orth_or_para_H2;
+            y.ntmp = (uint32_t)ntmp;
+            y.ncol = (uint32_t)ncol;
+            y.icol = c.icol.data();
+            y.jcol = c.jcol.data();
+            y.tmp = c.tmp.data();
+            y.Cd = c.Cd.data();
+            y.Ce = c.Ce.data();
+            t.colpar.push_back(y);
+        }
+        t.npop += (size_t)model.parameters.npoints() * d.nlev;
+    }
+    return t;
+}
+
+/* chemistry.species.abundance (P rows of nspecs) as one row-major block */
+std::vector<double> flat_abundance(const Model &model, Size &nspec)
+{
+    const auto &ab = model.chemistry.species.abundance;
+    const size_t P = model.parameters.npoints();
+    require(ab.size() == P, "chemistry.species.abundance must have a row per point");
+    nspec = P ? (Size)ab[0].size() : 0;
+    std::vector<double> flat;
+    flat.reserve(P * nspec);
+    for (const auto &row : ab) {
+        require(row.size() == nspec, "chemistry.species.abundance rows differ in length");
+        flat.insert(flat.end(), row.begin(), row.end());
+    }
+    return flat;
+}
+
 } // namespace
 
 Model::~Model()
@@ -91,51 +175,239 @@ Model::~Model()
         if (kv.second) (void)hipFree(kv.second);
 }
 
-void Model::set_array(const std::string &name, const void *data, size_t bytes)
+void Model::derive()
 {
-    bool known = false;
-    for (const Field &f : kFields) known = known || name == f.name;
-    if (!known) throw std::runtime_error("unknown model array '" + name + "'");
+    Parameters &p = parameters;
+    Points &pts = geometry.points;
+    Rays &rays = geometry.rays;
+    Boundary &bdy = geometry.boundary;
+    /* Points::read: npoints, the bounding-box centre (with the reference's
+       initialisation of the maxima at the smallest positive double), cum_n_neighbors */
+    const Size P = (Size)pts.position.size();
+    require(P > 0, "geometry.points.position is empty");
+    p.set_npoints(P);
+    require(pts.velocity.size() == P, "geometry.points.velocity must have npoints entries");
+    require(pts.n_neighbors.size() == P, "geometry.points.n_neighbors must have npoints entries");
+    Vector3D lo{DBL_MAX, DBL_MAX, DBL_MAX}, hi{DBL_MIN, DBL_MIN, DBL_MIN};
+    for (const Vector3D &x : pts.position.vec)
+        for (int i = 0; i < 3; i++) {
+            if (x[i] < lo[i]) lo[i] = x[i];
+            if (x[i] > hi[i]) hi[i] = x[i];
+        }
+    for (int i = 0; i < 3; i++) pts.center[i] = (lo[i] + hi[i]) / 2;
+    pts.cum_n_neighbors.resize(P);
+    uint64_t tot = 0;
+    for (Size q = 0; q < P; q++) {
+        pts.cum_n_neighbors[q] = (Size)tot;
+        tot += pts.n_neighbors[q];
+    }
+    require(tot == pts.neighbors.size(), "geometry.points.neighbors must hold sum(n_neighbors) entries");
+    for (Size n : pts.neighbors.vec) require(n < P, "a neighbour index is out of range");
+    /* Rays::read: nrays, hnrays, adaptive directions, antipodes (the last r2 with
+       |d1 + d2|^2 < 1e-9, from the first point's directions) */
+    const size_t nd = rays.direction.size();
+    require(nd > 0, "geometry.rays.direction is empty");
+    Size nrays = p.nrays_is_set() ? p.nrays() : (Size)nd;
+    if (nd == (size_t)nrays) {
+        rays.use_adaptive_directions = false;
+    } else if (nd == (size_t)nrays * P) {
+        rays.use_adaptive_directions = true;
+    } else {
+        p.set_nrays((Size)nd); /* throws with both sizes, as the reference does */
+    }
+    p.set_nrays(nrays);
+    p.set_hnrays(nrays / 2);
+    require(rays.weight.size() == nd, "geometry.rays.weight must match the directions");
+    if (rays.antipod.size() != nrays) {
+        rays.antipod.vec.assign(nrays, 0);
+        for (Size r1 = 0; r1 < nrays; r1++)
+            for (Size r2 = 0; r2 < nrays; r2++) {
+                const Vector3D &a = rays.direction[r1], &b = rays.direction[r2];
+                const Real s0 = a[0] + b[0], s1 = a[1] + b[1], s2 = a[2] + b[2];
+                if (s0 * s0 + s1 * s1 + s2 * s2 < 1.0e-9) rays.antipod[r1] = r2;
+            }
+    }
+    /* Boundary::read: nboundary, point2boundary, per-boundary-point conditions */
+    const Size nb = (Size)bdy.boundary2point.size();
+    require(nb > 0, "geometry.boundary.boundary2point is empty");
+    p.set_nboundary(nb);
+    bdy.point2boundary.vec.assign(P, P);
+    for (Size b = 0; b < nb; b++) {
+        require(bdy.boundary2point[b] < P, "a boundary point index is out of range");
+        bdy.point2boundary[bdy.boundary2point[b]] = b;
+    }
+    if (bdy.boundary_condition.size() < nb) bdy.boundary_condition.resize(nb, Zero);
+    if (bdy.boundary_temperature.size() < nb) bdy.boundary_temperature.vec.resize(nb, 0.0);
+    /* Lines::read: nlspecs, nrad_cum, nlines, the line list and max_inverse_mass */
+    auto &species = lines.lineProducingSpecies;
+    require(!species.empty(), "lines.lineProducingSpecies is empty");
+    p.set_nlspecs((Size)species.size());
+    lines.nrad_cum.vec.assign(species.size(), 0);
+    lines.line.vec.clear();
+    max_inverse_mass = 0.0;
+    for (size_t s = 0; s < species.size(); s++) {
+        const Linedata &d = species[s].linedata;
+        require(d.frequency.size() == d.nrad, "linedata.frequency must have nrad entries");
+        if (s > 0) lines.nrad_cum[s] = lines.nrad_cum[s - 1] + species[s - 1].linedata.nrad;
+        lines.line.vec.insert(lines.line.vec.end(), d.frequency.begin(), d.frequency.end());
+        max_inverse_mass = std::max(max_inverse_mass, d.inverse_mass);
+    }
+    p.set_nlines((Size)lines.line.size());
+    if (!chemistry.species.abundance.empty())
+        p.set_nspecs((Size)chemistry.species.abundance[0].size());
+    require(thermodynamics.temperature.gas.size() == P,
+            "thermodynamics.temperature.gas must have npoints entries");
+    require(thermodynamics.turbulence.vturb2.size() == P,
+            "thermodynamics.turbulence.vturb2 must have npoints entries");
+}
+
+void Model::put(const std::string &name, const void *data, size_t bytes)
+{
     auto &v = host[name];
+    if (v.size() == bytes && (bytes == 0 || std::memcmp(v.data(), data, bytes) == 0)) return;
     v.assign(static_cast<const unsigned char *>(data), static_cast<const unsigned char *>(data) + bytes);
     stale[name] = true;
 }
 
-const std::vector<unsigned char> &Model::get_array(const std::string &name) const
+/* the arrays of the device view, from the model's members */
+void Model::pack()
 {
+    derive();
+    const Parameters &p = parameters;
+    const size_t P = p.npoints(), L = p.nlines(), S = p.nlspecs();
+    const Geometry &g = geometry;
+    auto put_vec = [&](const char *name, const auto &v) {
+        put(name, v.data(), v.size() * sizeof(v[0]));
+    };
+    put_vec("position", g.points.position.vec);
+    put_vec("velocity", g.points.velocity.vec);
+    put_vec("n_neighbors", g.points.n_neighbors.vec);
+    put_vec("cum_n_neighbors", g.points.cum_n_neighbors.vec);
+    put_vec("neighbors", g.points.neighbors.vec);
+    put_vec("point2boundary", g.boundary.point2boundary.vec);
+    put_vec("boundary2point", g.boundary.boundary2point.vec);
+    std::vector<int32_t> bc(g.boundary.boundary_condition.begin(), g.boundary.boundary_condition.end());
+    put_vec("boundary_condition", bc);
+    put_vec("boundary_temperature", g.boundary.boundary_temperature.vec);
+    put_vec("ray_direction", g.rays.direction.vec);
+    put_vec("ray_weight", g.rays.weight.vec);
+    put_vec("antipod", g.rays.antipod.vec);
+    const auto &T = thermodynamics.temperature.gas.vec;
+    const auto &vt2 = thermodynamics.turbulence.vturb2.vec;
+    put_vec("temperature", T);
+    put_vec("vturb2", vt2);
+    /* the line list, its sorted copy and the per-line species tables */
+    put_vec("line", lines.line.vec);
+    std::vector<double> sl(L);
+    std::vector<uint32_t> slm(L), lsp, lk;
+    std::vector<double> lA, sim;
+    check(mgx_sort_lines((uint32_t)L, lines.line.vec.data(), sl.data(), slm.data()));
+    put_vec("sorted_line", sl);
+    put_vec("sorted_line_map", slm);
+    const auto &species = lines.lineProducingSpecies;
+    std::vector<double> qw;
+    const Size Q = p.nquads_is_set() ? p.nquads() : 0;
+    for (size_t s = 0; s < S; s++) {
+        const Linedata &d = species[s].linedata;
+        for (Size k = 0; k < d.nrad; k++) {
+            lsp.push_back((uint32_t)s);
+            lk.push_back(k);
+            lA.push_back(d.A.at(k));
+        }
+        sim.push_back(d.inverse_mass);
+        const auto &w = species[s].quadrature.weights.vec;
+        require(Q == 0 || w.size() == Q, "lineProducingSpecies[s].quadrature.weights must have nquads entries");
+        qw.insert(qw.end(), w.begin(), w.end());
+    }
+    put_vec("line_species", lsp);
+    put_vec("line_k", lk);
+    put_vec("line_A", lA);
+    put_vec("species_inverse_mass", sim);
+    put_vec("quad_weights", qw);
+    /* line data of the current populations */
+    const NlteTables t = nlte_tables(*this);
+    if (lines.emissivity.size() == P * L && lines.opacity.size() == P * L) {
+        put_vec("emissivity", lines.emissivity.vec);
+        put_vec("opacity", lines.opacity.vec);
+        std::vector<double> ptot;
+        for (const LineProducingSpecies &s : species) {
+            require(s.population_tot.size() == P, "population_tot must have npoints entries");
+            ptot.insert(ptot.end(), s.population_tot.begin(), s.population_tot.end());
+        }
+        std::vector<double> ic(P * L);
+        check(mgx_compute_interp_crit((uint32_t)P, (uint32_t)S, t.ld.data(), lines.opacity.vec.data(),
+                                      ptot.data(), ic.data()));
+        put_vec("interp_crit", ic);
+    }
+    if (lines.inverse_width.size() == P * L) put_vec("inverse_width", lines.inverse_width.vec);
+    /* Model::set_dshift_max, as Solver::setup calls it (solver.tpp:9, 26) */
+    std::vector<double> dsm(P);
+    check(mgx_compute_dshift_max((uint32_t)P, (uint32_t)S, t.ld.data(), T.data(), vt2.data(),
+                                 p.max_width_fraction, dsm.data()));
+    put_vec("dshift_max", dsm);
+    /* the frequency grid */
+    const Frequencies &fr = radiation.frequencies;
+    put_vec("nu", fr.nu.vec);
+    put_vec("f_line", fr.corresponding_line);
+    put_vec("f_species", fr.corresponding_l_for_spec);
+    put_vec("f_k", fr.corresponding_k_for_tran);
+    put_vec("f_z", fr.corresponding_z_for_line);
+    put_vec("f_in_line", fr.appears_in_line_integral);
+}
+
+const std::vector<unsigned char> &Model::get_array(const std::string &name)
+{
+    bool known = false;
+    for (const Field &f : kFields) known = known || name == f.name;
+    require(known, "unknown model array '" + name + "'");
+    pack();
     auto it = host.find(name);
-    if (it == host.end()) throw std::runtime_error("model array '" + name + "' is not set");
+    require(it != host.end(), "model array '" + name + "' is not set");
     return it->second;
 }
 
 const mgx_model &Model::device_view()
 {
+    pack();
+    const Parameters &p = parameters;
+    require(p.nquads_is_set(), "parameters.nquads is not set");
+    require(lines.inverse_width.size() == (size_t)p.npoints() * p.nlines(),
+            "no inverse line widths: call compute_inverse_line_widths()");
+    require(lines.emissivity.size() == (size_t)p.npoints() * p.nlines(),
+            "no line emissivity and opacity: call compute_LTE_level_populations()");
+    require(radiation.frequencies.nu.size() == (size_t)p.npoints() * p.nfreqs() && p.nfreqs() > 0,
+            "no frequency grid: call compute_spectral_discretisation()");
     for (auto &kv : host) {
         if (!stale[kv.first]) continue;
         void *&d = dev[kv.first];
-        if (d) (void)hipFree(d);
-        d = nullptr;
-        if (!kv.second.empty()) {
+        size_t &cap = dev_bytes[kv.first];
+        if (d && cap < kv.second.size()) {
+            (void)hipFree(d);
+            d = nullptr;
+            cap = 0;
+        }
+        if (!d && !kv.second.empty()) {
             hip_check(hipMalloc(&d, kv.second.size()), "hipMalloc");
+            cap = kv.second.size();
+        }
+        if (!kv.second.empty())
             hip_check(hipMemcpy(d, kv.second.data(), kv.second.size(), hipMemcpyHostToDevice),
                       "hipMemcpy");
-        }
         stale[kv.first] = false;
     }
-    const Parameters &p = parameters;
     mgx_model &v = view;
     std::memset(&v, 0, sizeof v);
-    v.npoints = p.npoints;
-    v.nrays = p.nrays;
-    v.hnrays = p.hnrays;
-    v.nboundary = p.nboundary;
-    v.nlines = p.nlines;
-    v.nlspecs = p.nlspecs;
-    v.nquads = p.nquads;
-    v.nfreqs = p.nfreqs;
-    v.dimension = (int32_t)p.dimension;
-    v.spherical_symmetry = p.spherical_symmetry;
-    v.adaptive_directions = p.adaptive_ray;
+    v.npoints = p.npoints();
+    v.nrays = p.nrays();
+    v.hnrays = p.hnrays();
+    v.nboundary = p.nboundary();
+    v.nlines = p.nlines();
+    v.nlspecs = p.nlspecs();
+    v.nquads = p.nquads();
+    v.nfreqs = p.nfreqs();
+    v.dimension = p.dimension_is_set() ? (int32_t)p.dimension() : 3;
+    v.spherical_symmetry = p.spherical_symmetry_is_set() && p.spherical_symmetry();
+    v.adaptive_directions = geometry.rays.use_adaptive_directions;
     v.position = field_ptr<double>(dev, "position");
     v.velocity = field_ptr<double>(dev, "velocity");
     v.n_neighbors = field_ptr<uint32_t>(dev, "n_neighbors");
@@ -148,7 +420,7 @@ const mgx_model &Model::device_view()
     v.ray_direction = field_ptr<double>(dev, "ray_direction");
     v.ray_weight = field_ptr<double>(dev, "ray_weight");
     v.antipod = field_ptr<uint32_t>(dev, "antipod");
-    for (int i = 0; i < 3; i++) v.center[i] = center[i];
+    for (int i = 0; i < 3; i++) v.center[i] = geometry.points.center[i];
     v.temperature = field_ptr<double>(dev, "temperature");
     v.vturb2 = field_ptr<double>(dev, "vturb2");
     v.line = field_ptr<double>(dev, "line");
@@ -190,11 +462,91 @@ mgx_options Model::options() const
     return o;
 }
 
+/* ------------------------------------------------------------------------- */
+/* producers                                                                 */
+
+int Model::compute_inverse_line_widths()
+{
+    derive();
+    const NlteTables t = nlte_tables(*this);
+    const Size P = parameters.npoints(), L = parameters.nlines();
+    lines.inverse_width.resize(P, L);
+    check(mgx_compute_inverse_widths(P, parameters.nlspecs(), t.ld.data(),
+                                     thermodynamics.temperature.gas.vec.data(),
+                                     thermodynamics.turbulence.vturb2.vec.data(),
+                                     lines.inverse_width.vec.data()));
+    return 0;
+}
+
+int Model::compute_spectral_discretisation()
+{
+    derive();
+    const Parameters &p = parameters;
+    require(p.nquads_is_set(), "parameters.nquads is not set");
+    const Size P = p.npoints(), Q = p.nquads(), F = p.nlines() * Q;
+    const auto &species = lines.lineProducingSpecies;
+    const auto &roots = species[0].quadrature.roots.vec;
+    require(roots.size() == Q, "lineProducingSpecies[0].quadrature.roots must have nquads entries");
+    for (const auto &s : species)
+        require(s.quadrature.roots.vec == roots,
+                "every species must use the same quadrature roots");
+    const NlteTables t = nlte_tables(*this);
+    Frequencies &fr = radiation.frequencies;
+    fr.nu.resize(P, F);
+    fr.corresponding_line.assign(F, 0);
+    fr.corresponding_l_for_spec.assign(F, 0);
+    fr.corresponding_k_for_tran.assign(F, 0);
+    fr.corresponding_z_for_line.assign(F, 0);
+    fr.appears_in_line_integral.assign(F, 0);
+    check(mgx_compute_spectral_discretisation(
+        P, p.nlspecs(), t.ld.data(), Q, roots.data(), thermodynamics.temperature.gas.vec.data(),
+        thermodynamics.turbulence.vturb2.vec.data(), fr.nu.vec.data(), fr.corresponding_line.data(),
+        fr.corresponding_l_for_spec.data(), fr.corresponding_k_for_tran.data(),
+        fr.corresponding_z_for_line.data(), fr.appears_in_line_integral.data()));
+    parameters.set_nfreqs(F);
+    return 0;
+}
+
+int Model::compute_spectral_discretisation(Real nu_min, Real nu_max)
+{
+    return compute_spectral_discretisation(nu_min, nu_max, parameters.nfreqs());
+}
+
+int Model::compute_spectral_discretisation(Real nu_min, Real nu_max, Size n)
+{
+    derive();
+    const Size P = parameters.npoints();
+    require(n > 0, "the image grid needs at least one frequency");
+    Frequencies &fr = radiation.frequencies;
+    fr.nu.resize(P, n);
+    fr.corresponding_line.assign(n, 0);
+    fr.corresponding_l_for_spec.assign(n, 0);
+    fr.corresponding_k_for_tran.assign(n, 0);
+    fr.corresponding_z_for_line.assign(n, 0);
+    fr.appears_in_line_integral.assign(n, 0);
+    check(mgx_compute_image_discretisation(P, nu_min, nu_max, n, fr.nu.vec.data(),
+                                           fr.corresponding_line.data(),
+                                           fr.corresponding_l_for_spec.data(),
+                                           fr.corresponding_k_for_tran.data(),
+                                           fr.corresponding_z_for_line.data(),
+                                           fr.appears_in_line_integral.data()));
+    parameters.set_nfreqs(n);
+    return 0;
+}
+
 int Model::compute_radiation_field_feautrier_order_2()
 {
     Solver solver;
     solver.setup(*this);
     solver.solve_feautrier_order_2(*this);
+    return 0;
+}
+
+int Model::compute_radiation_field_feautrier_order_2_sparse()
+{
+    Solver solver;
+    solver.setup(*this);
+    solver.solve_feautrier_order_2_sparse(*this);
     return 0;
 }
 
@@ -237,12 +589,9 @@ namespace {
 /* geometry.rays.get_direction<false>(0, ray_nr) */
 Vector3D ray_direction_of(const Model &model, Size ray_nr)
 {
-    const auto &d = model.get_array("ray_direction");
-    if ((size_t)(ray_nr + 1) * 3 * sizeof(double) > d.size())
-        throw std::runtime_error("ray_nr out of range");
-    Vector3D r;
-    std::memcpy(r.data(), d.data() + (size_t)ray_nr * 3 * sizeof(double), sizeof r);
-    return r;
+    const auto &d = model.geometry.rays.direction.vec;
+    if (ray_nr >= d.size()) throw std::runtime_error("ray_nr out of range");
+    return d[ray_nr];
 }
 
 void check_image_direction(const Model &model, const Vector3D &raydir)
@@ -301,34 +650,6 @@ int Model::compute_image_new(const Vector3D raydir, Size Nxpix, Size Nypix)
     return 0;
 }
 
-int Model::compute_spectral_discretisation(Real nu_min, Real nu_max, Size n)
-{
-    const Size P = parameters.npoints;
-    if (saved_line_grid.empty())
-        for (const char *f : kFreqFields) saved_line_grid[f] = host[f];
-    std::vector<double> nu((size_t)P * n);
-    std::vector<uint32_t> fl(n), fs(n), fk(n), fz(n);
-    std::vector<uint8_t> fin(n);
-    check(mgx_compute_image_discretisation(P, nu_min, nu_max, n, nu.data(), fl.data(), fs.data(),
-                                           fk.data(), fz.data(), fin.data()));
-    set_array("nu", nu.data(), nu.size() * sizeof(double));
-    set_array("f_line", fl.data(), n * 4);
-    set_array("f_species", fs.data(), n * 4);
-    set_array("f_k", fk.data(), n * 4);
-    set_array("f_z", fz.data(), n * 4);
-    set_array("f_in_line", fin.data(), n);
-    parameters.nfreqs = n;
-    return 0;
-}
-
-void Model::restore_line_grid()
-{
-    if (saved_line_grid.empty()) return;
-    for (auto &kv : saved_line_grid) set_array(kv.first, kv.second.data(), kv.second.size());
-    parameters.nfreqs = (Size)(get_array("f_line").size() / 4);
-    saved_line_grid.clear();
-}
-
 void Solver::setup(Model &model)
 {
     /* the traced-length pass of the reference's setup happens on the device
@@ -344,20 +665,20 @@ namespace {
 void fill_species_lambda(Model &model)
 {
     const Parameters &p = model.parameters;
-    if (model.lspecs.size() != p.nlspecs) return;
+    auto &species = model.lines.lineProducingSpecies;
     const auto &ls_b = model.get_array("line_species");
     const auto &lk_b = model.get_array("line_k");
     const uint32_t *lsp = reinterpret_cast<const uint32_t *>(ls_b.data());
     const uint32_t *lk = reinterpret_cast<const uint32_t *>(lk_b.data());
-    const size_t P = p.npoints, L = p.nlines;
-    for (auto &s : model.lspecs) {
+    const size_t P = p.npoints(), L = p.nlines();
+    for (auto &s : species) {
         s.lambda.Ls.assign(P, std::vector<std::vector<Real>>(s.linedata.nrad));
         s.lambda.nr.assign(P, std::vector<std::vector<Size>>(s.linedata.nrad));
     }
     const LambdaLists &G = model.lambda;
     for (size_t q = 0; q < P; q++)
         for (size_t l = 0; l < L; l++) {
-            LineProducingSpecies &s = model.lspecs.at(lsp[l]);
+            LineProducingSpecies &s = species.at(lsp[l]);
             if (lk[l] >= s.linedata.nrad) throw std::runtime_error("line_k out of range");
             auto &Ls = s.lambda.Ls[q][lk[l]];
             auto &nr = s.lambda.nr[q][lk[l]];
@@ -437,6 +758,32 @@ void Solver::solve_feautrier_order_2(Model &model)
     fill_species_lambda(model);
 }
 
+void Solver::solve_feautrier_order_2_sparse(Model &model)
+{
+    const mgx_model &v = model.device_view();
+    mgx_options o = model.options();
+    /* the sparse solve keeps the Lambda diagonal only (solver.tpp:297-363 calls
+       update_Lambda, whose bands the statistical equilibrium of the sparse loop
+       does not use) */
+    o.n_off_diag = 0;
+    const size_t P = v.npoints, L = v.nlines;
+    DevOut Jl(P * L), lam(P * L);
+    check(mgx_feautrier_sparse_device(&v, &o, Jl.p, lam.p, nullptr));
+    std::vector<double> jl;
+    Jl.copy_to(jl, P * L);
+    lam.copy_to(model.lambda_diag, P * L);
+    model.lambda = LambdaLists{};
+    auto &species = model.lines.lineProducingSpecies;
+    for (size_t s = 0; s < species.size(); s++) {
+        LineProducingSpecies &ls = species[s];
+        const Size nrad = ls.linedata.nrad, l0 = model.lines.nrad_cum[s];
+        ls.J.resize(P, nrad);
+        for (size_t q = 0; q < P; q++)
+            for (Size k = 0; k < nrad; k++) ls.J(q, k) = jl[q * L + l0 + k];
+    }
+    fill_species_lambda(model);
+}
+
 void Solver::solve_shortchar_order_0(Model &model)
 {
     const mgx_model &v = model.device_view();
@@ -477,9 +824,8 @@ namespace {
 /* the image frequencies: the grid at the first point (uniform over points) */
 void set_image_freqs(const Model &model, Image &img, Size F)
 {
-    const auto &nu = model.get_array("nu");
-    img.freqs.resize(F);
-    std::memcpy(img.freqs.data(), nu.data(), F * sizeof(double));
+    const auto &nu = model.radiation.frequencies.nu.vec;
+    img.freqs.assign(nu.begin(), nu.begin() + F);
 }
 
 } // namespace
@@ -580,66 +926,6 @@ void Solver::image_feautrier_order_2_new_imager(Model &model, const Vector3D ray
 
 namespace {
 
-/* the C ABI's view of the species' line and collision data (pointers into lspecs) */
-struct NlteTables {
-    std::vector<mgx_linedata> ld;
-    std::vector<uint32_t> ncolpar;
-    std::vector<mgx_colldata> colpar;
-    size_t npop = 0; /* sum over species of P * nlev */
-};
-
-NlteTables nlte_tables(const Model &model)
-{
-    const Parameters &p = model.parameters;
-    if (model.lspecs.size() != p.nlspecs)
-        throw std::runtime_error("lspecs must hold parameters.nlspecs line-producing species");
-    if (model.abundance.size() != (size_t)p.npoints * model.nspec)
-        throw std::runtime_error("abundance must be (npoints, nspec)");
-    NlteTables t;
-    for (const LineProducingSpecies &s : model.lspecs) {
-        const Linedata &d = s.linedata;
-        if (d.energy.size() != d.nlev || d.weight.size() != d.nlev || d.irad.size() != d.nrad ||
-            d.jrad.size() != d.nrad || d.A.size() != d.nrad || d.Bs.size() != d.nrad ||
-            d.Ba.size() != d.nrad || d.frequency.size() != d.nrad)
-            throw std::runtime_error("line data arrays do not match nlev / nrad");
-        mgx_linedata x;
-        std::memset(&x, 0, sizeof x);
-        x.num = d.num;
-        x.nlev = d.nlev;
-        x.nrad = d.nrad;
-        x.inverse_mass = d.inverse_mass;
-        x.energy = d.energy.data();
-        x.weight = d.weight.data();
-        x.irad = d.irad.data();
-        x.jrad = d.jrad.data();
-        x.A = d.A.data();
-        x.Bs = d.Bs.data();
-        x.Ba = d.Ba.data();
-        x.frequency = d.frequency.data();
-        t.ld.push_back(x);
-        t.ncolpar.push_back((uint32_t)d.colpar.size());
-        for (const CollisionPartner &c : d.colpar) {
-            const size_t ntmp = c.tmp.size(), ncol = c.icol.size();
-            if (c.jcol.size() != ncol || c.Cd.size() != ntmp * ncol || c.Ce.size() != ntmp * ncol)
-                throw std::runtime_error("collision data arrays do not match ntmp / ncol");
-            mgx_colldata y;
-            std::memset(&y, 0, sizeof y);
-            y.num_col_partner = c.num_col_partner;
-            y.orth_or_para_H2 = c.orth_or_para_H2;
-            y.ntmp = (uint32_t)ntmp;
-            y.ncol = (uint32_t)ncol;
-            y.icol = c.icol.data();
-            y.jcol = c.jcol.data();
-            y.tmp = c.tmp.data();
-            y.Cd = c.Cd.data();
-            y.Ce = c.Ce.data();
-            t.colpar.push_back(y);
-        }
-        t.npop += (size_t)p.npoints * d.nlev;
-    }
-    return t;
-}
-
 mgx_nlte_params nlte_params(const Parameters &p)
 {
     mgx_nlte_params q;
@@ -663,7 +949,7 @@ std::vector<double> joined_populations(const Model &model, size_t npop)
 {
     std::vector<double> pop;
     pop.reserve(npop);
-    for (const LineProducingSpecies &s : model.lspecs)
+    for (const LineProducingSpecies &s : model.lines.lineProducingSpecies)
         pop.insert(pop.end(), s.population.begin(), s.population.end());
     return pop;
 }
@@ -671,10 +957,11 @@ std::vector<double> joined_populations(const Model &model, size_t npop)
 void split_populations(Model &model, const std::vector<double> &pop,
                        const std::vector<double> &ptot)
 {
-    const size_t P = model.parameters.npoints;
+    const size_t P = model.parameters.npoints();
     size_t off = 0;
-    for (size_t s = 0; s < model.lspecs.size(); s++) {
-        LineProducingSpecies &ls = model.lspecs[s];
+    auto &species = model.lines.lineProducingSpecies;
+    for (size_t s = 0; s < species.size(); s++) {
+        LineProducingSpecies &ls = species[s];
         const size_t n = P * ls.linedata.nlev;
         ls.population.assign(pop.begin() + off, pop.begin() + off + n);
         ls.population_tot.assign(ptot.begin() + s * P, ptot.begin() + (s + 1) * P);
@@ -694,21 +981,19 @@ struct NlteHandle {
 
 int Model::compute_LTE_level_populations()
 {
+    derive();
     const NlteTables t = nlte_tables(*this);
     const Parameters &p = parameters;
-    const size_t P = p.npoints, PL = (size_t)p.npoints * p.nlines;
-    const auto &T = get_array("temperature");
-    if (T.size() != P * sizeof(double)) throw std::runtime_error("temperature must be (npoints)");
-    std::vector<double> pop(t.npop), ptot((size_t)p.nlspecs * P), em(PL), op(PL), ic(PL);
-    check(mgx_compute_lte(p.npoints, nspec, abundance.data(),
-                          reinterpret_cast<const double *>(T.data()), p.nlspecs, t.ld.data(),
-                          p.min_line_emissivity, p.min_line_opacity, pop.data(), ptot.data(),
-                          em.data(), op.data()));
-    check(mgx_compute_interp_crit(p.npoints, p.nlspecs, t.ld.data(), op.data(), ptot.data(),
-                                  ic.data()));
-    set_array("emissivity", em.data(), PL * sizeof(double));
-    set_array("opacity", op.data(), PL * sizeof(double));
-    set_array("interp_crit", ic.data(), PL * sizeof(double));
+    const size_t P = p.npoints(), L = p.nlines();
+    Size nspec = 0;
+    const std::vector<double> ab = flat_abundance(*this, nspec);
+    std::vector<double> pop(t.npop), ptot((size_t)p.nlspecs() * P);
+    lines.emissivity.resize(P, L);
+    lines.opacity.resize(P, L);
+    check(mgx_compute_lte(p.npoints(), nspec, ab.data(), thermodynamics.temperature.gas.vec.data(),
+                          p.nlspecs(), t.ld.data(), p.min_line_emissivity, p.min_line_opacity,
+                          pop.data(), ptot.data(), lines.emissivity.vec.data(),
+                          lines.opacity.vec.data()));
     split_populations(*this, pop, ptot);
     return 0;
 }
@@ -716,17 +1001,20 @@ int Model::compute_LTE_level_populations()
 /* kind: 0 the general loop, 1 the sparse loop, 2 the short-characteristics loop */
 int Model::level_population_loop(int kind, bool use_Ng_acceleration, long max_niterations)
 {
+    derive();
     const Parameters &p = parameters;
     NlteTables t = nlte_tables(*this);
     bool have = true;
-    for (const LineProducingSpecies &s : lspecs)
-        have = have && s.population.size() == (size_t)p.npoints * s.linedata.nlev;
+    for (const LineProducingSpecies &s : lines.lineProducingSpecies)
+        have = have && s.population.size() == (size_t)p.npoints() * s.linedata.nlev;
     if (!have) compute_LTE_level_populations();
+    Size nspec = 0;
+    const std::vector<double> abundance = flat_abundance(*this, nspec);
     const mgx_model &v = device_view();
     const mgx_options o = options();
     const mgx_nlte_params q = nlte_params(p);
-    const size_t P = p.npoints, PL = (size_t)p.npoints * p.nlines;
-    DevOut ab(abundance.size()), pd(t.npop), pt((size_t)p.nlspecs * P);
+    const size_t P = p.npoints(), PL = (size_t)p.npoints() * p.nlines();
+    DevOut ab(abundance.size()), pd(t.npop), pt((size_t)p.nlspecs() * P);
     hip_check(hipMemcpy(ab.p, abundance.data(), abundance.size() * sizeof(double),
                         hipMemcpyHostToDevice),
               "hipMemcpy");
@@ -734,7 +1022,7 @@ int Model::level_population_loop(int kind, bool use_Ng_acceleration, long max_ni
     hip_check(hipMemcpy(pd.p, pop0.data(), t.npop * sizeof(double), hipMemcpyHostToDevice),
               "hipMemcpy");
     NlteHandle h;
-    check(mgx_nlte_create(&v, p.nlspecs, t.ld.data(), t.ncolpar.data(),
+    check(mgx_nlte_create(&v, p.nlspecs(), t.ld.data(), t.ncolpar.data(),
                           t.colpar.empty() ? nullptr : t.colpar.data(), nspec, ab.p, &q, &h.h,
                           nullptr));
     /* the line data the loop rewrites are the device mirrors the view points to */
@@ -742,7 +1030,7 @@ int Model::level_population_loop(int kind, bool use_Ng_acceleration, long max_ni
     double *op = const_cast<double *>(v.opacity);
     double *ic = const_cast<double *>(v.interp_crit);
     check(mgx_nlte_set_populations(h.h, pd.p, em, op, ic, nullptr));
-    const size_t cap = (size_t)(max_niterations > 0 ? max_niterations : 1) * p.nlspecs;
+    const size_t cap = (size_t)(max_niterations > 0 ? max_niterations : 1) * p.nlspecs();
     std::vector<double> emean(cap), emax(cap);
     int32_t it = 0;
     if (kind == 1)
@@ -781,18 +1069,18 @@ int Model::level_population_loop(int kind, bool use_Ng_acceleration, long max_ni
     }
     std::vector<double> pop, ptot, line;
     pd.copy_to(pop, t.npop);
-    pt.copy_to(ptot, (size_t)p.nlspecs * P);
+    pt.copy_to(ptot, (size_t)p.nlspecs() * P);
     split_populations(*this, pop, ptot);
-    /* the host arrays take the loop's line data (re-uploaded on the next solve) */
-    const char *names[] = {"emissivity", "opacity", "interp_crit"};
-    double *arrays[] = {em, op, ic};
-    for (int i = 0; i < 3; i++) {
-        line.resize(PL);
-        hip_check(hipMemcpy(line.data(), arrays[i], PL * sizeof(double), hipMemcpyDeviceToHost),
-                  "copy");
-        set_array(names[i], line.data(), PL * sizeof(double));
-    }
-    const size_t n = (size_t)it * p.nlspecs;
+    /* lines.emissivity / opacity take the loop's line data (the interpolation
+       criterion follows from them and the populations on the next solve) */
+    lines.emissivity.resize(P, p.nlines());
+    lines.opacity.resize(P, p.nlines());
+    hip_check(hipMemcpy(lines.emissivity.vec.data(), em, PL * sizeof(double), hipMemcpyDeviceToHost),
+              "copy");
+    hip_check(hipMemcpy(lines.opacity.vec.data(), op, PL * sizeof(double), hipMemcpyDeviceToHost),
+              "copy");
+    (void)line;
+    const size_t n = (size_t)it * p.nlspecs();
     error_mean.assign(emean.begin(), emean.begin() + (n < cap ? n : cap));
     error_max.assign(emax.begin(), emax.begin() + (n < cap ? n : cap));
     return it;

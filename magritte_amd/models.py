@@ -506,31 +506,45 @@ class ModelData:
         return m
 
     def to_core(self):
-        """magritte_amd.core.Model (the C++ Model with the reference's method names)
-        holding this model's arrays."""
+        """magritte_amd.core.Model (the C++ Model with the reference's object model) built
+        from this model as the reference's scripts build theirs: parameters.set_*, .set()
+        on the geometry and thermodynamics vectors, the abundance table, the line data and
+        quadrature of every species; then its line grid and widths are computed and the
+        line emissivity, opacity and populations copied."""
         from . import core
 
         cm = core.Model()
         p = cm.parameters
-        for f in ("npoints", "nrays", "hnrays", "nboundary", "nlines", "nlspecs", "nquads",
-                  "nfreqs", "dimension"):
-            setattr(p, f, int(getattr(self, f)))
-        p.spherical_symmetry = bool(self.spherical_symmetry)
-        p.adaptive_ray = bool(self.adaptive_directions)
-        p.min_opacity = self.params["min_opacity"]
-        p.max_distance_opacity_contribution = self.params["max_distance_opacity_contribution"]
-        p.max_interpolation_diff = self.params["max_interpolation_diff"]
-        for k in self.ARRAYS:
-            cm.set_array(k, np.ascontiguousarray(getattr(self, k)))
-        cm.set_center(*[float(x) for x in self.center])
-        cm.set_max_inverse_mass(float(self.max_inverse_mass))
-        # chemistry and the line-producing species (the level-population loop's inputs)
-        p.min_line_emissivity = self.params["min_line_emissivity"]
-        p.min_line_opacity = self.params["min_line_opacity"]
-        cm.abundance = np.ascontiguousarray(self.abundance)
+        P = self.npoints
+        p.set_dimension(int(self.dimension))
+        p.set_npoints(P)
+        p.set_nrays(int(self.nrays))
+        p.set_nspecs(int(self.nspec))
+        p.set_nlspecs(int(self.nlspecs))
+        p.set_nquads(int(self.nquads))
+        p.set_spherical_symmetry(bool(self.spherical_symmetry))
+        for k in ("min_opacity", "max_distance_opacity_contribution", "max_interpolation_diff",
+                  "min_line_emissivity", "min_line_opacity", "max_width_fraction"):
+            setattr(p, k, float(self.params[k]))
+        g = cm.geometry
+        g.points.position.set(self.position.reshape(P, 3))
+        g.points.velocity.set(self.velocity.reshape(P, 3))
+        g.points.n_neighbors.set(self.n_neighbors)
+        g.points.neighbors.set(self.neighbors)
+        g.rays.direction.set(self.ray_direction.reshape(-1, 3))
+        g.rays.weight.set(self.ray_weight)
+        g.rays.antipod.set(self.antipod)
+        g.boundary.boundary2point.set(self.boundary2point)
+        g.boundary.boundary_temperature.set(self.boundary_temperature)
+        for b, c in enumerate(self.boundary_condition):
+            g.boundary.set_boundary_condition(b, core.BoundaryCondition(int(c)))
+        cm.thermodynamics.temperature.gas.set(self.temperature)
+        cm.thermodynamics.turbulence.vturb2.set(self.vturb2)
+        cm.chemistry.species.abundance = np.ascontiguousarray(self.abundance)
+        cm.lines.resize_LineProducingSpecies(self.nlspecs)
         off = 0
         for s, d in enumerate(self.linedata):
-            ls = core.LineProducingSpecies()
+            ls = cm.lines.lineProducingSpecies[s]
             ld = ls.linedata
             ld.num, ld.nlev, ld.nrad = int(d["num"]), int(d["nlev"]), int(d["nrad"])
             ld.inverse_mass = float(d["inverse_mass"])
@@ -547,13 +561,57 @@ class ModelData:
                 cp.Cd = [float(x) for x in np.ravel(c["Cd"])]
                 cp.Ce = [float(x) for x in np.ravel(c["Ce"])]
                 ld.colpar.append(cp)
+            ls.quadrature.roots.set(self.quad_roots[s])
+            ls.quadrature.weights.set(self.quad_weights[s])
             n = self.npoints * int(d["nlev"])
             if getattr(self, "population", None) is not None:
                 ls.population = self.population[off:off + n]
                 ls.population_tot = self.population_tot[s]
             off += n
-            cm.lspecs.append(ls)
+        cm.compute_spectral_discretisation()
+        cm.compute_inverse_line_widths()
+        cm.lines.emissivity.set(self.emissivity)
+        cm.lines.opacity.set(self.opacity)
         return cm
+
+    @classmethod
+    def from_core(cls, cm):
+        """The device view's arrays of a magritte_amd.core.Model (as it packs them for a
+        solve) in a ModelData, e.g. to run the oracle on a model built through the
+        reference's object model."""
+        m = cls()
+        u32, i32, f64, u8 = np.uint32, np.int32, np.float64, np.uint8
+        kinds = {"boundary_condition": i32, "f_in_line": u8}
+        for k in cls.ARRAYS:
+            raw = bytes(cm.get_array(k))
+            dt = kinds.get(k, f64 if k in cls._F64 else u32)
+            setattr(m, k, np.frombuffer(raw, dtype=dt).copy())
+        p = cm.parameters
+        m.npoints, m.nrays, m.hnrays, m.nboundary = p.npoints(), p.nrays(), p.hnrays(), p.nboundary()
+        m.nlines, m.nlspecs, m.nquads, m.nfreqs = p.nlines(), p.nlspecs(), p.nquads(), p.nfreqs()
+        m.dimension = p.dimension()
+        m.spherical_symmetry = int(p.spherical_symmetry())
+        m.adaptive_directions = int(cm.geometry.rays.use_adaptive_directions)
+        m.center = np.array(cm.geometry.points.center)
+        m.max_inverse_mass = float(max(0.0, *m.species_inverse_mass))
+        P = m.npoints
+        m.position = m.position.reshape(P, 3)
+        m.velocity = m.velocity.reshape(P, 3)
+        m.ray_direction = m.ray_direction.reshape(-1, 3)
+        m.nu = m.nu.reshape(P, -1)
+        for k in ("emissivity", "opacity", "inverse_width", "interp_crit"):
+            setattr(m, k, getattr(m, k).reshape(P, m.nlines))
+        m.quad_weights = m.quad_weights.reshape(m.nlspecs, m.nquads)
+        m.params.update(max_interpolation_diff=p.max_interpolation_diff,
+                        min_opacity=p.min_opacity,
+                        max_distance_opacity_contribution=p.max_distance_opacity_contribution,
+                        max_width_fraction=p.max_width_fraction)
+        return m
+
+    _F64 = ("position", "velocity", "boundary_temperature", "ray_direction", "ray_weight",
+            "temperature", "vturb2", "line", "sorted_line", "emissivity", "opacity",
+            "inverse_width", "interp_crit", "dshift_max", "line_A", "species_inverse_mass",
+            "quad_weights", "nu")
 
     def to_device(self, device="cuda"):
         """Copy every solver input to the GPU (torch tensors); returns (view, tensors)."""

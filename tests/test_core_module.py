@@ -1,6 +1,7 @@
 """The pybind11 module magritte_amd.core (C++ magritte::Model with the
-reference's method names, src/bindings/pybindings.cpp:123-271) loads, takes a
-model's arrays, and reports errors as RuntimeError with the library's message
+reference's object model and method names, src/bindings/pybindings.cpp:123-800)
+loads, is built through the nested objects, derives the arrays the reference's
+readers derive, and reports errors as RuntimeError with the library's message
 (no GPU is touched on the CPU)."""
 import numpy as np
 import pytest
@@ -12,12 +13,36 @@ def test_core_model_takes_arrays(line_model):
     cm = line_model.to_core()
     assert isinstance(cm, core.Model)
     p = cm.parameters
-    assert p.npoints == line_model.npoints and p.nfreqs == line_model.nfreqs
+    assert p.npoints() == line_model.npoints and p.nfreqs() == line_model.nfreqs
     assert not p.one_line_approximation and p.n_off_diag == 0
     with pytest.raises(RuntimeError, match="unknown model array"):
-        cm.set_array("no_such_array", np.zeros(3))
+        cm.get_array("no_such_array")
     p.one_line_approximation = True
     assert cm.parameters.one_line_approximation
+    # set-once parameters (SetOnce, tools/setOnce.hpp)
+    p.set_npoints(line_model.npoints)
+    with pytest.raises(RuntimeError, match="SetOnce"):
+        p.set_npoints(line_model.npoints + 1)
+
+
+@pytest.mark.parametrize("which", ["line_model", "small_wind", "vgrad_model"])
+def test_core_model_packs_the_generator_arrays(which, request):
+    """The nested objects, after the derivations of the reference's readers
+    (cum_n_neighbors, centre, antipodes, point2boundary, line list and sort) and the
+    producers (line grid, inverse widths, dshift_max, interpolation criterion), pack
+    to the very arrays the model generator hands the device solver."""
+    m = request.getfixturevalue(which)
+    cm = m.to_core()
+    for k in m.ARRAYS:
+        ref = np.ascontiguousarray(getattr(m, k))
+        got = np.frombuffer(bytes(cm.get_array(k)), dtype=ref.dtype)
+        np.testing.assert_array_equal(got, ref.ravel(), err_msg=k)
+    assert tuple(cm.geometry.points.center) == tuple(m.center)
+    p = cm.parameters
+    assert (p.nlines(), p.nboundary(), p.hnrays(), p.nlspecs()) == (m.nlines, m.nboundary,
+                                                                   m.hnrays, m.nlspecs)
+    np.testing.assert_array_equal(np.array(cm.radiation.frequencies.nu), m.nu)
+    np.testing.assert_array_equal(np.array(cm.lines.inverse_width), m.inverse_width)
 
 
 def test_core_image_direction_checks(line_model):
@@ -50,8 +75,8 @@ def test_core_model_matches_device_solver(small_wind):
                                np.array([0.3, -0.5, 0.81]), 12, 10, fr)
     np.testing.assert_array_equal(np.array(img.I), I.cpu().numpy())
     np.testing.assert_array_equal(np.array(img.ImX), X.cpu().numpy())
-    cm.restore_line_grid()
-    assert cm.parameters.nfreqs == small_wind.nfreqs
+    cm.compute_spectral_discretisation()
+    assert cm.parameters.nfreqs() == small_wind.nfreqs
 
 
 @pytest.mark.gpu
@@ -88,7 +113,7 @@ def test_core_sibling_solvers_match_device_calls(small_wind):
     fr = m.image_frequencies(nu0 * (1 - 2e-5), nu0 * (1 + 2e-5), 31)
     tau, _, _ = solver.image_optical_depth_new(dm, op, np.array([0.3, -0.5, 0.81]), 12, 10, fr)
     np.testing.assert_array_equal(np.array(cm.images[-1].I), tau.cpu().numpy())
-    cm.restore_line_grid()
+    cm.compute_spectral_discretisation()
     cm.parameters.n_off_diag = 2
     cm.compute_radiation_field_feautrier_order_2()
     op2 = m.options()
@@ -107,16 +132,18 @@ def test_core_lte_level_populations(small_wind):
     populations and the line data equal the model generator's LTE arrays."""
     m = small_wind
     cm = m.to_core()
-    assert len(cm.lspecs) == m.nlspecs and cm.abundance.shape == m.abundance.shape
-    cm.lspecs[0].population = np.zeros(0)  # forget the copied populations
+    species = cm.lines.lineProducingSpecies
+    assert len(species) == m.nlspecs
+    assert np.array(cm.chemistry.species.abundance).shape == m.abundance.shape
+    species[0].population = np.zeros(0)  # forget the copied populations
     cm.compute_LTE_level_populations()
     n = m.npoints * m.linedata[0]["nlev"]
-    np.testing.assert_array_equal(np.array(cm.lspecs[0].population), m.population[:n])
-    np.testing.assert_array_equal(np.array(cm.lspecs[0].population_tot), m.population_tot[0])
+    np.testing.assert_array_equal(np.array(cm.lines.lineProducingSpecies[0].population), m.population[:n])
+    np.testing.assert_array_equal(np.array(cm.lines.lineProducingSpecies[0].population_tot), m.population_tot[0])
     for k in ("emissivity", "opacity", "interp_crit"):
         got = np.frombuffer(bytes(cm.get_array(k)), dtype=np.float64).reshape(m.npoints, m.nlines)
         np.testing.assert_array_equal(got, getattr(m, k))
-    cm.lspecs[0].linedata.A = [1.0]  # the species list aliases the model's
+    cm.lines.lineProducingSpecies[0].linedata.A = [1.0]  # the species list aliases the model's
     with pytest.raises(RuntimeError, match="nlev / nrad"):
         cm.compute_LTE_level_populations()
 
@@ -150,13 +177,13 @@ def test_core_level_populations_match_device_loop():
         assert it_c == it
         np.testing.assert_allclose(np.array(cm.error_max), ex, rtol=1e-12, atol=0)
         pop, tot = lp.populations()
-        np.testing.assert_allclose(np.array(cm.lspecs[0].population), pop.cpu().numpy(),
+        np.testing.assert_allclose(np.array(cm.lines.lineProducingSpecies[0].population), pop.cpu().numpy(),
                                    rtol=1e-12, atol=1e-14 * float(tot.max()))
         got = np.frombuffer(bytes(cm.get_array("opacity")), dtype=np.float64)
         np.testing.assert_allclose(got, dm.tensors["opacity"].cpu().numpy().ravel(), rtol=1e-12)
         # after the loop, lspec.Lambda holds its last formal solve's Lambda: one diagonal
         # element per (point, line) for the sparse loop, the bands for the general one
-        lam = cm.lspecs[0].Lambda
+        lam = cm.lines.lineProducingSpecies[0].Lambda
         lens = [len(lam.Ls[p][0]) for p in range(m.npoints)]
         assert all(len(lam.nr[p][0]) == lens[p] for p in range(m.npoints))
         if noff == 0:
@@ -170,7 +197,7 @@ def test_core_level_populations_match_device_loop():
                                           np.array(cm.lambda_val)[:, 0, 0])
     # Lambda of the last formal solve in lspec.Lambda.Ls / nr
     cm.compute_radiation_field_feautrier_order_2()
-    lam = cm.lspecs[0].Lambda
+    lam = cm.lines.lineProducingSpecies[0].Lambda
     cnt, nr, val = np.array(cm.lambda_count), np.array(cm.lambda_nr), np.array(cm.lambda_val)
     for p in (0, 7, 50, 99):
         c = int(cnt[p, 0])
@@ -189,11 +216,11 @@ def test_core_shortchar_lambda_replaces_the_previous_lists(small_wind):
     cm = m.to_core()
     cm.parameters.n_off_diag = 2
     cm.compute_radiation_field_feautrier_order_2()
-    assert max(len(x) for x in cm.lspecs[0].Lambda.Ls[5]) > 1
+    assert max(len(x) for x in cm.lines.lineProducingSpecies[0].Lambda.Ls[5]) > 1
     cm.compute_radiation_field_shortchar_order_0()
     J2 = np.array(cm.radiation.J).copy()
     diag = np.array(cm.lambda_diag)
-    lam = cm.lspecs[0].Lambda
+    lam = cm.lines.lineProducingSpecies[0].Lambda
     line_k = np.frombuffer(bytes(cm.get_array("line_k")), dtype=np.uint32)
     for p in (0, 5, m.npoints // 2, m.npoints - 1):
         for l in range(m.nlines):
