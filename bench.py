@@ -216,11 +216,35 @@ def batch_compare(ms, dms, precisions, steps):
         t_batch = (time.perf_counter() - t2) / steps
         err = max(float(np.max(np.abs(J.cpu().numpy() - o["J"].cpu().numpy()) /
                                np.abs(o["J"].cpu().numpy()))) for J, o in zip(Js, outs))
-        plan.close()
         del outs, Js
+        # each model's 256^2 x 31 image along a seeded direction: in the batch's graph after
+        # the solve, against per-model image calls
+        rng = np.random.Generator(np.random.PCG64(1000 + len(ms)))
+        rds = rng.standard_normal((len(ms), 3))
+        rds /= np.linalg.norm(rds, axis=1)[:, None]
+        frs = [m.image_frequencies(m.line[0] * (1 - 1e-4), m.line[0] * (1 + 1e-4), 31) for m in ms]
+        for k, dm in enumerate(dms):  # warm-up
+            solver.image_new(dm, op, rds[k], 256, 256, frs[k])
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        for k, dm in enumerate(dms):
+            solver.image_new(dm, op, rds[k], 256, 256, frs[k])
+        torch.cuda.synchronize()
+        t_plain_img = time.perf_counter() - t3
+        plan.set_images(rds, 256, 256, frs)
+        plan.run()  # warm-up (graph capture with the image kernel)
+        torch.cuda.synchronize()
+        t4 = time.perf_counter()
+        plan.run()
+        torch.cuda.synchronize()
+        t_with = time.perf_counter() - t4
+        plan.close()
         res[name] = {"plain_ms": round(t_plain * 1e3, 2), "batch_ms": round(t_batch * 1e3, 2),
                      "speedup": round(t_plain / t_batch, 3), "tuned": best,
-                     "value": round(U / t_batch, 1), "J_rel_diff_batch_vs_plain": err}
+                     "value": round(U / t_batch, 1), "J_rel_diff_batch_vs_plain": err,
+                     "images_256x256x31": {"plain_ms": round(t_plain_img * 1e3, 2),
+                                           "batch_with_images_ms": round(t_with * 1e3, 2),
+                                           "images_in_batch_ms": round((t_with - t_batch) * 1e3, 2)}}
         print(f"# batch {name}: batch {t_batch * 1e3:.1f} ms, tuned {best}", file=sys.stderr,
               flush=True)
     return res

@@ -37,7 +37,9 @@
 #include <cstring>
 #include <algorithm>
 #include <cmath>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "mgx.h"
@@ -1678,6 +1680,46 @@ __global__ __launch_bounds__(MGX_WAVE) void trace_stats_kernel(const KParams *__
     }
 }
 
+/* The walk-buffer size a solve needs, before the solve (the reference sizes its
+   scratch from get_ray_lengths_max, solver.tpp:106-126): one wave per (origin,
+   ray pair) walks both halves without storing them and the longest half's step
+   count goes into *max_steps.  Z grows along a walk, so a general-geometry walk
+   visits no point twice, and a spherically symmetric one each shell at most twice
+   (in and out): 2 npoints + 2 steps bound it, and a longer walk flags ERR_NO_NEXT. */
+__global__ __launch_bounds__(MGX_WAVE) void walk_max_kernel(const KParams *__restrict__ kpp,
+                                                            uint32_t *max_steps)
+{
+    const KParams &kp = *kpp;
+    const mgx_model &m = kp.m;
+    const uint32_t o = blockIdx.x, rr = blockIdx.y, lane = threadIdx.x;
+    uint32_t longest = 0;
+    for (int h = 0; h < 2; h++) {
+        const uint32_t r = h ? m.antipod[rr] : rr;
+        const PointRec po = gld(&kp.pts[o]);
+        const double origin[3] = {po.x, po.y, po.z};
+        const double *dp = ray_dir(m, o, r);
+        const double dir[3] = {gld(&dp[0]), gld(&dp[1]), gld(&dp[2])};
+        double Z = 0.0, dZ = 0.0;
+        uint32_t crt = o, n = 0;
+        for (;;) {
+            const uint32_t nxt = m.spherical_symmetry ? next_spherical(kp, origin, dir, crt, Z, dZ)
+                                                      : next_general(kp, origin, dir, crt, Z, dZ);
+            if (nxt >= m.npoints) {
+                if (n != 0 && lane == 0) atomicOr(&kp.status[0], (uint32_t)ERR_NO_NEXT);
+                break;
+            }
+            if (++n > 2 * m.npoints + 2) {
+                if (lane == 0) atomicOr(&kp.status[0], (uint32_t)ERR_NO_NEXT);
+                break;
+            }
+            if (on_boundary(kp, nxt)) break;
+            crt = nxt;
+        }
+        longest = n > longest ? n : longest;
+    }
+    if (lane == 0) atomicMax(max_steps, longest);
+}
+
 /* one wave per selected (rr, o, l); lane 0 writes the expanded list */
 __global__ __launch_bounds__(MGX_WAVE) void trace_rays_kernel(
     const KParams *__restrict__ kpp, const uint32_t *sel_rr, const uint32_t *sel_o, const uint32_t *sel_l,
@@ -2002,31 +2044,54 @@ __device__ inline uint32_t n_interp_all(const KParams &kp, uint32_t c, uint32_t 
     return (k > 1) ? k : 1;
 }
 
-/* One pixel of the projection-surface imager (the pixel loop bodies of
+/* The projection-surface imager, one pixel at a time (the pixel loop bodies of
    Solver::image_feautrier_order_2_new_imager and image_optical_depth_new_imager,
-   solver.tpp:590-620, 701-729): the wave walks the
-   pixel's ray into the LDS walk buffer smem (walk_lds_bytes(kp.cap) bytes) and its
-   lanes take the frequencies.  Returns false when the walk needs more than kp.cap
-   steps (the status words then hold ERR_CAP and the capacity needed). */
-template <int APPROX, int PREC, int KIND>
-__device__ bool image_pixel(const KParams &kp, const ImageFrame &fr, uint32_t pix,
-                            unsigned char *smem, double *I, double *ImX, double *ImY)
+   solver.tpp:590-620, 701-729), in two parts: image_walk, where the whole wave
+   walks the pixel's ray into its lists in the LDS walk buffer, and image_freqs,
+   where lanes take the pixel's frequencies.  The walk buffer of capacity cap
+   (walk_lds_bytes(cap) bytes) holds two pixels' lists of up to cap steps each:
+   slot 0 and slot 1. */
+struct PixelWalk {
+    uint32_t *nxt;
+    double *dZ, *sh;
+    uint16_t *m;
+    uint32_t ip, S;
+    double shift0, shift_centre;
+};
+
+__device__ inline PixelWalk pixel_slot(unsigned char *smem, uint32_t cap, uint32_t slot)
 {
-    const double *s_erf = nullptr; /* erf needs no table */
+    PixelWalk w;
+    double *dZ = reinterpret_cast<double *>(smem + 8 * ((2 * cap * 4 + 7) / 8));
+    w.nxt = reinterpret_cast<uint32_t *>(smem) + slot * cap;
+    w.dZ = dZ + slot * cap;
+    w.sh = dZ + 2 * cap + slot * cap;
+    w.m = reinterpret_cast<uint16_t *>(dZ + 4 * cap) + slot * cap;
+    w.ip = 0;
+    w.S = 0;
+    w.shift0 = w.shift_centre = 0.0;
+    return w;
+}
+
+/* Walks pixel pix's ray into w's lists (the whole wave; lane 0 writes ImX, ImY and
+   the lists).  Returns false when the walk needs more than kp.cap steps (the status
+   words then hold ERR_CAP and the capacity needed). */
+__device__ bool image_walk(const KParams &kp, const ImageFrame &fr, uint32_t pix, PixelWalk &w,
+                           double *ImX, double *ImY)
+{
     const mgx_model &m = kp.m;
-    const uint32_t lane = threadIdx.x;
+    const uint32_t lane = threadIdx.x & (MGX_WAVE - 1);
     const uint32_t cap = kp.cap;
-    uint32_t *s_nxt = reinterpret_cast<uint32_t *>(smem);
-    double *s_dZ = reinterpret_cast<double *>(smem + 8 * ((2 * cap * 4 + 7) / 8));
-    double *s_sh = s_dZ + 2 * cap;
-    uint16_t *s_m = reinterpret_cast<uint16_t *>(s_dZ + 4 * cap);
-    const uint32_t F = m.nfreqs;
+    uint32_t *s_nxt = w.nxt;
+    double *s_dZ = w.dZ;
+    double *s_sh = w.sh;
+    uint16_t *s_m = w.m;
 
     /* pixel position on the projection surface and its 3D origin */
     const uint32_t x_idx = pix / fr.ny, y_idx = pix % fr.ny;
     const double xloc = fr.min_x + fr.deltax / 2.0 + x_idx * fr.deltax;
     const double yloc = fr.min_y + fr.deltay / 2.0 + y_idx * fr.deltay;
-    if (lane == 0) {
+    if (lane == 0 && ImX) {
         ImX[pix] = xloc;
         ImY[pix] = yloc;
     }
@@ -2113,8 +2178,28 @@ __device__ bool image_pixel(const KParams &kp, const ImageFrame &fr, uint32_t pi
         s_m[i] = (uint16_t)(mm > 0xffffu ? 0xffffu : mm);
     }
     __syncthreads();
+    w.ip = ip;
+    w.S = S;
+    w.shift0 = shift0;
+    w.shift_centre = shift_centre;
+    return true;
+}
 
-    for (uint32_t f = lane; f < F; f += MGX_WAVE) {
+/* The frequencies f0, f0 + fstep, ... of the walked pixel pix. */
+template <int APPROX, int PREC, int KIND>
+__device__ void image_freqs(const KParams &kp, const PixelWalk &w, uint32_t pix, uint32_t f0,
+                            uint32_t fstep, double *I)
+{
+    const double *s_erf = nullptr; /* erf needs no table */
+    const mgx_model &m = kp.m;
+    const uint32_t F = m.nfreqs;
+    const uint32_t ip = w.ip, S = w.S;
+    const double shift0 = w.shift0, shift_centre = w.shift_centre;
+    const uint32_t *s_nxt = w.nxt;
+    const double *s_dZ = w.dZ, *s_sh = w.sh;
+    const uint16_t *s_m = w.m;
+
+    for (uint32_t f = f0; f < F; f += fstep) {
         const double nu_ip = m.nu[(size_t)ip * F + f];
         double Ival;
         if (S == 0) {
@@ -2162,18 +2247,41 @@ __device__ bool image_pixel(const KParams &kp, const ImageFrame &fr, uint32_t pi
         }
         I[(size_t)pix * F + f] = Ival;
     }
+}
+
+/* One pixel per wave, or two when the image has at most 32 frequencies: then the
+   wave walks both pixels' rays and each half-wave takes one pixel's frequencies.
+   Returns false when a walk outgrew the walk buffer. */
+template <int APPROX, int PREC, int KIND>
+__device__ bool image_pixels(const KParams &kp, const ImageFrame &fr, uint32_t pix0, uint32_t npix,
+                             unsigned char *smem, double *I, double *ImX, double *ImY)
+{
+    const uint32_t lane = threadIdx.x & (MGX_WAVE - 1);
+    PixelWalk w0 = pixel_slot(smem, kp.cap, 0);
+    if (!image_walk(kp, fr, pix0, w0, ImX, ImY)) return false;
+    if (npix == 1) {
+        image_freqs<APPROX, PREC, KIND>(kp, w0, pix0, lane, MGX_WAVE, I);
+        return true;
+    }
+    PixelWalk w1 = pixel_slot(smem, kp.cap, 1);
+    if (!image_walk(kp, fr, pix0 + 1, w1, ImX, ImY)) return false;
+    const uint32_t half = lane >> 5;
+    image_freqs<APPROX, PREC, KIND>(kp, half ? w1 : w0, pix0 + half, lane & 31, 32, I);
     return true;
 }
 
-/* one wave per pixel of one model's image */
+
+/* one wave per ppw (1 or 2) consecutive pixels of one model's image */
 template <int APPROX, int PREC, int KIND>
 __global__ __launch_bounds__(MGX_WAVE) void image_kernel(const KParams *__restrict__ kpp, const ImageFrame *frp,
-                                                         double *I, double *ImX, double *ImY)
+                                                         double *I, double *ImX, double *ImY, uint32_t ppw)
 {
     mgx_ref_tables_init();
     extern __shared__ unsigned char smem[];
     const ImageFrame fr = *frp;
-    (void)image_pixel<APPROX, PREC, KIND>(*kpp, fr, blockIdx.x, smem, I, ImX, ImY);
+    const uint32_t npix_all = fr.nx * fr.ny, p0 = blockIdx.x * ppw;
+    const uint32_t n = (p0 + ppw <= npix_all) ? ppw : npix_all - p0;
+    (void)image_pixels<APPROX, PREC, KIND>(*kpp, fr, p0, n, smem, I, ImX, ImY);
 }
 
 /*
@@ -2217,8 +2325,10 @@ __global__ __launch_bounds__(MGX_WAVE) void image_batch_kernel(
         const uint32_t npix = fr.nx * fr.ny;
         const uint32_t p0 = (w - chunk_start[lo]) * kImageChunk;
         const uint32_t p1 = (p0 + kImageChunk < npix) ? p0 + kImageChunk : npix;
-        for (uint32_t pix = p0; pix < p1; pix++) {
-            if (!image_pixel<APPROX, PREC, IMG_INTENSITY>(kp, fr, pix, smem, I[mi], ImX[mi], ImY[mi]))
+        const uint32_t ppw = kp.m.nfreqs <= 32 ? 2 : 1;
+        for (uint32_t pix = p0; pix < p1; pix += ppw) {
+            const uint32_t n = (pix + ppw <= p1) ? ppw : p1 - pix;
+            if (!image_pixels<APPROX, PREC, IMG_INTENSITY>(kp, fr, pix, n, smem, I[mi], ImX[mi], ImY[mi]))
                 return;
             __syncthreads();
         }
@@ -3429,109 +3539,35 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_ADJ_WAVES_PER_SIMD) void image_adjoin
     const mgx_model &m = kp.m;
     const ImageFrame fr = *frp;
     const uint32_t lane = threadIdx.x;
-    const uint32_t cap = kp.cap;
-    uint32_t *s_nxt = reinterpret_cast<uint32_t *>(smem);
-    double *s_dZ = reinterpret_cast<double *>(smem + 8 * ((2 * cap * 4 + 7) / 8));
-    double *s_sh = s_dZ + 2 * cap;
-    uint16_t *s_m = reinterpret_cast<uint16_t *>(s_dZ + 4 * cap);
     const uint32_t F = m.nfreqs;
     const uint32_t npix = fr.nx * fr.ny;
+    /* two pixels per wave when a half-wave covers the frequencies (image_pixels) */
+    const uint32_t ppw = F <= 32 ? 2 : 1;
+    const uint32_t nitems = (npix + ppw - 1) / ppw;
     double *st_wave = scratch + (size_t)blockIdx.x * capE * MGX_TAPE_WORDS * MGX_WAVE;
-    const double *rd = fr.rd;
 
-    for (uint32_t pix = blockIdx.x; pix < npix; pix += gridDim.x) {
+    for (uint32_t q = blockIdx.x; q < nitems; q += gridDim.x) {
         /* a buffer of this launch overflowed: the host clears the gradients and
            relaunches, so the remaining pixels are skipped */
         if (__builtin_amdgcn_readfirstlane(__hip_atomic_load(&kp.status[0], __ATOMIC_RELAXED,
                                                              __HIP_MEMORY_SCOPE_AGENT)) &
             (ERR_CAP | ERR_CAPE))
             return;
-        const uint32_t x_idx = pix / fr.ny, y_idx = pix % fr.ny;
-        const double xloc = fr.min_x + fr.deltax / 2.0 + x_idx * fr.deltax;
-        const double yloc = fr.min_y + fr.deltay / 2.0 + y_idx * fr.deltay;
-        double origin[3];
-        if (fr.denominator >= 1.0e-9) {
-            origin[0] = (xloc * fr.ix + yloc * fr.jx) + fr.cpt[0];
-            origin[1] = (xloc * fr.iy + yloc * fr.jy) + fr.cpt[1];
-            origin[2] = (yloc * fr.jz) + fr.cpt[2];
-        } else {
-            origin[0] = xloc + fr.cpt[0];
-            origin[1] = yloc + fr.cpt[1];
-            origin[2] = 0.0 + fr.cpt[2];
-        }
-        uint32_t ip = fr.start_bdy;
-        for (uint32_t it = 0; it <= m.nboundary; it++) {
-            const uint32_t na = bdy_closer(m, origin, rd, ip);
-            if (na == ip) break;
-            ip = na;
-        }
-        double Z;
-        {
-            double R[3];
-            for (int i = 0; i < 3; i++) R[i] = m.position[3 * (size_t)ip + i] - origin[i];
-            Z = dot3(rd, R);
-            if (m.spherical_symmetry) {
-                const double Rcos = dot3(origin, rd);
-                const double R2 = dot3(origin, origin);
-                const double Rsin2 = R2 - Rcos * Rcos;
-                const double *pb = &m.position[3 * (size_t)ip];
-                const double h2 = dot3(pb, pb) - Rsin2;
-                Z = (h2 > 0) ? (-sqrt(h2) - Rcos) : -Rcos;
-            }
-        }
-        const double shift_centre = shift_rest(m, origin, rd, ip, Z);
+        const uint32_t p0 = q * ppw;
+        const uint32_t n = (p0 + ppw <= npix) ? ppw : npix - p0;
         __syncthreads();
-        uint32_t S = 0;
-        double shift0 = 0.0;
-        {
-            double dZ = 0.0;
-            uint32_t crt = ip;
-            uint32_t nxt = m.spherical_symmetry ? next_spherical(kp, origin, rd, crt, Z, dZ)
-                                                : next_general(kp, origin, rd, crt, Z, dZ);
-            bool accessed_non_boundary = false;
-            if (nxt < m.npoints) {
-                shift0 = shift_rest(m, origin, rd, crt, Z - dZ);
-                double shift_nxt = shift_rest(m, origin, rd, nxt, Z);
-                for (;;) {
-                    if (S >= cap) {
-                        if (lane == 0) {
-                            atomicOr(&kp.status[0], (uint32_t)ERR_CAP);
-                            atomicMax(&kp.status[1], cap + 1);
-                        }
-                        return;
-                    }
-                    if (lane == 0) {
-                        s_nxt[S] = nxt;
-                        s_dZ[S] = dZ;
-                        s_sh[S] = shift_nxt;
-                    }
-                    S++;
-                    double rel[3];
-                    for (int i = 0; i < 3; i++) rel[i] = m.position[3 * (size_t)nxt + i] - m.center[i];
-                    const bool interior = m.point2boundary[nxt] == m.npoints;
-                    if (!(interior || dot3(rel, rd) < 0)) break;
-                    crt = nxt;
-                    if (interior) accessed_non_boundary = true;
-                    nxt = m.spherical_symmetry ? next_spherical(kp, origin, rd, crt, Z, dZ)
-                                               : next_general(kp, origin, rd, crt, Z, dZ);
-                    if (nxt >= m.npoints) break;
-                    shift_nxt = shift_rest(m, origin, rd, nxt, Z);
-                }
-            }
-            if (!accessed_non_boundary && !m.spherical_symmetry) S = 0;
-        }
-        __syncthreads();
-        for (uint32_t i = lane; i < S; i += MGX_WAVE) {
-            const uint32_t crt = (i == 0) ? ip : s_nxt[i - 1];
-            const uint32_t mm = n_interp_all(kp, crt, s_nxt[i]);
-            if (mm > 0xffffu) atomicOr(&kp.status[0], (uint32_t)ERR_INTERP);
-            s_m[i] = (uint16_t)(mm > 0xffffu ? 0xffffu : mm);
-        }
-        __syncthreads();
-        if (S == 0) continue; /* I = I_bdy: no dependence */
-
-        for (uint32_t f0 = 0; f0 < F; f0 += MGX_WAVE) {
-            const uint32_t fl = f0 + lane;
+        PixelWalk w0 = pixel_slot(smem, kp.cap, 0), w1 = pixel_slot(smem, kp.cap, 1);
+        if (!image_walk(kp, fr, p0, w0, nullptr, nullptr)) return;
+        if (n == 2 && !image_walk(kp, fr, p0 + 1, w1, nullptr, nullptr)) return;
+        /* the lanes of a group (the wave, or a half-wave per pixel) take one pixel */
+        const uint32_t gw = (n == 2) ? 32 : 64;
+        const uint32_t half = (n == 2) ? (lane >> 5) : 0;
+        const PixelWalk &w = half ? w1 : w0;
+        const uint32_t pix = p0 + half;
+        const uint32_t S = w.S, ip = w.ip;
+        if (S == 0) continue; /* I = I_bdy: no dependence (for the whole group) */
+        for (uint32_t fb = 0; fb < F; fb += gw) {
+            const uint32_t fl = fb + (lane & (gw - 1));
             const uint32_t f = (fl < F) ? fl : F - 1;
             const double freq = m.nu[(size_t)ip * F + f];
             const uint32_t line = m.f_line[f];
@@ -3543,25 +3579,28 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_ADJ_WAVES_PER_SIMD) void image_adjoin
             fw.started = false;
             PointECT ec = {0.0, 0.0, 0.0, false};
             SweepStore st{st_wave + lane, capE, 0, 0.0};
-            sweep_half_tape<APPROX, PREC>(kp, s_erf, ol, false, S, ip, shift0, shift_centre, s_nxt,
-                                          s_dZ, s_sh, s_m, line, freq, fw, ec, st);
+            sweep_half_tape<APPROX, PREC>(kp, s_erf, ol, false, S, ip, w.shift0, w.shift_centre,
+                                          w.nxt, w.dZ, w.sh, w.m, line, freq, fw, ec, st);
             if (st.n > capE) {
-                if (lane == 0) {
+                if (lane == 0 || lane == 32) {
                     atomicOr(&kp.status[0], (uint32_t)ERR_CAPE);
                     atomicMax(&kp.status[2], st.n);
                 }
                 continue;
             }
-            const CentreRow c = centre_row<PREC>(kp, ip, freq, shift_centre, true, false, fw, fw);
+            const CentreRow c = centre_row<PREC>(kp, ip, freq, w.shift_centre, true, false, fw, fw);
             const double mu_c = ubar * c.xc * rcp(c.Dc);
             const double lam_c = ubar * c.xc;
-            const uint32_t l0 = __shfl(own_l, 0, MGX_WAVE);
-            const bool uniform = __all(own_l == l0);
-            GradSink g{g_eps, g_kap, m.nlines, own_l, uniform ? 64u : 0u, 0, 0, 0.0, 0.0, 0.0, 0.0,
+            /* one reduction per group when the group's lanes share their own line */
+            const uint32_t l0 = __shfl(own_l, (int)(lane & ~(gw - 1)), MGX_WAVE);
+            const uint64_t same = __ballot(own_l == l0);
+            const uint64_t gmask = (gw == 64) ? ~0ull : (0xffffffffull << (32 * half));
+            const bool uniform = (same & gmask) == gmask;
+            GradSink g{g_eps, g_kap, m.nlines, own_l, uniform ? gw : 0u, 0, 0, 0.0, 0.0, 0.0, 0.0,
                        g_iw, 0.0, 0.0};
             const RevCentre rc{c.u, mu_c, lam_c, -lam_c * c.dd_fw, true, ec};
-            reverse_half<APPROX, PREC, IW>(kp, s_erf, ol, false, S, ip, shift0, shift_centre,
-                                           s_nxt, s_dZ, s_sh, s_m, line, freq, st, rc, g);
+            reverse_half<APPROX, PREC, IW>(kp, s_erf, ol, false, S, ip, w.shift0, w.shift_centre,
+                                           w.nxt, w.dZ, w.sh, w.m, line, freq, st, rc, g);
         }
     }
 }
@@ -3614,7 +3653,9 @@ struct DevBuffers {
     PointIdx *pidx = nullptr;
     uint32_t *status = nullptr;
     KParams *kp_dev = nullptr; /* device copy of the launch parameters */
-    bool owned = true;         /* false: carved out of the no-allocation scratch */
+    ImageFrame *frame = nullptr; /* the imager's frame (carved with the rest) */
+    bool owned = true;         /* false: carved out of a shared buffer */
+    std::unique_lock<std::mutex> ws_lock; /* held while carved from the call workspace */
     ~DevBuffers()
     {
         if (!owned) return;
@@ -3661,7 +3702,7 @@ int check_view(const mgx_model *m, const mgx_options *op)
 }
 
 
-const uint32_t kMaxCap = 1280; /* 1280 steps per half: 60 KB of dynamic LDS */
+const uint32_t kMaxCap = 8192; /* walk steps per half ray: a ceiling above any LDS budget */
 
 bool g_timing = false;
 double g_last_kernel_ms = 0.0;
@@ -3711,14 +3752,57 @@ int prepare_tables(const mgx_model *m, const mgx_options *op, hipStream_t st, De
     return MGX_OK;
 }
 
-int prepare(const mgx_model *m, const mgx_options *op, hipStream_t st, DevBuffers &b, KParams &kp)
+/* Device memory of a call's packed tables, status words, launch parameters and
+   image frame: one grow-only buffer per device, reused by every call, so a call
+   allocates nothing once the buffer has grown to its model.  A call holds the
+   buffer (b.ws_lock) until it returns, so calls that use it run one at a time. */
+std::mutex g_prep_mutex;
+unsigned char *g_prep[64] = {};
+size_t g_prep_bytes[64] = {};
+
+/* the tables of model m in b: from the call workspace, or (persistent, for objects
+   that keep them, such as a batch) in allocations of their own */
+int prepare(const mgx_model *m, const mgx_options *op, hipStream_t st, DevBuffers &b, KParams &kp,
+            bool persistent = false)
 {
-    const size_t PL = (size_t)m->npoints * m->nlines;
-    HIP_TRY(hipMalloc(&b.lpack, PL * sizeof(double4)));
-    HIP_TRY(hipMalloc(&b.pts, (size_t)m->npoints * sizeof(PointRec)));
-    HIP_TRY(hipMalloc(&b.pidx, (size_t)m->npoints * sizeof(PointIdx)));
-    HIP_TRY(hipMalloc(&b.status, 4 * sizeof(uint32_t)));
-    HIP_TRY(hipMalloc(&b.kp_dev, sizeof(KParams)));
+    const size_t P = m->npoints, PL = (size_t)m->npoints * m->nlines;
+    if (persistent) {
+        HIP_TRY(hipMalloc(&b.lpack, PL * sizeof(double4)));
+        HIP_TRY(hipMalloc(&b.pts, P * sizeof(PointRec)));
+        HIP_TRY(hipMalloc(&b.pidx, P * sizeof(PointIdx)));
+        HIP_TRY(hipMalloc(&b.status, 4 * sizeof(uint32_t)));
+        HIP_TRY(hipMalloc(&b.kp_dev, sizeof(KParams)));
+        return prepare_tables(m, op, st, b, kp);
+    }
+    const auto rnd = [](size_t n) { return (n + 255) & ~(size_t)255; };
+    const size_t need = rnd(PL * sizeof(double4)) + rnd(P * sizeof(PointRec)) +
+                        rnd(P * sizeof(PointIdx)) + rnd(4 * sizeof(uint32_t)) +
+                        rnd(sizeof(KParams)) + rnd(sizeof(ImageFrame));
+    b.ws_lock = std::unique_lock<std::mutex>(g_prep_mutex);
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return mgx::fail(MGX_E_ARG, "device index %d out of range", dev);
+    if (g_prep_bytes[dev] < need) {
+        /* the previous holder has returned, so nothing uses the old buffer */
+        if (g_prep[dev]) (void)hipFree(g_prep[dev]);
+        g_prep[dev] = nullptr;
+        g_prep_bytes[dev] = 0;
+        HIP_TRY(hipMalloc(&g_prep[dev], need));
+        g_prep_bytes[dev] = need;
+    }
+    unsigned char *q = g_prep[dev];
+    b.owned = false;
+    b.lpack = reinterpret_cast<double4 *>(q);
+    q += rnd(PL * sizeof(double4));
+    b.pts = reinterpret_cast<PointRec *>(q);
+    q += rnd(P * sizeof(PointRec));
+    b.pidx = reinterpret_cast<PointIdx *>(q);
+    q += rnd(P * sizeof(PointIdx));
+    b.status = reinterpret_cast<uint32_t *>(q);
+    q += rnd(4 * sizeof(uint32_t));
+    b.kp_dev = reinterpret_cast<KParams *>(q);
+    q += rnd(sizeof(KParams));
+    b.frame = reinterpret_cast<ImageFrame *>(q);
     return prepare_tables(m, op, st, b, kp);
 }
 
@@ -3774,19 +3858,104 @@ int read_status(hipStream_t st, const DevBuffers &b, uint32_t *need, uint32_t *n
     return MGX_OK;
 }
 
-/* LDS per workgroup available to dynamic allocations: 64 KiB less the static LDS */
-const size_t kLdsBudget = 65536 - kStaticLds;
+/* LDS per workgroup available to dynamic allocations: the device's limit (160 KiB
+   on gfx950) less the static LDS, queried once per device */
+size_t lds_budget()
+{
+    static size_t cached[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    if (!cached[dev]) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+            v <= 0)
+            v = 65536;
+        cached[dev] = (size_t)v - kStaticLds;
+    }
+    return cached[dev];
+}
 
+/* the largest walk capacity whose buffers fit the LDS budget beside extra bytes */
+uint32_t max_walk_cap(size_t extra)
+{
+    const size_t budget = lds_budget();
+    uint32_t c = kMaxCap;
+    while (c > 16 && walk_lds_bytes(c) + extra > budget) c -= 16;
+    return c;
+}
+
+/* Walk capacities that solved without a relaunch, by geometry and ray set: a hint
+   for the next solve of the same model (a model changed in place under the same
+   arrays costs at most a relaunch). */
+struct WalkKey {
+    const void *pos, *nbs, *dirs;
+    uint32_t P, nrays, adaptive;
+    bool operator<(const WalkKey &o) const
+    {
+        return std::tie(pos, nbs, dirs, P, nrays, adaptive) <
+               std::tie(o.pos, o.nbs, o.dirs, o.P, o.nrays, o.adaptive);
+    }
+};
+std::mutex g_cap_mutex;
+std::map<WalkKey, uint32_t> g_cap_hint;
+
+WalkKey walk_key(const mgx_model &m)
+{
+    return WalkKey{m.position, m.neighbors, m.ray_direction, m.npoints, m.nrays,
+                   (uint32_t)m.adaptive_directions};
+}
+
+/*
+ * Launch loop of the solves: sets the LDS walk capacity, launches, and relaunches
+ * with a larger capacity when a walk outgrew it.  With pair_walks (the launch walks
+ * the model's ray pairs from every origin) the first capacity is the one that
+ * served this geometry and ray set before, or else the longest half-ray that
+ * walk_max_kernel finds, so no launch is aborted for want of walk buffer.
+ */
 template <typename Launch>
 int launch_with_capacity(hipStream_t st, DevBuffers &b, KParams &kp, Launch launch,
-                         size_t extra_lds = 0)
+                         size_t extra_lds = 0, bool pair_walks = false)
 {
     uint32_t cap = 128;
-    double total_ms = 0.0; /* kernel time summed over the relaunches */
+    double total_ms = 0.0; /* kernel time summed over the launches */
     struct Report {
         double &t;
         ~Report() { g_last_kernel_ms = t; }
     } report{total_ms};
+    const uint32_t max_cap = max_walk_cap(8 + extra_lds);
+    const WalkKey key = walk_key(kp.m);
+    if (pair_walks) {
+        bool have = false;
+        {
+            std::lock_guard<std::mutex> lk(g_cap_mutex);
+            auto it = g_cap_hint.find(key);
+            if (it != g_cap_hint.end()) {
+                cap = it->second;
+                have = true;
+            }
+        }
+        if (!have) {
+            HIP_TRY(hipMemsetAsync(b.status, 0, 4 * sizeof(uint32_t), st));
+            HIP_TRY(hipMemcpyAsync(b.kp_dev, &kp, sizeof(KParams), hipMemcpyHostToDevice, st));
+            g_last_kernel_ms = 0.0;
+            timed(st, [&] {
+                hipLaunchKernelGGL(walk_max_kernel, dim3(kp.m.npoints, kp.m.hnrays), dim3(MGX_WAVE), 0,
+                                   st, b.kp_dev, b.status + 3);
+            });
+            total_ms += g_last_kernel_ms;
+            HIP_TRY(hipGetLastError());
+            uint32_t h[4] = {0, 0, 0, 0};
+            HIP_TRY(hipMemcpyAsync(h, b.status, sizeof h, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            if (h[0] & ERR_NO_NEXT)
+                return mgx::fail(MGX_E_NO_NEXT, "a ray from an interior point found no valid next point");
+            if (h[3] > max_cap)
+                return mgx::fail(MGX_E_RAY_LENGTH, "a ray needs more than %u walk steps", max_cap);
+            cap = (h[3] + 15) / 16 * 16;
+            cap = cap < 16 ? 16 : cap;
+        }
+        cap = cap > max_cap ? max_cap : cap;
+    }
     for (;;) {
         kp.cap = cap;
         kp.j_lds = j_in_lds(cap, kp.m.nfreqs, kp.m.nlines) ? 1 : 0;
@@ -3798,9 +3967,12 @@ int launch_with_capacity(hipStream_t st, DevBuffers &b, KParams &kp, Launch laun
         HIP_TRY(hipGetLastError());
         uint32_t need = 0;
         const int e = read_status(st, b, &need);
+        if (e == MGX_OK && pair_walks && !t_no_events) {
+            std::lock_guard<std::mutex> lk(g_cap_mutex);
+            if (g_cap_hint.size() > 4096) g_cap_hint.clear();
+            g_cap_hint[key] = cap;
+        }
         if (e != MGX_E_RAY_LENGTH) return e;
-        uint32_t max_cap = kMaxCap;
-        while (max_cap > 16 && walk_lds_bytes(max_cap) + 8 + extra_lds > kLdsBudget) max_cap -= 16;
         if (cap >= max_cap)
             return mgx::fail(MGX_E_RAY_LENGTH, "a ray needs more than %u walk steps", max_cap);
         cap = cap * 2 > max_cap ? max_cap : cap * 2;
@@ -3809,7 +3981,7 @@ int launch_with_capacity(hipStream_t st, DevBuffers &b, KParams &kp, Launch laun
 
 /* argument checks, model preparation and the frame of the projection-surface
    imager (Model::compute_image_new, model.cpp:1008-1035; the frame reduction
-   runs on the device); dfr is a device ImageFrame the caller frees */
+   runs on the device); dfr is the device ImageFrame of b */
 int image_setup(const mgx_model *model, const mgx_options *opts, const double *raydir, uint32_t nx,
                 uint32_t ny, hipStream_t st, DevBuffers &b, KParams &kp, ImageFrame *&dfr)
 {
@@ -3836,7 +4008,7 @@ int image_setup(const mgx_model *model, const mgx_options *opts, const double *r
     fr.nx = nx;
     fr.ny = ny;
     if (int e = prepare(model, opts, st, b, kp)) return e;
-    HIP_TRY(hipMalloc(&dfr, sizeof(ImageFrame)));
+    dfr = b.frame;
     HIP_TRY(hipMemcpyAsync(dfr, &fr, sizeof fr, hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(image_frame_kernel, dim3(1), dim3(256), 0, st, *model, dfr);
     HIP_TRY(hipGetLastError());
@@ -3910,8 +4082,7 @@ int adjoint_launch(hipStream_t st, DevBuffers &b, KParams &kp, double *d_eps, do
         if (e != MGX_OK && e != MGX_E_RAY_LENGTH) return e;
         bool retry = false;
         if (e == MGX_E_RAY_LENGTH) {
-            uint32_t max_cap = kMaxCap;
-            while (max_cap > 16 && walk_lds_bytes(max_cap) + 8 > kLdsBudget) max_cap -= 16;
+            const uint32_t max_cap = max_walk_cap(8);
             if (cap >= max_cap)
                 return mgx::fail(MGX_E_RAY_LENGTH, "a ray needs more than %u walk steps", max_cap);
             cap = cap * 2 > max_cap ? max_cap : cap * 2;
@@ -4276,9 +4447,7 @@ int batch_execute(mgx_batch &B, hipStream_t st)
             return mgx::fail(MGX_E_UNSUPPORTED,
                              "a segment needs more than 65535 interpolation points");
         if (!(h[0] & ERR_CAP)) return MGX_OK;
-        uint32_t max_cap = kMaxCap;
-        const size_t extra = 8 + 8 * (size_t)B.maxL;
-        while (max_cap > 16 && walk_lds_bytes(max_cap) + extra > kLdsBudget) max_cap -= 16;
+        const uint32_t max_cap = max_walk_cap(8 + 8 * (size_t)B.maxL);
         if (B.cap >= max_cap)
             return mgx::fail(MGX_E_RAY_LENGTH, "a ray needs more than %u walk steps", max_cap);
         B.cap = B.cap * 2 > max_cap ? max_cap : B.cap * 2;
@@ -4360,11 +4529,23 @@ namespace {
 
 /* the plain-method solve on prepared buffers (shared by the allocating and the
    no-allocation entry points) */
-int forward_launch(const mgx_model *model, const mgx_options *opts, double *u, double *J,
+int forward_launch(const mgx_model *model, const mgx_options *opts, double *u, double *J_out,
                    double *lambda_diag, hipStream_t st, DevBuffers &b, KParams &kp,
-                   double *Jline = nullptr)
+                   double *Jline = nullptr, double **J_scratch = nullptr)
 {
-    return launch_with_capacity(st, b, kp, [&](size_t) {
+    bool scratch_failed = false;
+    const int e = launch_with_capacity(st, b, kp, [&](size_t) {
+        /* the line-integrated solve keeps J in LDS, or else in a scratch J (P, F) */
+        double *J = J_out;
+        if (Jline && !kp.j_lds) {
+            if (!*J_scratch &&
+                hipMalloc(J_scratch, (size_t)model->npoints * model->nfreqs * sizeof(double)) !=
+                    hipSuccess) {
+                scratch_failed = true;
+                return;
+            }
+            J = *J_scratch;
+        }
         const dim3 grid(model->npoints), block(MGX_WAVE);
         const size_t lds = solve_lds_bytes(kp.cap, model->nfreqs, model->nlines, kp.j_lds != 0);
         timed(st, [&] {
@@ -4395,7 +4576,9 @@ int forward_launch(const mgx_model *model, const mgx_options *opts, double *u, d
                                        grid, block, lds, st, b.kp_dev, u, J, lambda_diag, Jline);
             }
         });
-    }, 8 * (size_t)model->nlines);
+    }, 8 * (size_t)model->nlines, true);
+    if (scratch_failed) return mgx::fail(MGX_E_HIP, "allocation of the J scratch failed");
+    return e;
 }
 
 } // namespace
@@ -4440,15 +4623,14 @@ int mgx_feautrier_sparse_device(const mgx_model *model, const mgx_options *opts,
     DevBuffers b;
     KParams kp;
     if (int e = prepare(model, opts, st, b, kp)) return e;
-    /* J of an origin stays in LDS; when it does not fit, its row of a scratch J */
+    /* J of an origin stays in LDS; when it does not fit (long walks), its row of a
+       scratch J, allocated by the launch that needs it */
     double *Jtmp = nullptr;
     struct Free {
         double *&p;
         ~Free() { if (p) (void)hipFree(p); }
     } free_tmp{Jtmp};
-    if (!j_in_lds(kMaxCap, model->nfreqs, model->nlines))
-        HIP_TRY(hipMalloc(&Jtmp, (size_t)model->npoints * model->nfreqs * sizeof(double)));
-    return forward_launch(model, opts, nullptr, Jtmp, lambda_diag, st, b, kp, J_line);
+    return forward_launch(model, opts, nullptr, nullptr, lambda_diag, st, b, kp, J_line, &Jtmp);
 }
 
 static size_t noalloc_scratch_bytes(const mgx_model *model)
@@ -4549,7 +4731,7 @@ int mgx_feautrier_forward_lambda_device(const mgx_model *model, const mgx_option
                                        grid, block, lds, st, b.kp_dev, u, J, lb);
             }
         });
-    }, 8 * (size_t)model->nlines);
+    }, 8 * (size_t)model->nlines, true);
 }
 
 int mgx_shortchar_order_0_device(const mgx_model *model, const mgx_options *opts, double min_dtau,
@@ -4596,7 +4778,7 @@ int mgx_shortchar_order_0_device(const mgx_model *model, const mgx_options *opts
                                        grid, block, lds, st, b.kp_dev, sc, u, J, lambda_diag);
             }
         });
-    }, 8 * (size_t)model->nlines);
+    }, 8 * (size_t)model->nlines, true);
 }
 
 int mgx_feautrier_order_2_uv_device(const mgx_model *model, const mgx_options *opts, double *u,
@@ -4640,7 +4822,7 @@ int mgx_feautrier_order_2_uv_device(const mgx_model *model, const mgx_options *o
                                        grid, block, lds, st, b.kp_dev, u, v);
             }
         });
-    }, 8 * (size_t)model->nlines);
+    }, 8 * (size_t)model->nlines, true);
 }
 
 
@@ -4654,28 +4836,26 @@ int mgx_image_device(const mgx_model *model, const mgx_options *opts, const doub
     KParams kp;
     ImageFrame *dfr = nullptr;
     if (int e = image_setup(model, opts, raydir, nx, ny, st, b, kp, dfr)) return e;
-    struct Free {
-        ImageFrame *p;
-        ~Free() { (void)hipFree(p); }
-    } free_fr{dfr};
     return launch_with_capacity(st, b, kp, [&](size_t lds) {
-        const dim3 grid(nx * ny), block(MGX_WAVE);
+        /* two pixels per wave when a half-wave covers the frequencies */
+        const uint32_t ppw = model->nfreqs <= 32 ? 2 : 1;
+        const dim3 grid((nx * ny + ppw - 1) / ppw), block(MGX_WAVE);
         timed(st, [&] {
             const bool fp64 = opts->precision == MGX_PREC_FP64;
             if (opts->approx == MGX_APPROX_NONE) {
                 if (fp64)
                     hipLaunchKernelGGL((image_kernel<MGX_APPROX_NONE, MGX_PREC_FP64, IMG_INTENSITY>), grid, block,
-                                       lds, st, b.kp_dev, dfr, I, ImX, ImY);
+                                       lds, st, b.kp_dev, dfr, I, ImX, ImY, ppw);
                 else
                     hipLaunchKernelGGL((image_kernel<MGX_APPROX_NONE, MGX_PREC_MIXED, IMG_INTENSITY>), grid, block,
-                                       lds, st, b.kp_dev, dfr, I, ImX, ImY);
+                                       lds, st, b.kp_dev, dfr, I, ImX, ImY, ppw);
             } else {
                 if (fp64)
                     hipLaunchKernelGGL((image_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_FP64, IMG_INTENSITY>), grid,
-                                       block, lds, st, b.kp_dev, dfr, I, ImX, ImY);
+                                       block, lds, st, b.kp_dev, dfr, I, ImX, ImY, ppw);
                 else
                     hipLaunchKernelGGL((image_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_MIXED, IMG_INTENSITY>), grid,
-                                       block, lds, st, b.kp_dev, dfr, I, ImX, ImY);
+                                       block, lds, st, b.kp_dev, dfr, I, ImX, ImY, ppw);
             }
         });
     });
@@ -4692,30 +4872,28 @@ int mgx_image_optical_depth_device(const mgx_model *model, const mgx_options *op
     KParams kp;
     ImageFrame *dfr = nullptr;
     if (int e = image_setup(model, opts, raydir, nx, ny, st, b, kp, dfr)) return e;
-    struct Free {
-        ImageFrame *p;
-        ~Free() { (void)hipFree(p); }
-    } free_fr{dfr};
     return launch_with_capacity(st, b, kp, [&](size_t lds) {
-        const dim3 grid(nx * ny), block(MGX_WAVE);
+        /* two pixels per wave when a half-wave covers the frequencies */
+        const uint32_t ppw = model->nfreqs <= 32 ? 2 : 1;
+        const dim3 grid((nx * ny + ppw - 1) / ppw), block(MGX_WAVE);
         timed(st, [&] {
             const bool fp64 = opts->precision == MGX_PREC_FP64;
             if (opts->approx == MGX_APPROX_NONE) {
                 if (fp64)
                     hipLaunchKernelGGL((image_kernel<MGX_APPROX_NONE, MGX_PREC_FP64, IMG_OPTICAL_DEPTH>),
-                                       grid, block, lds, st, b.kp_dev, dfr, tau, ImX, ImY);
+                                       grid, block, lds, st, b.kp_dev, dfr, tau, ImX, ImY, ppw);
                 else
                     hipLaunchKernelGGL((image_kernel<MGX_APPROX_NONE, MGX_PREC_MIXED, IMG_OPTICAL_DEPTH>),
-                                       grid, block, lds, st, b.kp_dev, dfr, tau, ImX, ImY);
+                                       grid, block, lds, st, b.kp_dev, dfr, tau, ImX, ImY, ppw);
             } else {
                 if (fp64)
                     hipLaunchKernelGGL(
                         (image_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_FP64, IMG_OPTICAL_DEPTH>), grid,
-                        block, lds, st, b.kp_dev, dfr, tau, ImX, ImY);
+                        block, lds, st, b.kp_dev, dfr, tau, ImX, ImY, ppw);
                 else
                     hipLaunchKernelGGL(
                         (image_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_MIXED, IMG_OPTICAL_DEPTH>), grid,
-                        block, lds, st, b.kp_dev, dfr, tau, ImX, ImY);
+                        block, lds, st, b.kp_dev, dfr, tau, ImX, ImY, ppw);
             }
         });
     });
@@ -4791,16 +4969,15 @@ static int image_adjoint_entry(const mgx_model *model, const mgx_options *opts,
     KParams kp;
     ImageFrame *dfr = nullptr;
     if (int e = image_setup(model, opts, raydir, nx, ny, st, b, kp, dfr)) return e;
-    struct Free {
-        ImageFrame *p;
-        ~Free() { (void)hipFree(p); }
-    } free_fr{dfr};
     const size_t PL = (size_t)model->npoints * model->nlines;
     const bool fp64 = opts->precision == MGX_PREC_FP64;
     auto run = [&](auto kern) {
         return adjoint_launch(
             st, b, kp, d_eps, d_kap, d_iw, PL, 1,
-            [&](size_t lds) { return persistent_waves(kern, lds, nx * ny); },
+            [&](size_t lds) {
+                const uint32_t ppw = model->nfreqs <= 32 ? 2 : 1; /* pixels per wave */
+                return persistent_waves(kern, lds, (nx * ny + ppw - 1) / ppw);
+            },
             [&](size_t lds, uint32_t grid, uint32_t capE) {
                 timed(st, [&] {
                     hipLaunchKernelGGL(kern, dim3(grid), dim3(MGX_WAVE), lds, st, b.kp_dev, dfr,
@@ -4953,7 +5130,7 @@ int mgx_batch_create(const mgx_model *models, uint32_t n, const mgx_options *opt
         B->views.push_back(models[i]);
         B->bufs.push_back(new DevBuffers());
         KParams kp;
-        if (int e = prepare(&models[i], opts, st, *B->bufs.back(), kp)) return fail(e);
+        if (int e = prepare(&models[i], opts, st, *B->bufs.back(), kp, true)) return fail(e);
         kp.status = B->status;
         B->kps.push_back(kp);
         B->J.push_back(J[i]);
@@ -4966,6 +5143,24 @@ int mgx_batch_create(const mgx_model *models, uint32_t n, const mgx_options *opt
         (B->with_lam && hipMemcpyAsync(B->lam_dev, B->lam.data(), n * sizeof(double *),
                                        hipMemcpyHostToDevice, st) != hipSuccess))
         return fail(mgx::fail(MGX_E_HIP, "batch upload failed"));
+    if (int e = batch_upload(*B, st)) return fail(e);
+    /* the walk buffer from the longest half-ray of any model (walk_max_kernel) */
+    if (hipMemsetAsync(B->status, 0, 4 * sizeof(uint32_t), st) != hipSuccess)
+        return fail(mgx::fail(MGX_E_HIP, "batch status reset failed"));
+    for (uint32_t i = 0; i < n; i++)
+        hipLaunchKernelGGL(walk_max_kernel, dim3(models[i].npoints, models[i].hnrays), dim3(MGX_WAVE),
+                           0, st, B->kps_dev + i, B->status + 3);
+    uint32_t h[4] = {0, 0, 0, 0};
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(h, B->status, sizeof h, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return fail(mgx::fail(MGX_E_HIP, "batch walk-length pass failed"));
+    if (h[0] & ERR_NO_NEXT)
+        return fail(mgx::fail(MGX_E_NO_NEXT, "a ray from an interior point found no valid next point"));
+    const uint32_t max_cap = max_walk_cap(8 + 8 * (size_t)B->maxL);
+    if (h[3] > max_cap)
+        return fail(mgx::fail(MGX_E_RAY_LENGTH, "a ray needs more than %u walk steps", max_cap));
+    B->cap = std::max<uint32_t>(16, (h[3] + 15) / 16 * 16);
     if (int e = batch_upload(*B, st)) return fail(e);
     *out = B;
     return MGX_OK;

@@ -1,13 +1,15 @@
-"""Rays longer than the kernels' first walk buffer (128 steps per half ray).
+"""Rays longer than the kernels' first walk buffer.
 
-The reference sizes its scratch with a tracing pass; the kernels start with a
-128-step LDS buffer per half ray and relaunch with twice the buffer when a walk
-does not fit, up to 1280 steps (the LDS a workgroup may hold beside the erf
-table), then fail with MGX_E_RAY_LENGTH.  On a 1D chain of 1000 points the
-half rays from the ends have 999 steps: the forward solve (u, J and Lambda within
-1e-10 of the oracle) and the adjoint (whose per-wave tape also grows) must come
-out as on short rays, and so must the imager's pixel rays through 600 spherical
-shells (~1200 steps); a chain of 2000 points must fail cleanly with the
+The reference sizes its scratch with a tracing pass; the solves here size the
+LDS walk buffer from a walk-only pass (the longest half ray of the model) and
+the imager and adjoints start at 128 steps and relaunch with twice the buffer
+when a walk does not fit, up to what the device's LDS per workgroup holds
+(160 KiB on gfx950: about 3360 steps), then fail with MGX_E_RAY_LENGTH.  On a
+1D chain of 1000 points the half rays from the ends have 999 steps: the forward
+solve (u, J and Lambda within 1e-10 of the oracle) and the adjoint (whose
+per-wave tape also grows) must come out as on short rays, and so must the
+imager's pixel rays through 600 spherical shells (~1200 steps) and a chain of
+2000 points (1999 steps); a chain of 4000 points must fail cleanly with the
 library's message.
 """
 import numpy as np
@@ -78,11 +80,26 @@ def test_long_image_rays_through_many_shells():
     assert e < 1e-10
 
 
+def test_chain_of_2000_points():
+    """1999-step half rays: beyond a 64 KiB walk buffer, within gfx950's 160 KiB."""
+    from magritte_amd import solver
+    from oracle import oracle
+
+    m = models.model_line_1d(npoints=2000, nquads=3)
+    op = m.options()
+    print("[long rays] chain P=2000", flush=True)
+    out = solver.feautrier_forward(solver.DeviceModel(m), op, with_u=False)
+    _, J_r, lam_r = oracle.solve(m, op, with_u=False)
+    e_J, e_L = _rel(out["J"].cpu().numpy(), J_r), _rel(out["lambda_diag"].cpu().numpy(), lam_r)
+    print(f"  J {e_J:.2e}  Lambda {e_L:.2e}", flush=True)
+    assert e_J < 1e-10 and e_L < 1e-10
+
+
 def test_too_long_chain_fails_cleanly():
     from magritte_amd import solver
 
-    m = models.model_line_1d(npoints=2000, nquads=3)
-    print("[long rays] chain P=2000 (beyond the walk buffer)", flush=True)
+    m = models.model_line_1d(npoints=4000, nquads=3)
+    print("[long rays] chain P=4000 (beyond the walk buffer)", flush=True)
     dm = solver.DeviceModel(m)
     with pytest.raises(_abi.MgxError, match="walk steps"):
         solver.feautrier_forward(dm, m.options())
