@@ -377,12 +377,21 @@ __device__ inline double interp_linear(double a, double b, double t)
     return (b - a) * t + a;
 }
 
+/* the powf path of interpolate_log, taken only by interpolated entries: out of line,
+   so that its registers do not count against the solve's loop (with the reference's
+   double-internal powf inline, the FP64 forward kernel spilled 40 VGPRs) */
+template <int PREC>
+__device__ __attribute__((noinline)) double interp_log_pow(double a, double b, double t)
+{
+    return a * (double)f_pow<PREC>((float)(b / a), (float)t);
+}
+
 template <int PREC>
 __device__ inline double interp_log(double a, double b, double t)
 {
     if (t == 0.0) return a;
     if (t == 1.0) return b;
-    return a * (double)f_pow<PREC>((float)(b / a), (float)t);
+    return interp_log_pow<PREC>(a, b, t);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -3264,6 +3273,17 @@ __device__ __forceinline__ void reverse_half(const KParams &kp, const double *s_
     const double *q = store.st + (size_t)MGX_TAPE_WORDS * MGX_WAVE * (K - 1);
     double Y = q[0], X = q[MGX_WAVE], d = q[2 * MGX_WAVE], kind = q[3 * MGX_WAVE],
            phi = q[4 * MGX_WAVE];
+    /* the record of the segment after next, loaded one iteration ahead of its use so
+       that the global load's latency overlaps a segment's reverse */
+    double fY = 0.0, fX = 0.0, fd = 0.0, fk = 0.0, fp = 0.0;
+    if (K >= 2) {
+        const double *qf = store.st + (size_t)MGX_TAPE_WORDS * MGX_WAVE * (K - 2);
+        fY = qf[0];
+        fX = qf[MGX_WAVE];
+        fd = qf[2 * MGX_WAVE];
+        fk = qf[3 * MGX_WAVE];
+        fp = qf[4 * MGX_WAVE];
+    }
     for (uint32_t k = K; k-- > 0;) {
         const Entry outer = hi_entry(it, o);
         const double dz = it.dZ;
@@ -3273,12 +3293,19 @@ __device__ __forceinline__ void reverse_half(const KParams &kp, const double *s_
         const double Din = fma(-X * XI, u_in, Y); /* u_k - u_in */
         double lam_k, dbar_next = 0.0, Yn = 0.0, Xn = 0.0, dn = 0.0, kn = 0.0, pn = 0.0;
         if (k > 0) {
-            const double *qn = store.st + (size_t)MGX_TAPE_WORDS * MGX_WAVE * (k - 1);
-            Yn = qn[0];
-            Xn = qn[MGX_WAVE];
-            dn = qn[2 * MGX_WAVE];
-            kn = qn[3 * MGX_WAVE];
-            pn = qn[4 * MGX_WAVE];
+            Yn = fY;
+            Xn = fX;
+            dn = fd;
+            kn = fk;
+            pn = fp;
+            if (k >= 2) {
+                const double *qf = store.st + (size_t)MGX_TAPE_WORDS * MGX_WAVE * (k - 2);
+                fY = qf[0];
+                fX = qf[MGX_WAVE];
+                fd = qf[2 * MGX_WAVE];
+                fk = qf[3 * MGX_WAVE];
+                fp = qf[4 * MGX_WAVE];
+            }
             lam_k = 0.5 * (dn + d) * mu_k;
             /* u_k - u_{k-1} = X_{k-1} XI_{k-1} u_k - Y_{k-1} */
             const double XIn = rcp(1.0 + Xn);
