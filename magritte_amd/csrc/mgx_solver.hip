@@ -196,21 +196,72 @@ __device__ inline double shift_rest_ray(const KParams &kp, const double *origin,
 /* ------------------------------------------------------------------------- */
 /* next point on the ray, wave-cooperative                                    */
 
-/* lexicographic arg-min over the wave: smaller d2 wins, ties go to the smaller list index */
-__device__ inline void wave_argmin(double &d2, uint32_t &idx, double &zn)
+/* Cross-lane moves within a row of 16 lanes through DPP (every lane of the row
+   active): the value of the lane that the pattern CTRL selects.  The patterns
+   used here: 0xb1 quad_perm [1, 0, 3, 2] (lane ^ 1), 0x4e quad_perm [2, 3, 0, 1]
+   (lane ^ 2), 0x141 row_half_mirror (lane i of 8 takes lane 7 - i), 0x140
+   row_mirror (lane i of 16 takes lane 15 - i).  After the first two steps every
+   lane of a quad holds the quad's result, so the two mirrors pair quads and then
+   half-rows like lane ^ 4 and lane ^ 8 would. */
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_get(uint32_t v)
 {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        const double od2 = __shfl_xor(d2, off, MGX_WAVE);
-        const uint32_t oidx = __shfl_xor(idx, off, MGX_WAVE);
-        const double ozn = __shfl_xor(zn, off, MGX_WAVE);
-        const bool take = (od2 < d2) || (od2 == d2 && oidx < idx);
-        if (take) {
-            d2 = od2;
-            idx = oidx;
-            zn = ozn;
-        }
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xf, 0xf, false);
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_get(double v)
+{
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint64_t lo = dpp_get<CTRL>((uint32_t)b), hi = dpp_get<CTRL>((uint32_t)(b >> 32));
+    return __builtin_bit_cast(double, (hi << 32) | lo);
+}
+
+__device__ __forceinline__ double read_lane(double v, uint32_t l)
+{
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint64_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, (int)l);
+    const uint64_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), (int)l);
+    return __builtin_bit_cast(double, (hi << 32) | lo);
+}
+
+/* lexicographic arg-min step: smaller d2 wins, ties go to the smaller list index */
+__device__ __forceinline__ void argmin_take(double &d2, uint32_t &idx, double &zn, double od2,
+                                            uint32_t oidx, double ozn)
+{
+    const bool take = (od2 < d2) || (od2 == d2 && oidx < idx);
+    if (take) {
+        d2 = od2;
+        idx = oidx;
+        zn = ozn;
     }
+}
+
+template <int CTRL>
+__device__ __forceinline__ void argmin_dpp(double &d2, uint32_t &idx, double &zn)
+{
+    argmin_take(d2, idx, zn, dpp_get<CTRL>(d2), dpp_get<CTRL>(idx), dpp_get<CTRL>(zn));
+}
+
+/* Lexicographic arg-min over the wave, in every lane; only lanes below n_cand
+   hold candidates (the others hold d2 = inf).  The arg-min is unique (the list
+   indices differ), so the order of the pairings does not change it. */
+__device__ inline void wave_argmin(double &d2, uint32_t &idx, double &zn, uint32_t n_cand)
+{
+    argmin_dpp<0xb1>(d2, idx, zn);
+    argmin_dpp<0x4e>(d2, idx, zn);
+    argmin_dpp<0x141>(d2, idx, zn);
+    argmin_dpp<0x140>(d2, idx, zn);
+    /* every lane of a row holds the row's arg-min: combine the rows with candidates */
+    const uint32_t rows = n_cand > 48 ? 4 : n_cand > 32 ? 3 : n_cand > 16 ? 2 : 1;
+    double b_d2 = read_lane(d2, 0), b_zn = read_lane(zn, 0);
+    uint32_t b_idx = (uint32_t)__builtin_amdgcn_readlane((int)idx, 0);
+    for (uint32_t r = 1; r < rows; r++)
+        argmin_take(b_d2, b_idx, b_zn, read_lane(d2, 16 * r),
+                    (uint32_t)__builtin_amdgcn_readlane((int)idx, (int)(16 * r)), read_lane(zn, 16 * r));
+    d2 = b_d2;
+    idx = b_idx;
+    zn = b_zn;
 }
 
 /* get_next_general_geometry<Defaulttracer> (geometry.tpp:14-44), all lanes return the same */
@@ -239,7 +290,7 @@ __device__ inline uint32_t next_general(const KParams &kp, const double *origin,
             }
         }
     }
-    wave_argmin(best_d2, best_i, best_zn);
+    wave_argmin(best_d2, best_i, best_zn, pi.n_nbs);
     if (best_i == 0xffffffffu) return kp.m.npoints;
     dZ = best_zn - Z;
     Z += dZ;
@@ -1976,7 +2027,7 @@ __device__ inline uint32_t bdy_closer(const mgx_model &m, const double *origin, 
             }
         }
     }
-    wave_argmin(best, key, dummy);
+    wave_argmin(best, key, dummy, n_nbs);
     return (key == 0 || key == 0xffffffffu) ? crt : m.neighbors[cum + key - 1];
 }
 
@@ -2152,14 +2203,14 @@ __device__ bool image_walk(const KParams &kp, const ImageFrame &fr, uint32_t pix
             shift0 = shift_rest(m, origin, rd, crt, Z - dZ);
             double shift_nxt = shift_rest(m, origin, rd, nxt, Z);
             for (;;) {
-                if (S >= cap) {
-                    if (lane == 0) {
-                        atomicOr(&kp.status[0], (uint32_t)ERR_CAP);
-                        atomicMax(&kp.status[1], cap + 1);
-                    }
+                /* past the buffer the walk goes on uncounted in LDS, so that the
+                   relaunch gets the ray's exact length; a chord crosses every point
+                   at most twice (each shell, under spherical symmetry) */
+                if (S > 2 * m.npoints + 2) {
+                    if (lane == 0) atomicOr(&kp.status[0], (uint32_t)ERR_NO_NEXT);
                     return false;
                 }
-                if (lane == 0) {
+                if (S < cap && lane == 0) {
                     s_nxt[S] = nxt;
                     s_dZ[S] = dZ;
                     s_sh[S] = shift_nxt;
@@ -2178,6 +2229,13 @@ __device__ bool image_walk(const KParams &kp, const ImageFrame &fr, uint32_t pix
             }
         }
         if (!accessed_non_boundary && !m.spherical_symmetry) S = 0;
+    }
+    if (S > cap) {
+        if (lane == 0) {
+            atomicOr(&kp.status[0], (uint32_t)ERR_CAP);
+            atomicMax(&kp.status[1], S);
+        }
+        return false;
     }
     __syncthreads();
     for (uint32_t i = lane; i < S; i += MGX_WAVE) {
@@ -2267,13 +2325,17 @@ __device__ bool image_pixels(const KParams &kp, const ImageFrame &fr, uint32_t p
 {
     const uint32_t lane = threadIdx.x & (MGX_WAVE - 1);
     PixelWalk w0 = pixel_slot(smem, kp.cap, 0);
-    if (!image_walk(kp, fr, pix0, w0, ImX, ImY)) return false;
+    const bool ok0 = image_walk(kp, fr, pix0, w0, ImX, ImY);
     if (npix == 1) {
+        if (!ok0) return false;
         image_freqs<APPROX, PREC, KIND>(kp, w0, pix0, lane, MGX_WAVE, I);
         return true;
     }
+    /* the second ray is walked even when the first outgrew the buffer, so that
+       its length is reported too */
     PixelWalk w1 = pixel_slot(smem, kp.cap, 1);
-    if (!image_walk(kp, fr, pix0 + 1, w1, ImX, ImY)) return false;
+    const bool ok1 = image_walk(kp, fr, pix0 + 1, w1, ImX, ImY);
+    if (!ok0 || !ok1) return false;
     const uint32_t half = lane >> 5;
     image_freqs<APPROX, PREC, KIND>(kp, half ? w1 : w0, pix0 + half, lane & 31, 32, I);
     return true;
@@ -2647,12 +2709,37 @@ __device__ inline void segment_tape(const KParams &kp, const double *erf_tab, co
     g = MGX_SEG_TRAPEZOID;
 }
 
-/* Tape of one half ray: segment k (counted from the outer end) stores Y, X,
-   dtau, the segment kind g and the profile value phi of its outer entry,
-   lane-interleaved: st[(5 k + c) * 64] (st already offset by the lane). */
+/* Tape of one half ray: segment k (counted from the outer end) stores five
+   doubles, Y, X, dtau, the segment kind g and the profile value phi of its
+   outer entry, each lane-interleaved (64 lanes side by side) within the
+   segment's record.  The words stay doubles in mixed mode too: the reverse
+   forms u_k - u_{k+1} from Y and X, which cancels where u is smooth, and the
+   inverse-width gradient sums terms of both signs over the profile, so a float
+   tape moved it by up to 6e-3 relative. */
 #define MGX_TAPE_WORDS 5
+struct Tape {
+    /* bytes of one segment's record (all 64 lanes) */
+    static constexpr size_t kRecord = (size_t)MGX_WAVE * 8 * MGX_TAPE_WORDS;
+    static __device__ __forceinline__ size_t offset(uint32_t k, uint32_t c, uint32_t lane)
+    {
+        return kRecord * k + (size_t)MGX_WAVE * 8 * c + 8 * lane;
+    }
+    static __device__ __forceinline__ void put(unsigned char *base, uint32_t k, uint32_t c,
+                                               uint32_t lane, double v)
+    {
+        *reinterpret_cast<double *>(base + offset(k, c, lane)) = v;
+    }
+    static __device__ __forceinline__ double get(const unsigned char *base, uint32_t k, uint32_t c,
+                                                 uint32_t lane)
+    {
+        return *reinterpret_cast<const double *>(base + offset(k, c, lane));
+    }
+};
+
+template <int PREC>
 struct SweepStore {
-    double *st;
+    unsigned char *st; /* the half's tape (records of all 64 lanes) */
+    uint32_t lane;
     uint32_t cap; /* segments that fit */
     uint32_t n;   /* segments of the half (may exceed cap: nothing is stored past it) */
     double I_b;   /* boundary intensity of the outer boundary row */
@@ -2665,7 +2752,7 @@ __device__ inline void sweep_half_tape(const KParams &kp, const double *s_erf, c
                                        double shift_centre, const uint32_t *s_nxt,
                                        const double *s_dZ, const double *s_sh,
                                        const uint16_t *mtab, uint32_t l, double freq, Sweep &sw,
-                                       PointECT &ec_centre, SweepStore &store)
+                                       PointECT &ec_centre, SweepStore<PREC> &store)
 {
     HalfIter it;
     it.s = (int)S - 1;
@@ -2712,12 +2799,12 @@ __device__ inline void sweep_half_tape(const KParams &kp, const double *s_erf, c
         sw.S_inner = S_in;
         sw.S_prev_row = S_in;
         if (store.n < store.cap) {
-            double *q = store.st + (size_t)MGX_TAPE_WORDS * MGX_WAVE * store.n;
-            q[0] = sw.Y;
-            q[MGX_WAVE] = sw.X;
-            q[2 * MGX_WAVE] = dtau;
-            q[3 * MGX_WAVE] = g;
-            q[4 * MGX_WAVE] = ec_out.valid ? ec_out.phi : __builtin_nan("");
+            typedef Tape T;
+            T::put(store.st, store.n, 0, store.lane, sw.Y);
+            T::put(store.st, store.n, 1, store.lane, sw.X);
+            T::put(store.st, store.n, 2, store.lane, dtau);
+            T::put(store.st, store.n, 3, store.lane, g);
+            T::put(store.st, store.n, 4, store.lane, ec_out.valid ? ec_out.phi : __builtin_nan(""));
         }
         store.n++;
         outer = inner;
@@ -2775,6 +2862,22 @@ __device__ inline void sink_add(GradSink &g, uint32_t p, uint32_t l, double de, 
     }
 }
 
+/* The sum of v over the aligned group of width lanes (2, 4, ..., 64) that holds
+   the lane, in every lane of the group; all lanes of the group active.  Within a
+   row of 16 the partners come through DPP (lane ^ 1, lane ^ 2, then the mirror
+   images within 8 and 16 lanes, which pair the partial sums the same way), past
+   it through the LDS crossbar. */
+__device__ __forceinline__ double group_sum(double v, uint32_t width)
+{
+    v += dpp_get<0xb1>(v); /* quad_perm [1, 0, 3, 2] */
+    if (width > 2) v += dpp_get<0x4e>(v); /* quad_perm [2, 3, 0, 1] */
+    if (width > 4) v += dpp_get<0x141>(v); /* row_half_mirror */
+    if (width > 8) v += dpp_get<0x140>(v); /* row_mirror */
+    if (width > 16) v += __shfl_xor(v, 16, MGX_WAVE);
+    if (width > 32) v += __shfl_xor(v, 32, MGX_WAVE);
+    return v;
+}
+
 /* all lanes of a reduction group call this together */
 template <bool IW = false>
 __device__ inline void sink_flush(const GradSink &g, uint32_t p, double e, double k, double w)
@@ -2783,11 +2886,9 @@ __device__ inline void sink_flush(const GradSink &g, uint32_t p, double e, doubl
         sink_put<IW>(g, p, g.line, e, k, w);
         return;
     }
-    for (uint32_t off = g.width >> 1; off >= 1; off >>= 1) {
-        e += __shfl_xor(e, (int)off, MGX_WAVE);
-        k += __shfl_xor(k, (int)off, MGX_WAVE);
-        if (IW) w += __shfl_xor(w, (int)off, MGX_WAVE);
-    }
+    e = group_sum(e, g.width);
+    k = group_sum(k, g.width);
+    if (IW) w = group_sum(w, g.width);
     if ((threadIdx.x & (g.width - 1)) == 0) sink_put<IW>(g, p, g.line, e, k, w);
 }
 
@@ -3248,7 +3349,7 @@ __device__ __forceinline__ void reverse_half(const KParams &kp, const double *s_
                              bool backward, uint32_t S, uint32_t o, double shift0,
                              double shift_centre, const uint32_t *s_nxt, const double *s_dZ,
                              const double *s_sh, const uint16_t *mtab, uint32_t l, double freq,
-                             const SweepStore store, const RevCentre rc, GradSink g)
+                             const SweepStore<PREC> store, const RevCentre rc, GradSink g)
 {
     const uint32_t K = store.n;
     const uint32_t lown = kp.m.sorted_line_map[ol.g];
@@ -3270,19 +3371,20 @@ __device__ __forceinline__ void reverse_half(const KParams &kp, const double *s_
     double u_in = rc.u, mu_in = rc.mu, lam_in = rc.lam, dbar = rc.dbar;
     EntryBar ein = {rc.ec.eta, rc.ec.chi, rc.ec.valid ? rc.ec.phi : __builtin_nan(""), 0.0, 0.0,
                     rc.ec.valid};
-    const double *q = store.st + (size_t)MGX_TAPE_WORDS * MGX_WAVE * (K - 1);
-    double Y = q[0], X = q[MGX_WAVE], d = q[2 * MGX_WAVE], kind = q[3 * MGX_WAVE],
-           phi = q[4 * MGX_WAVE];
+    typedef Tape T;
+    const uint32_t ln = store.lane;
+    double Y = T::get(store.st, K - 1, 0, ln), X = T::get(store.st, K - 1, 1, ln),
+           d = T::get(store.st, K - 1, 2, ln), kind = T::get(store.st, K - 1, 3, ln),
+           phi = T::get(store.st, K - 1, 4, ln);
     /* the record of the segment after next, loaded one iteration ahead of its use so
        that the global load's latency overlaps a segment's reverse */
     double fY = 0.0, fX = 0.0, fd = 0.0, fk = 0.0, fp = 0.0;
     if (K >= 2) {
-        const double *qf = store.st + (size_t)MGX_TAPE_WORDS * MGX_WAVE * (K - 2);
-        fY = qf[0];
-        fX = qf[MGX_WAVE];
-        fd = qf[2 * MGX_WAVE];
-        fk = qf[3 * MGX_WAVE];
-        fp = qf[4 * MGX_WAVE];
+        fY = T::get(store.st, K - 2, 0, ln);
+        fX = T::get(store.st, K - 2, 1, ln);
+        fd = T::get(store.st, K - 2, 2, ln);
+        fk = T::get(store.st, K - 2, 3, ln);
+        fp = T::get(store.st, K - 2, 4, ln);
     }
     for (uint32_t k = K; k-- > 0;) {
         const Entry outer = hi_entry(it, o);
@@ -3299,12 +3401,11 @@ __device__ __forceinline__ void reverse_half(const KParams &kp, const double *s_
             kn = fk;
             pn = fp;
             if (k >= 2) {
-                const double *qf = store.st + (size_t)MGX_TAPE_WORDS * MGX_WAVE * (k - 2);
-                fY = qf[0];
-                fX = qf[MGX_WAVE];
-                fd = qf[2 * MGX_WAVE];
-                fk = qf[3 * MGX_WAVE];
-                fp = qf[4 * MGX_WAVE];
+                fY = T::get(store.st, k - 2, 0, ln);
+                fX = T::get(store.st, k - 2, 1, ln);
+                fd = T::get(store.st, k - 2, 2, ln);
+                fk = T::get(store.st, k - 2, 3, ln);
+                fp = T::get(store.st, k - 2, 4, ln);
             }
             lam_k = 0.5 * (dn + d) * mu_k;
             /* u_k - u_{k-1} = X_{k-1} XI_{k-1} u_k - Y_{k-1} */
@@ -3428,7 +3529,7 @@ __device__ inline CentreRow centre_row(const KParams &kp, uint32_t o, double fre
  * Adjoint of the formal solve: given J_bar (P, F), adds (dJ/d eps)^T J_bar and
  * (dJ/d kappa)^T J_bar into g_eps, g_kap (P, L).  Persistent grid: wave b takes
  * origins b, b + G, ...; it keeps the elimination state of its current ray
- * pair in its own slice of the scratch (2 halves x capE segments x 3 x 64).
+ * pair in its own slice of the scratch (2 halves x capE tape records).
  * The lane layout is the forward kernel's (two 32-lane line slots); lanes past
  * the last quadrature bin repeat it with a zero seed so every slot reduces its
  * gradients over 32 lanes.
@@ -3453,7 +3554,8 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_ADJ_WAVES_PER_SIMD) void feautrier_ad
     double *s_sh0 = s_dZ1 + cap;
     double *s_sh1 = s_sh0 + cap;
     uint16_t *s_m = reinterpret_cast<uint16_t *>(s_sh1 + cap);
-    double *st_wave = scratch + (size_t)blockIdx.x * 2 * capE * MGX_TAPE_WORDS * MGX_WAVE;
+    unsigned char *st_wave = reinterpret_cast<unsigned char *>(scratch) +
+                             (size_t)blockIdx.x * 2 * capE * Tape::kRecord;
 
     for (uint32_t o = blockIdx.x; o < P; o += gridDim.x) {
         /* a buffer of this launch overflowed: the host clears the gradients and
@@ -3509,8 +3611,8 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_ADJ_WAVES_PER_SIMD) void feautrier_ad
                 PointECT ec_cent = {0.0, 0.0, 0.0, false};
                 Sweep bw, fw;
                 bw.started = fw.started = false;
-                SweepStore st0{st_wave + lane, capE, 0, 0.0},
-                    st1{st_wave + (size_t)capE * MGX_TAPE_WORDS * MGX_WAVE + lane, capE, 0, 0.0};
+                SweepStore<PREC> st0{st_wave, lane, capE, 0, 0.0},
+                    st1{st_wave + (size_t)capE * Tape::kRecord, lane, capE, 0, 0.0};
                 if (S1 > 0)
                     sweep_half_tape<APPROX, PREC>(kp, s_erf, ol, true, S1, o, sh01, 1.0, s_nxt1,
                                                   s_dZ1, s_sh1, s_m + (jj * 2 + 1) * cap, l, freq,
@@ -3571,7 +3673,8 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_ADJ_WAVES_PER_SIMD) void image_adjoin
     /* two pixels per wave when a half-wave covers the frequencies (image_pixels) */
     const uint32_t ppw = F <= 32 ? 2 : 1;
     const uint32_t nitems = (npix + ppw - 1) / ppw;
-    double *st_wave = scratch + (size_t)blockIdx.x * capE * MGX_TAPE_WORDS * MGX_WAVE;
+    unsigned char *st_wave = reinterpret_cast<unsigned char *>(scratch) +
+                             (size_t)blockIdx.x * capE * Tape::kRecord;
 
     for (uint32_t q = blockIdx.x; q < nitems; q += gridDim.x) {
         /* a buffer of this launch overflowed: the host clears the gradients and
@@ -3584,8 +3687,9 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_ADJ_WAVES_PER_SIMD) void image_adjoin
         const uint32_t n = (p0 + ppw <= npix) ? ppw : npix - p0;
         __syncthreads();
         PixelWalk w0 = pixel_slot(smem, kp.cap, 0), w1 = pixel_slot(smem, kp.cap, 1);
-        if (!image_walk(kp, fr, p0, w0, nullptr, nullptr)) return;
-        if (n == 2 && !image_walk(kp, fr, p0 + 1, w1, nullptr, nullptr)) return;
+        const bool ok0 = image_walk(kp, fr, p0, w0, nullptr, nullptr);
+        const bool ok1 = n < 2 || image_walk(kp, fr, p0 + 1, w1, nullptr, nullptr);
+        if (!ok0 || !ok1) return;
         /* the lanes of a group (the wave, or a half-wave per pixel) take one pixel */
         const uint32_t gw = (n == 2) ? 32 : 64;
         const uint32_t half = (n == 2) ? (lane >> 5) : 0;
@@ -3605,7 +3709,7 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_ADJ_WAVES_PER_SIMD) void image_adjoin
             Sweep fw;
             fw.started = false;
             PointECT ec = {0.0, 0.0, 0.0, false};
-            SweepStore st{st_wave + lane, capE, 0, 0.0};
+            SweepStore<PREC> st{st_wave, lane, capE, 0, 0.0};
             sweep_half_tape<APPROX, PREC>(kp, s_erf, ol, false, S, ip, w.shift0, w.shift_centre,
                                           w.nxt, w.dZ, w.sh, w.m, line, freq, fw, ec, st);
             if (st.n > capE) {
@@ -3917,10 +4021,13 @@ uint32_t max_walk_cap(size_t extra)
 struct WalkKey {
     const void *pos, *nbs, *dirs;
     uint32_t P, nrays, adaptive;
+    double rd[3];    /* image: ray direction (zero for the ray pairs of a solve) */
+    uint32_t nx, ny; /* image: pixels (zero for a solve) */
     bool operator<(const WalkKey &o) const
     {
-        return std::tie(pos, nbs, dirs, P, nrays, adaptive) <
-               std::tie(o.pos, o.nbs, o.dirs, o.P, o.nrays, o.adaptive);
+        return std::tie(pos, nbs, dirs, P, nrays, adaptive, rd[0], rd[1], rd[2], nx, ny) <
+               std::tie(o.pos, o.nbs, o.dirs, o.P, o.nrays, o.adaptive, o.rd[0], o.rd[1], o.rd[2],
+                        o.nx, o.ny);
     }
 };
 std::mutex g_cap_mutex;
@@ -3929,7 +4036,47 @@ std::map<WalkKey, uint32_t> g_cap_hint;
 WalkKey walk_key(const mgx_model &m)
 {
     return WalkKey{m.position, m.neighbors, m.ray_direction, m.npoints, m.nrays,
-                   (uint32_t)m.adaptive_directions};
+                   (uint32_t)m.adaptive_directions, {0.0, 0.0, 0.0}, 0, 0};
+}
+
+/* the key of a projection-surface image's pixel rays */
+WalkKey image_key(const mgx_model &m, const double *raydir, uint32_t nx, uint32_t ny)
+{
+    WalkKey k = walk_key(m);
+    k.dirs = nullptr;
+    k.nrays = 0;
+    k.adaptive = 0;
+    for (int i = 0; i < 3; i++) k.rd[i] = raydir[i];
+    k.nx = nx;
+    k.ny = ny;
+    return k;
+}
+
+bool cap_hint(const WalkKey &key, uint32_t &cap)
+{
+    std::lock_guard<std::mutex> lk(g_cap_mutex);
+    auto it = g_cap_hint.find(key);
+    if (it == g_cap_hint.end()) return false;
+    cap = it->second;
+    return true;
+}
+
+void set_cap_hint(const WalkKey &key, uint32_t cap)
+{
+    std::lock_guard<std::mutex> lk(g_cap_mutex);
+    if (g_cap_hint.size() > 4096) g_cap_hint.clear();
+    g_cap_hint[key] = cap;
+}
+
+/* The walk capacity after a launch whose walks outgrew cap: need is the largest
+   length a walk reported.  The pixel walks of the imager report their exact
+   length, the ray-pair walks cap + 1 (they stop at the buffer's end); the
+   capacity then doubles.  0 when no capacity up to max_cap can hold the walk. */
+uint32_t grown_cap(uint32_t cap, uint32_t need, uint32_t max_cap)
+{
+    if (need > max_cap || cap >= max_cap) return 0;
+    const uint32_t want = (need > cap + 1) ? (need + 15) / 16 * 16 : cap * 2;
+    return want > max_cap ? max_cap : want;
 }
 
 /*
@@ -3941,7 +4088,7 @@ WalkKey walk_key(const mgx_model &m)
  */
 template <typename Launch>
 int launch_with_capacity(hipStream_t st, DevBuffers &b, KParams &kp, Launch launch,
-                         size_t extra_lds = 0, bool pair_walks = false)
+                         size_t extra_lds = 0, bool pair_walks = false, const WalkKey *img = nullptr)
 {
     uint32_t cap = 128;
     double total_ms = 0.0; /* kernel time summed over the launches */
@@ -3950,18 +4097,11 @@ int launch_with_capacity(hipStream_t st, DevBuffers &b, KParams &kp, Launch laun
         ~Report() { g_last_kernel_ms = t; }
     } report{total_ms};
     const uint32_t max_cap = max_walk_cap(8 + extra_lds);
-    const WalkKey key = walk_key(kp.m);
-    if (pair_walks) {
-        bool have = false;
-        {
-            std::lock_guard<std::mutex> lk(g_cap_mutex);
-            auto it = g_cap_hint.find(key);
-            if (it != g_cap_hint.end()) {
-                cap = it->second;
-                have = true;
-            }
-        }
-        if (!have) {
+    const WalkKey key = img ? *img : walk_key(kp.m);
+    const bool hinted = pair_walks || img;
+    if (hinted) {
+        const bool have = cap_hint(key, cap);
+        if (!have && pair_walks) {
             HIP_TRY(hipMemsetAsync(b.status, 0, 4 * sizeof(uint32_t), st));
             HIP_TRY(hipMemcpyAsync(b.kp_dev, &kp, sizeof(KParams), hipMemcpyHostToDevice, st));
             g_last_kernel_ms = 0.0;
@@ -3994,15 +4134,10 @@ int launch_with_capacity(hipStream_t st, DevBuffers &b, KParams &kp, Launch laun
         HIP_TRY(hipGetLastError());
         uint32_t need = 0;
         const int e = read_status(st, b, &need);
-        if (e == MGX_OK && pair_walks && !t_no_events) {
-            std::lock_guard<std::mutex> lk(g_cap_mutex);
-            if (g_cap_hint.size() > 4096) g_cap_hint.clear();
-            g_cap_hint[key] = cap;
-        }
+        if (e == MGX_OK && hinted && !t_no_events) set_cap_hint(key, cap);
         if (e != MGX_E_RAY_LENGTH) return e;
-        if (cap >= max_cap)
-            return mgx::fail(MGX_E_RAY_LENGTH, "a ray needs more than %u walk steps", max_cap);
-        cap = cap * 2 > max_cap ? max_cap : cap * 2;
+        cap = grown_cap(cap, need, max_cap);
+        if (!cap) return mgx::fail(MGX_E_RAY_LENGTH, "a ray needs more than %u walk steps", max_cap);
     }
 }
 
@@ -4084,18 +4219,20 @@ uint32_t persistent_waves(K kernel, size_t lds, uint32_t work)
  * receives the dynamic LDS size; grid_of(lds) the persistent grid size.
  */
 template <typename Launch, typename Grid>
-int adjoint_launch(hipStream_t st, DevBuffers &b, KParams &kp, double *d_eps, double *d_kap,
-                   double *d_iw, size_t PL, size_t per_wave_segments_factor, Grid grid_of,
-                   Launch launch)
+int adjoint_launch(hipStream_t st, DevBuffers &b, KParams &kp, const WalkKey &key, double *d_eps,
+                   double *d_kap, double *d_iw, size_t PL, size_t per_wave_segments_factor,
+                   Grid grid_of, Launch launch)
 {
+    const uint32_t max_cap = max_walk_cap(8);
     uint32_t cap = 128;
+    (void)cap_hint(key, cap); /* the walks of the forward solve or image */
+    if (cap > max_cap) cap = max_cap;
     uint32_t capE = g_capE_hint;
     for (;;) {
         kp.cap = cap;
         const size_t lds = walk_lds_bytes(cap);
         const uint32_t grid = grid_of(lds);
-        const size_t bytes = (size_t)grid * per_wave_segments_factor * capE * MGX_TAPE_WORDS *
-                             MGX_WAVE * sizeof(double);
+        const size_t bytes = (size_t)grid * per_wave_segments_factor * capE * Tape::kRecord;
         if (int e = ensure_workspace(bytes)) return e;
         HIP_TRY(hipMemsetAsync(b.status, 0, 4 * sizeof(uint32_t), st));
         HIP_TRY(hipMemsetAsync(d_eps, 0, PL * sizeof(double), st));
@@ -4109,10 +4246,8 @@ int adjoint_launch(hipStream_t st, DevBuffers &b, KParams &kp, double *d_eps, do
         if (e != MGX_OK && e != MGX_E_RAY_LENGTH) return e;
         bool retry = false;
         if (e == MGX_E_RAY_LENGTH) {
-            const uint32_t max_cap = max_walk_cap(8);
-            if (cap >= max_cap)
-                return mgx::fail(MGX_E_RAY_LENGTH, "a ray needs more than %u walk steps", max_cap);
-            cap = cap * 2 > max_cap ? max_cap : cap * 2;
+            cap = grown_cap(cap, need, max_cap);
+            if (!cap) return mgx::fail(MGX_E_RAY_LENGTH, "a ray needs more than %u walk steps", max_cap);
             retry = true;
         }
         if (need_e) {
@@ -4124,7 +4259,10 @@ int adjoint_launch(hipStream_t st, DevBuffers &b, KParams &kp, double *d_eps, do
             g_capE_hint = c;
             retry = true;
         }
-        if (!retry) return MGX_OK;
+        if (!retry) {
+            if (!t_no_events) set_cap_hint(key, cap);
+            return MGX_OK;
+        }
     }
 }
 
@@ -4135,7 +4273,7 @@ int run_J_adjoint_k(hipStream_t st, DevBuffers &b, KParams &kp, const double *J_
     auto kern = feautrier_adjoint_kernel<A, PR, IW>;
     const uint32_t P = kp.m.npoints;
     return adjoint_launch(
-        st, b, kp, d_eps, d_kap, d_iw, (size_t)P * kp.m.nlines, 2,
+        st, b, kp, walk_key(kp.m), d_eps, d_kap, d_iw, (size_t)P * kp.m.nlines, 2,
         [&](size_t lds) { return persistent_waves(kern, lds, P); },
         [&](size_t lds, uint32_t grid, uint32_t capE) {
             timed(st, [&] {
@@ -4475,9 +4613,9 @@ int batch_execute(mgx_batch &B, hipStream_t st)
                              "a segment needs more than 65535 interpolation points");
         if (!(h[0] & ERR_CAP)) return MGX_OK;
         const uint32_t max_cap = max_walk_cap(8 + 8 * (size_t)B.maxL);
-        if (B.cap >= max_cap)
-            return mgx::fail(MGX_E_RAY_LENGTH, "a ray needs more than %u walk steps", max_cap);
-        B.cap = B.cap * 2 > max_cap ? max_cap : B.cap * 2;
+        const uint32_t cap = grown_cap(B.cap, h[1], max_cap);
+        if (!cap) return mgx::fail(MGX_E_RAY_LENGTH, "a ray needs more than %u walk steps", max_cap);
+        B.cap = cap;
         batch_drop_graph(B);
         if (int e = batch_upload(B, st)) return e;
     }
@@ -4863,6 +5001,7 @@ int mgx_image_device(const mgx_model *model, const mgx_options *opts, const doub
     KParams kp;
     ImageFrame *dfr = nullptr;
     if (int e = image_setup(model, opts, raydir, nx, ny, st, b, kp, dfr)) return e;
+    const WalkKey ikey = image_key(*model, raydir, nx, ny);
     return launch_with_capacity(st, b, kp, [&](size_t lds) {
         /* two pixels per wave when a half-wave covers the frequencies */
         const uint32_t ppw = model->nfreqs <= 32 ? 2 : 1;
@@ -4885,7 +5024,7 @@ int mgx_image_device(const mgx_model *model, const mgx_options *opts, const doub
                                        block, lds, st, b.kp_dev, dfr, I, ImX, ImY, ppw);
             }
         });
-    });
+    }, 0, false, &ikey);
 }
 
 int mgx_image_optical_depth_device(const mgx_model *model, const mgx_options *opts,
@@ -4899,6 +5038,7 @@ int mgx_image_optical_depth_device(const mgx_model *model, const mgx_options *op
     KParams kp;
     ImageFrame *dfr = nullptr;
     if (int e = image_setup(model, opts, raydir, nx, ny, st, b, kp, dfr)) return e;
+    const WalkKey ikey = image_key(*model, raydir, nx, ny);
     return launch_with_capacity(st, b, kp, [&](size_t lds) {
         /* two pixels per wave when a half-wave covers the frequencies */
         const uint32_t ppw = model->nfreqs <= 32 ? 2 : 1;
@@ -4923,7 +5063,7 @@ int mgx_image_optical_depth_device(const mgx_model *model, const mgx_options *op
                         block, lds, st, b.kp_dev, dfr, tau, ImX, ImY, ppw);
             }
         });
-    });
+    }, 0, false, &ikey);
 }
 
 int mgx_image_ray_device(const mgx_model *model, const mgx_options *opts, uint32_t rr, double *I,
@@ -4998,9 +5138,10 @@ static int image_adjoint_entry(const mgx_model *model, const mgx_options *opts,
     if (int e = image_setup(model, opts, raydir, nx, ny, st, b, kp, dfr)) return e;
     const size_t PL = (size_t)model->npoints * model->nlines;
     const bool fp64 = opts->precision == MGX_PREC_FP64;
+    const WalkKey ikey = image_key(*model, raydir, nx, ny);
     auto run = [&](auto kern) {
         return adjoint_launch(
-            st, b, kp, d_eps, d_kap, d_iw, PL, 1,
+            st, b, kp, ikey, d_eps, d_kap, d_iw, PL, 1,
             [&](size_t lds) {
                 const uint32_t ppw = model->nfreqs <= 32 ? 2 : 1; /* pixels per wave */
                 return persistent_waves(kern, lds, (nx * ny + ppw - 1) / ppw);
