@@ -2863,18 +2863,30 @@ __device__ inline void sink_add(GradSink &g, uint32_t p, uint32_t l, double de, 
 }
 
 /* The sum of v over the aligned group of width lanes (2, 4, ..., 64) that holds
-   the lane, in every lane of the group; all lanes of the group active.  Within a
-   row of 16 the partners come through DPP (lane ^ 1, lane ^ 2, then the mirror
-   images within 8 and 16 lanes, which pair the partial sums the same way), past
-   it through the LDS crossbar. */
+   the lane, valid in the group's last lane; all lanes of the group active.
+   Within a row of 16 the partners come through DPP (lane ^ 1, lane ^ 2, then the
+   mirror images within 8 and 16 lanes, which pair the partial sums the same
+   way); rows are then joined by DPP broadcasts, row_bcast15 (0x142: lane 15 of
+   row r into row r + 1, here rows 1 and 3) and row_bcast31 (0x143: lane 31 into
+   rows 2 and 3), which leave the sum in the group's last row. */
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_rows(double v)
+{
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint64_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, ROWS, 0xf, false);
+    const uint64_t hi =
+        (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, ROWS, 0xf, false);
+    return __builtin_bit_cast(double, (hi << 32) | lo);
+}
+
 __device__ __forceinline__ double group_sum(double v, uint32_t width)
 {
     v += dpp_get<0xb1>(v); /* quad_perm [1, 0, 3, 2] */
     if (width > 2) v += dpp_get<0x4e>(v); /* quad_perm [2, 3, 0, 1] */
     if (width > 4) v += dpp_get<0x141>(v); /* row_half_mirror */
     if (width > 8) v += dpp_get<0x140>(v); /* row_mirror */
-    if (width > 16) v += __shfl_xor(v, 16, MGX_WAVE);
-    if (width > 32) v += __shfl_xor(v, 32, MGX_WAVE);
+    if (width > 16) v += dpp_rows<0x142, 0xa>(v); /* rows 1, 3 += lane 15, 47 */
+    if (width > 32) v += dpp_rows<0x143, 0xc>(v); /* rows 2, 3 += lane 31 */
     return v;
 }
 
@@ -2889,7 +2901,7 @@ __device__ inline void sink_flush(const GradSink &g, uint32_t p, double e, doubl
     e = group_sum(e, g.width);
     k = group_sum(k, g.width);
     if (IW) w = group_sum(w, g.width);
-    if ((threadIdx.x & (g.width - 1)) == 0) sink_put<IW>(g, p, g.line, e, k, w);
+    if ((threadIdx.x & (g.width - 1)) == g.width - 1) sink_put<IW>(g, p, g.line, e, k, w);
 }
 
 /* the walk moves outward to a step whose points are (crt = old nxt, nxt) */
