@@ -4240,6 +4240,11 @@ int adjoint_launch(hipStream_t st, DevBuffers &b, KParams &kp, const WalkKey &ke
     (void)cap_hint(key, cap); /* the walks of the forward solve or image */
     if (cap > max_cap) cap = max_cap;
     uint32_t capE = g_capE_hint;
+    double total_ms = 0.0; /* kernel time summed over the launches */
+    struct Report {
+        double &t;
+        ~Report() { g_last_kernel_ms = t; }
+    } report{total_ms};
     for (;;) {
         kp.cap = cap;
         const size_t lds = walk_lds_bytes(cap);
@@ -4251,7 +4256,9 @@ int adjoint_launch(hipStream_t st, DevBuffers &b, KParams &kp, const WalkKey &ke
         HIP_TRY(hipMemsetAsync(d_kap, 0, PL * sizeof(double), st));
         if (d_iw) HIP_TRY(hipMemsetAsync(d_iw, 0, PL * sizeof(double), st));
         HIP_TRY(hipMemcpyAsync(b.kp_dev, &kp, sizeof(KParams), hipMemcpyHostToDevice, st));
+        g_last_kernel_ms = 0.0;
         launch(lds, grid, capE);
+        total_ms += g_last_kernel_ms;
         HIP_TRY(hipGetLastError());
         uint32_t need = 0, need_e = 0;
         const int e = read_status(st, b, &need, &need_e);
