@@ -2982,12 +2982,12 @@ __device__ inline double dtau_line_dw(double avg_iw, double avg_op, double linef
 template <int PREC>
 __device__ inline void dtau_iw_bwd(double *g_iw, uint32_t L, uint32_t cp, uint32_t cpi,
                                    uint32_t np, uint32_t npi, uint32_t l, double4 c, double4 cI,
-                                   double4 n, double4 nI, double linefreq, double cf, double nf,
-                                   double ct, double nt, double dz, double bar)
+                                   double4 n, double4 nI, double op_c, double op_n, double linefreq,
+                                   double cf, double nf, double ct, double nt, double dz, double bar)
 {
     if (bar == 0.0) return;
     const double avg_iw = (interp_linear(c.x, cI.x, ct) + interp_linear(n.x, nI.x, nt)) / 2.0;
-    const double avg_op = (interp_log<PREC>(c.z, cI.z, ct) + interp_log<PREC>(n.z, nI.z, nt)) / 2.0;
+    const double avg_op = (op_c + op_n) / 2.0;
     const double wb = 0.5 * bar * dtau_line_dw(avg_iw, avg_op, linefreq, cf, nf, dz);
     iw_put(g_iw, L, cp, cpi, l, ct, wb);
     iw_put(g_iw, L, np, npi, l, nt, wb);
@@ -3049,16 +3049,15 @@ __device__ double dtau_single_line_dop(const double *erf_tab, double4 c, double4
     return dz * (avg_iw * ((double)erf_diff * (0.5 * rcp(diff_pos))));
 }
 
-/* reverse of the line terms at one end of a segment: opacity (kappa interpolation)
-   and line source (interpolation of eps/kappa), added directly */
-template <int PREC>
+/* reverse of the line terms at one end of a segment: opacity (kappa interpolation,
+   value op) and line source (interpolation of eps/kappa, value S), added directly */
 __device__ inline void line_end_bwd(double *g_eps, double *g_kap, uint32_t L, uint32_t p,
                                     uint32_t pi, uint32_t l, double t, double4 a, double4 b,
-                                    double op_bar, double S_bar)
+                                    double op, double S, double op_bar, double S_bar)
 {
     double dk1, dk2, dR1, dR2;
-    interp_log_partials(a.z, b.z, t, interp_log<PREC>(a.z, b.z, t), dk1, dk2);
-    interp_log_partials(a.w, b.w, t, interp_log<PREC>(a.w, b.w, t), dR1, dR2);
+    interp_log_partials(a.z, b.z, t, op, dk1, dk2);
+    interp_log_partials(a.w, b.w, t, S, dR1, dR2);
     /* R = eps/kappa: dR/deps = 1/kappa, dR/dkappa = -R/kappa */
     const double r1 = S_bar * dR1 * rcp(a.z);
     grad_put(g_eps, g_kap, L, p, l, r1, op_bar * dk1 - r1 * a.w);
@@ -3086,11 +3085,16 @@ __device__ __attribute__((noinline)) void S_dtau_line_integrated_bwd(
         const double dop = dtau_single_line_dop<PREC>(erf_tab, c, cI, n, nI, m.line[lineidx],
                                                       currfreq, nextfreq, curr_interp,
                                                       next_interp, dZ);
-        line_end_bwd<PREC>(g_eps, g_kap, L, cp, cpi, lineidx, curr_interp, c, cI, 0.5 * dop * dtau_bar, Sc_bar);
-        line_end_bwd<PREC>(g_eps, g_kap, L, np, npi, lineidx, next_interp, n, nI, 0.5 * dop * dtau_bar, Sn_bar);
+        const double op_c = interp_log<PREC>(c.z, cI.z, curr_interp);
+        const double op_n = interp_log<PREC>(n.z, nI.z, next_interp);
+        line_end_bwd(g_eps, g_kap, L, cp, cpi, lineidx, curr_interp, c, cI, op_c,
+                     interp_log<PREC>(c.w, cI.w, curr_interp), 0.5 * dop * dtau_bar, Sc_bar);
+        line_end_bwd(g_eps, g_kap, L, np, npi, lineidx, next_interp, n, nI, op_n,
+                     interp_log<PREC>(n.w, nI.w, next_interp), 0.5 * dop * dtau_bar, Sn_bar);
         if (IW)
-            dtau_iw_bwd<PREC>(g_iw, L, cp, cpi, np, npi, lineidx, c, cI, n, nI, m.line[lineidx],
-                              currfreq, nextfreq, curr_interp, next_interp, dZ, dtau_bar);
+            dtau_iw_bwd<PREC>(g_iw, L, cp, cpi, np, npi, lineidx, c, cI, n, nI, op_c, op_n,
+                              m.line[lineidx], currfreq, nextfreq, curr_interp, next_interp, dZ,
+                              dtau_bar);
         return;
     }
     uint32_t lo = 0, hi = m.nlines;
@@ -3152,11 +3156,13 @@ __device__ __attribute__((noinline)) void S_dtau_line_integrated_bwd(
         /* one line: unclamped, S = S_l exactly and the adjoint of the line depth is
            dtau_bar; clamped, the depth is the bound and S scales with the line depth */
         const double ld_bar = clamped ? sSc_bar * Sc + sSn_bar * Sn : dtau_bar;
-        line_end_bwd<PREC>(g_eps, g_kap, L, cp, cpi, l, curr_interp, c, cI, 0.5 * dop * ld_bar, sSc_bar * line_dtau);
-        line_end_bwd<PREC>(g_eps, g_kap, L, np, npi, l, next_interp, n, nI, 0.5 * dop * ld_bar, sSn_bar * line_dtau);
+        line_end_bwd(g_eps, g_kap, L, cp, cpi, l, curr_interp, c, cI, op_c, Sc, 0.5 * dop * ld_bar,
+                     sSc_bar * line_dtau);
+        line_end_bwd(g_eps, g_kap, L, np, npi, l, next_interp, n, nI, op_n, Sn, 0.5 * dop * ld_bar,
+                     sSn_bar * line_dtau);
         if (IW)
-            dtau_iw_bwd<PREC>(g_iw, L, cp, cpi, np, npi, l, c, cI, n, nI, linefreq, currfreq,
-                              nextfreq, curr_interp, next_interp, dZ, ld_bar);
+            dtau_iw_bwd<PREC>(g_iw, L, cp, cpi, np, npi, l, c, cI, n, nI, op_c, op_n, linefreq,
+                              currfreq, nextfreq, curr_interp, next_interp, dZ, ld_bar);
         return;
     }
     /* forward sums (the same operations as S_dtau_line_integrated) */
@@ -3194,14 +3200,18 @@ __device__ __attribute__((noinline)) void S_dtau_line_integrated_bwd(
                                                         next_interp, dZ);
         const double Sc = interp_log<PREC>(c.w, cI.w, curr_interp);
         const double Sn = interp_log<PREC>(n.w, nI.w, next_interp);
+        const double op_c = interp_log<PREC>(c.z, cI.z, curr_interp);
+        const double op_n = interp_log<PREC>(n.z, nI.z, next_interp);
         const double ld_bar = sdt_bar + sSc_bar * Sc + sSn_bar * Sn;
         const double dop = dtau_single_line_dop<PREC>(erf_tab, c, cI, n, nI, m.line[l], currfreq,
                                                       nextfreq, curr_interp, next_interp, dZ);
-        line_end_bwd<PREC>(g_eps, g_kap, L, cp, cpi, l, curr_interp, c, cI, 0.5 * dop * ld_bar, sSc_bar * line_dtau);
-        line_end_bwd<PREC>(g_eps, g_kap, L, np, npi, l, next_interp, n, nI, 0.5 * dop * ld_bar, sSn_bar * line_dtau);
+        line_end_bwd(g_eps, g_kap, L, cp, cpi, l, curr_interp, c, cI, op_c, Sc, 0.5 * dop * ld_bar,
+                     sSc_bar * line_dtau);
+        line_end_bwd(g_eps, g_kap, L, np, npi, l, next_interp, n, nI, op_n, Sn, 0.5 * dop * ld_bar,
+                     sSn_bar * line_dtau);
         if (IW)
-            dtau_iw_bwd<PREC>(g_iw, L, cp, cpi, np, npi, l, c, cI, n, nI, m.line[l], currfreq,
-                              nextfreq, curr_interp, next_interp, dZ, ld_bar);
+            dtau_iw_bwd<PREC>(g_iw, L, cp, cpi, np, npi, l, c, cI, n, nI, op_c, op_n, m.line[l],
+                              currfreq, nextfreq, curr_interp, next_interp, dZ, ld_bar);
     }
 }
 
