@@ -2406,42 +2406,43 @@ __global__ __launch_bounds__(MGX_WAVE) void image_batch_kernel(
     }
 }
 
-/*
- * The old imager at every model point along ray rr (Solver::image_feautrier_order_2
- * and Solver::image_optical_depth, solver.tpp:543-574, 656-686).  One wave per
- * point o: the ray pair (rr, antipod) is walked with the Rest-frame shifts of
- * trace_ray<Rest, false> (reversed on rays >= hnrays) and the all-lines
- * interpolation count of every step; lanes take frequencies and run over the
- * whole ray from the rr end (first) to the antipod end (last): the rr half
- * inward, the origin, then the antipod half outward.  KIND selects
- * I(o, f) = 2 Su[last] - B(nr[last], nu(o, f)) (the boundary term at the
- * unshifted frequency, as the reference) or the summed optical depth.  ImX, ImY:
- * the point's coordinates on the image plane
- * (Image::set_coordinates_all_model_points, image.cpp:154-212).
- */
-template <int APPROX, int PREC, int KIND>
-__global__ __launch_bounds__(MGX_WAVE) void image_ray_kernel(const KParams *__restrict__ kpp,
-                                                             uint32_t rr, double *I, double *ImX,
-                                                             double *ImY)
-{
-    mgx_ref_tables_init();
-    const KParams &kp = *kpp;
-    extern __shared__ unsigned char smem[];
-    const double *s_erf = nullptr; /* erf needs no table */
-    const mgx_model &m = kp.m;
-    const uint32_t o = blockIdx.x;
-    const uint32_t lane = threadIdx.x;
-    const uint32_t cap = kp.cap;
-    const uint32_t F = m.nfreqs;
-    uint32_t *s_nxt0 = reinterpret_cast<uint32_t *>(smem);
-    uint32_t *s_nxt1 = s_nxt0 + cap;
-    double *s_dZ0 = reinterpret_cast<double *>(smem + 8 * ((2 * cap * 4 + 7) / 8));
-    double *s_dZ1 = s_dZ0 + cap;
-    double *s_sh0 = s_dZ1 + cap;
-    double *s_sh1 = s_sh0 + cap;
-    uint16_t *s_m0 = reinterpret_cast<uint16_t *>(s_sh1 + cap);
-    uint16_t *s_m1 = s_m0 + cap;
+/* The walked ray pair of one model point for the old imager: the rr half and the
+   antipod half (steps, lengths, shifts, all-lines interpolation counts) in one
+   slot of walk_lds_bytes(cap) bytes of LDS. */
+struct RayPairWalk {
+    uint32_t *nxt0, *nxt1;
+    double *dZ0, *dZ1, *sh0, *sh1;
+    uint16_t *m0, *m1;
+    uint32_t o, S0, S1;
+    double sh00, sh01;
+};
 
+__device__ inline RayPairWalk ray_pair_slot(unsigned char *smem, uint32_t cap, uint32_t slot)
+{
+    unsigned char *base = smem + (size_t)slot * walk_lds_bytes(cap);
+    RayPairWalk w;
+    w.nxt0 = reinterpret_cast<uint32_t *>(base);
+    w.nxt1 = w.nxt0 + cap;
+    w.dZ0 = reinterpret_cast<double *>(base + 8 * ((2 * (size_t)cap * 4 + 7) / 8));
+    w.dZ1 = w.dZ0 + cap;
+    w.sh0 = w.dZ1 + cap;
+    w.sh1 = w.sh0 + cap;
+    w.m0 = reinterpret_cast<uint16_t *>(w.sh1 + cap);
+    w.m1 = w.m0 + cap;
+    w.o = 0;
+    w.S0 = w.S1 = 0;
+    w.sh00 = w.sh01 = 0.0;
+    return w;
+}
+
+/* The image-plane coordinates of point o (lane 0 writes them), then the walk of
+   its ray pair into w and the interpolation counts; the whole wave takes part.
+   Returns 0 or an ERR_* code. */
+__device__ inline int ray_pair_walk(const KParams &kp, uint32_t rr, uint32_t ar, uint32_t o,
+                                    RayPairWalk &w, double *ImX, double *ImY)
+{
+    const mgx_model &m = kp.m;
+    const uint32_t lane = threadIdx.x & (MGX_WAVE - 1);
     if (lane == 0) {
         const PointRec po = gld(&kp.pts[o]);
         double X = po.x, Y = 0.0;
@@ -2466,12 +2467,61 @@ __global__ __launch_bounds__(MGX_WAVE) void image_ray_kernel(const KParams *__re
         ImX[o] = X;
         ImY[o] = Y;
     }
+    w.o = o;
+    int e = walk_half<true>(kp, o, rr, w.nxt0, w.dZ0, w.sh0, w.S0, w.sh00);
+    if (!e) e = walk_half<true>(kp, o, ar, w.nxt1, w.dZ1, w.sh1, w.S1, w.sh01);
+    if (e) return e;
+    __syncthreads();
+    for (uint32_t i = lane; i < w.S0 + w.S1; i += MGX_WAVE) {
+        const uint32_t h = (i < w.S0) ? 0 : 1;
+        const uint32_t st = h ? i - w.S0 : i;
+        const uint32_t *nx = h ? w.nxt1 : w.nxt0;
+        const uint32_t crt = (st == 0) ? o : nx[st - 1];
+        const uint32_t mm = n_interp_all(kp, crt, nx[st]);
+        if (mm > 0xffffu) atomicOr(&kp.status[0], (uint32_t)ERR_INTERP);
+        (h ? w.m1 : w.m0)[st] = (uint16_t)(mm > 0xffffu ? 0xffffu : mm);
+    }
+    __syncthreads();
+    return 0;
+}
 
+/*
+ * The old imager at every model point along ray rr (Solver::image_feautrier_order_2
+ * and Solver::image_optical_depth, solver.tpp:543-574, 656-686).  One wave per
+ * ppw (1 or 2) consecutive points: each point's ray pair (rr, antipod) is walked
+ * with the Rest-frame shifts of trace_ray<Rest, false> (reversed on rays >=
+ * hnrays) and the all-lines interpolation count of every step, into its own LDS
+ * slot; lanes take frequencies (with two points, each half-wave takes one
+ * point's) and run over the whole ray from the rr end (first) to the antipod
+ * end (last): the rr half inward, the origin, then the antipod half outward.
+ * KIND selects I(o, f) = 2 Su[last] - B(nr[last], nu(o, f)) (the boundary term
+ * at the unshifted frequency, as the reference) or the summed optical depth.
+ * ImX, ImY: the point's coordinates on the image plane
+ * (Image::set_coordinates_all_model_points, image.cpp:154-212).
+ */
+template <int APPROX, int PREC, int KIND>
+__global__ __launch_bounds__(MGX_WAVE) void image_ray_kernel(const KParams *__restrict__ kpp,
+                                                             uint32_t rr, double *I, double *ImX,
+                                                             double *ImY, uint32_t ppw)
+{
+    mgx_ref_tables_init();
+    const KParams &kp = *kpp;
+    extern __shared__ unsigned char smem[];
+    const double *s_erf = nullptr; /* erf needs no table */
+    const mgx_model &m = kp.m;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t F = m.nfreqs;
+    const uint32_t o0 = blockIdx.x * ppw;
+    const uint32_t npt = (o0 + ppw <= m.npoints) ? ppw : m.npoints - o0;
     const uint32_t ar = gld(&m.antipod[rr]);
-    uint32_t S0, S1;
-    double sh00, sh01;
-    int e = walk_half<true>(kp, o, rr, s_nxt0, s_dZ0, s_sh0, S0, sh00);
-    if (!e) e = walk_half<true>(kp, o, ar, s_nxt1, s_dZ1, s_sh1, S1, sh01);
+
+    RayPairWalk w0 = ray_pair_slot(smem, kp.cap, 0);
+    RayPairWalk w1 = w0;
+    int e = ray_pair_walk(kp, rr, ar, o0, w0, ImX, ImY);
+    if (!e && npt == 2) {
+        w1 = ray_pair_slot(smem, kp.cap, 1);
+        e = ray_pair_walk(kp, rr, ar, o0 + 1, w1, ImX, ImY);
+    }
     if (e) {
         if (lane == 0) {
             atomicOr(&kp.status[0], (uint32_t)e);
@@ -2479,20 +2529,19 @@ __global__ __launch_bounds__(MGX_WAVE) void image_ray_kernel(const KParams *__re
         }
         return;
     }
-    __syncthreads();
-    for (uint32_t i = lane; i < S0 + S1; i += MGX_WAVE) {
-        const uint32_t h = (i < S0) ? 0 : 1;
-        const uint32_t st = h ? i - S0 : i;
-        const uint32_t *nx = h ? s_nxt1 : s_nxt0;
-        const uint32_t crt = (st == 0) ? o : nx[st - 1];
-        const uint32_t mm = n_interp_all(kp, crt, nx[st]);
-        if (mm > 0xffffu) atomicOr(&kp.status[0], (uint32_t)ERR_INTERP);
-        (h ? s_m1 : s_m0)[st] = (uint16_t)(mm > 0xffffu ? 0xffffu : mm);
-    }
-    __syncthreads();
+    /* with two points, each half-wave takes one point's frequencies */
+    const uint32_t half = (npt == 2) ? (lane >> 5) : 0;
+    const uint32_t f0 = (npt == 2) ? (lane & 31) : lane;
+    const uint32_t fstep = (npt == 2) ? 32 : MGX_WAVE;
+    const RayPairWalk &w = half ? w1 : w0;
+    const uint32_t o = w.o, S0 = w.S0, S1 = w.S1;
+    const double sh00 = w.sh00, sh01 = w.sh01;
+    const uint32_t *s_nxt0 = w.nxt0, *s_nxt1 = w.nxt1;
+    const double *s_dZ0 = w.dZ0, *s_dZ1 = w.dZ1, *s_sh0 = w.sh0, *s_sh1 = w.sh1;
+    const uint16_t *s_m0 = w.m0, *s_m1 = w.m1;
 
     const uint32_t last_nr = (S1 > 0) ? s_nxt1[S1 - 1] : o;
-    for (uint32_t f = lane; f < F; f += MGX_WAVE) {
+    for (uint32_t f = f0; f < F; f += fstep) {
         const double freq = m.nu[(size_t)o * F + f];
         double val;
         if (S0 + S1 == 0) {
@@ -4689,23 +4738,27 @@ int run_image_ray(const mgx_model *model, const mgx_options *opts, uint32_t rr, 
     KParams kp;
     if (int e = prepare(model, opts, st, b, kp)) return e;
     return launch_with_capacity(st, b, kp, [&](size_t lds) {
-        const dim3 grid(model->npoints), block(MGX_WAVE);
+        /* two points per wave when a half-wave covers the frequencies and two walk
+           slots fit the LDS */
+        const uint32_t ppw = (model->nfreqs <= 32 && 2 * lds <= lds_budget()) ? 2 : 1;
+        const dim3 grid((model->npoints + ppw - 1) / ppw), block(MGX_WAVE);
+        const size_t l2 = ppw * lds;
         timed(st, [&] {
             const bool fp64 = opts->precision == MGX_PREC_FP64;
             if (opts->approx == MGX_APPROX_NONE) {
                 if (fp64)
                     hipLaunchKernelGGL((image_ray_kernel<MGX_APPROX_NONE, MGX_PREC_FP64, KIND>), grid,
-                                       block, lds, st, b.kp_dev, rr, out, ImX, ImY);
+                                       block, l2, st, b.kp_dev, rr, out, ImX, ImY, ppw);
                 else
                     hipLaunchKernelGGL((image_ray_kernel<MGX_APPROX_NONE, MGX_PREC_MIXED, KIND>), grid,
-                                       block, lds, st, b.kp_dev, rr, out, ImX, ImY);
+                                       block, l2, st, b.kp_dev, rr, out, ImX, ImY, ppw);
             } else {
                 if (fp64)
                     hipLaunchKernelGGL((image_ray_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_FP64, KIND>),
-                                       grid, block, lds, st, b.kp_dev, rr, out, ImX, ImY);
+                                       grid, block, l2, st, b.kp_dev, rr, out, ImX, ImY, ppw);
                 else
                     hipLaunchKernelGGL((image_ray_kernel<MGX_APPROX_CLOSE_LINES, MGX_PREC_MIXED, KIND>),
-                                       grid, block, lds, st, b.kp_dev, rr, out, ImX, ImY);
+                                       grid, block, l2, st, b.kp_dev, rr, out, ImX, ImY, ppw);
             }
         });
     });

@@ -88,6 +88,32 @@ def test_optical_depth_image_fp64_parity(which, raydir, nx, ny, request):
     assert err < RTOL
 
 
+@pytest.mark.parametrize("optical_depth", [False, True])
+def test_image_ray_odd_point_count(optical_depth):
+    """With at most 32 image frequencies the old imager takes two points per wave;
+    51 points leave the last wave one, and every point must match the oracle."""
+    import os
+
+    from magritte_amd import models, solver
+    from oracle import oracle
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    ld = models.read_lamda(os.path.join(root, "tests", "golden", "lamda_test.txt"), num=0)
+    m = models.model_velocity_gradient_1d(ld, npoints=51, nrays=4, nquads=10,
+                                          max_width_fraction=0.5)
+    rr = m.hnrays - 1
+    fq = _image_case(m)
+    print(f"[image_ray odd] P={m.npoints} F={fq['nu'].shape[1]} optical_depth={optical_depth}",
+          flush=True)
+    out, X, Y = solver.image_ray(_dev(m), m.options(), rr, fq, optical_depth=optical_depth)
+    ref, Xr, Yr = oracle.image_ray(m, m.options(), rr, fq, optical_depth=optical_depth)
+    np.testing.assert_array_equal(X.cpu().numpy().view(np.uint64), Xr.view(np.uint64))
+    np.testing.assert_array_equal(Y.cpu().numpy().view(np.uint64), Yr.view(np.uint64))
+    err = _err(out.cpu().numpy(), ref)
+    print(f"  max rel err {err:.3e}", flush=True)
+    assert err < RTOL
+
+
 def test_image_ray_mixed_precision(small_wind):
     from magritte_amd import solver
     from oracle import oracle
