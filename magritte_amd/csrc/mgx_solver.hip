@@ -3906,9 +3906,29 @@ int check_view(const mgx_model *m, const mgx_options *op)
 
 const uint32_t kMaxCap = 8192; /* walk steps per half ray: a ceiling above any LDS budget */
 
-bool g_timing = false;
-double g_last_kernel_ms = 0.0;
-thread_local bool t_no_events = false; /* set by the no-allocation entry point */
+} // namespace
+
+/*
+ * State shared by the solver's two translation units: this file, and
+ * mgx_adjoint.hip, which compiles it again with MGX_ADJOINT_TU for the adjoint
+ * entry points (so that the adjoint kernels get code-generation options of their
+ * own).  Kernel timing, the per-call workspace and the walk-capacity hints are
+ * defined here and declared in the adjoint unit.
+ */
+#ifdef MGX_ADJOINT_TU
+#define MGX_SHARED(decl, init) extern decl
+#else
+#define MGX_SHARED(decl, init) decl init
+#endif
+#define MGX_COMMA ,
+namespace mgx_shared {
+MGX_SHARED(bool g_timing, = false);
+MGX_SHARED(double g_last_kernel_ms, = 0.0);
+MGX_SHARED(thread_local bool t_no_events, = false); /* set by the no-allocation entry point */
+} // namespace mgx_shared
+
+namespace {
+using namespace mgx_shared;
 
 /* launches fn() between two events on st when timing is enabled */
 template <typename Fn>
@@ -3958,9 +3978,13 @@ int prepare_tables(const mgx_model *m, const mgx_options *op, hipStream_t st, De
    image frame: one grow-only buffer per device, reused by every call, so a call
    allocates nothing once the buffer has grown to its model.  A call holds the
    buffer (b.ws_lock) until it returns, so calls that use it run one at a time. */
-std::mutex g_prep_mutex;
-unsigned char *g_prep[64] = {};
-size_t g_prep_bytes[64] = {};
+} // namespace
+namespace mgx_shared {
+MGX_SHARED(std::mutex g_prep_mutex, );
+MGX_SHARED(unsigned char *g_prep[64], = {});
+MGX_SHARED(size_t g_prep_bytes[64], = {});
+} // namespace mgx_shared
+namespace {
 
 /* the tables of model m in b: from the call workspace, or (persistent, for objects
    that keep them, such as a batch) in allocations of their own */
@@ -4016,6 +4040,7 @@ int prepare(const mgx_model *m, const mgx_options *op, hipStream_t st, DevBuffer
    per-call allocations.  Each GPU holds its own copy for the life of the
    process; host code reaches it only through the address hipGetSymbolAddress
    returns on that GPU. */
+#ifndef MGX_ADJOINT_TU
 static __device__ __align__(256) unsigned char scratch[MGX_NOALLOC_SCRATCH_BYTES];
 
 std::mutex g_scratch_mutex; /* guards g_scratch_ptr and serialises the calls that use it */
@@ -4038,6 +4063,7 @@ int scratch_base(unsigned char **out)
     *out = g_scratch_ptr[dev];
     return MGX_OK;
 }
+#endif
 
 /* reads the status words; returns MGX_OK, or an error (needed capacity in *need;
    segments needed by the adjoint's scratch in *need_e) */
@@ -4089,6 +4115,8 @@ uint32_t max_walk_cap(size_t extra)
 /* Walk capacities that solved without a relaunch, by geometry and ray set: a hint
    for the next solve of the same model (a model changed in place under the same
    arrays costs at most a relaunch). */
+} // namespace
+namespace mgx_shared {
 struct WalkKey {
     const void *pos, *nbs, *dirs;
     uint32_t P, nrays, adaptive;
@@ -4101,8 +4129,10 @@ struct WalkKey {
                         o.nx, o.ny);
     }
 };
-std::mutex g_cap_mutex;
-std::map<WalkKey, uint32_t> g_cap_hint;
+MGX_SHARED(std::mutex g_cap_mutex, );
+MGX_SHARED(std::map<WalkKey MGX_COMMA uint32_t> g_cap_hint, );
+} // namespace mgx_shared
+namespace {
 
 WalkKey walk_key(const mgx_model &m)
 {
@@ -4768,6 +4798,7 @@ int run_image_ray(const mgx_model *model, const mgx_options *opts, uint32_t rr, 
 
 extern "C" {
 
+#ifndef MGX_ADJOINT_TU
 const char *mgx_last_error(void) { return mgx::error_buffer(); }
 
 const char *mgx_version(void) { return "magritte_amd 0.1.0 (gfx950)"; }
@@ -5163,6 +5194,9 @@ int mgx_image_optical_depth_ray_device(const mgx_model *model, const mgx_options
     return run_image_ray<IMG_OPTICAL_DEPTH>(model, opts, rr, tau, ImX, ImY, (hipStream_t)stream);
 }
 
+#endif /* !MGX_ADJOINT_TU */
+
+#ifdef MGX_ADJOINT_TU
 static int J_adjoint_entry(const mgx_model *model, const mgx_options *opts, const double *J_bar,
                            double *d_eps, double *d_kap, double *d_iw, void *stream)
 {
@@ -5280,6 +5314,9 @@ void mgx_release_workspace(void)
     g_ws_bytes = 0;
 }
 
+#endif /* MGX_ADJOINT_TU */
+
+#ifndef MGX_ADJOINT_TU
 void mgx_set_kernel_timing(int enable) { g_timing = enable != 0; }
 
 double mgx_last_kernel_ms(void) { return g_last_kernel_ms; }
@@ -5605,5 +5642,7 @@ int mgx_batch_tune(mgx_batch *B, uint32_t sample_stride, uint32_t *best, void *s
 double mgx_batch_last_ms(const mgx_batch *B) { return B ? B->last_ms : 0.0; }
 
 void mgx_batch_destroy(mgx_batch *B) { delete B; }
+
+#endif /* !MGX_ADJOINT_TU */
 
 } // extern "C"
