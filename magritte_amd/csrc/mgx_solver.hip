@@ -521,7 +521,9 @@ __device__ inline uint32_t upper_bound_d(const double *a, uint32_t n, double v)
  */
 struct OwnLine {
     uint32_t g;
+    uint32_t lidx;   /* sorted_line_map[g]: the line at position g */
     double a_lo, a_g, a_hi;
+    double f_own;    /* line[lidx] */
 };
 
 __device__ inline OwnLine own_line(const mgx_model &m, uint32_t l)
@@ -531,6 +533,8 @@ __device__ inline OwnLine own_line(const mgx_model &m, uint32_t l)
     for (uint32_t s = 0; s < m.nlines; s++)
         if (gld(&m.sorted_line_map[s]) == l) g = s;
     r.g = g;
+    r.lidx = gld(&m.sorted_line_map[g]);
+    r.f_own = gld(&m.line[r.lidx]);
     r.a_g = gld(&m.sorted_line[g]);
     r.a_lo = (g > 0) ? gld(&m.sorted_line[g - 1]) : -__builtin_inf();
     r.a_hi = (g + 1 < m.nlines) ? gld(&m.sorted_line[g + 1]) : __builtin_inf();
@@ -553,7 +557,8 @@ __device__ inline double4 lp(const KParams &kp, uint32_t p, uint32_t l)
 /* get_eta_and_chi_interpolated<approx> (solver.tpp:1177-1202, 1260-1302, 1345-1351) */
 template <int APPROX, int PREC>
 __device__ void eta_chi_interp(const KParams &kp, const OwnLine &ol, uint32_t p1, uint32_t p2,
-                               double t, uint32_t ll, double freq, double &eta, double &chi)
+                               double t, uint32_t ll, double freq, double ub1, double &eta,
+                               double &chi)
 {
     const mgx_model &m = kp.m;
     if (APPROX == MGX_APPROX_ONE_LINE) {
@@ -568,15 +573,20 @@ __device__ void eta_chi_interp(const KParams &kp, const OwnLine &ol, uint32_t p1
     chi = kp.op.min_opacity;
     uint32_t lo = 0, hi = m.nlines;
     if (APPROX == MGX_APPROX_CLOSE_LINES) {
-        const double W = kp.op.max_distance_opacity_contribution * (freq * gld(&kp.pts[p1].ub));
+        const double W = kp.op.max_distance_opacity_contribution * (freq * ub1);
         line_window(m, ol, freq - W, freq + W, lo, hi);
     }
     for (uint32_t s = lo; s < hi; s++) {
         uint32_t l;
         double diff;
         if (APPROX == MGX_APPROX_CLOSE_LINES) {
-            l = gld(&m.sorted_line_map[s]);
-            diff = freq - gld(&m.sorted_line[s]);
+            if (s == ol.g) {
+                l = ol.lidx;
+                diff = freq - ol.a_g;
+            } else {
+                l = gld(&m.sorted_line_map[s]);
+                diff = freq - gld(&m.sorted_line[s]);
+            }
         } else {
             l = s;
             diff = freq - m.line[l];
@@ -622,7 +632,8 @@ __device__ void S_dtau_line_integrated(const KParams &kp, const double *erf_tab,
                                        uint32_t cp, uint32_t np, uint32_t cpi, uint32_t npi,
                                        uint32_t lineidx, double currfreq, double nextfreq,
                                        double curr_interp, double next_interp, double dZ,
-                                       double &dtau, double &Scurr, double &Snext)
+                                       double ub_c, double ub_n, double &dtau, double &Scurr,
+                                       double &Snext)
 {
     const mgx_model &m = kp.m;
     if (APPROX == MGX_APPROX_ONE_LINE) {
@@ -645,16 +656,26 @@ __device__ void S_dtau_line_integrated(const KParams &kp, const double *erf_tab,
             right_freq = currfreq;
             left_freq = nextfreq;
         }
-        const double cb = kp.op.max_distance_opacity_contribution * (right_freq * gld(&kp.pts[cp].ub));
-        const double nb = kp.op.max_distance_opacity_contribution * (right_freq * gld(&kp.pts[np].ub));
+        const double cb = kp.op.max_distance_opacity_contribution * (right_freq * ub_c);
+        const double nb = kp.op.max_distance_opacity_contribution * (right_freq * ub_n);
         const double W = (cb < nb) ? nb : cb;
         line_window(m, ol, left_freq - W, right_freq + W, lo, hi);
     }
     for (uint32_t s = lo; s < hi; s++) {
-        const uint32_t l = (APPROX == MGX_APPROX_CLOSE_LINES) ? gld(&m.sorted_line_map[s]) : s;
+        /* the own line's index and frequency come from OwnLine, so the usual
+           one-line window waits on no table load before its gathers */
+        uint32_t l = s;
+        double linefreq;
+        if (APPROX == MGX_APPROX_CLOSE_LINES && s == ol.g) {
+            l = ol.lidx;
+            linefreq = ol.f_own;
+        } else {
+            if (APPROX == MGX_APPROX_CLOSE_LINES) l = gld(&m.sorted_line_map[s]);
+            linefreq = gld(&m.line[l]);
+        }
         const double4 c = lp(kp, cp, l), cI = lp(kp, cpi, l);
         const double4 n = lp(kp, np, l), nI = lp(kp, npi, l);
-        const double line_dtau = dtau_single_line<PREC>(kp, erf_tab, c, cI, n, nI, gld(&m.line[l]),
+        const double line_dtau = dtau_single_line<PREC>(kp, erf_tab, c, cI, n, nI, linefreq,
                                                         currfreq, nextfreq, curr_interp,
                                                         next_interp, dZ);
         const double Sc = interp_log<PREC>(c.w, cI.w, curr_interp);
@@ -694,6 +715,15 @@ struct PointEC {
  * entries cu (closer to the rr-side boundary) and nx.  Returns dtau and the
  * source values of the two rows; uses/updates the caches of the two entries.
  */
+/* (dshift_max, ub) of point p: adjacent fields of its record, read as one 16-byte word */
+__device__ __forceinline__ double2 dsm_ub(const KParams &kp, uint32_t p)
+{
+    static_assert(offsetof(PointRec, ub) == offsetof(PointRec, dshift_max) + 8 &&
+                      offsetof(PointRec, dshift_max) % 16 == 0 && sizeof(PointRec) % 16 == 0,
+                  "dshift_max and ub form one aligned 16-byte word");
+    return gld(reinterpret_cast<const double2 *>(&kp.pts[p].dshift_max));
+}
+
 template <int APPROX, int PREC>
 __device__ inline bool segment(const KParams &kp, const double *erf_tab, const OwnLine &ol,
                                const Entry &cu, const Entry &nx, double dZ, double freq,
@@ -702,22 +732,25 @@ __device__ inline bool segment(const KParams &kp, const double *erf_tab, const O
 {
     const double dshift = nx.shift - cu.shift;
     const double dshift_abs = fabs(dshift);
-    const double a = gld(&kp.pts[cu.nr].dshift_max), b = gld(&kp.pts[nx.nr].dshift_max);
+    /* dshift_max and ub of both points, each pair in one 16-byte load */
+    const double2 du_c = dsm_ub(kp, cu.nr), du_n = dsm_ub(kp, nx.nr);
+    const double a = du_c.x, b = du_n.x;
     const double dshift_max = (b < a) ? b : a;
     const double cf = freq * cu.shift, nf = freq * nx.shift;
     if (dshift_abs > dshift_max) {
         S_dtau_line_integrated<APPROX, PREC>(kp, erf_tab, ol, cu.nr, nx.nr, cu.nri, nx.nri, line,
-                                             cf, nf, cu.ifac, nx.ifac, dZ, dtau, S_cu, S_nx);
+                                             cf, nf, cu.ifac, nx.ifac, dZ, du_c.y, du_n.y, dtau,
+                                             S_cu, S_nx);
         return false;
     } else {
         if (!ec_cu.valid) {
-            eta_chi_interp<APPROX, PREC>(kp, ol, cu.nr, cu.nri, cu.ifac, line, cf, ec_cu.eta,
-                                         ec_cu.chi);
+            eta_chi_interp<APPROX, PREC>(kp, ol, cu.nr, cu.nri, cu.ifac, line, cf, du_c.y,
+                                         ec_cu.eta, ec_cu.chi);
             ec_cu.valid = true;
         }
         if (!ec_nx.valid) {
-            eta_chi_interp<APPROX, PREC>(kp, ol, nx.nr, nx.nri, nx.ifac, line, nf, ec_nx.eta,
-                                         ec_nx.chi);
+            eta_chi_interp<APPROX, PREC>(kp, ol, nx.nr, nx.nri, nx.ifac, line, nf, du_n.y,
+                                         ec_nx.eta, ec_nx.chi);
             ec_nx.valid = true;
         }
         S_cu = ec_cu.eta * rcp(ec_cu.chi);
@@ -1029,7 +1062,7 @@ __device__ inline LamPoint lambda_point(const KParams &kp, const OwnLine &ol, ui
     LamPoint r;
     r.constante = gld(&m.line_A[l]) * gld(&m.quad_weights[(size_t)spec * Q + gld(&m.f_z[f])]) * w_ang;
     double eta, chi;
-    eta_chi_interp<APPROX, PREC>(kp, ol, o, o, 0.0, k, freq_line, eta, chi);
+    eta_chi_interp<APPROX, PREC>(kp, ol, o, o, 0.0, k, freq_line, gld(&kp.pts[o].ub), eta, chi);
     r.inverse_chi = rcp(chi);
     r.frq = freq * 1.0;
     const double inv_mass = gld(&m.species_inverse_mass[spec]);
@@ -2656,7 +2689,8 @@ __device__ inline void eta_chi_tape(const KParams &kp, const OwnLine &ol, const 
             return;
         }
     }
-    eta_chi_interp<APPROX, PREC>(kp, ol, e.nr, e.nri, e.ifac, line, freq, ec.eta, ec.chi);
+    eta_chi_interp<APPROX, PREC>(kp, ol, e.nr, e.nri, e.ifac, line, freq, gld(&kp.pts[e.nr].ub),
+                                 ec.eta, ec.chi);
 }
 
 /* Segment kinds recorded for the reverse pass */
@@ -2680,7 +2714,8 @@ __device__ inline void segment_tape(const KParams &kp, const double *erf_tab, co
     const mgx_model &m = kp.m;
     const double dshift = nx.shift - cu.shift;
     const double dshift_abs = fabs(dshift);
-    const double a = gld(&kp.pts[cu.nr].dshift_max), b = gld(&kp.pts[nx.nr].dshift_max);
+    const double2 du_c = dsm_ub(kp, cu.nr), du_n = dsm_ub(kp, nx.nr);
+    const double a = du_c.x, b = du_n.x;
     const double dshift_max = (b < a) ? b : a;
     const double cf = freq * cu.shift, nf = freq * nx.shift;
     if (dshift_abs > dshift_max) {
@@ -2693,8 +2728,8 @@ __device__ inline void segment_tape(const KParams &kp, const double *erf_tab, co
                 right_freq = cf;
                 left_freq = nf;
             }
-            const double cb = kp.op.max_distance_opacity_contribution * (right_freq * gld(&kp.pts[cu.nr].ub));
-            const double nb = kp.op.max_distance_opacity_contribution * (right_freq * gld(&kp.pts[nx.nr].ub));
+            const double cb = kp.op.max_distance_opacity_contribution * (right_freq * du_c.y);
+            const double nb = kp.op.max_distance_opacity_contribution * (right_freq * du_n.y);
             const double W = (cb < nb) ? nb : cb;
             const double left = left_freq - W, right = right_freq + W;
             if (!(ol.a_g < left) && ol.a_lo < left && !(right < ol.a_g) && right < ol.a_hi) {
@@ -2746,7 +2781,8 @@ __device__ inline void segment_tape(const KParams &kp, const double *erf_tab, co
             }
         }
         S_dtau_line_integrated<APPROX, PREC>(kp, erf_tab, ol, cu.nr, nx.nr, cu.nri, nx.nri, line,
-                                             cf, nf, cu.ifac, nx.ifac, dZ, dtau, S_cu, S_nx);
+                                             cf, nf, cu.ifac, nx.ifac, dZ, du_c.y, du_n.y, dtau, S_cu,
+                                             S_nx);
         g = MGX_SEG_ERF_GENERIC;
         return;
     }
@@ -3279,7 +3315,7 @@ __device__ __attribute__((noinline)) double2 eta_chi_generic(const KParams *kpp,
                                                              uint32_t line)
 {
     double2 r;
-    eta_chi_interp<APPROX, PREC>(*kpp, ol, nr, nri, ifac, line, f, r.x, r.y);
+    eta_chi_interp<APPROX, PREC>(*kpp, ol, nr, nri, ifac, line, f, gld(&kpp->pts[nr].ub), r.x, r.y);
     return r;
 }
 
