@@ -114,6 +114,13 @@ struct PointRec {
     double ub;         /* sqrt(2kT/(amu c^2) * max_inverse_mass + vturb2) */
 };
 
+/* The leading 32 bytes of a PointRec (the walk reads only the position), as two
+   16-byte words */
+struct PointPos {
+    double x, y, z, vx;
+};
+static_assert(sizeof(PointRec) % 16 == 0, "PointRec is read in 16-byte words");
+
 /* Per-point integers: neighbour count and offset, boundary index (P = interior). */
 struct PointIdx {
     uint32_t n_nbs, cum, p2b, pad;
@@ -264,18 +271,18 @@ __device__ inline void wave_argmin(double &d2, uint32_t &idx, double &zn, uint32
     zn = b_zn;
 }
 
-/* get_next_general_geometry<Defaulttracer> (geometry.tpp:14-44), all lanes return the same */
+/* get_next_general_geometry<Defaulttracer> (geometry.tpp:14-44), all lanes return the same;
+   pi is the current point's neighbour-list record */
 __device__ inline uint32_t next_general(const KParams &kp, const double *origin,
-                                        const double *dir, uint32_t c, double &Z, double &dZ)
+                                        const double *dir, const PointIdx &pi, double &Z, double &dZ)
 {
     const uint32_t lane = threadIdx.x & (MGX_WAVE - 1);
-    const PointIdx pi = gld(&kp.pidx[c]);
     double best_d2 = __builtin_inf();
-    uint32_t best_i = 0xffffffffu;
+    uint32_t best_i = 0xffffffffu, best_n = 0;
     double best_zn = 0.0;
     for (uint32_t i = lane; i < pi.n_nbs; i += MGX_WAVE) {
         const uint32_t n = gld(&kp.m.neighbors[pi.cum + i]);
-        const PointRec pn = gld(&kp.pts[n]);
+        const PointPos pn = gld(reinterpret_cast<const PointPos *>(&kp.pts[n]));
         double R[3];
         R[0] = pn.x - origin[0];
         R[1] = pn.y - origin[1];
@@ -287,6 +294,7 @@ __device__ inline uint32_t next_general(const KParams &kp, const double *origin,
                 best_d2 = d2;
                 best_i = i;
                 best_zn = Z_new;
+                best_n = n;
             }
         }
     }
@@ -294,7 +302,9 @@ __device__ inline uint32_t next_general(const KParams &kp, const double *origin,
     if (best_i == 0xffffffffu) return kp.m.npoints;
     dZ = best_zn - Z;
     Z += dZ;
-    return gld(&kp.m.neighbors[pi.cum + best_i]);
+    /* list entry i was examined by lane i % 64, whose own best it is: its point
+       index comes from that lane instead of a second read of the list */
+    return (uint32_t)__builtin_amdgcn_readlane((int)best_n, (int)(best_i & (MGX_WAVE - 1)));
 }
 
 __device__ inline double sqnorm_point(const KParams &kp, uint32_t p)
@@ -359,14 +369,17 @@ __device__ int walk_half(const KParams &kp, uint32_t o, uint32_t r, uint32_t *s_
     const bool reverse = r >= m.hnrays;
     shift0 = REST ? shift_rest_ray(kp, origin, dir, o, 0.0, reverse)
                   : shift_comoving(kp, origin, ovel, dir, o, 0.0);
+    PointIdx pc = gld(&kp.pidx[o]);
     for (;;) {
         const uint32_t nxt = m.spherical_symmetry ? next_spherical(kp, origin, dir, crt, Z, dZ)
-                                                  : next_general(kp, origin, dir, crt, Z, dZ);
+                                                  : next_general(kp, origin, dir, pc, Z, dZ);
         if (nxt >= m.npoints) {
             if (nsteps == 0) break; /* nothing in this direction */
             return ERR_NO_NEXT;
         }
         if (nsteps >= kp.cap) return ERR_CAP;
+        /* the next point's record serves the boundary test and the next step */
+        const PointIdx pn = gld(&kp.pidx[nxt]);
         const double sh = REST ? shift_rest_ray(kp, origin, dir, nxt, Z, reverse)
                                : shift_comoving(kp, origin, ovel, dir, nxt, Z);
         if (lane == 0) {
@@ -375,8 +388,9 @@ __device__ int walk_half(const KParams &kp, uint32_t o, uint32_t r, uint32_t *s_
             s_shift[nsteps] = sh;
         }
         nsteps++;
-        if (on_boundary(kp, nxt)) break;
+        if (pn.p2b != m.npoints) break; /* on the boundary */
         crt = nxt;
+        pc = pn;
     }
     return 0;
 }
@@ -1794,9 +1808,10 @@ __global__ __launch_bounds__(MGX_WAVE) void walk_max_kernel(const KParams *__res
         const double dir[3] = {gld(&dp[0]), gld(&dp[1]), gld(&dp[2])};
         double Z = 0.0, dZ = 0.0;
         uint32_t crt = o, n = 0;
+        PointIdx pc = gld(&kp.pidx[o]);
         for (;;) {
             const uint32_t nxt = m.spherical_symmetry ? next_spherical(kp, origin, dir, crt, Z, dZ)
-                                                      : next_general(kp, origin, dir, crt, Z, dZ);
+                                                      : next_general(kp, origin, dir, pc, Z, dZ);
             if (nxt >= m.npoints) {
                 if (n != 0 && lane == 0) atomicOr(&kp.status[0], (uint32_t)ERR_NO_NEXT);
                 break;
@@ -1805,8 +1820,10 @@ __global__ __launch_bounds__(MGX_WAVE) void walk_max_kernel(const KParams *__res
                 if (lane == 0) atomicOr(&kp.status[0], (uint32_t)ERR_NO_NEXT);
                 break;
             }
-            if (on_boundary(kp, nxt)) break;
+            const PointIdx pn = gld(&kp.pidx[nxt]);
+            if (pn.p2b != m.npoints) break; /* on the boundary */
             crt = nxt;
+            pc = pn;
         }
         longest = n > longest ? n : longest;
     }
@@ -2230,7 +2247,7 @@ __device__ bool image_walk(const KParams &kp, const ImageFrame &fr, uint32_t pix
         double dZ = 0.0;
         uint32_t crt = ip;
         uint32_t nxt = m.spherical_symmetry ? next_spherical(kp, origin, rd, crt, Z, dZ)
-                                            : next_general(kp, origin, rd, crt, Z, dZ);
+                                            : next_general(kp, origin, rd, gld(&kp.pidx[crt]), Z, dZ);
         bool accessed_non_boundary = false;
         if (nxt < m.npoints) {
             shift0 = shift_rest(m, origin, rd, crt, Z - dZ);
@@ -2251,12 +2268,14 @@ __device__ bool image_walk(const KParams &kp, const ImageFrame &fr, uint32_t pix
                 S++;
                 double rel[3];
                 for (int i = 0; i < 3; i++) rel[i] = m.position[3 * (size_t)nxt + i] - m.center[i];
-                const bool interior = m.point2boundary[nxt] == m.npoints;
+                /* the record of nxt serves the boundary test and the next step */
+                const PointIdx pn = gld(&kp.pidx[nxt]);
+                const bool interior = pn.p2b == m.npoints;
                 if (!(interior || dot3(rel, rd) < 0)) break;
                 crt = nxt;
                 if (interior) accessed_non_boundary = true;
                 nxt = m.spherical_symmetry ? next_spherical(kp, origin, rd, crt, Z, dZ)
-                                           : next_general(kp, origin, rd, crt, Z, dZ);
+                                           : next_general(kp, origin, rd, pn, Z, dZ);
                 if (nxt >= m.npoints) break;
                 shift_nxt = shift_rest(m, origin, rd, nxt, Z);
             }
