@@ -540,6 +540,17 @@ struct OwnLine {
     double f_own;    /* line[lidx] */
 };
 
+/* Whether the line-integrated segments take the own line's frequency from OwnLine
+   (a register) or read line[lidx] (a load that needs no other load first).  The
+   adjoint unit, at 256 VGPRs, keeps it in a register; the solve unit, at 168,
+   reads it: measured on the 32^3 wind, the register cost the forward 4-6% in
+   spills, the load cost the adjoint 8%. */
+#ifdef MGX_ADJOINT_TU
+#define MGX_OWN_FREQ_IN_REGS 1
+#else
+#define MGX_OWN_FREQ_IN_REGS 0
+#endif
+
 __device__ inline OwnLine own_line(const mgx_model &m, uint32_t l)
 {
     OwnLine r;
@@ -682,7 +693,7 @@ __device__ void S_dtau_line_integrated(const KParams &kp, const double *erf_tab,
         double linefreq;
         if (APPROX == MGX_APPROX_CLOSE_LINES && s == ol.g) {
             l = ol.lidx;
-            linefreq = ol.f_own;
+            linefreq = MGX_OWN_FREQ_IN_REGS ? ol.f_own : gld(&m.line[l]);
         } else {
             if (APPROX == MGX_APPROX_CLOSE_LINES) l = gld(&m.sorted_line_map[s]);
             linefreq = gld(&m.line[l]);
