@@ -3664,8 +3664,8 @@ __device__ inline CentreRow centre_row(const KParams &kp, uint32_t o, double fre
 
 /*
  * Adjoint of the formal solve: given J_bar (P, F), adds (dJ/d eps)^T J_bar and
- * (dJ/d kappa)^T J_bar into g_eps, g_kap (P, L).  Persistent grid: wave b takes
- * origins b, b + G, ...; it keeps the elimination state of its current ray
+ * (dJ/d kappa)^T J_bar into g_eps, g_kap (P, L).  Persistent grid: each wave takes
+ * the next origin from a counter; it keeps the elimination state of its current ray
  * pair in its own slice of the scratch (2 halves x capE tape records).
  * The lane layout is the forward kernel's (two 32-lane line slots); lanes past
  * the last quadrature bin repeat it with a zero seed so every slot reduces its
@@ -3694,7 +3694,13 @@ __global__ __launch_bounds__(MGX_WAVE, MGX_ADJ_WAVES_PER_SIMD) void feautrier_ad
     unsigned char *st_wave = reinterpret_cast<unsigned char *>(scratch) +
                              (size_t)blockIdx.x * 2 * capE * Tape::kRecord;
 
-    for (uint32_t o = blockIdx.x; o < P; o += gridDim.x) {
+    /* origins are handed out by an atomic counter (status word 3, cleared before
+       the launch), so the waves finish together however the origins' costs vary */
+    for (;;) {
+        uint32_t onext = 0;
+        if (lane == 0) onext = atomicAdd(&kp.status[3], 1u);
+        const uint32_t o = (uint32_t)__builtin_amdgcn_readlane((int)onext, 0);
+        if (o >= P) break;
         /* a buffer of this launch overflowed: the host clears the gradients and
            relaunches, so the remaining origins are skipped */
         if (__builtin_amdgcn_readfirstlane(__hip_atomic_load(&kp.status[0], __ATOMIC_RELAXED,
