@@ -83,7 +83,7 @@ def parse():
 
 
 def algorithmic_bytes(stats, model, precision):
-    """Bytes one solve must move at minimum (per-unit figures of the survey):
+    """Bytes one solve must move at minimum, per unit of work:
     36 B per walk step (neighbour count/offset, boundary flag, velocity),
     28 B per neighbour examined (index + position), line data per traced entry
     (inverse width, emissivity, opacity: 24 B FP64 / 12 B FP32) and again per

@@ -854,10 +854,10 @@ struct mgx_nlte {
     double *part = nullptr;     /* (nlspecs, 3, P) per-point partials */
     double *red = nullptr;      /* 3 nlspecs totals + Gram scratch */
     double *gram_partial = nullptr;
-    double *J = nullptr, *lambda = nullptr; /* driver buffers */
+    double *J = nullptr, *lambda = nullptr; /* the loop's J and Lambda */
     size_t J_elems = 0, lambda_elems = 0;
     double *gx = nullptr; /* global update: two iterates and the extra diagonal (3 nvec), dmax */
-    uint32_t *lam_count = nullptr, *lam_nr = nullptr; /* driver's Lambda lists */
+    uint32_t *lam_count = nullptr, *lam_nr = nullptr; /* the loop's Lambda lists */
     double *lam_val = nullptr;
     size_t lam_elems = 0, lam_count_elems = 0;
     int last_loop = -1;     /* kind of the loop whose formal solve filled J and Lambda last */
